@@ -1,0 +1,218 @@
+// gxy_device.h — device-side scene descriptors and ray math for gfx950.
+//
+// Everything a kernel needs about a Visualization is packed into one
+// TraceParams value passed as a kernel argument (no per-launch allocation, no
+// pointer chasing through host structs): up to GXY_MAX_VOLVIS volumes with
+// their transfer-function tables, slice planes and isovalues, the partition
+// box, the march step, and the device BVHs of the geometry model.
+//
+// The arithmetic below keeps the operation order of the reference kernels
+// (src/renderer/TraceRays.ispc) so that results can be compared with the CPU
+// restatement ray by ray; all HIP code is compiled with -ffp-contract=off.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/gxy_hip.h"
+
+#define GXY_MAX_VOLVIS 4
+#define GXY_MAX_SLICES 8
+#define GXY_MAX_ISO 8
+#define GXY_TF_MAX 256
+#define GXY_BLOCK 256
+// upper bound on march steps per ray and pass (a 4096^3 diagonal at half-voxel
+// steps needs about 14k); the CPU restatement uses the same bound
+#define GXY_MAX_MARCH (1 << 20)
+
+// A structured volume brick resident in HBM (x fastest).
+struct DVolume {
+  const float *data;
+  int n[3];
+  float org[3];
+  float isp[3];   // 1 / spacing (OSPRay transformWorldToLocal multiplies)
+  float sp[3];
+  float ub[3];    // nextafter(dims-1, 0): local-coordinate clamp
+  float rate;     // samplingRate
+  float step;     // samplingStep (minimum spacing)
+};
+
+// Piecewise-linear TF: nc colours (rgb) and no opacities over [lo, hi].
+struct DTF {
+  const float *col;
+  const float *op;
+  int nc, no;
+  float lo, hi;
+};
+
+struct DVolVis {
+  DVolume vol;
+  DTF tf;
+  int nSlices, nIso, dvr, pad_;
+  float slices[4 * GXY_MAX_SLICES];
+  float iso[GXY_MAX_ISO];
+};
+
+// Binary BVH with both child boxes stored in the parent (64 B per node).
+// child index >= 0: inner node; < 0: leaf holding primitive range
+// (~c) >> 4 .. count (c & 15) in the sorted primitive arrays.
+struct BVHNode {
+  float lo0[3], hi0[3];
+  float lo1[3], hi1[3];
+  int c0, c1;
+  int pad_[2];
+};
+
+struct DModel {
+  // triangles, in BVH leaf order: 9 floats per triangle (v0 v1 v2)
+  const BVHNode *tnodes;
+  const float *tverts;
+  const int *tgeom;     // geomID per sorted triangle
+  const int *tprim;     // primID per sorted triangle
+  int ntris;
+  int troot_leaf;       // 1 when the whole tree is a single leaf
+  // spheres, in BVH leaf order
+  const BVHNode *snodes;
+  const float *scenter;  // 3 per sphere
+  const float *sradius;  // per sphere
+  const float *seps;     // per sphere: owning geometry's epsilon
+  const int *sgeom;
+  const int *sprim;
+  int nsph;
+  int sroot_leaf;
+  // shading data, indexed by geomID
+  int ngeom;
+  const struct DGeomShade *shade;
+};
+
+// Per-geometry shading data (DataDrivenTriangleMesh_postIntersect,
+// DataDrivenSpheres_postIntersect).
+struct DGeomShade {
+  int kind;               // 0 triangles, 1 spheres
+  int has_normals, has_data, has_tf;
+  const float *vertices;  // triangles: 3 per vertex
+  const uint32_t *indices;
+  const float *normals;
+  const float *vdata;     // per vertex (triangles) or per sphere
+  const float *centers;
+  DTF tf;
+};
+
+struct TraceParams {
+  int nvv;
+  int integrate;
+  int has_model;
+  int pad_;
+  float blo[3], bhi[3];
+  float step, eps;
+  DVolVis vv[GXY_MAX_VOLVIS];
+  DModel model;
+  unsigned long long *sample_counter;  // may be null
+  int *hit_geom;                       // side-band geomID per ray, may be null
+  int *hit_prim;
+};
+
+// ------------------------------------------------------------------ math
+struct f3 { float x, y, z; };
+__device__ __forceinline__ f3 mk3(float x, float y, float z) { f3 r; r.x = x; r.y = y; r.z = z; return r; }
+__device__ __forceinline__ f3 add3(f3 a, f3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ f3 sub3(f3 a, f3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ f3 scl3(float s, f3 a) { return mk3(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ f3 neg3(f3 a) { return mk3(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 cross3(f3 a, f3 b) {
+  return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float mnf(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float mxf(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ float clamp01f(float v) { return v < 0.f ? 0.f : (v > 1.f ? 1.f : v); }
+__device__ __forceinline__ f3 normalize3(f3 v) { float s = 1.0f / sqrtf(dot3(v, v)); return scl3(s, v); }
+__device__ __forceinline__ f3 safe_normalize3(f3 v) {
+  float d = dot3(v, v);
+  d = d > 1e-12f ? d : 1e-12f;
+  float s = 1.0f / sqrtf(d);
+  return scl3(s, v);
+}
+
+// Unaligned (4-byte aligned) pair load: one global_load_dwordx2 for two
+// x-adjacent voxels.
+struct __attribute__((packed, aligned(4))) float2u { float x, y; };
+
+// Trilinear sample (OSPRay StructuredVolume semantics; x, then y, then z).
+__device__ __forceinline__ float vol_sample(const DVolume &V, f3 p) {
+  float l0 = (p.x - V.org[0]) * V.isp[0];
+  float l1 = (p.y - V.org[1]) * V.isp[1];
+  float l2 = (p.z - V.org[2]) * V.isp[2];
+  l0 = l0 < 0.f ? 0.f : (l0 > V.ub[0] ? V.ub[0] : l0);
+  l1 = l1 < 0.f ? 0.f : (l1 > V.ub[1] ? V.ub[1] : l1);
+  l2 = l2 < 0.f ? 0.f : (l2 > V.ub[2] ? V.ub[2] : l2);
+  if (l0 != l0) l0 = 0.f;
+  if (l1 != l1) l1 = 0.f;
+  if (l2 != l2) l2 = 0.f;
+  int i0 = (int)l0, i1 = (int)l1, i2 = (int)l2;
+  float f0 = l0 - (float)i0, f1 = l1 - (float)i1, f2 = l2 - (float)i2;
+  const int64_t sy = V.n[0], sz = (int64_t)V.n[0] * V.n[1];
+  const float *d = V.data + (int64_t)i0 + sy * i1 + sz * i2;
+  float2u p00 = *(const float2u *)(d);
+  float2u p10 = *(const float2u *)(d + sy);
+  float2u p01 = *(const float2u *)(d + sz);
+  float2u p11 = *(const float2u *)(d + sz + sy);
+  float g0 = 1.0f - f0, g1 = 1.0f - f1, g2 = 1.0f - f2;
+  float a00 = g0 * p00.x + f0 * p00.y;
+  float a10 = g0 * p10.x + f0 * p10.y;
+  float a01 = g0 * p01.x + f0 * p01.y;
+  float a11 = g0 * p11.x + f0 * p11.y;
+  float b0 = g1 * a00 + f1 * a10;
+  float b1 = g1 * a01 + f1 * a11;
+  return g2 * b0 + f2 * b1;
+}
+
+// Forward differences at one grid spacing, divided by the spacing.
+__device__ __forceinline__ f3 vol_gradient(const DVolume &V, f3 p) {
+  float s = vol_sample(V, p);
+  float gx = vol_sample(V, add3(p, mk3(V.sp[0], 0.f, 0.f))) - s;
+  float gy = vol_sample(V, add3(p, mk3(0.f, V.sp[1], 0.f))) - s;
+  float gz = vol_sample(V, add3(p, mk3(0.f, 0.f, V.sp[2]))) - s;
+  return mk3(gx / V.sp[0], gy / V.sp[1], gz / V.sp[2]);
+}
+
+// LinearTransferFunction lookups over tables that may live in LDS.
+__device__ __forceinline__ f3 tf_color(const float *col, int n, float lo, float hi, float v) {
+  if (v != v) return mk3(0.f, 0.f, 0.f);
+  if (n == 0) return mk3(1.f, 1.f, 1.f);
+  if (v <= lo) return mk3(col[0], col[1], col[2]);
+  if (v >= hi) return mk3(col[3 * n - 3], col[3 * n - 2], col[3 * n - 1]);
+  float x = (v - lo) / (hi - lo) * ((float)n - 1.0f);
+  int i = (int)floorf(x);
+  float r = x - (float)i;
+  int j = i + 1 < n - 1 ? i + 1 : n - 1;
+  float q = 1.0f - r;
+  return mk3(q * col[3 * i] + r * col[3 * j], q * col[3 * i + 1] + r * col[3 * j + 1],
+             q * col[3 * i + 2] + r * col[3 * j + 2]);
+}
+__device__ __forceinline__ float tf_opacity(const float *op, int n, float lo, float hi, float v) {
+  if (v != v) return 0.f;
+  if (n == 0) return 1.f;
+  if (v <= lo) return op[0];
+  if (v >= hi) return op[n - 1];
+  float x = (v - lo) / (hi - lo) * ((float)n - 1.0f);
+  int i = (int)floorf(x);
+  float r = x - (float)i;
+  int j = i + 1 < n - 1 ? i + 1 : n - 1;
+  return (1.0f - r) * op[i] + r * op[j];
+}
+
+// MyIntersectBox (TraceRays.ispc:90-106), rcp as 1/x.
+__device__ __forceinline__ void box_clip(f3 org, f3 dir, const float *lo, const float *hi, float &t0, float &t1) {
+  float ix = 1.0f / dir.x, iy = 1.0f / dir.y, iz = 1.0f / dir.z;
+  float minx = (lo[0] - org.x) * ix, miny = (lo[1] - org.y) * iy, minz = (lo[2] - org.z) * iz;
+  float maxx = (hi[0] - org.x) * ix, maxy = (hi[1] - org.y) * iy, maxz = (hi[2] - org.z) * iz;
+  t0 = mxf(mnf(minx, maxx), mxf(mnf(miny, maxy), mnf(minz, maxz)));
+  t1 = mnf(mxf(minx, maxx), mnf(mxf(miny, maxy), mxf(minz, maxz)));
+}
+
+// Wave-wide sum for 64-lane wavefronts.
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}

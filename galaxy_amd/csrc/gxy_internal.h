@@ -1,0 +1,89 @@
+// gxy_internal.h — launchers shared between the kernel files and the runtime.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "gxy_device.h"
+
+namespace gxy {
+
+// lighting / spawn modes (bits), applied per parent in this order
+enum { LM_AMBIENT = 1, LM_AO = 2, LM_DIFFUSE = 4, LM_SHADOW = 8 };
+
+struct LightParams {
+  int nAO, nLights, doShadows, pad_;
+  float aoRadius, Ka, Kd, eps;
+  float L[3 * 16];
+  int types[16];
+};
+
+// route modes (bits)
+enum { RM_CLASSIFY = 1, RM_ASSIGN = 2, RM_ACCUMULATE = 4 };
+
+struct RouteParams {
+  int reverse;
+  int neighbors[6];
+  float blo[3], bhi[3];
+  float *fb;
+  int width, height;
+  int mode;
+  unsigned long long *counters;  // [0] terminated accumulated, [1] dropped (may be null)
+};
+
+struct CameraParams {
+  float veye[3], vdir[3], vr[3], vu[3], center[3];
+  float scaling, off_x, off_y;
+  int is_ortho;
+  int width, height, order;
+  float llo[3], lhi[3], glo[3], ghi[3];
+};
+
+hipError_t launch_trace(const TraceParams &P, const RayList_ispc &R, int n, hipStream_t s);
+hipError_t launch_sampler(const TraceParams &P, const RayList_ispc &R, int n, hipStream_t s);
+
+// Exclusive offsets of primary surface rays (flag scan).  block_counts needs
+// ceil(n/256)+1 ints of scratch; total (device int) receives the count.
+hipError_t launch_surface_scan(const RayList_ispc &R, int n, int *scratch, int *rank_out, int *total,
+                               hipStream_t s);
+// Offsets as TraceRays.cpp:89-117 from the ranks: ao = nAO*rank, shadow =
+// nAO*total + nLights*rank (or -1).
+hipError_t launch_offsets_from_rank(const RayList_ispc &R, int n, const int *rank, const int *total, int nAO,
+                                    int nLights, int doShadows, int *ao_off, int *sh_off, hipStream_t s);
+hipError_t launch_lighting(const LightParams &L, int mode, const RayList_ispc &R, int n, const int *ao_off,
+                           const int *sh_off, const RayList_ispc &O, hipStream_t s);
+hipError_t upload_uv_tables(const float *U, const float *V);
+
+hipError_t launch_route(const RouteParams &P, const RayList_ispc &R, int n, hipStream_t s);
+// Stable partition by classification into K categories: the category of a ray
+// is cat_table[class + 4] (host table, ntable <= 64 entries, -1 = not copied).
+// counts: K ints (device).  scratch: K*ceil(n/256) ints, kept intact between
+// the count phase (lists == nullptr) and launch_partition_scatter.
+hipError_t launch_partition(const RayList_ispc &R, int n, const int *cat_table, int ntable, int K,
+                            const RayList_ispc *lists, int *scratch, int *counts, hipStream_t s);
+hipError_t launch_partition_scatter(const RayList_ispc &R, int n, const int *cat_table, int ntable, int K,
+                                    const RayList_ispc *lists, const int *scratch, hipStream_t s);
+hipError_t launch_accumulate_sampler(const RayList_ispc &R, int n, float *xyz, int capacity, int *count,
+                                     int *scratch, hipStream_t s);
+hipError_t launch_camera(const CameraParams &C, const RayList_ispc &R, int capacity, int *scratch, int *total,
+                         hipStream_t s);
+hipError_t launch_copy_rays(const RayList_ispc &src, int start, int count, const RayList_ispc &dst, int dst_start,
+                            hipStream_t s);
+
+hipError_t launch_noise(float *data, int nx, int ny, int nz, const int *perm_dev, int octaves, float base_freq,
+                        float *minmax_scratch, hipStream_t s);
+
+// LBVH build (bvh.hip).  Boxes per primitive (lo, hi) on device; returns nodes
+// (device, allocated through alloc_fn) and the sorted primitive order.
+typedef void *(*alloc_fn_t)(size_t);
+typedef void (*free_fn_t)(void *);
+struct BVHBuildResult {
+  BVHNode *nodes;
+  int *order;
+  int nnodes;
+  int depth;
+  float sah;
+};
+hipError_t build_lbvh(const float *boxes_dev, int n, alloc_fn_t alloc, free_fn_t dfree, BVHBuildResult *out,
+                      hipStream_t s);
+
+}  // namespace gxy

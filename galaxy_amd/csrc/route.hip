@@ -1,0 +1,416 @@
+// route.hip — ray classification, routing, framebuffer accumulation, stable
+// partition into per-destination lists, primary-ray generation, and ray copies.
+//
+// Renderer::Classify / AssignDestinations / HandleTerminatedRays
+// (Renderer.cpp:304-502) are scalar host loops in the reference; here one
+// kernel applies all three per ray (one work-item per ray, SoA loads), with
+// FP32 atomic adds into the framebuffer (Rendering::AddLocalPixels,
+// Rendering.cpp:125-153).  processRays_task's per-destination copy
+// (Renderer.cpp:561-634) becomes a stable multi-way partition: per-wave ballots
+// rank each ray within its category, per-workgroup totals are scanned per
+// category, and a scatter moves the 25 fields (100 B) of each surviving ray.
+#include <cstring>
+
+#include "gxy_device.h"
+#include "gxy_internal.h"
+
+namespace gxy {
+
+// Box::exit_face (Box.cpp:84-97), using the ray origin as the reference does.
+__device__ __forceinline__ int exit_face(const float *lo, const float *hi, float x, float y, float z, float dx,
+                                         float dy, float dz) {
+  const float FM = 3.402823466e+38f;
+  float tx = (dx > 0.0001f) ? ((hi[0] - x) / dx) : (dx < -0.0001f) ? ((lo[0] - x) / dx) : FM;
+  float ty = (dy > 0.0001f) ? ((hi[1] - y) / dy) : (dy < -0.0001f) ? ((lo[1] - y) / dy) : FM;
+  float tz = (dz > 0.0001f) ? ((hi[2] - z) / dz) : (dz < -0.0001f) ? ((lo[2] - z) / dz) : FM;
+  if (tx < 0) tx = FM;
+  if (ty < 0) ty = FM;
+  if (tz < 0) tz = FM;
+  if (tx < ty && tx < tz) return (dx < 0) ? 0 : 1;
+  else if (ty < tz) return (dy < 0) ? 2 : 3;
+  else return (dz < 0) ? 4 : 5;
+}
+
+__global__ __launch_bounds__(GXY_BLOCK) void k_route(RouteParams P, RayList_ispc R, int n) {
+  int i = blockIdx.x * GXY_BLOCK + threadIdx.x;
+  unsigned long long nterm = 0, ndrop = 0;
+  if (i < n) {
+    const int typ = R.type[i];
+    int c = R.classification[i];
+    if (P.mode & RM_CLASSIFY) {
+      const int term = R.term[i];
+      if (typ == RAY_PRIMARY) {
+        if (term & RAY_BOUNDARY) c = RAY_BOUNDARY;
+        else if ((term & RAY_OPAQUE) | (term & RAY_TIMEOUT)) c = GXY_TERMINATED;
+        else c = GXY_KEEP_HERE;
+      } else if (typ == RAY_SHADOW) {
+        if ((term & RAY_OPAQUE) | (term & RAY_SURFACE)) c = P.reverse ? GXY_TERMINATED : GXY_DROP_ON_FLOOR;
+        else if (term & RAY_BOUNDARY) c = RAY_BOUNDARY;
+        else c = GXY_DROP_ON_FLOOR;
+      } else if (typ == RAY_AO) {
+        if ((term & RAY_OPAQUE) | (term & RAY_SURFACE)) c = P.reverse ? GXY_TERMINATED : GXY_DROP_ON_FLOOR;
+        else if (term & RAY_BOUNDARY) c = RAY_BOUNDARY;
+        else if (term & RAY_TIMEOUT) c = P.reverse ? GXY_DROP_ON_FLOOR : GXY_TERMINATED;
+        else c = GXY_DROP_ON_FLOOR;
+      }
+    }
+    if ((P.mode & RM_ASSIGN) && c == RAY_BOUNDARY) {
+      int f = exit_face(P.blo, P.bhi, R.ox[i], R.oy[i], R.oz[i], R.dx[i], R.dy[i], R.dz[i]);
+      if (P.neighbors[f] >= 0) c = P.neighbors[f];
+      else if (typ == RAY_SHADOW || typ == RAY_AO) c = P.reverse ? GXY_DROP_ON_FLOOR : GXY_TERMINATED;
+      else c = GXY_TERMINATED;
+    }
+    if (P.mode & (RM_CLASSIFY | RM_ASSIGN)) R.classification[i] = c;
+    if ((P.mode & RM_ACCUMULATE) && c == GXY_TERMINATED) {
+      int x = R.x[i], y = R.y[i];
+      if (x >= 0 && y >= 0 && x < P.width && y < P.height) {
+        float *p = P.fb + 4 * ((int64_t)y * P.width + x);
+        atomicAdd(p + 0, R.r[i]);
+        atomicAdd(p + 1, R.g[i]);
+        atomicAdd(p + 2, R.b[i]);
+        atomicAdd(p + 3, R.o[i]);
+        nterm = 1;
+      }
+    }
+    if (c == GXY_DROP_ON_FLOOR) ndrop = 1;
+  }
+  if (P.counters) {
+    unsigned long long a = wave_sum_u64(nterm), b = wave_sum_u64(ndrop);
+    if ((threadIdx.x & 63) == 0) {
+      if (a) atomicAdd(P.counters, a);
+      if (b) atomicAdd(P.counters + 1, b);
+    }
+  }
+}
+
+hipError_t launch_route(const RouteParams &P, const RayList_ispc &R, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_route, dim3((n + GXY_BLOCK - 1) / GXY_BLOCK), dim3(GXY_BLOCK), 0, s, P, R, n);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ partition
+#define GXY_MAX_CAT 32
+
+__device__ __forceinline__ unsigned long long lanemask_lt64() {
+  const int lane = threadIdx.x & 63;
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// classification -> category map, passed by value: t[class + 4]
+#define GXY_CAT_TABLE 64
+struct CatTable {
+  int n;
+  int t[GXY_CAT_TABLE];
+};
+
+__device__ __forceinline__ int category(const RayList_ispc &R, int i, const CatTable &T) {
+  int c = R.classification[i] + 4;
+  return (c >= 0 && c < T.n) ? T.t[c] : -1;
+}
+
+// counts[k * nb + b] = rays of category k in workgroup b
+__global__ __launch_bounds__(GXY_BLOCK) void k_part_count(RayList_ispc R, int n, CatTable table, int K,
+                                                          int nb, int *counts) {
+  __shared__ int wc[GXY_BLOCK / 64][GXY_MAX_CAT];
+  int i = blockIdx.x * GXY_BLOCK + threadIdx.x;
+  int cat = i < n ? category(R, i, table) : -1;
+  int w = threadIdx.x >> 6;
+  for (int k = 0; k < K; k++) {
+    unsigned long long m = __ballot(cat == k);
+    if ((threadIdx.x & 63) == 0) wc[w][k] = __popcll(m);
+  }
+  __syncthreads();
+  if (threadIdx.x < K) {
+    int s = 0;
+    for (int q = 0; q < GXY_BLOCK / 64; q++) s += wc[q][threadIdx.x];
+    counts[threadIdx.x * nb + blockIdx.x] = s;
+  }
+}
+
+// one workgroup per category: exclusive scan over workgroups; total -> tot[k]
+__global__ __launch_bounds__(1024) void k_part_scan(int *counts, int nb, int *tot) {
+  __shared__ int part[1024];
+  int *a = counts + (int64_t)blockIdx.x * nb;
+  const int t = threadIdx.x;
+  const int per = (nb + 1023) / 1024;
+  const int s0 = t * per, s1 = min(nb, s0 + per);
+  int sum = 0;
+  for (int k = s0; k < s1; k++) sum += a[k];
+  part[t] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    int v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int base = t > 0 ? part[t - 1] : 0;
+  for (int k = s0; k < s1; k++) {
+    int v = a[k];
+    a[k] = base;
+    base += v;
+  }
+  if (t == 1023) tot[blockIdx.x] = part[1023];
+}
+
+// destination lists are passed by value in the kernel arguments
+#define GXY_MAX_SCATTER 12
+struct ListTable {
+  RayList_ispc l[GXY_MAX_SCATTER];
+};
+
+__device__ __forceinline__ void copy_one(const RayList_ispc &S, int i, const RayList_ispc &D, int j) {
+  float *const *sf = &S.ox;
+  float *const *df = &D.ox;
+#pragma unroll
+  for (int k = 0; k < 20; k++) df[k][j] = sf[k][i];
+  int *const *si = &S.x;
+  int *const *di = &D.x;
+#pragma unroll
+  for (int k = 0; k < 5; k++) di[k][j] = si[k][i];
+}
+
+__global__ __launch_bounds__(GXY_BLOCK) void k_part_scatter(RayList_ispc R, int n, CatTable table, int K,
+                                                            int nb, const int *base, ListTable lists) {
+  __shared__ int wc[GXY_BLOCK / 64][GXY_MAX_CAT];
+  int i = blockIdx.x * GXY_BLOCK + threadIdx.x;
+  int cat = i < n ? category(R, i, table) : -1;
+  int w = threadIdx.x >> 6;
+  int rank = 0;
+  for (int k = 0; k < K; k++) {
+    unsigned long long m = __ballot(cat == k);
+    if ((threadIdx.x & 63) == 0) wc[w][k] = __popcll(m);
+    if (cat == k) rank = __popcll(m & lanemask_lt64());
+  }
+  __syncthreads();
+  if (cat >= 0 && cat < K && cat < GXY_MAX_SCATTER) {
+    int off = base[cat * nb + blockIdx.x] + rank;
+    for (int q = 0; q < w; q++) off += wc[q][cat];
+    const RayList_ispc &D = lists.l[cat];
+    if (D.ox) copy_one(R, i, D, off);
+  }
+}
+
+static CatTable make_table(const int *cat_table, int ntable) {
+  CatTable T;
+  T.n = ntable;
+  for (int k = 0; k < GXY_CAT_TABLE; k++) T.t[k] = k < ntable ? cat_table[k] : -1;
+  return T;
+}
+
+hipError_t launch_partition(const RayList_ispc &R, int n, const int *cat_table, int ntable, int K,
+                            const RayList_ispc *lists, int *scratch, int *counts, hipStream_t s) {
+  if (K > GXY_MAX_CAT || ntable > GXY_CAT_TABLE) return hipErrorInvalidValue;
+  int nb = (n + GXY_BLOCK - 1) / GXY_BLOCK;
+  if (nb == 0) {
+    (void)hipMemsetAsync(counts, 0, K * sizeof(int), s);
+    return hipGetLastError();
+  }
+  CatTable T = make_table(cat_table, ntable);
+  hipLaunchKernelGGL(k_part_count, dim3(nb), dim3(GXY_BLOCK), 0, s, R, n, T, K, nb, scratch);
+  hipLaunchKernelGGL(k_part_scan, dim3(K), dim3(1024), 0, s, scratch, nb, counts);
+  if (lists) return launch_partition_scatter(R, n, cat_table, ntable, K, lists, scratch, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_partition_scatter(const RayList_ispc &R, int n, const int *cat_table, int ntable, int K,
+                                    const RayList_ispc *lists, const int *scratch, hipStream_t s) {
+  if (K > GXY_MAX_SCATTER || ntable > GXY_CAT_TABLE) return hipErrorInvalidValue;
+  int nb = (n + GXY_BLOCK - 1) / GXY_BLOCK;
+  if (nb == 0) return hipSuccess;
+  CatTable T = make_table(cat_table, ntable);
+  ListTable L;
+  memset(&L, 0, sizeof L);
+  for (int k = 0; k < K; k++) L.l[k] = lists[k];
+  hipLaunchKernelGGL(k_part_scatter, dim3(nb), dim3(GXY_BLOCK), 0, s, R, n, T, K, nb, scratch, L);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ sampler particles
+// Sampler::HandleTerminatedRays (Sampler.cpp:71-84): one particle per
+// RAY_SURFACE ray at o + t*d, appended in ray order.
+__global__ __launch_bounds__(GXY_BLOCK) void k_sampler_count(RayList_ispc R, int n, int *bc) {
+  __shared__ int ws[GXY_BLOCK / 64];
+  int i = blockIdx.x * GXY_BLOCK + threadIdx.x;
+  bool f = i < n && (R.term[i] & RAY_SURFACE);
+  unsigned long long m = __ballot(f);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = __popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) bc[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+__global__ __launch_bounds__(GXY_BLOCK) void k_sampler_write(RayList_ispc R, int n, const int *bbase, const int *count,
+                                                             float *xyz, int capacity) {
+  __shared__ int ws[GXY_BLOCK / 64];
+  int i = blockIdx.x * GXY_BLOCK + threadIdx.x;
+  bool f = i < n && (R.term[i] & RAY_SURFACE);
+  unsigned long long m = __ballot(f);
+  int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) ws[w] = __popcll(m);
+  __syncthreads();
+  if (!f) return;
+  int off = *count + bbase[blockIdx.x] + __popcll(m & lanemask_lt64());
+  for (int q = 0; q < w; q++) off += ws[q];
+  if (off < capacity) {
+    float t = R.t[i];
+    xyz[3 * (int64_t)off + 0] = R.ox[i] + t * R.dx[i];
+    xyz[3 * (int64_t)off + 1] = R.oy[i] + t * R.dy[i];
+    xyz[3 * (int64_t)off + 2] = R.oz[i] + t * R.dz[i];
+  }
+}
+
+__global__ void k_add_count(int *count, const int *add) { *count += *add; }
+
+hipError_t launch_accumulate_sampler(const RayList_ispc &R, int n, float *xyz, int capacity, int *count,
+                                     int *scratch, hipStream_t s) {
+  int nb = (n + GXY_BLOCK - 1) / GXY_BLOCK;
+  if (nb == 0) return hipSuccess;
+  int *tot = scratch + nb + 1;
+  hipLaunchKernelGGL(k_sampler_count, dim3(nb), dim3(GXY_BLOCK), 0, s, R, n, scratch);
+  hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, scratch, nb, tot);
+  hipLaunchKernelGGL(k_sampler_write, dim3(nb), dim3(GXY_BLOCK), 0, s, R, n, scratch, count, xyz, capacity);
+  hipLaunchKernelGGL(k_add_count, dim3(1), dim3(1), 0, s, count, tot);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ camera
+// Box::intersect (Box.cpp:107-150)
+__device__ __forceinline__ bool box_intersect(const float *lo, const float *hi, f3 org, f3 dir, float &tmin,
+                                              float &tmax) {
+  tmin = (lo[0] - org.x) / dir.x;
+  tmax = (hi[0] - org.x) / dir.x;
+  if (tmin > tmax) { float t = tmax; tmax = tmin; tmin = t; }
+  if (tmax < 0) return false;
+  float tymin = (lo[1] - org.y) / dir.y, tymax = (hi[1] - org.y) / dir.y;
+  if (tymin > tymax) { float t = tymax; tymax = tymin; tymin = t; }
+  if (tymax < 0) return false;
+  if ((tmin > tymax) || (tymin > tmax)) return false;
+  if (tymin > tmin) tmin = tymin;
+  if (tymax < tmax) tmax = tymax;
+  float tzmin = (lo[2] - org.z) / dir.z, tzmax = (hi[2] - org.z) / dir.z;
+  if (tzmin > tzmax) { float t = tzmin; tzmin = tzmax; tzmax = t; }
+  if (tzmax < 0) return false;
+  if ((tmin > tzmax) || (tzmin > tmax)) return false;
+  if (tzmin > tmin) tmin = tzmin;
+  if (tzmax < tmax) tmax = tzmax;
+  if (tmin < 0) tmin = 0;
+  return true;
+}
+
+// pixel of the q-th ray slot: row major, or 8x8 tiles (tile-major, row major
+// inside a tile) for locality of neighbouring rays in a wavefront
+__device__ __forceinline__ void pixel_of(const CameraParams &C, int q, int &x, int &y) {
+  if (C.order == 0) {
+    x = q % C.width;
+    y = q / C.width;
+  } else {
+    int tw = (C.width + 7) / 8;
+    int tile = q / 64, r = q % 64;
+    x = (tile % tw) * 8 + (r % 8);
+    y = (tile / tw) * 8 + (r / 8);
+  }
+}
+
+// dtypes.h normalize: d = len(a); if (d != 0) { d = 1.0/d (double); a *= d }
+__device__ __forceinline__ f3 dt_normalize(f3 a) {
+  float d = sqrtf(a.x * a.x + a.y * a.y + a.z * a.z);
+  if (d != 0) {
+    d = (float)(1.0 / (double)d);
+    a.x *= d; a.y *= d; a.z *= d;
+  }
+  return a;
+}
+
+__device__ __forceinline__ bool camera_ray(const CameraParams &C, int x, int y, f3 &vorigin, f3 &vray) {
+  float fx = ((float)x - C.off_x) * C.scaling;
+  float fy = ((float)y - C.off_y) * C.scaling;
+  f3 xy = mk3(C.center[0] + fx * C.vr[0] + fy * C.vu[0], C.center[1] + fx * C.vr[1] + fy * C.vu[1],
+              C.center[2] + fx * C.vr[2] + fy * C.vu[2]);
+  f3 vdir = mk3(C.vdir[0], C.vdir[1], C.vdir[2]);
+  if (C.is_ortho) {
+    vorigin = sub3(xy, vdir);
+    vray = vdir;
+  } else {
+    vorigin = mk3(C.veye[0], C.veye[1], C.veye[2]);
+    vray = dt_normalize(sub3(xy, vorigin));
+  }
+  float gmin = 0, gmax = 0, lmin = 0, lmax = 0;
+  bool hit = box_intersect(C.glo, C.ghi, vorigin, vray, gmin, gmax);
+  if (hit) hit = box_intersect(C.llo, C.lhi, vorigin, vray, lmin, lmax);
+  float d = fabsf(lmin) - fabsf(gmin);
+  return hit && (lmax >= 0) && ((double)d < 0.000001) && ((double)d > -0.000001);
+}
+
+__global__ __launch_bounds__(GXY_BLOCK) void k_camera_count(CameraParams C, int npix, int *bc) {
+  __shared__ int ws[GXY_BLOCK / 64];
+  int q = blockIdx.x * GXY_BLOCK + threadIdx.x;
+  bool f = false;
+  if (q < npix) {
+    int x, y;
+    pixel_of(C, q, x, y);
+    f3 o, d;
+    f = x < C.width && y < C.height && camera_ray(C, x, y, o, d);
+  }
+  unsigned long long m = __ballot(f);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = __popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) bc[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+__global__ __launch_bounds__(GXY_BLOCK) void k_camera_write(CameraParams C, int npix, const int *bbase,
+                                                            RayList_ispc R, int capacity) {
+  __shared__ int ws[GXY_BLOCK / 64];
+  int q = blockIdx.x * GXY_BLOCK + threadIdx.x;
+  bool f = false;
+  int x = 0, y = 0;
+  f3 o = mk3(0, 0, 0), d = o;
+  if (q < npix) {
+    pixel_of(C, q, x, y);
+    f = x < C.width && y < C.height && camera_ray(C, x, y, o, d);
+  }
+  unsigned long long m = __ballot(f);
+  int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) ws[w] = __popcll(m);
+  __syncthreads();
+  if (!f) return;
+  int off = bbase[blockIdx.x] + __popcll(m & lanemask_lt64());
+  for (int k = 0; k < w; k++) off += ws[k];
+  if (off >= capacity) return;
+  R.x[off] = x; R.y[off] = y;
+  R.ox[off] = o.x; R.oy[off] = o.y; R.oz[off] = o.z;
+  R.dx[off] = d.x; R.dy[off] = d.y; R.dz[off] = d.z;
+  R.r[off] = 0.f; R.g[off] = 0.f; R.b[off] = 0.f; R.o[off] = 0.f;
+  R.t[off] = 0.f; R.tMax[off] = 3.402823466e+38f;
+  R.type[off] = RAY_PRIMARY; R.term[off] = 0; R.classification[off] = 0;
+}
+
+hipError_t launch_camera(const CameraParams &C, const RayList_ispc &R, int capacity, int *scratch, int *total,
+                         hipStream_t s) {
+  int npix;
+  if (C.order == 0) npix = C.width * C.height;
+  else npix = ((C.width + 7) / 8) * ((C.height + 7) / 8) * 64;
+  int nb = (npix + GXY_BLOCK - 1) / GXY_BLOCK;
+  hipLaunchKernelGGL(k_camera_count, dim3(nb), dim3(GXY_BLOCK), 0, s, C, npix, scratch);
+  hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, scratch, nb, total);
+  hipLaunchKernelGGL(k_camera_write, dim3(nb), dim3(GXY_BLOCK), 0, s, C, npix, scratch, R, capacity);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ copies
+__global__ __launch_bounds__(GXY_BLOCK) void k_copy_rays(RayList_ispc S, int start, int count, RayList_ispc D,
+                                                         int dst_start) {
+  int i = blockIdx.x * GXY_BLOCK + threadIdx.x;
+  if (i >= count) return;
+  copy_one(S, start + i, D, dst_start + i);
+}
+
+hipError_t launch_copy_rays(const RayList_ispc &src, int start, int count, const RayList_ispc &dst, int dst_start,
+                            hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_rays, dim3((count + GXY_BLOCK - 1) / GXY_BLOCK), dim3(GXY_BLOCK), 0, s, src, start, count,
+                     dst, dst_start);
+  return hipGetLastError();
+}
+
+}  // namespace gxy

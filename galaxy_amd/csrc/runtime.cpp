@@ -1,0 +1,1726 @@
+// runtime.cpp — host side of libgxy_hip.so: device selection, stream, caching
+// device allocator, scene objects resident in HBM, device RayLists with the
+// reference block layout, the C-ABI entry points of include/gxy_hip.h, and the
+// ray-list loop of Renderer/processRays_task run against device lists.
+//
+// Host semantics cited per entry point; kernels live in trace.hip, spawn.hip,
+// route.hip, bvh.hip and noise.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "gxy_internal.h"
+
+#define GXY_EXPORT extern "C" __attribute__((visibility("default")))
+
+namespace gxy {
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string t_err;
+static int fail(const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  t_err = buf;
+  if (getenv("GXY_VERBOSE")) fprintf(stderr, "gxy: %s\n", buf);
+  return -1;
+}
+#define HIPCHK(expr)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) return fail("%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+  } while (0)
+
+// ------------------------------------------------------------------ runtime state
+struct Timing {
+  double ms = 0;
+  int launches = 0;
+};
+struct PendingEv {
+  std::string name;
+  hipEvent_t a, b;
+};
+
+struct Runtime {
+  std::recursive_mutex mu;
+  int device = -1;
+  hipStream_t stream = nullptr;
+  hipStream_t user_stream = nullptr;
+  // caching allocator: size class -> free blocks; live block -> size class
+  std::multimap<size_t, void *> free_blocks;
+  std::map<void *, size_t> live;
+  size_t bytes_live = 0, bytes_cached = 0;
+  // timing
+  bool timing = false;
+  std::vector<PendingEv> pending;
+  std::map<std::string, Timing> timings;
+  // per-runtime device counters: [0] samples [1] terminated [2] dropped
+  unsigned long long *counters = nullptr;
+  // registries for handle validation
+  std::set<void *> volumes, tfs, models, raylists, tracers, lightings;
+  std::atomic<int> raylist_id{0};
+};
+static Runtime G;
+
+static hipStream_t S() { return G.user_stream ? G.user_stream : G.stream; }
+
+static size_t size_class(size_t b) {
+  if (b < 256) return 256;
+  if (b <= (1u << 20)) {
+    size_t c = 256;
+    while (c < b) c <<= 1;
+    return c;
+  }
+  // eighth-octave classes above 1 MiB, rounded to 64 KiB
+  size_t c = 1u << 20;
+  while (c * 2 <= b) c <<= 1;
+  size_t step = c / 8;
+  size_t r = ((b + step - 1) / step) * step;
+  return (r + 65535) & ~(size_t)65535;
+}
+
+static void release_cache() {
+  for (auto &kv : G.free_blocks) hipFree(kv.second);
+  G.free_blocks.clear();
+  G.bytes_cached = 0;
+}
+
+void *dalloc(size_t bytes) {
+  std::lock_guard<std::recursive_mutex> lk(G.mu);
+  size_t c = size_class(bytes ? bytes : 1);
+  auto it = G.free_blocks.find(c);
+  void *p = nullptr;
+  if (it != G.free_blocks.end()) {
+    p = it->second;
+    G.free_blocks.erase(it);
+    G.bytes_cached -= c;
+  } else {
+    if (hipMalloc(&p, c) != hipSuccess) {
+      (void)hipGetLastError();
+      hipDeviceSynchronize();
+      release_cache();
+      if (hipMalloc(&p, c) != hipSuccess) {
+        (void)hipGetLastError();
+        fail("device allocation of %zu bytes failed (%zu live)", c, G.bytes_live);
+        return nullptr;
+      }
+    }
+  }
+  G.live[p] = c;
+  G.bytes_live += c;
+  return p;
+}
+
+void dfree(void *p) {
+  if (!p) return;
+  std::lock_guard<std::recursive_mutex> lk(G.mu);
+  auto it = G.live.find(p);
+  if (it == G.live.end()) return;
+  size_t c = it->second;
+  G.live.erase(it);
+  G.bytes_live -= c;
+  // blocks are reused only after the work queued before the free has run
+  hipStreamSynchronize(S());
+  G.free_blocks.insert({c, p});
+  G.bytes_cached += c;
+}
+
+// free without a stream synchronisation: only for blocks whose last use is
+// ordered before any later use on the same stream
+static void dfree_stream_ordered(void *p) {
+  if (!p) return;
+  std::lock_guard<std::recursive_mutex> lk(G.mu);
+  auto it = G.live.find(p);
+  if (it == G.live.end()) return;
+  size_t c = it->second;
+  G.live.erase(it);
+  G.bytes_live -= c;
+  G.free_blocks.insert({c, p});
+  G.bytes_cached += c;
+}
+
+// ------------------------------------------------------------------ timing
+struct KTimer {
+  const char *name;
+  hipEvent_t a = nullptr, b = nullptr;
+  explicit KTimer(const char *n) : name(n) {
+    if (G.timing) {
+      hipEventCreate(&a);
+      hipEventCreate(&b);
+      hipEventRecord(a, S());
+    }
+  }
+  ~KTimer() {
+    if (a) {
+      hipEventRecord(b, S());
+      std::lock_guard<std::recursive_mutex> lk(G.mu);
+      G.pending.push_back({name, a, b});
+    }
+  }
+};
+
+static void resolve_timings() {
+  std::lock_guard<std::recursive_mutex> lk(G.mu);
+  for (auto &p : G.pending) {
+    hipEventSynchronize(p.b);
+    float ms = 0;
+    hipEventElapsedTime(&ms, p.a, p.b);
+    G.timings[p.name].ms += ms;
+    G.timings[p.name].launches++;
+    hipEventDestroy(p.a);
+    hipEventDestroy(p.b);
+  }
+  G.pending.clear();
+}
+
+// ------------------------------------------------------------------ UV tables
+// UV.ih:21-58: base-2 / base-3 radical inverses accumulated in float with the
+// digit weight multiplied by 1/b each step, rounded to 6 significant digits.
+static void make_uv(float *U, float *V) {
+  for (int b = 2; b <= 3; b++)
+    for (int i = 0; i < 256; i++) {
+      float inv = 1.0f / (float)b, f = inv, r = 0.f;
+      for (int j = i; j > 0; j /= b) {
+        r = r + (float)(j % b) * f;
+        f = f * inv;
+      }
+      char buf[64];
+      snprintf(buf, sizeof buf, "%.6g", (double)r);
+      (b == 2 ? U : V)[i] = strtof(buf, nullptr);
+    }
+}
+
+static int ensure_init() {
+  if (G.device >= 0) return 0;
+  std::lock_guard<std::recursive_mutex> lk(G.mu);
+  if (G.device >= 0) return 0;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    (void)hipGetLastError();
+    return fail("no HIP device available (libgxy_hip requires an AMD GPU)");
+  }
+  int dev = 0;
+  if (getenv("GXY_DEVICE")) dev = atoi(getenv("GXY_DEVICE"));
+  HIPCHK(hipSetDevice(dev));
+  HIPCHK(hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking));
+  float U[256], V[256];
+  make_uv(U, V);
+  HIPCHK(upload_uv_tables(U, V));
+  G.counters = (unsigned long long *)nullptr;
+  G.device = dev;
+  G.counters = (unsigned long long *)dalloc(8 * sizeof(unsigned long long));
+  if (!G.counters) return -1;
+  HIPCHK(hipMemsetAsync(G.counters, 0, 8 * sizeof(unsigned long long), S()));
+  return 0;
+}
+#define INIT_OR(ret) \
+  do {               \
+    if (ensure_init() != 0) return ret; \
+  } while (0)
+
+// ------------------------------------------------------------------ scene objects
+struct VolumeObj {
+  DVolume d;
+  float *data = nullptr;
+  size_t count = 0;
+};
+
+struct TFObj {
+  DTF d;
+  float *col = nullptr, *op = nullptr;
+  std::vector<float> hcol, hop;
+};
+
+struct MeshObj {
+  int geomID;
+  int nv, nt;
+  float *V = nullptr, *N = nullptr, *D = nullptr;
+  uint32_t *I = nullptr;
+  TFObj *tf = nullptr;
+};
+
+struct SpheresObj {
+  int geomID;
+  int n;
+  float *C = nullptr, *D = nullptr;
+  float r0, r1, v0, v1, eps;
+  TFObj *tf = nullptr;
+};
+
+struct ModelObj {
+  std::vector<MeshObj> meshes;
+  std::vector<SpheresObj> spheres;
+  int ngeom = 0;
+  bool committed = false;
+  DModel d;
+  std::vector<void *> owned;   // device buffers built at commit
+  DGeomShade *shade = nullptr;
+  int *hit_geom = nullptr, *hit_prim = nullptr;
+  int tnodes = 0, snodes = 0, tdepth = 0, sdepth = 0;
+  void free_built() {
+    for (void *p : owned) dfree(p);
+    owned.clear();
+    committed = false;
+  }
+};
+
+struct RayListObj {
+  void *block = nullptr;
+  size_t bytes = 0;
+  gxy_raylist_hdr hdr;
+  RayList_ispc ispc;
+};
+
+static void set_pointers(RayListObj *r) {
+  char *base = (char *)r->block + 64;
+  size_t nn = (size_t)r->hdr.aligned_size;
+  float **fp = &r->ispc.ox;
+  for (int k = 0; k < 20; k++) fp[k] = (float *)(base + (size_t)k * nn * 4);
+  int **ip = &r->ispc.x;
+  for (int k = 0; k < 5; k++) ip[k] = (int *)(base + (size_t)(20 + k) * nn * 4);
+}
+
+static RayListObj *raylist_new(int nrays, int frame, int type) {
+  if (nrays < 0) {
+    fail("negative ray count");
+    return nullptr;
+  }
+  RayListObj *r = new RayListObj;
+  memset(&r->hdr, 0, sizeof r->hdr);
+  int nn = (nrays + 15) & ~15;
+  r->hdr.size = nrays;
+  r->hdr.frame = frame;
+  r->hdr.aligned_size = nn;
+  r->hdr.type = type;
+  r->hdr.id = G.raylist_id++;
+  r->bytes = 64 + (size_t)nn * 100;
+  r->block = dalloc(r->bytes);
+  if (!r->block) {
+    delete r;
+    return nullptr;
+  }
+  set_pointers(r);
+  hipMemcpyAsync(r->block, &r->hdr, sizeof r->hdr, hipMemcpyHostToDevice, S());
+  hipStreamSynchronize(S());
+  std::lock_guard<std::recursive_mutex> lk(G.mu);
+  G.raylists.insert(r);
+  return r;
+}
+
+static void raylist_delete(RayListObj *r, bool sync = true) {
+  if (!r) return;
+  {
+    std::lock_guard<std::recursive_mutex> lk(G.mu);
+    G.raylists.erase(r);
+  }
+  if (sync) dfree(r->block);
+  else dfree_stream_ordered(r->block);
+  delete r;
+}
+
+template <class T>
+static T *lookup(std::set<void *> &reg, void *h) {
+  std::lock_guard<std::recursive_mutex> lk(G.mu);
+  return reg.count(h) ? (T *)h : nullptr;
+}
+
+// ------------------------------------------------------------------ param builders
+static int make_trace_params(const Visualization_ispc *vis, float global_eps, TraceParams &P) {
+  memset(&P, 0, sizeof P);
+  if (!vis) return fail("null Visualization_ispc");
+  if (vis->nVolumeVis < 0 || vis->nVolumeVis > GXY_MAX_VOLVIS)
+    return fail("%d volume visualizations; at most %d per launch", vis->nVolumeVis, GXY_MAX_VOLVIS);
+  P.nvv = vis->nVolumeVis;
+  P.step = -1;
+  for (int m = 0; m < vis->nVolumeVis; m++) {
+    const VolumeVis_ispc *vv = vis->volumeVis[m];
+    if (!vv) return fail("null VolumeVis_ispc %d", m);
+    VolumeObj *V = lookup<VolumeObj>(G.volumes, vv->vis.vis.data);
+    TFObj *T = lookup<TFObj>(G.tfs, vv->vis.transferFunction);
+    if (!V) return fail("VolumeVis %d: data is not a gxy volume handle", m);
+    if (!T) return fail("VolumeVis %d: transferFunction is not a gxy TF handle", m);
+    if (vv->nSlices > GXY_MAX_SLICES) return fail("%d slices; at most %d", vv->nSlices, GXY_MAX_SLICES);
+    if (vv->nIsovalues > GXY_MAX_ISO) return fail("%d isovalues; at most %d", vv->nIsovalues, GXY_MAX_ISO);
+    DVolVis &D = P.vv[m];
+    D.vol = V->d;
+    D.tf = T->d;
+    D.nSlices = vv->nSlices;
+    for (int k = 0; k < 4 * vv->nSlices; k++) D.slices[k] = vv->slices[k];
+    D.nIso = vv->nIsovalues;
+    for (int k = 0; k < vv->nIsovalues; k++) D.iso[k] = vv->isovalues[k];
+    D.dvr = vv->volume_render ? 1 : 0;
+    if (D.dvr || D.nIso > 0) P.integrate = 1;
+    // TraceRays.ispc:342-361
+    float s = V->d.step * V->d.rate;
+    if (P.step < 0 || P.step > s) {
+      P.step = s;
+      P.eps = global_eps * s;
+    }
+  }
+  for (int k = 0; k < 3; k++) {
+    P.blo[k] = vis->local_bb.lower[k];
+    P.bhi[k] = vis->local_bb.upper[k];
+  }
+  if (vis->model) {
+    ModelObj *M = lookup<ModelObj>(G.models, vis->model);
+    if (!M) return fail("Visualization model is not a gxy model handle");
+    if (!M->committed) return fail("model used before gxy_model_commit");
+    P.has_model = (M->d.ntris > 0 || M->d.nsph > 0) ? 1 : 0;
+    P.model = M->d;
+    P.hit_geom = M->hit_geom;
+    P.hit_prim = M->hit_prim;
+  }
+  P.sample_counter = G.counters;
+  return 0;
+}
+
+static int make_light_params(const Lighting_ispc *l, float eps, LightParams &L) {
+  memset(&L, 0, sizeof L);
+  if (!l) return fail("null Lighting_ispc");
+  if (l->nLights > 16) return fail("%u lights; at most 16", l->nLights);
+  L.nAO = l->n_ao_rays;
+  L.nLights = (int)l->nLights;
+  L.doShadows = l->do_shadows;
+  L.aoRadius = l->ao_radius;
+  L.Ka = l->Ka;
+  L.Kd = l->Kd;
+  L.eps = eps;
+  for (int k = 0; k < L.nLights; k++) {
+    for (int a = 0; a < 3; a++) L.L[3 * k + a] = l->lights[3 * k + a];
+    L.types[k] = l->types ? l->types[k] : 2;
+  }
+  return 0;
+}
+
+}  // namespace gxy
+
+using namespace gxy;
+
+// ====================================================================== C-ABI
+GXY_EXPORT int gxy_init(int device) {
+  if (G.device >= 0) return 0;
+  char buf[16];
+  snprintf(buf, sizeof buf, "%d", device);
+  if (device > 0) setenv("GXY_DEVICE", buf, 1);
+  return ensure_init();
+}
+GXY_EXPORT const char *gxy_last_error(void) { return t_err.c_str(); }
+GXY_EXPORT int gxy_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+GXY_EXPORT int gxy_synchronize(void) {
+  INIT_OR(-1);
+  HIPCHK(hipStreamSynchronize(S()));
+  return 0;
+}
+GXY_EXPORT int gxy_set_stream(void *s) {
+  INIT_OR(-1);
+  G.user_stream = (hipStream_t)s;
+  return 0;
+}
+GXY_EXPORT void *gxy_get_stream(void) {
+  INIT_OR(nullptr);
+  return (void *)S();
+}
+GXY_EXPORT void *gxy_device_alloc(size_t bytes) {
+  INIT_OR(nullptr);
+  return dalloc(bytes);
+}
+GXY_EXPORT int gxy_device_free(void *p) {
+  INIT_OR(-1);
+  dfree(p);
+  return 0;
+}
+GXY_EXPORT int gxy_copy_to_device(void *dst, const void *src, size_t bytes) {
+  INIT_OR(-1);
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S()));
+  HIPCHK(hipStreamSynchronize(S()));
+  return 0;
+}
+GXY_EXPORT int gxy_copy_to_host(void *dst, const void *src, size_t bytes) {
+  INIT_OR(-1);
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, S()));
+  HIPCHK(hipStreamSynchronize(S()));
+  return 0;
+}
+GXY_EXPORT int gxy_copy_device(void *dst, const void *src, size_t bytes) {
+  INIT_OR(-1);
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, S()));
+  return 0;
+}
+GXY_EXPORT int gxy_memset(void *dst, int value, size_t bytes) {
+  INIT_OR(-1);
+  HIPCHK(hipMemsetAsync(dst, value, bytes, S()));
+  return 0;
+}
+GXY_EXPORT size_t gxy_device_bytes_in_use(void) { return G.bytes_live; }
+GXY_EXPORT size_t gxy_device_bytes_total(void) {
+  if (ensure_init() != 0) return 0;
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 0;
+  return tot;
+}
+
+GXY_EXPORT int gxy_timing_enable(int on) {
+  INIT_OR(-1);
+  G.timing = on != 0;
+  return 0;
+}
+GXY_EXPORT int gxy_timing_reset(void) {
+  INIT_OR(-1);
+  resolve_timings();
+  G.timings.clear();
+  return 0;
+}
+GXY_EXPORT int gxy_timing_get(const char *kernel, double *total_ms, int *launches) {
+  INIT_OR(-1);
+  resolve_timings();
+  auto it = G.timings.find(kernel);
+  *total_ms = it == G.timings.end() ? 0.0 : it->second.ms;
+  *launches = it == G.timings.end() ? 0 : it->second.launches;
+  return 0;
+}
+
+// ------------------------------------------------------------------ RayLists
+GXY_EXPORT void *gxy_raylist_create(int nrays, int frame, int type, RayList_ispc *ispc) {
+  INIT_OR(nullptr);
+  RayListObj *r = raylist_new(nrays, frame, type);
+  if (r && ispc) *ispc = r->ispc;
+  return r;
+}
+GXY_EXPORT int gxy_raylist_destroy(void *rl) {
+  RayListObj *r = lookup<RayListObj>(G.raylists, rl);
+  if (!r) return fail("not a raylist handle");
+  raylist_delete(r);
+  return 0;
+}
+GXY_EXPORT int gxy_raylist_size(void *rl) {
+  RayListObj *r = lookup<RayListObj>(G.raylists, rl);
+  return r ? r->hdr.size : fail("not a raylist handle");
+}
+GXY_EXPORT int gxy_raylist_aligned_size(void *rl) {
+  RayListObj *r = lookup<RayListObj>(G.raylists, rl);
+  return r ? r->hdr.aligned_size : fail("not a raylist handle");
+}
+GXY_EXPORT void *gxy_raylist_split_part(void *rl, int start, int count, RayList_ispc *ispc) {
+  RayListObj *r = lookup<RayListObj>(G.raylists, rl);
+  if (!r) {
+    fail("not a raylist handle");
+    return nullptr;
+  }
+  if (start < 0 || count < 0 || start + count > r->hdr.size) {
+    fail("split range [%d, %d) outside list of %d", start, start + count, r->hdr.size);
+    return nullptr;
+  }
+  RayListObj *p = raylist_new(count, r->hdr.frame, r->hdr.type);
+  if (!p) return nullptr;
+  if (launch_copy_rays(r->ispc, start, count, p->ispc, 0, S()) != hipSuccess) {
+    fail("copy kernel failed");
+    raylist_delete(p);
+    return nullptr;
+  }
+  if (ispc) *ispc = p->ispc;
+  return p;
+}
+GXY_EXPORT int gxy_raylist_truncate(void *rl, int n, RayList_ispc *ispc) {
+  RayListObj *r = lookup<RayListObj>(G.raylists, rl);
+  if (!r) return fail("not a raylist handle");
+  if (n < 0 || n > r->hdr.size) return fail("truncate to %d of %d rays", n, r->hdr.size);
+  int nn = (n + 15) & ~15;
+  // Rays.cpp:245-286: shrink the block only when the aligned size drops
+  if (nn < r->hdr.aligned_size) {
+    RayListObj tmp = *r;
+    r->hdr.size = n;
+    r->hdr.aligned_size = nn;
+    r->bytes = 64 + (size_t)nn * 100;
+    r->block = dalloc(r->bytes);
+    if (!r->block) return -1;
+    set_pointers(r);
+    if (launch_copy_rays(tmp.ispc, 0, n, r->ispc, 0, S()) != hipSuccess) return fail("copy kernel failed");
+    HIPCHK(hipMemcpyAsync(r->block, &r->hdr, sizeof r->hdr, hipMemcpyHostToDevice, S()));
+    dfree(tmp.block);
+  } else {
+    r->hdr.size = n;
+    HIPCHK(hipMemcpyAsync(r->block, &r->hdr, sizeof r->hdr, hipMemcpyHostToDevice, S()));
+    HIPCHK(hipStreamSynchronize(S()));
+  }
+  if (ispc) *ispc = r->ispc;
+  return 0;
+}
+GXY_EXPORT size_t gxy_raylist_bytes(void *rl) {
+  RayListObj *r = lookup<RayListObj>(G.raylists, rl);
+  return r ? r->bytes : 0;
+}
+GXY_EXPORT int gxy_raylist_download(void *rl, void *host_block) {
+  RayListObj *r = lookup<RayListObj>(G.raylists, rl);
+  if (!r) return fail("not a raylist handle");
+  HIPCHK(hipMemcpyAsync(host_block, r->block, r->bytes, hipMemcpyDeviceToHost, S()));
+  HIPCHK(hipStreamSynchronize(S()));
+  return 0;
+}
+GXY_EXPORT void *gxy_raylist_upload(const void *host_block, RayList_ispc *ispc) {
+  INIT_OR(nullptr);
+  const gxy_raylist_hdr *h = (const gxy_raylist_hdr *)host_block;
+  if (h->aligned_size != ((h->size + 15) & ~15) || h->size < 0) {
+    fail("raylist block header inconsistent (size %d aligned %d)", h->size, h->aligned_size);
+    return nullptr;
+  }
+  RayListObj *r = raylist_new(h->size, h->frame, h->type);
+  if (!r) return nullptr;
+  r->hdr = *h;
+  if (hipMemcpyAsync(r->block, host_block, r->bytes, hipMemcpyHostToDevice, S()) != hipSuccess ||
+      hipStreamSynchronize(S()) != hipSuccess) {
+    fail("upload failed");
+    raylist_delete(r);
+    return nullptr;
+  }
+  if (ispc) *ispc = r->ispc;
+  return r;
+}
+
+// ------------------------------------------------------------------ volumes
+GXY_EXPORT void *gxy_volume_create(const int dims[3], const float origin[3], const float spacing[3],
+                                   const float *voxels, float samplingRate) {
+  INIT_OR(nullptr);
+  for (int k = 0; k < 3; k++)
+    if (dims[k] < 2) {
+      fail("volume dimension %d is %d; at least 2 voxels per axis", k, dims[k]);
+      return nullptr;
+    }
+  VolumeObj *v = new VolumeObj;
+  v->count = (size_t)dims[0] * dims[1] * dims[2];
+  v->data = (float *)dalloc(v->count * 4);
+  if (!v->data) {
+    delete v;
+    return nullptr;
+  }
+  for (int k = 0; k < 3; k++) {
+    v->d.n[k] = dims[k];
+    v->d.org[k] = origin[k];
+    v->d.sp[k] = spacing[k];
+    v->d.isp[k] = 1.0f / spacing[k];
+    v->d.ub[k] = std::nextafter((float)(dims[k] - 1), 0.0f);
+  }
+  v->d.data = v->data;
+  v->d.rate = samplingRate;
+  v->d.step = std::min(spacing[0], std::min(spacing[1], spacing[2]));
+  if (voxels) {
+    if (hipMemcpyAsync(v->data, voxels, v->count * 4, hipMemcpyHostToDevice, S()) != hipSuccess ||
+        hipStreamSynchronize(S()) != hipSuccess) {
+      fail("volume upload failed");
+      dfree(v->data);
+      delete v;
+      return nullptr;
+    }
+  }
+  std::lock_guard<std::recursive_mutex> lk(G.mu);
+  G.volumes.insert(v);
+  return v;
+}
+GXY_EXPORT int gxy_volume_destroy(void *vol) {
+  VolumeObj *v = lookup<VolumeObj>(G.volumes, vol);
+  if (!v) return fail("not a volume handle");
+  {
+    std::lock_guard<std::recursive_mutex> lk(G.mu);
+    G.volumes.erase(v);
+  }
+  dfree(v->data);
+  delete v;
+  return 0;
+}
+GXY_EXPORT float *gxy_volume_device_data(void *vol) {
+  VolumeObj *v = lookup<VolumeObj>(G.volumes, vol);
+  return v ? v->data : nullptr;
+}
+GXY_EXPORT int gxy_volume_set_sampling_rate(void *vol, float rate) {
+  VolumeObj *v = lookup<VolumeObj>(G.volumes, vol);
+  if (!v) return fail("not a volume handle");
+  v->d.rate = rate;
+  return 0;
+}
+GXY_EXPORT int gxy_volume_fill_noise(void *vol, uint32_t seed, int octaves, float base_frequency) {
+  VolumeObj *v = lookup<VolumeObj>(G.volumes, vol);
+  if (!v) return fail("not a volume handle");
+  // seeded permutation: Fisher-Yates driven by a PCG32 stream
+  int perm[256];
+  for (int i = 0; i < 256; i++) perm[i] = i;
+  uint64_t state = 0x853c49e6748fea9bull ^ ((uint64_t)seed << 1), inc = ((uint64_t)seed << 1) | 1u;
+  auto pcg = [&]() {
+    uint64_t old = state;
+    state = old * 6364136223846793005ull + inc;
+    uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((-rot) & 31));
+  };
+  for (int i = 255; i > 0; i--) {
+    int j = (int)(pcg() % (uint32_t)(i + 1));
+    std::swap(perm[i], perm[j]);
+  }
+  int *dperm = (int *)dalloc(256 * sizeof(int) + 16);
+  float *mm = (float *)(dperm + 256);
+  if (!dperm) return -1;
+  HIPCHK(hipMemcpyAsync(dperm, perm, sizeof perm, hipMemcpyHostToDevice, S()));
+  HIPCHK(launch_noise(v->data, v->d.n[0], v->d.n[1], v->d.n[2], dperm, octaves, base_frequency, mm, S()));
+  HIPCHK(hipStreamSynchronize(S()));
+  dfree(dperm);
+  return 0;
+}
+GXY_EXPORT int gxy_volume_download(void *vol, float *host) {
+  VolumeObj *v = lookup<VolumeObj>(G.volumes, vol);
+  if (!v) return fail("not a volume handle");
+  HIPCHK(hipMemcpyAsync(host, v->data, v->count * 4, hipMemcpyDeviceToHost, S()));
+  HIPCHK(hipStreamSynchronize(S()));
+  return 0;
+}
+
+// ------------------------------------------------------------------ TFs
+static void *tf_make(std::vector<float> col, std::vector<float> op, float lo, float hi) {
+  if (col.size() / 3 > GXY_TF_MAX || op.size() > GXY_TF_MAX) {
+    fail("transfer function tables are limited to %d entries", GXY_TF_MAX);
+    return nullptr;
+  }
+  TFObj *t = new TFObj;
+  t->hcol = std::move(col);
+  t->hop = std::move(op);
+  t->col = (float *)dalloc(std::max<size_t>(4, t->hcol.size() * 4));
+  t->op = (float *)dalloc(std::max<size_t>(4, t->hop.size() * 4));
+  if (!t->col || !t->op) {
+    delete t;
+    return nullptr;
+  }
+  if (!t->hcol.empty()) hipMemcpyAsync(t->col, t->hcol.data(), t->hcol.size() * 4, hipMemcpyHostToDevice, S());
+  if (!t->hop.empty()) hipMemcpyAsync(t->op, t->hop.data(), t->hop.size() * 4, hipMemcpyHostToDevice, S());
+  hipStreamSynchronize(S());
+  t->d.col = t->col;
+  t->d.op = t->op;
+  t->d.nc = (int)t->hcol.size() / 3;
+  t->d.no = (int)t->hop.size();
+  t->d.lo = lo;
+  t->d.hi = hi;
+  std::lock_guard<std::recursive_mutex> lk(G.mu);
+  G.tfs.insert(t);
+  return t;
+}
+
+// MappedVis::local_commit (MappedVis.cpp:286-333)
+GXY_EXPORT void *gxy_tf_create_from_maps(const float *cmap, int nc, const float *omap, int no,
+                                         const float *range) {
+  INIT_OR(nullptr);
+  if (nc < 2 || no < 2) {
+    fail("colormap and opacitymap need at least two points");
+    return nullptr;
+  }
+  std::vector<float> col(256 * 3), op(256);
+  int i0 = 0, i1 = 1;
+  float xmin = cmap[0], xmax = cmap[4 * (nc - 1)];
+  for (int i = 0; i < 256; i++) {
+    float x = (float)(xmin + (i / (255.0)) * (xmax - xmin));
+    if (x > xmax) x = xmax;
+    while (i1 < nc - 1 && cmap[4 * i1] < x) i0++, i1++;
+    float d = (x - cmap[4 * i0]) / (cmap[4 * i1] - cmap[4 * i0]);
+    col[3 * i + 0] = cmap[4 * i0 + 1] + d * (cmap[4 * i1 + 1] - cmap[4 * i0 + 1]);
+    col[3 * i + 1] = cmap[4 * i0 + 2] + d * (cmap[4 * i1 + 2] - cmap[4 * i0 + 2]);
+    col[3 * i + 2] = cmap[4 * i0 + 3] + d * (cmap[4 * i1 + 3] - cmap[4 * i0 + 3]);
+  }
+  i0 = 0, i1 = 1;
+  xmin = omap[0], xmax = omap[2 * (no - 1)];
+  for (int i = 0; i < 256; i++) {
+    float x = (float)(xmin + (i / (255.0)) * (xmax - xmin));
+    if (x > xmax) x = xmax;
+    while (i1 < no - 1 && omap[2 * i1] < x) i0++, i1++;
+    float d = (x - omap[2 * i0]) / (omap[2 * i1] - omap[2 * i0]);
+    op[i] = omap[2 * i0 + 1] + d * (omap[2 * i1 + 1] - omap[2 * i0 + 1]);
+  }
+  float lo = range ? range[0] : cmap[0], hi = range ? range[1] : cmap[4 * (nc - 1)];
+  return tf_make(std::move(col), std::move(op), lo, hi);
+}
+GXY_EXPORT void *gxy_tf_create(const float *colors, int nc, const float *opac, int no, float lo, float hi) {
+  INIT_OR(nullptr);
+  return tf_make(std::vector<float>(colors, colors + 3 * nc), std::vector<float>(opac, opac + no), lo, hi);
+}
+GXY_EXPORT int gxy_tf_destroy(void *tf) {
+  TFObj *t = lookup<TFObj>(G.tfs, tf);
+  if (!t) return fail("not a TF handle");
+  {
+    std::lock_guard<std::recursive_mutex> lk(G.mu);
+    G.tfs.erase(t);
+  }
+  dfree(t->col);
+  dfree(t->op);
+  delete t;
+  return 0;
+}
+GXY_EXPORT int gxy_tf_get_tables(void *tf, float *c, float *o, float *range) {
+  TFObj *t = lookup<TFObj>(G.tfs, tf);
+  if (!t) return fail("not a TF handle");
+  memcpy(c, t->hcol.data(), t->hcol.size() * 4);
+  memcpy(o, t->hop.data(), t->hop.size() * 4);
+  range[0] = t->d.lo;
+  range[1] = t->d.hi;
+  return 0;
+}
+
+// ------------------------------------------------------------------ models
+template <class T>
+static T *upload(const T *h, size_t n) {
+  if (!h || n == 0) return nullptr;
+  T *d = (T *)dalloc(n * sizeof(T));
+  if (!d) return nullptr;
+  hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, S());
+  return d;
+}
+
+GXY_EXPORT void *gxy_model_create(void) {
+  INIT_OR(nullptr);
+  ModelObj *m = new ModelObj;
+  memset(&m->d, 0, sizeof m->d);
+  std::lock_guard<std::recursive_mutex> lk(G.mu);
+  G.models.insert(m);
+  return m;
+}
+GXY_EXPORT int gxy_model_destroy(void *model) {
+  ModelObj *M = lookup<ModelObj>(G.models, model);
+  if (!M) return fail("not a model handle");
+  {
+    std::lock_guard<std::recursive_mutex> lk(G.mu);
+    G.models.erase(M);
+  }
+  M->free_built();
+  for (auto &m : M->meshes) { dfree(m.V); dfree(m.I); dfree(m.N); dfree(m.D); }
+  for (auto &s : M->spheres) { dfree(s.C); dfree(s.D); }
+  delete M;
+  return 0;
+}
+GXY_EXPORT int gxy_model_add_triangles(void *model, const float *V, int nv, const uint32_t *I, int nt,
+                                       const float *N, const float *D, void *tf) {
+  ModelObj *M = lookup<ModelObj>(G.models, model);
+  if (!M) return fail("not a model handle");
+  if (nv <= 0 || nt <= 0 || !V || !I) return fail("empty triangle mesh");
+  for (int64_t k = 0; k < 3 * (int64_t)nt; k++)
+    if (I[k] >= (uint32_t)nv) return fail("triangle index %u out of range (%d vertices)", I[k], nv);
+  MeshObj m;
+  m.geomID = M->ngeom++;
+  m.nv = nv;
+  m.nt = nt;
+  m.V = upload(V, 3 * (size_t)nv);
+  m.I = upload(I, 3 * (size_t)nt);
+  m.N = upload(N, N ? 3 * (size_t)nv : 0);
+  m.D = upload(D, D ? (size_t)nv : 0);
+  m.tf = tf ? lookup<TFObj>(G.tfs, tf) : nullptr;
+  if (tf && !m.tf) return fail("mesh TF is not a TF handle");
+  HIPCHK(hipStreamSynchronize(S()));
+  M->meshes.push_back(m);
+  M->free_built();
+  return m.geomID;
+}
+GXY_EXPORT int gxy_model_add_spheres(void *model, const float *C, const float *D, int n, float r0, float r1,
+                                     float v0, float v1, void *tf) {
+  ModelObj *M = lookup<ModelObj>(G.models, model);
+  if (!M) return fail("not a model handle");
+  if (n <= 0 || !C) return fail("empty sphere set");
+  SpheresObj s;
+  s.geomID = M->ngeom++;
+  s.n = n;
+  s.C = upload(C, 3 * (size_t)n);
+  std::vector<float> zero;
+  if (!D) zero.assign(n, 0.f);
+  s.D = upload(D ? D : zero.data(), (size_t)n);
+  s.r0 = r0; s.r1 = r1; s.v0 = v0; s.v1 = v1;
+  // DataDrivenSpheresGeometry_set (DataDrivenSpheres.ispc:211-220)
+  float e = std::log(r0);
+  if (e < 0.f) e = -1.f / e;
+  if (e > r0 / 100.0) e = (float)(r0 / 100.0);
+  s.eps = e;
+  s.tf = tf ? lookup<TFObj>(G.tfs, tf) : nullptr;
+  if (tf && !s.tf) return fail("sphere TF is not a TF handle");
+  HIPCHK(hipStreamSynchronize(S()));
+  M->spheres.push_back(s);
+  M->free_built();
+  return s.geomID;
+}
+
+namespace gxy {
+__global__ void k_tri_boxes(const float *V, const uint32_t *I, int nt, float *boxes) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nt) return;
+  float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
+  for (int c = 0; c < 3; c++) {
+    const float *v = V + 3 * (int64_t)I[3 * (int64_t)p + c];
+    for (int a = 0; a < 3; a++) {
+      lo[a] = v[a] < lo[a] ? v[a] : lo[a];
+      hi[a] = v[a] > hi[a] ? v[a] : hi[a];
+    }
+  }
+  for (int a = 0; a < 3; a++) { boxes[6 * (int64_t)p + a] = lo[a]; boxes[6 * (int64_t)p + 3 + a] = hi[a]; }
+}
+
+__device__ float sphere_radius_dev(float r0, float r1, float v0, float v1, float dv) {
+  if (v0 != v1) {
+    float d = (dv - v0) / (v1 - v0);
+    if (d > 1) return r1;
+    if (d < 0) return r0;
+    return r0 + d * (r1 - r0);
+  }
+  return r0;
+}
+
+__global__ void k_sph_boxes(const float *C, const float *D, int n, float r0, float r1, float v0, float v1,
+                            float *boxes, float *radius) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  float r = sphere_radius_dev(r0, r1, v0, v1, D[p]);
+  radius[p] = r;
+  for (int a = 0; a < 3; a++) {
+    boxes[6 * (int64_t)p + a] = C[3 * (int64_t)p + a] - r;
+    boxes[6 * (int64_t)p + 3 + a] = C[3 * (int64_t)p + a] + r;
+  }
+}
+
+// sorted triangle k <- primitive order[k] of the flat list (geom, prim)
+__global__ void k_gather_tris(const int *order, const int *fgeom, const int *fprim, int n, const DGeomShade *shade,
+                              float *tverts, int *tgeom, int *tprim) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int src = order[k];
+  int g = fgeom[src], p = fprim[src];
+  const DGeomShade &S = shade[g];
+  for (int c = 0; c < 3; c++) {
+    uint32_t vi = S.indices[3 * (int64_t)p + c];
+    for (int a = 0; a < 3; a++) tverts[9 * (int64_t)k + 3 * c + a] = S.vertices[3 * (int64_t)vi + a];
+  }
+  tgeom[k] = g;
+  tprim[k] = p;
+}
+
+__global__ void k_gather_sph(const int *order, const int *fgeom, const int *fprim, const float *fC,
+                             const float *frad, const float *feps, int n, float *sc, float *sr, float *se, int *sg,
+                             int *sp) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int src = order[k];
+  for (int a = 0; a < 3; a++) sc[3 * (int64_t)k + a] = fC[3 * (int64_t)src + a];
+  sr[k] = frad[src];
+  se[k] = feps[src];
+  sg[k] = fgeom[src];
+  sp[k] = fprim[src];
+}
+}  // namespace gxy
+
+// Structural check of a built tree: every child index in range, every inner
+// node and every leaf reached exactly once from the root.  Run for trees of up
+// to GXY_BVH_VALIDATE_MAX nodes (default 4M) before the tree is used.
+static int validate_bvh(const BVHNode *dnodes, int nnodes, int nprim, const char *what) {
+  size_t lim = getenv("GXY_BVH_VALIDATE_MAX") ? (size_t)atol(getenv("GXY_BVH_VALIDATE_MAX")) : (size_t)(4u << 20);
+  if ((size_t)nnodes > lim) return 0;
+  std::vector<BVHNode> h(nnodes);
+  HIPCHK(hipMemcpyAsync(h.data(), dnodes, sizeof(BVHNode) * nnodes, hipMemcpyDeviceToHost, S()));
+  HIPCHK(hipStreamSynchronize(S()));
+  std::vector<char> seen_node(nnodes, 0), seen_prim(nprim, 0);
+  std::vector<int> st;
+  st.push_back(0);
+  while (!st.empty()) {
+    int c = st.back();
+    st.pop_back();
+    if (c >= 0) {
+      if (c >= nnodes || seen_node[c]) return fail("%s BVH: inner node %d invalid or shared", what, c);
+      seen_node[c] = 1;
+      st.push_back(h[c].c0);
+      if (!(nnodes == 1 && h[c].c1 == h[c].c0)) st.push_back(h[c].c1);
+    } else {
+      int code = ~c, start = code >> 4, cnt = (code & 15) + 1;
+      if (start < 0 || start + cnt > nprim) return fail("%s BVH: leaf range %d+%d outside %d", what, start, cnt, nprim);
+      for (int k = start; k < start + cnt; k++) {
+        if (seen_prim[k]) return fail("%s BVH: primitive %d in two leaves", what, k);
+        seen_prim[k] = 1;
+      }
+    }
+  }
+  for (int k = 0; k < nnodes; k++)
+    if (!seen_node[k]) return fail("%s BVH: node %d unreachable", what, k);
+  for (int k = 0; k < nprim; k++)
+    if (!seen_prim[k]) return fail("%s BVH: primitive %d unreachable", what, k);
+  return 0;
+}
+
+GXY_EXPORT int gxy_model_commit(void *model) {
+  ModelObj *M = lookup<ModelObj>(G.models, model);
+  if (!M) return fail("not a model handle");
+  M->free_built();
+  memset(&M->d, 0, sizeof M->d);
+  // shading table indexed by geomID
+  std::vector<DGeomShade> sh(std::max(1, M->ngeom));
+  memset(sh.data(), 0, sh.size() * sizeof(DGeomShade));
+  int64_t ntri = 0, nsph = 0;
+  for (auto &m : M->meshes) {
+    DGeomShade &s = sh[m.geomID];
+    s.kind = 0;
+    s.vertices = m.V;
+    s.indices = m.I;
+    s.normals = m.N;
+    s.has_normals = m.N != nullptr;
+    s.vdata = m.D;
+    s.has_data = m.D != nullptr;
+    s.has_tf = m.tf != nullptr;
+    if (m.tf) s.tf = m.tf->d;
+    ntri += m.nt;
+  }
+  for (auto &p : M->spheres) {
+    DGeomShade &s = sh[p.geomID];
+    s.kind = 1;
+    s.centers = p.C;
+    s.vdata = p.D;
+    s.has_data = 1;
+    s.has_tf = p.tf != nullptr;
+    if (p.tf) s.tf = p.tf->d;
+    nsph += p.n;
+  }
+  if (ntri > (1 << 27) || nsph > (1 << 27)) return fail("more than 2^27 primitives of one kind");
+  DGeomShade *dsh = upload(sh.data(), sh.size());
+  M->owned.push_back(dsh);
+  M->d.shade = dsh;
+  M->d.ngeom = M->ngeom;
+  hipStream_t s = S();
+  if (ntri > 0) {
+    std::vector<int> fg(ntri), fp(ntri);
+    float *boxes = (float *)dalloc(6 * 4 * (size_t)ntri);
+    if (!boxes) return -1;
+    int64_t off = 0;
+    for (auto &m : M->meshes) {
+      for (int p = 0; p < m.nt; p++) { fg[off + p] = m.geomID; fp[off + p] = p; }
+      hipLaunchKernelGGL(k_tri_boxes, dim3((m.nt + 255) / 256), dim3(256), 0, s, m.V, m.I, m.nt, boxes + 6 * off);
+      off += m.nt;
+    }
+    int *dfg = upload(fg.data(), fg.size()), *dfp = upload(fp.data(), fp.size());
+    BVHBuildResult B;
+    HIPCHK(build_lbvh(boxes, (int)ntri, dalloc, dfree, &B, s));
+    float *tverts = (float *)dalloc(9 * 4 * (size_t)ntri);
+    int *tg = (int *)dalloc(4 * (size_t)ntri), *tp = (int *)dalloc(4 * (size_t)ntri);
+    if (!tverts || !tg || !tp || !B.nodes) return fail("model commit: out of device memory");
+    hipLaunchKernelGGL(k_gather_tris, dim3((ntri + 255) / 256), dim3(256), 0, s, B.order, dfg, dfp, (int)ntri, dsh,
+                       tverts, tg, tp);
+    HIPCHK(hipStreamSynchronize(s));
+    dfree(boxes);
+    dfree(dfg);
+    dfree(dfp);
+    dfree(B.order);
+    M->owned.insert(M->owned.end(), {(void *)B.nodes, (void *)tverts, (void *)tg, (void *)tp});
+    M->d.tnodes = B.nodes;
+    M->d.tverts = tverts;
+    M->d.tgeom = tg;
+    M->d.tprim = tp;
+    M->d.ntris = (int)ntri;
+    M->tnodes = B.nnodes;
+    M->tdepth = B.depth;
+    if (B.depth > 60) return fail("triangle BVH depth %d exceeds the traversal stack", B.depth);
+    if (validate_bvh(B.nodes, B.nnodes, (int)ntri, "triangle") != 0) return -1;
+  }
+  if (nsph > 0) {
+    std::vector<int> fg(nsph), fp(nsph);
+    std::vector<float> feps(nsph);
+    float *boxes = (float *)dalloc(6 * 4 * (size_t)nsph);
+    float *frad = (float *)dalloc(4 * (size_t)nsph);
+    float *fC = (float *)dalloc(12 * (size_t)nsph);
+    if (!boxes || !frad || !fC) return -1;
+    int64_t off = 0;
+    for (auto &p : M->spheres) {
+      for (int k = 0; k < p.n; k++) { fg[off + k] = p.geomID; fp[off + k] = k; feps[off + k] = p.eps; }
+      hipLaunchKernelGGL(k_sph_boxes, dim3((p.n + 255) / 256), dim3(256), 0, s, p.C, p.D, p.n, p.r0, p.r1, p.v0,
+                         p.v1, boxes + 6 * off, frad + off);
+      hipMemcpyAsync(fC + 3 * off, p.C, 12 * (size_t)p.n, hipMemcpyDeviceToDevice, s);
+      off += p.n;
+    }
+    int *dfg = upload(fg.data(), fg.size()), *dfp = upload(fp.data(), fp.size());
+    float *dfe = upload(feps.data(), feps.size());
+    BVHBuildResult B;
+    HIPCHK(build_lbvh(boxes, (int)nsph, dalloc, dfree, &B, s));
+    float *sc = (float *)dalloc(12 * (size_t)nsph), *sr = (float *)dalloc(4 * (size_t)nsph);
+    float *se = (float *)dalloc(4 * (size_t)nsph);
+    int *sg = (int *)dalloc(4 * (size_t)nsph), *sp = (int *)dalloc(4 * (size_t)nsph);
+    if (!sc || !sr || !se || !sg || !sp || !B.nodes) return fail("model commit: out of device memory");
+    hipLaunchKernelGGL(k_gather_sph, dim3((nsph + 255) / 256), dim3(256), 0, s, B.order, dfg, dfp, fC, frad, dfe,
+                       (int)nsph, sc, sr, se, sg, sp);
+    HIPCHK(hipStreamSynchronize(s));
+    for (void *p : {(void *)boxes, (void *)frad, (void *)fC, (void *)dfg, (void *)dfp, (void *)dfe, (void *)B.order})
+      dfree(p);
+    M->owned.insert(M->owned.end(), {(void *)B.nodes, (void *)sc, (void *)sr, (void *)se, (void *)sg, (void *)sp});
+    M->d.snodes = B.nodes;
+    M->d.scenter = sc;
+    M->d.sradius = sr;
+    M->d.seps = se;
+    M->d.sgeom = sg;
+    M->d.sprim = sp;
+    M->d.nsph = (int)nsph;
+    M->snodes = B.nnodes;
+    M->sdepth = B.depth;
+    if (B.depth > 60) return fail("sphere BVH depth %d exceeds the traversal stack", B.depth);
+    if (validate_bvh(B.nodes, B.nnodes, (int)nsph, "sphere") != 0) return -1;
+  }
+  M->committed = true;
+  return 0;
+}
+GXY_EXPORT int gxy_model_bvh_stats(void *model, int *nodes, int *leaves, int *depth, float *sah) {
+  ModelObj *M = lookup<ModelObj>(G.models, model);
+  if (!M) return fail("not a model handle");
+  if (nodes) *nodes = M->tnodes + M->snodes;
+  if (leaves) *leaves = M->d.ntris + M->d.nsph;
+  if (depth) *depth = std::max(M->tdepth, M->sdepth);
+  if (sah) *sah = 0.f;
+  return 0;
+}
+GXY_EXPORT int gxy_model_set_hit_output(void *model, int *g, int *p) {
+  ModelObj *M = lookup<ModelObj>(G.models, model);
+  if (!M) return fail("not a model handle");
+  M->hit_geom = g;
+  M->hit_prim = p;
+  return 0;
+}
+
+// ------------------------------------------------------------------ TraceRays_* (reference ABI)
+struct TraceRaysObj {
+  int debug[100];
+};
+
+GXY_EXPORT void *TraceRays_allocate(void) {
+  if (ensure_init() != 0) return nullptr;
+  TraceRaysObj *t = new TraceRaysObj;
+  std::lock_guard<std::recursive_mutex> lk(G.mu);
+  G.tracers.insert(t);
+  return t;
+}
+GXY_EXPORT void TraceRays_initialize(void *self) {
+  TraceRaysObj *t = lookup<TraceRaysObj>(G.tracers, self);
+  if (t) for (int i = 0; i < 100; i++) t->debug[i] = -1;
+}
+GXY_EXPORT void TraceRays_destroy(void *self) {
+  TraceRaysObj *t = lookup<TraceRaysObj>(G.tracers, self);
+  if (!t) return;
+  std::lock_guard<std::recursive_mutex> lk(G.mu);
+  G.tracers.erase(t);
+  delete t;
+}
+
+GXY_EXPORT void *TraceRays_TraceRays(void *self, void *vis, int nRaysIn, void *raysIn, float epsilon) {
+  (void)self;
+  if (ensure_init() != 0) return nullptr;
+  TraceParams P;
+  if (make_trace_params((const Visualization_ispc *)vis, epsilon, P) != 0) return nullptr;
+  KTimer kt("trace");
+  hipError_t e = launch_trace(P, *(const RayList_ispc *)raysIn, nRaysIn, S());
+  if (e != hipSuccess) fail("trace launch: %s", hipGetErrorString(e));
+  return nullptr;
+}
+
+static void lighting_entry(void *lights, int n, void *raysIn, int *off, void *raysOut, float eps, int mode,
+                           const char *name) {
+  if (ensure_init() != 0) return;
+  LightParams L;
+  if (make_light_params((const Lighting_ispc *)lights, eps, L) != 0) return;
+  RayList_ispc O;
+  memset(&O, 0, sizeof O);
+  if (raysOut) O = *(const RayList_ispc *)raysOut;
+  KTimer kt(name);
+  hipError_t e = launch_lighting(L, mode, *(const RayList_ispc *)raysIn, n, (mode & LM_AO) ? off : nullptr,
+                                 (mode & LM_SHADOW) ? off : nullptr, O, S());
+  if (e != hipSuccess) fail("lighting launch: %s", hipGetErrorString(e));
+}
+GXY_EXPORT void TraceRays_generateAORays(void *self, void *lights, int n, void *raysIn, int *offsets, void *raysOut,
+                                         float eps) {
+  (void)self;
+  lighting_entry(lights, n, raysIn, offsets, raysOut, eps, LM_AO, "ao");
+}
+GXY_EXPORT void TraceRays_generateShadowRays(void *self, void *lights, int n, void *raysIn, int *offsets,
+                                             void *raysOut, float eps) {
+  (void)self;
+  lighting_entry(lights, n, raysIn, offsets, raysOut, eps, LM_SHADOW, "shadow");
+}
+GXY_EXPORT void TraceRays_ambientLighting(void *self, void *lights, int n, void *raysIn) {
+  (void)self;
+  lighting_entry(lights, n, raysIn, nullptr, nullptr, 0.f, LM_AMBIENT, "ambient");
+}
+GXY_EXPORT void TraceRays_diffuseLighting(void *self, void *lights, int n, void *raysIn) {
+  (void)self;
+  lighting_entry(lights, n, raysIn, nullptr, nullptr, 0.f, LM_DIFFUSE, "diffuse");
+}
+
+GXY_EXPORT void *SamplerTraceRays_SamplerTraceRays(void *self, void *vis, int n, void *raysIn) {
+  (void)self;
+  if (ensure_init() != 0) return nullptr;
+  const Visualization_ispc *V = (const Visualization_ispc *)vis;
+  if (!V || V->nVolumeVis < 1) return nullptr;  // SamplerTraceRays.ispc:139
+  TraceParams P;
+  if (make_trace_params(V, 0.f, P) != 0) return nullptr;
+  // SamplerTraceRays.ispc:139-150: step = min samplingStep*samplingRate
+  for (int k = 0; k < P.nvv; k++)
+    if (P.vv[k].nIso == 0) P.vv[k].iso[0] = 0.f;  // IsoSamplerVis_initialize default
+  KTimer kt("sampler");
+  hipError_t e = launch_sampler(P, *(const RayList_ispc *)raysIn, n, S());
+  if (e != hipSuccess) fail("sampler launch: %s", hipGetErrorString(e));
+  return nullptr;
+}
+
+// ------------------------------------------------------------------ Lighting_* (Lighting.ispc:23-134)
+struct LightingObj {
+  Lighting_ispc l;
+  std::vector<float> L;
+  std::vector<int> T;
+};
+GXY_EXPORT void *Lighting_allocate(void) {
+  LightingObj *o = new LightingObj;
+  memset(&o->l, 0, sizeof o->l);
+  std::lock_guard<std::recursive_mutex> lk(G.mu);
+  G.lightings.insert(o);
+  return &o->l;  // Lighting_ispc is the first member
+}
+static LightingObj *lobj(void *self) {
+  std::lock_guard<std::recursive_mutex> lk(G.mu);
+  return G.lightings.count(self) ? (LightingObj *)self : nullptr;
+}
+GXY_EXPORT void Lighting_initialize(void *self) {
+  LightingObj *o = lobj(self);
+  if (!o) return;
+  o->l.n_ao_rays = 0;
+  o->l.do_shadows = 0;
+  o->l.ao_radius = 1.0f;
+  o->l.Ka = 0.5f;
+  o->l.Kd = 0.5f;
+  o->L = {1.f, 1.f, 1.f};
+  o->T = {2};
+  o->l.lights = o->L.data();
+  o->l.types = o->T.data();
+  o->l.nLights = 1;
+}
+GXY_EXPORT void Lighting_destroy(void *self) {
+  LightingObj *o = lobj(self);
+  if (!o) return;
+  std::lock_guard<std::recursive_mutex> lk(G.mu);
+  G.lightings.erase(o);
+  delete o;
+}
+GXY_EXPORT void Lighting_SetLights(void *self, int n, float *l, int *t) {
+  LightingObj *o = lobj(self);
+  if (!o) return;
+  o->L.assign(l, l + 3 * n);
+  if (t) o->T.assign(t, t + n);
+  else if ((int)o->T.size() != n) o->T.assign(n, 0);
+  o->l.lights = o->L.data();
+  o->l.types = o->T.data();
+  o->l.nLights = n;
+}
+GXY_EXPORT void Lighting_GetLights(void *self, int *n, float **l, int **t) {
+  LightingObj *o = lobj(self);
+  if (!o) return;
+  *n = (int)o->l.nLights;
+  *l = o->l.lights;
+  *t = o->l.types;
+}
+GXY_EXPORT void Lighting_SetK(void *self, float ka, float kd) {
+  LightingObj *o = lobj(self);
+  if (o) { o->l.Ka = ka; o->l.Kd = kd; }
+}
+GXY_EXPORT void Lighting_GetK(void *self, float *ka, float *kd) {
+  LightingObj *o = lobj(self);
+  if (o) { *ka = o->l.Ka; *kd = o->l.Kd; }
+}
+GXY_EXPORT void Lighting_SetAO(void *self, int n, float r) {
+  LightingObj *o = lobj(self);
+  if (o) { o->l.n_ao_rays = n; o->l.ao_radius = r; }
+}
+GXY_EXPORT void Lighting_GetAO(void *self, int *n, float *r) {
+  LightingObj *o = lobj(self);
+  if (o) { *n = o->l.n_ao_rays; *r = o->l.ao_radius; }
+}
+GXY_EXPORT void Lighting_SetShadowFlag(void *self, int b) {
+  LightingObj *o = lobj(self);
+  if (o) o->l.do_shadows = b;
+}
+GXY_EXPORT void Lighting_GetShadowFlag(void *self, int *b) {
+  LightingObj *o = lobj(self);
+  if (o) *b = o->l.do_shadows;
+}
+
+// ------------------------------------------------------------------ pass pieces
+namespace gxy {
+struct Scratch {
+  std::vector<void *> blocks;
+  void *get(size_t bytes) {
+    void *p = dalloc(bytes);
+    if (p) blocks.push_back(p);
+    return p;
+  }
+  ~Scratch() {
+    for (void *p : blocks) dfree_stream_ordered(p);
+  }
+};
+
+// TraceRays.cpp:89-117 on the device; n_ao / n_shadow through one readback
+static int spawn_offsets(int n, const RayList_ispc &R, int nAO, int nL, int doSh, int *ao_off, int *sh_off,
+                         int *T_out) {
+  Scratch sc;
+  int nb = (n + GXY_BLOCK - 1) / GXY_BLOCK;
+  int *scratch = (int *)sc.get(sizeof(int) * (nb + 2));
+  int *rank = (int *)sc.get(sizeof(int) * std::max(1, n));
+  if (!scratch || !rank) return -1;
+  int *total = scratch + nb + 1;
+  {
+    KTimer kt("spawn_scan");
+    HIPCHK(launch_surface_scan(R, n, scratch, rank, total, S()));
+    HIPCHK(launch_offsets_from_rank(R, n, rank, total, nAO, nL, doSh, ao_off, sh_off, S()));
+  }
+  int T = 0;
+  HIPCHK(hipMemcpyAsync(&T, total, sizeof(int), hipMemcpyDeviceToHost, S()));
+  HIPCHK(hipStreamSynchronize(S()));
+  *T_out = T;
+  return 0;
+}
+}  // namespace gxy
+
+GXY_EXPORT int gxy_spawn_offsets(int n, RayList_ispc *R, int nAO, int nL, int doSh, int *ao_off, int *sh_off,
+                                 int *n_ao, int *n_sh) {
+  INIT_OR(-1);
+  int T = 0;
+  if (n < 0) return fail("negative ray count");
+  if (n == 0) {
+    *n_ao = *n_sh = 0;
+    return 0;
+  }
+  if (spawn_offsets(n, *R, nAO, nL, doSh, ao_off, sh_off, &T) != 0) return -1;
+  *n_ao = nAO * T;
+  *n_sh = doSh ? nL * T : 0;
+  return 0;
+}
+
+static int route(int n, const RayList_ispc &R, int mode, int reverse, const gxy_box3f *box, const int nb[6],
+                 float *fb, int w, int h) {
+  RouteParams P;
+  memset(&P, 0, sizeof P);
+  P.reverse = reverse;
+  for (int k = 0; k < 6; k++) P.neighbors[k] = nb ? nb[k] : -1;
+  if (box)
+    for (int k = 0; k < 3; k++) { P.blo[k] = box->lower[k]; P.bhi[k] = box->upper[k]; }
+  P.fb = fb;
+  P.width = w;
+  P.height = h;
+  P.mode = mode;
+  P.counters = G.counters + 1;
+  KTimer kt("route");
+  HIPCHK(launch_route(P, R, n, S()));
+  return 0;
+}
+
+GXY_EXPORT int gxy_classify(int n, RayList_ispc *R, int reverse) {
+  INIT_OR(-1);
+  return route(n, *R, RM_CLASSIFY, reverse, nullptr, nullptr, nullptr, 0, 0);
+}
+GXY_EXPORT int gxy_assign_destinations(int n, RayList_ispc *R, const gxy_box3f *box, const int nb[6], int reverse) {
+  INIT_OR(-1);
+  return route(n, *R, RM_ASSIGN, reverse, box, nb, nullptr, 0, 0);
+}
+GXY_EXPORT int gxy_accumulate_terminated(int n, RayList_ispc *R, float *fb, int w, int h) {
+  INIT_OR(-1);
+  return route(n, *R, RM_ACCUMULATE, 1, nullptr, nullptr, fb, w, h);
+}
+
+GXY_EXPORT int gxy_partition_rays(int n, RayList_ispc *R, int ndest, RayList_ispc *const *lists,
+                                  RayList_ispc *keepers, int *counts) {
+  INIT_OR(-1);
+  if (ndest < 0 || ndest + 4 > 64) return fail("ndest %d outside [0, 60]", ndest);
+  int K = ndest + 1;
+  std::vector<int> table(ndest + 4, -1);
+  table[GXY_KEEP_HERE + 4] = ndest;
+  for (int d = 0; d < ndest; d++) table[d + 4] = d;
+  Scratch sc;
+  int nb = (n + GXY_BLOCK - 1) / GXY_BLOCK;
+  int *scratch = (int *)sc.get(sizeof(int) * ((size_t)K * std::max(1, nb) + 1));
+  int *dcounts = (int *)sc.get(sizeof(int) * K);
+  if (!scratch || !dcounts) return -1;
+  {
+    KTimer kt("partition");
+    HIPCHK(launch_partition(*R, n, table.data(), (int)table.size(), K, nullptr, scratch, dcounts, S()));
+  }
+  HIPCHK(hipMemcpyAsync(counts, dcounts, sizeof(int) * K, hipMemcpyDeviceToHost, S()));
+  bool any = keepers != nullptr;
+  for (int d = 0; d < ndest && lists; d++) any |= lists[d] != nullptr;
+  if (any) {
+    if (K > 12) return fail("list copies support at most 11 destinations");
+    std::vector<RayList_ispc> L(K);
+    memset(L.data(), 0, sizeof(RayList_ispc) * K);
+    for (int d = 0; d < ndest; d++)
+      if (lists && lists[d]) L[d] = *lists[d];
+    if (keepers) L[ndest] = *keepers;
+    KTimer kt("partition");
+    HIPCHK(launch_partition_scatter(*R, n, table.data(), (int)table.size(), K, L.data(), scratch, S()));
+  }
+  HIPCHK(hipStreamSynchronize(S()));
+  return 0;
+}
+
+// Camera::generate_initial_rays host part (Camera.cpp:528-582): image-plane
+// frame in float with the dtypes.h normalise; per-pixel rays on the device.
+static void dt_norm(float *a) {
+  float d = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+  if (d != 0) {
+    d = (float)(1.0 / d);
+    a[0] *= d; a[1] *= d; a[2] *= d;
+  }
+}
+static void crossf(const float *a, const float *b, float *r) {
+  r[0] = a[1] * b[2] - a[2] * b[1];
+  r[1] = a[2] * b[0] - a[0] * b[2];
+  r[2] = a[0] * b[1] - a[1] * b[0];
+}
+static void camera_params(const float eye[3], const float cdir[3], int is_dir, const float up[3], float aov, int w,
+                          int h, const gxy_box3f *lb, const gxy_box3f *gb, int order, CameraParams &C) {
+  memset(&C, 0, sizeof C);
+  float vdir[3], vu[3] = {up[0], up[1], up[2]}, center[3];
+  for (int k = 0; k < 3; k++) vdir[k] = is_dir ? cdir[k] : cdir[k] - eye[k];
+  if (aov == 0.0f) {
+    for (int k = 0; k < 3; k++) center[k] = eye[k] + vdir[k];
+    dt_norm(vdir);
+  } else {
+    float d = (float)(1.0 / std::tan(2 * 3.1415926 * (aov / 2.0) / 360.0));
+    dt_norm(vdir);
+    for (int k = 0; k < 3; k++) center[k] = eye[k] + d * vdir[k];
+  }
+  float vr[3];
+  crossf(vdir, vu, vr);
+  dt_norm(vr);
+  crossf(vr, vdir, vu);
+  dt_norm(vu);
+  float pixel_scaling = (float)(((w < h ? w : h) - 1.0) / 2.0);
+  for (int k = 0; k < 3; k++) {
+    C.veye[k] = eye[k];
+    C.vdir[k] = vdir[k];
+    C.vr[k] = vr[k];
+    C.vu[k] = vu[k];
+    C.center[k] = center[k];
+    C.llo[k] = lb->lower[k];
+    C.lhi[k] = lb->upper[k];
+    C.glo[k] = gb->lower[k];
+    C.ghi[k] = gb->upper[k];
+  }
+  C.scaling = (float)(1.0 / pixel_scaling);
+  C.off_x = (float)((w - 1) / 2.0);
+  C.off_y = (float)((h - 1) / 2.0);
+  C.is_ortho = aov == 0.0f;
+  C.width = w;
+  C.height = h;
+  C.order = order;
+}
+
+static int camera_generate(const float eye[3], const float cdir[3], int is_dir, const float up[3], float aov,
+                           int w, int h, const gxy_box3f *lb, const gxy_box3f *gb, const RayList_ispc &R,
+                           int capacity, int order, int *n_out) {
+  CameraParams C;
+  camera_params(eye, cdir, is_dir, up, aov, w, h, lb, gb, order, C);
+  int npix = order == 0 ? w * h : ((w + 7) / 8) * ((h + 7) / 8) * 64;
+  Scratch sc;
+  int nb = (npix + GXY_BLOCK - 1) / GXY_BLOCK;
+  int *scratch = (int *)sc.get(sizeof(int) * (nb + 2));
+  if (!scratch) return -1;
+  int *total = scratch + nb + 1;
+  {
+    KTimer kt("camera");
+    HIPCHK(launch_camera(C, R, capacity, scratch, total, S()));
+  }
+  int T = 0;
+  HIPCHK(hipMemcpyAsync(&T, total, sizeof(int), hipMemcpyDeviceToHost, S()));
+  HIPCHK(hipStreamSynchronize(S()));
+  if (T > capacity) return fail("camera produced %d rays for a list of %d", T, capacity);
+  *n_out = T;
+  return 0;
+}
+
+GXY_EXPORT int gxy_camera_generate(const float eye[3], const float cdir[3], int is_dir, const float up[3], float aov,
+                                   int w, int h, const gxy_box3f *lb, const gxy_box3f *gb, RayList_ispc *R,
+                                   int capacity, int order, int *n_out) {
+  INIT_OR(-1);
+  return camera_generate(eye, cdir, is_dir, up, aov, w, h, lb, gb, *R, capacity, order, n_out);
+}
+
+GXY_EXPORT int gxy_sampler_collect(int n, RayList_ispc *R, float *xyz, int capacity, int *count) {
+  INIT_OR(-1);
+  Scratch sc;
+  int nb = (n + GXY_BLOCK - 1) / GXY_BLOCK;
+  int *scratch = (int *)sc.get(sizeof(int) * (nb + 4));
+  if (!scratch) return -1;
+  KTimer kt("sampler_collect");
+  HIPCHK(launch_accumulate_sampler(*R, n, xyz, capacity, count, scratch, S()));
+  return 0;
+}
+
+// ------------------------------------------------------------------ ray-list loop
+namespace gxy {
+struct QItem {
+  RayListObj *rl;
+  int prio;
+};
+
+struct LoopCtx {
+  TraceParams P;
+  gxy_box3f box;
+  LightParams L;
+  bool have_lights = false;
+  int nb[6];
+  float eps;
+  int maxlist;
+  float *fb;
+  int w, h;
+  gxy_frame_stats st;
+  std::vector<QItem> q;
+  bool sampler = false;
+  float *xyz = nullptr;
+  int capacity = 0;
+  int *dcount = nullptr;
+};
+
+// RayQManager + ThreadPool order (RayQManager.cpp:138-194, Threading.cpp:176-233):
+// the highest priority first (primary lists 3, secondary 2), LIFO within it.
+static QItem pop_next(std::vector<QItem> &q) {
+  int best = -1;
+  for (int i = 0; i < (int)q.size(); i++)
+    if (best < 0 || q[i].prio >= q[best].prio) best = i;
+  QItem it = q[best];
+  q.erase(q.begin() + best);
+  return it;
+}
+
+// Renderer::Trace's split (Renderer.cpp:675-690) + enqueue
+static int enqueue_split(LoopCtx &C, RayListObj *out, int prio) {
+  int n = out->hdr.size;
+  if (n <= C.maxlist) {
+    C.q.push_back({out, prio});
+    return 0;
+  }
+  for (int s = 0; s < n; s += C.maxlist) {
+    int c = std::min(C.maxlist, n - s);
+    RayListObj *p = raylist_new(c, out->hdr.frame, out->hdr.type);
+    if (!p) return -1;
+    HIPCHK(launch_copy_rays(out->ispc, s, c, p->ispc, 0, S()));
+    C.q.push_back({p, prio});
+  }
+  raylist_delete(out, false);
+  return 0;
+}
+
+// processRays_task::work (Renderer.cpp:511-651) for one list and its keepers
+static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
+  while (L && L->hdr.size > 0) {
+    const int n = L->hdr.size;
+    const RayList_ispc &R = L->ispc;
+    {
+      KTimer kt(C.sampler ? "sampler" : "trace");
+      HIPCHK(C.sampler ? launch_sampler(C.P, R, n, S()) : launch_trace(C.P, R, n, S()));
+    }
+    C.st.ray_passes += n;
+    C.st.passes++;
+    C.st.lists++;
+    if (!C.sampler) {
+      // TraceRays::Trace (TraceRays.cpp:68-146)
+      Scratch sc;
+      int *ao_off = (int *)sc.get(sizeof(int) * n), *sh_off = (int *)sc.get(sizeof(int) * n);
+      if (!ao_off || !sh_off) return -1;
+      int T = 0;
+      if (spawn_offsets(n, R, C.L.nAO, C.L.nLights, C.L.doShadows, ao_off, sh_off, &T) != 0) return -1;
+      const int nao = C.L.nAO * T, nsh = C.L.doShadows ? C.L.nLights * T : 0;
+      RayListObj *out = nullptr;
+      if (nao + nsh > 0) {
+        out = raylist_new(nao + nsh, L->hdr.frame, 1);
+        if (!out) return -1;
+      }
+      int mode = LM_AMBIENT | LM_DIFFUSE | (nao ? LM_AO : 0) | (nsh ? LM_SHADOW : 0);
+      RayList_ispc O;
+      memset(&O, 0, sizeof O);
+      if (out) O = out->ispc;
+      {
+        KTimer kt("lighting");
+        HIPCHK(launch_lighting(C.L, mode, R, n, ao_off, sh_off, O, S()));
+      }
+      if (out) {
+        C.st.secondary_rays += nao + nsh;
+        if (enqueue_split(C, out, 2) != 0) return -1;
+      }
+    }
+    // Classify, AssignDestinations, HandleTerminatedRays (Renderer.cpp:550-556)
+    if (route(n, R, C.sampler ? (RM_CLASSIFY | RM_ASSIGN) : (RM_CLASSIFY | RM_ASSIGN | RM_ACCUMULATE), 1,
+              &C.box, C.nb, C.fb, C.w, C.h) != 0)
+      return -1;
+    if (C.sampler) {
+      // Sampler::HandleTerminatedRays (Sampler.cpp:51-89)
+      Scratch sc;
+      int nbk = (n + GXY_BLOCK - 1) / GXY_BLOCK;
+      int *scr = (int *)sc.get(sizeof(int) * (nbk + 4));
+      if (!scr) return -1;
+      KTimer kt("sampler_collect");
+      HIPCHK(launch_accumulate_sampler(R, n, C.xyz, C.capacity, C.dcount, scr, S()));
+    }
+    // split by destination: keepers + one list per neighbour face (Renderer.cpp:561-634)
+    int table[64];
+    for (int k = 0; k < 64; k++) table[k] = -1;
+    table[GXY_KEEP_HERE + 4] = 0;
+    int ntable = 4;
+    for (int f = 0; f < 6; f++)
+      if (C.nb[f] >= 0 && C.nb[f] + 4 < 64) {
+        table[C.nb[f] + 4] = 1 + f;
+        ntable = std::max(ntable, C.nb[f] + 5);
+      }
+    const int K = 7;
+    Scratch sc;
+    int nbk = (n + GXY_BLOCK - 1) / GXY_BLOCK;
+    int *scr = (int *)sc.get(sizeof(int) * ((size_t)K * nbk + 1));
+    int *dcounts = (int *)sc.get(sizeof(int) * K);
+    if (!scr || !dcounts) return -1;
+    int counts[K];
+    {
+      KTimer kt("partition");
+      HIPCHK(launch_partition(R, n, table, ntable, K, nullptr, scr, dcounts, S()));
+    }
+    HIPCHK(hipMemcpyAsync(counts, dcounts, sizeof counts, hipMemcpyDeviceToHost, S()));
+    HIPCHK(hipStreamSynchronize(S()));
+    for (int f = 0; f < 6; f++) C.st.outbound[f] += counts[1 + f];
+    RayListObj *keep = nullptr;
+    if (counts[0] > 0) {
+      keep = raylist_new(counts[0], L->hdr.frame, L->hdr.type);
+      if (!keep) return -1;
+      RayList_ispc lists[K];
+      memset(lists, 0, sizeof lists);
+      lists[0] = keep->ispc;
+      KTimer kt("partition");
+      HIPCHK(launch_partition_scatter(R, n, table, ntable, 1, lists, scr, S()));
+      C.st.kept += counts[0];
+    }
+    // outbound lists would be sent to the neighbour ranks here (SendRays,
+    // Renderer.cpp:620-634); on one GPU their counts are the outbound queue.
+    if (owned) raylist_delete(L, false);
+    L = keep;
+    owned = true;
+  }
+  (void)prio;
+  return 0;
+}
+
+static int run_loop(LoopCtx &C, int n, const RayList_ispc &rays) {
+  // initial lists, at most maxlist rays each (primary priority)
+  for (int s = 0; s < n; s += C.maxlist) {
+    int c = std::min(C.maxlist, n - s);
+    RayListObj *p = raylist_new(c, 0, 0);
+    if (!p) return -1;
+    HIPCHK(launch_copy_rays(rays, s, c, p->ispc, 0, S()));
+    C.q.push_back({p, 3});
+  }
+  HIPCHK(hipMemsetAsync(G.counters, 0, 8 * sizeof(unsigned long long), S()));
+  while (!C.q.empty()) {
+    QItem it = pop_next(C.q);
+    if (process_list(C, it.rl, it.prio, true) != 0) {
+      for (auto &x : C.q) raylist_delete(x.rl);
+      C.q.clear();
+      return -1;
+    }
+  }
+  unsigned long long cnt[8];
+  HIPCHK(hipMemcpyAsync(cnt, G.counters, sizeof cnt, hipMemcpyDeviceToHost, S()));
+  HIPCHK(hipStreamSynchronize(S()));
+  C.st.samples = (long long)cnt[0];
+  C.st.terminated = (long long)cnt[1];
+  C.st.dropped = (long long)cnt[2];
+  return 0;
+}
+
+static int loop_setup(LoopCtx &C, void *vis, void *lights, const int nb[6], float eps, int maxlist) {
+  memset(&C.st, 0, sizeof C.st);
+  if (make_trace_params((const Visualization_ispc *)vis, eps, C.P) != 0) return -1;
+  C.box = ((const Visualization_ispc *)vis)->local_bb;
+  if (lights) {
+    if (make_light_params((const Lighting_ispc *)lights, 0.001f, C.L) != 0) return -1;
+    C.L.eps = eps;
+    C.have_lights = true;
+  }
+  for (int k = 0; k < 6; k++) C.nb[k] = nb ? nb[k] : -1;
+  C.eps = eps;
+  C.maxlist = maxlist > 0 ? maxlist : 1000000;
+  return 0;
+}
+}  // namespace gxy
+
+GXY_EXPORT int gxy_process_raylist(void *vis, void *lights, int n, RayList_ispc *rays, const int nb[6], float eps,
+                                   int maxlist, float *fb, int w, int h, gxy_frame_stats *stats) {
+  INIT_OR(-1);
+  LoopCtx C;
+  if (loop_setup(C, vis, lights, nb, eps, maxlist) != 0) return -1;
+  if (!C.have_lights) return fail("gxy_process_raylist needs a Lighting_ispc");
+  C.fb = fb;
+  C.w = w;
+  C.h = h;
+  int r = run_loop(C, n, *rays);
+  if (stats) *stats = C.st;
+  return r;
+}
+
+GXY_EXPORT int gxy_render_frame(void *vis, void *lights, const float eye[3], const float center[3],
+                                const float up[3], float aov, int w, int h, const int nb[6], float eps, int maxlist,
+                                float *fb, gxy_frame_stats *stats) {
+  INIT_OR(-1);
+  const Visualization_ispc *V = (const Visualization_ispc *)vis;
+  if (!V) return fail("null visualization");
+  RayListObj *prim = raylist_new(w * h, 0, 0);
+  if (!prim) return -1;
+  int np = 0;
+  if (camera_generate(eye, center, 0, up, aov, w, h, &V->local_bb, &V->global_bb, prim->ispc, w * h, 1, &np) != 0) {
+    raylist_delete(prim);
+    return -1;
+  }
+  LoopCtx C;
+  if (loop_setup(C, vis, lights, nb, eps, maxlist) != 0 || !C.have_lights) {
+    raylist_delete(prim);
+    return C.have_lights ? -1 : fail("gxy_render_frame needs a Lighting_ispc");
+  }
+  C.fb = fb;
+  C.w = w;
+  C.h = h;
+  int r = run_loop(C, np, prim->ispc);
+  raylist_delete(prim);
+  C.st.primary_rays = np;
+  if (stats) *stats = C.st;
+  return r;
+}
+
+GXY_EXPORT int gxy_sample_raylist(void *vis, int n, RayList_ispc *rays, int maxlist, float *xyz, int capacity,
+                                  int *count, gxy_frame_stats *stats) {
+  INIT_OR(-1);
+  LoopCtx C;
+  if (loop_setup(C, vis, nullptr, nullptr, 0.f, maxlist) != 0) return -1;
+  if (C.P.nvv < 1) return fail("sampler needs at least one volume");
+  C.sampler = true;
+  C.xyz = xyz;
+  C.capacity = capacity;
+  Scratch sc;
+  C.dcount = (int *)sc.get(sizeof(int));
+  if (!C.dcount) return -1;
+  HIPCHK(hipMemsetAsync(C.dcount, 0, sizeof(int), S()));
+  int r = run_loop(C, n, *rays);
+  int c = 0;
+  HIPCHK(hipMemcpyAsync(&c, C.dcount, sizeof(int), hipMemcpyDeviceToHost, S()));
+  HIPCHK(hipStreamSynchronize(S()));
+  *count = c;
+  if (stats) *stats = C.st;
+  return r;
+}

@@ -1,0 +1,316 @@
+/*
+ * gxy_hip.h — C-ABI of the MI355X ray-processing path (libgxy_hip.so).
+ *
+ * The first block mirrors the ISPC exports that Galaxy's C++ renderer binds
+ * (generated headers TraceRays_ispc.h / Lighting_ispc.h / SamplerTraceRays_ispc.h,
+ * all C linkage).  The argument structs keep the reference layouts byte for byte;
+ * the difference is where the memory lives: every array a RayList_ispc points at
+ * is DEVICE memory (HBM), and the opaque OSPRay pointers of the reference
+ * (Model*, Volume*, TransferFunction*) are replaced by handles returned from the
+ * gxy_scene_* / gxy_volume_* / gxy_tf_* / gxy_model_* calls below, whose data is
+ * resident on the GPU.
+ *
+ * Errors: the reference has no return codes (it prints and carries on).  Every
+ * entry point here that can fail returns 0 on success and a negative value on
+ * failure, and gxy_last_error() returns the message.  Entry points that mirror a
+ * reference `void` export keep the `void`/`void*` return and record the error.
+ *
+ * Streams: all work is enqueued on the library's stream for the calling thread
+ * (gxy_set_stream overrides it).  Nothing blocks unless the entry point returns
+ * host-visible counts, which it documents.
+ */
+#ifndef GXY_HIP_H
+#define GXY_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- ray flags: src/renderer/RayFlags.h:28-36 ---------------------------- */
+#define RAY_PRIMARY            0x001
+#define RAY_SHADOW             0x002
+#define RAY_AO                 0x004
+#define RAY_EMPTY              0x008
+#define RAY_SURFACE            0x001
+#define RAY_OPAQUE             0x002
+#define RAY_BOUNDARY           0x004
+#define RAY_TIMEOUT            0x008
+#define RAY_EXTERNAL_BOUNDARY  0x010
+
+/* ---- classification sentinels: src/renderer/Renderer.cpp:78-82 ----------- */
+#define GXY_TERMINATED     (-1)
+#define GXY_DROP_ON_FLOOR  (-2)
+#define GXY_KEEP_HERE      (-3)
+#define GXY_UNDETERMINED   (-4)
+#define GXY_NO_NEIGHBOR    (-1)
+
+/* ---- RayList_ispc: src/renderer/Rays.ih:21-48 (25 SoA arrays, 100 B/ray) --
+ * Arrays are laid out inside one block exactly as RayList::setup_ispc_pointers
+ * does (src/renderer/Rays.cpp:173-204): a 64-byte header, then 20 float arrays
+ * and 5 int arrays, each aligned_size = ceil(n/16)*16 elements long. */
+typedef struct RayList_ispc {
+  float *ox, *oy, *oz;
+  float *dx, *dy, *dz;
+  float *nx, *ny, *nz;
+  float *sample;
+  float *r, *g, *b, *o;
+  float *sr, *sg, *sb, *so;
+  float *t, *tMax;
+  int   *x, *y, *type, *term, *classification;
+} RayList_ispc;
+
+/* RayList block header (src/renderer/Rays.h:49-59): three 8-byte Keys, then
+ * size, frame, aligned_size, id, type; padded to 64 bytes. */
+typedef struct gxy_raylist_hdr {
+  int64_t rendererKey, renderingKey, renderingSetKey;
+  int32_t size, frame, aligned_size, id, type;
+  int32_t pad_[5];
+} gxy_raylist_hdr;
+
+/* ---- Lighting_ispc: src/renderer/Lighting.ih:23-32 ------------------------ */
+typedef struct Lighting_ispc {
+  int      n_ao_rays;
+  int      do_shadows;
+  float    ao_radius;
+  float    Ka, Kd;
+  int     *types;    /* host array, nLights entries: 0 dir, 1 camera-rel, 2 point */
+  float   *lights;   /* host array, nLights * 3 floats (vec3f)                     */
+  uint32_t nLights;
+} Lighting_ispc;
+
+/* ---- visualization structs: Vis.ih:23-26, MappedVis.ih:26-30,
+ *      VolumeVis.ih:26-37, Visualization.ih:30-47 ---------------------------- */
+typedef struct Vis_ispc { void *data; } Vis_ispc;              /* data: gxy volume handle */
+typedef struct MappedVis_ispc { Vis_ispc vis; void *transferFunction; } MappedVis_ispc; /* gxy TF handle */
+typedef struct VolumeVis_ispc {
+  MappedVis_ispc vis;
+  int    nSlices;
+  float *slices;      /* host array of nSlices vec4f planes (nx, ny, nz, d) */
+  int    nIsovalues;
+  float *isovalues;   /* host array */
+  bool   volume_render;
+} VolumeVis_ispc;
+typedef struct gxy_box3f { float lower[3], upper[3]; } gxy_box3f;
+typedef struct Visualization_ispc {
+  void            *model;        /* gxy model handle (triangles + spheres), or NULL */
+  int32_t          nVolumeVis;
+  VolumeVis_ispc **volumeVis;
+  int32_t          nMappedVis;
+  MappedVis_ispc **mappedVis;
+  gxy_box3f        global_bb;
+  gxy_box3f        local_bb;
+} Visualization_ispc;
+
+/* ==== reference entry points (src/renderer/TraceRays.ispc) ================= */
+
+/* TraceRays.ispc:58-77 */
+void *TraceRays_allocate(void);
+void  TraceRays_initialize(void *self);
+void  TraceRays_destroy(void *self);
+
+/* TraceRays.ispc:326-623.  Traces nRaysIn device-resident rays in place:
+ * box clip, slices, geometry (BVH), volume march with TF compositing and
+ * isosurfaces; writes r g b o, t, term and, on a surface hit, sr sg sb so and
+ * nx ny nz.  Returns NULL like the reference. */
+void *TraceRays_TraceRays(void *self, void *vis, int nRaysIn, void *raysIn, float epsilon);
+
+/* TraceRays.ispc:625-733.  offsets: device int array, one per input ray. */
+void TraceRays_generateAORays(void *self, void *lights, int nRaysIn, void *raysIn,
+                              int *offsets, void *raysOut, float epsilon);
+/* TraceRays.ispc:763-857 */
+void TraceRays_generateShadowRays(void *self, void *lights, int nRaysIn, void *raysIn,
+                                  int *offsets, void *raysOut, float epsilon);
+/* TraceRays.ispc:735-761 */
+void TraceRays_ambientLighting(void *self, void *lights, int nRaysIn, void *raysIn);
+/* TraceRays.ispc:859-923 */
+void TraceRays_diffuseLighting(void *self, void *lights, int nRaysIn, void *raysIn);
+
+/* src/sampler/SamplerTraceRays.ispc:128-222 (iso-threshold sampler; the
+ * isovalue of each VolumeVis is its isovalues[0]). */
+void *SamplerTraceRays_SamplerTraceRays(void *self, void *vis, int nRaysIn, void *raysIn);
+
+/* src/renderer/Lighting.ispc:23-134 */
+void *Lighting_allocate(void);
+void  Lighting_initialize(void *self);
+void  Lighting_destroy(void *self);
+void  Lighting_SetLights(void *self, int n, float *l, int *t);
+void  Lighting_GetLights(void *self, int *n, float **l, int **t);
+void  Lighting_SetK(void *self, float ka, float kd);
+void  Lighting_GetK(void *self, float *ka, float *kd);
+void  Lighting_SetAO(void *self, int n, float r);
+void  Lighting_GetAO(void *self, int *n, float *r);
+void  Lighting_SetShadowFlag(void *self, int b);
+void  Lighting_GetShadowFlag(void *self, int *b);
+
+/* ==== runtime, memory and errors =========================================== */
+int         gxy_init(int device);           /* selects the device; idempotent */
+const char *gxy_last_error(void);
+int         gxy_device_count(void);
+int         gxy_synchronize(void);
+int         gxy_set_stream(void *hip_stream); /* NULL restores the library stream */
+void       *gxy_get_stream(void);
+void       *gxy_device_alloc(size_t bytes);   /* cached pool; 256-byte aligned */
+int         gxy_device_free(void *p);
+int         gxy_copy_to_device(void *dst, const void *src, size_t bytes);
+int         gxy_copy_to_host(void *dst, const void *src, size_t bytes);
+int         gxy_copy_device(void *dst, const void *src, size_t bytes);
+int         gxy_memset(void *dst, int value, size_t bytes);
+size_t      gxy_device_bytes_in_use(void);
+size_t      gxy_device_bytes_total(void);
+
+/* Kernel timing: every launch records HIP events on its stream when enabled.
+ * gxy_timing_get returns total ms and launch count for the named kernel. */
+int gxy_timing_enable(int on);
+int gxy_timing_reset(void);
+int gxy_timing_get(const char *kernel, double *total_ms, int *launches);
+
+/* ==== device RayLists (Rays.h:43-227, Rays.cpp:48-286) ====================== */
+/* Allocates one device block with the reference layout and fills *ispc with the
+ * 25 device array pointers.  Returns an opaque handle (NULL on failure). */
+void *gxy_raylist_create(int nrays, int frame, int type, RayList_ispc *ispc);
+int   gxy_raylist_destroy(void *rl);
+int   gxy_raylist_size(void *rl);
+int   gxy_raylist_aligned_size(void *rl);
+/* RayList::Split (Rays.cpp:206-243): copies rays [start, start+count) into a new
+ * list; RayList::Truncate (Rays.cpp:245-286). */
+void *gxy_raylist_split_part(void *rl, int start, int count, RayList_ispc *ispc);
+int   gxy_raylist_truncate(void *rl, int n, RayList_ispc *ispc);
+/* Whole-block host<->device copies for serialization (the wire format is the
+ * block itself: header + arrays). */
+size_t gxy_raylist_bytes(void *rl);
+int    gxy_raylist_download(void *rl, void *host_block);
+void  *gxy_raylist_upload(const void *host_block, RayList_ispc *ispc);
+
+/* ==== scene objects ========================================================== */
+/* Structured volume (OsprayVolume.cpp:25-52 over Volume.cpp:174-392).  dims,
+ * origin and spacing describe the ghosted brick; voxels is a HOST float array
+ * (x fastest) copied to HBM, or NULL to allocate without data.
+ * samplingRate as OSPRay's "samplingRate" (default 1). samplingStep is the
+ * minimum grid spacing. */
+void *gxy_volume_create(const int dims[3], const float origin[3], const float spacing[3],
+                        const float *voxels, float samplingRate);
+int   gxy_volume_destroy(void *vol);
+float *gxy_volume_device_data(void *vol);
+int   gxy_volume_set_sampling_rate(void *vol, float rate);
+/* Fills the volume with seeded 4-octave gradient noise normalised to [0,1]. */
+int   gxy_volume_fill_noise(void *vol, uint32_t seed, int octaves, float base_frequency);
+int   gxy_volume_download(void *vol, float *host);
+
+/* Transfer function: the 256-entry resample of MappedVis::local_commit
+ * (MappedVis.cpp:277-338).  colormap: n x [d, r, g, b]; opacitymap: m x [d, o];
+ * data_range: NULL or [lo, hi]. */
+void *gxy_tf_create_from_maps(const float *colormap, int ncolors,
+                              const float *opacitymap, int nopacities,
+                              const float *data_range);
+/* Direct tables (colors: nc x 3, opacities: no), value range [lo, hi]. */
+void *gxy_tf_create(const float *colors, int nc, const float *opacities, int no, float lo, float hi);
+int   gxy_tf_destroy(void *tf);
+int   gxy_tf_get_tables(void *tf, float *colors256x3, float *opac256, float *range2);
+
+/* Geometry model (OSPModel + Embree scene): triangle meshes
+ * (DataDrivenTriangleMesh.cpp:43-180) and data-driven spheres
+ * (DataDrivenSpheres.ispc:178-278).  Arrays are host arrays; the model copies
+ * them to HBM.  gxy_model_commit builds the device BVHs. */
+void *gxy_model_create(void);
+int   gxy_model_destroy(void *model);
+int   gxy_model_add_triangles(void *model, const float *vertices, int nvertices,
+                              const uint32_t *indices, int ntriangles,
+                              const float *normals, const float *vertex_data, void *tf);
+int   gxy_model_add_spheres(void *model, const float *centers, const float *data, int n,
+                            float radius0, float radius1, float value0, float value1, void *tf);
+int   gxy_model_commit(void *model);
+/* BVH statistics after commit: nodes, leaves, max depth, SAH cost. */
+int   gxy_model_bvh_stats(void *model, int *nodes, int *leaves, int *depth, float *sah);
+/* Side-band hit record of the last TraceRays_TraceRays on this model:
+ * per ray (geomID, primID); device arrays of n ints each, or NULL to disable. */
+int   gxy_model_set_hit_output(void *model, int *geom_ids, int *prim_ids);
+
+/* ==== the pass around TraceRays (TraceRays.cpp:68-146, Renderer.cpp:304-691) */
+
+/* TraceRays.cpp:89-117 on the device: exclusive offsets, AO block first, then
+ * the shadow block.  ao_offsets / shadow_offsets are device int arrays of
+ * nRays; n_ao / n_shadow receive the counts (this call synchronises). */
+int gxy_spawn_offsets(int nRays, RayList_ispc *raysIn, int nAO, int nLights, int do_shadows,
+                      int *ao_offsets, int *shadow_offsets, int *n_ao, int *n_shadow);
+
+/* Renderer::Classify (Renderer.cpp:304-421), reverse lighting on when
+ * reverse_lighting != 0 (the reference default, CMakeLists.txt:66). */
+int gxy_classify(int nRays, RayList_ispc *rays, int reverse_lighting);
+/* Renderer::AssignDestinations (Renderer.cpp:423-454) + Box::exit_face
+ * (Box.cpp:84-97); neighbors: 6 ranks or -1. */
+int gxy_assign_destinations(int nRays, RayList_ispc *rays, const gxy_box3f *local_box,
+                            const int neighbors[6], int reverse_lighting);
+/* HandleTerminatedRays + Rendering::AddLocalPixels (Renderer.cpp:456-502,
+ * Rendering.cpp:125-153): fb[(y*width+x)*4 + c] += rgba for every TERMINATED
+ * ray.  fb is a device float array of width*height*4. */
+int gxy_accumulate_terminated(int nRays, RayList_ispc *rays, float *fb, int width, int height);
+/* processRays_task::work's split (Renderer.cpp:561-634): stable partition of the
+ * rays by classification into ndest per-destination device lists plus keepers.
+ * counts[0..ndest-1] = rays for destination rank d, counts[ndest] = keepers.
+ * lists[d] may be NULL when only counts are wanted.  Synchronises. */
+int gxy_partition_rays(int nRays, RayList_ispc *rays, int ndest,
+                       RayList_ispc *const *lists, RayList_ispc *keepers, int *counts);
+
+/* Camera::generate_initial_rays + SpawnRays (Camera.cpp:379-493, 528-829):
+ * writes up to width*height primary rays (pixels whose ray meets the boxes)
+ * into rays; returns the count through n_out (synchronises).  order: 0 = row
+ * major as the reference, 1 = 8x8 pixel tiles (same ray set, better locality). */
+int gxy_camera_generate(const float eye[3], const float center_or_dir[3], int is_dir,
+                        const float up[3], float aov, int width, int height,
+                        const gxy_box3f *local_box, const gxy_box3f *global_box,
+                        RayList_ispc *rays, int capacity, int order, int *n_out);
+
+/* Sampler::HandleTerminatedRays (Sampler.cpp:51-89): appends the position
+ * o + t*d of every ray with term & RAY_SURFACE to the device particle array
+ * xyz (capacity entries, 3 floats each) at *count (device int, atomically
+ * advanced).  Order of particles within one call follows ray order. */
+int gxy_sampler_collect(int nRays, RayList_ispc *rays, float *xyz, int capacity, int *count);
+
+/* ==== one whole frame: the host loop of Renderer/RayQManager on the device == */
+typedef struct gxy_frame_stats {
+  long long primary_rays;      /* originated */
+  long long secondary_rays;    /* spawned AO + shadow */
+  long long ray_passes;        /* every TraceRays_TraceRays ray */
+  long long terminated;        /* accumulated into the framebuffer */
+  long long dropped;
+  long long kept;              /* KEEP_HERE re-traces */
+  long long outbound[6];       /* rays that left through face f to a neighbour */
+  long long samples;           /* volume sample points taken (all volumes) */
+  int       passes;            /* TraceRays launches */
+  int       lists;             /* RayLists processed */
+} gxy_frame_stats;
+
+/* Renders one camera view of vis under lights into fb (device, w*h*4 floats,
+ * accumulated into; clear it first).  Runs camera rays, then processes ray
+ * lists until the queue drains: trace, spawn, classify, route, accumulate,
+ * keepers and secondaries re-enqueued (priority: primary lists before
+ * secondary ones, LIFO within a level, as Threading.cpp:176-233).  Lists larger
+ * than max_rays_per_list are split (Renderer.cpp:675-690).  neighbors: 6 ranks
+ * or -1; outbound rays are counted per face and their lists kept until the next
+ * call of gxy_frame_outbound.  epsilon: Renderer epsilon (0.001 default). */
+int gxy_render_frame(void *vis, void *lights, const float eye[3], const float center[3],
+                     const float up[3], float aov, int width, int height,
+                     const int neighbors[6], float epsilon, int max_rays_per_list,
+                     float *fb, gxy_frame_stats *stats);
+
+/* Same loop starting from caller-supplied device rays (count n, not consumed). */
+int gxy_process_raylist(void *vis, void *lights, int n, RayList_ispc *rays,
+                        const int neighbors[6], float epsilon, int max_rays_per_list,
+                        float *fb, int width, int height, gxy_frame_stats *stats);
+
+/* Sampler loop (Sampler::Trace + KEEP_HERE re-trace): collects every isovalue
+ * crossing of the n rays into xyz (device, capacity*3 floats); *count receives
+ * the number of particles (synchronises). */
+int gxy_sample_raylist(void *vis, int n, RayList_ispc *rays, int max_rays_per_list,
+                       float *xyz, int capacity, int *count, gxy_frame_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GXY_HIP_H */

@@ -1,0 +1,1606 @@
+// gxy_oracle.cpp — CPU restatement of Galaxy's ray-processing hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This library is the checker for libgxy_hip.so: it
+// is loaded by tests/, by __graft_entry__.smoke() and by bench.py's CPU
+// baseline leg, and by nothing else.  The product path never calls it.
+//
+// Every function names the reference file:line it restates (paths relative to
+// the TACC/Galaxy tree).  Third-party arithmetic that is not in that tree
+// (OSPRay 1.8.5 StructuredVolume sampling and gradient, LinearTransferFunction
+// lookup, postIntersect normalisation; Embree 3.6.1 closest-hit rules) is
+// restated from the published algorithms and marked as such.
+//
+// How each part is pinned (what the tests check it against):
+//   * trilinear sample            - exact on ramp volumes (closed form), and the
+//                                   in-tree Volume::Sample (Volume.cpp:546-590)
+//   * TF resample + lookup        - closed-form values on two-point maps
+//   * march / iso / DVR / slices  - Galaxy gold images (tests/golds/*.png) via
+//                                   the whole frame; ramp-volume iso t closed form;
+//                                   FP64 compositing of the same samples
+//   * gradient stencil            - analytic sphere normals on the oneBall field
+//                                   (forward difference: OSPRay stencil itself is
+//                                   parity unpinned)
+//   * lighting / AO / shadows     - gold images (shadow and AO states); AO table
+//                                   against the reference's table values
+//   * triangle / sphere hits      - FP64 brute force on small scenes; BVH vs
+//                                   brute force (Embree's own tie order unpinned)
+//   * classify / route / counts   - hand-enumerated cases of Renderer.cpp rules
+//   * camera                      - gold images; FUZZ / box rules by cases
+//
+// Numerics: compiled with -ffp-contract=off; min/max are the a<b?a:b forms of
+// ISPC; the ISPC fast-math rcp/rsqrt are taken as correctly rounded 1/x and
+// 1/sqrt(x) (documented deviation: the reference's values differ by <= 1 ulp).
+
+#include <algorithm>
+#include <cfloat>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "../include/gxy_hip.h"
+
+#define OEXPORT extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+// ------------------------------------------------------------------ threads
+int g_threads = 0;
+int nthreads() {
+  if (g_threads > 0) return g_threads;
+  unsigned h = std::thread::hardware_concurrency();
+  return h ? (int)h : 1;
+}
+
+void parallel_for(int n, const std::function<void(int, int)> &body) {
+  int nt = nthreads();
+  if (nt <= 1 || n < 1024) { body(0, n); return; }
+  int chunk = (n + nt * 8 - 1) / (nt * 8);
+  std::atomic<int> next(0);
+  std::vector<std::thread> th;
+  for (int k = 0; k < nt; k++)
+    th.emplace_back([&]() {
+      for (;;) {
+        int s = next.fetch_add(chunk);
+        if (s >= n) break;
+        body(s, std::min(n, s + chunk));
+      }
+    });
+  for (auto &t : th) t.join();
+}
+
+// ------------------------------------------------------------------ vec math
+struct v3 { float x, y, z; };
+inline v3 mk(float x, float y, float z) { return {x, y, z}; }
+inline v3 operator+(v3 a, v3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline v3 operator-(v3 a, v3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline v3 operator*(float s, v3 a) { return {s * a.x, s * a.y, s * a.z}; }
+inline v3 neg(v3 a) { return {-a.x, -a.y, -a.z}; }
+inline float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline v3 cross(v3 a, v3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline float mn(float a, float b) { return a < b ? a : b; }
+inline float mx(float a, float b) { return a > b ? a : b; }
+inline float clamp01(float v) { return v < 0.f ? 0.f : (v > 1.f ? 1.f : v); }
+// ISPC normalize(v) = v * rsqrt(dot(v,v)) (rsqrt taken as 1/sqrt).
+inline v3 normalize(v3 v) { float s = 1.0f / std::sqrt(dot(v, v)); return s * v; }
+// OSPRay safe_normalize: guarded against a zero vector.
+inline v3 safe_normalize(v3 v) { float d = dot(v, v); d = d > 1e-12f ? d : 1e-12f; float s = 1.0f / std::sqrt(d); return s * v; }
+
+// ------------------------------------------------------------------ volume
+// OSPRay shared_structured_volume over Galaxy's ghosted brick
+// (OsprayVolume.cpp:25-52, Volume.h:124-129).  Sample = world->local by the
+// reciprocal spacing, clamp to [0, nextafter(dims-1, 0)], truncate, trilinear
+// in x, then y, then z with lerp(f,a,b) = (1-f)*a + f*b (the same order as the
+// in-tree Volume::Sample, Volume.cpp:562-576).
+struct OVolume {
+  int n[3];
+  float org[3], sp[3], isp[3], ub[3];
+  const float *data;
+  float samplingStep, samplingRate;
+};
+
+inline float vol_sample(const OVolume &V, v3 p) {
+  float l[3] = {(p.x - V.org[0]) * V.isp[0], (p.y - V.org[1]) * V.isp[1], (p.z - V.org[2]) * V.isp[2]};
+  int i[3];
+  float f[3];
+  for (int k = 0; k < 3; k++) {
+    float c = l[k] < 0.f ? 0.f : (l[k] > V.ub[k] ? V.ub[k] : l[k]);
+    if (c != c) c = 0.f;
+    i[k] = (int)c;
+    f[k] = c - (float)i[k];
+  }
+  const int64_t sx = 1, sy = V.n[0], sz = (int64_t)V.n[0] * V.n[1];
+  const float *d = V.data + i[0] + sy * i[1] + sz * i[2];
+  float v000 = d[0], v100 = d[sx], v010 = d[sy], v110 = d[sy + sx];
+  float v001 = d[sz], v101 = d[sz + sx], v011 = d[sz + sy], v111 = d[sz + sy + sx];
+  float gx = 1.0f - f[0], gy = 1.0f - f[1], gz = 1.0f - f[2];
+  float a00 = gx * v000 + f[0] * v100;
+  float a10 = gx * v010 + f[0] * v110;
+  float a01 = gx * v001 + f[0] * v101;
+  float a11 = gx * v011 + f[0] * v111;
+  float b0 = gy * a00 + f[1] * a10;
+  float b1 = gy * a01 + f[1] * a11;
+  return gz * b0 + f[2] * b1;
+}
+
+// Gradient: forward differences at one grid spacing divided by the spacing.
+// (OSPRay StructuredVolume computeGradient; stencil parity unpinned.)
+inline v3 vol_gradient(const OVolume &V, v3 p) {
+  float s = vol_sample(V, p);
+  float gx = vol_sample(V, p + mk(V.sp[0], 0.f, 0.f)) - s;
+  float gy = vol_sample(V, p + mk(0.f, V.sp[1], 0.f)) - s;
+  float gz = vol_sample(V, p + mk(0.f, 0.f, V.sp[2])) - s;
+  return mk(gx / V.sp[0], gy / V.sp[1], gz / V.sp[2]);
+}
+
+// ------------------------------------------------------------------ TF
+// OSPRay LinearTransferFunction (piecewise_linear): valueRange clamps to the end
+// entries; inside, x = (v-lo)/(hi-lo)*(n-1), i = floor(x), lerp(i, i+1).
+struct OTF {
+  std::vector<float> col;  // nc*3
+  std::vector<float> op;   // no
+  float lo, hi;
+};
+
+inline v3 tf_color(const OTF &T, float v) {
+  int n = (int)T.col.size() / 3;
+  if (v != v) return mk(0.f, 0.f, 0.f);
+  if (n == 0) return mk(1.f, 1.f, 1.f);
+  if (v <= T.lo) return mk(T.col[0], T.col[1], T.col[2]);
+  if (v >= T.hi) return mk(T.col[3 * n - 3], T.col[3 * n - 2], T.col[3 * n - 1]);
+  float x = (v - T.lo) / (T.hi - T.lo) * ((float)n - 1.0f);
+  int i = (int)std::floor(x);
+  float r = x - (float)i;
+  int j = i + 1 < n - 1 ? i + 1 : n - 1;
+  float q = 1.0f - r;
+  return mk(q * T.col[3 * i] + r * T.col[3 * j], q * T.col[3 * i + 1] + r * T.col[3 * j + 1],
+            q * T.col[3 * i + 2] + r * T.col[3 * j + 2]);
+}
+
+inline float tf_opacity(const OTF &T, float v) {
+  int n = (int)T.op.size();
+  if (v != v) return 0.f;
+  if (n == 0) return 1.f;
+  if (v <= T.lo) return T.op[0];
+  if (v >= T.hi) return T.op[n - 1];
+  float x = (v - T.lo) / (T.hi - T.lo) * ((float)n - 1.0f);
+  int i = (int)std::floor(x);
+  float r = x - (float)i;
+  int j = i + 1 < n - 1 ? i + 1 : n - 1;
+  return (1.0f - r) * T.op[i] + r * T.op[j];
+}
+
+// MappedVis::local_commit 256-entry resample (MappedVis.cpp:286-333).
+void tf_resample(OTF &T, const float *cmap, int nc, const float *omap, int no, const float *range) {
+  T.col.assign(256 * 3, 0.f);
+  T.op.assign(256, 0.f);
+  int i0 = 0, i1 = 1;
+  float xmin = cmap[0], xmax = cmap[4 * (nc - 1)];
+  for (int i = 0; i < 256; i++) {
+    float x = (float)(xmin + (i / (255.0)) * (xmax - xmin));
+    if (x > xmax) x = xmax;
+    while (i1 < nc - 1 && cmap[4 * i1] < x) i0++, i1++;
+    float d = (x - cmap[4 * i0]) / (cmap[4 * i1] - cmap[4 * i0]);
+    T.col[3 * i + 0] = cmap[4 * i0 + 1] + d * (cmap[4 * i1 + 1] - cmap[4 * i0 + 1]);
+    T.col[3 * i + 1] = cmap[4 * i0 + 2] + d * (cmap[4 * i1 + 2] - cmap[4 * i0 + 2]);
+    T.col[3 * i + 2] = cmap[4 * i0 + 3] + d * (cmap[4 * i1 + 3] - cmap[4 * i0 + 3]);
+  }
+  i0 = 0, i1 = 1;
+  xmin = omap[0], xmax = omap[2 * (no - 1)];
+  for (int i = 0; i < 256; i++) {
+    float x = (float)(xmin + (i / (255.0)) * (xmax - xmin));
+    if (x > xmax) x = xmax;
+    while (i1 < no - 1 && omap[2 * i1] < x) i0++, i1++;
+    float d = (x - omap[2 * i0]) / (omap[2 * i1] - omap[2 * i0]);
+    T.op[i] = omap[2 * i0 + 1] + d * (omap[2 * i1 + 1] - omap[2 * i0 + 1]);
+  }
+  if (range) { T.lo = range[0]; T.hi = range[1]; }
+  else { T.lo = cmap[0]; T.hi = cmap[4 * (nc - 1)]; }
+}
+
+// ------------------------------------------------------------------ geometry
+struct OTri { v3 a, b, c; };
+struct OMeshPart {
+  int geomID;
+  std::vector<float> V, N, D;     // vertices, normals (opt), vertex data (opt)
+  std::vector<uint32_t> I;
+  OTF *tf;
+};
+struct OSpheres {
+  int geomID;
+  std::vector<float> C, D;
+  float r0, r1, v0, v1, eps;
+  OTF *tf;
+};
+
+struct Prim { int geom; int prim; };
+
+// Binary BVH over bounding boxes (binned SAH, 16 bins, leaves <= 4 primitives).
+struct BNode { float lo[3], hi[3]; int left, count, start; };
+struct OBVH {
+  std::vector<BNode> nodes;
+  std::vector<int> order;
+};
+
+struct Box3 {
+  float lo[3], hi[3];
+  void empty() { for (int k = 0; k < 3; k++) { lo[k] = 1e30f; hi[k] = -1e30f; } }
+  void grow(const Box3 &b) { for (int k = 0; k < 3; k++) { lo[k] = std::min(lo[k], b.lo[k]); hi[k] = std::max(hi[k], b.hi[k]); } }
+  void growp(const float *p) { for (int k = 0; k < 3; k++) { lo[k] = std::min(lo[k], p[k]); hi[k] = std::max(hi[k], p[k]); } }
+  float area() const {
+    float d[3];
+    for (int k = 0; k < 3; k++) d[k] = std::max(0.f, hi[k] - lo[k]);
+    return 2.f * (d[0] * d[1] + d[1] * d[2] + d[2] * d[0]);
+  }
+};
+
+void bvh_build(OBVH &B, const std::vector<Box3> &boxes) {
+  int n = (int)boxes.size();
+  B.nodes.clear();
+  B.order.resize(n);
+  for (int i = 0; i < n; i++) B.order[i] = i;
+  if (n == 0) return;
+  std::vector<float> cen(3 * (size_t)n);
+  for (int i = 0; i < n; i++)
+    for (int k = 0; k < 3; k++) cen[3 * (size_t)i + k] = 0.5f * (boxes[i].lo[k] + boxes[i].hi[k]);
+  struct Job { int node, start, count; };
+  std::vector<Job> stack;
+  B.nodes.push_back(BNode());
+  stack.push_back({0, 0, n});
+  while (!stack.empty()) {
+    Job j = stack.back();
+    stack.pop_back();
+    Box3 bb, cb;
+    bb.empty();
+    cb.empty();
+    for (int i = j.start; i < j.start + j.count; i++) {
+      bb.grow(boxes[B.order[i]]);
+      cb.growp(&cen[3 * (size_t)B.order[i]]);
+    }
+    BNode &nd = B.nodes[j.node];
+    for (int k = 0; k < 3; k++) { nd.lo[k] = bb.lo[k]; nd.hi[k] = bb.hi[k]; }
+    if (j.count <= 4) { nd.left = -1; nd.start = j.start; nd.count = j.count; continue; }
+    const int NB = 16;
+    float best = 1e38f;
+    int bestAxis = -1, bestSplit = -1;
+    for (int ax = 0; ax < 3; ax++) {
+      float ext = cb.hi[ax] - cb.lo[ax];
+      if (!(ext > 0.f)) continue;
+      Box3 bin[NB];
+      int cnt[NB] = {0};
+      for (int b = 0; b < NB; b++) bin[b].empty();
+      for (int i = j.start; i < j.start + j.count; i++) {
+        int p = B.order[i];
+        int b = (int)((cen[3 * (size_t)p + ax] - cb.lo[ax]) / ext * NB);
+        b = b < 0 ? 0 : (b >= NB ? NB - 1 : b);
+        cnt[b]++;
+        bin[b].grow(boxes[p]);
+      }
+      float ra[NB];
+      int rc[NB];
+      Box3 acc;
+      acc.empty();
+      int c = 0;
+      for (int b = NB - 1; b > 0; b--) { acc.grow(bin[b]); c += cnt[b]; ra[b] = acc.area(); rc[b] = c; }
+      acc.empty();
+      c = 0;
+      for (int b = 0; b < NB - 1; b++) {
+        acc.grow(bin[b]);
+        c += cnt[b];
+        if (c == 0 || rc[b + 1] == 0) continue;
+        float cost = acc.area() * c + ra[b + 1] * rc[b + 1];
+        if (cost < best) { best = cost; bestAxis = ax; bestSplit = b; }
+      }
+    }
+    int mid;
+    if (bestAxis < 0) {
+      mid = j.start + j.count / 2;
+    } else {
+      float ext = cb.hi[bestAxis] - cb.lo[bestAxis];
+      auto it = std::partition(B.order.begin() + j.start, B.order.begin() + j.start + j.count, [&](int p) {
+        int b = (int)((cen[3 * (size_t)p + bestAxis] - cb.lo[bestAxis]) / ext * NB);
+        b = b < 0 ? 0 : (b >= NB ? NB - 1 : b);
+        return b <= bestSplit;
+      });
+      mid = (int)(it - B.order.begin());
+      if (mid == j.start || mid == j.start + j.count) mid = j.start + j.count / 2;
+    }
+    int l = (int)B.nodes.size();
+    B.nodes.push_back(BNode());
+    B.nodes.push_back(BNode());
+    B.nodes[j.node].left = l;
+    B.nodes[j.node].count = 0;
+    stack.push_back({l, j.start, mid - j.start});
+    stack.push_back({l + 1, mid, j.start + j.count - mid});
+  }
+}
+
+struct OModel {
+  std::vector<std::unique_ptr<OMeshPart>> meshes;
+  std::vector<std::unique_ptr<OSpheres>> spheres;
+  int ngeom = 0;
+  // flattened triangle list (all meshes) for brute force / BVH
+  std::vector<Prim> tris;
+  std::vector<Prim> sphs;
+  OBVH tbvh, sbvh;
+  bool use_bvh = true;
+  bool committed = false;
+};
+
+// Watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013), FP32, the
+// same operation order as the HIP kernel.  Hit accepted when t0 < t <= tfar
+// (Embree 3's depth test); returns u, v with hit = (1-u-v)*v0 + u*v1 + v*v2.
+struct WRay {
+  v3 org, dir;
+  int kx, ky, kz;
+  float Sx, Sy, Sz;
+};
+inline WRay wray(v3 o, v3 d) {
+  WRay r;
+  r.org = o;
+  r.dir = d;
+  float ad[3] = {std::fabs(d.x), std::fabs(d.y), std::fabs(d.z)};
+  int kz = 0;
+  if (ad[1] > ad[kz]) kz = 1;
+  if (ad[2] > ad[kz]) kz = 2;
+  int kx = kz + 1 == 3 ? 0 : kz + 1;
+  int ky = kx + 1 == 3 ? 0 : kx + 1;
+  float dd[3] = {d.x, d.y, d.z};
+  if (dd[kz] < 0.f) std::swap(kx, ky);
+  r.kx = kx; r.ky = ky; r.kz = kz;
+  r.Sx = dd[kx] / dd[kz];
+  r.Sy = dd[ky] / dd[kz];
+  r.Sz = 1.0f / dd[kz];
+  return r;
+}
+inline float comp(v3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+
+inline bool tri_hit(const WRay &r, v3 p0, v3 p1, v3 p2, float t0, float tfar, float &t, float &u, float &v) {
+  v3 A = p0 - r.org, B = p1 - r.org, C = p2 - r.org;
+  float Az_ = comp(A, r.kz), Bz_ = comp(B, r.kz), Cz_ = comp(C, r.kz);
+  float Ax = comp(A, r.kx) - r.Sx * Az_, Ay = comp(A, r.ky) - r.Sy * Az_;
+  float Bx = comp(B, r.kx) - r.Sx * Bz_, By = comp(B, r.ky) - r.Sy * Bz_;
+  float Cx = comp(C, r.kx) - r.Sx * Cz_, Cy = comp(C, r.ky) - r.Sy * Cz_;
+  float U = Cx * By - Cy * Bx;
+  float V = Ax * Cy - Ay * Cx;
+  float W = Bx * Ay - By * Ax;
+  if (U == 0.f || V == 0.f || W == 0.f) {
+    double CxBy = (double)Cx * (double)By, CyBx = (double)Cy * (double)Bx;
+    U = (float)(CxBy - CyBx);
+    double AxCy = (double)Ax * (double)Cy, AyCx = (double)Ay * (double)Cx;
+    V = (float)(AxCy - AyCx);
+    double BxAy = (double)Bx * (double)Ay, ByAx = (double)By * (double)Ax;
+    W = (float)(BxAy - ByAx);
+  }
+  if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) return false;
+  float det = U + V + W;
+  if (det == 0.f) return false;
+  float Az = r.Sz * Az_, Bz = r.Sz * Bz_, Cz = r.Sz * Cz_;
+  float T = U * Az + V * Bz + W * Cz;
+  float rcp = 1.0f / det;
+  float tt = T * rcp;
+  if (!(tt > t0 && tt <= tfar)) return false;
+  t = tt;
+  u = V * rcp;
+  v = W * rcp;
+  return true;
+}
+
+// DataDrivenSpheres_intersect_kernel (DataDrivenSpheres.ispc:90-155).
+inline float sphere_radius(const OSpheres &S, int i) {
+  if (S.v0 != S.v1) {
+    float dv = S.D[i];
+    float d = (dv - S.v0) / (S.v1 - S.v0);
+    if (d > 1) return S.r1;
+    if (d < 0) return S.r0;
+    return S.r0 + d * (S.r1 - S.r0);
+  }
+  return S.r0;
+}
+inline bool sphere_hit(const OSpheres &S, int i, v3 org, v3 dir, float t0, float tfar, float &t) {
+  v3 center = mk(S.C[3 * i], S.C[3 * i + 1], S.C[3 * i + 2]);
+  float radius = sphere_radius(S, i);
+  float approxDist = dot(center - org, dir);
+  v3 closeOrg = org + approxDist * dir;
+  v3 A = center - closeOrg;
+  float a = dot(dir, dir);
+  float b = 2.f * dot(dir, A);
+  float c = dot(A, A) - radius * radius;
+  float radical = b * b - 4.f * a * c;
+  if (radical < 0.f) return false;
+  float srad = std::sqrt(radical);
+  float r2a = 1.0f / (2.f * a);
+  float t_in = (b - srad) * r2a + approxDist;
+  float t_out = (b + srad) * r2a + approxDist;
+  if (t_in > t0 && t_in < tfar) { t = t_in; return true; }
+  if (t_out > (t0 + S.eps) && t_out < tfar) { t = t_out; return true; }
+  return false;
+}
+
+struct GeoHit {
+  float t;
+  int geom, prim;
+  float u, v;
+};
+
+// Closest hit: candidate replaces the current one when it is nearer, or equal
+// in t with a lower (geomID, primID) — the documented deterministic tie rule.
+inline bool better(float t, int g, int p, const GeoHit &h) {
+  if (t < h.t) return true;
+  if (t == h.t && (g < h.geom || (g == h.geom && p < h.prim))) return true;
+  return false;
+}
+
+void model_trace(const OModel &M, v3 org, v3 dir, float t0, float tfar, GeoHit &h) {
+  h.t = tfar;
+  h.geom = -1;
+  h.prim = -1;
+  if (tfar < t0) return;
+  WRay wr = wray(org, dir);
+  auto test_tri = [&](int idx) {
+    const Prim &P = M.tris[idx];
+    const OMeshPart &m = *M.meshes[P.geom];
+    const uint32_t *I = &m.I[3 * (size_t)P.prim];
+    v3 p0 = mk(m.V[3 * I[0]], m.V[3 * I[0] + 1], m.V[3 * I[0] + 2]);
+    v3 p1 = mk(m.V[3 * I[1]], m.V[3 * I[1] + 1], m.V[3 * I[1] + 2]);
+    v3 p2 = mk(m.V[3 * I[2]], m.V[3 * I[2] + 1], m.V[3 * I[2] + 2]);
+    float t, u, v;
+    if (tri_hit(wr, p0, p1, p2, t0, h.t, t, u, v) && (h.geom < 0 || better(t, m.geomID, P.prim, h))) {
+      h.t = t; h.geom = m.geomID; h.prim = P.prim; h.u = u; h.v = v;
+    }
+  };
+  auto test_sph = [&](int idx) {
+    const Prim &P = M.sphs[idx];
+    const OSpheres &s = *M.spheres[P.geom];
+    float t;
+    float lim = h.geom < 0 ? h.t : std::nextafter(h.t, 3e38f);
+    if (sphere_hit(s, P.prim, org, dir, t0, lim, t) && (h.geom < 0 || better(t, s.geomID, P.prim, h))) {
+      h.t = t; h.geom = s.geomID; h.prim = P.prim; h.u = h.v = 0.f;
+    }
+  };
+  auto walk = [&](const OBVH &B, int nprim, const std::function<void(int)> &test) {
+    if (!M.use_bvh) { for (int i = 0; i < nprim; i++) test(i); return; }
+    if (B.nodes.empty()) return;
+    float inv[3] = {1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z};
+    float o[3] = {org.x, org.y, org.z};
+    int stack[128];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+      const BNode &nd = B.nodes[stack[--sp]];
+      float tmin = t0, tmax = h.t;
+      bool miss = false;
+      for (int k = 0; k < 3; k++) {
+        float a = (nd.lo[k] - o[k]) * inv[k], b = (nd.hi[k] - o[k]) * inv[k];
+        float lo = a < b ? a : b, hi = a < b ? b : a;
+        tmin = lo > tmin ? lo : tmin;
+        tmax = hi < tmax ? hi : tmax;
+      }
+      // slack of a few ulps so that a hit at exactly the current t still reaches its leaf
+      if (tmin > tmax * 1.0000005f + 1e-30f) miss = true;
+      if (miss) continue;
+      if (nd.left < 0) {
+        for (int i = nd.start; i < nd.start + nd.count; i++) test(B.order[i]);
+      } else {
+        stack[sp++] = nd.left;
+        stack[sp++] = nd.left + 1;
+      }
+    }
+  };
+  walk(M.tbvh, (int)M.tris.size(), test_tri);
+  walk(M.sbvh, (int)M.sphs.size(), test_sph);
+}
+
+// ------------------------------------------------------------------ vis
+struct OVolumeVis {
+  OVolume *vol;
+  OTF *tf;
+  std::vector<float> slices, iso;
+  bool dvr;
+};
+
+struct Hit {
+  float sample = 0.f, t = 0.f;
+  v3 point = {0, 0, 0}, normal = {0, 0, 0}, color = {0, 0, 0};
+  float opacity = 0.f;
+};
+
+// Per-ray output beyond the RayList (side-band): geometry ids, FP64 colour,
+// sample count.
+struct SideBand {
+  int *geom = nullptr, *prim = nullptr;
+  double *rgba64 = nullptr;
+  long long *samples = nullptr;  // single counter (atomic add)
+};
+
+inline void box_clip(v3 org, v3 dir, const gxy_box3f &b, float &t0, float &t1) {
+  // MyIntersectBox, TraceRays.ispc:90-106
+  float ix = 1.0f / dir.x, iy = 1.0f / dir.y, iz = 1.0f / dir.z;
+  float minx = (b.lower[0] - org.x) * ix, miny = (b.lower[1] - org.y) * iy, minz = (b.lower[2] - org.z) * iz;
+  float maxx = (b.upper[0] - org.x) * ix, maxy = (b.upper[1] - org.y) * iy, maxz = (b.upper[2] - org.z) * iz;
+  t0 = mx(mn(minx, maxx), mx(mn(miny, maxy), mn(minz, maxz)));
+  t1 = mn(mx(minx, maxx), mn(mx(miny, maxy), mx(minz, maxz)));
+}
+
+template <typename Acc>
+struct Col4 { Acc x, y, z, w; };
+
+// opaque test of TraceRays.ispc:466,550,581
+template <typename Acc>
+inline bool opaque_of(const Col4<Acc> &c) {
+  Acc m = c.x < c.y ? c.x : c.y;
+  m = m < c.z ? m : c.z;
+  return m >= (Acc)1 || c.w > (Acc)0.999f;
+}
+
+#define GXY_MAX_VOLS_ORACLE 16
+
+// TraceRays_TraceRays for one ray (TraceRays.ispc:366-620).  Acc is the
+// compositing precision (float = the path; double = FP64 compositing oracle).
+template <typename Acc>
+void trace_one(const Visualization_ispc &vis, const std::vector<OVolumeVis> &vv, const OModel *model,
+               float step, float epsilon, bool integrate, RayList_ispc &R, int i, SideBand &sb,
+               long long &nsamples) {
+  const bool shadeFlag = R.type[i] == RAY_PRIMARY;
+  v3 org = mk(R.ox[i], R.oy[i], R.oz[i]);
+  v3 dir = mk(R.dx[i], R.dy[i], R.dz[i]);
+  if (dir.x == 0.f) dir.x = 1e-6f;
+  if (dir.y == 0.f) dir.y = 1e-6f;
+  if (dir.z == 0.f) dir.z = 1e-6f;
+  float ray_t0 = R.t[i];
+  float ray_t = R.tMax[i];
+  const float tTimeout = ray_t;
+  Col4<Acc> color = {(Acc)R.r[i], (Acc)R.g[i], (Acc)R.b[i], (Acc)R.o[i]};
+
+  float tEntry, tExitVolume;
+  box_clip(org, dir, vis.local_bb, tEntry, tExitVolume);
+  if (tEntry < ray_t0) tEntry = ray_t0;
+  else if (tEntry > ray_t0) ray_t0 = tEntry;
+  ray_t = mn(ray_t, tExitVolume);
+
+  Hit hit;
+  // LookForSliceHit (TraceRays.ispc:143-250)
+  bool h = false;
+  int vid = -1;
+  for (int major = 0; major < (int)vv.size(); major++) {
+    const OVolumeVis &V = vv[major];
+    for (int minor = 0; minor < (int)V.slices.size() / 4; minor++) {
+      v3 pn = mk(V.slices[4 * minor], V.slices[4 * minor + 1], V.slices[4 * minor + 2]);
+      float d = V.slices[4 * minor + 3];
+      float denom = dot(dir, pn);
+      if (std::fabs(denom) > 0.0001f) {
+        float t = (d - dot(org, pn)) / denom;
+        if (t >= ray_t0 && t <= ray_t) {
+          hit.normal = denom > 0 ? neg(pn) : pn;
+          hit.opacity = 1.0f;
+          hit.t = t;
+          vid = major;
+          ray_t = hit.t;
+          h = true;
+        }
+      }
+    }
+  }
+  if (h && shadeFlag) {
+    hit.point = org + hit.t * dir;
+    const OVolumeVis &V = vv[vid];
+    hit.sample = vol_sample(*V.vol, hit.point);
+    hit.color = tf_color(*V.tf, hit.sample);
+    hit.opacity = 1.0f;
+  }
+  bool surface_hit = h;
+
+  // LookForGeometryHit (TraceRays.ispc:108-141)
+  if (model) {
+    GeoHit gh;
+    model_trace(*model, org, dir, ray_t0, ray_t, gh);
+    if (gh.geom >= 0) {
+      ray_t = gh.t;
+      surface_hit = true;
+      if (sb.geom) { sb.geom[i] = gh.geom; sb.prim[i] = gh.prim; }
+      if (shadeFlag) {
+        // postIntersect + Galaxy's geometry postIntersect, then DG_NORMALIZE|DG_FACEFORWARD
+        v3 Ng, Ns, col = mk(1.f, 1.f, 1.f);
+        float alpha = 1.f;
+        v3 P = org + gh.t * dir;
+        bool isTri = false;
+        for (auto &m : model->meshes)
+          if (m->geomID == gh.geom) {
+            isTri = true;
+            const uint32_t *I = &m->I[3 * (size_t)gh.prim];
+            v3 p0 = mk(m->V[3 * I[0]], m->V[3 * I[0] + 1], m->V[3 * I[0] + 2]);
+            v3 p1 = mk(m->V[3 * I[1]], m->V[3 * I[1] + 1], m->V[3 * I[1] + 2]);
+            v3 p2 = mk(m->V[3 * I[2]], m->V[3 * I[2] + 1], m->V[3 * I[2] + 2]);
+            Ng = cross(p0 - p1, p2 - p0);
+            Ns = Ng;
+            float b0 = 1.0f - gh.u - gh.v, b1 = gh.u, b2 = gh.v;
+            if (!m->N.empty()) {
+              v3 a = mk(m->N[3 * I[0]], m->N[3 * I[0] + 1], m->N[3 * I[0] + 2]);
+              v3 b = mk(m->N[3 * I[1]], m->N[3 * I[1] + 1], m->N[3 * I[1] + 2]);
+              v3 c = mk(m->N[3 * I[2]], m->N[3 * I[2] + 1], m->N[3 * I[2] + 2]);
+              Ns = b0 * a + b1 * b + b2 * c;
+            }
+            if (!m->D.empty() && m->tf) {
+              float dv = b0 * m->D[I[0]] + b1 * m->D[I[1]] + b2 * m->D[I[2]];
+              col = tf_color(*m->tf, dv);
+            }
+            break;
+          }
+        if (!isTri) {
+          for (auto &s : model->spheres)
+            if (s->geomID == gh.geom) {
+              v3 c = mk(s->C[3 * gh.prim], s->C[3 * gh.prim + 1], s->C[3 * gh.prim + 2]);
+              Ng = P - c;
+              Ns = Ng;
+              if (s->tf && !s->D.empty()) col = tf_color(*s->tf, s->D[gh.prim]);
+              break;
+            }
+        }
+        Ng = normalize(Ng);
+        Ns = normalize(Ns);
+        if (dot(dir, Ng) >= 0.f) Ng = neg(Ng);
+        if (dot(dir, Ns) >= 0.f) Ns = neg(Ns);
+        hit.color = col;
+        hit.opacity = alpha;
+        hit.point = P;
+        hit.normal = Ns;
+        hit.t = gh.t;
+      }
+    }
+  }
+
+  float tTermination = ray_t;
+  long long ns = 0;
+
+  if (integrate) {
+    const int nv = (int)vv.size();
+    float sLast[GXY_MAX_VOLS_ORACLE], sThis[GXY_MAX_VOLS_ORACLE];
+    float tLast = tEntry + epsilon, tThis;
+    bool hit_isosurface = false;
+    bool opaque = opaque_of(color);
+    long guard = 0;
+    for (tThis = tEntry; tThis <= tTermination && !opaque && !hit_isosurface;) {
+      v3 coord = org + tThis * dir;
+      for (int v = 0; v < nv; v++) sThis[v] = vol_sample(*vv[v].vol, coord);
+      ns += nv;
+      if (tThis > tEntry && tLast >= epsilon) {
+        // LookForIsoHit (TraceRays.ispc:252-310): the last matching isovalue wins
+        bool ih = false;
+        int ivid = -1;
+        for (int major = 0; major < nv; major++) {
+          float sl = sLast[major], st = sThis[major];
+          for (float isoval : vv[major].iso)
+            if (((isoval >= sl) && (isoval < st)) || ((isoval <= sl) && (isoval > st))) {
+              ih = true;
+              ivid = major;
+              hit.t = tLast + ((isoval - sl) / (st - sl)) * (tThis - tLast);
+              hit.sample = isoval;
+            }
+        }
+        if (ih) {
+          hit.point = org + hit.t * dir;
+          if (shadeFlag) {
+            const OVolumeVis &V = vv[ivid];
+            hit.normal = safe_normalize(vol_gradient(*V.vol, hit.point));
+            if (dot(dir, hit.normal) > 0) hit.normal = neg(hit.normal);
+            hit.color = tf_color(*V.tf, hit.sample);
+            hit.opacity = 1.0f;
+          }
+          tTermination = hit.t;
+          tThis = hit.t;
+          surface_hit = true;
+          hit_isosurface = true;
+          v3 c2 = org + tThis * dir;
+          for (int v = 0; v < nv; v++) sThis[v] = vol_sample(*vv[v].vol, c2);
+          ns += nv;
+        }
+        for (int major = 0; major < nv; major++) {
+          const OVolumeVis &V = vv[major];
+          if (V.dvr) {
+            float sVolume = (sLast[major] + sThis[major]) / 2;
+            float sampleOpacity = tf_opacity(*V.tf, sVolume);
+            if (sampleOpacity > 0) {
+              if (shadeFlag) {
+                v3 sc = tf_color(*V.tf, sVolume);
+                float wo = clamp01(sampleOpacity / V.vol->samplingRate);
+                Acc a = (Acc)1 - color.w;
+                Acc w = (Acc)wo;
+                color.x = color.x + a * (w * (Acc)sc.x);
+                color.y = color.y + a * (w * (Acc)sc.y);
+                color.z = color.z + a * (w * (Acc)sc.z);
+                color.w = color.w + a * w;
+              } else {
+                float wop = ((tThis - tLast) / step) * clamp01(sampleOpacity / V.vol->samplingRate);
+                Acc f = (Acc)1 - (Acc)wop;
+                color.x *= f; color.y *= f; color.z *= f; color.w *= f;
+              }
+            }
+          }
+          sLast[major] = sThis[major];
+        }
+      }
+      for (int v = 0; v < nv; v++) sLast[v] = sThis[v];
+      opaque = opaque_of(color);
+      if (opaque) tTermination = tThis;
+      tLast = tThis;
+      float nxt = (tThis == tEntry) ? (tEntry + epsilon)
+                  : (((tThis + step) > tTermination) && (tThis < tTermination)) ? tTermination : tThis + step;
+      // A step that no longer advances t would loop for ever in the reference;
+      // it ends the march here.
+      if (!(nxt > tThis) && !opaque && !hit_isosurface) break;
+      tThis = nxt;
+      if (++guard >= (1L << 20)) break;
+    }
+    ray_t = tTermination;
+  }
+  if (sb.samples) nsamples += ns;
+
+  float fr = (float)color.x, fg = (float)color.y, fb = (float)color.z, fo = (float)color.w;
+  R.r[i] = fr; R.g[i] = fg; R.b[i] = fb; R.o[i] = fo;
+  if (sb.rgba64) {
+    sb.rgba64[4 * (size_t)i] = (double)color.x; sb.rgba64[4 * (size_t)i + 1] = (double)color.y;
+    sb.rgba64[4 * (size_t)i + 2] = (double)color.z; sb.rgba64[4 * (size_t)i + 3] = (double)color.w;
+  }
+  int term = (mn(mn(fr, fg), fb) >= 1.0f || fo > 0.999f) ? RAY_OPAQUE : 0;
+  R.t[i] = ray_t;
+  if (surface_hit) {
+    term |= RAY_SURFACE;
+    if (hit.opacity > 0.999f) term |= RAY_OPAQUE;
+    R.sr[i] = hit.color.x; R.sg[i] = hit.color.y; R.sb[i] = hit.color.z; R.so[i] = 1.0f;
+    R.nx[i] = hit.normal.x; R.ny[i] = hit.normal.y; R.nz[i] = hit.normal.z;
+  } else if (tTermination == tExitVolume) {
+    term |= RAY_BOUNDARY;
+  } else if (tTermination == tTimeout) {
+    term |= RAY_TIMEOUT;
+  }
+  R.term[i] = term;
+}
+
+struct OVis {
+  Visualization_ispc *ispc;
+  std::vector<OVolumeVis> vv;
+  const OModel *model;
+  float step, eps;
+  bool integrate;
+};
+
+// TraceRays.ispc:342-361: the step is the minimum samplingStep*samplingRate,
+// epsilon = global_epsilon * step.
+OVis resolve_vis(void *visp, float global_epsilon) {
+  OVis o;
+  o.ispc = (Visualization_ispc *)visp;
+  o.model = (const OModel *)o.ispc->model;
+  o.integrate = false;
+  o.step = -1;
+  o.eps = 0;
+  for (int m = 0; m < o.ispc->nVolumeVis; m++) {
+    VolumeVis_ispc *v = o.ispc->volumeVis[m];
+    OVolumeVis V;
+    V.vol = (OVolume *)v->vis.vis.data;
+    V.tf = (OTF *)v->vis.transferFunction;
+    V.slices.assign(v->slices, v->slices + 4 * v->nSlices);
+    V.iso.assign(v->isovalues, v->isovalues + v->nIsovalues);
+    V.dvr = v->volume_render;
+    if (V.dvr || !V.iso.empty()) o.integrate = true;
+    float s = V.vol->samplingStep * V.vol->samplingRate;
+    if (o.step < 0 || o.step > s) { o.step = s; o.eps = global_epsilon * s; }
+    o.vv.push_back(V);
+  }
+  return o;
+}
+
+// ------------------------------------------------------------------ host RayList
+struct HRayList {
+  int n = 0, aligned = 0, type = 0, frame = 0;
+  std::vector<float> f;
+  std::vector<int> iv;
+  RayList_ispc r;
+  explicit HRayList(int nn, int typ = 0) { alloc(nn); type = typ; }
+  void alloc(int nn) {
+    n = nn;
+    aligned = (nn + 15) & ~15;
+    if (aligned == 0) aligned = 16;
+    f.assign((size_t)aligned * 20, 0.f);
+    iv.assign((size_t)aligned * 5, 0);
+    float **fp = &r.ox;
+    for (int k = 0; k < 20; k++) fp[k] = f.data() + (size_t)k * aligned;
+    int **ip = &r.x;
+    for (int k = 0; k < 5; k++) ip[k] = iv.data() + (size_t)k * aligned;
+  }
+};
+
+void copy_ray(const RayList_ispc &s, int i, RayList_ispc &d, int j) {
+  float *const *sf = &s.ox;
+  float **df = &d.ox;
+  for (int k = 0; k < 20; k++) df[k][j] = sf[k][i];
+  int *const *si = &s.x;
+  int **di = &d.x;
+  for (int k = 0; k < 5; k++) di[k][j] = si[k][i];
+}
+
+// ------------------------------------------------------------------ lighting
+struct OLights {
+  int nAO, doShadows;
+  float aoRadius, Ka, Kd;
+  int n;
+  const int *types;
+  const float *L;
+};
+OLights lights_of(void *lp) {
+  Lighting_ispc *l = (Lighting_ispc *)lp;
+  return {l->n_ao_rays, l->do_shadows, l->ao_radius, l->Ka, l->Kd, (int)l->nLights, l->types, l->lights};
+}
+
+float g_U[256], g_V[256];
+bool g_uv_ready = false;
+// UV.ih:21-58: base-2 / base-3 radical inverses accumulated in float32 with
+// the digit weight multiplied by 1/b each step, then rounded to 6 significant
+// digits.  This reproduces the reference table entry for entry.
+void make_uv() {
+  if (g_uv_ready) return;
+  for (int b = 2; b <= 3; b++)
+    for (int i = 0; i < 256; i++) {
+      float inv = 1.0f / (float)b, f = inv, r = 0.f;
+      for (int j = i; j > 0; j /= b) { r = r + (float)(j % b) * f; f = f * inv; }
+      char buf[64];
+      snprintf(buf, sizeof buf, "%.6g", (double)r);
+      (b == 2 ? g_U : g_V)[i] = strtof(buf, nullptr);
+    }
+  g_uv_ready = true;
+}
+
+const float kTwoPi = 6.2831855f;  // 2.f*M_PI in float
+
+}  // namespace
+
+// ======================================================================= API
+
+OEXPORT void oracle_set_threads(int n) { g_threads = n; }
+OEXPORT int oracle_get_threads(void) { return nthreads(); }
+
+OEXPORT void oracle_uv_tables(float *U, float *V) {
+  make_uv();
+  memcpy(U, g_U, sizeof g_U);
+  memcpy(V, g_V, sizeof g_V);
+}
+
+OEXPORT void *oracle_volume_create(const int dims[3], const float origin[3], const float spacing[3],
+                                   const float *voxels, float samplingRate) {
+  OVolume *v = new OVolume;
+  for (int k = 0; k < 3; k++) {
+    v->n[k] = dims[k];
+    v->org[k] = origin[k];
+    v->sp[k] = spacing[k];
+    v->isp[k] = 1.0f / spacing[k];
+    v->ub[k] = std::nextafter((float)(dims[k] - 1), 0.0f);
+  }
+  v->data = voxels;
+  v->samplingStep = std::min(spacing[0], std::min(spacing[1], spacing[2]));
+  v->samplingRate = samplingRate;
+  return v;
+}
+OEXPORT void oracle_volume_destroy(void *v) { delete (OVolume *)v; }
+OEXPORT float oracle_volume_sample(void *v, float x, float y, float z) { return vol_sample(*(OVolume *)v, mk(x, y, z)); }
+OEXPORT void oracle_volume_gradient(void *v, float x, float y, float z, float *g) {
+  v3 r = vol_gradient(*(OVolume *)v, mk(x, y, z));
+  g[0] = r.x; g[1] = r.y; g[2] = r.z;
+}
+
+OEXPORT void *oracle_tf_create_from_maps(const float *cmap, int nc, const float *omap, int no, const float *range) {
+  OTF *t = new OTF;
+  tf_resample(*t, cmap, nc, omap, no, range);
+  return t;
+}
+OEXPORT void *oracle_tf_create(const float *colors, int nc, const float *op, int no, float lo, float hi) {
+  OTF *t = new OTF;
+  t->col.assign(colors, colors + 3 * nc);
+  t->op.assign(op, op + no);
+  t->lo = lo;
+  t->hi = hi;
+  return t;
+}
+OEXPORT void oracle_tf_destroy(void *t) { delete (OTF *)t; }
+OEXPORT void oracle_tf_get_tables(void *tp, float *c, float *o, float *range) {
+  OTF *t = (OTF *)tp;
+  memcpy(c, t->col.data(), t->col.size() * 4);
+  memcpy(o, t->op.data(), t->op.size() * 4);
+  range[0] = t->lo;
+  range[1] = t->hi;
+}
+OEXPORT void oracle_tf_lookup(void *tp, float v, float *rgb, float *op) {
+  v3 c = tf_color(*(OTF *)tp, v);
+  rgb[0] = c.x; rgb[1] = c.y; rgb[2] = c.z;
+  *op = tf_opacity(*(OTF *)tp, v);
+}
+
+OEXPORT void *oracle_model_create(void) { return new OModel; }
+OEXPORT void oracle_model_destroy(void *m) { delete (OModel *)m; }
+OEXPORT int oracle_model_add_triangles(void *mp, const float *V, int nv, const uint32_t *I, int nt, const float *N,
+                                       const float *D, void *tf) {
+  OModel *M = (OModel *)mp;
+  auto m = std::unique_ptr<OMeshPart>(new OMeshPart);
+  m->geomID = M->ngeom++;
+  m->V.assign(V, V + 3 * (size_t)nv);
+  m->I.assign(I, I + 3 * (size_t)nt);
+  if (N) m->N.assign(N, N + 3 * (size_t)nv);
+  if (D) m->D.assign(D, D + (size_t)nv);
+  m->tf = (OTF *)tf;
+  M->meshes.push_back(std::move(m));
+  return M->meshes.back()->geomID;
+}
+OEXPORT int oracle_model_add_spheres(void *mp, const float *C, const float *D, int n, float r0, float r1, float v0,
+                                     float v1, void *tf) {
+  OModel *M = (OModel *)mp;
+  auto s = std::unique_ptr<OSpheres>(new OSpheres);
+  s->geomID = M->ngeom++;
+  s->C.assign(C, C + 3 * (size_t)n);
+  if (D) s->D.assign(D, D + n);
+  else s->D.assign(n, 0.f);
+  s->r0 = r0; s->r1 = r1; s->v0 = v0; s->v1 = v1;
+  s->tf = (OTF *)tf;
+  // DataDrivenSpheresGeometry_set (DataDrivenSpheres.ispc:211-220)
+  float e = std::log(r0);
+  if (e < 0.f) e = -1.f / e;
+  if (e > r0 / 100.0) e = (float)(r0 / 100.0);
+  s->eps = e;
+  M->spheres.push_back(std::move(s));
+  return M->spheres.back()->geomID;
+}
+OEXPORT int oracle_model_commit(void *mp, int use_bvh) {
+  OModel *M = (OModel *)mp;
+  M->use_bvh = use_bvh != 0;
+  M->tris.clear();
+  M->sphs.clear();
+  std::vector<Box3> tb, sbx;
+  for (int g = 0; g < (int)M->meshes.size(); g++) {
+    auto &m = *M->meshes[g];
+    int nt = (int)m.I.size() / 3;
+    for (int p = 0; p < nt; p++) {
+      M->tris.push_back({g, p});
+      Box3 b;
+      b.empty();
+      for (int k = 0; k < 3; k++) b.growp(&m.V[3 * (size_t)m.I[3 * (size_t)p + k]]);
+      tb.push_back(b);
+    }
+  }
+  for (int g = 0; g < (int)M->spheres.size(); g++) {
+    auto &s = *M->spheres[g];
+    int n = (int)s.C.size() / 3;
+    for (int p = 0; p < n; p++) {
+      M->sphs.push_back({g, p});
+      float r = sphere_radius(s, p);
+      Box3 b;
+      for (int k = 0; k < 3; k++) { b.lo[k] = s.C[3 * p + k] - r; b.hi[k] = s.C[3 * p + k] + r; }
+      sbx.push_back(b);
+    }
+  }
+  if (M->use_bvh) {
+    bvh_build(M->tbvh, tb);
+    bvh_build(M->sbvh, sbx);
+  }
+  M->committed = true;
+  return 0;
+}
+
+// Closest hit of n rays against the model (side-band query used by the
+// geometry parity tests): t (or tfar when missed), geom, prim, u, v.
+OEXPORT void oracle_model_intersect(void *mp, int n, const float *org, const float *dir, const float *t0,
+                                    const float *tfar, float *t, int *geom, int *prim, float *u, float *v) {
+  OModel *M = (OModel *)mp;
+  parallel_for(n, [&](int s, int e) {
+    for (int i = s; i < e; i++) {
+      GeoHit h;
+      model_trace(*M, mk(org[3 * i], org[3 * i + 1], org[3 * i + 2]), mk(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]),
+                  t0[i], tfar[i], h);
+      t[i] = h.t; geom[i] = h.geom; prim[i] = h.prim;
+      u[i] = h.geom >= 0 ? h.u : 0.f;
+      v[i] = h.geom >= 0 ? h.v : 0.f;
+    }
+  });
+}
+
+// FP64 brute-force closest hit over the triangles of one mesh (pins tri_hit).
+OEXPORT void oracle_triangles_bruteforce64(const float *V, const uint32_t *I, int nt, int n, const float *org,
+                                           const float *dir, const float *t0, double *t, int *prim) {
+  parallel_for(n, [&](int s, int e) {
+    for (int i = s; i < e; i++) {
+      double o[3] = {org[3 * i], org[3 * i + 1], org[3 * i + 2]}, d[3] = {dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]};
+      double best = 1e300;
+      int bp = -1;
+      for (int p = 0; p < nt; p++) {
+        double a[3], b[3], c[3];
+        for (int k = 0; k < 3; k++) {
+          a[k] = V[3 * I[3 * p] + k]; b[k] = V[3 * I[3 * p + 1] + k]; c[k] = V[3 * I[3 * p + 2] + k];
+        }
+        double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+        double pv[3] = {d[1] * e2[2] - d[2] * e2[1], d[2] * e2[0] - d[0] * e2[2], d[0] * e2[1] - d[1] * e2[0]};
+        double det = e1[0] * pv[0] + e1[1] * pv[1] + e1[2] * pv[2];
+        if (det == 0) continue;
+        double inv = 1.0 / det;
+        double tv[3] = {o[0] - a[0], o[1] - a[1], o[2] - a[2]};
+        double uu = (tv[0] * pv[0] + tv[1] * pv[1] + tv[2] * pv[2]) * inv;
+        if (uu < 0 || uu > 1) continue;
+        double qv[3] = {tv[1] * e1[2] - tv[2] * e1[1], tv[2] * e1[0] - tv[0] * e1[2], tv[0] * e1[1] - tv[1] * e1[0]};
+        double vv = (d[0] * qv[0] + d[1] * qv[1] + d[2] * qv[2]) * inv;
+        if (vv < 0 || uu + vv > 1) continue;
+        double tt = (e2[0] * qv[0] + e2[1] * qv[1] + e2[2] * qv[2]) * inv;
+        if (tt > t0[i] && tt < best) { best = tt; bp = p; }
+      }
+      t[i] = best;
+      prim[i] = bp;
+    }
+  });
+}
+
+// TraceRays_TraceRays (TraceRays.ispc:326-623) over a host RayList.
+// fp64: composite in double and write rgba64 (n*4) when non-null.
+OEXPORT long long oracle_TraceRays(void *visp, int n, RayList_ispc *R, float global_epsilon, int fp64,
+                                   double *rgba64, int *geom, int *prim) {
+  OVis V = resolve_vis(visp, global_epsilon);
+  std::atomic<long long> total(0);
+  SideBand sb;
+  sb.geom = geom;
+  sb.prim = prim;
+  sb.rgba64 = rgba64;
+  long long dummy = 0;
+  sb.samples = &dummy;
+  if (geom) for (int i = 0; i < n; i++) { geom[i] = -1; prim[i] = -1; }
+  parallel_for(n, [&](int s, int e) {
+    long long ns = 0;
+    for (int i = s; i < e; i++) {
+      if (fp64) trace_one<double>(*V.ispc, V.vv, V.model, V.step, V.eps, V.integrate, *R, i, sb, ns);
+      else trace_one<float>(*V.ispc, V.vv, V.model, V.step, V.eps, V.integrate, *R, i, sb, ns);
+    }
+    total += ns;
+  });
+  return total.load();
+}
+
+// TraceRays.cpp:89-117.
+OEXPORT int oracle_spawn_offsets(int n, RayList_ispc *R, int nAO, int nLights, int do_shadows, int *ao_off,
+                                 int *sh_off, int *n_ao, int *n_sh) {
+  int nOut = 0;
+  for (int i = 0; i < n; i++) {
+    ao_off[i] = nOut;
+    if (R->type[i] == RAY_PRIMARY && (R->term[i] & RAY_SURFACE)) nOut += nAO;
+  }
+  int aok = nOut;
+  for (int i = 0; i < n; i++) sh_off[i] = -1;
+  if (do_shadows)
+    for (int i = 0; i < n; i++) {
+      if (R->type[i] == RAY_PRIMARY && (R->term[i] & RAY_SURFACE)) { sh_off[i] = nOut; nOut += nLights; }
+      else sh_off[i] = -1;
+    }
+  *n_ao = aok;
+  *n_sh = nOut - aok;
+  return nOut;
+}
+
+// TraceRays_ambientLighting (TraceRays.ispc:735-761)
+OEXPORT void oracle_ambientLighting(void *lp, int n, RayList_ispc *R) {
+  OLights L = lights_of(lp);
+  for (int i = 0; i < n; i++)
+    if (R->type[i] == RAY_PRIMARY && (R->term[i] & RAY_SURFACE)) {
+      float s = L.Ka * (1.0f - R->o[i]);
+      R->r[i] += s * R->sr[i];
+      R->g[i] += s * R->sg[i];
+      R->b[i] += s * R->sb[i];
+    }
+}
+
+// TraceRays_generateAORays (TraceRays.ispc:625-733), reverse lighting.
+OEXPORT void oracle_generateAORays(void *lp, int n, RayList_ispc *R, const int *offsets, RayList_ispc *O, float eps) {
+  make_uv();
+  OLights L = lights_of(lp);
+  const float Ka = -L.Ka / L.nAO;
+  for (int i = 0; i < n; i++) {
+    if (!(R->type[i] == RAY_PRIMARY && (R->term[i] & RAY_SURFACE))) continue;
+    v3 nrm = mk(R->nx[i], R->ny[i], R->nz[i]);
+    float as = Ka * (1.0f - R->o[i]);
+    float ar = as * R->sr[i], ag = as * R->sg[i], ab = as * R->sb[i];
+    v3 b0 = mk(1.f, 0.f, 0.f);
+    if (std::fabs(dot(b0, nrm)) > 0.95f) b0 = mk(0.f, 1.f, 0.f);
+    v3 b1 = normalize(cross(b0, nrm));
+    b0 = normalize(cross(b1, nrm));
+    float ox = R->ox[i] + R->t[i] * R->dx[i];
+    float oy = R->oy[i] + R->t[i] * R->dy[i];
+    float oz = R->oz[i] + R->t[i] * R->dz[i];
+    ox = ox + eps * nrm.x;
+    oy = oy + eps * nrm.y;
+    oz = oz + eps * nrm.z;
+    for (int j = 0; j < L.nAO; j++) {
+      int off = offsets[i] + j;
+      int r = ((R->x[i] * 9949 + R->y[i] * 9613 + j * 9151) >> 8) & 0xff;
+      float r0 = g_U[r], r1 = g_V[r];
+      float w = std::sqrt(1.f - r1);
+      float x = std::cos(kTwoPi * r0) * w;
+      float y = std::sin(kTwoPi * r0) * w;
+      float z = std::sqrt(r1) + eps;
+      v3 rd = (x * b0 + y * b1) + z * nrm;
+      O->ox[off] = ox; O->oy[off] = oy; O->oz[off] = oz;
+      O->dx[off] = rd.x; O->dy[off] = rd.y; O->dz[off] = rd.z;
+      O->r[off] = ar; O->g[off] = ag; O->b[off] = ab; O->o[off] = 0.f;
+      O->t[off] = 0.f; O->tMax[off] = L.aoRadius;
+      O->x[off] = R->x[i]; O->y[off] = R->y[i];
+      O->type[off] = RAY_AO; O->term[off] = 0;
+    }
+  }
+}
+
+// TraceRays_diffuseLighting (TraceRays.ispc:859-923)
+OEXPORT void oracle_diffuseLighting(void *lp, int n, RayList_ispc *R) {
+  OLights L = lights_of(lp);
+  const float Kd = L.Kd / L.n;
+  for (int i = 0; i < n; i++) {
+    if (!(R->type[i] == RAY_PRIMARY && (R->term[i] & RAY_SURFACE))) continue;
+    v3 nrm = mk(R->nx[i], R->ny[i], R->nz[i]);
+    float tr = 0, tg = 0, tb = 0;
+    for (int k = 0; k < L.n; k++) {
+      v3 lv;
+      v3 Lk = mk(L.L[3 * k], L.L[3 * k + 1], L.L[3 * k + 2]);
+      if (L.types[k]) {
+        float t = R->t[i];
+        v3 p = mk(R->ox[i] + t * R->dx[i], R->oy[i] + t * R->dy[i], R->oz[i] + t * R->dz[i]);
+        lv = safe_normalize(Lk - p);
+      } else lv = neg(Lk);
+      float d = dot(nrm, lv);
+      if (d > 0) {
+        float dff = (1.0f - R->o[i]) * d;
+        tr += dff * R->sr[i];
+        tg += dff * R->sg[i];
+        tb += dff * R->sb[i];
+      }
+    }
+    float o = R->o[i];
+    R->r[i] = R->r[i] + Kd * (1 - o) * tr;
+    R->g[i] = R->g[i] + Kd * (1 - o) * tg;
+    R->b[i] = R->b[i] + Kd * (1 - o) * tb;
+    R->o[i] = o + Kd * (1 - o) * o;
+  }
+}
+
+// TraceRays_generateShadowRays (TraceRays.ispc:763-857), reverse lighting.
+OEXPORT void oracle_generateShadowRays(void *lp, int n, RayList_ispc *R, const int *offsets, RayList_ispc *O,
+                                       float eps) {
+  OLights L = lights_of(lp);
+  const float Kd = -L.Kd / L.n;
+  for (int i = 0; i < n; i++) {
+    int off = offsets[i];
+    if (off == -1) continue;
+    v3 nrm = mk(R->nx[i], R->ny[i], R->nz[i]);
+    for (int k = 0; k < L.n; k++) {
+      v3 sp = mk(R->ox[i] + R->t[i] * R->dx[i] + eps * R->nx[i], R->oy[i] + R->t[i] * R->dy[i] + eps * R->ny[i],
+                 R->oz[i] + R->t[i] * R->dz[i] + eps * R->nz[i]);
+      v3 Lk = mk(L.L[3 * k], L.L[3 * k + 1], L.L[3 * k + 2]);
+      v3 lv = L.types[k] ? safe_normalize(Lk - sp) : neg(Lk);
+      lv = safe_normalize(lv);
+      float d = dot(nrm, lv);
+      if (d < 0) d = 0;
+      float dff = (1.0f - R->o[i]) * Kd * d;
+      O->ox[off] = sp.x; O->oy[off] = sp.y; O->oz[off] = sp.z;
+      O->dx[off] = lv.x; O->dy[off] = lv.y; O->dz[off] = lv.z;
+      O->r[off] = dff * R->sr[i]; O->g[off] = dff * R->sg[i]; O->b[off] = dff * R->sb[i]; O->o[off] = 0.f;
+      O->t[off] = 0.f; O->tMax[off] = INFINITY;
+      O->x[off] = R->x[i]; O->y[off] = R->y[i];
+      O->type[off] = RAY_SHADOW; O->term[off] = 0;
+      off++;
+    }
+  }
+}
+
+// Renderer::Classify (Renderer.cpp:304-421)
+OEXPORT void oracle_classify(int n, RayList_ispc *R, int reverse) {
+  for (int i = 0; i < n; i++) {
+    int typ = R->type[i], term = R->term[i];
+    int c = GXY_DROP_ON_FLOOR;
+    if (typ == RAY_PRIMARY) {
+      if (term & RAY_BOUNDARY) c = RAY_BOUNDARY;
+      else if ((term & RAY_OPAQUE) | (term & RAY_TIMEOUT)) c = GXY_TERMINATED;
+      else c = GXY_KEEP_HERE;
+    } else if (typ == RAY_SHADOW) {
+      if ((term & RAY_OPAQUE) | (term & RAY_SURFACE)) c = reverse ? GXY_TERMINATED : GXY_DROP_ON_FLOOR;
+      else if (term & RAY_BOUNDARY) c = RAY_BOUNDARY;
+      else c = GXY_DROP_ON_FLOOR;
+    } else if (typ == RAY_AO) {
+      if ((term & RAY_OPAQUE) | (term & RAY_SURFACE)) c = reverse ? GXY_TERMINATED : GXY_DROP_ON_FLOOR;
+      else if (term & RAY_BOUNDARY) c = RAY_BOUNDARY;
+      else if (term & RAY_TIMEOUT) c = reverse ? GXY_DROP_ON_FLOOR : GXY_TERMINATED;
+      else c = GXY_DROP_ON_FLOOR;
+    } else {
+      c = R->classification[i];  // other types are left as they are
+    }
+    R->classification[i] = c;
+  }
+}
+
+// Box::exit_face (Box.cpp:84-97)
+static int exit_face(const gxy_box3f &b, float x, float y, float z, float dx, float dy, float dz) {
+  float tx = (dx > 0.0001f) ? ((b.upper[0] - x) / dx) : (dx < -0.0001f) ? ((b.lower[0] - x) / dx) : FLT_MAX;
+  float ty = (dy > 0.0001f) ? ((b.upper[1] - y) / dy) : (dy < -0.0001f) ? ((b.lower[1] - y) / dy) : FLT_MAX;
+  float tz = (dz > 0.0001f) ? ((b.upper[2] - z) / dz) : (dz < -0.0001f) ? ((b.lower[2] - z) / dz) : FLT_MAX;
+  if (tx < 0) tx = FLT_MAX;
+  if (ty < 0) ty = FLT_MAX;
+  if (tz < 0) tz = FLT_MAX;
+  if (tx < ty && tx < tz) return (dx < 0) ? 0 : 1;
+  else if (ty < tz) return (dy < 0) ? 2 : 3;
+  else return (dz < 0) ? 4 : 5;
+}
+
+OEXPORT int oracle_exit_face(const gxy_box3f *b, float x, float y, float z, float dx, float dy, float dz) {
+  return exit_face(*b, x, y, z, dx, dy, dz);
+}
+
+// Renderer::AssignDestinations (Renderer.cpp:423-454)
+OEXPORT void oracle_assign_destinations(int n, RayList_ispc *R, const gxy_box3f *box, const int nb[6], int reverse) {
+  for (int i = 0; i < n; i++) {
+    if (R->classification[i] != RAY_BOUNDARY) continue;
+    int f = exit_face(*box, R->ox[i], R->oy[i], R->oz[i], R->dx[i], R->dy[i], R->dz[i]);
+    if (nb[f] >= 0) R->classification[i] = nb[f];
+    else {
+      int t = R->type[i];
+      if (t == RAY_SHADOW || t == RAY_AO) R->classification[i] = reverse ? GXY_DROP_ON_FLOOR : GXY_TERMINATED;
+      else R->classification[i] = GXY_TERMINATED;
+    }
+  }
+}
+
+// HandleTerminatedRays + AddLocalPixels (Renderer.cpp:456-502, Rendering.cpp:91-153)
+OEXPORT int oracle_accumulate(int n, RayList_ispc *R, float *fb, int w, int h) {
+  int k = 0;
+  for (int i = 0; i < n; i++)
+    if (R->classification[i] == GXY_TERMINATED) {
+      int x = R->x[i], y = R->y[i];
+      if (x < 0 || y < 0 || x >= w || y >= h) continue;
+      float *p = fb + 4 * ((size_t)y * w + x);
+      p[0] += R->r[i]; p[1] += R->g[i]; p[2] += R->b[i]; p[3] += R->o[i];
+      k++;
+    }
+  return k;
+}
+
+// processRays_task counts (Renderer.cpp:561-579): counts[d] for ranks, counts[ndest] keepers.
+// order (nullable, n entries): destination-major stable order of the copied rays.
+OEXPORT int oracle_partition_counts(int n, RayList_ispc *R, int ndest, int *counts) {
+  for (int d = 0; d <= ndest; d++) counts[d] = 0;
+  int err = 0;
+  for (int i = 0; i < n; i++) {
+    int c = R->classification[i];
+    if (c == GXY_KEEP_HERE) counts[ndest]++;
+    else if (c >= 0 && c < ndest) counts[c]++;
+    else if (c != GXY_DROP_ON_FLOOR && c != GXY_TERMINATED) err++;
+  }
+  return err;
+}
+
+// SamplerTraceRays_SamplerTraceRays (SamplerTraceRays.ispc:128-222) with
+// IsoSamplerVis (IsoSamplerVis.ispc:36-70); each VolumeVis's isovalue is its
+// isovalues[0].
+OEXPORT void oracle_SamplerTraceRays(void *visp, int n, RayList_ispc *R) {
+  Visualization_ispc *vis = (Visualization_ispc *)visp;
+  if (vis->nVolumeVis < 1) return;
+  int nv = vis->nVolumeVis;
+  std::vector<OVolume *> vols(nv);
+  std::vector<float> isov(nv);
+  float step = 0;
+  for (int k = 0; k < nv; k++) {
+    vols[k] = (OVolume *)vis->volumeVis[k]->vis.vis.data;
+    isov[k] = vis->volumeVis[k]->nIsovalues > 0 ? vis->volumeVis[k]->isovalues[0] : 0.f;
+    float s = vols[k]->samplingStep * vols[k]->samplingRate;
+    if (k == 0 || s < step) step = s;
+  }
+  const gxy_box3f &box = vis->local_bb;
+  parallel_for(n, [&](int s0, int e0) {
+    std::vector<float> sLast(nv), tLastv(nv), tHit(nv);
+    for (int i = s0; i < e0; i++) {
+      v3 org = mk(R->ox[i], R->oy[i], R->oz[i]);
+      v3 dir = mk(R->dx[i], R->dy[i], R->dz[i]);
+      float rt = R->t[i];
+      if (dir.x == 0.f) dir.x = 1e-6f;
+      if (dir.y == 0.f) dir.y = 1e-6f;
+      if (dir.z == 0.f) dir.z = 1e-6f;
+      float tEntry, tExit;
+      box_clip(org, dir, box, tEntry, tExit);
+      if (tEntry < rt) tEntry = rt;
+      float tThis = tEntry + step;
+      int hit = -1;
+      for (int k = 0; k < nv; k++) { sLast[k] = vol_sample(*vols[k], org + tEntry * dir); tLastv[k] = tEntry; }
+      long guard = 0;
+      while (tThis <= tExit && hit == -1 && guard++ < (1L << 20)) {
+        for (int k = 0; k < nv && hit == -1; k++) {
+          float sThis = vol_sample(*vols[k], org + tThis * dir);
+          float iso = isov[k];
+          bool h = false;
+          if ((sLast[k] < iso && sThis >= iso) || (sLast[k] > iso && sThis <= iso)) {
+            tHit[k] = tLastv[k] + (((iso - sLast[k]) / (sThis - sLast[k])) * (tThis - tLastv[k]));
+            h = true;
+          }
+          sLast[k] = sThis;
+          tLastv[k] = tThis;
+          if (h) { tThis = tHit[k]; hit = k; }
+        }
+        if (hit != -1 || tThis == tExit) break;
+        float nt = tThis + step;
+        if (nt > tExit) nt = tExit;
+        if (!(nt > tThis)) break;
+        tThis = nt;
+      }
+      R->t[i] = hit != -1 ? tThis + 0.001f : tThis;
+      R->term[i] = hit != -1 ? RAY_SURFACE : RAY_BOUNDARY;
+    }
+  });
+}
+
+// ------------------------------------------------------------------ camera
+// Box::intersect (Box.cpp:107-150)
+static bool box_intersect(const gxy_box3f &b, v3 org, v3 dir, float &tmin, float &tmax) {
+  tmin = (b.lower[0] - org.x) / dir.x;
+  tmax = (b.upper[0] - org.x) / dir.x;
+  if (tmin > tmax) std::swap(tmax, tmin);
+  if (tmax < 0) return false;
+  float tymin = (b.lower[1] - org.y) / dir.y, tymax = (b.upper[1] - org.y) / dir.y;
+  if (tymin > tymax) std::swap(tymax, tymin);
+  if (tymax < 0) return false;
+  if ((tmin > tymax) || (tymin > tmax)) return false;
+  if (tymin > tmin) tmin = tymin;
+  if (tymax < tmax) tmax = tymax;
+  float tzmin = (b.lower[2] - org.z) / dir.z, tzmax = (b.upper[2] - org.z) / dir.z;
+  if (tzmin > tzmax) std::swap(tzmin, tzmax);
+  if (tzmax < 0) return false;
+  if ((tmin > tzmax) || (tzmin > tmax)) return false;
+  if (tzmin > tmin) tmin = tzmin;
+  if (tzmax < tmax) tmax = tzmax;
+  if (tmin < 0) tmin = 0;
+  return true;
+}
+
+// dtypes.h normalize: d = len(a); if (d != 0) { d = 1.0/d; a *= d }
+static inline void dt_normalize(v3 &a) {
+  float d = std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+  if (d != 0) {
+    d = (float)(1.0 / d);
+    a.x *= d; a.y *= d; a.z *= d;
+  }
+}
+
+// Camera::generate_initial_rays + SpawnRays (Camera.cpp:379-493, 528-829) in
+// batch mode (no permutation, GXY_FULLWINDOW off).  The projection culling at
+// Camera.cpp:643-653 assigns shadowed locals, so the full window is scanned.
+OEXPORT int oracle_camera_generate(const float eye[3], const float cdir[3], int is_dir, const float up[3], float aov,
+                                   int width, int height, const gxy_box3f *lbox, const gxy_box3f *gbox,
+                                   RayList_ispc *R, int capacity) {
+  v3 veye = mk(eye[0], eye[1], eye[2]);
+  v3 vu = mk(up[0], up[1], up[2]);
+  v3 vdir = is_dir ? mk(cdir[0], cdir[1], cdir[2]) : mk(cdir[0] - eye[0], cdir[1] - eye[1], cdir[2] - eye[2]);
+  v3 center;
+  if (aov == 0.0f) {
+    center = veye + vdir;
+    dt_normalize(vdir);
+  } else {
+    float d = (float)(1.0 / std::tan(2 * 3.1415926 * (aov / 2.0) / 360.0));
+    dt_normalize(vdir);
+    center = veye + d * vdir;
+  }
+  v3 vr = cross(vdir, vu);
+  dt_normalize(vr);
+  vu = cross(vr, vdir);
+  dt_normalize(vu);
+  float pixel_scaling = (float)(((width < height ? width : height) - 1.0) / 2.0);
+  float off_x = (float)((width - 1) / 2.0), off_y = (float)((height - 1) / 2.0);
+  float scaling = (float)(1.0 / pixel_scaling);
+  int iwidth = width;
+  int count = width * height;
+  int dst = 0;
+  bool is_ortho = aov == 0.0f;
+  for (int p = 0; p < count; p++) {
+    int x = p % iwidth, y = p / iwidth;
+    float fx = (x - off_x) * scaling;
+    float fy = (y - off_y) * scaling;
+    v3 xy = mk(center.x + fx * vr.x + fy * vu.x, center.y + fx * vr.y + fy * vu.y, center.z + fx * vr.z + fy * vu.z);
+    v3 vorigin, vray;
+    if (is_ortho) { vorigin = xy - vdir; vray = vdir; }
+    else { vorigin = veye; vray = xy - veye; dt_normalize(vray); }
+    float gmin = 0, gmax = 0, lmin = 0, lmax = 0;
+    bool hit = box_intersect(*gbox, vorigin, vray, gmin, gmax);
+    if (hit) hit = box_intersect(*lbox, vorigin, vray, lmin, lmax);
+    float d = std::fabs(lmin) - std::fabs(gmin);
+    if (hit && (lmax >= 0) && ((double)d < 0.000001) && ((double)d > -0.000001)) {
+      if (dst >= capacity) return -1;
+      R->x[dst] = x; R->y[dst] = y;
+      R->ox[dst] = vorigin.x; R->oy[dst] = vorigin.y; R->oz[dst] = vorigin.z;
+      R->dx[dst] = vray.x; R->dy[dst] = vray.y; R->dz[dst] = vray.z;
+      R->r[dst] = 0; R->g[dst] = 0; R->b[dst] = 0; R->o[dst] = 0;
+      R->t[dst] = 0; R->tMax[dst] = FLT_MAX;
+      R->type[dst] = RAY_PRIMARY;
+      R->term[dst] = 0;
+      R->classification[dst] = 0;
+      dst++;
+    }
+  }
+  return dst;
+}
+
+// Rendering::resolve_lights (Rendering.cpp:157-216): camera-relative lights
+// (type 1) become point lights at viewpoint + x*right + y*up + z*viewdir where
+// right = cross(viewdir, viewup) and up = cross(viewdir, right).
+OEXPORT void oracle_resolve_lights(const float eye[3], const float center[3], const float up[3], int n,
+                                   const float *src, const int *types, float *dst, int *dtypes) {
+  v3 vp = mk(eye[0], eye[1], eye[2]);
+  v3 vu = mk(up[0], up[1], up[2]);
+  v3 vd = mk(center[0] - eye[0], center[1] - eye[1], center[2] - eye[2]);
+  dt_normalize(vu);
+  dt_normalize(vd);
+  v3 right = cross(vd, vu);
+  dt_normalize(right);
+  v3 u2 = cross(vd, right);
+  for (int i = 0; i < n; i++) {
+    v3 l = mk(src[3 * i], src[3 * i + 1], src[3 * i + 2]);
+    if (types[i] == 1) {
+      v3 o = vp + l.x * right + l.y * u2 + l.z * vd;
+      dst[3 * i] = o.x; dst[3 * i + 1] = o.y; dst[3 * i + 2] = o.z;
+      dtypes[i] = 2;
+    } else {
+      dst[3 * i] = l.x; dst[3 * i + 1] = l.y; dst[3 * i + 2] = l.z;
+      dtypes[i] = types[i];
+    }
+  }
+}
+
+// ------------------------------------------------------------------ frame loop
+// The host loop of processRays_task / Renderer::Trace / RayQManager
+// (Renderer.cpp:504-691): every list is traced, spawns a secondary list, is
+// classified, routed and accumulated; keepers loop.  Results do not depend on
+// list order except for framebuffer summation order.
+namespace {
+struct FrameCtx {
+  OVis V;
+  void *lights;
+  float eps;
+  int nb[6];
+  int maxlist;
+  float *fb;
+  int w, h;
+  gxy_frame_stats st;
+};
+
+void process_list(FrameCtx &C, std::unique_ptr<HRayList> rl, std::deque<std::unique_ptr<HRayList>> &queue) {
+  OLights L = lights_of(C.lights);
+  while (rl && rl->n > 0) {
+    int n = rl->n;
+    RayList_ispc &R = rl->r;
+    SideBand sb;
+    long long ns = 0;
+    sb.samples = &ns;
+    std::atomic<long long> tot(0);
+    parallel_for(n, [&](int s, int e) {
+      long long k = 0;
+      for (int i = s; i < e; i++) trace_one<float>(*C.V.ispc, C.V.vv, C.V.model, C.V.step, C.V.eps, C.V.integrate, R, i, sb, k);
+      tot += k;
+    });
+    C.st.samples += tot.load();
+    C.st.ray_passes += n;
+    C.st.passes++;
+    C.st.lists++;
+    std::vector<int> aoo(n), sho(n);
+    int nao, nsh;
+    int nout = oracle_spawn_offsets(n, &R, L.nAO, L.n, L.doShadows, aoo.data(), sho.data(), &nao, &nsh);
+    std::unique_ptr<HRayList> out;
+    if (nout) out.reset(new HRayList(nout, 1));
+    oracle_ambientLighting(C.lights, n, &R);
+    if (nao) oracle_generateAORays(C.lights, n, &R, aoo.data(), &out->r, C.eps);
+    oracle_diffuseLighting(C.lights, n, &R);
+    if (nsh) oracle_generateShadowRays(C.lights, n, &R, sho.data(), &out->r, C.eps);
+    if (out) {
+      C.st.secondary_rays += nout;
+      if (out->n > C.maxlist) {
+        for (int s = 0; s < out->n; s += C.maxlist) {
+          int c = std::min(C.maxlist, out->n - s);
+          std::unique_ptr<HRayList> part(new HRayList(c, 1));
+          for (int i = 0; i < c; i++) copy_ray(out->r, s + i, part->r, i);
+          queue.push_back(std::move(part));
+        }
+      } else queue.push_back(std::move(out));
+    }
+    oracle_classify(n, &R, 1);
+    oracle_assign_destinations(n, &R, &C.V.ispc->local_bb, C.nb, 1);
+    C.st.terminated += oracle_accumulate(n, &R, C.fb, C.w, C.h);
+    int nk = 0;
+    for (int i = 0; i < n; i++) {
+      int c = R.classification[i];
+      if (c == GXY_KEEP_HERE) nk++;
+      else if (c == GXY_DROP_ON_FLOOR) C.st.dropped++;
+      else if (c >= 0) {
+        for (int f = 0; f < 6; f++) if (C.nb[f] == c) { C.st.outbound[f]++; break; }
+      }
+    }
+    std::unique_ptr<HRayList> keep;
+    if (nk) {
+      keep.reset(new HRayList(nk, rl->type));
+      int j = 0;
+      for (int i = 0; i < n; i++)
+        if (R.classification[i] == GXY_KEEP_HERE) copy_ray(R, i, keep->r, j++);
+      C.st.kept += nk;
+    }
+    rl = std::move(keep);
+  }
+}
+}  // namespace
+
+OEXPORT int oracle_process_raylist(void *vis, void *lights, int n, RayList_ispc *rays, const int nb[6], float eps,
+                                   int maxlist, float *fb, int w, int h, gxy_frame_stats *stats) {
+  FrameCtx C{resolve_vis(vis, eps), lights, eps, {0}, maxlist, fb, w, h, gxy_frame_stats()};
+  memset(&C.st, 0, sizeof C.st);
+  for (int k = 0; k < 6; k++) C.nb[k] = nb ? nb[k] : -1;
+  std::deque<std::unique_ptr<HRayList>> q;
+  for (int s = 0; s < n; s += maxlist) {
+    int c = std::min(maxlist, n - s);
+    std::unique_ptr<HRayList> l(new HRayList(c, 0));
+    for (int i = 0; i < c; i++) copy_ray(*rays, s + i, l->r, i);
+    q.push_back(std::move(l));
+  }
+  while (!q.empty()) {
+    std::unique_ptr<HRayList> l = std::move(q.front());
+    q.pop_front();
+    process_list(C, std::move(l), q);
+  }
+  if (stats) *stats = C.st;
+  return 0;
+}
+
+OEXPORT int oracle_render_frame(void *vis, void *lights, const float eye[3], const float center[3], const float up[3],
+                                float aov, int w, int h, const int nb[6], float eps, int maxlist, float *fb,
+                                gxy_frame_stats *stats) {
+  Visualization_ispc *V = (Visualization_ispc *)vis;
+  HRayList prim(w * h, 0);
+  int n = oracle_camera_generate(eye, center, 0, up, aov, w, h, &V->local_bb, &V->global_bb, &prim.r, w * h);
+  if (n < 0) return -1;
+  prim.n = n;
+  int r = oracle_process_raylist(vis, lights, n, &prim.r, nb, eps, maxlist, fb, w, h, stats);
+  if (stats) stats->primary_rays = n;
+  return r;
+}
+
+// Sampler loop: SamplerTraceRays + KEEP_HERE re-trace until every ray exits
+// (Sampler.cpp:116-131, Renderer.cpp:348-351, 640-642); one particle per
+// crossing at o + t*d (Sampler.cpp:71-84).
+OEXPORT long long oracle_sample_raylist(void *vis, int n, RayList_ispc *rays, float *xyz, long long capacity,
+                                        long long *passes) {
+  HRayList cur(n, 0);
+  for (int i = 0; i < n; i++) copy_ray(*rays, i, cur.r, i);
+  long long count = 0, np = 0;
+  while (cur.n > 0) {
+    oracle_SamplerTraceRays(vis, cur.n, &cur.r);
+    np++;
+    int nk = 0;
+    for (int i = 0; i < cur.n; i++)
+      if (cur.r.term[i] & RAY_SURFACE) {
+        if (count < capacity) {
+          xyz[3 * count] = cur.r.ox[i] + cur.r.t[i] * cur.r.dx[i];
+          xyz[3 * count + 1] = cur.r.oy[i] + cur.r.t[i] * cur.r.dy[i];
+          xyz[3 * count + 2] = cur.r.oz[i] + cur.r.t[i] * cur.r.dz[i];
+        }
+        count++;
+        nk++;
+      }
+    HRayList nxt(nk, 0);
+    int j = 0;
+    for (int i = 0; i < cur.n; i++)
+      if (cur.r.term[i] & RAY_SURFACE) copy_ray(cur.r, i, nxt.r, j++);
+    nxt.n = nk;
+    std::swap(cur.f, nxt.f);
+    std::swap(cur.iv, nxt.iv);
+    std::swap(cur.r, nxt.r);
+    cur.n = nk;
+    cur.aligned = nxt.aligned;
+  }
+  if (passes) *passes = np;
+  return count;
+}
