@@ -1,0 +1,257 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X ray-processing path.
+
+Headline workload: one frame of the volume render,
+1920x1080 primary rays through a 512^3 FP32 noise volume (DVR with an 8-point
+TF, 0.5-voxel step, early termination at alpha 0.99, isosurface 0.5) plus the
+shadow rays the isosurface hits spawn, classified and accumulated into the
+framebuffer.  A step is one whole frame: camera rays, every trace pass, spawn,
+lighting, routing and accumulation, all on the GPU with the volume resident in
+HBM.  value = ray passes (primary + shadow) / second; Gsamples/s beside it.
+
+Also measured, as extra fields: the sphere-mesh frame and the
+nine-icosphere AO frame.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
+FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 (vector and matrix)
+BYTES_PER_RAY_PASS = 104     # 52 B read + 52 B written per traced ray
+FLOPS_PER_SAMPLE = 36        # trilinear sample incl. ray point and grid transform
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_time(L, name):
+    ms, n = C.c_double(0), C.c_int(0)
+    L.gxy_timing_get(name.encode(), C.byref(ms), C.byref(n))
+    return ms.value, n.value
+
+
+def run_frames(L, hip, built, cam, steps, warmup, eps=0.001):
+    from galaxy_amd import abi, lib as glib
+    w, h = cam.width, cam.height
+    fb = glib.DeviceArray(w * h * 16)
+    a3 = glib.a3
+    eye, cen, up = a3(cam.eye), a3(cam.center), a3(cam.up)
+    nb = np.full(6, -1, np.int32)
+    st = abi.FrameStats()
+
+    def frame():
+        glib.check(L.gxy_memset(fb.ptr, 0, fb.nbytes), "memset")
+        glib.check(L.gxy_render_frame(C.byref(built.vis), C.byref(built.lights), eye.ctypes.data, cen.ctypes.data,
+                                      up.ctypes.data, cam.aov, w, h, nb.ctypes.data, eps, 1 << 30, fb.ptr,
+                                      C.byref(st)), "render_frame")
+
+    for _ in range(warmup):
+        frame()
+    glib.check(L.gxy_synchronize())
+    L.gxy_timing_reset()
+    L.gxy_timing_enable(1)
+    totals = dict(ray_passes=0, samples=0, primary=0, secondary=0, trace_launches=0)
+    glib.check(L.gxy_synchronize())
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        frame()
+        totals["ray_passes"] += st.ray_passes
+        totals["samples"] += st.samples
+        totals["primary"] += st.primary_rays
+        totals["secondary"] += st.secondary_rays
+    glib.check(L.gxy_synchronize())
+    t1 = time.perf_counter()
+    L.gxy_timing_enable(0)
+    kt = {k: kernel_time(L, k) for k in ("trace", "lighting", "route", "partition", "spawn_scan", "camera")}
+    image = fb.to_host(np.float32)
+    fb.free()
+    return (t1 - t0), totals, kt, st, image
+
+
+def cpu_baseline(hip, sc, cam, alpha, target_s=15.0):
+    """The CPU oracle (C++ restatement, std::thread on the host cores) on a
+    bounded sample of the same frame: primary rays of every k-th image row,
+    traced, lit, spawned, classified and accumulated like the GPU frame."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_backend import OracleBackend, lib as olib
+    from galaxy_amd import abi, scenes
+    threads = min(16, os.cpu_count() or 1)
+    olib().oracle_set_threads(threads)
+    olib().oracle_set_opacity_threshold(alpha)
+    ob = OracleBackend()
+    vv0 = sc.volume_vis[0]
+    vol = vv0.volume
+    if vol.voxels is None:
+        # the same voxels the GPU rendered (generated on the device, copied here)
+        hv = hip.volume_create(vol.dims, vol.origin, vol.spacing, None, vol.sampling_rate)
+        hip.volume_fill_noise(hv, *vol.noise)
+        vox = hip.volume_download(hv, vol.dims)
+        hip.L.gxy_volume_destroy(hv)
+        vol.voxels = vox
+    Bo = scenes.build_scene(ob, sc)
+    rays = ob.camera_rays(cam, sc.local_box, sc.global_box)
+    nb = np.full(6, -1, np.int32)
+
+    def run(stride):
+        idx = np.nonzero(rays["y"] % stride == 0)[0]
+        sub = rays.subset(idx)
+        fb = np.zeros(cam.width * cam.height * 4, np.float32)
+        st = abi.FrameStats()
+        t = time.perf_counter()
+        ob.L.oracle_process_raylist(C.byref(Bo.vis), C.byref(Bo.lights), sub.n, C.byref(sub.ispc), nb.ctypes.data,
+                                    sc.epsilon, 1 << 30, fb.ctypes.data, cam.width, cam.height, C.byref(st))
+        return time.perf_counter() - t, st, sub.n
+
+    dt, st, n = run(64)
+    stride = 64
+    if dt > 0:
+        want = int(64 * dt / target_s)
+        stride = max(1, min(64, want if want > 0 else 1))
+    if stride != 64:
+        dt, st, n = run(stride)
+    vol.voxels = None
+    return dict(value=round(st.ray_passes / dt / 1e6, 4), unit="Mrays/s", cores=threads, kind="port",
+                gsamples_per_s=round(st.samples / dt / 1e9, 5),
+                sample="primary rays of every %d-th image row (%d primary, %d ray passes, %.1f s), same scene and "
+                       "seeds, host CPU %s" % (stride, n, st.ray_passes, dt, platform.processor() or "x86_64"))
+
+
+def traffic_from_profile(name):
+    """HBM bytes per trace launch measured by rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE, copy-kernel calibrated) and committed under profiles/."""
+    p = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("traffic_bytes_per_launch"):
+            try:
+                return float(line.split("=")[1].split()[0])
+            except (IndexError, ValueError):
+                return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--c3-steps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs one partition on one MI355X; "
+              "--gpus %d is not supported (use --gpus 1)" % args.gpus, flush=True)
+        sys.exit(2)
+
+    from galaxy_amd import lib as glib, scenes, workloads
+    L = glib.lib()
+    hip = glib.HipBackend()
+
+    # ---------------------------------------------------------------- c2
+    sc, alpha = workloads.c2()
+    cam = sc.cameras[0]
+    glib.check(L.gxy_set_opacity_threshold(alpha))
+    log("building c2 (512^3 noise volume)")
+    built = scenes.build_scene(hip, sc)
+    dt, tot, kt, st, image = run_frames(L, hip, built, cam, args.steps, args.warmup, sc.epsilon)
+    ms_step = dt / args.steps * 1e3
+    mrays = tot["ray_passes"] / dt / 1e6
+    gsamp = tot["samples"] / dt / 1e9
+    tr_ms, tr_n = kt["trace"]
+    # algorithmic work of the trace launches: 104 B per ray pass + the volume once per launch
+    voxels = int(np.prod(np.array(sc.volume_vis[0].volume.dims, dtype=np.int64)))
+    alg_bytes = BYTES_PER_RAY_PASS * tot["ray_passes"] + 4 * voxels * tr_n
+    alg_flops = FLOPS_PER_SAMPLE * tot["samples"]
+    gbs = alg_bytes / (tr_ms * 1e-3) / 1e9 if tr_ms > 0 else 0.0
+    tfl = alg_flops / (tr_ms * 1e-3) / 1e12 if tr_ms > 0 else 0.0
+    if gbs / HBM_PEAK_GBS >= tfl / FP32_PEAK_TFLOPS:
+        roof = dict(bound="hbm", achieved=round(gbs, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(gbs / HBM_PEAK_GBS, 5))
+    else:
+        roof = dict(bound="mfma", achieved=round(tfl, 4), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+                    frac=round(tfl / FP32_PEAK_TFLOPS, 5),
+                    note="FP32 VALU work (no MFMA on this path); MI355X FP32 peak")
+    roof["traffic"] = traffic_from_profile("trace_c2_traffic.txt")
+    roof["kernel"] = "k_trace<1,false>"
+    roof["kernel_ms_per_launch"] = round(tr_ms / max(1, tr_n), 4)
+    roof["alg_bytes_per_launch"] = int(alg_bytes / max(1, tr_n))
+    roof["alg_flops_per_launch"] = int(alg_flops / max(1, tr_n))
+
+    out = {
+        "metric": "Mrays/s + volume Gsamples/s on 1 MI355X; p50/p99 request latency",
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded 4-octave gradient noise generated on the device)",
+        "config": {"workload": "volume render: 1920x1080 rays, 512^3 FP32 noise, 8-point RGBA TF, step 0.5 voxel, "
+                               "early termination alpha 0.99, iso 0.5 + shadow rays",
+                   "rays_per_frame": tot["ray_passes"] // args.steps,
+                   "primary_per_frame": tot["primary"] // args.steps,
+                   "shadow_per_frame": tot["secondary"] // args.steps},
+        "gsamples_per_s": round(gsamp, 3),
+        "roofline": roof,
+        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt.items()},
+    }
+    built = None
+
+    # ---------------------------------------------------------------- extras
+    if not args.no_extra:
+        extra = {}
+        for name, fn, steps in (("c1_sphere_mesh_512", workloads.c1, args.steps),
+                                ("c3_nine_icospheres_4k_ao16", workloads.c3, args.c3_steps)):
+            try:
+                sc2, a2 = fn()
+                glib.check(L.gxy_set_opacity_threshold(a2))
+                log("building", name)
+                t = time.perf_counter()
+                b2 = scenes.build_scene(hip, sc2)
+                build_s = time.perf_counter() - t
+                dt2, tot2, kt2, st2, _ = run_frames(L, hip, b2, sc2.cameras[0], steps, 1, sc2.epsilon)
+                extra[name] = dict(mrays_per_s=round(tot2["ray_passes"] / dt2 / 1e6, 2),
+                                   ms_per_frame=round(dt2 / steps * 1e3, 3),
+                                   rays_per_frame=tot2["ray_passes"] // steps,
+                                   primary=tot2["primary"] // steps, secondary=tot2["secondary"] // steps,
+                                   scene_build_s=round(build_s, 3),
+                                   trace_ms_per_frame=round(kt2["trace"][0] / steps, 3))
+                b2 = None
+            except Exception as e:  # reported, not hidden
+                extra[name] = dict(error=str(e))
+        out["extra_workloads"] = extra
+        glib.check(L.gxy_set_opacity_threshold(alpha))
+
+    # ---------------------------------------------------------------- CPU baseline
+    if not args.no_cpu:
+        try:
+            log("cpu baseline")
+            sc, alpha = workloads.c2()
+            out["cpu_baseline"] = cpu_baseline(hip, sc, sc.cameras[0], alpha)
+        except Exception as e:
+            out["cpu_baseline"] = dict(error=str(e))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

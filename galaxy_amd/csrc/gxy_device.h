@@ -105,9 +105,11 @@ struct TraceParams {
   int pad_;
   float blo[3], bhi[3];
   float step, eps;
+  float opaque_alpha;  // early termination: alpha above this is opaque (0.999)
+  int pad2_;
   DVolVis vv[GXY_MAX_VOLVIS];
   DModel model;
-  unsigned long long *sample_counter;  // may be null
+  unsigned long long *sample_partials; // per-workgroup sample counts (may be null)
   int *hit_geom;                       // side-band geomID per ray, may be null
   int *hit_prim;
 };

@@ -27,7 +27,6 @@ struct RouteParams {
   float *fb;
   int width, height;
   int mode;
-  unsigned long long *counters;  // [0] terminated accumulated, [1] dropped (may be null)
 };
 
 struct CameraParams {
@@ -40,6 +39,7 @@ struct CameraParams {
 
 hipError_t launch_trace(const TraceParams &P, const RayList_ispc &R, int n, hipStream_t s);
 hipError_t launch_sampler(const TraceParams &P, const RayList_ispc &R, int n, hipStream_t s);
+hipError_t launch_reduce_add(const unsigned long long *part, int nb, unsigned long long *acc, hipStream_t s);
 
 // Exclusive offsets of primary surface rays (flag scan).  block_counts needs
 // ceil(n/256)+1 ints of scratch; total (device int) receives the count.

@@ -33,7 +33,6 @@ __device__ __forceinline__ int exit_face(const float *lo, const float *hi, float
 
 __global__ __launch_bounds__(GXY_BLOCK) void k_route(RouteParams P, RayList_ispc R, int n) {
   int i = blockIdx.x * GXY_BLOCK + threadIdx.x;
-  unsigned long long nterm = 0, ndrop = 0;
   if (i < n) {
     const int typ = R.type[i];
     int c = R.classification[i];
@@ -69,16 +68,7 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_route(RouteParams P, RayList_ispc
         atomicAdd(p + 1, R.g[i]);
         atomicAdd(p + 2, R.b[i]);
         atomicAdd(p + 3, R.o[i]);
-        nterm = 1;
       }
-    }
-    if (c == GXY_DROP_ON_FLOOR) ndrop = 1;
-  }
-  if (P.counters) {
-    unsigned long long a = wave_sum_u64(nterm), b = wave_sum_u64(ndrop);
-    if ((threadIdx.x & 63) == 0) {
-      if (a) atomicAdd(P.counters, a);
-      if (b) atomicAdd(P.counters + 1, b);
     }
   }
 }

@@ -74,6 +74,8 @@ struct Runtime {
   // registries for handle validation
   std::set<void *> volumes, tfs, models, raylists, tracers, lightings;
   std::atomic<int> raylist_id{0};
+  float opaque_alpha = 0.999f;
+  std::vector<void *> outbound[6];   // RayListObj* per exit face, last frames
 };
 static Runtime G;
 
@@ -385,7 +387,8 @@ static int make_trace_params(const Visualization_ispc *vis, float global_eps, Tr
     P.hit_geom = M->hit_geom;
     P.hit_prim = M->hit_prim;
   }
-  P.sample_counter = G.counters;
+  P.sample_partials = nullptr;
+  P.opaque_alpha = G.opaque_alpha;
   return 0;
 }
 
@@ -474,6 +477,12 @@ GXY_EXPORT int gxy_memset(void *dst, int value, size_t bytes) {
   return 0;
 }
 GXY_EXPORT size_t gxy_device_bytes_in_use(void) { return G.bytes_live; }
+GXY_EXPORT int gxy_set_opacity_threshold(float alpha) {
+  if (!(alpha > 0.f && alpha <= 1.f)) return fail("opacity threshold %g outside (0, 1]", alpha);
+  G.opaque_alpha = alpha;
+  return 0;
+}
+GXY_EXPORT float gxy_get_opacity_threshold(void) { return G.opaque_alpha; }
 GXY_EXPORT size_t gxy_device_bytes_total(void) {
   if (ensure_init() != 0) return 0;
   size_t fr = 0, tot = 0;
@@ -1095,6 +1104,23 @@ GXY_EXPORT int gxy_model_set_hit_output(void *model, int *g, int *p) {
   return 0;
 }
 
+// Trace (or sampler) launch with the per-workgroup sample counts reduced into
+// the runtime's sample counter (counters[0]).
+static int trace_launch(TraceParams P, const RayList_ispc &R, int n, bool sampler) {
+  if (n <= 0) return 0;
+  int nb = (n + GXY_BLOCK - 1) / GXY_BLOCK;
+  unsigned long long *part = (unsigned long long *)dalloc(sizeof(unsigned long long) * nb);
+  if (!part) return -1;
+  P.sample_partials = part;
+  {
+    KTimer kt(sampler ? "sampler" : "trace");
+    HIPCHK(sampler ? launch_sampler(P, R, n, S()) : launch_trace(P, R, n, S()));
+  }
+  HIPCHK(launch_reduce_add(part, nb, G.counters, S()));
+  dfree_stream_ordered(part);
+  return 0;
+}
+
 // ------------------------------------------------------------------ TraceRays_* (reference ABI)
 struct TraceRaysObj {
   int debug[100];
@@ -1124,9 +1150,7 @@ GXY_EXPORT void *TraceRays_TraceRays(void *self, void *vis, int nRaysIn, void *r
   if (ensure_init() != 0) return nullptr;
   TraceParams P;
   if (make_trace_params((const Visualization_ispc *)vis, epsilon, P) != 0) return nullptr;
-  KTimer kt("trace");
-  hipError_t e = launch_trace(P, *(const RayList_ispc *)raysIn, nRaysIn, S());
-  if (e != hipSuccess) fail("trace launch: %s", hipGetErrorString(e));
+  trace_launch(P, *(const RayList_ispc *)raysIn, nRaysIn, false);
   return nullptr;
 }
 
@@ -1172,9 +1196,7 @@ GXY_EXPORT void *SamplerTraceRays_SamplerTraceRays(void *self, void *vis, int n,
   // SamplerTraceRays.ispc:139-150: step = min samplingStep*samplingRate
   for (int k = 0; k < P.nvv; k++)
     if (P.vv[k].nIso == 0) P.vv[k].iso[0] = 0.f;  // IsoSamplerVis_initialize default
-  KTimer kt("sampler");
-  hipError_t e = launch_sampler(P, *(const RayList_ispc *)raysIn, n, S());
-  if (e != hipSuccess) fail("sampler launch: %s", hipGetErrorString(e));
+  trace_launch(P, *(const RayList_ispc *)raysIn, n, true);
   return nullptr;
 }
 
@@ -1321,7 +1343,6 @@ static int route(int n, const RayList_ispc &R, int mode, int reverse, const gxy_
   P.width = w;
   P.height = h;
   P.mode = mode;
-  P.counters = G.counters + 1;
   KTimer kt("route");
   HIPCHK(launch_route(P, R, n, S()));
   return 0;
@@ -1527,10 +1548,7 @@ static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
   while (L && L->hdr.size > 0) {
     const int n = L->hdr.size;
     const RayList_ispc &R = L->ispc;
-    {
-      KTimer kt(C.sampler ? "sampler" : "trace");
-      HIPCHK(C.sampler ? launch_sampler(C.P, R, n, S()) : launch_trace(C.P, R, n, S()));
-    }
+    if (trace_launch(C.P, R, n, C.sampler) != 0) return -1;
     C.st.ray_passes += n;
     C.st.passes++;
     C.st.lists++;
@@ -1573,17 +1591,20 @@ static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
       KTimer kt("sampler_collect");
       HIPCHK(launch_accumulate_sampler(R, n, C.xyz, C.capacity, C.dcount, scr, S()));
     }
-    // split by destination: keepers + one list per neighbour face (Renderer.cpp:561-634)
+    // split by destination: keepers + one list per neighbour face
+    // (Renderer.cpp:561-634); terminated and dropped rays are counted only
     int table[64];
     for (int k = 0; k < 64; k++) table[k] = -1;
     table[GXY_KEEP_HERE + 4] = 0;
+    table[GXY_TERMINATED + 4] = 7;
+    table[GXY_DROP_ON_FLOOR + 4] = 8;
     int ntable = 4;
     for (int f = 0; f < 6; f++)
       if (C.nb[f] >= 0 && C.nb[f] + 4 < 64) {
         table[C.nb[f] + 4] = 1 + f;
         ntable = std::max(ntable, C.nb[f] + 5);
       }
-    const int K = 7;
+    const int K = 9, KL = 7;
     Scratch sc;
     int nbk = (n + GXY_BLOCK - 1) / GXY_BLOCK;
     int *scr = (int *)sc.get(sizeof(int) * ((size_t)K * nbk + 1));
@@ -1596,38 +1617,61 @@ static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
     }
     HIPCHK(hipMemcpyAsync(counts, dcounts, sizeof counts, hipMemcpyDeviceToHost, S()));
     HIPCHK(hipStreamSynchronize(S()));
-    for (int f = 0; f < 6; f++) C.st.outbound[f] += counts[1 + f];
+    C.st.terminated += counts[7];
+    C.st.dropped += counts[8];
     RayListObj *keep = nullptr;
+    RayList_ispc lists[KL];
+    memset(lists, 0, sizeof lists);
+    bool any = false;
     if (counts[0] > 0) {
       keep = raylist_new(counts[0], L->hdr.frame, L->hdr.type);
       if (!keep) return -1;
-      RayList_ispc lists[K];
-      memset(lists, 0, sizeof lists);
       lists[0] = keep->ispc;
-      KTimer kt("partition");
-      HIPCHK(launch_partition_scatter(R, n, table, ntable, 1, lists, scr, S()));
       C.st.kept += counts[0];
+      any = true;
     }
-    // outbound lists would be sent to the neighbour ranks here (SendRays,
-    // Renderer.cpp:620-634); on one GPU their counts are the outbound queue.
+    // outbound queue: one list per face with a neighbour (SendRays,
+    // Renderer.cpp:620-634); kept for gxy_outbound_take
+    for (int f = 0; f < 6; f++) {
+      C.st.outbound[f] += counts[1 + f];
+      if (counts[1 + f] > 0) {
+        RayListObj *ob = raylist_new(counts[1 + f], L->hdr.frame, L->hdr.type);
+        if (!ob) return -1;
+        lists[1 + f] = ob->ispc;
+        std::lock_guard<std::recursive_mutex> lk(G.mu);
+        G.outbound[f].push_back(ob);
+        any = true;
+      }
+    }
+    if (any) {
+      KTimer kt("partition");
+      HIPCHK(launch_partition_scatter(R, n, table, ntable, KL, lists, scr, S()));
+    }
     if (owned) raylist_delete(L, false);
     L = keep;
     owned = true;
   }
+  if (L && owned) raylist_delete(L, false);
   (void)prio;
   return 0;
 }
 
-static int run_loop(LoopCtx &C, int n, const RayList_ispc &rays) {
-  // initial lists, at most maxlist rays each (primary priority)
-  for (int s = 0; s < n; s += C.maxlist) {
-    int c = std::min(C.maxlist, n - s);
-    RayListObj *p = raylist_new(c, 0, 0);
-    if (!p) return -1;
-    HIPCHK(launch_copy_rays(rays, s, c, p->ispc, 0, S()));
-    C.q.push_back({p, 3});
-  }
+static int run_loop(LoopCtx &C, int n, const RayList_ispc &rays, RayListObj *adopt = nullptr) {
   HIPCHK(hipMemsetAsync(G.counters, 0, 8 * sizeof(unsigned long long), S()));
+  if (adopt && n <= C.maxlist) {
+    // the caller's list is processed in place (it is the loop's to consume)
+    C.q.push_back({adopt, 3});
+  } else {
+    // initial lists, at most maxlist rays each (primary priority)
+    for (int s = 0; s < n; s += C.maxlist) {
+      int c = std::min(C.maxlist, n - s);
+      RayListObj *p = raylist_new(c, 0, 0);
+      if (!p) return -1;
+      HIPCHK(launch_copy_rays(rays, s, c, p->ispc, 0, S()));
+      C.q.push_back({p, 3});
+    }
+    if (adopt) raylist_delete(adopt, false);
+  }
   while (!C.q.empty()) {
     QItem it = pop_next(C.q);
     if (process_list(C, it.rl, it.prio, true) != 0) {
@@ -1640,8 +1684,6 @@ static int run_loop(LoopCtx &C, int n, const RayList_ispc &rays) {
   HIPCHK(hipMemcpyAsync(cnt, G.counters, sizeof cnt, hipMemcpyDeviceToHost, S()));
   HIPCHK(hipStreamSynchronize(S()));
   C.st.samples = (long long)cnt[0];
-  C.st.terminated = (long long)cnt[1];
-  C.st.dropped = (long long)cnt[2];
   return 0;
 }
 
@@ -1660,6 +1702,33 @@ static int loop_setup(LoopCtx &C, void *vis, void *lights, const int nb[6], floa
   return 0;
 }
 }  // namespace gxy
+
+GXY_EXPORT int gxy_outbound_count(int face) {
+  if (face < 0 || face > 5) return fail("face %d outside 0..5", face);
+  std::lock_guard<std::recursive_mutex> lk(G.mu);
+  return (int)G.outbound[face].size();
+}
+GXY_EXPORT void *gxy_outbound_take(int face, RayList_ispc *ispc, int *n) {
+  if (face < 0 || face > 5) {
+    fail("face %d outside 0..5", face);
+    return nullptr;
+  }
+  std::lock_guard<std::recursive_mutex> lk(G.mu);
+  if (G.outbound[face].empty()) return nullptr;
+  RayListObj *r = (RayListObj *)G.outbound[face].front();
+  G.outbound[face].erase(G.outbound[face].begin());
+  if (ispc) *ispc = r->ispc;
+  if (n) *n = r->hdr.size;
+  return r;
+}
+GXY_EXPORT int gxy_outbound_clear(void) {
+  std::lock_guard<std::recursive_mutex> lk(G.mu);
+  for (int f = 0; f < 6; f++) {
+    for (void *r : G.outbound[f]) raylist_delete((RayListObj *)r);
+    G.outbound[f].clear();
+  }
+  return 0;
+}
 
 GXY_EXPORT int gxy_process_raylist(void *vis, void *lights, int n, RayList_ispc *rays, const int nb[6], float eps,
                                    int maxlist, float *fb, int w, int h, gxy_frame_stats *stats) {
@@ -1696,8 +1765,9 @@ GXY_EXPORT int gxy_render_frame(void *vis, void *lights, const float eye[3], con
   C.fb = fb;
   C.w = w;
   C.h = h;
-  int r = run_loop(C, np, prim->ispc);
-  raylist_delete(prim);
+  // Truncate (Rays.cpp:245-286) keeps the block; the loop consumes the list
+  prim->hdr.size = np;
+  int r = run_loop(C, np, prim->ispc, prim);
   C.st.primary_rays = np;
   if (stats) *stats = C.st;
   return r;

@@ -250,9 +250,45 @@ __device__ void geom_shade(const DModel &M, const GeoHit &gh, f3 org, f3 dir, Hi
   hit.t = gh.t;
 }
 
+// Workgroup total of a per-item count written to out[blockIdx.x] (no atomics:
+// same-address atomics from every wave serialise in one L2 channel).
+__device__ __forceinline__ void block_sum_store(unsigned long long v, unsigned long long *out) {
+  __shared__ unsigned long long ws[GXY_BLOCK / 64];
+  v = wave_sum_u64(v);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < GXY_BLOCK / 64; w++) t += ws[w];
+    out[blockIdx.x] = t;
+  }
+}
+
+// adds the nb workgroup partials into *acc (one workgroup)
+__global__ __launch_bounds__(1024) void k_reduce_add(const unsigned long long *part, int nb, unsigned long long *acc) {
+  __shared__ unsigned long long ws[16];
+  unsigned long long v = 0;
+  for (int k = threadIdx.x; k < nb; k += 1024) v += part[k];
+  v = wave_sum_u64(v);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < 16; w++) t += ws[w];
+    *acc += t;
+  }
+}
+
+hipError_t launch_reduce_add(const unsigned long long *part, int nb, unsigned long long *acc, hipStream_t s) {
+  if (nb <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_reduce_add, dim3(1), dim3(1024), 0, s, part, nb, acc);
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------------- the trace kernel
-__device__ __forceinline__ bool opaque_of(float r, float g, float b, float a) {
-  return mnf(mnf(r, g), b) >= 1.0f || a > 0.999f;
+// TraceRays.ispc:466,550,581 with the alpha threshold as a parameter
+__device__ __forceinline__ bool opaque_of(float r, float g, float b, float a, float thr) {
+  return mnf(mnf(r, g), b) >= 1.0f || a > thr;
 }
 
 template <int NV, bool GEOM>
@@ -348,7 +384,8 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_trace(TraceParams P, RayList_ispc
       for (int v = 0; v < NVS; v++) { sLast[v] = 0.f; sThis[v] = 0.f; }
       float tLast = tEntry + epsilon;
       bool hit_iso = false;
-      bool opaque = opaque_of(cr, cg, cb, ca);
+      int iso_vid = -1;
+      bool opaque = opaque_of(cr, cg, cb, ca, P.opaque_alpha);
       float tThis = tEntry;
       int iter = 0;
       while (tThis <= tTermination && !opaque && !hit_iso && iter++ < GXY_MAX_MARCH) {
@@ -375,18 +412,9 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_trace(TraceParams P, RayList_ispc
             }
           }
           if (ih) {
-            hit.point = add3(org, scl3(hit.t, dir));
-            if (shadeFlag) {
-#pragma unroll
-              for (int major = 0; major < NV; major++)
-                if (major == ivid) {
-                  hit.normal = safe_normalize3(vol_gradient(P.vv[major].vol, hit.point));
-                  if (dot3(dir, hit.normal) > 0) hit.normal = neg3(hit.normal);
-                  hit.color = tf_color(s_col[major], P.vv[major].tf.nc, P.vv[major].tf.lo, P.vv[major].tf.hi,
-                                       hit.sample);
-                  hit.opacity = 1.0f;
-                }
-            }
+            // the shading of the hit (gradient normal, TF colour) reads nothing
+            // the rest of the march changes, so it is done once after the loop
+            iso_vid = ivid;
             tTermination = hit.t;
             tThis = hit.t;
             surface_hit = true;
@@ -422,7 +450,7 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_trace(TraceParams P, RayList_ispc
         }
 #pragma unroll
         for (int v = 0; v < NV; v++) sLast[v] = sThis[v];
-        opaque = opaque_of(cr, cg, cb, ca);
+        opaque = opaque_of(cr, cg, cb, ca, P.opaque_alpha);
         if (opaque) tTermination = tThis;
         tLast = tThis;
         float nxt = (tThis == tEntry) ? (tEntry + epsilon)
@@ -432,10 +460,25 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_trace(TraceParams P, RayList_ispc
         tThis = nxt;
       }
       ray_t = tTermination;
+      if (hit_iso) {
+        // LookForIsoHit shading (TraceRays.ispc:284-307)
+        hit.point = add3(org, scl3(hit.t, dir));
+        if (shadeFlag) {
+#pragma unroll
+          for (int major = 0; major < NV; major++)
+            if (major == iso_vid) {
+              hit.normal = safe_normalize3(vol_gradient(P.vv[major].vol, hit.point));
+              if (dot3(dir, hit.normal) > 0) hit.normal = neg3(hit.normal);
+              hit.color = tf_color(s_col[major], P.vv[major].tf.nc, P.vv[major].tf.lo, P.vv[major].tf.hi,
+                                   hit.sample);
+              hit.opacity = 1.0f;
+            }
+        }
+      }
     }
 
     R.r[i] = cr; R.g[i] = cg; R.b[i] = cb; R.o[i] = ca;
-    int term = opaque_of(cr, cg, cb, ca) ? RAY_OPAQUE : 0;
+    int term = opaque_of(cr, cg, cb, ca, P.opaque_alpha) ? RAY_OPAQUE : 0;
     R.t[i] = ray_t;
     if (surface_hit) {
       term |= RAY_SURFACE;
@@ -449,10 +492,7 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_trace(TraceParams P, RayList_ispc
     }
     R.term[i] = term;
   }
-  if (P.sample_counter) {
-    unsigned long long s = wave_sum_u64(nsamp);
-    if ((threadIdx.x & 63) == 0 && s) atomicAdd(P.sample_counter, s);
-  }
+  if (P.sample_partials) block_sum_store(nsamp, P.sample_partials);
 }
 
 // ------------------------------------------------------------- sampler kernel
@@ -508,10 +548,7 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_sampler(TraceParams P, RayList_is
     R.t[i] = hit != -1 ? tThis + 0.001f : tThis;
     R.term[i] = hit != -1 ? RAY_SURFACE : RAY_BOUNDARY;
   }
-  if (P.sample_counter) {
-    unsigned long long s = wave_sum_u64(nsamp);
-    if ((threadIdx.x & 63) == 0 && s) atomicAdd(P.sample_counter, s);
-  }
+  if (P.sample_partials) block_sum_store(nsamp, P.sample_partials);
 }
 
 // ------------------------------------------------------------- launchers

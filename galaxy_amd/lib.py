@@ -35,6 +35,8 @@ def _declare(L):
         "gxy_memset": (C.c_int, [vp, C.c_int, C.c_size_t]),
         "gxy_device_bytes_in_use": (C.c_size_t, []),
         "gxy_device_bytes_total": (C.c_size_t, []),
+        "gxy_set_opacity_threshold": (C.c_int, [C.c_float]),
+        "gxy_get_opacity_threshold": (C.c_float, []),
         "gxy_timing_enable": (C.c_int, [C.c_int]),
         "gxy_timing_reset": (C.c_int, []),
         "gxy_timing_get": (C.c_int, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
@@ -77,6 +79,9 @@ def _declare(L):
         "gxy_process_raylist": (C.c_int, [vp, vp, C.c_int, vp, vp, C.c_float, C.c_int, vp, C.c_int, C.c_int,
                                           vp]),
         "gxy_sample_raylist": (C.c_int, [vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, vp]),
+        "gxy_outbound_count": (C.c_int, [C.c_int]),
+        "gxy_outbound_take": (vp, [C.c_int, vp, vp]),
+        "gxy_outbound_clear": (C.c_int, []),
         "TraceRays_allocate": (vp, []),
         "TraceRays_initialize": (None, [vp]),
         "TraceRays_destroy": (None, [vp]),
@@ -235,6 +240,14 @@ class HipBackend:
         return checkp(self.L.gxy_volume_create(d.ctypes.data, o.ctypes.data, s.ctypes.data,
                                                None if voxels is None else voxels.ctypes.data, float(rate)),
                       "volume_create")
+
+    def volume_fill_noise(self, h, seed, octaves, freq):
+        return check(self.L.gxy_volume_fill_noise(h, int(seed), int(octaves), float(freq)), "fill_noise")
+
+    def volume_download(self, h, dims):
+        out = np.empty(int(dims[0]) * int(dims[1]) * int(dims[2]), np.float32)
+        check(self.L.gxy_volume_download(h, out.ctypes.data), "volume_download")
+        return out.reshape(dims[2], dims[1], dims[0])
 
     def tf_create_from_maps(self, cmap, omap, rng):
         cmap, omap = a3(cmap), a3(omap)

@@ -27,16 +27,20 @@ f32 = np.float32
 # --------------------------------------------------------------------- volumes
 @dataclass
 class VolumeDesc:
-    voxels: np.ndarray                 # float32, shape (nz, ny, nx), x fastest
+    voxels: Optional[np.ndarray]       # float32, shape (nz, ny, nx), x fastest
     origin: tuple                      # ghosted brick origin (OSPRay gridOrigin)
     spacing: tuple                     # OSPRay gridSpacing
     sampling_rate: float = 1.0         # OSPRay samplingRate (OsprayVolume.cpp:45)
     local_box: Optional[tuple] = None  # (lo, hi) of the partition without ghosts
     global_box: Optional[tuple] = None
     neighbors: tuple = (-1, -1, -1, -1, -1, -1)
+    shape: Optional[tuple] = None      # (nx, ny, nz) when voxels is None
+    noise: Optional[tuple] = None      # (seed, octaves, base_frequency): filled on the device
 
     @property
     def dims(self):
+        if self.voxels is None:
+            return tuple(self.shape)
         nz, ny, nx = self.voxels.shape
         return (nx, ny, nz)
 
@@ -398,11 +402,19 @@ def build_scene(be, sc: SceneDesc, lights: Optional[LightingDesc] = None):
     .lights (Lighting_ispc)."""
     B = Built()
     vvs = []
+    made = {}
     for vv in sc.volume_vis:
         vol = vv.volume
-        vox = np.ascontiguousarray(vol.voxels, f32)
-        B.keep.append(vox)
-        hv = be.volume_create(vol.dims, vol.origin, vol.spacing, vox, vol.sampling_rate)
+        if id(vol) in made:
+            hv = made[id(vol)]
+        elif vol.voxels is None:
+            hv = be.volume_create(vol.dims, vol.origin, vol.spacing, None, vol.sampling_rate)
+            be.volume_fill_noise(hv, *vol.noise)
+        else:
+            vox = np.ascontiguousarray(vol.voxels, f32)
+            B.keep.append(vox)
+            hv = be.volume_create(vol.dims, vol.origin, vol.spacing, vox, vol.sampling_rate)
+        made[id(vol)] = hv
         cm = np.ascontiguousarray(vv.colormap, f32)
         om = np.ascontiguousarray(vv.opacitymap, f32)
         rng = None if vv.data_range is None else np.array(vv.data_range, f32)

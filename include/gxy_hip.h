@@ -162,6 +162,11 @@ int         gxy_copy_to_host(void *dst, const void *src, size_t bytes);
 int         gxy_copy_device(void *dst, const void *src, size_t bytes);
 int         gxy_memset(void *dst, int value, size_t bytes);
 size_t      gxy_device_bytes_in_use(void);
+/* Early ray termination: TraceRays.ispc:466,550,581 treat a ray as opaque when
+ * min(r,g,b) >= 1 or alpha > 0.999.  The alpha threshold is a process-wide
+ * setting here (default 0.999, the reference value). */
+int         gxy_set_opacity_threshold(float alpha);
+float       gxy_get_opacity_threshold(void);
 size_t      gxy_device_bytes_total(void);
 
 /* Kernel timing: every launch records HIP events on its stream when enabled.
@@ -292,12 +297,20 @@ typedef struct gxy_frame_stats {
  * keepers and secondaries re-enqueued (priority: primary lists before
  * secondary ones, LIFO within a level, as Threading.cpp:176-233).  Lists larger
  * than max_rays_per_list are split (Renderer.cpp:675-690).  neighbors: 6 ranks
- * or -1; outbound rays are counted per face and their lists kept until the next
- * call of gxy_frame_outbound.  epsilon: Renderer epsilon (0.001 default). */
+ * or -1; outbound rays are counted per face and their lists queued for
+ * gxy_outbound_take.  epsilon: Renderer epsilon (0.001 default). */
 int gxy_render_frame(void *vis, void *lights, const float eye[3], const float center[3],
                      const float up[3], float aov, int width, int height,
                      const int neighbors[6], float epsilon, int max_rays_per_list,
                      float *fb, gxy_frame_stats *stats);
+
+/* Outbound queue: the rays each pass routed to a neighbour (exit face f with
+ * neighbors[f] >= 0), one device RayList per pass and face, in pass order.
+ * gxy_outbound_take hands ownership of the next list to the caller (free with
+ * gxy_raylist_destroy); gxy_outbound_clear drops them all. */
+int   gxy_outbound_count(int face);
+void *gxy_outbound_take(int face, RayList_ispc *ispc, int *n);
+int   gxy_outbound_clear(void);
 
 /* Same loop starting from caller-supplied device rays (count n, not consumed). */
 int gxy_process_raylist(void *vis, void *lights, int n, RayList_ispc *rays,

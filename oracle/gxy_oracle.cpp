@@ -535,11 +535,13 @@ template <typename Acc>
 struct Col4 { Acc x, y, z, w; };
 
 // opaque test of TraceRays.ispc:466,550,581
+float g_opaque_alpha = 0.999f;  // reference value; the C2 workload uses 0.99
+
 template <typename Acc>
 inline bool opaque_of(const Col4<Acc> &c) {
   Acc m = c.x < c.y ? c.x : c.y;
   m = m < c.z ? m : c.z;
-  return m >= (Acc)1 || c.w > (Acc)0.999f;
+  return m >= (Acc)1 || c.w > (Acc)g_opaque_alpha;
 }
 
 #define GXY_MAX_VOLS_ORACLE 16
@@ -750,7 +752,7 @@ void trace_one(const Visualization_ispc &vis, const std::vector<OVolumeVis> &vv,
     sb.rgba64[4 * (size_t)i] = (double)color.x; sb.rgba64[4 * (size_t)i + 1] = (double)color.y;
     sb.rgba64[4 * (size_t)i + 2] = (double)color.z; sb.rgba64[4 * (size_t)i + 3] = (double)color.w;
   }
-  int term = (mn(mn(fr, fg), fb) >= 1.0f || fo > 0.999f) ? RAY_OPAQUE : 0;
+  int term = (mn(mn(fr, fg), fb) >= 1.0f || fo > g_opaque_alpha) ? RAY_OPAQUE : 0;
   R.t[i] = ray_t;
   if (surface_hit) {
     term |= RAY_SURFACE;
@@ -865,6 +867,7 @@ const float kTwoPi = 6.2831855f;  // 2.f*M_PI in float
 // ======================================================================= API
 
 OEXPORT void oracle_set_threads(int n) { g_threads = n; }
+OEXPORT void oracle_set_opacity_threshold(float a) { g_opaque_alpha = a; }
 OEXPORT int oracle_get_threads(void) { return nthreads(); }
 
 OEXPORT void oracle_uv_tables(float *U, float *V) {
@@ -1511,10 +1514,11 @@ void process_list(FrameCtx &C, std::unique_ptr<HRayList> rl, std::deque<std::uni
     }
     oracle_classify(n, &R, 1);
     oracle_assign_destinations(n, &R, &C.V.ispc->local_bb, C.nb, 1);
-    C.st.terminated += oracle_accumulate(n, &R, C.fb, C.w, C.h);
+    oracle_accumulate(n, &R, C.fb, C.w, C.h);
     int nk = 0;
     for (int i = 0; i < n; i++) {
       int c = R.classification[i];
+      if (c == GXY_TERMINATED) C.st.terminated++;
       if (c == GXY_KEEP_HERE) nk++;
       else if (c == GXY_DROP_ON_FLOOR) C.st.dropped++;
       else if (c >= 0) {

@@ -1,0 +1,321 @@
+"""GPU parity: libgxy_hip.so against the CPU oracle on the same seeded inputs.
+
+Bars (from the project brief): ray classification / hit primitive IDs and
+spawned-ray counts agree on >= 99.99% of rays; hit t within 1e-5 relative;
+composited RGBA within 1e-4 absolute of the FP64 compositing oracle; counts
+(primary, spawned, terminated, dropped, kept, outbound per face) exact."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import GOLD_TESTS, gold_image, gold_scenes, image_error, random_rays
+from galaxy_amd import abi, scenes
+
+pytestmark = pytest.mark.gpu
+f32 = np.float32
+
+CASES = [(t, k) for t in GOLD_TESTS for k in range(3 if t == "nineBalls" else 1)]
+
+
+def same_stats(a, b):
+    return a.as_dict() == b.as_dict()
+
+
+@pytest.mark.parametrize("name,cam", CASES)
+def test_gold_frames_match_oracle_and_gold(name, cam, radial_vols, oracle, hip):
+    sc = gold_scenes(radial_vols, name)[0]
+    c = sc.cameras[cam]
+    L = scenes.resolve_lights(sc.lighting, c)
+    fb_g, st_g = hip.render_frame(scenes.build_scene(hip, sc, L), c)
+    fb_o, st_o = oracle.render_frame(scenes.build_scene(oracle, sc, L), c)
+    assert same_stats(st_g, st_o), (st_g.as_dict(), st_o.as_dict())
+    assert np.abs(fb_g - fb_o).max() <= 1e-4
+    rmse, bad = image_error(scenes.fb_to_rgba8(fb_g, c.width, c.height), gold_image(name, cam))
+    assert rmse <= 1e-3 and bad <= 0.005
+
+
+def _dvr_scene(vols, nvol=1):
+    names = ["radial-oneBall.vol", "radial-eightBalls.vol", "radial-xramp.vol", "radial-zramp.vol"]
+    vv = []
+    for k in range(nvol):
+        vol = vols[names[k]]
+        vv.append(scenes.VolumeVisDesc(
+            vol, colormap=np.array([[0, 1, .5, .5], [.5, .5, 1, .5], [1, .5, .5, 1]], f32),
+            opacitymap=np.array([[0, .05], [.5, .02], [1, 0]], f32),
+            slices=np.array([[0, 0, 1, 0.1 * k]], f32) if k < 2 else np.zeros((0, 4), f32),
+            isovalues=np.array([0.6, 0.3][: (k % 2) + 1], f32), volume_render=k != 2))
+    v0 = vols[names[0]]
+    return scenes.SceneDesc(volume_vis=vv, local_box=v0.local_box, global_box=v0.global_box)
+
+
+@pytest.mark.parametrize("nvol", [1, 2, 4])
+def test_trace_random_rays(nvol, radial_vols_small, oracle, hip):
+    from galaxy_amd import lib as glib
+    sc = _dvr_scene(radial_vols_small, nvol)
+    Bh, Bo = scenes.build_scene(hip, sc), scenes.build_scene(oracle, sc)
+    hr = random_rays(60000, seed=10 + nvol, tmax=1e30)
+    # a third of the rays get a finite lifetime (AO-style timeouts)
+    hr["tMax"] = np.where(np.arange(hr.n) % 3 == 0, 0.7, 1e30).astype(f32)
+    d = glib.DeviceRays.from_host(hr)
+    assert hip.trace(Bh.vis, d) == ""
+    g = d.download()
+    o32 = hr.copy()
+    oracle.trace(Bo.vis, o32)
+    o64 = hr.copy()
+    res = oracle.trace(Bo.vis, o64, fp64=True)
+    assert (g["term"] == o32["term"]).mean() >= 0.9999
+    same = g["term"] == o32["term"]
+    assert np.allclose(g["t"][same], o32["t"][same], rtol=1e-5, atol=1e-7)
+    rgba = np.stack([g["r"], g["g"], g["b"], g["o"]], 1).astype(np.float64)
+    assert np.abs(rgba - res["rgba64"])[same].max() <= 1e-4
+    surf = same & ((g["term"] & abi.RAY_SURFACE) > 0) & (g["type"] == 1)
+    for f in ("nx", "ny", "nz", "sr", "sg", "sb"):
+        assert np.abs(g[f][surf] - o32[f][surf]).max() <= 1e-5
+
+
+def test_spawn_lighting_route_partition(radial_vols_small, oracle, hip):
+    from galaxy_amd import lib as glib
+    L_ = glib.lib()
+    sc = _dvr_scene(radial_vols_small, 1)
+    Ld = scenes.LightingDesc(np.array([[2, 3, 4], [-1, -1, -1]], f32), np.array([2, 0], np.int32), Ka=0.3,
+                             Kd=0.7, n_ao=8, ao_radius=0.5, shadows=True)
+    Bh, Bo = scenes.build_scene(hip, sc, Ld), scenes.build_scene(oracle, sc, Ld)
+    hr = random_rays(40000, seed=21, typ_mix=(1.0, 0.0, 0.0))
+    d = glib.DeviceRays.from_host(hr)
+    hip.trace(Bh.vis, d)
+    o = hr.copy()
+    oracle.trace(Bo.vis, o)
+    g = d.download()
+    assert (g["term"] == o["term"]).all()
+    n = hr.n
+    # offsets (TraceRays.cpp:89-117)
+    ao_d, sh_d = glib.DeviceArray(4 * n), glib.DeviceArray(4 * n)
+    na, ns = C.c_int(), C.c_int()
+    glib.check(L_.gxy_spawn_offsets(n, C.byref(d.ispc), 8, 2, 1, ao_d.ptr, sh_d.ptr, C.byref(na), C.byref(ns)))
+    ao_o, sh_o = np.zeros(n, np.int32), np.zeros(n, np.int32)
+    na_o, ns_o = C.c_int(), C.c_int()
+    oracle.L.oracle_spawn_offsets(n, C.byref(o.ispc), 8, 2, 1, ao_o.ctypes.data, sh_o.ctypes.data, C.byref(na_o),
+                                  C.byref(ns_o))
+    assert (na.value, ns.value) == (na_o.value, ns_o.value) and na.value > 1000
+    assert np.array_equal(ao_d.to_host(np.int32), ao_o) and np.array_equal(sh_d.to_host(np.int32), sh_o)
+    nout = na.value + ns.value
+    # lighting in the reference order, separately callable entry points
+    out_d = glib.DeviceRays(nout)
+    out_o = abi.HostRays(nout)
+    lh, lo = C.byref(Bh.lights), C.byref(Bo.lights)
+    L_.TraceRays_ambientLighting(None, lh, n, C.byref(d.ispc))
+    L_.TraceRays_generateAORays(None, lh, n, C.byref(d.ispc), ao_d.ptr, C.byref(out_d.ispc), 0.001)
+    L_.TraceRays_diffuseLighting(None, lh, n, C.byref(d.ispc))
+    L_.TraceRays_generateShadowRays(None, lh, n, C.byref(d.ispc), sh_d.ptr, C.byref(out_d.ispc), 0.001)
+    glib.check(L_.gxy_synchronize())
+    oracle.L.oracle_ambientLighting(lo, n, C.byref(o.ispc))
+    oracle.L.oracle_generateAORays(lo, n, C.byref(o.ispc), ao_o.ctypes.data, C.byref(out_o.ispc), 0.001)
+    oracle.L.oracle_diffuseLighting(lo, n, C.byref(o.ispc))
+    oracle.L.oracle_generateShadowRays(lo, n, C.byref(o.ispc), sh_o.ctypes.data, C.byref(out_o.ispc), 0.001)
+    g = d.download()
+    for f in ("r", "g", "b", "o"):
+        assert np.abs(g[f] - o[f]).max() <= 1e-6
+    gout = out_d.download()
+    for f in ("ox", "oy", "oz", "dx", "dy", "dz", "r", "g", "b", "o", "t", "tMax"):
+        assert np.allclose(gout[f], out_o[f], rtol=1e-5, atol=2e-6), f
+    for f in ("x", "y", "type", "term"):
+        assert np.array_equal(gout[f], out_o[f]), f
+    # classify + assign + accumulate
+    box = Bh.vis.local_bb
+    nb = np.array([-1, 3, -1, 5, -1, -1], np.int32)
+    glib.check(L_.gxy_classify(n, C.byref(d.ispc), 1))
+    glib.check(L_.gxy_assign_destinations(n, C.byref(d.ispc), C.byref(box), nb.ctypes.data, 1))
+    fb = glib.DeviceArray(64 * 64 * 16).zero()
+    glib.check(L_.gxy_accumulate_terminated(n, C.byref(d.ispc), fb.ptr, 64, 64))
+    oracle.L.oracle_classify(n, C.byref(o.ispc), 1)
+    oracle.L.oracle_assign_destinations(n, C.byref(o.ispc), C.byref(Bo.vis.local_bb), nb.ctypes.data, 1)
+    fbo = np.zeros(64 * 64 * 4, f32)
+    oracle.L.oracle_accumulate(n, C.byref(o.ispc), fbo.ctypes.data, 64, 64)
+    g = d.download()
+    assert np.array_equal(g["classification"], o["classification"])
+    assert np.abs(fb.to_host(f32) - fbo).max() <= 1e-4
+    # partition into destination lists (stable, exact)
+    ndest = 6
+    counts = np.zeros(ndest + 1, np.int32)
+    oc = np.zeros(ndest + 1, np.int32)
+    oracle.L.oracle_partition_counts(n, C.byref(o.ispc), ndest, oc.ctypes.data)
+    lists = [glib.DeviceRays(max(1, int(c))) for c in oc]
+    ptrs = (C.c_void_p * ndest)(*[C.cast(C.pointer(lists[k].ispc), C.c_void_p) for k in range(ndest)])
+    glib.check(L_.gxy_partition_rays(n, C.byref(d.ispc), ndest, ptrs, C.byref(lists[ndest].ispc),
+                                     counts.ctypes.data))
+    assert np.array_equal(counts, oc) and counts[3] > 0 and counts[5] > 0
+    for dst in (3, 5):
+        lists[dst].n = int(counts[dst])
+        got = lists[dst].download()
+        idx = np.nonzero(o["classification"] == dst)[0]
+        for f in ("ox", "dx", "r", "x", "term"):
+            assert np.array_equal(got[f], o[f][idx]), f
+
+
+def test_camera_rays_bit_exact(hip, oracle):
+    cam = scenes.CameraDesc((2.2, 1.6, 2.8), (0.1, 0.0, -0.2), (0.0, 1.0, 0.0), 35.0, 301, 173)
+    box = ((-1.0, -0.8, -1.2), (1.0, 0.9, 1.1))
+    R_o = oracle.camera_rays(cam, box, box)
+    d = hip.camera_rays(cam, box, box, order=0)
+    g = d.download()
+    assert d.n == R_o.n > 0
+    for f in ("x", "y", "ox", "oy", "oz", "dx", "dy", "dz", "tMax", "type"):
+        assert np.array_equal(g[f], R_o[f]), f
+    d1 = hip.camera_rays(cam, box, box, order=1)
+    g1 = d1.download()
+    assert d1.n == R_o.n
+    key0 = g["y"].astype(np.int64) * 10000 + g["x"]
+    key1 = g1["y"].astype(np.int64) * 10000 + g1["x"]
+    o1 = np.argsort(key1)
+    assert np.array_equal(key1[o1], key0)
+    assert np.array_equal(g1["dx"][o1], g["dx"])
+
+
+def test_geometry_meshes_and_spheres(hip, oracle):
+    from galaxy_amd import lib as glib
+    L_ = glib.lib()
+    m1 = scenes.uv_sphere(1.0, 128, 80)
+    m2 = scenes.icosphere(5, 0.4, (1.3, 0.2, -0.3))
+    rng = np.random.default_rng(4)
+    sp = scenes.SpheresDesc(rng.uniform(-1.5, 1.5, (3000, 3)).astype(f32), rng.uniform(0, 1, 3000).astype(f32),
+                            radius0=0.01, radius1=0.05, value0=0.0, value1=1.0,
+                            colormap=np.array([[0, 1, 0, 0], [1, 0, 0, 1]], f32))
+    m1.colormap = np.array([[-1, 0.2, 0.3, 0.9], [1, 0.9, 0.3, 0.2]], f32)
+    box = ((-2.0,) * 3, (2.0,) * 3)
+    sc = scenes.SceneDesc(meshes=[m1, m2], spheres=[sp], local_box=box, global_box=box)
+    Bh, Bo = scenes.build_scene(hip, sc), scenes.build_scene(oracle, sc)
+    n = 100000
+    hr = random_rays(n, seed=8, lo=-2.5, hi=2.5)
+    gd, pd = glib.DeviceArray(4 * n), glib.DeviceArray(4 * n)
+    glib.check(L_.gxy_model_set_hit_output(Bh.vis.model, gd.ptr, pd.ptr))
+    try:
+        d = glib.DeviceRays.from_host(hr)
+        hip.trace(Bh.vis, d)
+        g = d.download()
+        gg, gp = gd.to_host(np.int32), pd.to_host(np.int32)
+    finally:
+        glib.check(L_.gxy_model_set_hit_output(Bh.vis.model, None, None))
+    o = hr.copy()
+    res = oracle.trace(Bo.vis, o, geom_ids=True)
+    hits = res["geom"] >= 0
+    assert hits.sum() > 10000 and (res["geom"] == 2).sum() > 100
+    agree = (gg == res["geom"]) & (gp == res["prim"])
+    assert agree.mean() >= 0.9999
+    assert np.allclose(g["t"][hits & agree], o["t"][hits & agree], rtol=1e-5)
+    prim = agree & hits & (o["type"] == 1)
+    for f in ("nx", "ny", "nz", "sr", "sg", "sb"):
+        assert np.abs(g[f][prim] - o[f][prim]).max() <= 1e-5, f
+    nodes, leaves, depth = C.c_int(), C.c_int(), C.c_int()
+    glib.check(L_.gxy_model_bvh_stats(Bh.vis.model, C.byref(nodes), C.byref(leaves), C.byref(depth), None))
+    assert leaves.value == 20480 + 20480 + 3000 and depth.value <= 60
+
+
+def test_mesh_frame_with_shadows_matches_oracle(hip, oracle):
+    from galaxy_amd import workloads
+    sc, alpha = workloads.c1(256, 256)
+    c = sc.cameras[0]
+    fb_g, st_g = hip.render_frame(scenes.build_scene(hip, sc), c)
+    fb_o, st_o = oracle.render_frame(scenes.build_scene(oracle, sc), c)
+    assert st_g.primary_rays == st_o.primary_rays and st_g.secondary_rays == st_o.secondary_rays
+    assert abs(st_g.terminated - st_o.terminated) <= max(1, st_o.ray_passes // 10000)
+    assert np.mean(np.abs(fb_g - fb_o) <= 1e-4) >= 0.9999
+
+
+def test_sampler_loop_matches_oracle(radial_vols_small, hip, oracle):
+    from galaxy_amd import lib as glib
+    L_ = glib.lib()
+    vol = radial_vols_small["radial-eightBalls.vol"]
+    vv = scenes.VolumeVisDesc(vol, isovalues=np.array([0.3], f32))
+    sc = scenes.SceneDesc(volume_vis=[vv], local_box=vol.local_box, global_box=vol.global_box)
+    Bh, Bo = scenes.build_scene(hip, sc), scenes.build_scene(oracle, sc)
+    hr = random_rays(20000, seed=31, typ_mix=(1.0, 0, 0))
+    cap = 200000
+    xyz_o = np.zeros((cap, 3), f32)
+    passes = C.c_longlong()
+    cnt_o = oracle.L.oracle_sample_raylist(C.byref(Bo.vis), hr.n, C.byref(hr.ispc), xyz_o.ctypes.data, cap,
+                                           C.byref(passes))
+    d = glib.DeviceRays.from_host(hr)
+    xyz = glib.DeviceArray(cap * 12)
+    cnt = C.c_int()
+    st = abi.FrameStats()
+    glib.check(L_.gxy_sample_raylist(C.byref(Bh.vis), hr.n, C.byref(d.ispc), 1 << 30, xyz.ptr, cap, C.byref(cnt),
+                                     C.byref(st)))
+    assert cnt.value == cnt_o > 3000
+    assert st.passes == passes.value
+    # particles are emitted pass by pass in ray order on both sides
+    assert np.abs(xyz.to_host(f32, 3 * cnt.value).reshape(-1, 3) - xyz_o[:cnt_o]).max() <= 1e-5
+
+
+def test_partitioned_volume_outbound_counts(oracle, hip):
+    """One rank of a 2x2x2 partition of eightBalls: rays leaving through faces
+    with a neighbour go to the outbound queue; per-face counts equal the
+    oracle's and the lists hold exactly those rays."""
+    from galaxy_amd import lib as glib
+    L_ = glib.lib()
+    F = scenes.radial_fields(64)
+    org, dl = scenes.vol_header_from_vti(64)
+    vol = scenes.partitioned_volume(F["eightBalls"], org, dl, rank=5, size=8)
+    assert sorted(x for x in vol.neighbors if x >= 0) == [1, 4, 7]
+    vv = scenes.VolumeVisDesc(vol, isovalues=np.array([0.2], f32), volume_render=True,
+                              opacitymap=np.array([[0, 0.02], [1, 0.02]], f32))
+    Ld = scenes.LightingDesc(np.array([[0, 0, 4]], f32), np.array([0], np.int32), shadows=True, n_ao=4,
+                             ao_radius=0.3)
+    sc = scenes.SceneDesc(volume_vis=[vv], local_box=vol.local_box, global_box=vol.local_box,
+                          neighbors=vol.neighbors)
+    Bh, Bo = scenes.build_scene(hip, sc, Ld), scenes.build_scene(oracle, sc, Ld)
+    hr = random_rays(30000, seed=41, lo=-1.0, hi=1.0, typ_mix=(1.0, 0, 0))
+    nb = np.array(vol.neighbors, np.int32)
+    fb = glib.DeviceArray(64 * 64 * 16).zero()
+    st = abi.FrameStats()
+    L_.gxy_outbound_clear()
+    d = glib.DeviceRays.from_host(hr)
+    glib.check(L_.gxy_process_raylist(C.byref(Bh.vis), C.byref(Bh.lights), hr.n, C.byref(d.ispc), nb.ctypes.data,
+                                      0.001, 7000, fb.ptr, 64, 64, C.byref(st)))
+    fbo = np.zeros(64 * 64 * 4, f32)
+    so = abi.FrameStats()
+    oracle.L.oracle_process_raylist(C.byref(Bo.vis), C.byref(Bo.lights), hr.n, C.byref(hr.ispc), nb.ctypes.data,
+                                    0.001, 7000, fbo.ctypes.data, 64, 64, C.byref(so))
+    assert st.as_dict() == so.as_dict(), (st.as_dict(), so.as_dict())
+    assert sum(st.outbound) > 1000
+    assert np.abs(fb.to_host(f32) - fbo).max() <= 1e-4
+    for f in range(6):
+        tot = 0
+        while True:
+            isp, k = abi.RayList_ispc(), C.c_int()
+            h = L_.gxy_outbound_take(f, C.byref(isp), C.byref(k))
+            if not h:
+                break
+            tot += k.value
+            L_.gxy_raylist_destroy(h)
+        assert tot == st.outbound[f]
+
+
+def test_raylist_block_roundtrip(hip):
+    from galaxy_amd import lib as glib
+    L_ = glib.lib()
+    hr = random_rays(1000, seed=3)
+    d = glib.DeviceRays.from_host(hr)
+    assert L_.gxy_raylist_aligned_size(d.h) == 1008
+    nbytes = L_.gxy_raylist_bytes(d.h)
+    assert nbytes == 64 + 1008 * 100
+    blk = np.zeros(nbytes, np.uint8)
+    glib.check(L_.gxy_raylist_download(d.h, blk.ctypes.data))
+    hdr = blk[:64].view(np.int32)
+    assert hdr[6] == 1000 and hdr[8] == 1008
+    assert np.array_equal(blk[64:64 + 4000].view(f32), hr["ox"])              # ox first
+    assert np.array_equal(blk[64 + 20 * 4032:64 + 20 * 4032 + 4000].view(np.int32), hr["x"])  # x after 20 floats
+    isp = abi.RayList_ispc()
+    h2 = glib.checkp(L_.gxy_raylist_upload(blk.ctypes.data, C.byref(isp)))
+    part = abi.RayList_ispc()
+    h3 = glib.checkp(L_.gxy_raylist_split_part(h2, 100, 300, C.byref(part)))
+    assert L_.gxy_raylist_size(h3) == 300
+    got = np.zeros(300, f32)
+    glib.check(L_.gxy_copy_to_host(got.ctypes.data, part.dx, 1200))
+    assert np.array_equal(got, hr["dx"][100:400])
+    glib.check(L_.gxy_raylist_truncate(h3, 17, C.byref(part)))
+    assert L_.gxy_raylist_aligned_size(h3) == 32 and L_.gxy_raylist_size(h3) == 17
+    glib.check(L_.gxy_copy_to_host(got.ctypes.data, part.dx, 68))
+    assert np.array_equal(got[:17], hr["dx"][100:117])
+    for h in (h2, h3):
+        L_.gxy_raylist_destroy(h)
