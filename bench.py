@@ -44,6 +44,23 @@ def kernel_time(L, name):
     return ms.value, n.value
 
 
+def frame_fn(L, built, cam, fb, eps=0.001):
+    from galaxy_amd import abi, lib as glib
+    w, h = cam.width, cam.height
+    a3 = glib.a3
+    eye, cen, up = a3(cam.eye), a3(cam.center), a3(cam.up)
+    nb = np.full(6, -1, np.int32)
+    st = abi.FrameStats()
+
+    def frame():
+        glib.check(L.gxy_memset(fb.ptr, 0, fb.nbytes), "memset")
+        glib.check(L.gxy_render_frame(C.byref(built.vis), C.byref(built.lights), eye.ctypes.data, cen.ctypes.data,
+                                      up.ctypes.data, cam.aov, w, h, nb.ctypes.data, eps, 1 << 30, fb.ptr,
+                                      C.byref(st)), "render_frame")
+        return st
+    return frame
+
+
 def run_frames(L, hip, built, cam, steps, warmup, eps=0.001):
     from galaxy_amd import abi, lib as glib
     w, h = cam.width, cam.height
@@ -153,6 +170,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--c3-steps", type=int, default=3)
+    ap.add_argument("--no-tune", action="store_true", help="keep the default trace launch configuration")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 256 MiB device-to-device copy (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs one partition on one MI355X; "
@@ -163,12 +183,28 @@ def main():
     L = glib.lib()
     hip = glib.HipBackend()
 
+    if args.copy_calibration:
+        nbytes = 256 << 20
+        a, b = glib.DeviceArray(nbytes).zero(), glib.DeviceArray(nbytes)
+        glib.check(L.gxy_copy_device(b.ptr, a.ptr, nbytes))
+        glib.check(L.gxy_synchronize())
+        a.free()
+        b.free()
+
     # ---------------------------------------------------------------- c2
     sc, alpha = workloads.c2()
     cam = sc.cameras[0]
     glib.check(L.gxy_set_opacity_threshold(alpha))
     log("building c2 (512^3 noise volume)")
     built = scenes.build_scene(hip, sc)
+    tuned = None
+    if not args.no_tune:
+        from galaxy_amd import autotune
+        fbt = glib.DeviceArray(cam.width * cam.height * 16)
+        best, table = autotune.tune_trace(L, frame_fn(L, built, cam, fbt, sc.epsilon))
+        fbt.free()
+        tuned = dict(best=list(best), trace_ms=[[list(c), round(t, 4)] for c, t in table])
+        log("autotune", tuned)
     dt, tot, kt, st, image = run_frames(L, hip, built, cam, args.steps, args.warmup, sc.epsilon)
     ms_step = dt / args.steps * 1e3
     mrays = tot["ray_passes"] / dt / 1e6
@@ -188,7 +224,9 @@ def main():
                     frac=round(tfl / FP32_PEAK_TFLOPS, 5),
                     note="FP32 VALU work (no MFMA on this path); MI355X FP32 peak")
     roof["traffic"] = traffic_from_profile("trace_c2_traffic.txt")
-    roof["kernel"] = "k_trace<1,false>"
+    kcfg = (C.c_int(), C.c_int(), C.c_int())
+    L.gxy_get_trace_config(*[C.byref(x) for x in kcfg])
+    roof["kernel"] = "k_trace_pl<1>" if kcfg[0].value == 1 else "k_trace<1,false>"
     roof["kernel_ms_per_launch"] = round(tr_ms / max(1, tr_n), 4)
     roof["alg_bytes_per_launch"] = int(alg_bytes / max(1, tr_n))
     roof["alg_flops_per_launch"] = int(alg_flops / max(1, tr_n))
@@ -214,6 +252,7 @@ def main():
         "gsamples_per_s": round(gsamp, 3),
         "roofline": roof,
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt.items()},
+        "trace_config": tuned,
     }
     built = None
 

@@ -1,0 +1,48 @@
+"""Autotuner for the trace launch configuration.
+
+Every configuration computes identical results (the per-ray arithmetic does
+not change; only how rays are mapped onto lanes does), so the tuner only
+measures: it renders the caller's frame once per candidate with HIP-event
+timing of the trace kernel enabled and keeps the fastest.
+
+Candidates: one ray per work-item (kind 0, and its 5- and 6-waves-per-SIMD
+occupancy variants 15 and 16), or persistent lanes (kind 1) with a per-wave
+chunk of 128 or 256 consecutive rays and a refill threshold of 16 or 32 idle
+lanes out of 64.  (Measured on the volume-render frame: the persistent form
+was 2-5x slower than one ray per work-item; it stays a candidate for other
+ray mixes.)
+"""
+import ctypes as C
+
+DEFAULT_CANDIDATES = [(0, 256, 32), (15, 256, 32), (16, 256, 32)] + [(1, chunk, refill) for chunk in (128, 256)
+                                                                     for refill in (16, 32)]
+
+
+def _trace_ms(L):
+    ms, n = C.c_double(0), C.c_int(0)
+    L.gxy_timing_get(b"trace", C.byref(ms), C.byref(n))
+    return ms.value
+
+
+def tune_trace(L, render_once, candidates=None, reps=2):
+    """render_once(): renders one frame with the current configuration.
+    Returns (best (kind, chunk, refill), [(config, trace ms per frame), ...])
+    and leaves the best configuration set."""
+    from . import lib as glib
+    candidates = candidates or DEFAULT_CANDIDATES
+    table = []
+    for cfg in candidates:
+        glib.check(L.gxy_set_trace_config(*cfg), "set_trace_config")
+        render_once()                      # warm the configuration
+        glib.check(L.gxy_synchronize())
+        L.gxy_timing_reset()
+        L.gxy_timing_enable(1)
+        for _ in range(reps):
+            render_once()
+        glib.check(L.gxy_synchronize())
+        L.gxy_timing_enable(0)
+        table.append((cfg, _trace_ms(L) / reps))
+    best = min(table, key=lambda x: x[1])[0]
+    glib.check(L.gxy_set_trace_config(*best), "set_trace_config")
+    L.gxy_timing_reset()
+    return best, table

@@ -35,6 +35,8 @@ struct DVolume {
   float ub[3];    // nextafter(dims-1, 0): local-coordinate clamp
   float rate;     // samplingRate
   float step;     // samplingStep (minimum spacing)
+  float irate;    // 1 / samplingRate
+  int small;      // voxel count < 2^31: 32-bit index arithmetic
 };
 
 // Piecewise-linear TF: nc colours (rgb) and no opacities over [lo, hi].
@@ -43,6 +45,7 @@ struct DTF {
   const float *op;
   int nc, no;
   float lo, hi;
+  float sc, so;   // (nc-1)/(hi-lo), (no-1)/(hi-lo)
 };
 
 struct DVolVis {
@@ -105,8 +108,8 @@ struct TraceParams {
   int pad_;
   float blo[3], bhi[3];
   float step, eps;
+  float istep;         // 1 / step
   float opaque_alpha;  // early termination: alpha above this is opaque (0.999)
-  int pad2_;
   DVolVis vv[GXY_MAX_VOLVIS];
   DModel model;
   unsigned long long *sample_partials; // per-workgroup sample counts (may be null)
@@ -141,24 +144,32 @@ __device__ __forceinline__ f3 safe_normalize3(f3 v) {
 struct __attribute__((packed, aligned(4))) float2u { float x, y; };
 
 // Trilinear sample (OSPRay StructuredVolume semantics; x, then y, then z).
+// The clamp to [0, nextafter(dims-1, 0)] is one v_med3_f32 per axis (equal to
+// the compare-select clamp for every non-NaN input; NaN is mapped to 0 after).
 __device__ __forceinline__ float vol_sample(const DVolume &V, f3 p) {
   float l0 = (p.x - V.org[0]) * V.isp[0];
   float l1 = (p.y - V.org[1]) * V.isp[1];
   float l2 = (p.z - V.org[2]) * V.isp[2];
-  l0 = l0 < 0.f ? 0.f : (l0 > V.ub[0] ? V.ub[0] : l0);
-  l1 = l1 < 0.f ? 0.f : (l1 > V.ub[1] ? V.ub[1] : l1);
-  l2 = l2 < 0.f ? 0.f : (l2 > V.ub[2] ? V.ub[2] : l2);
+  l0 = __builtin_amdgcn_fmed3f(l0, 0.f, V.ub[0]);
+  l1 = __builtin_amdgcn_fmed3f(l1, 0.f, V.ub[1]);
+  l2 = __builtin_amdgcn_fmed3f(l2, 0.f, V.ub[2]);
   if (l0 != l0) l0 = 0.f;
   if (l1 != l1) l1 = 0.f;
   if (l2 != l2) l2 = 0.f;
   int i0 = (int)l0, i1 = (int)l1, i2 = (int)l2;
   float f0 = l0 - (float)i0, f1 = l1 - (float)i1, f2 = l2 - (float)i2;
-  const int64_t sy = V.n[0], sz = (int64_t)V.n[0] * V.n[1];
-  const float *d = V.data + (int64_t)i0 + sy * i1 + sz * i2;
+  const float *d;
+  int sy = V.n[0], sz = V.n[0] * V.n[1];
+  if (V.small) {
+    d = V.data + (unsigned)(i0 + sy * i1 + sz * i2);
+  } else {
+    d = V.data + ((int64_t)i0 + (int64_t)sy * i1 + (int64_t)V.n[0] * V.n[1] * i2);
+  }
+  const int64_t SZ = V.small ? (int64_t)sz : (int64_t)V.n[0] * V.n[1];
   float2u p00 = *(const float2u *)(d);
   float2u p10 = *(const float2u *)(d + sy);
-  float2u p01 = *(const float2u *)(d + sz);
-  float2u p11 = *(const float2u *)(d + sz + sy);
+  float2u p01 = *(const float2u *)(d + SZ);
+  float2u p11 = *(const float2u *)(d + SZ + sy);
   float g0 = 1.0f - f0, g1 = 1.0f - f1, g2 = 1.0f - f2;
   float a00 = g0 * p00.x + f0 * p00.y;
   float a10 = g0 * p10.x + f0 * p10.y;
@@ -178,30 +189,36 @@ __device__ __forceinline__ f3 vol_gradient(const DVolume &V, f3 p) {
   return mk3(gx / V.sp[0], gy / V.sp[1], gz / V.sp[2]);
 }
 
-// LinearTransferFunction lookups over tables that may live in LDS.
-__device__ __forceinline__ f3 tf_color(const float *col, int n, float lo, float hi, float v) {
+// LinearTransferFunction lookups over tables that may live in LDS; the
+// division by (hi-lo) is a multiplication by the precomputed (n-1)/(hi-lo).
+__device__ __forceinline__ f3 tf_color(const float *col, int n, float lo, float hi, float sc, float v) {
   if (v != v) return mk3(0.f, 0.f, 0.f);
   if (n == 0) return mk3(1.f, 1.f, 1.f);
   if (v <= lo) return mk3(col[0], col[1], col[2]);
   if (v >= hi) return mk3(col[3 * n - 3], col[3 * n - 2], col[3 * n - 1]);
-  float x = (v - lo) / (hi - lo) * ((float)n - 1.0f);
-  int i = (int)floorf(x);
+  float x = (v - lo) * sc;
+  int i = (int)x;
+  i = i < n - 1 ? i : n - 1;
   float r = x - (float)i;
   int j = i + 1 < n - 1 ? i + 1 : n - 1;
   float q = 1.0f - r;
   return mk3(q * col[3 * i] + r * col[3 * j], q * col[3 * i + 1] + r * col[3 * j + 1],
              q * col[3 * i + 2] + r * col[3 * j + 2]);
 }
-__device__ __forceinline__ float tf_opacity(const float *op, int n, float lo, float hi, float v) {
+__device__ __forceinline__ float tf_opacity(const float *op, int n, float lo, float hi, float so, float v) {
   if (v != v) return 0.f;
   if (n == 0) return 1.f;
   if (v <= lo) return op[0];
   if (v >= hi) return op[n - 1];
-  float x = (v - lo) / (hi - lo) * ((float)n - 1.0f);
-  int i = (int)floorf(x);
+  float x = (v - lo) * so;
+  int i = (int)x;
+  i = i < n - 1 ? i : n - 1;
   float r = x - (float)i;
   int j = i + 1 < n - 1 ? i + 1 : n - 1;
   return (1.0f - r) * op[i] + r * op[j];
+}
+__device__ __forceinline__ f3 tf_color(const float *col, const DTF &T, float v) {
+  return tf_color(col, T.nc, T.lo, T.hi, T.sc, v);
 }
 
 // MyIntersectBox (TraceRays.ispc:90-106), rcp as 1/x.

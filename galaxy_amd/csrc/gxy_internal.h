@@ -37,8 +37,15 @@ struct CameraParams {
   float llo[3], lhi[3], glo[3], ghi[3];
 };
 
-hipError_t launch_trace(const TraceParams &P, const RayList_ispc &R, int n, hipStream_t s);
+// wpe: occupancy variant of the volume-only kernel (0 none, 5 or 6 waves/SIMD)
+hipError_t launch_trace(const TraceParams &P, const RayList_ispc &R, int n, int wpe, hipStream_t s);
 hipError_t launch_sampler(const TraceParams &P, const RayList_ispc &R, int n, hipStream_t s);
+// Persistent-lane trace for volume-only visualizations (1..4 volumes): each
+// wave owns `chunk` consecutive rays; idle lanes are refilled when at least
+// `refill` of the 64 are idle.  Sample partials: trace_pl_blocks(n, chunk).
+hipError_t launch_trace_pl(const TraceParams &P, const RayList_ispc &R, int n, int chunk, int refill,
+                           hipStream_t s);
+int trace_pl_blocks(int n, int chunk);
 hipError_t launch_reduce_add(const unsigned long long *part, int nb, unsigned long long *acc, hipStream_t s);
 
 // Exclusive offsets of primary surface rays (flag scan).  block_counts needs

@@ -56,30 +56,43 @@ struct PendingEv {
   hipEvent_t a, b;
 };
 
+// An execution context: one HIP stream (default priority), its device
+// counters ([0] samples), and a cache of freed device blocks that are reused
+// only on this stream (stream order makes the reuse safe without a sync).
+struct Ctx {
+  hipStream_t stream = nullptr;
+  unsigned long long *counters = nullptr;
+  std::multimap<size_t, void *> free_blocks;
+  size_t bytes_cached = 0;
+};
+
 struct Runtime {
   std::recursive_mutex mu;
   int device = -1;
-  hipStream_t stream = nullptr;
+  Ctx def;                      // the library's default context
   hipStream_t user_stream = nullptr;
-  // caching allocator: size class -> free blocks; live block -> size class
-  std::multimap<size_t, void *> free_blocks;
-  std::map<void *, size_t> live;
-  size_t bytes_live = 0, bytes_cached = 0;
+  std::map<void *, size_t> live;  // live block -> size class
+  size_t bytes_live = 0;
   // timing
   bool timing = false;
   std::vector<PendingEv> pending;
   std::map<std::string, Timing> timings;
-  // per-runtime device counters: [0] samples [1] terminated [2] dropped
-  unsigned long long *counters = nullptr;
   // registries for handle validation
-  std::set<void *> volumes, tfs, models, raylists, tracers, lightings;
+  std::set<void *> volumes, tfs, models, raylists, tracers, lightings, contexts;
   std::atomic<int> raylist_id{0};
   float opaque_alpha = 0.999f;
   std::vector<void *> outbound[6];   // RayListObj* per exit face, last frames
+  // volume-only trace kernel: 0 one ray per work-item, 1 persistent lanes
+  int trace_kind = 0, trace_chunk = 256, trace_refill = 32, trace_wpe = 0;
 };
 static Runtime G;
+static thread_local Ctx *t_ctx = nullptr;
 
-static hipStream_t S() { return G.user_stream ? G.user_stream : G.stream; }
+static Ctx &CX() { return t_ctx ? *t_ctx : G.def; }
+static hipStream_t S() {
+  if (t_ctx) return t_ctx->stream;
+  return G.user_stream ? G.user_stream : G.def.stream;
+}
 
 static size_t size_class(size_t b) {
   if (b < 256) return 256;
@@ -96,26 +109,27 @@ static size_t size_class(size_t b) {
   return (r + 65535) & ~(size_t)65535;
 }
 
-static void release_cache() {
-  for (auto &kv : G.free_blocks) hipFree(kv.second);
-  G.free_blocks.clear();
-  G.bytes_cached = 0;
+static void release_cache(Ctx &c) {
+  for (auto &kv : c.free_blocks) hipFree(kv.second);
+  c.free_blocks.clear();
+  c.bytes_cached = 0;
 }
 
 void *dalloc(size_t bytes) {
   std::lock_guard<std::recursive_mutex> lk(G.mu);
+  Ctx &cx = CX();
   size_t c = size_class(bytes ? bytes : 1);
-  auto it = G.free_blocks.find(c);
+  auto it = cx.free_blocks.find(c);
   void *p = nullptr;
-  if (it != G.free_blocks.end()) {
+  if (it != cx.free_blocks.end()) {
     p = it->second;
-    G.free_blocks.erase(it);
-    G.bytes_cached -= c;
+    cx.free_blocks.erase(it);
+    cx.bytes_cached -= c;
   } else {
     if (hipMalloc(&p, c) != hipSuccess) {
       (void)hipGetLastError();
-      hipDeviceSynchronize();
-      release_cache();
+      hipStreamSynchronize(S());
+      release_cache(cx);
       if (hipMalloc(&p, c) != hipSuccess) {
         (void)hipGetLastError();
         fail("device allocation of %zu bytes failed (%zu live)", c, G.bytes_live);
@@ -128,22 +142,8 @@ void *dalloc(size_t bytes) {
   return p;
 }
 
-void dfree(void *p) {
-  if (!p) return;
-  std::lock_guard<std::recursive_mutex> lk(G.mu);
-  auto it = G.live.find(p);
-  if (it == G.live.end()) return;
-  size_t c = it->second;
-  G.live.erase(it);
-  G.bytes_live -= c;
-  // blocks are reused only after the work queued before the free has run
-  hipStreamSynchronize(S());
-  G.free_blocks.insert({c, p});
-  G.bytes_cached += c;
-}
-
-// free without a stream synchronisation: only for blocks whose last use is
-// ordered before any later use on the same stream
+// Frees into the current context's cache; the block is reused on this stream
+// only, after the work queued before the free (stream order).
 static void dfree_stream_ordered(void *p) {
   if (!p) return;
   std::lock_guard<std::recursive_mutex> lk(G.mu);
@@ -152,8 +152,17 @@ static void dfree_stream_ordered(void *p) {
   size_t c = it->second;
   G.live.erase(it);
   G.bytes_live -= c;
-  G.free_blocks.insert({c, p});
-  G.bytes_cached += c;
+  Ctx &cx = CX();
+  cx.free_blocks.insert({c, p});
+  cx.bytes_cached += c;
+}
+
+// Frees a block that other streams may have used: waits for the device first.
+void dfree(void *p) {
+  if (!p) return;
+  if (G.contexts.empty()) hipStreamSynchronize(S());
+  else hipDeviceSynchronize();   // another context's stream may still read it
+  dfree_stream_ordered(p);
 }
 
 // ------------------------------------------------------------------ timing
@@ -219,15 +228,14 @@ static int ensure_init() {
   int dev = 0;
   if (getenv("GXY_DEVICE")) dev = atoi(getenv("GXY_DEVICE"));
   HIPCHK(hipSetDevice(dev));
-  HIPCHK(hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&G.def.stream, hipStreamNonBlocking));
   float U[256], V[256];
   make_uv(U, V);
   HIPCHK(upload_uv_tables(U, V));
-  G.counters = (unsigned long long *)nullptr;
   G.device = dev;
-  G.counters = (unsigned long long *)dalloc(8 * sizeof(unsigned long long));
-  if (!G.counters) return -1;
-  HIPCHK(hipMemsetAsync(G.counters, 0, 8 * sizeof(unsigned long long), S()));
+  G.def.counters = (unsigned long long *)dalloc(8 * sizeof(unsigned long long));
+  if (!G.def.counters) return -1;
+  HIPCHK(hipMemsetAsync(G.def.counters, 0, 8 * sizeof(unsigned long long), S()));
   return 0;
 }
 #define INIT_OR(ret) \
@@ -378,6 +386,7 @@ static int make_trace_params(const Visualization_ispc *vis, float global_eps, Tr
     P.blo[k] = vis->local_bb.lower[k];
     P.bhi[k] = vis->local_bb.upper[k];
   }
+  P.istep = P.step > 0 ? 1.0f / P.step : 0.f;
   if (vis->model) {
     ModelObj *M = lookup<ModelObj>(G.models, vis->model);
     if (!M) return fail("Visualization model is not a gxy model handle");
@@ -436,6 +445,58 @@ GXY_EXPORT int gxy_synchronize(void) {
   HIPCHK(hipStreamSynchronize(S()));
   return 0;
 }
+GXY_EXPORT void *gxy_context_create(void) {
+  INIT_OR(nullptr);
+  Ctx *c = new Ctx;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    fail("hipStreamCreate failed");
+    delete c;
+    return nullptr;
+  }
+  Ctx *prev = t_ctx;
+  t_ctx = c;
+  c->counters = (unsigned long long *)dalloc(8 * sizeof(unsigned long long));
+  t_ctx = prev;
+  if (!c->counters) return nullptr;
+  hipMemsetAsync(c->counters, 0, 8 * sizeof(unsigned long long), c->stream);
+  hipStreamSynchronize(c->stream);
+  std::lock_guard<std::recursive_mutex> lk(G.mu);
+  G.contexts.insert(c);
+  return c;
+}
+GXY_EXPORT int gxy_context_bind(void *ctx) {
+  if (ctx == nullptr) {
+    t_ctx = nullptr;
+    return 0;
+  }
+  std::lock_guard<std::recursive_mutex> lk(G.mu);
+  if (!G.contexts.count(ctx)) return fail("not a context handle");
+  t_ctx = (Ctx *)ctx;
+  return 0;
+}
+GXY_EXPORT int gxy_context_destroy(void *ctx) {
+  Ctx *c;
+  {
+    std::lock_guard<std::recursive_mutex> lk(G.mu);
+    if (!G.contexts.count(ctx)) return fail("not a context handle");
+    G.contexts.erase(ctx);
+    c = (Ctx *)ctx;
+  }
+  hipStreamSynchronize(c->stream);
+  Ctx *prev = t_ctx;
+  t_ctx = c;
+  dfree_stream_ordered(c->counters);
+  t_ctx = prev;
+  {
+    std::lock_guard<std::recursive_mutex> lk(G.mu);
+    release_cache(*c);
+  }
+  hipStreamDestroy(c->stream);
+  if (t_ctx == c) t_ctx = nullptr;
+  delete c;
+  return 0;
+}
+
 GXY_EXPORT int gxy_set_stream(void *s) {
   INIT_OR(-1);
   G.user_stream = (hipStream_t)s;
@@ -483,6 +544,30 @@ GXY_EXPORT int gxy_set_opacity_threshold(float alpha) {
   return 0;
 }
 GXY_EXPORT float gxy_get_opacity_threshold(void) { return G.opaque_alpha; }
+GXY_EXPORT int gxy_set_trace_config(int kind, int chunk, int refill) {
+  // kind 0: one ray per work-item; 10 + w: the same with a w-waves-per-SIMD
+  // occupancy hint (w = 5, 6); 1: persistent lanes
+  int wpe = 0;
+  if (kind >= 10) {
+    wpe = kind - 10;
+    if (wpe != 5 && wpe != 6) return fail("trace occupancy variant %d: 15 or 16", kind);
+    kind = 0;
+  }
+  if (kind < 0 || kind > 1) return fail("trace kernel kind %d: 0, 1, 15 or 16", kind);
+  if (chunk < 64 || chunk % 64 || refill < 1 || refill > 64)
+    return fail("trace config chunk %d (multiple of 64) refill %d (1..64)", chunk, refill);
+  G.trace_kind = kind;
+  G.trace_wpe = wpe;
+  G.trace_chunk = chunk;
+  G.trace_refill = refill;
+  return 0;
+}
+GXY_EXPORT int gxy_get_trace_config(int *kind, int *chunk, int *refill) {
+  *kind = G.trace_kind == 0 && G.trace_wpe ? 10 + G.trace_wpe : G.trace_kind;
+  *chunk = G.trace_chunk;
+  *refill = G.trace_refill;
+  return 0;
+}
 GXY_EXPORT size_t gxy_device_bytes_total(void) {
   if (ensure_init() != 0) return 0;
   size_t fr = 0, tot = 0;
@@ -632,6 +717,8 @@ GXY_EXPORT void *gxy_volume_create(const int dims[3], const float origin[3], con
   }
   v->d.data = v->data;
   v->d.rate = samplingRate;
+  v->d.irate = 1.0f / samplingRate;
+  v->d.small = v->count < (size_t)1 << 31 ? 1 : 0;
   v->d.step = std::min(spacing[0], std::min(spacing[1], spacing[2]));
   if (voxels) {
     if (hipMemcpyAsync(v->data, voxels, v->count * 4, hipMemcpyHostToDevice, S()) != hipSuccess ||
@@ -665,6 +752,7 @@ GXY_EXPORT int gxy_volume_set_sampling_rate(void *vol, float rate) {
   VolumeObj *v = lookup<VolumeObj>(G.volumes, vol);
   if (!v) return fail("not a volume handle");
   v->d.rate = rate;
+  v->d.irate = 1.0f / rate;
   return 0;
 }
 GXY_EXPORT int gxy_volume_fill_noise(void *vol, uint32_t seed, int octaves, float base_frequency) {
@@ -726,6 +814,8 @@ static void *tf_make(std::vector<float> col, std::vector<float> op, float lo, fl
   t->d.no = (int)t->hop.size();
   t->d.lo = lo;
   t->d.hi = hi;
+  t->d.sc = t->d.nc > 1 ? (float)(t->d.nc - 1) / (hi - lo) : 0.f;
+  t->d.so = t->d.no > 1 ? (float)(t->d.no - 1) / (hi - lo) : 0.f;
   std::lock_guard<std::recursive_mutex> lk(G.mu);
   G.tfs.insert(t);
   return t;
@@ -1108,15 +1198,17 @@ GXY_EXPORT int gxy_model_set_hit_output(void *model, int *g, int *p) {
 // the runtime's sample counter (counters[0]).
 static int trace_launch(TraceParams P, const RayList_ispc &R, int n, bool sampler) {
   if (n <= 0) return 0;
-  int nb = (n + GXY_BLOCK - 1) / GXY_BLOCK;
+  const bool pl = !sampler && G.trace_kind == 1 && P.nvv >= 1 && !P.has_model;
+  int nb = pl ? trace_pl_blocks(n, G.trace_chunk) : (n + GXY_BLOCK - 1) / GXY_BLOCK;
   unsigned long long *part = (unsigned long long *)dalloc(sizeof(unsigned long long) * nb);
   if (!part) return -1;
   P.sample_partials = part;
   {
     KTimer kt(sampler ? "sampler" : "trace");
-    HIPCHK(sampler ? launch_sampler(P, R, n, S()) : launch_trace(P, R, n, S()));
+    if (pl) HIPCHK(launch_trace_pl(P, R, n, G.trace_chunk, G.trace_refill, S()));
+    else HIPCHK(sampler ? launch_sampler(P, R, n, S()) : launch_trace(P, R, n, G.trace_wpe, S()));
   }
-  HIPCHK(launch_reduce_add(part, nb, G.counters, S()));
+  HIPCHK(launch_reduce_add(part, nb, CX().counters, S()));
   dfree_stream_ordered(part);
   return 0;
 }
@@ -1657,7 +1749,7 @@ static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
 }
 
 static int run_loop(LoopCtx &C, int n, const RayList_ispc &rays, RayListObj *adopt = nullptr) {
-  HIPCHK(hipMemsetAsync(G.counters, 0, 8 * sizeof(unsigned long long), S()));
+  HIPCHK(hipMemsetAsync(CX().counters, 0, 8 * sizeof(unsigned long long), S()));
   if (adopt && n <= C.maxlist) {
     // the caller's list is processed in place (it is the loop's to consume)
     C.q.push_back({adopt, 3});
@@ -1681,7 +1773,7 @@ static int run_loop(LoopCtx &C, int n, const RayList_ispc &rays, RayListObj *ado
     }
   }
   unsigned long long cnt[8];
-  HIPCHK(hipMemcpyAsync(cnt, G.counters, sizeof cnt, hipMemcpyDeviceToHost, S()));
+  HIPCHK(hipMemcpyAsync(cnt, CX().counters, sizeof cnt, hipMemcpyDeviceToHost, S()));
   HIPCHK(hipStreamSynchronize(S()));
   C.st.samples = (long long)cnt[0];
   return 0;

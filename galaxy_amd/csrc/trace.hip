@@ -231,13 +231,13 @@ __device__ void geom_shade(const DModel &M, const GeoHit &gh, f3 org, f3 dir, Hi
     }
     if (S.has_data && S.has_tf) {
       float dv = b0 * S.vdata[i0] + b1 * S.vdata[i1] + b2 * S.vdata[i2];
-      col = tf_color(S.tf.col, S.tf.nc, S.tf.lo, S.tf.hi, dv);
+      col = tf_color(S.tf.col, S.tf, dv);
     }
   } else {
     f3 c = mk3(S.centers[3 * gh.prim], S.centers[3 * gh.prim + 1], S.centers[3 * gh.prim + 2]);
     Ng = sub3(P, c);
     Ns = Ng;
-    if (S.has_tf && S.has_data) col = tf_color(S.tf.col, S.tf.nc, S.tf.lo, S.tf.hi, S.vdata[gh.prim]);
+    if (S.has_tf && S.has_data) col = tf_color(S.tf.col, S.tf, S.vdata[gh.prim]);
   }
   Ng = normalize3(Ng);
   Ns = normalize3(Ns);
@@ -291,8 +291,11 @@ __device__ __forceinline__ bool opaque_of(float r, float g, float b, float a, fl
   return mnf(mnf(r, g), b) >= 1.0f || a > thr;
 }
 
-template <int NV, bool GEOM>
-__global__ __launch_bounds__(GXY_BLOCK) void k_trace(TraceParams P, RayList_ispc R, int n) {
+// WPE > 0 asks the compiler for at least WPE waves per SIMD (fewer VGPRs);
+// the autotuner picks the variant.
+template <int NV, bool GEOM, int WPE>
+__global__ __launch_bounds__(GXY_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, 8)))
+void k_trace(TraceParams P, RayList_ispc R, int n) {
   constexpr int NVS = NV > 0 ? NV : 1;
   __shared__ float s_col[NVS][3 * GXY_TF_MAX];
   __shared__ float s_op[NVS][GXY_TF_MAX];
@@ -357,7 +360,7 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_trace(TraceParams P, RayList_ispc
       for (int major = 0; major < NV; major++)
         if (major == vid) {
           hit.sample = vol_sample(P.vv[major].vol, hit.point);
-          hit.color = tf_color(s_col[major], P.vv[major].tf.nc, P.vv[major].tf.lo, P.vv[major].tf.hi, hit.sample);
+          hit.color = tf_color(s_col[major], P.vv[major].tf, hit.sample);
           hit.opacity = 1.0f;
         }
     }
@@ -429,18 +432,18 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_trace(TraceParams P, RayList_ispc
             if (P.vv[major].dvr) {
               const DTF &T = P.vv[major].tf;
               float sVolume = (sLast[major] + sThis[major]) / 2;
-              float sop = tf_opacity(s_op[major], T.no, T.lo, T.hi, sVolume);
+              float sop = tf_opacity(s_op[major], T.no, T.lo, T.hi, T.so, sVolume);
               if (sop > 0) {
                 if (shadeFlag) {
-                  f3 sc = tf_color(s_col[major], T.nc, T.lo, T.hi, sVolume);
-                  float wo = clamp01f(sop / P.vv[major].vol.rate);
+                  f3 sc = tf_color(s_col[major], T, sVolume);
+                  float wo = clamp01f(sop * P.vv[major].vol.irate);
                   float a = 1.0f - ca;
                   cr = cr + a * (wo * sc.x);
                   cg = cg + a * (wo * sc.y);
                   cb = cb + a * (wo * sc.z);
                   ca = ca + a * wo;
                 } else {
-                  float wop = ((tThis - tLast) / step) * clamp01f(sop / P.vv[major].vol.rate);
+                  float wop = ((tThis - tLast) * P.istep) * clamp01f(sop * P.vv[major].vol.irate);
                   float f = 1.0f - wop;
                   cr *= f; cg *= f; cb *= f; ca *= f;
                 }
@@ -469,8 +472,7 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_trace(TraceParams P, RayList_ispc
             if (major == iso_vid) {
               hit.normal = safe_normalize3(vol_gradient(P.vv[major].vol, hit.point));
               if (dot3(dir, hit.normal) > 0) hit.normal = neg3(hit.normal);
-              hit.color = tf_color(s_col[major], P.vv[major].tf.nc, P.vv[major].tf.lo, P.vv[major].tf.hi,
-                                   hit.sample);
+              hit.color = tf_color(s_col[major], P.vv[major].tf, hit.sample);
               hit.opacity = 1.0f;
             }
         }
@@ -493,6 +495,287 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_trace(TraceParams P, RayList_ispc
     R.term[i] = term;
   }
   if (P.sample_partials) block_sum_store(nsamp, P.sample_partials);
+}
+
+// ------------------------------------------------------------- persistent-lane march
+// The same per-ray computation as k_trace (volumes, no geometry), organised so
+// that a lane whose ray has finished takes the next ray of its wave's chunk
+// instead of idling until the wave's longest ray is done.  Each wave owns
+// `chunk` consecutive rays (camera tiles stay together); idle lanes are
+// refilled once at least `refill` of them are idle (or all are), with ray
+// indices handed out by a wave-wide ballot + popcount (no atomics).
+struct MarchState {
+  f3 org, dir;
+  float tEntry, tExitVolume, tTimeout, tTerm, tThis, tLast;
+  float cr, cg, cb, ca;
+  float hit_t, hit_sample;
+  int i;        // ray index, -1 when idle
+  int flags;    // 1 shade, 2 surface_hit, 4 hit_iso, 8 opaque-surface (hit.opacity 1)
+  int iso_vid;
+};
+
+template <int NV>
+__device__ __forceinline__ void pl_finish(const TraceParams &P, const RayList_ispc &R, MarchState &m,
+                                          const float (*s_col)[3 * GXY_TF_MAX]) {
+  const int i = m.i;
+  float ray_t = m.tTerm;
+  if (m.flags & 4) {
+    // LookForIsoHit shading (TraceRays.ispc:284-307)
+    if (m.flags & 1) {
+      f3 pt = add3(m.org, scl3(m.hit_t, m.dir));
+      f3 nrm = mk3(0.f, 0.f, 0.f), col = nrm;
+#pragma unroll
+      for (int major = 0; major < NV; major++)
+        if (major == m.iso_vid) {
+          nrm = safe_normalize3(vol_gradient(P.vv[major].vol, pt));
+          if (dot3(m.dir, nrm) > 0) nrm = neg3(nrm);
+          col = tf_color(s_col[major], P.vv[major].tf, m.hit_sample);
+        }
+      R.sr[i] = col.x; R.sg[i] = col.y; R.sb[i] = col.z; R.so[i] = 1.0f;
+      R.nx[i] = nrm.x; R.ny[i] = nrm.y; R.nz[i] = nrm.z;
+      m.flags |= 8;
+    } else if (!(m.flags & 8)) {
+      // a non-primary iso hit with no earlier slice hit: zero colour/normal
+      R.sr[i] = 0.f; R.sg[i] = 0.f; R.sb[i] = 0.f; R.so[i] = 1.0f;
+      R.nx[i] = 0.f; R.ny[i] = 0.f; R.nz[i] = 0.f;
+    }
+  }
+  R.r[i] = m.cr; R.g[i] = m.cg; R.b[i] = m.cb; R.o[i] = m.ca;
+  int term = opaque_of(m.cr, m.cg, m.cb, m.ca, P.opaque_alpha) ? RAY_OPAQUE : 0;
+  R.t[i] = ray_t;
+  if (m.flags & 2) {
+    term |= RAY_SURFACE;
+    if (m.flags & 8) term |= RAY_OPAQUE;
+  } else if (m.tTerm == m.tExitVolume) {
+    term |= RAY_BOUNDARY;
+  } else if (m.tTerm == m.tTimeout) {
+    term |= RAY_TIMEOUT;
+  }
+  R.term[i] = term;
+  m.i = -1;
+}
+
+// Loads ray i and runs everything before the march (TraceRays.ispc:371-442);
+// returns false when the ray is already finished (no march needed).
+template <int NV>
+__device__ __forceinline__ bool pl_start(const TraceParams &P, const RayList_ispc &R, MarchState &m, int i,
+                                         float *sLast, const float (*s_col)[3 * GXY_TF_MAX]) {
+  m.i = i;
+  m.flags = R.type[i] == RAY_PRIMARY ? 1 : 0;
+  m.org = mk3(R.ox[i], R.oy[i], R.oz[i]);
+  f3 dir = mk3(R.dx[i], R.dy[i], R.dz[i]);
+  if (dir.x == 0.f) dir.x = 1e-6f;
+  if (dir.y == 0.f) dir.y = 1e-6f;
+  if (dir.z == 0.f) dir.z = 1e-6f;
+  m.dir = dir;
+  float ray_t0 = R.t[i];
+  float ray_t = R.tMax[i];
+  m.tTimeout = ray_t;
+  m.cr = R.r[i]; m.cg = R.g[i]; m.cb = R.b[i]; m.ca = R.o[i];
+  float tEntry, tExitVolume;
+  box_clip(m.org, dir, P.blo, P.bhi, tEntry, tExitVolume);
+  if (tEntry < ray_t0) tEntry = ray_t0;
+  else if (tEntry > ray_t0) ray_t0 = tEntry;
+  ray_t = mnf(ray_t, tExitVolume);
+  m.tEntry = tEntry;
+  m.tExitVolume = tExitVolume;
+  m.iso_vid = -1;
+  // LookForSliceHit (TraceRays.ispc:143-250); its outputs are written now and
+  // replaced at the end if an isosurface is hit later
+  bool sh = false;
+  int vid = -1;
+  f3 snrm = mk3(0.f, 0.f, 0.f);
+  float st = 0.f;
+#pragma unroll
+  for (int major = 0; major < NV; major++) {
+    const int ns = P.vv[major].nSlices;
+    for (int minor = 0; minor < ns; minor++) {
+      f3 pn = mk3(P.vv[major].slices[4 * minor], P.vv[major].slices[4 * minor + 1], P.vv[major].slices[4 * minor + 2]);
+      float d = P.vv[major].slices[4 * minor + 3];
+      float denom = dot3(dir, pn);
+      if (fabsf(denom) > 0.0001f) {
+        float t = (d - dot3(m.org, pn)) / denom;
+        if (t >= ray_t0 && t <= ray_t) {
+          snrm = denom > 0 ? neg3(pn) : pn;
+          st = t;
+          vid = major;
+          ray_t = t;
+          sh = true;
+        }
+      }
+    }
+  }
+  if (sh) {
+    f3 col = mk3(0.f, 0.f, 0.f);
+    if (m.flags & 1) {
+      f3 pt = add3(m.org, scl3(st, dir));
+#pragma unroll
+      for (int major = 0; major < NV; major++)
+        if (major == vid) {
+          float smp = vol_sample(P.vv[major].vol, pt);
+          col = tf_color(s_col[major], P.vv[major].tf, smp);
+        }
+    }
+    R.sr[i] = col.x; R.sg[i] = col.y; R.sb[i] = col.z; R.so[i] = 1.0f;
+    R.nx[i] = snrm.x; R.ny[i] = snrm.y; R.nz[i] = snrm.z;
+    m.flags |= 2 | 8;
+  }
+  m.tTerm = ray_t;
+  if (!P.integrate) return false;
+  m.tLast = tEntry + P.eps;
+  m.tThis = tEntry;
+#pragma unroll
+  for (int v = 0; v < NV; v++) sLast[v] = 0.f;
+  return m.tThis <= m.tTerm && !opaque_of(m.cr, m.cg, m.cb, m.ca, P.opaque_alpha);
+}
+
+// One march iteration (TraceRays.ispc:470-556); returns false when the ray is done.
+template <int NV>
+__device__ __forceinline__ bool pl_step(const TraceParams &P, MarchState &m, float *sLast, int &iter,
+                                        const float (*s_col)[3 * GXY_TF_MAX], const float (*s_op)[GXY_TF_MAX],
+                                        unsigned long long &nsamp) {
+  const float step = P.step, epsilon = P.eps;
+  float tThis = m.tThis;
+  float sThis[NV];
+  f3 coord = add3(m.org, scl3(tThis, m.dir));
+#pragma unroll
+  for (int v = 0; v < NV; v++) sThis[v] = vol_sample(P.vv[v].vol, coord);
+  nsamp += NV;
+  bool hit_iso = false;
+  if (tThis > m.tEntry && m.tLast >= epsilon) {
+    bool ih = false;
+    int ivid = -1;
+    float ht = 0.f, hs = 0.f;
+#pragma unroll
+    for (int major = 0; major < NV; major++) {
+      float sl = sLast[major], st = sThis[major];
+      const int ni = P.vv[major].nIso;
+      for (int k = 0; k < ni; k++) {
+        float isoval = P.vv[major].iso[k];
+        if (((isoval >= sl) && (isoval < st)) || ((isoval <= sl) && (isoval > st))) {
+          ih = true;
+          ivid = major;
+          ht = m.tLast + ((isoval - sl) / (st - sl)) * (tThis - m.tLast);
+          hs = isoval;
+        }
+      }
+    }
+    if (ih) {
+      m.iso_vid = ivid;
+      m.hit_t = ht;
+      m.hit_sample = hs;
+      m.tTerm = ht;
+      tThis = ht;
+      m.flags |= 2 | 4;
+      hit_iso = true;
+      f3 c2 = add3(m.org, scl3(tThis, m.dir));
+#pragma unroll
+      for (int v = 0; v < NV; v++) sThis[v] = vol_sample(P.vv[v].vol, c2);
+      nsamp += NV;
+    }
+#pragma unroll
+    for (int major = 0; major < NV; major++) {
+      if (P.vv[major].dvr) {
+        const DTF &T = P.vv[major].tf;
+        float sVolume = (sLast[major] + sThis[major]) / 2;
+        float sop = tf_opacity(s_op[major], T.no, T.lo, T.hi, T.so, sVolume);
+        if (sop > 0) {
+          if (m.flags & 1) {
+            f3 sc = tf_color(s_col[major], T, sVolume);
+            float wo = clamp01f(sop * P.vv[major].vol.irate);
+            float a = 1.0f - m.ca;
+            m.cr = m.cr + a * (wo * sc.x);
+            m.cg = m.cg + a * (wo * sc.y);
+            m.cb = m.cb + a * (wo * sc.z);
+            m.ca = m.ca + a * wo;
+          } else {
+            float wop = ((tThis - m.tLast) * P.istep) * clamp01f(sop * P.vv[major].vol.irate);
+            float f = 1.0f - wop;
+            m.cr *= f; m.cg *= f; m.cb *= f; m.ca *= f;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < NV; v++) sLast[v] = sThis[v];
+  bool opaque = opaque_of(m.cr, m.cg, m.cb, m.ca, P.opaque_alpha);
+  if (opaque) m.tTerm = tThis;
+  m.tLast = tThis;
+  float nxt = (tThis == m.tEntry) ? (m.tEntry + epsilon)
+              : (((tThis + step) > m.tTerm) && (tThis < m.tTerm)) ? m.tTerm : tThis + step;
+  if (!(nxt > tThis) && !opaque && !hit_iso) return false;
+  m.tThis = nxt;
+  if (++iter >= GXY_MAX_MARCH) return false;
+  return nxt <= m.tTerm && !opaque && !hit_iso;
+}
+
+template <int NV>
+__global__ __launch_bounds__(GXY_BLOCK) void k_trace_pl(TraceParams P, RayList_ispc R, int n, int chunk,
+                                                        int refill) {
+  __shared__ float s_col[NV][3 * GXY_TF_MAX];
+  __shared__ float s_op[NV][GXY_TF_MAX];
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    for (int k = threadIdx.x; k < 3 * P.vv[v].tf.nc; k += GXY_BLOCK) s_col[v][k] = P.vv[v].tf.col[k];
+    for (int k = threadIdx.x; k < P.vv[v].tf.no; k += GXY_BLOCK) s_op[v][k] = P.vv[v].tf.op[k];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  const int wave = (blockIdx.x * GXY_BLOCK + threadIdx.x) >> 6;
+  const long long start = (long long)wave * chunk;
+  const long long end = start + chunk < n ? start + chunk : n;
+  long long next = start;
+  unsigned long long nsamp = 0;
+  MarchState m;
+  m.i = -1;
+  float sLast[NV];
+  int iter = 0;
+  for (;;) {
+    unsigned long long idle = __ballot(m.i < 0);
+    const int nidle = __popcll(idle);
+    if (nidle == 64 && next >= end) break;
+    if (next < end && (nidle >= refill || nidle == 64)) {
+      if (m.i < 0) {
+        long long j = next + __popcll(idle & lt);
+        if (j < end) {
+          iter = 0;
+          if (!pl_start<NV>(P, R, m, (int)j, sLast, s_col)) pl_finish<NV>(P, R, m, s_col);
+        }
+      }
+      next += nidle;
+      continue;
+    }
+    if (m.i >= 0) {
+      if (!pl_step<NV>(P, m, sLast, iter, s_col, s_op, nsamp)) pl_finish<NV>(P, R, m, s_col);
+    }
+  }
+  if (P.sample_partials) block_sum_store(nsamp, P.sample_partials);
+}
+
+template <int NV>
+static hipError_t launch_trace_pl_nv(const TraceParams &P, const RayList_ispc &R, int n, int chunk, int refill,
+                                     hipStream_t s) {
+  int waves = (n + chunk - 1) / chunk;
+  dim3 g((waves + 3) / 4), b(GXY_BLOCK);
+  hipLaunchKernelGGL(k_trace_pl<NV>, g, b, 0, s, P, R, n, chunk, refill);
+  return hipGetLastError();
+}
+
+int trace_pl_blocks(int n, int chunk) { return ((n + chunk - 1) / chunk + 3) / 4; }
+
+hipError_t launch_trace_pl(const TraceParams &P, const RayList_ispc &R, int n, int chunk, int refill,
+                           hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (chunk < 64 || refill < 1 || refill > 64) return hipErrorInvalidValue;
+  switch (P.nvv) {
+    case 1: return launch_trace_pl_nv<1>(P, R, n, chunk, refill, s);
+    case 2: return launch_trace_pl_nv<2>(P, R, n, chunk, refill, s);
+    case 3: return launch_trace_pl_nv<3>(P, R, n, chunk, refill, s);
+    case 4: return launch_trace_pl_nv<4>(P, R, n, chunk, refill, s);
+  }
+  return hipErrorInvalidValue;
 }
 
 // ------------------------------------------------------------- sampler kernel
@@ -553,21 +836,23 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_sampler(TraceParams P, RayList_is
 
 // ------------------------------------------------------------- launchers
 template <int NV>
-static hipError_t launch_trace_nv(const TraceParams &P, const RayList_ispc &R, int n, hipStream_t s) {
+static hipError_t launch_trace_nv(const TraceParams &P, const RayList_ispc &R, int n, int wpe, hipStream_t s) {
   dim3 g((n + GXY_BLOCK - 1) / GXY_BLOCK), b(GXY_BLOCK);
-  if (P.has_model) hipLaunchKernelGGL((k_trace<NV, true>), g, b, 0, s, P, R, n);
-  else hipLaunchKernelGGL((k_trace<NV, false>), g, b, 0, s, P, R, n);
+  if (P.has_model) hipLaunchKernelGGL((k_trace<NV, true, 0>), g, b, 0, s, P, R, n);
+  else if (wpe == 5) hipLaunchKernelGGL((k_trace<NV, false, 5>), g, b, 0, s, P, R, n);
+  else if (wpe == 6) hipLaunchKernelGGL((k_trace<NV, false, 6>), g, b, 0, s, P, R, n);
+  else hipLaunchKernelGGL((k_trace<NV, false, 0>), g, b, 0, s, P, R, n);
   return hipGetLastError();
 }
 
-hipError_t launch_trace(const TraceParams &P, const RayList_ispc &R, int n, hipStream_t s) {
+hipError_t launch_trace(const TraceParams &P, const RayList_ispc &R, int n, int wpe, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   switch (P.nvv) {
-    case 0: return launch_trace_nv<0>(P, R, n, s);
-    case 1: return launch_trace_nv<1>(P, R, n, s);
-    case 2: return launch_trace_nv<2>(P, R, n, s);
-    case 3: return launch_trace_nv<3>(P, R, n, s);
-    case 4: return launch_trace_nv<4>(P, R, n, s);
+    case 0: return launch_trace_nv<0>(P, R, n, wpe, s);
+    case 1: return launch_trace_nv<1>(P, R, n, wpe, s);
+    case 2: return launch_trace_nv<2>(P, R, n, wpe, s);
+    case 3: return launch_trace_nv<3>(P, R, n, wpe, s);
+    case 4: return launch_trace_nv<4>(P, R, n, wpe, s);
   }
   return hipErrorInvalidValue;
 }

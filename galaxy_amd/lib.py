@@ -27,6 +27,9 @@ def _declare(L):
         "gxy_synchronize": (C.c_int, []),
         "gxy_set_stream": (C.c_int, [vp]),
         "gxy_get_stream": (vp, []),
+        "gxy_context_create": (vp, []),
+        "gxy_context_bind": (C.c_int, [vp]),
+        "gxy_context_destroy": (C.c_int, [vp]),
         "gxy_device_alloc": (vp, [C.c_size_t]),
         "gxy_device_free": (C.c_int, [vp]),
         "gxy_copy_to_device": (C.c_int, [vp, vp, C.c_size_t]),
@@ -37,6 +40,8 @@ def _declare(L):
         "gxy_device_bytes_total": (C.c_size_t, []),
         "gxy_set_opacity_threshold": (C.c_int, [C.c_float]),
         "gxy_get_opacity_threshold": (C.c_float, []),
+        "gxy_set_trace_config": (C.c_int, [C.c_int, C.c_int, C.c_int]),
+        "gxy_get_trace_config": (C.c_int, [vp, vp, vp]),
         "gxy_timing_enable": (C.c_int, [C.c_int]),
         "gxy_timing_reset": (C.c_int, []),
         "gxy_timing_get": (C.c_int, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),

@@ -154,6 +154,13 @@ const char *gxy_last_error(void);
 int         gxy_device_count(void);
 int         gxy_synchronize(void);
 int         gxy_set_stream(void *hip_stream); /* NULL restores the library stream */
+/* Execution contexts: each has its own HIP stream (default priority), device
+ * counters and block cache.  A context bound to a host thread carries all of
+ * that thread's work, so several threads can run frames concurrently on one
+ * GPU (the serving loop does this).  NULL unbinds (back to the default). */
+void       *gxy_context_create(void);
+int         gxy_context_bind(void *ctx);
+int         gxy_context_destroy(void *ctx);
 void       *gxy_get_stream(void);
 void       *gxy_device_alloc(size_t bytes);   /* cached pool; 256-byte aligned */
 int         gxy_device_free(void *p);
@@ -167,6 +174,14 @@ size_t      gxy_device_bytes_in_use(void);
  * setting here (default 0.999, the reference value). */
 int         gxy_set_opacity_threshold(float alpha);
 float       gxy_get_opacity_threshold(void);
+/* Launch configuration of the volume-only trace (no reference counterpart):
+ * kind 0 = one ray per work-item; 15 / 16 = the same kernel compiled for at
+ * least 5 / 6 waves per SIMD; kind 1 = persistent lanes, each wave owning
+ * `chunk` consecutive rays (multiple of 64) and refilling idle lanes once
+ * `refill` (1..64) of them are idle.  Results are identical for every setting;
+ * galaxy_amd/autotune.py picks the fastest. */
+int         gxy_set_trace_config(int kind, int chunk, int refill);
+int         gxy_get_trace_config(int *kind, int *chunk, int *refill);
 size_t      gxy_device_bytes_total(void);
 
 /* Kernel timing: every launch records HIP events on its stream when enabled.

@@ -106,6 +106,7 @@ struct OVolume {
   float org[3], sp[3], isp[3], ub[3];
   const float *data;
   float samplingStep, samplingRate;
+  float irate;  // 1 / samplingRate: the opacity correction multiplies by it
 };
 
 inline float vol_sample(const OVolume &V, v3 p) {
@@ -144,11 +145,19 @@ inline v3 vol_gradient(const OVolume &V, v3 p) {
 
 // ------------------------------------------------------------------ TF
 // OSPRay LinearTransferFunction (piecewise_linear): valueRange clamps to the end
-// entries; inside, x = (v-lo)/(hi-lo)*(n-1), i = floor(x), lerp(i, i+1).
+// entries; inside, x = (v-lo)/(hi-lo)*(n-1), i = floor(x), lerp(i, i+1).  The
+// division is taken as a multiplication by the precomputed (n-1)/(hi-lo), as
+// ISPC's fast-math turns it into a reciprocal multiply.
 struct OTF {
   std::vector<float> col;  // nc*3
   std::vector<float> op;   // no
   float lo, hi;
+  float sc, so;            // (nc-1)/(hi-lo), (no-1)/(hi-lo)
+  void scales() {
+    int nc = (int)col.size() / 3, no = (int)op.size();
+    sc = nc > 1 ? (float)(nc - 1) / (hi - lo) : 0.f;
+    so = no > 1 ? (float)(no - 1) / (hi - lo) : 0.f;
+  }
 };
 
 inline v3 tf_color(const OTF &T, float v) {
@@ -157,8 +166,9 @@ inline v3 tf_color(const OTF &T, float v) {
   if (n == 0) return mk(1.f, 1.f, 1.f);
   if (v <= T.lo) return mk(T.col[0], T.col[1], T.col[2]);
   if (v >= T.hi) return mk(T.col[3 * n - 3], T.col[3 * n - 2], T.col[3 * n - 1]);
-  float x = (v - T.lo) / (T.hi - T.lo) * ((float)n - 1.0f);
-  int i = (int)std::floor(x);
+  float x = (v - T.lo) * T.sc;
+  int i = (int)x;
+  i = i < n - 1 ? i : n - 1;
   float r = x - (float)i;
   int j = i + 1 < n - 1 ? i + 1 : n - 1;
   float q = 1.0f - r;
@@ -172,8 +182,9 @@ inline float tf_opacity(const OTF &T, float v) {
   if (n == 0) return 1.f;
   if (v <= T.lo) return T.op[0];
   if (v >= T.hi) return T.op[n - 1];
-  float x = (v - T.lo) / (T.hi - T.lo) * ((float)n - 1.0f);
-  int i = (int)std::floor(x);
+  float x = (v - T.lo) * T.so;
+  int i = (int)x;
+  i = i < n - 1 ? i : n - 1;
   float r = x - (float)i;
   int j = i + 1 < n - 1 ? i + 1 : n - 1;
   return (1.0f - r) * T.op[i] + r * T.op[j];
@@ -205,6 +216,7 @@ void tf_resample(OTF &T, const float *cmap, int nc, const float *omap, int no, c
   }
   if (range) { T.lo = range[0]; T.hi = range[1]; }
   else { T.lo = cmap[0]; T.hi = cmap[4 * (nc - 1)]; }
+  T.scales();
 }
 
 // ------------------------------------------------------------------ geometry
@@ -665,6 +677,7 @@ void trace_one(const Visualization_ispc &vis, const std::vector<OVolumeVis> &vv,
 
   if (integrate) {
     const int nv = (int)vv.size();
+    const float istep = 1.0f / step;
     float sLast[GXY_MAX_VOLS_ORACLE], sThis[GXY_MAX_VOLS_ORACLE];
     float tLast = tEntry + epsilon, tThis;
     bool hit_isosurface = false;
@@ -713,7 +726,7 @@ void trace_one(const Visualization_ispc &vis, const std::vector<OVolumeVis> &vv,
             if (sampleOpacity > 0) {
               if (shadeFlag) {
                 v3 sc = tf_color(*V.tf, sVolume);
-                float wo = clamp01(sampleOpacity / V.vol->samplingRate);
+                float wo = clamp01(sampleOpacity * V.vol->irate);
                 Acc a = (Acc)1 - color.w;
                 Acc w = (Acc)wo;
                 color.x = color.x + a * (w * (Acc)sc.x);
@@ -721,7 +734,7 @@ void trace_one(const Visualization_ispc &vis, const std::vector<OVolumeVis> &vv,
                 color.z = color.z + a * (w * (Acc)sc.z);
                 color.w = color.w + a * w;
               } else {
-                float wop = ((tThis - tLast) / step) * clamp01(sampleOpacity / V.vol->samplingRate);
+                float wop = ((tThis - tLast) * istep) * clamp01(sampleOpacity * V.vol->irate);
                 Acc f = (Acc)1 - (Acc)wop;
                 color.x *= f; color.y *= f; color.z *= f; color.w *= f;
               }
@@ -889,6 +902,7 @@ OEXPORT void *oracle_volume_create(const int dims[3], const float origin[3], con
   v->data = voxels;
   v->samplingStep = std::min(spacing[0], std::min(spacing[1], spacing[2]));
   v->samplingRate = samplingRate;
+  v->irate = 1.0f / samplingRate;
   return v;
 }
 OEXPORT void oracle_volume_destroy(void *v) { delete (OVolume *)v; }
@@ -909,6 +923,7 @@ OEXPORT void *oracle_tf_create(const float *colors, int nc, const float *op, int
   t->op.assign(op, op + no);
   t->lo = lo;
   t->hi = hi;
+  t->scales();
   return t;
 }
 OEXPORT void oracle_tf_destroy(void *t) { delete (OTF *)t; }

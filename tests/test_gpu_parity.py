@@ -319,3 +319,29 @@ def test_raylist_block_roundtrip(hip):
     assert np.array_equal(got[:17], hr["dx"][100:117])
     for h in (h2, h3):
         L_.gxy_raylist_destroy(h)
+
+
+def test_every_trace_launch_configuration_gives_identical_results(radial_vols_small, hip):
+    """The autotuner's candidates change only how rays map onto lanes, never
+    the per-ray arithmetic: every configuration must give bit-identical rays."""
+    from galaxy_amd import autotune, lib as glib
+    L_ = glib.lib()
+    sc = _dvr_scene(radial_vols_small, 2)
+    Bh = scenes.build_scene(hip, sc)
+    hr = random_rays(50000, seed=77, tmax=1e30)
+    ref = None
+    saved = (C.c_int(), C.c_int(), C.c_int())
+    L_.gxy_get_trace_config(*[C.byref(x) for x in saved])
+    try:
+        for cfg in autotune.DEFAULT_CANDIDATES:
+            glib.check(L_.gxy_set_trace_config(*cfg))
+            d = glib.DeviceRays.from_host(hr)
+            hip.trace(Bh.vis, d)
+            g = d.download()
+            if ref is None:
+                ref = g
+                continue
+            for f in abi.FIELDS:
+                assert np.array_equal(g[f], ref[f]), (cfg, f)
+    finally:
+        L_.gxy_set_trace_config(*[x.value for x in saved])
