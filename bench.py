@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X ray-processing path.
 
-Headline workload (BASELINE configs[1]): one frame of the volume render,
+Headline workload: one frame of the volume render,
 1920x1080 primary rays through a 512^3 FP32 noise volume (DVR with an 8-point
 TF, 0.5-voxel step, early termination at alpha 0.99, isosurface 0.5) plus the
 shadow rays the isosurface hits spawn, classified and accumulated into the
@@ -9,8 +9,11 @@ framebuffer.  A step is one whole frame: camera rays, every trace pass, spawn,
 lighting, routing and accumulation, all on the GPU with the volume resident in
 HBM.  value = ray passes (primary + shadow) / second; Gsamples/s beside it.
 
-Also measured, as extra fields: the sphere-mesh frame and the
-nine-icosphere AO frame.
+Also measured, as extra fields: the sphere-mesh frame and the nine-icosphere
+AO frame; the batched form (256 ray lists over 64 resident datasets, one
+persistent launch per pass) against the per-list loop; the served form (a
+Poisson request stream with continuous batching, p50/p99 latency) against
+serving one request at a time.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra]
 """
@@ -158,6 +161,138 @@ def cpu_baseline(hip, sc, cam, alpha, target_s=15.0):
                                                                platform.processor() or "x86_64"))
 
 
+def release(L, built):
+    """Frees the device objects of a built scene (each destroy call checks the
+    handle's kind and ignores handles of other kinds)."""
+    for h in getattr(built, "handles", []):
+        for fn in (L.gxy_volume_destroy, L.gxy_tf_destroy, L.gxy_model_destroy):
+            fn(h)
+    built.handles = []
+
+
+def bvh_depth(L, built):
+    if not built.vis.model:
+        return 0
+    nodes, leaves, dp, sah = C.c_int(), C.c_int(), C.c_int(), C.c_float()
+    L.gxy_model_bvh_stats(built.vis.model, C.byref(nodes), C.byref(leaves), C.byref(dp), C.byref(sah))
+    return dp.value
+
+
+def bench_batched(L, hip, steps, tune=True):
+    """256 ray lists (1k-1M rays, log-uniform) over 64 resident datasets (blob
+    meshes of 50k-5M triangles, 128^3-1024^3 noise volumes), shadow + 2 AO rays
+    per surface hit: the batched form (one persistent launch per pass, passes
+    captured in a HIP graph) against the per-list loop over the same lists."""
+    from galaxy_amd import abi, batch as gb, lib as glib, scenes, workloads
+    descs, lists, Ld = workloads.c4()
+    log("building c4: %d datasets, %d lists, %d rays" % (len(descs), len(lists), sum(n for n, _, _ in lists)))
+    t = time.perf_counter()
+    built = [scenes.build_scene(hip, workloads.dataset_scene(d)) for d in descs]
+    glib.check(L.gxy_synchronize())
+    build_s = time.perf_counter() - t
+    B = gb.Batch(256)
+    ids = [B.add_dataset(b) for b in built]
+    keep = []
+    lights = scenes.make_lighting(Ld, keep)
+    B.set_lighting(lights)
+    rays = []
+    for k, (n, ds, seed) in enumerate(lists):
+        d = glib.DeviceRays(n)
+        l, fb = B.add_render_list(ids[ds], d, n, 1)
+        B.set_seed(l, seed)
+        rays.append((l, d, fb, n, ds, seed))
+    depth = max(bvh_depth(L, b) for b in built)
+    res = dict(lists=len(lists), datasets=len(descs), primary_rays=sum(n for n, _, _ in lists),
+               triangles=int(sum(x[1] for x in descs if x[0] == "mesh")),
+               volume_voxels=int(sum(x[1] ** 3 for x in descs if x[0] == "volume")), max_bvh_depth=depth,
+               build_s=round(build_s, 2))
+    if tune:
+        best, table = gb.tune(B, max_depth=depth, graph_passes=2)
+        res["tuned"] = dict(best=list(best), ms=[[list(c), round(m, 2)] for c, m in table])
+        log("c4 tuned", res["tuned"])
+    st = B.run(2)                      # warm (graph captured)
+    glib.check(L.gxy_synchronize())
+    t = time.perf_counter()
+    tot = dict(rays=0, samples=0)
+    for _ in range(steps):
+        st = B.run(2)
+        tot["rays"] += st.rays
+        tot["samples"] += st.samples
+    glib.check(L.gxy_synchronize())
+    dt = time.perf_counter() - t
+    res.update(mrays_per_s=round(tot["rays"] / dt / 1e6, 2), gsamples_per_s=round(tot["samples"] / dt / 1e9, 3),
+               ms_per_batch=round(dt / steps * 1e3, 3), ray_passes_per_batch=int(st.rays),
+               spawned_per_batch=int(st.spawned), passes=int(st.passes), steps=steps,
+               config=list(B.config()), pool_bytes=int(B.nbytes()))
+    batched_fb = [fb.to_host(np.float32) for (_, _, fb, _, _, _) in rays[:16]]
+    # the per-list loop over the same lists, one after another
+    nb = np.full(6, -1, np.int32)
+    fs = abi.FrameStats()
+    prays = psamples = 0
+    lo, hi = glib.a3((0, 0, 0)), glib.a3((1, 1, 1))
+    glib.check(L.gxy_synchronize())
+    t = time.perf_counter()
+    for (l, d, fb, n, ds, seed) in rays:
+        glib.check(L.gxy_rays_random(C.byref(d.ispc), n, seed, n, 1, lo.ctypes.data, hi.ctypes.data))
+        glib.check(L.gxy_memset(fb.ptr, 0, fb.nbytes))
+        glib.check(L.gxy_process_raylist(C.byref(built[ds].vis), C.byref(lights), n, C.byref(d.ispc),
+                                          nb.ctypes.data, 0.001, 1 << 30, fb.ptr, n, 1, C.byref(fs)))
+        prays += fs.ray_passes
+        psamples += fs.samples
+    glib.check(L.gxy_synchronize())
+    pdt = time.perf_counter() - t
+    diff = max(float(np.abs(fb.to_host(np.float32) - a).max()) for (_, _, fb, _, _, _), a in zip(rays, batched_fb))
+    res["per_list_loop"] = dict(mrays_per_s=round(prays / pdt / 1e6, 2), ms=round(pdt * 1e3, 2),
+                                ray_passes=int(prays), same_ray_passes=bool(prays == st.rays),
+                                same_samples=bool(psamples == st.samples), max_abs_fb_diff_first16=diff)
+    res["speedup_vs_per_list_loop"] = round(res["mrays_per_s"] / max(1e-9, res["per_list_loop"]["mrays_per_s"]), 3)
+    B.destroy()
+    for b in built:
+        release(L, b)
+    return res
+
+
+def bench_served(L, hip, nreq, seed=5):
+    """Poisson stream (200 requests/s) of render requests (256^2 to 3840x2160)
+    and sampler requests (1M-50M rays) over resident 512^3-1024^3 noise volumes
+    and 1M-20M triangle blob meshes, admitted under an HBM budget and served
+    with continuous batching; the same stream served one request at a time
+    through the per-list loop beside it."""
+    from galaxy_amd import abi, lib as glib, serve, workloads
+    descs = [("volume", 512, 11), ("volume", 768, 12), ("volume", 1024, 13),
+             ("mesh", 1_000_000, 21), ("mesh", 5_000_000, 22), ("mesh", 20_000_000, 23)]
+    log("building c5 datasets")
+    t = time.perf_counter()
+    S, built = serve.build_server(hip, [workloads.dataset_scene(d) for d in descs], budget=160 << 30, slots=256)
+    glib.check(L.gxy_synchronize())
+    build_s = time.perf_counter() - t
+    rng = np.random.default_rng(seed)
+    reqs, _ = serve.request_stream(nreq, 200.0, 0.5, rng, volume_ds=(0, 1, 2), any_ds=(0, 1, 2, 3, 4, 5))
+    nrender = sum(1 for r in reqs if r.kind == 0)
+    res = dict(requests=nreq, render=nrender, sample=nreq - nrender, arrival_rate=200.0,
+               arrival_span_s=round(reqs[nreq - 1].arrival, 3), budget_bytes=160 << 30, build_s=round(build_s, 2),
+               datasets=["%s %d" % (k, s) for k, s, _ in descs])
+    # warm-up: one request of each kind
+    w = (abi.Request * 2)()
+    w[0] = serve.render_request(0, 256, 256, (0.5, 0.5, 2.4))
+    w[1] = serve.sample_request(0, 100000, 1, 200000)
+    S.run(w, abi.SERVE_CONTINUOUS)
+    for mode, name in ((abi.SERVE_CONTINUOUS, "continuous"), (abi.SERVE_ONE_AT_A_TIME, "one_at_a_time")):
+        log("serving", name)
+        r, st = S.run(reqs, mode)
+        d = serve.latency_summary(reqs, r)
+        d.update(mrays_per_s=round(st.rays / st.wall / 1e6, 2), gsamples_per_s=round(st.samples / st.wall / 1e9, 3),
+                 wall_s=round(st.wall, 3), ray_passes=int(st.rays), passes=int(st.passes),
+                 max_in_flight=int(st.max_in_flight), peak_bytes=int(st.peak_bytes))
+        res[name] = d
+    res["throughput_ratio"] = round(res["continuous"]["mrays_per_s"] / max(1e-9, res["one_at_a_time"]["mrays_per_s"]),
+                                    3)
+    S.destroy()
+    for b in built:
+        release(L, b)
+    return res
+
+
 def traffic_from_profile(name):
     """HBM bytes per trace launch measured by rocprofv3 --pmc (FETCH_SIZE and
     WRITE_SIZE, copy-kernel calibrated) and committed under profiles/."""
@@ -181,6 +316,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--c3-steps", type=int, default=3)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--serve-requests", type=int, default=160)
     ap.add_argument("--no-tune", action="store_true", help="keep the default trace launch configuration")
     ap.add_argument("--trace-config", default=None,
                     help="kind,chunk,refill: use this trace launch configuration (implies --no-tune)")
@@ -296,6 +435,18 @@ def main():
                 extra[name] = dict(error=str(e))
         out["extra_workloads"] = extra
         glib.check(L.gxy_set_opacity_threshold(alpha))
+
+    # ---------------------------------------------------------------- batched and served forms
+    if not args.no_batched:
+        try:
+            out["batched"] = bench_batched(L, hip, args.batch_steps)
+        except Exception as e:
+            out["batched"] = dict(error=str(e))
+    if not args.no_served:
+        try:
+            out["served"] = bench_served(L, hip, args.serve_requests)
+        except Exception as e:
+            out["served"] = dict(error=str(e))
 
     # ---------------------------------------------------------------- CPU baseline
     if not args.no_cpu:

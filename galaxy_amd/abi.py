@@ -67,6 +67,41 @@ class FrameStats(C.Structure):
         return d
 
 
+class BatchStats(C.Structure):
+    _fields_ = [("rays", C.c_ulonglong), ("samples", C.c_ulonglong), ("spawned", C.c_ulonglong),
+                ("terminated", C.c_ulonglong), ("dropped", C.c_ulonglong), ("kept", C.c_ulonglong),
+                ("passes", C.c_int), ("overflow", C.c_int)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+BATCH_RENDER, BATCH_SAMPLE = 0, 1
+
+
+class Request(C.Structure):
+    _fields_ = [("kind", C.c_int), ("dataset", C.c_int), ("width", C.c_int), ("height", C.c_int),
+                ("nrays", C.c_longlong), ("seed", C.c_uint), ("arrival", C.c_double),
+                ("eye", C.c_float * 3), ("center", C.c_float * 3), ("up", C.c_float * 3), ("aov", C.c_float),
+                ("out", C.c_void_p), ("capacity", C.c_longlong)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("admitted", C.c_double), ("finished", C.c_double), ("particles", C.c_longlong),
+                ("rays", C.c_longlong), ("status", C.c_int), ("pad_", C.c_int)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("wall", C.c_double), ("rays", C.c_ulonglong), ("samples", C.c_ulonglong),
+                ("peak_bytes", C.c_longlong), ("passes", C.c_int), ("max_in_flight", C.c_int)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+SERVE_CONTINUOUS, SERVE_ONE_AT_A_TIME = 0, 1
+
+
 def make_box(lo, hi):
     b = box3f()
     for k in range(3):

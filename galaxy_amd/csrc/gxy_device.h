@@ -139,6 +139,16 @@ __device__ __forceinline__ f3 safe_normalize3(f3 v) {
   return scl3(s, v);
 }
 
+// A pointer read from device memory (rather than passed as a kernel argument)
+// is a generic pointer, so its loads would be flat loads with a 64-bit address
+// per lane; the round trip through the global address space lets the compiler
+// use global loads with a scalar base instead.  Every such pointer here points
+// into device memory allocated with hipMalloc.
+template <class T>
+__device__ __forceinline__ T *gptr(T *p) {
+  return (T *)(__attribute__((address_space(1))) T *)p;
+}
+
 // Unaligned (4-byte aligned) pair load: one global_load_dwordx2 for two
 // x-adjacent voxels.
 struct __attribute__((packed, aligned(4))) float2u { float x, y; };
@@ -159,11 +169,12 @@ __device__ __forceinline__ float vol_sample(const DVolume &V, f3 p) {
   int i0 = (int)l0, i1 = (int)l1, i2 = (int)l2;
   float f0 = l0 - (float)i0, f1 = l1 - (float)i1, f2 = l2 - (float)i2;
   const float *d;
+  const float *vd = gptr(V.data);
   int sy = V.n[0], sz = V.n[0] * V.n[1];
   if (V.small) {
-    d = V.data + (unsigned)(i0 + sy * i1 + sz * i2);
+    d = vd + (unsigned)(i0 + sy * i1 + sz * i2);
   } else {
-    d = V.data + ((int64_t)i0 + (int64_t)sy * i1 + (int64_t)V.n[0] * V.n[1] * i2);
+    d = vd + ((int64_t)i0 + (int64_t)sy * i1 + (int64_t)V.n[0] * V.n[1] * i2);
   }
   const int64_t SZ = V.small ? (int64_t)sz : (int64_t)V.n[0] * V.n[1];
   float2u p00 = *(const float2u *)(d);

@@ -15,6 +15,7 @@ struct LightParams {
   float aoRadius, Ka, Kd, eps;
   float L[3 * 16];
   int types[16];
+  const float *U, *V;  // AO direction tables (256 entries each, device memory)
 };
 
 // route modes (bits)
@@ -58,7 +59,6 @@ hipError_t launch_offsets_from_rank(const RayList_ispc &R, int n, const int *ran
                                     int nLights, int doShadows, int *ao_off, int *sh_off, hipStream_t s);
 hipError_t launch_lighting(const LightParams &L, int mode, const RayList_ispc &R, int n, const int *ao_off,
                            const int *sh_off, const RayList_ispc &O, hipStream_t s);
-hipError_t upload_uv_tables(const float *U, const float *V);
 
 hipError_t launch_route(const RouteParams &P, const RayList_ispc &R, int n, hipStream_t s);
 // Stable partition by classification into K categories: the category of a ray
@@ -78,6 +78,72 @@ hipError_t launch_copy_rays(const RayList_ispc &src, int start, int count, const
 
 hipError_t launch_noise(float *data, int nx, int ny, int nz, const int *perm_dev, int octaves, float base_freq,
                         float *minmax_scratch, hipStream_t s);
+
+// ---------------------------------------------------------------- batched passes
+enum { BM_RENDER = 0, BM_SAMPLE = 1 };
+// statistics slots
+enum { BS_RAYS = 0, BS_SAMPLES, BS_SPAWNED, BS_TERMINATED, BS_DROPPED, BS_KEPT, BS_COUNT };
+
+struct BatchList {
+  RayList_ispc R;   // the list's own (primary) rays, device
+  float *fb;        // BM_RENDER: RGBA accumulation, width * height * 4
+  float *xyz;       // BM_SAMPLE: particles, 3 floats each
+  int *pcount;      // BM_SAMPLE: particle count (device)
+  int width, height;
+  int pcap;         // particle capacity
+  int ds;           // dataset index
+  int mode;         // BM_RENDER / BM_SAMPLE
+  int n;
+};
+
+// rays [start, start + count) of one list, read from ray list src of the
+// RayList table (0, 1: the two ray pools; 2 + l: list l's own rays)
+struct BatchChunk {
+  int src, start, count, list;
+};
+
+struct BatchArgs {
+  const BatchList *lists;
+  int nlists;
+  int chunk;                  // rays per chunk (CH)
+  const TraceParams *ds;      // per dataset
+  const RayList_ispc *rls;    // RayList table: pools 0 and 1, then every list's own rays
+  int pool_cap;
+  int out_pool;               // pool (0 or 1) written by this pass
+  const BatchChunk *tin;
+  const int *nin;
+  BatchChunk *tout;
+  int *nout;
+  int tcap;
+  int *nrays_out;
+  int *head;                  // work-queue head of the trace kernel
+  int *cpar, *cout, *cchk;    // per input chunk: parents, rays out, chunks out
+  int *coff, *cchkoff;        // their exclusive offsets
+  long long *segr, *segc;     // scan segments (256 each)
+  int *lpend;                 // per list: rays in the next pass
+  unsigned long long *part;   // per-workgroup partials, BS_COUNT * nblocks
+  int nblocks;
+  unsigned long long *stats;  // BS_COUNT totals
+  int *overflow;              // bit 0: pool or table full; bit 1: append overflow
+  LightParams L;
+};
+
+struct BatchLaunch {
+  int trace_blocks;  // persistent trace grid (workgroups)
+  int post_blocks;   // count / write grid
+  int stack;         // traversal stack: 0 private (64 entries), or 16/24/32 entries in LDS
+  int wpe;           // occupancy request (waves per SIMD), 0 none
+};
+
+// nblocks of A must equal max(trace_blocks, post_blocks)
+hipError_t launch_batch_pass(const BatchArgs &A, const BatchLaunch &K, hipStream_t s);
+hipError_t launch_batch_reduce(const BatchArgs &A, hipStream_t s);
+hipError_t launch_batch_append(BatchChunk *tab, int *ntab, int cap, const BatchChunk *add, int n, int *overflow,
+                               hipStream_t s);
+hipError_t launch_set_count(int *p, int v, hipStream_t s);
+hipError_t launch_fill_f32(float *p, long long n, float v, hipStream_t s);
+hipError_t launch_random_rays(const RayList_ispc &R, int n, unsigned seed, int width, int type, const float *lo,
+                              const float *hi, hipStream_t s);
 
 // LBVH build (bvh.hip).  Boxes per primitive (lo, hi) on device; returns nodes
 // (device, allocated through alloc_fn) and the sorted primitive order.

@@ -11,48 +11,16 @@
 // category, and a scatter moves the 25 fields (100 B) of each surviving ray.
 #include <cstring>
 
-#include "gxy_device.h"
-#include "gxy_internal.h"
+#include "route_dev.h"
 
 namespace gxy {
-
-// Box::exit_face (Box.cpp:84-97), using the ray origin as the reference does.
-__device__ __forceinline__ int exit_face(const float *lo, const float *hi, float x, float y, float z, float dx,
-                                         float dy, float dz) {
-  const float FM = 3.402823466e+38f;
-  float tx = (dx > 0.0001f) ? ((hi[0] - x) / dx) : (dx < -0.0001f) ? ((lo[0] - x) / dx) : FM;
-  float ty = (dy > 0.0001f) ? ((hi[1] - y) / dy) : (dy < -0.0001f) ? ((lo[1] - y) / dy) : FM;
-  float tz = (dz > 0.0001f) ? ((hi[2] - z) / dz) : (dz < -0.0001f) ? ((lo[2] - z) / dz) : FM;
-  if (tx < 0) tx = FM;
-  if (ty < 0) ty = FM;
-  if (tz < 0) tz = FM;
-  if (tx < ty && tx < tz) return (dx < 0) ? 0 : 1;
-  else if (ty < tz) return (dy < 0) ? 2 : 3;
-  else return (dz < 0) ? 4 : 5;
-}
 
 __global__ __launch_bounds__(GXY_BLOCK) void k_route(RouteParams P, RayList_ispc R, int n) {
   int i = blockIdx.x * GXY_BLOCK + threadIdx.x;
   if (i < n) {
     const int typ = R.type[i];
     int c = R.classification[i];
-    if (P.mode & RM_CLASSIFY) {
-      const int term = R.term[i];
-      if (typ == RAY_PRIMARY) {
-        if (term & RAY_BOUNDARY) c = RAY_BOUNDARY;
-        else if ((term & RAY_OPAQUE) | (term & RAY_TIMEOUT)) c = GXY_TERMINATED;
-        else c = GXY_KEEP_HERE;
-      } else if (typ == RAY_SHADOW) {
-        if ((term & RAY_OPAQUE) | (term & RAY_SURFACE)) c = P.reverse ? GXY_TERMINATED : GXY_DROP_ON_FLOOR;
-        else if (term & RAY_BOUNDARY) c = RAY_BOUNDARY;
-        else c = GXY_DROP_ON_FLOOR;
-      } else if (typ == RAY_AO) {
-        if ((term & RAY_OPAQUE) | (term & RAY_SURFACE)) c = P.reverse ? GXY_TERMINATED : GXY_DROP_ON_FLOOR;
-        else if (term & RAY_BOUNDARY) c = RAY_BOUNDARY;
-        else if (term & RAY_TIMEOUT) c = P.reverse ? GXY_DROP_ON_FLOOR : GXY_TERMINATED;
-        else c = GXY_DROP_ON_FLOOR;
-      }
-    }
+    if (P.mode & RM_CLASSIFY) c = classify_ray(typ, R.term[i], c, P.reverse);
     if ((P.mode & RM_ASSIGN) && c == RAY_BOUNDARY) {
       int f = exit_face(P.blo, P.bhi, R.ox[i], R.oy[i], R.oz[i], R.dx[i], R.dy[i], R.dz[i]);
       if (P.neighbors[f] >= 0) c = P.neighbors[f];

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -20,6 +21,7 @@
 #include <mutex>
 #include <set>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "gxy_internal.h"
@@ -84,6 +86,7 @@ struct Runtime {
   std::vector<void *> outbound[6];   // RayListObj* per exit face, last frames
   // volume-only trace kernel: 0 one ray per work-item, 1 persistent lanes
   int trace_kind = 0, trace_chunk = 256, trace_refill = 32, trace_wpe = 0;
+  float *uv = nullptr;          // AO direction tables U[256], V[256] (device)
 };
 static Runtime G;
 static thread_local Ctx *t_ctx = nullptr;
@@ -229,10 +232,12 @@ static int ensure_init() {
   if (getenv("GXY_DEVICE")) dev = atoi(getenv("GXY_DEVICE"));
   HIPCHK(hipSetDevice(dev));
   HIPCHK(hipStreamCreateWithFlags(&G.def.stream, hipStreamNonBlocking));
-  float U[256], V[256];
-  make_uv(U, V);
-  HIPCHK(upload_uv_tables(U, V));
   G.device = dev;
+  float UV[512];
+  make_uv(UV, UV + 256);
+  G.uv = (float *)dalloc(sizeof UV);
+  if (!G.uv) return -1;
+  HIPCHK(hipMemcpy(G.uv, UV, sizeof UV, hipMemcpyHostToDevice));
   G.def.counters = (unsigned long long *)dalloc(8 * sizeof(unsigned long long));
   if (!G.def.counters) return -1;
   HIPCHK(hipMemsetAsync(G.def.counters, 0, 8 * sizeof(unsigned long long), S()));
@@ -412,6 +417,8 @@ static int make_light_params(const Lighting_ispc *l, float eps, LightParams &L) 
   L.Ka = l->Ka;
   L.Kd = l->Kd;
   L.eps = eps;
+  L.U = G.uv;
+  L.V = G.uv + 256;
   for (int k = 0; k < L.nLights; k++) {
     for (int a = 0; a < 3; a++) L.L[3 * k + a] = l->lights[3 * k + a];
     L.types[k] = l->types ? l->types[k] : 2;
@@ -1885,4 +1892,969 @@ GXY_EXPORT int gxy_sample_raylist(void *vis, int n, RayList_ispc *rays, int maxl
   *count = c;
   if (stats) *stats = C.st;
   return r;
+}
+
+// ====================================================================== batched form
+namespace gxy {
+
+struct BatchObj {
+  // launch configuration
+  int chunk = 256, stack = 0, wpe = 0, bpc = 4;
+  int cus = 0;
+  // datasets and lighting
+  std::vector<TraceParams> ds;
+  std::vector<int> ds_depth;
+  LightParams L;
+  bool have_L = false;
+  // lists (slot index = list id); rls[2 + l] = list l's rays
+  std::vector<BatchList> lists;
+  std::vector<char> live;
+  std::vector<unsigned> seed;
+  std::vector<std::array<float, 6>> seed_box;
+  std::vector<char> has_seed;
+  std::vector<BatchChunk> fresh;     // primary chunks of lists added since the last pass
+  // device state
+  TraceParams *d_ds = nullptr;
+  int d_ds_cap = 0;
+  BatchList *d_lists = nullptr;
+  RayList_ispc *d_rls = nullptr;
+  int d_lists_cap = 0;
+  RayListObj *pool[2] = {nullptr, nullptr};
+  BatchChunk *tab[2] = {nullptr, nullptr};
+  int tcap[2] = {0, 0};
+  BatchChunk *d_init = nullptr, *d_fresh = nullptr;
+  int init_cap = 0, fresh_cap = 0, ninit = 0;
+  int *pc = nullptr;                 // per-chunk arrays: cpar, cout, cchk, coff, cchkoff (pc_cap each)
+  int pc_cap = 0;
+  long long *seg = nullptr;          // 2 * 256
+  int *lpend = nullptr;
+  int lpend_cap = 0;
+  unsigned long long *part = nullptr;
+  int part_blocks = 0;
+  unsigned long long *stats = nullptr;  // BS_COUNT
+  int *ints = nullptr;               // 0,1: table counts; 2 nrays_out; 3 head; 4 overflow
+  int pass = 0;
+  long long queued = 0;              // rays queued for the next pass (host view)
+  int passes_run = 0;
+  bool dirty_tables = true;          // list / dataset tables need upload
+  bool dirty_graph = true;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<uint64_t> gsig;
+  std::vector<void *> owned;
+  void free_all() {
+    if (gexec) hipGraphExecDestroy(gexec);
+    gexec = nullptr;
+    for (int k = 0; k < 2; k++)
+      if (pool[k]) raylist_delete(pool[k]);
+    pool[0] = pool[1] = nullptr;
+    void *bufs[] = {d_ds, d_lists, d_rls, tab[0], tab[1], d_init, d_fresh, pc, seg, lpend, part, stats, ints};
+    for (void *p : bufs) dfree(p);
+  }
+};
+
+static std::set<void *> g_batches;
+
+static int nsp_of(const LightParams &L) { return L.nAO + (L.doShadows ? L.nLights : 0); }
+
+// grows a device buffer of `elem`-byte entries to at least n entries (keeps
+// the first `keep` entries)
+static int grow(void **p, int *cap, long long n, size_t elem, long long keep = 0) {
+  if (n <= *cap) return 0;
+  if (n > INT32_MAX) return fail("batch table of %lld entries exceeds 2^31", n);
+  long long c = std::max<long long>(n, (long long)*cap * 3 / 2);
+  if (c > INT32_MAX) c = n;
+  void *q = dalloc((size_t)c * elem);
+  if (!q) return -1;
+  if (keep > 0 && *p) HIPCHK(hipMemcpyAsync(q, *p, (size_t)keep * elem, hipMemcpyDeviceToDevice, S()));
+  if (*p) dfree(*p);
+  *p = q;
+  *cap = (int)c;
+  return 0;
+}
+
+static int batch_blocks(BatchObj *B) { return std::max(1, B->cus * B->bpc); }
+
+static int batch_upload_tables(BatchObj *B) {
+  if (!B->dirty_tables) return 0;
+  const int nl = (int)B->lists.size(), nd = (int)B->ds.size();
+  if (grow((void **)&B->d_ds, &B->d_ds_cap, std::max(1, nd), sizeof(TraceParams)) != 0) return -1;
+  int lc = B->d_lists_cap;
+  if (nl + 2 > lc) {
+    int c1 = lc, c2 = lc;
+    if (grow((void **)&B->d_lists, &c1, nl + 2, sizeof(BatchList)) != 0) return -1;
+    if (grow((void **)&B->d_rls, &c2, nl + 2, sizeof(RayList_ispc)) != 0) return -1;
+    B->d_lists_cap = std::min(c1, c2);
+  }
+  if (nl > B->lpend_cap && grow((void **)&B->lpend, &B->lpend_cap, nl, sizeof(int)) != 0) return -1;
+  if (nd) HIPCHK(hipMemcpyAsync(B->d_ds, B->ds.data(), nd * sizeof(TraceParams), hipMemcpyHostToDevice, S()));
+  if (nl) HIPCHK(hipMemcpyAsync(B->d_lists, B->lists.data(), nl * sizeof(BatchList), hipMemcpyHostToDevice, S()));
+  std::vector<RayList_ispc> rl(nl + 2);
+  memset(rl.data(), 0, rl.size() * sizeof(RayList_ispc));
+  for (int k = 0; k < 2; k++)
+    if (B->pool[k]) rl[k] = B->pool[k]->ispc;
+  for (int l = 0; l < nl; l++) rl[2 + l] = B->lists[l].R;
+  HIPCHK(hipMemcpyAsync(B->d_rls, rl.data(), rl.size() * sizeof(RayList_ispc), hipMemcpyHostToDevice, S()));
+  HIPCHK(hipStreamSynchronize(S()));   // host vectors may change after return
+  B->dirty_tables = false;
+  return 0;
+}
+
+// pool k holds at least n rays
+static int batch_pool(BatchObj *B, int k, long long n) {
+  if (B->pool[k] && B->pool[k]->hdr.aligned_size >= n) return 0;
+  if (n > INT32_MAX - 16) return fail("ray pool of %lld rays exceeds 2^31", n);
+  long long c = std::max<long long>(n, B->pool[k] ? (long long)B->pool[k]->hdr.aligned_size * 3 / 2 : 0);
+  if (c > INT32_MAX - 16) c = n;
+  RayListObj *r = raylist_new((int)c, 0, 1);
+  if (!r) return -1;
+  if (B->pool[k]) raylist_delete(B->pool[k]);
+  B->pool[k] = r;
+  B->dirty_tables = true;
+  B->dirty_graph = true;
+  return 0;
+}
+
+static void batch_chunks_of(BatchObj *B, int l, std::vector<BatchChunk> &out) {
+  const int n = B->lists[l].n;
+  for (int s = 0; s < n; s += B->chunk) {
+    BatchChunk c;
+    c.src = 2 + l;
+    c.start = s;
+    c.count = std::min(B->chunk, n - s);
+    c.list = l;
+    out.push_back(c);
+  }
+}
+
+static BatchArgs batch_args(BatchObj *B, int pass) {
+  BatchArgs A;
+  memset(&A, 0, sizeof A);
+  const int in = (pass + 1) & 1, out = pass & 1;
+  A.lists = B->d_lists;
+  A.nlists = (int)B->lists.size();
+  A.chunk = B->chunk;
+  A.ds = B->d_ds;
+  A.rls = B->d_rls;
+  A.pool_cap = B->pool[out] ? B->pool[out]->hdr.aligned_size : 0;
+  A.out_pool = out;
+  A.tin = B->tab[in];
+  A.nin = B->ints + in;
+  A.tout = B->tab[out];
+  A.nout = B->ints + out;
+  A.tcap = std::min(B->tcap[out], B->pc_cap);
+  A.nrays_out = B->ints + 2;
+  A.head = B->ints + 3;
+  A.overflow = B->ints + 4;
+  A.cpar = B->pc;
+  A.cout = B->pc + (size_t)B->pc_cap;
+  A.cchk = B->pc + 2 * (size_t)B->pc_cap;
+  A.coff = B->pc + 3 * (size_t)B->pc_cap;
+  A.cchkoff = B->pc + 4 * (size_t)B->pc_cap;
+  A.segr = B->seg;
+  A.segc = B->seg + 256;
+  A.lpend = B->lpend;
+  A.part = B->part;
+  A.nblocks = B->part_blocks;
+  A.stats = B->stats;
+  A.L = B->L;
+  return A;
+}
+
+static BatchLaunch batch_launch(BatchObj *B) {
+  BatchLaunch K;
+  K.trace_blocks = batch_blocks(B);
+  K.post_blocks = B->part_blocks;   // equal to trace_blocks: every partial slot is written
+  K.stack = B->stack;
+  K.wpe = B->wpe;
+  return K;
+}
+
+// capacity for a pass whose input holds in_rays rays (at most prim of them
+// primary, the only rays that spawn) in in_chunks chunks
+static int batch_reserve_pass(BatchObj *B, int pass, long long in_rays, long long in_chunks, long long prim) {
+  const int in = (pass + 1) & 1, out = pass & 1;
+  const long long out_rays = in_rays + std::min(in_rays, prim) * nsp_of(B->L) + 16;
+  if (batch_pool(B, out, out_rays) != 0) return -1;
+  const long long out_chunks = out_rays / B->chunk + in_chunks + 16;
+  if (grow((void **)&B->tab[out], &B->tcap[out], out_chunks, sizeof(BatchChunk)) != 0) return -1;
+  (void)in;
+  const long long need_pc = std::max(in_chunks, out_chunks) + 16;
+  if (need_pc > B->pc_cap) {
+    int c = B->pc_cap;
+    void *p = B->pc;
+    long long want = std::max<long long>(need_pc, (long long)c * 3 / 2);
+    if (want > INT32_MAX / 5) return fail("batch chunk arrays too large");
+    void *q = dalloc((size_t)want * 5 * sizeof(int));
+    if (!q) return -1;
+    if (p) dfree(p);
+    B->pc = (int *)q;
+    B->pc_cap = (int)want;
+    B->dirty_graph = true;
+  }
+  return 0;
+}
+
+static int batch_ensure_base(BatchObj *B) {
+  if (!B->ints) {
+    B->ints = (int *)dalloc(8 * sizeof(int));
+    B->seg = (long long *)dalloc(512 * sizeof(long long));
+    B->stats = (unsigned long long *)dalloc(BS_COUNT * sizeof(unsigned long long));
+    if (!B->ints || !B->seg || !B->stats) return -1;
+    HIPCHK(hipMemsetAsync(B->ints, 0, 8 * sizeof(int), S()));
+    HIPCHK(hipMemsetAsync(B->stats, 0, BS_COUNT * sizeof(unsigned long long), S()));
+  }
+  const int nb = batch_blocks(B);
+  if (nb != B->part_blocks) {
+    dfree(B->part);
+    B->part = (unsigned long long *)dalloc((size_t)nb * BS_COUNT * sizeof(unsigned long long));
+    if (!B->part) return -1;
+    B->part_blocks = nb;
+    B->dirty_graph = true;
+  }
+  return 0;
+}
+
+static int batch_check(BatchObj *B) {
+  if (!B->have_L) return fail("gxy_batch_set_lighting not called");
+  if (B->stack > 0)
+    for (size_t d = 0; d < B->ds.size(); d++)
+      if (B->ds_depth[d] + 2 > B->stack)
+        return fail("dataset %zu: BVH depth %d needs a traversal stack of at least %d entries (configured %d)", d,
+                    B->ds_depth[d], B->ds_depth[d] + 2, B->stack);
+  return 0;
+}
+
+// the set-up of a run: statistics and outputs cleared, seeded lists
+// regenerated, every list's primary chunks queued as the input of pass 0
+// everything a captured run bakes into its kernel arguments
+static std::vector<uint64_t> batch_graph_sig(BatchObj *B, int graph_passes) {
+  std::vector<uint64_t> v = {(uint64_t)graph_passes, (uint64_t)(uintptr_t)B->tab[0], (uint64_t)(uintptr_t)B->tab[1],
+                             (uint64_t)B->tcap[0], (uint64_t)B->tcap[1], (uint64_t)(uintptr_t)B->pc,
+                             (uint64_t)B->pc_cap, (uint64_t)(uintptr_t)B->d_init, (uint64_t)B->ninit,
+                             (uint64_t)(uintptr_t)B->d_ds, (uint64_t)(uintptr_t)B->d_lists,
+                             (uint64_t)(uintptr_t)B->d_rls, (uint64_t)(uintptr_t)B->part, (uint64_t)B->part_blocks,
+                             (uint64_t)(uintptr_t)B->lpend, (uint64_t)B->chunk, (uint64_t)B->stack,
+                             (uint64_t)B->wpe, (uint64_t)B->bpc, (uint64_t)B->lists.size()};
+  for (int k = 0; k < 2; k++) {
+    v.push_back((uint64_t)(uintptr_t)B->pool[k]);
+    v.push_back(B->pool[k] ? (uint64_t)(uintptr_t)B->pool[k]->block : 0);
+  }
+  const unsigned char *lp = (const unsigned char *)&B->L;
+  for (size_t k = 0; k < sizeof B->L; k += 8) {
+    uint64_t w = 0;
+    memcpy(&w, lp + k, std::min<size_t>(8, sizeof B->L - k));
+    v.push_back(w);
+  }
+  for (size_t l = 0; l < B->lists.size(); l++) {
+    const BatchList &Ls = B->lists[l];
+    v.push_back(B->live[l]);
+    v.push_back((uint64_t)(uintptr_t)Ls.fb);
+    v.push_back((uint64_t)(uintptr_t)Ls.pcount);
+    v.push_back((uint64_t)(uintptr_t)Ls.R.ox);
+    v.push_back(((uint64_t)Ls.n << 32) | (uint32_t)Ls.width);
+    v.push_back(((uint64_t)Ls.height << 32) | (uint32_t)Ls.mode);
+    v.push_back(B->has_seed[l] ? (((uint64_t)1 << 32) | B->seed[l]) : 0);
+    for (int k = 0; k < 6; k++) {
+      uint32_t w;
+      memcpy(&w, &B->seed_box[l][k], 4);
+      v.push_back(w);
+    }
+  }
+  return v;
+}
+
+static int batch_setup_kernels(BatchObj *B) {
+  HIPCHK(launch_fill_f32((float *)B->stats, 2 * BS_COUNT, 0.f, S()));
+  HIPCHK(launch_set_count(B->ints + 4, 0, S()));
+  for (size_t l = 0; l < B->lists.size(); l++) {
+    if (!B->live[l]) continue;
+    const BatchList &Ls = B->lists[l];
+    if (Ls.mode == BM_RENDER) HIPCHK(launch_fill_f32(Ls.fb, 4LL * Ls.width * Ls.height, 0.f, S()));
+    else HIPCHK(launch_set_count(Ls.pcount, 0, S()));
+    if (B->has_seed[l]) {
+      const float *bx = B->seed_box[l].data();
+      HIPCHK(launch_random_rays(Ls.R, Ls.n, B->seed[l], Ls.width, RAY_PRIMARY, bx, bx + 3, S()));
+    }
+  }
+  HIPCHK(launch_set_count(B->ints + 1, 0, S()));
+  HIPCHK(launch_set_count(B->ints + 0, 0, S()));
+  HIPCHK(launch_batch_append(B->tab[1], B->ints + 1, B->tcap[1], B->d_init, B->ninit, B->ints + 4, S()));
+  return 0;
+}
+
+static int batch_pass_kernels(BatchObj *B, int pass) {
+  BatchArgs A = batch_args(B, pass);
+  HIPCHK(launch_batch_pass(A, batch_launch(B), S()));
+  HIPCHK(launch_batch_reduce(A, S()));
+  return 0;
+}
+
+static int batch_read_counts(BatchObj *B, int pass, long long *rays, int *chunks, int *overflow) {
+  int v[8];
+  HIPCHK(hipMemcpyAsync(v, B->ints, sizeof v, hipMemcpyDeviceToHost, S()));
+  HIPCHK(hipStreamSynchronize(S()));
+  *chunks = v[pass & 1];
+  *rays = v[2];
+  *overflow = v[4];
+  return 0;
+}
+
+static int batch_stats_read(BatchObj *B, gxy_batch_stats *st) {
+  if (!st) return 0;
+  unsigned long long v[BS_COUNT];
+  HIPCHK(hipMemcpyAsync(v, B->stats, sizeof v, hipMemcpyDeviceToHost, S()));
+  int ov = 0;
+  HIPCHK(hipMemcpyAsync(&ov, B->ints + 4, sizeof ov, hipMemcpyDeviceToHost, S()));
+  HIPCHK(hipStreamSynchronize(S()));
+  st->rays = v[BS_RAYS];
+  st->samples = v[BS_SAMPLES];
+  st->spawned = v[BS_SPAWNED];
+  st->terminated = v[BS_TERMINATED];
+  st->dropped = v[BS_DROPPED];
+  st->kept = v[BS_KEPT];
+  st->passes = B->passes_run;
+  st->overflow = ov;
+  return 0;
+}
+
+}  // namespace gxy
+
+GXY_EXPORT void *gxy_batch_create(int chunk) {
+  INIT_OR(nullptr);
+  if (chunk < 64 || chunk > 4096 || chunk % 64) {
+    fail("chunk %d: must be a multiple of 64 in [64, 4096]", chunk);
+    return nullptr;
+  }
+  BatchObj *B = new BatchObj;
+  B->chunk = chunk;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, G.device) != hipSuccess) {
+    delete B;
+    fail("hipGetDeviceProperties failed");
+    return nullptr;
+  }
+  B->cus = prop.multiProcessorCount;
+  memset(&B->L, 0, sizeof B->L);
+  if (batch_ensure_base(B) != 0) {
+    B->free_all();
+    delete B;
+    return nullptr;
+  }
+  std::lock_guard<std::recursive_mutex> lk(G.mu);
+  g_batches.insert(B);
+  return B;
+}
+
+GXY_EXPORT int gxy_batch_destroy(void *b) {
+  INIT_OR(-1);
+  BatchObj *B = lookup<BatchObj>(g_batches, b);
+  if (!B) return fail("not a batch handle");
+  {
+    std::lock_guard<std::recursive_mutex> lk(G.mu);
+    g_batches.erase(B);
+  }
+  HIPCHK(hipStreamSynchronize(S()));
+  B->free_all();
+  delete B;
+  return 0;
+}
+
+GXY_EXPORT int gxy_batch_add_dataset(void *b, const Visualization_ispc *vis, float eps) {
+  INIT_OR(-1);
+  BatchObj *B = lookup<BatchObj>(g_batches, b);
+  if (!B) return fail("not a batch handle");
+  TraceParams P;
+  if (make_trace_params(vis, eps, P) != 0) return -1;
+  const bool volume = P.nvv == 1 && !P.has_model, geometry = P.nvv == 0 && P.has_model;
+  if (!volume && !geometry)
+    return fail("a batch dataset is one volume without geometry or geometry without volumes (got %d volumes%s)",
+                P.nvv, P.has_model ? " and a model" : "");
+  P.hit_geom = nullptr;
+  P.hit_prim = nullptr;
+  P.sample_partials = nullptr;
+  int depth = 0;
+  if (geometry) {
+    ModelObj *M = lookup<ModelObj>(G.models, vis->model);
+    depth = M ? std::max(M->tdepth, M->sdepth) : 64;
+  }
+  B->ds.push_back(P);
+  B->ds_depth.push_back(depth);
+  B->dirty_tables = B->dirty_graph = true;
+  return (int)B->ds.size() - 1;
+}
+
+GXY_EXPORT int gxy_batch_set_lighting(void *b, const Lighting_ispc *lights, float eps) {
+  INIT_OR(-1);
+  BatchObj *B = lookup<BatchObj>(g_batches, b);
+  if (!B) return fail("not a batch handle");
+  if (make_light_params(lights, eps, B->L) != 0) return -1;
+  B->have_L = true;
+  B->dirty_graph = true;
+  return 0;
+}
+
+GXY_EXPORT int gxy_batch_add_list(void *b, int ds, int mode, int n, const RayList_ispc *rays, float *fb, int width,
+                                  int height, float *xyz, int capacity, int *count) {
+  INIT_OR(-1);
+  BatchObj *B = lookup<BatchObj>(g_batches, b);
+  if (!B) return fail("not a batch handle");
+  if (ds < 0 || ds >= (int)B->ds.size()) return fail("dataset %d out of range", ds);
+  if (n < 0 || !rays) return fail("bad ray list");
+  if (mode != BM_RENDER && mode != BM_SAMPLE) return fail("mode %d", mode);
+  if (mode == BM_RENDER && (!fb || width <= 0 || height <= 0)) return fail("render list needs a framebuffer");
+  if (mode == BM_SAMPLE && (!xyz || !count || capacity < 0)) return fail("sample list needs a particle buffer");
+  if (mode == BM_SAMPLE && B->ds[ds].nvv != 1) return fail("sample list over a dataset without a volume");
+  BatchList Ls;
+  memset(&Ls, 0, sizeof Ls);
+  Ls.R = *rays;
+  Ls.fb = fb;
+  Ls.xyz = xyz;
+  Ls.pcount = count;
+  Ls.width = width;
+  Ls.height = height;
+  Ls.pcap = capacity;
+  Ls.ds = ds;
+  Ls.mode = mode;
+  Ls.n = n;
+  int l = -1;
+  for (size_t k = 0; k < B->live.size(); k++)
+    if (!B->live[k]) {
+      l = (int)k;
+      break;
+    }
+  if (l < 0) {
+    l = (int)B->lists.size();
+    B->lists.push_back(Ls);
+    B->live.push_back(1);
+    B->seed.push_back(0);
+    B->seed_box.push_back({0, 0, 0, 1, 1, 1});
+    B->has_seed.push_back(0);
+  } else {
+    B->lists[l] = Ls;
+    B->live[l] = 1;
+    B->has_seed[l] = 0;
+  }
+  if (mode == BM_RENDER) HIPCHK(launch_fill_f32(fb, 4LL * width * height, 0.f, S()));
+  else HIPCHK(launch_set_count(count, 0, S()));
+  batch_chunks_of(B, l, B->fresh);
+  B->dirty_tables = B->dirty_graph = true;
+  return l;
+}
+
+GXY_EXPORT int gxy_batch_set_list_seed(void *b, int l, unsigned seed, const float lo[3], const float hi[3]) {
+  INIT_OR(-1);
+  BatchObj *B = lookup<BatchObj>(g_batches, b);
+  if (!B) return fail("not a batch handle");
+  if (l < 0 || l >= (int)B->lists.size() || !B->live[l]) return fail("no list %d", l);
+  B->seed[l] = seed;
+  for (int k = 0; k < 3; k++) {
+    B->seed_box[l][k] = lo ? lo[k] : 0.f;
+    B->seed_box[l][3 + k] = hi ? hi[k] : 1.f;
+  }
+  B->has_seed[l] = 1;
+  B->dirty_graph = true;
+  return 0;
+}
+
+GXY_EXPORT int gxy_batch_remove_list(void *b, int l) {
+  INIT_OR(-1);
+  BatchObj *B = lookup<BatchObj>(g_batches, b);
+  if (!B) return fail("not a batch handle");
+  if (l < 0 || l >= (int)B->lists.size() || !B->live[l]) return fail("no list %d", l);
+  for (const BatchChunk &c : B->fresh)
+    if (c.list == l) return fail("list %d has not been traced yet", l);
+  B->live[l] = 0;
+  B->lists[l].n = 0;
+  B->dirty_tables = B->dirty_graph = true;
+  return 0;
+}
+
+GXY_EXPORT int gxy_batch_set_config(void *b, int chunk, int stack, int wpe, int bpc) {
+  INIT_OR(-1);
+  BatchObj *B = lookup<BatchObj>(g_batches, b);
+  if (!B) return fail("not a batch handle");
+  if (chunk < 64 || chunk > 4096 || chunk % 64) return fail("chunk %d", chunk);
+  if (stack != 0 && stack != 16 && stack != 24 && stack != 32) return fail("stack %d: 0, 16, 24 or 32", stack);
+  if (wpe != 0 && wpe != 4 && wpe != 5) return fail("wpe %d: 0, 4 or 5", wpe);
+  if (wpe == 5 && stack != 0) return fail("wpe 5 is built for the private-memory stack only");
+  if (bpc < 1 || bpc > 16) return fail("blocks per CU %d", bpc);
+  if (chunk != B->chunk) {
+    if (B->queued > 0) return fail("chunk size changes only with no rays queued");
+    std::vector<int> ls;
+    for (const BatchChunk &c : B->fresh)
+      if (ls.empty() || ls.back() != c.list) ls.push_back(c.list);
+    B->fresh.clear();
+    B->chunk = chunk;
+    for (int l : ls) batch_chunks_of(B, l, B->fresh);
+  }
+  B->chunk = chunk;
+  B->stack = stack;
+  B->wpe = wpe;
+  B->bpc = bpc;
+  B->dirty_graph = true;
+  return batch_ensure_base(B);
+}
+
+GXY_EXPORT int gxy_batch_get_config(void *b, int *chunk, int *stack, int *wpe, int *bpc) {
+  INIT_OR(-1);
+  BatchObj *B = lookup<BatchObj>(g_batches, b);
+  if (!B) return fail("not a batch handle");
+  if (chunk) *chunk = B->chunk;
+  if (stack) *stack = B->stack;
+  if (wpe) *wpe = B->wpe;
+  if (bpc) *bpc = B->bpc;
+  return 0;
+}
+
+GXY_EXPORT int gxy_batch_run(void *b, int graph_passes, gxy_batch_stats *st) {
+  INIT_OR(-1);
+  BatchObj *B = lookup<BatchObj>(g_batches, b);
+  if (!B) return fail("not a batch handle");
+  if (batch_check(B) != 0) return -1;
+  if (B->queued > 0) return fail("gxy_batch_run with rays still queued by gxy_batch_step");
+  B->fresh.clear();   // the run queues every list itself
+  // capacities for the worst case of pass 0 (every primary ray spawning)
+  std::vector<BatchChunk> init;
+  long long nprim = 0;
+  for (size_t l = 0; l < B->lists.size(); l++)
+    if (B->live[l]) {
+      batch_chunks_of(B, (int)l, init);
+      nprim += B->lists[l].n;
+    }
+  if (batch_ensure_base(B) != 0) return -1;
+  const long long nch = (long long)init.size();
+  if (grow((void **)&B->tab[1], &B->tcap[1], nch + 16, sizeof(BatchChunk)) != 0) return -1;
+  // pass 0 reads the primary chunks; later passes read at most what pass 0 wrote
+  if (batch_reserve_pass(B, 0, nprim, nch, nprim) != 0) return -1;
+  if (batch_reserve_pass(B, 1, nprim * (1 + nsp_of(B->L)), (long long)B->tcap[0], 0) != 0) return -1;
+  if ((int)nch > B->init_cap || B->dirty_graph) {
+    if (grow((void **)&B->d_init, &B->init_cap, std::max<long long>(1, nch), sizeof(BatchChunk)) != 0) return -1;
+    if (nch) HIPCHK(hipMemcpyAsync(B->d_init, init.data(), nch * sizeof(BatchChunk), hipMemcpyHostToDevice, S()));
+    B->ninit = (int)nch;
+    HIPCHK(hipStreamSynchronize(S()));
+  }
+  if (batch_upload_tables(B) != 0) return -1;
+  B->passes_run = 0;
+  int pass = 0;
+  if (graph_passes > 0) {
+    std::vector<uint64_t> sig = batch_graph_sig(B, graph_passes);
+    if (!B->gexec || sig != B->gsig) {
+      if (B->gexec) HIPCHK(hipGraphExecDestroy(B->gexec));
+      B->gexec = nullptr;
+      hipGraph_t g;
+      HIPCHK(hipStreamBeginCapture(S(), hipStreamCaptureModeThreadLocal));
+      int rc = batch_setup_kernels(B);
+      for (int p = 0; rc == 0 && p < graph_passes; p++) rc = batch_pass_kernels(B, p);
+      hipError_t e = hipStreamEndCapture(S(), &g);
+      if (rc != 0) return -1;
+      HIPCHK(e);
+      e = hipGraphInstantiate(&B->gexec, g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      HIPCHK(e);
+      B->gsig = sig;
+      B->dirty_graph = false;
+    }
+    {
+      KTimer kt("batch_graph");
+      HIPCHK(hipGraphLaunch(B->gexec, S()));
+    }
+    pass = graph_passes;
+    B->passes_run = graph_passes;
+  } else {
+    if (batch_setup_kernels(B) != 0) return -1;
+  }
+  for (;;) {
+    long long rays = 0;
+    int chunks = 0, ov = 0;
+    if (pass > 0) {
+      if (batch_read_counts(B, pass - 1, &rays, &chunks, &ov) != 0) return -1;
+      if (ov) return fail("batch pass %d overflowed its ray pool or chunk table (flags %d)", pass - 1, ov);
+      if (chunks == 0) break;
+      if (batch_reserve_pass(B, pass, rays, chunks, nprim) != 0) return -1;
+      if (B->dirty_tables && batch_upload_tables(B) != 0) return -1;
+    }
+    {
+      KTimer kt("batch_pass");
+      if (batch_pass_kernels(B, pass) != 0) return -1;
+    }
+    pass++;
+    B->passes_run++;
+    if (pass > 4096) return fail("batch did not drain after 4096 passes");
+  }
+  B->pass = 0;
+  B->queued = 0;
+  return batch_stats_read(B, st);
+}
+
+GXY_EXPORT int gxy_batch_step(void *b, long long *next_rays) {
+  INIT_OR(-1);
+  BatchObj *B = lookup<BatchObj>(g_batches, b);
+  if (!B) return fail("not a batch handle");
+  if (batch_check(B) != 0) return -1;
+  const int p = B->pass, in = (p + 1) & 1;
+  int nin = 0;
+  HIPCHK(hipMemcpyAsync(&nin, B->ints + in, sizeof nin, hipMemcpyDeviceToHost, S()));
+  HIPCHK(hipStreamSynchronize(S()));
+  if (B->queued == 0) nin = 0;
+  long long in_rays = B->queued;
+  for (const BatchChunk &c : B->fresh) in_rays += c.count;
+  const long long in_chunks = (long long)nin + (long long)B->fresh.size();
+  if (grow((void **)&B->tab[in], &B->tcap[in], in_chunks + 16, sizeof(BatchChunk), nin) != 0) return -1;
+  long long prim = 0;
+  for (size_t l = 0; l < B->lists.size(); l++)
+    if (B->live[l] && B->lists[l].mode == BM_RENDER) prim += B->lists[l].n;
+  if (batch_reserve_pass(B, p, in_rays, in_chunks, prim) != 0) return -1;
+  if (batch_upload_tables(B) != 0) return -1;
+  if (B->queued == 0) HIPCHK(launch_set_count(B->ints + in, 0, S()));
+  if (!B->fresh.empty()) {
+    const int nf = (int)B->fresh.size();
+    if (grow((void **)&B->d_fresh, &B->fresh_cap, nf, sizeof(BatchChunk)) != 0) return -1;
+    HIPCHK(hipMemcpyAsync(B->d_fresh, B->fresh.data(), nf * sizeof(BatchChunk), hipMemcpyHostToDevice, S()));
+    HIPCHK(launch_batch_append(B->tab[in], B->ints + in, B->tcap[in], B->d_fresh, nf, B->ints + 4, S()));
+    HIPCHK(hipStreamSynchronize(S()));
+    B->fresh.clear();
+  }
+  {
+    KTimer kt("batch_pass");
+    if (batch_pass_kernels(B, p) != 0) return -1;
+  }
+  long long rays = 0;
+  int chunks = 0, ov = 0;
+  if (batch_read_counts(B, p, &rays, &chunks, &ov) != 0) return -1;
+  if (ov) return fail("batch pass overflowed its ray pool or chunk table (flags %d)", ov);
+  B->pass = p + 1;
+  B->passes_run++;
+  B->queued = rays;
+  if (next_rays) *next_rays = rays;
+  return 0;
+}
+
+GXY_EXPORT int gxy_batch_pending(void *b, int *per_list, int n) {
+  INIT_OR(-1);
+  BatchObj *B = lookup<BatchObj>(g_batches, b);
+  if (!B) return fail("not a batch handle");
+  const int nl = std::min(n, (int)B->lists.size());
+  if (nl <= 0) return 0;
+  if (B->passes_run == 0) {
+    for (int l = 0; l < nl; l++) per_list[l] = 0;
+    return 0;
+  }
+  HIPCHK(hipMemcpyAsync(per_list, B->lpend, nl * sizeof(int), hipMemcpyDeviceToHost, S()));
+  HIPCHK(hipStreamSynchronize(S()));
+  return 0;
+}
+
+GXY_EXPORT int gxy_batch_get_stats(void *b, gxy_batch_stats *st) {
+  INIT_OR(-1);
+  BatchObj *B = lookup<BatchObj>(g_batches, b);
+  if (!B) return fail("not a batch handle");
+  return batch_stats_read(B, st);
+}
+
+GXY_EXPORT long long gxy_batch_bytes(void *b) {
+  INIT_OR(-1);
+  BatchObj *B = lookup<BatchObj>(g_batches, b);
+  if (!B) return fail("not a batch handle");
+  long long t = 0;
+  for (int k = 0; k < 2; k++)
+    if (B->pool[k]) t += (long long)B->pool[k]->bytes;
+  t += (long long)(B->tcap[0] + B->tcap[1] + B->init_cap + B->fresh_cap) * sizeof(BatchChunk);
+  t += (long long)B->pc_cap * 5 * sizeof(int);
+  t += (long long)B->part_blocks * BS_COUNT * 8;
+  return t;
+}
+
+GXY_EXPORT int gxy_rays_random(const RayList_ispc *rays, int n, unsigned seed, int width, int type, const float lo[3],
+                               const float hi[3]) {
+  INIT_OR(-1);
+  if (!rays || n < 0) return fail("bad ray list");
+  const float dlo[3] = {0, 0, 0}, dhi[3] = {1, 1, 1};
+  HIPCHK(launch_random_rays(*rays, n, seed, width, type, lo ? lo : dlo, hi ? hi : dhi, S()));
+  return 0;
+}
+
+// ====================================================================== served form
+namespace gxy {
+
+struct ServerObj {
+  long long budget = 0;
+  int slots = 1;
+  void *batch = nullptr;
+  std::vector<const Visualization_ispc *> vis;
+  std::vector<float> eps;
+  const Lighting_ispc *lights = nullptr;
+  float leps = 0.001f;
+  int nsp = 0;
+};
+
+static std::set<void *> g_servers;
+
+struct Flight {
+  int req = -1, list = -1;
+  RayListObj *rays = nullptr;
+  float *fb = nullptr, *xyz = nullptr;
+  int *cnt = nullptr;
+  long long bytes = 0;
+  int n = 0;
+};
+
+static long long request_bytes(const ServerObj *S_, const gxy_request &r) {
+  if (r.kind == 0) {
+    const long long px = (long long)r.width * r.height;
+    return px * (100 + 16) + 2 * px * (1 + S_->nsp) * 100;
+  }
+  return r.nrays * 100 * 3 + r.capacity * 12;
+}
+
+static void flight_free(Flight &f) {
+  if (f.rays) raylist_delete(f.rays);
+  dfree(f.fb);
+  dfree(f.xyz);
+  dfree(f.cnt);
+  f.rays = nullptr;
+  f.fb = f.xyz = nullptr;
+  f.cnt = nullptr;
+}
+
+// the request's rays and outputs on the device
+static int flight_make(ServerObj *S_, const gxy_request &r, Flight &f) {
+  if (r.dataset < 0 || r.dataset >= (int)S_->vis.size()) return fail("request dataset %d out of range", r.dataset);
+  const Visualization_ispc *v = S_->vis[r.dataset];
+  if (r.kind == 0) {
+    if (r.width <= 0 || r.height <= 0 || (long long)r.width * r.height > INT32_MAX / 2)
+      return fail("render request of %dx%d", r.width, r.height);
+    const int px = r.width * r.height;
+    f.rays = raylist_new(px, 0, 0);
+    f.fb = (float *)dalloc((size_t)px * 16);
+    if (!f.rays || !f.fb) return -1;
+    int nout = 0;
+    if (camera_generate(r.eye, r.center, 0, r.up, r.aov, r.width, r.height, &v->local_bb, &v->global_bb,
+                        f.rays->ispc, px, 1, &nout) != 0)
+      return -1;
+    f.n = nout;
+    HIPCHK(launch_fill_f32(f.fb, 4LL * px, 0.f, S()));
+  } else {
+    if (r.nrays <= 0 || r.nrays > INT32_MAX / 2 || r.capacity < 0) return fail("sample request of %lld rays", r.nrays);
+    f.n = (int)r.nrays;
+    f.rays = raylist_new(f.n, 0, 0);
+    f.xyz = (float *)dalloc((size_t)std::max<long long>(1, r.capacity) * 12);
+    f.cnt = (int *)dalloc(sizeof(int));
+    if (!f.rays || !f.xyz || !f.cnt) return -1;
+    HIPCHK(launch_random_rays(f.rays->ispc, f.n, r.seed, f.n, RAY_PRIMARY, v->local_bb.lower, v->local_bb.upper, S()));
+    HIPCHK(launch_set_count(f.cnt, 0, S()));
+  }
+  return 0;
+}
+
+// results to the host buffer of the request
+static int flight_finish(const gxy_request &r, Flight &f, gxy_request_result &res) {
+  if (r.kind == 0) {
+    res.rays = f.n;
+    if (r.out) HIPCHK(hipMemcpyAsync(r.out, f.fb, (size_t)r.width * r.height * 16, hipMemcpyDeviceToHost, S()));
+  } else {
+    int c = 0;
+    HIPCHK(hipMemcpyAsync(&c, f.cnt, sizeof c, hipMemcpyDeviceToHost, S()));
+    HIPCHK(hipStreamSynchronize(S()));
+    res.particles = c;
+    res.rays = f.n;
+    const long long k = std::min<long long>(c, r.capacity);
+    if (r.out && k > 0) HIPCHK(hipMemcpyAsync(r.out, f.xyz, (size_t)k * 12, hipMemcpyDeviceToHost, S()));
+  }
+  HIPCHK(hipStreamSynchronize(S()));
+  res.status = 0;
+  return 0;
+}
+
+}  // namespace gxy
+
+GXY_EXPORT void *gxy_server_create(long long budget, int slots, int chunk) {
+  INIT_OR(nullptr);
+  if (budget <= 0 || slots <= 0) {
+    fail("server needs a positive HBM budget and slot count");
+    return nullptr;
+  }
+  void *b = gxy_batch_create(chunk);
+  if (!b) return nullptr;
+  ServerObj *S_ = new ServerObj;
+  S_->budget = budget;
+  S_->slots = slots;
+  S_->batch = b;
+  std::lock_guard<std::recursive_mutex> lk(G.mu);
+  g_servers.insert(S_);
+  return S_;
+}
+
+GXY_EXPORT int gxy_server_destroy(void *s) {
+  INIT_OR(-1);
+  ServerObj *S_ = lookup<ServerObj>(g_servers, s);
+  if (!S_) return fail("not a server handle");
+  {
+    std::lock_guard<std::recursive_mutex> lk(G.mu);
+    g_servers.erase(S_);
+  }
+  int rc = gxy_batch_destroy(S_->batch);
+  delete S_;
+  return rc;
+}
+
+GXY_EXPORT void *gxy_server_batch(void *s) {
+  INIT_OR(nullptr);
+  ServerObj *S_ = lookup<ServerObj>(g_servers, s);
+  if (!S_) {
+    fail("not a server handle");
+    return nullptr;
+  }
+  return S_->batch;
+}
+
+GXY_EXPORT int gxy_server_add_dataset(void *s, const Visualization_ispc *vis, float eps) {
+  INIT_OR(-1);
+  ServerObj *S_ = lookup<ServerObj>(g_servers, s);
+  if (!S_) return fail("not a server handle");
+  int k = gxy_batch_add_dataset(S_->batch, vis, eps);
+  if (k < 0) return -1;
+  S_->vis.push_back(vis);
+  S_->eps.push_back(eps);
+  return k;
+}
+
+GXY_EXPORT int gxy_server_set_lighting(void *s, const Lighting_ispc *lights, float eps) {
+  INIT_OR(-1);
+  ServerObj *S_ = lookup<ServerObj>(g_servers, s);
+  if (!S_) return fail("not a server handle");
+  if (gxy_batch_set_lighting(S_->batch, lights, eps) != 0) return -1;
+  S_->lights = lights;
+  S_->leps = eps;
+  S_->nsp = (int)lights->n_ao_rays + (lights->do_shadows ? (int)lights->nLights : 0);
+  return 0;
+}
+
+GXY_EXPORT int gxy_server_run(void *s, const gxy_request *reqs, int n, int mode, gxy_request_result *res,
+                              gxy_serve_stats *st) {
+  INIT_OR(-1);
+  ServerObj *S_ = lookup<ServerObj>(g_servers, s);
+  if (!S_) return fail("not a server handle");
+  if (!S_->lights) return fail("gxy_server_set_lighting not called");
+  for (int k = 1; k < n; k++)
+    if (reqs[k].arrival < reqs[k - 1].arrival) return fail("requests must be sorted by arrival");
+  for (int k = 0; k < n; k++) memset(&res[k], 0, sizeof res[k]), res[k].status = -1;
+  gxy_serve_stats out;
+  memset(&out, 0, sizeof out);
+  const auto t0 = std::chrono::steady_clock::now();
+  auto el = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+  auto wait_until = [&](double t) {
+    for (;;) {
+      double d = t - el();
+      if (d <= 0) return;
+      std::this_thread::sleep_for(std::chrono::microseconds(d > 0.002 ? 1000 : 50));
+    }
+  };
+  const long long base = (long long)G.bytes_live;
+  long long peak = 0;
+  if (mode == 1) {
+    // one request at a time through the per-list loop
+    const int nb[6] = {-1, -1, -1, -1, -1, -1};
+    for (int k = 0; k < n; k++) {
+      const gxy_request &r = reqs[k];
+      wait_until(r.arrival);
+      res[k].admitted = el();
+      Flight f;
+      if (flight_make(S_, r, f) != 0) {
+        flight_free(f);
+        return -1;
+      }
+      gxy_frame_stats fs;
+      memset(&fs, 0, sizeof fs);
+      int rc;
+      if (r.kind == 0) {
+        rc = gxy_process_raylist((void *)S_->vis[r.dataset], (void *)S_->lights, f.n, &f.rays->ispc, nb,
+                                 S_->eps[r.dataset], 1 << 30, f.fb, r.width, r.height, &fs);
+      } else {
+        int c = 0;
+        rc = gxy_sample_raylist((void *)S_->vis[r.dataset], f.n, &f.rays->ispc, 1 << 30, f.xyz, (int)r.capacity, &c,
+                                &fs);
+        if (rc == 0) HIPCHK(hipMemcpyAsync(f.cnt, &c, sizeof c, hipMemcpyHostToDevice, S()));
+      }
+      peak = std::max(peak, (long long)G.bytes_live - base);
+      if (rc != 0 || flight_finish(r, f, res[k]) != 0) {
+        flight_free(f);
+        return -1;
+      }
+      flight_free(f);
+      res[k].finished = el();
+      out.rays += fs.ray_passes;
+      out.samples += fs.samples;
+      out.passes += fs.passes;
+    }
+    out.max_in_flight = n > 0 ? 1 : 0;
+  } else {
+    // continuous batching over the batch's passes
+    BatchObj *B = (BatchObj *)S_->batch;
+    gxy_batch_stats b0;
+    if (gxy_batch_get_stats(B, &b0) != 0) return -1;
+    const int passes0 = B->passes_run;
+    std::vector<Flight> fl;
+    std::vector<int> pend;
+    int next = 0, done = 0;
+    while (done < n) {
+      while (next < n && reqs[next].arrival <= el() && (int)fl.size() < S_->slots) {
+        const gxy_request &r = reqs[next];
+        const long long est = request_bytes(S_, r);
+        if (!fl.empty() && (long long)G.bytes_live - base + est > S_->budget) break;   // wait for memory
+        if (fl.empty() && est > S_->budget) return fail("request %d needs %lld bytes, over the budget", next, est);
+        Flight f;
+        f.req = next;
+        f.bytes = est;
+        if (flight_make(S_, r, f) != 0) {
+          flight_free(f);
+          return -1;
+        }
+        f.list = r.kind == 0 ? gxy_batch_add_list(B, r.dataset, BM_RENDER, f.n, &f.rays->ispc, f.fb, r.width,
+                                                  r.height, nullptr, 0, nullptr)
+                             : gxy_batch_add_list(B, r.dataset, BM_SAMPLE, f.n, &f.rays->ispc, nullptr, 0, 0, f.xyz,
+                                                  (int)std::min<long long>(r.capacity, INT32_MAX), f.cnt);
+        if (f.list < 0) {
+          flight_free(f);
+          return -1;
+        }
+        res[next].admitted = el();
+        fl.push_back(f);
+        next++;
+      }
+      out.max_in_flight = std::max(out.max_in_flight, (int)fl.size());
+      if (fl.empty()) {
+        if (next < n) wait_until(reqs[next].arrival);
+        continue;
+      }
+      long long q = 0;
+      if (gxy_batch_step(B, &q) != 0) return -1;
+      peak = std::max(peak, (long long)G.bytes_live - base);
+      pend.assign(B->lists.size(), 0);
+      if (gxy_batch_pending(B, pend.data(), (int)pend.size()) != 0) return -1;
+      for (size_t k = 0; k < fl.size();) {
+        Flight &f = fl[k];
+        if (pend[f.list] == 0) {
+          if (flight_finish(reqs[f.req], f, res[f.req]) != 0) return -1;
+          res[f.req].finished = el();
+          if (gxy_batch_remove_list(B, f.list) != 0) return -1;
+          flight_free(f);
+          fl.erase(fl.begin() + k);
+          done++;
+        } else {
+          k++;
+        }
+      }
+    }
+    gxy_batch_stats b1;
+    if (gxy_batch_get_stats(B, &b1) != 0) return -1;
+    out.rays = b1.rays - b0.rays;
+    out.samples = b1.samples - b0.samples;
+    out.passes = B->passes_run - passes0;
+  }
+  double first = n > 0 ? reqs[0].arrival : 0, last = 0;
+  for (int k = 0; k < n; k++) last = std::max(last, res[k].finished);
+  out.wall = last - first;
+  out.peak_bytes = peak;
+  if (st) *st = out;
+  return 0;
 }

@@ -1,0 +1,530 @@
+// trace_dev.h — per-ray device code of TraceRays_TraceRays shared by the
+// one-ray-per-work-item kernels (trace.hip) and the batched persistent kernel
+// (batch.hip): ray/triangle and ray/sphere tests, BVH traversal with a
+// pluggable per-lane stack, geometry shading and the full per-ray trace.
+#pragma once
+
+#include "gxy_device.h"
+#include "gxy_internal.h"
+
+namespace gxy {
+
+struct Hit {
+  float sample, t;
+  f3 point, normal, color;
+  float opacity;
+};
+
+struct GeoHit {
+  float t, u, v;
+  int geom, prim;
+};
+
+// ------------------------------------------------------------- geometry
+struct WRay {
+  f3 org, dir;
+  int kx, ky, kz;
+  float Sx, Sy, Sz;
+};
+
+__device__ __forceinline__ float comp3(f3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+
+__device__ __forceinline__ WRay make_wray(f3 o, f3 d) {
+  WRay r;
+  r.org = o;
+  r.dir = d;
+  float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+  int kz = 0;
+  float m = ax;
+  if (ay > m) { kz = 1; m = ay; }
+  if (az > m) { kz = 2; }
+  int kx = kz + 1 == 3 ? 0 : kz + 1;
+  int ky = kx + 1 == 3 ? 0 : kx + 1;
+  if (comp3(d, kz) < 0.f) { int t = kx; kx = ky; ky = t; }
+  r.kx = kx; r.ky = ky; r.kz = kz;
+  float dz = comp3(d, kz);
+  r.Sx = comp3(d, kx) / dz;
+  r.Sy = comp3(d, ky) / dz;
+  r.Sz = 1.0f / dz;
+  return r;
+}
+
+// Watertight ray/triangle test (Woop, Benthin, Wald 2013); accepts t0 < t <= tfar.
+__device__ __forceinline__ bool tri_hit(const WRay &r, f3 p0, f3 p1, f3 p2, float t0, float tfar, float &t,
+                                        float &u, float &v) {
+  f3 A = sub3(p0, r.org), B = sub3(p1, r.org), Cc = sub3(p2, r.org);
+  float Az_ = comp3(A, r.kz), Bz_ = comp3(B, r.kz), Cz_ = comp3(Cc, r.kz);
+  float Ax = comp3(A, r.kx) - r.Sx * Az_, Ay = comp3(A, r.ky) - r.Sy * Az_;
+  float Bx = comp3(B, r.kx) - r.Sx * Bz_, By = comp3(B, r.ky) - r.Sy * Bz_;
+  float Cx = comp3(Cc, r.kx) - r.Sx * Cz_, Cy = comp3(Cc, r.ky) - r.Sy * Cz_;
+  float U = Cx * By - Cy * Bx;
+  float V = Ax * Cy - Ay * Cx;
+  float W = Bx * Ay - By * Ax;
+  if (U == 0.f || V == 0.f || W == 0.f) {
+    double CxBy = (double)Cx * (double)By, CyBx = (double)Cy * (double)Bx;
+    U = (float)(CxBy - CyBx);
+    double AxCy = (double)Ax * (double)Cy, AyCx = (double)Ay * (double)Cx;
+    V = (float)(AxCy - AyCx);
+    double BxAy = (double)Bx * (double)Ay, ByAx = (double)By * (double)Ax;
+    W = (float)(BxAy - ByAx);
+  }
+  if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) return false;
+  float det = U + V + W;
+  if (det == 0.f) return false;
+  float Az = r.Sz * Az_, Bz = r.Sz * Bz_, Cz = r.Sz * Cz_;
+  float T = U * Az + V * Bz + W * Cz;
+  float rcp = 1.0f / det;
+  float tt = T * rcp;
+  if (!(tt > t0 && tt <= tfar)) return false;
+  t = tt;
+  u = V * rcp;
+  v = W * rcp;
+  return true;
+}
+
+// DataDrivenSpheres_intersect_kernel (DataDrivenSpheres.ispc:118-141).
+__device__ __forceinline__ bool sphere_hit(f3 center, float radius, float eps, f3 org, f3 dir, float t0,
+                                           float tfar, float &t) {
+  float approxDist = dot3(sub3(center, org), dir);
+  f3 closeOrg = add3(org, scl3(approxDist, dir));
+  f3 A = sub3(center, closeOrg);
+  float a = dot3(dir, dir);
+  float b = 2.f * dot3(dir, A);
+  float c = dot3(A, A) - radius * radius;
+  float radical = b * b - 4.f * a * c;
+  if (radical < 0.f) return false;
+  float srad = sqrtf(radical);
+  float r2a = 1.0f / (2.f * a);
+  float t_in = (b - srad) * r2a + approxDist;
+  float t_out = (b + srad) * r2a + approxDist;
+  if (t_in > t0 && t_in < tfar) { t = t_in; return true; }
+  if (t_out > (t0 + eps) && t_out < tfar) { t = t_out; return true; }
+  return false;
+}
+
+__device__ __forceinline__ bool better(float t, int g, int p, const GeoHit &h) {
+  if (t < h.t) return true;
+  if (t == h.t && (g < h.geom || (g == h.geom && p < h.prim))) return true;
+  return false;
+}
+
+#define GXY_BVH_STACK 64
+#define GXY_SENTINEL 0x7fffffff
+
+// Per-lane traversal stacks.  PrivStack keeps the entries in private memory
+// (scratch, cached per lane); LdsStack keeps them in LDS with entry k of a
+// lane at base[k * stride] (stride = lanes of the workgroup, so lanes at the
+// same depth touch consecutive words).  DEPTH entries; a model whose tree is
+// deeper than DEPTH - 2 must not be traced with it (checked on the host).
+template <int SD>
+struct PrivStack {
+  static constexpr int DEPTH = SD;
+  int s[SD];
+  __device__ __forceinline__ void put(int k, int v) { s[k] = v; }
+  __device__ __forceinline__ int get(int k) const { return s[k]; }
+};
+
+template <int SD>
+struct LdsStack {
+  static constexpr int DEPTH = SD;
+  int *base;
+  int stride;
+  __device__ __forceinline__ void put(int k, int v) { base[k * stride] = v; }
+  __device__ __forceinline__ int get(int k) const { return base[k * stride]; }
+};
+
+// Slab test of one child box against [t0, tmax] with a few ulps of slack so
+// that a primitive hit at exactly the current t still reaches its leaf.
+__device__ __forceinline__ bool child_hit(const float *lo, const float *hi, f3 o, f3 inv, float t0, float tmax,
+                                          float &tenter) {
+  float ax = (lo[0] - o.x) * inv.x, bx = (hi[0] - o.x) * inv.x;
+  float ay = (lo[1] - o.y) * inv.y, by = (hi[1] - o.y) * inv.y;
+  float az = (lo[2] - o.z) * inv.z, bz = (hi[2] - o.z) * inv.z;
+  float nx = inv.x < 0.f ? bx : ax, fx = inv.x < 0.f ? ax : bx;
+  float ny = inv.y < 0.f ? by : ay, fy = inv.y < 0.f ? ay : by;
+  float nz = inv.z < 0.f ? bz : az, fz = inv.z < 0.f ? az : bz;
+  float tn = mxf(mxf(nx, ny), mxf(nz, t0));
+  float tf = mnf(mnf(fx, fy), mnf(fz, tmax));
+  tenter = tn;
+  return tn <= tf * 1.0000005f + 1e-30f;
+}
+
+template <class Stack, class LeafFn>
+__device__ __forceinline__ void bvh_walk(const BVHNode *nodes, f3 o, f3 inv, float t0, const GeoHit &h,
+                                         Stack &stack, LeafFn &&leaf) {
+  int sp = 0;
+  stack.put(sp++, GXY_SENTINEL);
+  int cur = 0;
+  while (cur != GXY_SENTINEL) {
+    if (cur >= 0) {
+      const float4 *np = (const float4 *)(gptr(nodes) + cur);
+      float4 q0 = np[0], q1 = np[1], q2 = np[2];
+      int4 q3 = ((const int4 *)np)[3];
+      float lo0[3] = {q0.x, q0.y, q0.z}, hi0[3] = {q0.w, q1.x, q1.y};
+      float lo1[3] = {q1.z, q1.w, q2.x}, hi1[3] = {q2.y, q2.z, q2.w};
+      float e0, e1;
+      bool h0 = child_hit(lo0, hi0, o, inv, t0, h.t, e0);
+      bool h1 = child_hit(lo1, hi1, o, inv, t0, h.t, e1);
+      if (h0 && h1) {
+        int nearc = e0 <= e1 ? q3.x : q3.y;
+        int farc = e0 <= e1 ? q3.y : q3.x;
+        if (sp < Stack::DEPTH) stack.put(sp++, farc);
+        cur = nearc;
+      } else if (h0) {
+        cur = q3.x;
+      } else if (h1) {
+        cur = q3.y;
+      } else {
+        cur = stack.get(--sp);
+      }
+    } else {
+      int code = ~cur;
+      leaf(code >> 4, (code & 15) + 1);
+      cur = stack.get(--sp);
+    }
+  }
+}
+
+template <class Stack>
+__device__ __forceinline__ void model_trace(const DModel &M, f3 org, f3 dir, float t0, float tfar, GeoHit &h,
+                                            Stack &stack) {
+  h.t = tfar;
+  h.geom = -1;
+  h.prim = -1;
+  h.u = h.v = 0.f;
+  if (tfar < t0) return;
+  f3 inv = mk3(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
+  if (M.ntris > 0) {
+    WRay wr = make_wray(org, dir);
+    bvh_walk(M.tnodes, org, inv, t0, h, stack, [&](int start, int count) {
+      for (int k = start; k < start + count; k++) {
+        const float *p = gptr(M.tverts) + 9 * (int64_t)k;
+        f3 p0 = mk3(p[0], p[1], p[2]), p1 = mk3(p[3], p[4], p[5]), p2 = mk3(p[6], p[7], p[8]);
+        float t, u, v;
+        if (tri_hit(wr, p0, p1, p2, t0, h.t, t, u, v)) {
+          int g = gptr(M.tgeom)[k], pr = gptr(M.tprim)[k];
+          if (h.geom < 0 || better(t, g, pr, h)) { h.t = t; h.geom = g; h.prim = pr; h.u = u; h.v = v; }
+        }
+      }
+    });
+  }
+  if (M.nsph > 0) {
+    bvh_walk(M.snodes, org, inv, t0, h, stack, [&](int start, int count) {
+      for (int k = start; k < start + count; k++) {
+        const float *sc = gptr(M.scenter);
+        f3 c = mk3(sc[3 * k], sc[3 * k + 1], sc[3 * k + 2]);
+        float lim = h.geom < 0 ? h.t : nextafterf(h.t, 3e38f);
+        float t;
+        if (sphere_hit(c, gptr(M.sradius)[k], gptr(M.seps)[k], org, dir, t0, lim, t)) {
+          int g = gptr(M.sgeom)[k], pr = gptr(M.sprim)[k];
+          if (h.geom < 0 || better(t, g, pr, h)) { h.t = t; h.geom = g; h.prim = pr; h.u = h.v = 0.f; }
+        }
+      }
+    });
+  }
+}
+
+// postIntersect for the hit geometry, then DG_NORMALIZE | DG_FACEFORWARD.
+__device__ void geom_shade(const DModel &M, const GeoHit &gh, f3 org, f3 dir, Hit &hit) {
+  const DGeomShade &S = gptr(M.shade)[gh.geom];
+  f3 P = add3(org, scl3(gh.t, dir));
+  f3 Ng, Ns, col = mk3(1.f, 1.f, 1.f);
+  if (S.kind == 0) {
+    const uint32_t *I = gptr(S.indices) + 3 * (int64_t)gh.prim;
+    uint32_t i0 = I[0], i1 = I[1], i2 = I[2];
+    const float *V = gptr(S.vertices);
+    f3 p0 = mk3(V[3 * i0], V[3 * i0 + 1], V[3 * i0 + 2]);
+    f3 p1 = mk3(V[3 * i1], V[3 * i1 + 1], V[3 * i1 + 2]);
+    f3 p2 = mk3(V[3 * i2], V[3 * i2 + 1], V[3 * i2 + 2]);
+    Ng = cross3(sub3(p0, p1), sub3(p2, p0));
+    Ns = Ng;
+    float b0 = 1.0f - gh.u - gh.v, b1 = gh.u, b2 = gh.v;
+    if (S.has_normals) {
+      const float *N = gptr(S.normals);
+      f3 a = mk3(N[3 * i0], N[3 * i0 + 1], N[3 * i0 + 2]);
+      f3 b = mk3(N[3 * i1], N[3 * i1 + 1], N[3 * i1 + 2]);
+      f3 c = mk3(N[3 * i2], N[3 * i2 + 1], N[3 * i2 + 2]);
+      Ns = add3(add3(scl3(b0, a), scl3(b1, b)), scl3(b2, c));
+    }
+    if (S.has_data && S.has_tf) {
+      const float *vdat = gptr(S.vdata);
+      float dv = b0 * vdat[i0] + b1 * vdat[i1] + b2 * vdat[i2];
+      col = tf_color(gptr(S.tf.col), S.tf, dv);
+    }
+  } else {
+    const float *cs = gptr(S.centers);
+    f3 c = mk3(cs[3 * gh.prim], cs[3 * gh.prim + 1], cs[3 * gh.prim + 2]);
+    Ng = sub3(P, c);
+    Ns = Ng;
+    if (S.has_tf && S.has_data) col = tf_color(gptr(S.tf.col), S.tf, gptr(S.vdata)[gh.prim]);
+  }
+  Ng = normalize3(Ng);
+  Ns = normalize3(Ns);
+  if (dot3(dir, Ng) >= 0.f) Ng = neg3(Ng);
+  if (dot3(dir, Ns) >= 0.f) Ns = neg3(Ns);
+  hit.color = col;
+  hit.opacity = 1.0f;
+  hit.point = P;
+  hit.normal = Ns;
+  hit.t = gh.t;
+}
+
+// ------------------------------------------------------------- the trace kernel
+// TraceRays.ispc:466,550,581 with the alpha threshold as a parameter
+__device__ __forceinline__ bool opaque_of(float r, float g, float b, float a, float thr) {
+  return mnf(mnf(r, g), b) >= 1.0f || a > thr;
+}
+
+// One ray of TraceRays_TraceRays (TraceRays.ispc:371-622): reads ray i of R,
+// writes its colour, t, term and surface fields back.  s_col / s_op hold the
+// NV transfer-function tables (LDS).  Returns the number of volume samples.
+template <int NV, bool GEOM, class Stack>
+__device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayList_ispc &R, const int i,
+                                              const float (*s_col)[3 * GXY_TF_MAX],
+                                              const float (*s_op)[GXY_TF_MAX], Stack &stack) {
+  constexpr int NVS = NV > 0 ? NV : 1;
+  unsigned nsamp = 0;
+  {
+    const bool shadeFlag = R.type[i] == RAY_PRIMARY;
+    f3 org = mk3(R.ox[i], R.oy[i], R.oz[i]);
+    f3 dir = mk3(R.dx[i], R.dy[i], R.dz[i]);
+    if (dir.x == 0.f) dir.x = 1e-6f;
+    if (dir.y == 0.f) dir.y = 1e-6f;
+    if (dir.z == 0.f) dir.z = 1e-6f;
+    float ray_t0 = R.t[i];
+    float ray_t = R.tMax[i];
+    const float tTimeout = ray_t;
+    float cr = R.r[i], cg = R.g[i], cb = R.b[i], ca = R.o[i];
+
+    // box clip (TraceRays.ispc:406-418)
+    float tEntry, tExitVolume;
+    box_clip(org, dir, P.blo, P.bhi, tEntry, tExitVolume);
+    if (tEntry < ray_t0) tEntry = ray_t0;
+    else if (tEntry > ray_t0) ray_t0 = tEntry;
+    ray_t = mnf(ray_t, tExitVolume);
+
+    Hit hit;
+    hit.sample = 0.f; hit.t = 0.f; hit.opacity = 0.f;
+    hit.point = mk3(0.f, 0.f, 0.f); hit.normal = hit.point; hit.color = hit.point;
+
+    // LookForSliceHit (TraceRays.ispc:143-250)
+    bool surface_hit = false;
+    int vid = -1;
+#pragma unroll
+    for (int major = 0; major < NV; major++) {
+      const int ns = P.vv[major].nSlices;
+      for (int minor = 0; minor < ns; minor++) {
+        f3 pn = mk3(P.vv[major].slices[4 * minor], P.vv[major].slices[4 * minor + 1], P.vv[major].slices[4 * minor + 2]);
+        float d = P.vv[major].slices[4 * minor + 3];
+        float denom = dot3(dir, pn);
+        if (fabsf(denom) > 0.0001f) {
+          float t = (d - dot3(org, pn)) / denom;
+          if (t >= ray_t0 && t <= ray_t) {
+            hit.normal = denom > 0 ? neg3(pn) : pn;
+            hit.opacity = 1.0f;
+            hit.t = t;
+            vid = major;
+            ray_t = t;
+            surface_hit = true;
+          }
+        }
+      }
+    }
+    if (surface_hit && shadeFlag) {
+      hit.point = add3(org, scl3(hit.t, dir));
+#pragma unroll
+      for (int major = 0; major < NV; major++)
+        if (major == vid) {
+          hit.sample = vol_sample(P.vv[major].vol, hit.point);
+          hit.color = tf_color(s_col[major], P.vv[major].tf, hit.sample);
+          hit.opacity = 1.0f;
+        }
+    }
+
+    // LookForGeometryHit (TraceRays.ispc:108-141)
+    if (GEOM) {
+      GeoHit gh;
+      model_trace(P.model, org, dir, ray_t0, ray_t, gh, stack);
+      if (P.hit_geom) { P.hit_geom[i] = gh.geom; P.hit_prim[i] = gh.prim; }
+      if (gh.geom >= 0) {
+        ray_t = gh.t;
+        surface_hit = true;
+        if (shadeFlag) geom_shade(P.model, gh, org, dir, hit);
+      }
+    }
+
+    float tTermination = ray_t;
+
+    if (NV > 0 && P.integrate) {
+      // volume march (TraceRays.ispc:446-570)
+      const float step = P.step, epsilon = P.eps;
+      float sLast[NVS], sThis[NVS];
+#pragma unroll
+      for (int v = 0; v < NVS; v++) { sLast[v] = 0.f; sThis[v] = 0.f; }
+      float tLast = tEntry + epsilon;
+      bool hit_iso = false;
+      int iso_vid = -1;
+      bool opaque = opaque_of(cr, cg, cb, ca, P.opaque_alpha);
+      float tThis = tEntry;
+      int iter = 0;
+      while (tThis <= tTermination && !opaque && !hit_iso && iter++ < GXY_MAX_MARCH) {
+        f3 coord = add3(org, scl3(tThis, dir));
+#pragma unroll
+        for (int v = 0; v < NV; v++) sThis[v] = vol_sample(P.vv[v].vol, coord);
+        nsamp += NV;
+        if (tThis > tEntry && tLast >= epsilon) {
+          // LookForIsoHit: the last matching isovalue in list order wins
+          bool ih = false;
+          int ivid = -1;
+#pragma unroll
+          for (int major = 0; major < NV; major++) {
+            float sl = sLast[major], st = sThis[major];
+            const int ni = P.vv[major].nIso;
+            for (int m = 0; m < ni; m++) {
+              float isoval = P.vv[major].iso[m];
+              if (((isoval >= sl) && (isoval < st)) || ((isoval <= sl) && (isoval > st))) {
+                ih = true;
+                ivid = major;
+                hit.t = tLast + ((isoval - sl) / (st - sl)) * (tThis - tLast);
+                hit.sample = isoval;
+              }
+            }
+          }
+          if (ih) {
+            // the shading of the hit (gradient normal, TF colour) reads nothing
+            // the rest of the march changes, so it is done once after the loop
+            iso_vid = ivid;
+            tTermination = hit.t;
+            tThis = hit.t;
+            surface_hit = true;
+            hit_iso = true;
+            f3 c2 = add3(org, scl3(tThis, dir));
+#pragma unroll
+            for (int v = 0; v < NV; v++) sThis[v] = vol_sample(P.vv[v].vol, c2);
+            nsamp += NV;
+          }
+#pragma unroll
+          for (int major = 0; major < NV; major++) {
+            if (P.vv[major].dvr) {
+              const DTF &T = P.vv[major].tf;
+              float sVolume = (sLast[major] + sThis[major]) / 2;
+              float sop = tf_opacity(s_op[major], T.no, T.lo, T.hi, T.so, sVolume);
+              if (sop > 0) {
+                if (shadeFlag) {
+                  f3 sc = tf_color(s_col[major], T, sVolume);
+                  float wo = clamp01f(sop * P.vv[major].vol.irate);
+                  float a = 1.0f - ca;
+                  cr = cr + a * (wo * sc.x);
+                  cg = cg + a * (wo * sc.y);
+                  cb = cb + a * (wo * sc.z);
+                  ca = ca + a * wo;
+                } else {
+                  float wop = ((tThis - tLast) * P.istep) * clamp01f(sop * P.vv[major].vol.irate);
+                  float f = 1.0f - wop;
+                  cr *= f; cg *= f; cb *= f; ca *= f;
+                }
+              }
+            }
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; v++) sLast[v] = sThis[v];
+        opaque = opaque_of(cr, cg, cb, ca, P.opaque_alpha);
+        if (opaque) tTermination = tThis;
+        tLast = tThis;
+        float nxt = (tThis == tEntry) ? (tEntry + epsilon)
+                    : (((tThis + step) > tTermination) && (tThis < tTermination)) ? tTermination : tThis + step;
+        // a step that no longer advances t ends the march (the reference would spin)
+        if (!(nxt > tThis) && !opaque && !hit_iso) break;
+        tThis = nxt;
+      }
+      ray_t = tTermination;
+      if (hit_iso) {
+        // LookForIsoHit shading (TraceRays.ispc:284-307)
+        hit.point = add3(org, scl3(hit.t, dir));
+        if (shadeFlag) {
+#pragma unroll
+          for (int major = 0; major < NV; major++)
+            if (major == iso_vid) {
+              hit.normal = safe_normalize3(vol_gradient(P.vv[major].vol, hit.point));
+              if (dot3(dir, hit.normal) > 0) hit.normal = neg3(hit.normal);
+              hit.color = tf_color(s_col[major], P.vv[major].tf, hit.sample);
+              hit.opacity = 1.0f;
+            }
+        }
+      }
+    }
+
+    R.r[i] = cr; R.g[i] = cg; R.b[i] = cb; R.o[i] = ca;
+    int term = opaque_of(cr, cg, cb, ca, P.opaque_alpha) ? RAY_OPAQUE : 0;
+    R.t[i] = ray_t;
+    if (surface_hit) {
+      term |= RAY_SURFACE;
+      if (hit.opacity > 0.999f) term |= RAY_OPAQUE;
+      R.sr[i] = hit.color.x; R.sg[i] = hit.color.y; R.sb[i] = hit.color.z; R.so[i] = 1.0f;
+      R.nx[i] = hit.normal.x; R.ny[i] = hit.normal.y; R.nz[i] = hit.normal.z;
+    } else if (tTermination == tExitVolume) {
+      term |= RAY_BOUNDARY;
+    } else if (tTermination == tTimeout) {
+      term |= RAY_TIMEOUT;
+    }
+    R.term[i] = term;
+  }
+  return nsamp;
+}
+
+// One ray of SamplerTraceRays_SamplerTraceRays (SamplerTraceRays.ispc:128-222)
+// with the IsoSamplerVis interval test (IsoSamplerVis.ispc:36-64): marches to
+// the next crossing of the first isovalue; t and term of ray i are written
+// back.  Returns the number of volume samples.
+template <int NV>
+__device__ __forceinline__ unsigned sample_one(const TraceParams &P, const RayList_ispc &R, const int i) {
+  unsigned nsamp = 0;
+  f3 org = mk3(R.ox[i], R.oy[i], R.oz[i]);
+  f3 dir = mk3(R.dx[i], R.dy[i], R.dz[i]);
+  float rt = R.t[i];
+  if (dir.x == 0.f) dir.x = 1e-6f;
+  if (dir.y == 0.f) dir.y = 1e-6f;
+  if (dir.z == 0.f) dir.z = 1e-6f;
+  float tEntry, tExit;
+  box_clip(org, dir, P.blo, P.bhi, tEntry, tExit);
+  if (tEntry < rt) tEntry = rt;
+  const float step = P.step;
+  float tThis = tEntry + step;
+  int hit = -1;
+  float sLast[NV], tLast[NV];
+  f3 c0 = add3(org, scl3(tEntry, dir));
+#pragma unroll
+  for (int k = 0; k < NV; k++) { sLast[k] = vol_sample(P.vv[k].vol, c0); tLast[k] = tEntry; }
+  nsamp += NV;
+  int iter = 0;
+  while (tThis <= tExit && hit == -1 && iter++ < GXY_MAX_MARCH) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+      if (hit == -1) {
+        float sThis = vol_sample(P.vv[k].vol, add3(org, scl3(tThis, dir)));
+        nsamp++;
+        float iso = P.vv[k].iso[0];
+        bool h = false;
+        float tHit = 0.f;
+        if ((sLast[k] < iso && sThis >= iso) || (sLast[k] > iso && sThis <= iso)) {
+          tHit = tLast[k] + (((iso - sLast[k]) / (sThis - sLast[k])) * (tThis - tLast[k]));
+          h = true;
+        }
+        sLast[k] = sThis;
+        tLast[k] = tThis;
+        if (h) { tThis = tHit; hit = k; }
+      }
+    }
+    if (hit != -1 || tThis == tExit) break;
+    float nt = tThis + step;
+    if (nt > tExit) nt = tExit;
+    if (!(nt > tThis)) break;
+    tThis = nt;
+  }
+  R.t[i] = hit != -1 ? tThis + 0.001f : tThis;
+  R.term[i] = hit != -1 ? RAY_SURFACE : RAY_BOUNDARY;
+  return nsamp;
+}
+
+}  // namespace gxy

@@ -84,6 +84,28 @@ def _declare(L):
         "gxy_process_raylist": (C.c_int, [vp, vp, C.c_int, vp, vp, C.c_float, C.c_int, vp, C.c_int, C.c_int,
                                           vp]),
         "gxy_sample_raylist": (C.c_int, [vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, vp]),
+        "gxy_batch_create": (vp, [C.c_int]),
+        "gxy_batch_destroy": (C.c_int, [vp]),
+        "gxy_batch_add_dataset": (C.c_int, [vp, vp, C.c_float]),
+        "gxy_batch_set_lighting": (C.c_int, [vp, vp, C.c_float]),
+        "gxy_batch_add_list": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, vp, C.c_int, C.c_int, vp, C.c_int,
+                                         vp]),
+        "gxy_batch_set_list_seed": (C.c_int, [vp, C.c_int, C.c_uint, vp, vp]),
+        "gxy_batch_remove_list": (C.c_int, [vp, C.c_int]),
+        "gxy_batch_set_config": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "gxy_batch_get_config": (C.c_int, [vp, vp, vp, vp, vp]),
+        "gxy_batch_run": (C.c_int, [vp, C.c_int, vp]),
+        "gxy_batch_step": (C.c_int, [vp, vp]),
+        "gxy_batch_pending": (C.c_int, [vp, vp, C.c_int]),
+        "gxy_batch_get_stats": (C.c_int, [vp, vp]),
+        "gxy_batch_bytes": (C.c_longlong, [vp]),
+        "gxy_rays_random": (C.c_int, [vp, C.c_int, C.c_uint, C.c_int, C.c_int, vp, vp]),
+        "gxy_server_create": (vp, [C.c_longlong, C.c_int, C.c_int]),
+        "gxy_server_destroy": (C.c_int, [vp]),
+        "gxy_server_add_dataset": (C.c_int, [vp, vp, C.c_float]),
+        "gxy_server_set_lighting": (C.c_int, [vp, vp, C.c_float]),
+        "gxy_server_batch": (vp, [vp]),
+        "gxy_server_run": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, vp]),
         "gxy_outbound_count": (C.c_int, [C.c_int]),
         "gxy_outbound_take": (vp, [C.c_int, vp, vp]),
         "gxy_outbound_clear": (C.c_int, []),

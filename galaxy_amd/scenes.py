@@ -351,6 +351,45 @@ def uv_sphere(radius=1.0, nlon=128, nlat=80):
     return MeshDesc(V, np.array(idx, np.uint32), n.astype(f32), V[:, 2].copy())
 
 
+def uv_grid_indices(nlon, nlat):
+    """Triangles of a UV sphere grid ((nlat+1) rings of nlon vertices), two per
+    quad, in the order uv_sphere builds them."""
+    i = np.arange(nlat, dtype=np.int64)[:, None]
+    j = np.arange(nlon, dtype=np.int64)[None, :]
+    a = i * nlon + j
+    b = i * nlon + (j + 1) % nlon
+    c = (i + 1) * nlon + j
+    d = (i + 1) * nlon + (j + 1) % nlon
+    t = np.stack([np.stack([a, c, b], -1), np.stack([b, c, d], -1)], 2)   # (nlat, nlon, 2, 3)
+    return t.reshape(-1, 3).astype(np.uint32)
+
+
+def blob_mesh(ntris, seed, center=(0.5, 0.5, 0.5), radius=0.32, amp=0.15):
+    """A random blob: a UV sphere with about ntris triangles (4 nlat^2, nlon =
+    2 nlat) whose radius is modulated by six seeded sinusoids of the direction,
+    r = radius (1 + amp sum_k sin(f_k (u . a_k) + p_k) / (k+1)); vertex data =
+    the modulation scaled to [0, 1]; no vertex normals (face normals shade)."""
+    nlat = max(8, int(round((ntris / 4.0) ** 0.5)))
+    nlon = 2 * nlat
+    rng = np.random.default_rng(seed)
+    axes = rng.normal(size=(6, 3))
+    axes /= np.linalg.norm(axes, axis=1, keepdims=True)
+    phase = rng.uniform(0, 2 * np.pi, 6)
+    th = np.linspace(0, np.pi, nlat + 1)
+    ph = np.linspace(0, 2 * np.pi, nlon, endpoint=False)
+    st, ct = np.sin(th)[:, None], np.cos(th)[:, None]
+    u = np.stack([np.broadcast_to(st * np.cos(ph)[None, :], (nlat + 1, nlon)),
+                  np.broadcast_to(st * np.sin(ph)[None, :], (nlat + 1, nlon)),
+                  np.broadcast_to(ct, (nlat + 1, nlon))], -1).reshape(-1, 3)
+    f = np.zeros(len(u))
+    for k in range(6):
+        f += np.sin(1.5 * (k + 1) * (u @ axes[k]) + phase[k]) / (k + 1)
+    r = radius * (1.0 + amp * f)
+    V = (u * r[:, None] + np.asarray(center)).astype(f32)
+    data = ((f - f.min()) / max(1e-9, f.max() - f.min())).astype(f32)
+    return MeshDesc(V, uv_grid_indices(nlon, nlat), None, data)
+
+
 def icosphere(subdiv, radius=1.0, center=(0, 0, 0)):
     """Icosahedron subdivided `subdiv` times (20*4^subdiv triangles), projected
     to the sphere; vertex normals radial; vertex data = distance to the

@@ -11,6 +11,11 @@
       11,796,480 total), 3840x2160 primary rays, 1 shadow ray and 16
       cosine-weighted AO rays per hit, every secondary ray spawned through the
       device RayList queue
+  c4  batched: 256 ray lists of log-uniform sizes in [1k, 1M] (origins uniform
+      in the unit cube, isotropic directions) over 64 resident datasets, each a
+      random blob mesh of 50k-5M triangles or a 128^3-1024^3 FP32 noise volume
+      (log-uniform sizes), traced together pass by pass with shadow and AO
+      secondary rays
 """
 import math
 
@@ -81,3 +86,58 @@ def c3(width=3840, height=2160, subdiv=8, n_ao=16, ao_radius=0.5):
                             n_ao=n_ao, ao_radius=ao_radius, shadows=True)
     cam = scenes.CameraDesc((0.0, -1.2, 4.2), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 50.0, width, height)
     return scenes.SceneDesc(meshes=meshes, local_box=box, global_box=box, lighting=L, cameras=[cam]), 0.999
+
+
+C2_TF = (np.array([[0.23, 0.30, 0.75], [0.34, 0.51, 0.95], [0.55, 0.69, 1.00], [0.75, 0.82, 0.93],
+                   [0.93, 0.82, 0.76], [0.97, 0.66, 0.54], [0.90, 0.45, 0.34], [0.71, 0.02, 0.15]]),
+         np.array([0.02, 0.0, 0.04, 0.0, 0.0, 0.04, 0.0, 0.03]))
+
+UNIT_BOX = ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+
+
+def unit_volume_scene(edge, seed, octaves=4, base_frequency=4.0):
+    """A noise volume of edge^3 voxels spanning the unit cube, with the volume
+    render's 8-point TF, iso 0.5 and compositing on; step 0.5 voxel."""
+    d = np.linspace(0.0, 1.0, 8)
+    rgb, op = C2_TF
+    delta = f32(1.0 / (edge - 1))
+    vol = scenes.VolumeDesc(None, (f32(0.0),) * 3, (delta,) * 3, sampling_rate=0.5, local_box=UNIT_BOX,
+                            global_box=UNIT_BOX, shape=(edge, edge, edge), noise=(seed, octaves, base_frequency))
+    vv = scenes.VolumeVisDesc(vol, colormap=np.c_[d, rgb].astype(f32), opacitymap=np.c_[d, op].astype(f32),
+                              isovalues=np.array([0.5], f32), volume_render=True)
+    return scenes.SceneDesc(volume_vis=[vv], local_box=UNIT_BOX, global_box=UNIT_BOX)
+
+
+def blob_scene(ntris, seed):
+    m = scenes.blob_mesh(ntris, seed)
+    m.colormap = np.array([[0.0, 0.9, 0.5, 0.3], [1.0, 0.3, 0.6, 0.9]], f32)
+    return scenes.SceneDesc(meshes=[m], local_box=UNIT_BOX, global_box=UNIT_BOX)
+
+
+def batch_lighting():
+    return scenes.LightingDesc(np.array([[2.0, 3.0, 4.0]], f32), np.array([2], np.int32), Ka=0.3, Kd=0.7,
+                               n_ao=2, ao_radius=0.1, shadows=True)
+
+
+def c4(nlists=256, ndatasets=64, rays=(1_000, 1_000_000), edges=(128, 1024), tris=(50_000, 5_000_000), seed=4):
+    """Returns (dataset scene descriptions, [(rays, dataset, seed), ...] per
+    list, lighting).  Dataset k is a volume when k is even, a blob mesh when
+    odd; sizes and list sizes are log-uniform in the given ranges."""
+    rng = np.random.default_rng(seed)
+
+    def logu(lo, hi):
+        return float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+
+    descs = []
+    for k in range(ndatasets):
+        if k % 2 == 0:
+            descs.append(("volume", int(round(logu(*edges))), 1000 + k))
+        else:
+            descs.append(("mesh", int(round(logu(*tris))), 2000 + k))
+    lists = [(int(round(logu(*rays))), int(rng.integers(ndatasets)), 5000 + i) for i in range(nlists)]
+    return descs, lists, batch_lighting()
+
+
+def dataset_scene(desc):
+    kind, size, s = desc
+    return unit_volume_scene(size, s) if kind == "volume" else blob_scene(size, s)

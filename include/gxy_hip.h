@@ -338,6 +338,136 @@ int gxy_process_raylist(void *vis, void *lights, int n, RayList_ispc *rays,
 int gxy_sample_raylist(void *vis, int n, RayList_ispc *rays, int max_rays_per_list,
                        float *xyz, int capacity, int *count, gxy_frame_stats *stats);
 
+
+/* ---- Batched form -------------------------------------------------------
+ * No counterpart in the reference, which traces one RayList per thread-pool
+ * task (Renderer.cpp:659-691).  Many uneven ray lists over many resident
+ * datasets are traced pass by pass: each pass is one persistent trace launch
+ * whose waves pop chunks of at most `chunk` rays of one list from a device
+ * work queue, followed by a post-pass that applies, per list, exactly the
+ * reference's per-list steps (lighting and secondary spawn of
+ * TraceRays::Trace, Classify / AssignDestinations with no neighbours,
+ * HandleTerminatedRays or Sampler::HandleTerminatedRays, KEEP_HERE re-trace)
+ * and compacts spawned and kept rays into the next pass.  Every per-ray result
+ * equals that of gxy_process_raylist / gxy_sample_raylist on the list alone;
+ * framebuffer sums may differ in float addition order. */
+typedef struct gxy_batch_stats {
+  unsigned long long rays;        /* ray passes traced */
+  unsigned long long samples;     /* volume sample points */
+  unsigned long long spawned;     /* shadow + AO rays generated */
+  unsigned long long terminated;
+  unsigned long long dropped;
+  unsigned long long kept;        /* KEEP_HERE re-traces */
+  int passes;                     /* passes run */
+  int overflow;                   /* nonzero: a pass ran out of pool or table space (results incomplete) */
+} gxy_batch_stats;
+
+enum { GXY_BATCH_RENDER = 0, GXY_BATCH_SAMPLE = 1 };
+
+/* chunk: rays per work-queue entry (64..4096, multiple of 64). */
+void *gxy_batch_create(int chunk);
+int   gxy_batch_destroy(void *batch);
+/* A resident dataset: vis holds either one VolumeVis and no model, or a
+ * model and no VolumeVis (handles as for TraceRays_TraceRays).  Returns its
+ * index.  epsilon: Renderer epsilon of the march step (0.001). */
+int   gxy_batch_add_dataset(void *batch, const Visualization_ispc *vis, float epsilon);
+/* Lighting shared by every render list; epsilon offsets spawned rays. */
+int   gxy_batch_set_lighting(void *batch, const Lighting_ispc *lights, float epsilon);
+/* A list of n device rays (consumed: traced in place) over dataset ds.
+ * GXY_BATCH_RENDER: terminated rays accumulate into fb (device, width*height*4
+ * floats, cleared when the list is added and at each gxy_batch_run).
+ * GXY_BATCH_SAMPLE: every isovalue crossing appends a particle to xyz (device,
+ * capacity*3 floats); *count (device int) is the particle total.  Returns the
+ * list index (indices of removed lists are reused). */
+int   gxy_batch_add_list(void *batch, int ds, int mode, int n, const RayList_ispc *rays, float *fb, int width,
+                         int height, float *xyz, int capacity, int *count);
+/* Regenerate the list's rays at the start of every gxy_batch_run with
+ * gxy_rays_random(seed, width, lo, hi). */
+int   gxy_batch_set_list_seed(void *batch, int list, unsigned seed, const float lo[3], const float hi[3]);
+int   gxy_batch_remove_list(void *batch, int list);
+/* Launch configuration: chunk (rays per queue entry), stack (BVH traversal
+ * stack: 0 = 64 entries in private memory, 16/24/32 = entries in LDS; must
+ * exceed the deepest tree by 2), wpe (waves per SIMD requested: 0, 4 or 5),
+ * blocks_per_cu (persistent workgroups per compute unit, 1..16). */
+int   gxy_batch_set_config(void *batch, int chunk, int stack, int wpe, int blocks_per_cu);
+int   gxy_batch_get_config(void *batch, int *chunk, int *stack, int *wpe, int *blocks_per_cu);
+/* Runs every list from its own rays to completion (clears framebuffers and
+ * particle counts first).  graph_passes > 0: the set-up and that many passes
+ * are captured once in a HIP graph and replayed on later runs (re-captured
+ * after any change of lists, datasets or configuration); passes still needed
+ * afterwards run one by one. */
+int   gxy_batch_run(void *batch, int graph_passes, gxy_batch_stats *stats);
+/* Continuous form: one pass over the rays now queued, including the primary
+ * rays of lists added since the last pass.  *next_rays receives the number
+ * of rays queued for the following pass (synchronises). */
+int   gxy_batch_step(void *batch, long long *next_rays);
+/* per_list[l] = rays of list l queued for the next pass (l < n). */
+int   gxy_batch_pending(void *batch, int *per_list, int n);
+int   gxy_batch_get_stats(void *batch, gxy_batch_stats *stats);
+/* Device bytes held by the batch's ray pools and tables. */
+long long gxy_batch_bytes(void *batch);
+
+/* Synthetic rays: origins uniform in [lo, hi], isotropic directions, t = 0,
+ * tMax = inf, colour 0, pixel (i % width, i / width), from a PCG hash of
+ * (seed, i).  type: RAY_PRIMARY etc. */
+int gxy_rays_random(const RayList_ispc *rays, int n, unsigned seed, int width, int type, const float lo[3],
+                    const float hi[3]);
+
+
+/* ---- Served form ---------------------------------------------------------
+ * No counterpart in the reference (whose multiserver renders one client
+ * command at a time).  A request loop over resident datasets: requests are
+ * admitted in arrival order as slots and device memory (an HBM budget) allow,
+ * their rays join the batch of in-flight rays at the next pass boundary
+ * (continuous batching over gxy_batch_step), and each request completes when
+ * none of its rays remain queued.  The same stream can be served one request
+ * at a time through the per-list loop (gxy_process_raylist /
+ * gxy_sample_raylist) for comparison. */
+typedef struct gxy_request {
+  int kind;                 /* 0 render (camera rays, lighting, shadows/AO), 1 sample (isovalue crossings) */
+  int dataset;              /* server dataset index */
+  int width, height;        /* render: image size */
+  long long nrays;          /* sample: random rays through the dataset's box */
+  unsigned seed;            /* sample: ray seed */
+  double arrival;           /* seconds after the start of the stream */
+  float eye[3], center[3], up[3], aov;   /* render: camera */
+  float *out;               /* optional host buffer: render width*height*4 floats, sample capacity*3 */
+  long long capacity;       /* sample: particle capacity */
+} gxy_request;
+
+typedef struct gxy_request_result {
+  double admitted, finished;   /* seconds after the start of the stream */
+  long long particles;         /* sample: isovalue crossings found */
+  long long rays;              /* render: primary rays generated; sample: rays */
+  int status;                  /* 0 done, -1 failed */
+  int pad_;
+} gxy_request_result;
+
+typedef struct gxy_serve_stats {
+  double wall;                 /* seconds from the first arrival to the last completion */
+  unsigned long long rays;     /* ray passes traced */
+  unsigned long long samples;  /* volume samples */
+  long long peak_bytes;        /* device bytes in use, maximum over the run */
+  int passes;                  /* batch passes (continuous) or trace launches (one at a time) */
+  int max_in_flight;
+} gxy_serve_stats;
+
+enum { GXY_SERVE_CONTINUOUS = 0, GXY_SERVE_ONE_AT_A_TIME = 1 };
+
+/* hbm_budget: device bytes the server may hold (requests + ray pools on top of
+ * what is already allocated); max_slots: requests in flight at once. */
+void *gxy_server_create(long long hbm_budget, int max_slots, int chunk);
+int   gxy_server_destroy(void *server);
+/* vis: one VolumeVis or one model (kept by the caller while the server runs). */
+int   gxy_server_add_dataset(void *server, const Visualization_ispc *vis, float epsilon);
+int   gxy_server_set_lighting(void *server, const Lighting_ispc *lights, float epsilon);
+/* The batch the server drives (for gxy_batch_set_config). */
+void *gxy_server_batch(void *server);
+/* Serves n requests (sorted by arrival) in real time; res[k] receives request
+ * k's admission and completion times. */
+int   gxy_server_run(void *server, const gxy_request *reqs, int n, int mode, gxy_request_result *res,
+                     gxy_serve_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -50,6 +50,11 @@ int main(void) {
   P(Visualization_ispc, local_bb);
   S(gxy_raylist_hdr); P(gxy_raylist_hdr, size); P(gxy_raylist_hdr, aligned_size);
   S(gxy_frame_stats); P(gxy_frame_stats, outbound); P(gxy_frame_stats, samples); P(gxy_frame_stats, lists);
+  S(gxy_batch_stats); P(gxy_batch_stats, kept); P(gxy_batch_stats, passes); P(gxy_batch_stats, overflow);
+  S(gxy_request); P(gxy_request, nrays); P(gxy_request, arrival); P(gxy_request, aov); P(gxy_request, out);
+  P(gxy_request, capacity);
+  S(gxy_request_result); P(gxy_request_result, rays); P(gxy_request_result, status);
+  S(gxy_serve_stats); P(gxy_serve_stats, peak_bytes); P(gxy_serve_stats, max_in_flight);
   return 0;
 }
 '''
@@ -77,6 +82,16 @@ int main(void) {
         "Visualization_ispc.local_bb": abi.Visualization_ispc.local_bb.offset,
         "gxy_frame_stats": C.sizeof(abi.FrameStats), "gxy_frame_stats.outbound": abi.FrameStats.outbound.offset,
         "gxy_frame_stats.samples": abi.FrameStats.samples.offset, "gxy_frame_stats.lists": abi.FrameStats.lists.offset,
+        "gxy_batch_stats": C.sizeof(abi.BatchStats), "gxy_batch_stats.kept": abi.BatchStats.kept.offset,
+        "gxy_batch_stats.passes": abi.BatchStats.passes.offset,
+        "gxy_batch_stats.overflow": abi.BatchStats.overflow.offset,
+        "gxy_request": C.sizeof(abi.Request), "gxy_request.nrays": abi.Request.nrays.offset,
+        "gxy_request.arrival": abi.Request.arrival.offset, "gxy_request.aov": abi.Request.aov.offset,
+        "gxy_request.out": abi.Request.out.offset, "gxy_request.capacity": abi.Request.capacity.offset,
+        "gxy_request_result": C.sizeof(abi.RequestResult), "gxy_request_result.rays": abi.RequestResult.rays.offset,
+        "gxy_request_result.status": abi.RequestResult.status.offset,
+        "gxy_serve_stats": C.sizeof(abi.ServeStats), "gxy_serve_stats.peak_bytes": abi.ServeStats.peak_bytes.offset,
+        "gxy_serve_stats.max_in_flight": abi.ServeStats.max_in_flight.offset,
     }
     for k, v in py.items():
         assert got[k] == v, (k, got[k], v)

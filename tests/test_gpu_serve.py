@@ -1,0 +1,102 @@
+"""Served form: a Poisson stream of render and sample requests over resident
+datasets gives every request the same result served with continuous batching,
+served one at a time through the per-list loop, and served alone; an HBM
+budget limits how many requests are in flight without changing any result."""
+import numpy as np
+import pytest
+
+from batch_scenes import dataset_descs
+from galaxy_amd import abi
+
+pytestmark = pytest.mark.gpu
+f32 = np.float32
+
+
+@pytest.fixture(scope="module")
+def server(hip):
+    from galaxy_amd import serve
+    S, built = serve.build_server(hip, dataset_descs(), budget=8 << 30, slots=64, chunk=256)
+    return S
+
+
+def stream(seed=3, n=14):
+    from galaxy_amd import serve
+    rng = np.random.default_rng(seed)
+    return serve.request_stream(n, 200.0, 0.55, rng, render_px=(40 * 40, 110 * 110), sample_rays=(2000, 20000),
+                                volume_ds=(0, 1), any_ds=(0, 1, 2, 3), keep_outputs=True)
+
+
+def outputs(reqs, res, outs):
+    got = []
+    for k, o in enumerate(outs):
+        r = reqs[k]
+        if r.kind == 0:
+            got.append(o.copy())
+        else:
+            c = int(res[k].particles)
+            a = o[:3 * min(c, r.capacity)].reshape(-1, 3).copy()
+            got.append((c, a[np.lexsort(a.T)]))
+    return got
+
+
+def same(a, b):
+    for k, (x, y) in enumerate(zip(a, b)):
+        if isinstance(x, tuple):
+            assert x[0] == y[0], k
+            assert np.array_equal(x[1], y[1]), k
+        else:
+            assert np.abs(x - y).max() <= 1e-5, k
+
+
+def test_stream_continuous_equals_one_at_a_time(server):
+    from galaxy_amd import serve
+    reqs, outs = stream()
+    res, st = server.run(reqs, abi.SERVE_CONTINUOUS)
+    a = outputs(reqs, res, outs)
+    for o in outs:
+        o[:] = 0
+    res1, st1 = server.run(reqs, abi.SERVE_ONE_AT_A_TIME)
+    b = outputs(reqs, res1, outs)
+    same(a, b)
+    assert st.rays == st1.rays and st.samples == st1.samples
+    assert st.max_in_flight >= 2
+    for k in range(len(reqs)):
+        assert res[k].status == 0 and res1[k].status == 0
+        assert reqs[k].arrival <= res[k].admitted + 1e-6 <= res[k].finished + 2e-6
+    lat = serve.latency_summary(reqs, res)
+    assert lat["p99_ms"] >= lat["p50_ms"] > 0
+
+
+def test_each_request_alone_equals_the_mix(server):
+    reqs, outs = stream(seed=5, n=10)
+    res, _ = server.run(reqs, abi.SERVE_CONTINUOUS)
+    mix = outputs(reqs, res, outs)
+    for k in range(len(reqs)):
+        one = (abi.Request * 1)()
+        one[0] = reqs[k]
+        one[0].arrival = 0.0
+        outs[k][:] = 0
+        r1, _ = server.run(one, abi.SERVE_CONTINUOUS)
+        alone = outputs(one, r1, [outs[k]])
+        same([mix[k]], alone)
+
+
+def test_budget_limits_requests_in_flight(hip):
+    from galaxy_amd import serve
+    reqs, outs = stream(seed=7, n=8)
+    for r in reqs:
+        r.arrival = 0.0       # all at once: without a budget every request is in flight together
+    S0, _ = serve.build_server(hip, dataset_descs(), budget=8 << 30, slots=64)
+    res, st = S0.run(reqs, abi.SERVE_CONTINUOUS)
+    ref = outputs(reqs, res, outs)
+    assert st.max_in_flight == len(reqs)
+    # the server's estimate: rays, outputs and two ray pools (1 + 3 secondaries per parent)
+    est = [r.width * r.height * (116 + 2 * 4 * 100) if r.kind == 0 else r.nrays * 300 + r.capacity * 12
+           for r in reqs]
+    S1, _ = serve.build_server(hip, dataset_descs(), budget=max(est) + 1, slots=64)
+    for o in outs:
+        o[:] = 0
+    res1, st1 = S1.run(reqs, abi.SERVE_CONTINUOUS)
+    same(ref, outputs(reqs, res1, outs))
+    assert st1.max_in_flight < len(reqs)
+    assert st1.rays == st.rays
