@@ -149,8 +149,12 @@ void k_btrace(BatchArgs A) {
     const TraceParams &P = kconst(A.ds + ds);
     const RayList_ispc R = chunk_rays(A, ch);
     acc[BS_RAYS] += ch.count;
+    const bool small = P.vv[0].vol.small != 0;
     if (mode == BM_SAMPLE) {
-      for (int k = lane; k < ch.count; k += 64) acc[BS_SAMPLES] += sample_one<1>(P, R, ch.start + k);
+      if (small)
+        for (int k = lane; k < ch.count; k += 64) acc[BS_SAMPLES] += sample_one<1, 1>(P, R, ch.start + k);
+      else
+        for (int k = lane; k < ch.count; k += 64) acc[BS_SAMPLES] += sample_one<1, 0>(P, R, ch.start + k);
     } else if (P.has_model) {
       for (int k = lane; k < ch.count; k += 64) {
         if (STK > 0) acc[BS_SAMPLES] += trace_one<0, true>(P, R, ch.start + k, s_col[w], s_op[w], lstack);
@@ -164,8 +168,14 @@ void k_btrace(BatchArgs A) {
         wave_lds_sync();
         cur_tf = ds;
       }
-      for (int k = lane; k < ch.count; k += 64)
-        acc[BS_SAMPLES] += trace_one<1, false>(P, R, ch.start + k, s_col[w], s_op[w], pstack);
+      if (small)
+        for (int k = lane; k < ch.count; k += 64)
+          acc[BS_SAMPLES] += trace_one<1, false, PrivStack<GXY_BVH_STACK>, 1>(P, R, ch.start + k, s_col[w], s_op[w],
+                                                                            pstack);
+      else
+        for (int k = lane; k < ch.count; k += 64)
+          acc[BS_SAMPLES] += trace_one<1, false, PrivStack<GXY_BVH_STACK>, 0>(P, R, ch.start + k, s_col[w], s_op[w],
+                                                                            pstack);
     }
   }
   // lane sums of the sample counts (ray counts are per wave already)

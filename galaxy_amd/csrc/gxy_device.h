@@ -36,7 +36,7 @@ struct DVolume {
   float rate;     // samplingRate
   float step;     // samplingStep (minimum spacing)
   float irate;    // 1 / samplingRate
-  int small;      // voxel count < 2^31: 32-bit index arithmetic
+  int small;      // <= 2^30 voxels, x-y plane and z extent < 2^24: 32-bit byte offsets
 };
 
 // Piecewise-linear TF: nc colours (rgb) and no opacities over [lo, hi].
@@ -156,7 +156,21 @@ struct __attribute__((packed, aligned(4))) float2u { float x, y; };
 // Trilinear sample (OSPRay StructuredVolume semantics; x, then y, then z).
 // The clamp to [0, nextafter(dims-1, 0)] is one v_med3_f32 per axis (equal to
 // the compare-select clamp for every non-NaN input; NaN is mapped to 0 after).
-__device__ __forceinline__ float vol_sample(const DVolume &V, f3 p) {
+// SM = 1: the volume is "small" (at most 2^30 voxels, an x-y plane and the
+// z extent below 2^24), so the voxel offset is two full-rate 24-bit
+// multiply-adds and the four pair loads are scalar base + one 32-bit byte
+// offset (no 64-bit address arithmetic per sample); SM = 0: 64-bit offsets;
+// SM = -1: chosen per call from V.small.
+// The gathers of one sample (issued) and their interpolation (which waits for
+// them), separable so that a march can issue the next sample's loads before it
+// composites the current one.
+struct VFetch {
+  float2u p00, p10, p01, p11;
+  float f0, f1, f2;
+};
+
+template <int SM>
+__device__ __forceinline__ VFetch vol_fetch(const DVolume &V, f3 p) {
   float l0 = (p.x - V.org[0]) * V.isp[0];
   float l1 = (p.y - V.org[1]) * V.isp[1];
   float l2 = (p.z - V.org[2]) * V.isp[2];
@@ -167,29 +181,48 @@ __device__ __forceinline__ float vol_sample(const DVolume &V, f3 p) {
   if (l1 != l1) l1 = 0.f;
   if (l2 != l2) l2 = 0.f;
   int i0 = (int)l0, i1 = (int)l1, i2 = (int)l2;
-  float f0 = l0 - (float)i0, f1 = l1 - (float)i1, f2 = l2 - (float)i2;
-  const float *d;
+  VFetch F;
+  F.f0 = l0 - (float)i0;
+  F.f1 = l1 - (float)i1;
+  F.f2 = l2 - (float)i2;
   const float *vd = gptr(V.data);
-  int sy = V.n[0], sz = V.n[0] * V.n[1];
-  if (V.small) {
-    d = vd + (unsigned)(i0 + sy * i1 + sz * i2);
+  const bool small = SM == 1 || (SM < 0 && V.small);
+  if (small) {
+    const unsigned sy = (unsigned)V.n[0], sz = (unsigned)V.n[0] * (unsigned)V.n[1];
+    const unsigned off = (unsigned)i0 + __umul24((unsigned)i1, sy) + __umul24((unsigned)i2, sz);
+    const size_t bo = (size_t)(off * 4u);
+    F.p00 = *(const float2u *)((const char *)vd + bo);
+    F.p10 = *(const float2u *)((const char *)(vd + sy) + bo);
+    F.p01 = *(const float2u *)((const char *)(vd + sz) + bo);
+    F.p11 = *(const float2u *)((const char *)(vd + sz + sy) + bo);
   } else {
-    d = vd + ((int64_t)i0 + (int64_t)sy * i1 + (int64_t)V.n[0] * V.n[1] * i2);
+    const int64_t sy = V.n[0], SZ = (int64_t)V.n[0] * V.n[1];
+    const float *d = vd + ((int64_t)i0 + sy * i1 + SZ * i2);
+    F.p00 = *(const float2u *)(d);
+    F.p10 = *(const float2u *)(d + sy);
+    F.p01 = *(const float2u *)(d + SZ);
+    F.p11 = *(const float2u *)(d + SZ + sy);
   }
-  const int64_t SZ = V.small ? (int64_t)sz : (int64_t)V.n[0] * V.n[1];
-  float2u p00 = *(const float2u *)(d);
-  float2u p10 = *(const float2u *)(d + sy);
-  float2u p01 = *(const float2u *)(d + SZ);
-  float2u p11 = *(const float2u *)(d + SZ + sy);
-  float g0 = 1.0f - f0, g1 = 1.0f - f1, g2 = 1.0f - f2;
-  float a00 = g0 * p00.x + f0 * p00.y;
-  float a10 = g0 * p10.x + f0 * p10.y;
-  float a01 = g0 * p01.x + f0 * p01.y;
-  float a11 = g0 * p11.x + f0 * p11.y;
-  float b0 = g1 * a00 + f1 * a10;
-  float b1 = g1 * a01 + f1 * a11;
-  return g2 * b0 + f2 * b1;
+  return F;
 }
+
+__device__ __forceinline__ float vol_finish(const VFetch &F) {
+  float g0 = 1.0f - F.f0, g1 = 1.0f - F.f1, g2 = 1.0f - F.f2;
+  float a00 = g0 * F.p00.x + F.f0 * F.p00.y;
+  float a10 = g0 * F.p10.x + F.f0 * F.p10.y;
+  float a01 = g0 * F.p01.x + F.f0 * F.p01.y;
+  float a11 = g0 * F.p11.x + F.f0 * F.p11.y;
+  float b0 = g1 * a00 + F.f1 * a10;
+  float b1 = g1 * a01 + F.f1 * a11;
+  return g2 * b0 + F.f2 * b1;
+}
+
+template <int SM>
+__device__ __forceinline__ float vol_sample_t(const DVolume &V, f3 p) {
+  return vol_finish(vol_fetch<SM>(V, p));
+}
+
+__device__ __forceinline__ float vol_sample(const DVolume &V, f3 p) { return vol_sample_t<-1>(V, p); }
 
 // Forward differences at one grid spacing, divided by the spacing.
 __device__ __forceinline__ f3 vol_gradient(const DVolume &V, f3 p) {

@@ -725,7 +725,9 @@ GXY_EXPORT void *gxy_volume_create(const int dims[3], const float origin[3], con
   v->d.data = v->data;
   v->d.rate = samplingRate;
   v->d.irate = 1.0f / samplingRate;
-  v->d.small = v->count < (size_t)1 << 31 ? 1 : 0;
+  // 32-bit byte offsets (at most 4 (2^30 - 1)) and 24-bit index products
+  v->d.small = (v->count <= (size_t)1 << 30 && (size_t)v->d.n[0] * v->d.n[1] < ((size_t)1 << 24) &&
+                v->d.n[2] < (1 << 24)) ? 1 : 0;
   v->d.step = std::min(spacing[0], std::min(spacing[1], spacing[2]));
   if (voxels) {
     if (hipMemcpyAsync(v->data, voxels, v->count * 4, hipMemcpyHostToDevice, S()) != hipSuccess ||

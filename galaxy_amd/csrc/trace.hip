@@ -52,7 +52,8 @@ hipError_t launch_reduce_add(const unsigned long long *part, int nb, unsigned lo
 
 // WPE > 0 asks the compiler for at least WPE waves per SIMD (fewer VGPRs);
 // the autotuner picks the variant.
-template <int NV, bool GEOM, int WPE>
+// SM = 1 when every volume is small (32-bit voxel offsets), -1 otherwise.
+template <int NV, bool GEOM, int WPE, int SM>
 __global__ __launch_bounds__(GXY_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, 8)))
 void k_trace(TraceParams P, RayList_ispc R, int n) {
   constexpr int NVS = NV > 0 ? NV : 1;
@@ -68,7 +69,7 @@ void k_trace(TraceParams P, RayList_ispc R, int n) {
   const int i = blockIdx.x * GXY_BLOCK + threadIdx.x;
   unsigned long long nsamp = 0;
   PrivStack<GXY_BVH_STACK> stack;
-  if (i < n) nsamp = trace_one<NV, GEOM>(P, R, i, s_col, s_op, stack);
+  if (i < n) nsamp = trace_one<NV, GEOM, PrivStack<GXY_BVH_STACK>, SM>(P, R, i, s_col, s_op, stack);
   if (P.sample_partials) block_sum_store(nsamp, P.sample_partials);
 }
 
@@ -225,13 +226,16 @@ __device__ __forceinline__ bool pl_step(const TraceParams &P, MarchState &m, flo
     for (int major = 0; major < NV; major++) {
       float sl = sLast[major], st = sThis[major];
       const int ni = P.vv[major].nIso;
-      for (int k = 0; k < ni; k++) {
-        float isoval = P.vv[major].iso[k];
-        if (((isoval >= sl) && (isoval < st)) || ((isoval <= sl) && (isoval > st))) {
-          ih = true;
-          ivid = major;
-          ht = m.tLast + ((isoval - sl) / (st - sl)) * (tThis - m.tLast);
-          hs = isoval;
+#pragma unroll
+      for (int k = 0; k < GXY_MAX_ISO; k++) {
+        if (k < ni) {
+          float isoval = P.vv[major].iso[k];
+          if (((isoval >= sl) && (isoval < st)) || ((isoval <= sl) && (isoval > st))) {
+            ih = true;
+            ivid = major;
+            ht = m.tLast + ((isoval - sl) / (st - sl)) * (tThis - m.tLast);
+            hs = isoval;
+          }
         }
       }
     }
@@ -354,11 +358,11 @@ hipError_t launch_trace_pl(const TraceParams &P, const RayList_ispc &R, int n, i
 }
 
 // ------------------------------------------------------------- sampler kernel
-template <int NV>
+template <int NV, int SM>
 __global__ __launch_bounds__(GXY_BLOCK) void k_sampler(TraceParams P, RayList_ispc R, int n) {
   const int i = blockIdx.x * GXY_BLOCK + threadIdx.x;
   unsigned long long nsamp = 0;
-  if (i < n) nsamp = sample_one<NV>(P, R, i);
+  if (i < n) nsamp = sample_one<NV, SM>(P, R, i);
   if (P.sample_partials) block_sum_store(nsamp, P.sample_partials);
 }
 
@@ -366,10 +370,15 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_sampler(TraceParams P, RayList_is
 template <int NV>
 static hipError_t launch_trace_nv(const TraceParams &P, const RayList_ispc &R, int n, int wpe, hipStream_t s) {
   dim3 g((n + GXY_BLOCK - 1) / GXY_BLOCK), b(GXY_BLOCK);
-  if (P.has_model) hipLaunchKernelGGL((k_trace<NV, true, 0>), g, b, 0, s, P, R, n);
-  else if (wpe == 5) hipLaunchKernelGGL((k_trace<NV, false, 5>), g, b, 0, s, P, R, n);
-  else if (wpe == 6) hipLaunchKernelGGL((k_trace<NV, false, 6>), g, b, 0, s, P, R, n);
-  else hipLaunchKernelGGL((k_trace<NV, false, 0>), g, b, 0, s, P, R, n);
+  bool small = true;
+  for (int v = 0; v < NV; v++) small = small && P.vv[v].vol.small;
+  if (P.has_model) hipLaunchKernelGGL((k_trace<NV, true, 0, -1>), g, b, 0, s, P, R, n);
+  else if (small && wpe == 5) hipLaunchKernelGGL((k_trace<NV, false, 5, 1>), g, b, 0, s, P, R, n);
+  else if (small && wpe == 6) hipLaunchKernelGGL((k_trace<NV, false, 6, 1>), g, b, 0, s, P, R, n);
+  else if (small) hipLaunchKernelGGL((k_trace<NV, false, 0, 1>), g, b, 0, s, P, R, n);
+  else if (wpe == 5) hipLaunchKernelGGL((k_trace<NV, false, 5, -1>), g, b, 0, s, P, R, n);
+  else if (wpe == 6) hipLaunchKernelGGL((k_trace<NV, false, 6, -1>), g, b, 0, s, P, R, n);
+  else hipLaunchKernelGGL((k_trace<NV, false, 0, -1>), g, b, 0, s, P, R, n);
   return hipGetLastError();
 }
 
@@ -388,11 +397,17 @@ hipError_t launch_trace(const TraceParams &P, const RayList_ispc &R, int n, int 
 hipError_t launch_sampler(const TraceParams &P, const RayList_ispc &R, int n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   dim3 g((n + GXY_BLOCK - 1) / GXY_BLOCK), b(GXY_BLOCK);
-  switch (P.nvv) {
-    case 1: hipLaunchKernelGGL(k_sampler<1>, g, b, 0, s, P, R, n); break;
-    case 2: hipLaunchKernelGGL(k_sampler<2>, g, b, 0, s, P, R, n); break;
-    case 3: hipLaunchKernelGGL(k_sampler<3>, g, b, 0, s, P, R, n); break;
-    case 4: hipLaunchKernelGGL(k_sampler<4>, g, b, 0, s, P, R, n); break;
+  bool small = P.nvv > 0;
+  for (int v = 0; v < P.nvv && v < GXY_MAX_VOLVIS; v++) small = small && P.vv[v].vol.small;
+  switch (P.nvv * 2 + (small ? 1 : 0)) {
+    case 2: hipLaunchKernelGGL((k_sampler<1, -1>), g, b, 0, s, P, R, n); break;
+    case 3: hipLaunchKernelGGL((k_sampler<1, 1>), g, b, 0, s, P, R, n); break;
+    case 4: hipLaunchKernelGGL((k_sampler<2, -1>), g, b, 0, s, P, R, n); break;
+    case 5: hipLaunchKernelGGL((k_sampler<2, 1>), g, b, 0, s, P, R, n); break;
+    case 6: hipLaunchKernelGGL((k_sampler<3, -1>), g, b, 0, s, P, R, n); break;
+    case 7: hipLaunchKernelGGL((k_sampler<3, 1>), g, b, 0, s, P, R, n); break;
+    case 8: hipLaunchKernelGGL((k_sampler<4, -1>), g, b, 0, s, P, R, n); break;
+    case 9: hipLaunchKernelGGL((k_sampler<4, 1>), g, b, 0, s, P, R, n); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -278,7 +278,7 @@ __device__ __forceinline__ bool opaque_of(float r, float g, float b, float a, fl
 // One ray of TraceRays_TraceRays (TraceRays.ispc:371-622): reads ray i of R,
 // writes its colour, t, term and surface fields back.  s_col / s_op hold the
 // NV transfer-function tables (LDS).  Returns the number of volume samples.
-template <int NV, bool GEOM, class Stack>
+template <int NV, bool GEOM, class Stack, int SM = -1>
 __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayList_ispc &R, const int i,
                                               const float (*s_col)[3 * GXY_TF_MAX],
                                               const float (*s_op)[GXY_TF_MAX], Stack &stack) {
@@ -335,7 +335,7 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
 #pragma unroll
       for (int major = 0; major < NV; major++)
         if (major == vid) {
-          hit.sample = vol_sample(P.vv[major].vol, hit.point);
+          hit.sample = vol_sample_t<SM>(P.vv[major].vol, hit.point);
           hit.color = tf_color(s_col[major], P.vv[major].tf, hit.sample);
           hit.opacity = 1.0f;
         }
@@ -367,11 +367,29 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
       bool opaque = opaque_of(cr, cg, cb, ca, P.opaque_alpha);
       float tThis = tEntry;
       int iter = 0;
-      while (tThis <= tTermination && !opaque && !hit_iso && iter++ < GXY_MAX_MARCH) {
-        f3 coord = add3(org, scl3(tThis, dir));
+      // the gathers of the sample at tThis are issued one iteration ahead
+      // (every fetch position is clamped into the volume, so a fetch that the
+      // march does not use is harmless)
+      VFetch F[NVS];
+      {
+        f3 c0 = add3(org, scl3(tThis, dir));
 #pragma unroll
-        for (int v = 0; v < NV; v++) sThis[v] = vol_sample(P.vv[v].vol, coord);
+        for (int v = 0; v < NV; v++) F[v] = vol_fetch<SM>(P.vv[v].vol, c0);
+      }
+      while (tThis <= tTermination && !opaque && !hit_iso && iter++ < GXY_MAX_MARCH) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) sThis[v] = vol_finish(F[v]);
         nsamp += NV;
+        {
+          // the next position when neither an iso hit nor opacity ends the
+          // march here (the only cases in which the loop continues)
+          const float nx0 = (tThis == tEntry) ? (tEntry + epsilon)
+                            : (((tThis + step) > tTermination) && (tThis < tTermination)) ? tTermination
+                                                                                          : tThis + step;
+          f3 c1 = add3(org, scl3(nx0, dir));
+#pragma unroll
+          for (int v = 0; v < NV; v++) F[v] = vol_fetch<SM>(P.vv[v].vol, c1);
+        }
         if (tThis > tEntry && tLast >= epsilon) {
           // LookForIsoHit: the last matching isovalue in list order wins
           bool ih = false;
@@ -380,13 +398,17 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
           for (int major = 0; major < NV; major++) {
             float sl = sLast[major], st = sThis[major];
             const int ni = P.vv[major].nIso;
-            for (int m = 0; m < ni; m++) {
-              float isoval = P.vv[major].iso[m];
-              if (((isoval >= sl) && (isoval < st)) || ((isoval <= sl) && (isoval > st))) {
-                ih = true;
-                ivid = major;
-                hit.t = tLast + ((isoval - sl) / (st - sl)) * (tThis - tLast);
-                hit.sample = isoval;
+            // unrolled to the fixed maximum: the isovalues stay in registers
+#pragma unroll
+            for (int m = 0; m < GXY_MAX_ISO; m++) {
+              if (m < ni) {
+                float isoval = P.vv[major].iso[m];
+                if (((isoval >= sl) && (isoval < st)) || ((isoval <= sl) && (isoval > st))) {
+                  ih = true;
+                  ivid = major;
+                  hit.t = tLast + ((isoval - sl) / (st - sl)) * (tThis - tLast);
+                  hit.sample = isoval;
+                }
               }
             }
           }
@@ -400,7 +422,7 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
             hit_iso = true;
             f3 c2 = add3(org, scl3(tThis, dir));
 #pragma unroll
-            for (int v = 0; v < NV; v++) sThis[v] = vol_sample(P.vv[v].vol, c2);
+            for (int v = 0; v < NV; v++) sThis[v] = vol_sample_t<SM>(P.vv[v].vol, c2);
             nsamp += NV;
           }
 #pragma unroll
@@ -477,7 +499,7 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
 // with the IsoSamplerVis interval test (IsoSamplerVis.ispc:36-64): marches to
 // the next crossing of the first isovalue; t and term of ray i are written
 // back.  Returns the number of volume samples.
-template <int NV>
+template <int NV, int SM = -1>
 __device__ __forceinline__ unsigned sample_one(const TraceParams &P, const RayList_ispc &R, const int i) {
   unsigned nsamp = 0;
   f3 org = mk3(R.ox[i], R.oy[i], R.oz[i]);
@@ -495,14 +517,32 @@ __device__ __forceinline__ unsigned sample_one(const TraceParams &P, const RayLi
   float sLast[NV], tLast[NV];
   f3 c0 = add3(org, scl3(tEntry, dir));
 #pragma unroll
-  for (int k = 0; k < NV; k++) { sLast[k] = vol_sample(P.vv[k].vol, c0); tLast[k] = tEntry; }
+  for (int k = 0; k < NV; k++) { sLast[k] = vol_sample_t<SM>(P.vv[k].vol, c0); tLast[k] = tEntry; }
   nsamp += NV;
   int iter = 0;
+  // gathers issued one step ahead, as in the trace march (positions clamp
+  // into the volume, so an unused fetch is harmless)
+  VFetch F[NV];
+  {
+    f3 c1 = add3(org, scl3(tThis, dir));
+#pragma unroll
+    for (int k = 0; k < NV; k++) F[k] = vol_fetch<SM>(P.vv[k].vol, c1);
+  }
   while (tThis <= tExit && hit == -1 && iter++ < GXY_MAX_MARCH) {
+    float sCur[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) sCur[k] = vol_finish(F[k]);
+    {
+      float nt0 = tThis + step;
+      if (nt0 > tExit) nt0 = tExit;
+      f3 c2 = add3(org, scl3(nt0, dir));
+#pragma unroll
+      for (int k = 0; k < NV; k++) F[k] = vol_fetch<SM>(P.vv[k].vol, c2);
+    }
 #pragma unroll
     for (int k = 0; k < NV; k++) {
       if (hit == -1) {
-        float sThis = vol_sample(P.vv[k].vol, add3(org, scl3(tThis, dir)));
+        float sThis = sCur[k];
         nsamp++;
         float iso = P.vv[k].iso[0];
         bool h = false;
