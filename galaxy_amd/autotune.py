@@ -16,6 +16,9 @@ import ctypes as C
 
 DEFAULT_CANDIDATES = [(0, 256, 32), (15, 256, 32), (16, 256, 32)] + [(1, chunk, refill) for chunk in (128, 256)
                                                                      for refill in (16, 32)]
+# BVH traversal stacks of geometry-only traces: private memory (64 entries) or
+# 16 / 24 / 32 entries in LDS (a model deeper than the stack keeps the private one)
+GEOMETRY_STACKS = [0, 16, 24, 32]
 
 
 def _trace_ms(L):
@@ -44,5 +47,27 @@ def tune_trace(L, render_once, candidates=None, reps=2):
         table.append((cfg, _trace_ms(L) / reps))
     best = min(table, key=lambda x: x[1])[0]
     glib.check(L.gxy_set_trace_config(*best), "set_trace_config")
+    L.gxy_timing_reset()
+    return best, table
+
+
+def tune_geometry_stack(L, render_once, stacks=None, reps=2):
+    """The same measurement over the geometry traversal stacks.  Returns
+    (best entries, [(entries, trace ms per frame), ...]) and leaves the best set."""
+    from . import lib as glib
+    table = []
+    for st in stacks or GEOMETRY_STACKS:
+        glib.check(L.gxy_set_geometry_stack(st), "set_geometry_stack")
+        render_once()
+        glib.check(L.gxy_synchronize())
+        L.gxy_timing_reset()
+        L.gxy_timing_enable(1)
+        for _ in range(reps):
+            render_once()
+        glib.check(L.gxy_synchronize())
+        L.gxy_timing_enable(0)
+        table.append((st, _trace_ms(L) / reps))
+    best = min(table, key=lambda x: x[1])[0]
+    glib.check(L.gxy_set_geometry_stack(best), "set_geometry_stack")
     L.gxy_timing_reset()
     return best, table

@@ -86,6 +86,7 @@ struct DModel {
   // shading data, indexed by geomID
   int ngeom;
   const struct DGeomShade *shade;
+  int depth;            // deepest leaf of either tree (traversal stack bound)
 };
 
 // Per-geometry shading data (DataDrivenTriangleMesh_postIntersect,

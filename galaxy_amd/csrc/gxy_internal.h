@@ -38,8 +38,10 @@ struct CameraParams {
   float llo[3], lhi[3], glo[3], ghi[3];
 };
 
-// wpe: occupancy variant of the volume-only kernel (0 none, 5 or 6 waves/SIMD)
-hipError_t launch_trace(const TraceParams &P, const RayList_ispc &R, int n, int wpe, hipStream_t s);
+// wpe: occupancy variant of the volume-only kernel (0 none, 5 or 6 waves/SIMD);
+// stack: BVH traversal stack of geometry-only launches (0: 64 entries in
+// private memory; 16, 24, 32: entries in LDS, used when the tree fits)
+hipError_t launch_trace(const TraceParams &P, const RayList_ispc &R, int n, int wpe, int stack, hipStream_t s);
 hipError_t launch_sampler(const TraceParams &P, const RayList_ispc &R, int n, hipStream_t s);
 // Persistent-lane trace for volume-only visualizations (1..4 volumes): each
 // wave owns `chunk` consecutive rays; idle lanes are refilled when at least

@@ -86,6 +86,7 @@ struct Runtime {
   std::vector<void *> outbound[6];   // RayListObj* per exit face, last frames
   // volume-only trace kernel: 0 one ray per work-item, 1 persistent lanes
   int trace_kind = 0, trace_chunk = 256, trace_refill = 32, trace_wpe = 0;
+  int geom_stack = 0;           // BVH traversal stack of geometry-only traces (0 private, or LDS entries)
   float *uv = nullptr;          // AO direction tables U[256], V[256] (device)
 };
 static Runtime G;
@@ -569,6 +570,13 @@ GXY_EXPORT int gxy_set_trace_config(int kind, int chunk, int refill) {
   G.trace_refill = refill;
   return 0;
 }
+GXY_EXPORT int gxy_set_geometry_stack(int entries) {
+  if (entries != 0 && entries != 16 && entries != 24 && entries != 32)
+    return fail("geometry stack %d: 0 (private, 64 entries), 16, 24 or 32 (LDS)", entries);
+  G.geom_stack = entries;
+  return 0;
+}
+GXY_EXPORT int gxy_get_geometry_stack(void) { return G.geom_stack; }
 GXY_EXPORT int gxy_get_trace_config(int *kind, int *chunk, int *refill) {
   *kind = G.trace_kind == 0 && G.trace_wpe ? 10 + G.trace_wpe : G.trace_kind;
   *chunk = G.trace_chunk;
@@ -1183,6 +1191,7 @@ GXY_EXPORT int gxy_model_commit(void *model) {
     if (B.depth > 60) return fail("sphere BVH depth %d exceeds the traversal stack", B.depth);
     if (validate_bvh(B.nodes, B.nnodes, (int)nsph, "sphere") != 0) return -1;
   }
+  M->d.depth = std::max(M->tdepth, M->sdepth);
   M->committed = true;
   return 0;
 }
@@ -1215,7 +1224,7 @@ static int trace_launch(TraceParams P, const RayList_ispc &R, int n, bool sample
   {
     KTimer kt(sampler ? "sampler" : "trace");
     if (pl) HIPCHK(launch_trace_pl(P, R, n, G.trace_chunk, G.trace_refill, S()));
-    else HIPCHK(sampler ? launch_sampler(P, R, n, S()) : launch_trace(P, R, n, G.trace_wpe, S()));
+    else HIPCHK(sampler ? launch_sampler(P, R, n, S()) : launch_trace(P, R, n, G.trace_wpe, G.geom_stack, S()));
   }
   HIPCHK(launch_reduce_add(part, nb, CX().counters, S()));
   dfree_stream_ordered(part);

@@ -53,7 +53,9 @@ hipError_t launch_reduce_add(const unsigned long long *part, int nb, unsigned lo
 // WPE > 0 asks the compiler for at least WPE waves per SIMD (fewer VGPRs);
 // the autotuner picks the variant.
 // SM = 1 when every volume is small (32-bit voxel offsets), -1 otherwise.
-template <int NV, bool GEOM, int WPE, int SM>
+// STK > 0: the BVH traversal stack holds STK entries per work-item in LDS
+// (instead of 64 in private memory).
+template <int NV, bool GEOM, int WPE, int SM, int STK = 0>
 __global__ __launch_bounds__(GXY_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, 8)))
 void k_trace(TraceParams P, RayList_ispc R, int n) {
   constexpr int NVS = NV > 0 ? NV : 1;
@@ -68,8 +70,16 @@ void k_trace(TraceParams P, RayList_ispc R, int n) {
 
   const int i = blockIdx.x * GXY_BLOCK + threadIdx.x;
   unsigned long long nsamp = 0;
-  PrivStack<GXY_BVH_STACK> stack;
-  if (i < n) nsamp = trace_one<NV, GEOM, PrivStack<GXY_BVH_STACK>, SM>(P, R, i, s_col, s_op, stack);
+  if (STK > 0) {
+    __shared__ int s_stack[STK > 0 ? STK : 1][GXY_BLOCK];
+    LdsStack<(STK > 0 ? STK : 1)> stack;
+    stack.base = &s_stack[0][threadIdx.x];
+    stack.stride = GXY_BLOCK;
+    if (i < n) nsamp = trace_one<NV, GEOM, LdsStack<(STK > 0 ? STK : 1)>, SM>(P, R, i, s_col, s_op, stack);
+  } else {
+    PrivStack<GXY_BVH_STACK> stack;
+    if (i < n) nsamp = trace_one<NV, GEOM, PrivStack<GXY_BVH_STACK>, SM>(P, R, i, s_col, s_op, stack);
+  }
   if (P.sample_partials) block_sum_store(nsamp, P.sample_partials);
 }
 
@@ -368,11 +378,17 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_sampler(TraceParams P, RayList_is
 
 // ------------------------------------------------------------- launchers
 template <int NV>
-static hipError_t launch_trace_nv(const TraceParams &P, const RayList_ispc &R, int n, int wpe, hipStream_t s) {
+static hipError_t launch_trace_nv(const TraceParams &P, const RayList_ispc &R, int n, int wpe, int stack,
+                                  hipStream_t s) {
   dim3 g((n + GXY_BLOCK - 1) / GXY_BLOCK), b(GXY_BLOCK);
   bool small = true;
   for (int v = 0; v < NV; v++) small = small && P.vv[v].vol.small;
-  if (P.has_model) hipLaunchKernelGGL((k_trace<NV, true, 0, -1>), g, b, 0, s, P, R, n);
+  // an LDS stack only when it holds the deepest path of the model
+  if (NV == 0 && P.has_model && stack > 0 && P.model.depth + 2 > stack) stack = 0;
+  if (NV == 0 && P.has_model && stack == 16) hipLaunchKernelGGL((k_trace<0, true, 0, -1, 16>), g, b, 0, s, P, R, n);
+  else if (NV == 0 && P.has_model && stack == 24) hipLaunchKernelGGL((k_trace<0, true, 0, -1, 24>), g, b, 0, s, P, R, n);
+  else if (NV == 0 && P.has_model && stack == 32) hipLaunchKernelGGL((k_trace<0, true, 0, -1, 32>), g, b, 0, s, P, R, n);
+  else if (P.has_model) hipLaunchKernelGGL((k_trace<NV, true, 0, -1>), g, b, 0, s, P, R, n);
   else if (small && wpe == 5) hipLaunchKernelGGL((k_trace<NV, false, 5, 1>), g, b, 0, s, P, R, n);
   else if (small && wpe == 6) hipLaunchKernelGGL((k_trace<NV, false, 6, 1>), g, b, 0, s, P, R, n);
   else if (small) hipLaunchKernelGGL((k_trace<NV, false, 0, 1>), g, b, 0, s, P, R, n);
@@ -382,14 +398,14 @@ static hipError_t launch_trace_nv(const TraceParams &P, const RayList_ispc &R, i
   return hipGetLastError();
 }
 
-hipError_t launch_trace(const TraceParams &P, const RayList_ispc &R, int n, int wpe, hipStream_t s) {
+hipError_t launch_trace(const TraceParams &P, const RayList_ispc &R, int n, int wpe, int stack, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   switch (P.nvv) {
-    case 0: return launch_trace_nv<0>(P, R, n, wpe, s);
-    case 1: return launch_trace_nv<1>(P, R, n, wpe, s);
-    case 2: return launch_trace_nv<2>(P, R, n, wpe, s);
-    case 3: return launch_trace_nv<3>(P, R, n, wpe, s);
-    case 4: return launch_trace_nv<4>(P, R, n, wpe, s);
+    case 0: return launch_trace_nv<0>(P, R, n, wpe, stack, s);
+    case 1: return launch_trace_nv<1>(P, R, n, wpe, stack, s);
+    case 2: return launch_trace_nv<2>(P, R, n, wpe, stack, s);
+    case 3: return launch_trace_nv<3>(P, R, n, wpe, stack, s);
+    case 4: return launch_trace_nv<4>(P, R, n, wpe, stack, s);
   }
   return hipErrorInvalidValue;
 }

@@ -84,6 +84,8 @@ def _declare(L):
         "gxy_process_raylist": (C.c_int, [vp, vp, C.c_int, vp, vp, C.c_float, C.c_int, vp, C.c_int, C.c_int,
                                           vp]),
         "gxy_sample_raylist": (C.c_int, [vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, vp]),
+        "gxy_set_geometry_stack": (C.c_int, [C.c_int]),
+        "gxy_get_geometry_stack": (C.c_int, []),
         "gxy_batch_create": (vp, [C.c_int]),
         "gxy_batch_destroy": (C.c_int, [vp]),
         "gxy_batch_add_dataset": (C.c_int, [vp, vp, C.c_float]),

@@ -181,6 +181,12 @@ float       gxy_get_opacity_threshold(void);
  * `refill` (1..64) of them are idle.  Results are identical for every setting;
  * galaxy_amd/autotune.py picks the fastest. */
 int         gxy_set_trace_config(int kind, int chunk, int refill);
+/* BVH traversal stack of geometry-only traces (no reference counterpart):
+ * 0 = 64 entries per ray in private memory; 16 / 24 / 32 = that many entries
+ * in LDS, used for models whose tree depth + 2 fits (others keep the private
+ * stack).  Results are identical for every setting. */
+int         gxy_set_geometry_stack(int entries);
+int         gxy_get_geometry_stack(void);
 int         gxy_get_trace_config(int *kind, int *chunk, int *refill);
 size_t      gxy_device_bytes_total(void);
 

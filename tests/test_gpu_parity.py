@@ -345,3 +345,37 @@ def test_every_trace_launch_configuration_gives_identical_results(radial_vols_sm
                 assert np.array_equal(g[f], ref[f]), (cfg, f)
     finally:
         L_.gxy_set_trace_config(*[x.value for x in saved])
+
+
+def test_every_geometry_stack_gives_identical_hits(hip):
+    """The LDS and private traversal stacks visit the same nodes in the same
+    order: hits, t, colours and side-band geometry/primitive IDs are identical."""
+    from galaxy_amd import autotune, lib as glib, workloads
+    L_ = glib.lib()
+    sc = workloads.blob_scene(60000, 4)
+    Bh = scenes.build_scene(hip, sc)
+    hr = random_rays(40000, seed=79, lo=0.0, hi=1.0, tmax=1e30)
+    nodes, leaves, depth, sah = C.c_int(), C.c_int(), C.c_int(), C.c_float()
+    L_.gxy_model_bvh_stats(Bh.vis.model, C.byref(nodes), C.byref(leaves), C.byref(depth), C.byref(sah))
+    assert depth.value + 2 <= 32
+    geom, prim = glib.DeviceArray(4 * hr.n), glib.DeviceArray(4 * hr.n)
+    glib.check(L_.gxy_model_set_hit_output(Bh.vis.model, geom.ptr, prim.ptr))
+    ref = None
+    saved = L_.gxy_get_geometry_stack()
+    try:
+        for st in autotune.GEOMETRY_STACKS:
+            glib.check(L_.gxy_set_geometry_stack(st))
+            d = glib.DeviceRays.from_host(hr)
+            hip.trace(Bh.vis, d)
+            g = d.download()
+            ids = (geom.to_host(np.int32, hr.n), prim.to_host(np.int32, hr.n))
+            if ref is None:
+                ref = (g, ids)
+                assert (ids[0] >= 0).sum() > 10000
+                continue
+            for f in abi.FIELDS:
+                assert np.array_equal(g[f], ref[0][f]), (st, f)
+            assert np.array_equal(ids[0], ref[1][0]) and np.array_equal(ids[1], ref[1][1]), st
+    finally:
+        L_.gxy_set_geometry_stack(saved)
+        L_.gxy_model_set_hit_output(Bh.vis.model, None, None)
