@@ -206,9 +206,28 @@ def bench_batched(L, hip, steps, tune=True):
                triangles=int(sum(x[1] for x in descs if x[0] == "mesh")),
                volume_voxels=int(sum(x[1] ** 3 for x in descs if x[0] == "volume")), max_bvh_depth=depth,
                build_s=round(build_s, 2))
+    vols = [b.handles[0] for b, d in zip(built, descs) if d[0] == "volume"]
     if tune:
         best, table = gb.tune(B, max_depth=depth, graph_passes=2)
         res["tuned"] = dict(best=list(best), ms=[[list(c), round(m, 2)] for c, m in table])
+        # voxel layout of every volume dataset: linear or 8^3 bricks (datasets re-registered)
+        lay_ms = {}
+        for lay in (0, 8):
+            for v in vols:
+                glib.check(L.gxy_volume_set_layout(v, lay))
+            B.refresh_datasets(built)
+            B.run(2)
+            glib.check(L.gxy_synchronize())
+            t = time.perf_counter()
+            B.run(2)
+            glib.check(L.gxy_synchronize())
+            lay_ms[lay] = (time.perf_counter() - t) * 1e3
+        layout = min(lay_ms, key=lay_ms.get)
+        for v in vols:
+            glib.check(L.gxy_volume_set_layout(v, layout))
+        B.refresh_datasets(built)
+        res["tuned"]["volume_layout"] = layout
+        res["tuned"]["layout_ms"] = {str(k): round(v, 2) for k, v in lay_ms.items()}
         log("c4 tuned", res["tuned"])
     st = B.run(2)                      # warm (graph captured)
     glib.check(L.gxy_synchronize())
@@ -221,6 +240,7 @@ def bench_batched(L, hip, steps, tune=True):
     glib.check(L.gxy_synchronize())
     dt = time.perf_counter() - t
     res.update(mrays_per_s=round(tot["rays"] / dt / 1e6, 2), gsamples_per_s=round(tot["samples"] / dt / 1e9, 3),
+               volume_layout=L.gxy_volume_get_layout(vols[0]) if vols else 0,
                ms_per_batch=round(dt / steps * 1e3, 3), ray_passes_per_batch=int(st.rays),
                spawned_per_batch=int(st.spawned), passes=int(st.passes), steps=steps,
                config=list(B.config()), pool_bytes=int(B.nbytes()))
@@ -252,7 +272,7 @@ def bench_batched(L, hip, steps, tune=True):
     return res
 
 
-def bench_served(L, hip, nreq, seed=5):
+def bench_served(L, hip, nreq, seed=5, layout=0):
     """Poisson stream (200 requests/s) of render requests (256^2 to 3840x2160)
     and sampler requests (1M-50M rays) over resident 512^3-1024^3 noise volumes
     and 1M-20M triangle blob meshes, admitted under an HBM budget and served
@@ -263,7 +283,8 @@ def bench_served(L, hip, nreq, seed=5):
              ("mesh", 1_000_000, 21), ("mesh", 5_000_000, 22), ("mesh", 20_000_000, 23)]
     log("building c5 datasets")
     t = time.perf_counter()
-    S, built = serve.build_server(hip, [workloads.dataset_scene(d) for d in descs], budget=160 << 30, slots=256)
+    S, built = serve.build_server(hip, [workloads.dataset_scene(d) for d in descs], budget=160 << 30, slots=256,
+                                  layout=layout)
     glib.check(L.gxy_synchronize())
     build_s = time.perf_counter() - t
     rng = np.random.default_rng(seed)
@@ -271,7 +292,7 @@ def bench_served(L, hip, nreq, seed=5):
     nrender = sum(1 for r in reqs if r.kind == 0)
     res = dict(requests=nreq, render=nrender, sample=nreq - nrender, arrival_rate=200.0,
                arrival_span_s=round(reqs[nreq - 1].arrival, 3), budget_bytes=160 << 30, build_s=round(build_s, 2),
-               datasets=["%s %d" % (k, s) for k, s, _ in descs])
+               datasets=["%s %d" % (k, s) for k, s, _ in descs], volume_layout=layout)
     # warm-up: one request of each kind
     w = (abi.Request * 2)()
     w[0] = serve.render_request(0, 256, 256, (0.5, 0.5, 2.4))
@@ -323,6 +344,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the default trace launch configuration")
     ap.add_argument("--trace-config", default=None,
                     help="kind,chunk,refill: use this trace launch configuration (implies --no-tune)")
+    ap.add_argument("--volume-layout", type=int, default=None,
+                    help="voxel layout of the volume: 0 linear, 8 or 4 bricks (implies --no-tune)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device-to-device copy (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -352,12 +375,16 @@ def main():
     tuned = None
     if args.trace_config:
         glib.check(L.gxy_set_trace_config(*[int(x) for x in args.trace_config.split(",")]), "set_trace_config")
-    elif not args.no_tune:
+    if args.volume_layout is not None:
+        glib.check(L.gxy_volume_set_layout(built.handles[0], args.volume_layout), "volume_set_layout")
+    elif not args.no_tune and args.volume_layout is None:
         from galaxy_amd import autotune
         fbt = glib.DeviceArray(cam.width * cam.height * 16)
-        best, table = autotune.tune_trace(L, frame_fn(L, built, cam, fbt, sc.epsilon))
+        lay, best, table = autotune.tune_volume_layout(L, [built.handles[0]], frame_fn(L, built, cam, fbt,
+                                                                                        sc.epsilon))
         fbt.free()
-        tuned = dict(best=list(best), trace_ms=[[list(c), round(t, 4)] for c, t in table])
+        tuned = dict(best=list(best), volume_layout=lay,
+                     trace_ms=[[l_, list(c), round(t, 4)] for l_, c, t in table])
         log("autotune", tuned)
     dt, tot, kt, st, image = run_frames(L, hip, built, cam, args.steps, args.warmup, sc.epsilon)
     ms_step = dt / args.steps * 1e3
@@ -380,8 +407,11 @@ def main():
     roof["traffic"] = traffic_from_profile("trace_c2_traffic.txt")
     kcfg = (C.c_int(), C.c_int(), C.c_int())
     L.gxy_get_trace_config(*[C.byref(x) for x in kcfg])
-    roof["kernel"] = {0: "k_trace<1,false,0>", 1: "k_trace_pl<1>", 15: "k_trace<1,false,5>",
-                      16: "k_trace<1,false,6>"}.get(kcfg[0].value, str(kcfg[0].value))
+    lay = L.gxy_volume_get_layout(built.handles[0])
+    sm = lay if lay else 1
+    roof["kernel"] = {0: "k_trace<1,false,0,%d>" % sm, 1: "k_trace_pl<1>", 15: "k_trace<1,false,5,%d>" % sm,
+                      16: "k_trace<1,false,6,%d>" % sm}.get(kcfg[0].value, str(kcfg[0].value))
+    roof["volume_layout"] = lay
     roof["kernel_ms_per_launch"] = round(tr_ms / max(1, tr_n), 4)
     roof["alg_bytes_per_launch"] = int(alg_bytes / max(1, tr_n))
     roof["alg_flops_per_launch"] = int(alg_flops / max(1, tr_n))
@@ -444,7 +474,8 @@ def main():
             out["batched"] = dict(error=str(e))
     if not args.no_served:
         try:
-            out["served"] = bench_served(L, hip, args.serve_requests)
+            lay = out.get("batched", {}).get("volume_layout", 0)
+            out["served"] = bench_served(L, hip, args.serve_requests, layout=lay)
         except Exception as e:
             out["served"] = dict(error=str(e))
 

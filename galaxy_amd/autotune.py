@@ -71,3 +71,25 @@ def tune_geometry_stack(L, render_once, stacks=None, reps=2):
     glib.check(L.gxy_set_geometry_stack(best), "set_geometry_stack")
     L.gxy_timing_reset()
     return best, table
+
+
+VOLUME_LAYOUTS = [0, 8, 4]
+
+
+def tune_volume_layout(L, volumes, render_once, candidates=None, reps=2):
+    """Joint search over the voxel layout of `volumes` (linear, 8^3 or 4^3
+    bricks) and the trace kernel configuration.  Returns (best layout, best
+    config, [(layout, config, ms), ...]) and leaves both set."""
+    from . import lib as glib
+    table = []
+    for lay in VOLUME_LAYOUTS:
+        for v in volumes:
+            glib.check(L.gxy_volume_set_layout(v, lay), "volume_set_layout")
+        best, t = tune_trace(L, render_once, candidates, reps)
+        table += [(lay, cfg, ms) for cfg, ms in t]
+    lay, cfg, _ = min(table, key=lambda x: x[2])
+    for v in volumes:
+        glib.check(L.gxy_volume_set_layout(v, lay), "volume_set_layout")
+    glib.check(L.gxy_set_trace_config(*cfg), "set_trace_config")
+    L.gxy_timing_reset()
+    return lay, cfg, table

@@ -43,6 +43,12 @@ class Batch:
         self.keep.append(built)
         return glib.check(self.L.gxy_batch_add_dataset(self.h, C.byref(built.vis), eps), "batch_add_dataset")
 
+    def refresh_datasets(self, built, eps=0.001):
+        """Re-reads every dataset (built[k] was added as dataset k), e.g. after
+        gxy_volume_set_layout."""
+        for k, b in enumerate(built):
+            glib.check(self.L.gxy_batch_update_dataset(self.h, k, C.byref(b.vis), eps), "batch_update_dataset")
+
     def set_lighting(self, lights, eps=0.001):
         self.keep.append(lights)
         glib.check(self.L.gxy_batch_set_lighting(self.h, C.byref(lights), eps), "batch_set_lighting")

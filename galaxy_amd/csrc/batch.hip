@@ -149,9 +149,12 @@ void k_btrace(BatchArgs A) {
     const TraceParams &P = kconst(A.ds + ds);
     const RayList_ispc R = chunk_rays(A, ch);
     acc[BS_RAYS] += ch.count;
-    const bool small = P.vv[0].vol.small != 0;
+    // voxel addressing of the dataset's volume: bricked (8), small linear (1), large linear (0)
+    const int lay = P.vv[0].vol.brick == 8 ? 8 : (P.vv[0].vol.small ? 1 : 0);
     if (mode == BM_SAMPLE) {
-      if (small)
+      if (lay == 8)
+        for (int k = lane; k < ch.count; k += 64) acc[BS_SAMPLES] += sample_one<1, 8>(P, R, ch.start + k);
+      else if (lay == 1)
         for (int k = lane; k < ch.count; k += 64) acc[BS_SAMPLES] += sample_one<1, 1>(P, R, ch.start + k);
       else
         for (int k = lane; k < ch.count; k += 64) acc[BS_SAMPLES] += sample_one<1, 0>(P, R, ch.start + k);
@@ -168,7 +171,11 @@ void k_btrace(BatchArgs A) {
         wave_lds_sync();
         cur_tf = ds;
       }
-      if (small)
+      if (lay == 8)
+        for (int k = lane; k < ch.count; k += 64)
+          acc[BS_SAMPLES] += trace_one<1, false, PrivStack<GXY_BVH_STACK>, 8>(P, R, ch.start + k, s_col[w], s_op[w],
+                                                                            pstack);
+      else if (lay == 1)
         for (int k = lane; k < ch.count; k += 64)
           acc[BS_SAMPLES] += trace_one<1, false, PrivStack<GXY_BVH_STACK>, 1>(P, R, ch.start + k, s_col[w], s_op[w],
                                                                             pstack);

@@ -37,7 +37,14 @@ struct DVolume {
   float step;     // samplingStep (minimum spacing)
   float irate;    // 1 / samplingRate
   int small;      // <= 2^30 voxels, x-y plane and z extent < 2^24: 32-bit byte offsets
+  int brick;      // 0, or B = 4 / 8: a bricked copy of the voxels is in bdata
+  int nb[3];      // bricks per axis: ceil((n - 1) / B)
+  const float *bdata;  // bricks of (B+1)^3 voxels (one-voxel overlap), x fastest
+                       // inside a brick, each padded to a multiple of 128 bytes
 };
+
+// Floats per stored brick: (B+1)^3 rounded up to 32 (128 bytes).
+__host__ __device__ constexpr int brick_floats(int B) { return ((B + 1) * (B + 1) * (B + 1) + 31) / 32 * 32; }
 
 // Piecewise-linear TF: nc colours (rgb) and no opacities over [lo, hi].
 struct DTF {
@@ -170,6 +177,11 @@ struct VFetch {
   float f0, f1, f2;
 };
 
+// SM = 4 / 8: the bricked copy.  Cell (i0, i1, i2) lies in brick (i >> log2 B)
+// at local (i & (B-1)) and its eight corners are inside that brick (the
+// overlap voxel), so the four pair loads sit at constant offsets from one
+// address: a ray's consecutive samples stay inside one (B+1)^3 block instead
+// of touching new cache lines of four x-rows per voxel step.
 template <int SM>
 __device__ __forceinline__ VFetch vol_fetch(const DVolume &V, f3 p) {
   float l0 = (p.x - V.org[0]) * V.isp[0];
@@ -186,6 +198,18 @@ __device__ __forceinline__ VFetch vol_fetch(const DVolume &V, f3 p) {
   F.f0 = l0 - (float)i0;
   F.f1 = l1 - (float)i1;
   F.f2 = l2 - (float)i2;
+  if (SM == 4 || SM == 8) {
+    constexpr int B = SM == 4 ? 4 : 8, LB = SM == 4 ? 2 : 3, E = B + 1, PL = E * E;
+    const unsigned brick = (unsigned)(i0 >> LB) + __umul24((unsigned)(i1 >> LB), (unsigned)V.nb[0]) +
+                           __umul24((unsigned)(i2 >> LB), (unsigned)V.nb[0] * (unsigned)V.nb[1]);
+    const unsigned loc = (unsigned)(i0 & (B - 1)) + (unsigned)(i1 & (B - 1)) * E + (unsigned)(i2 & (B - 1)) * PL;
+    const float *d = gptr(V.bdata) + ((size_t)brick * brick_floats(B) + loc);
+    F.p00 = *(const float2u *)(d);
+    F.p10 = *(const float2u *)(d + E);
+    F.p01 = *(const float2u *)(d + PL);
+    F.p11 = *(const float2u *)(d + PL + E);
+    return F;
+  }
   const float *vd = gptr(V.data);
   const bool small = SM == 1 || (SM < 0 && V.small);
   if (small) {

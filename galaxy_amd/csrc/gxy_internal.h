@@ -78,6 +78,8 @@ hipError_t launch_camera(const CameraParams &C, const RayList_ispc &R, int capac
 hipError_t launch_copy_rays(const RayList_ispc &src, int start, int count, const RayList_ispc &dst, int dst_start,
                             hipStream_t s);
 
+// bricked copy of a linear volume (B = 4 or 8; out holds nb0*nb1*nb2*brick_floats(B))
+hipError_t launch_brickify(const float *data, const int n[3], int B, const int nb[3], float *out, hipStream_t s);
 hipError_t launch_noise(float *data, int nx, int ny, int nz, const int *perm_dev, int octaves, float base_freq,
                         float *minmax_scratch, hipStream_t s);
 

@@ -101,4 +101,41 @@ hipError_t launch_noise(float *data, int nx, int ny, int nz, const int *perm_dev
   return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ bricked copy
+// out[b * BF + (lz*(B+1) + ly)*(B+1) + lx] = voxel (B*bx + lx, B*by + ly,
+// B*bz + lz) clamped to the volume (clamped voxels beyond the last cell are
+// never read by a sample); padding entries are 0.
+template <int B>
+__global__ __launch_bounds__(256) void k_brickify(const float *data, int nx, int ny, int nz, int nbx, int nby,
+                                                  long long total, float *out) {
+  constexpr int E = B + 1, BF = brick_floats(B);
+  for (long long k = (long long)blockIdx.x * 256 + threadIdx.x; k < total; k += (long long)gridDim.x * 256) {
+    const long long b = k / BF;
+    const int r = (int)(k - b * BF);
+    if (r >= E * E * E) {
+      out[k] = 0.f;
+      continue;
+    }
+    const int lx = r % E, ly = (r / E) % E, lz = r / (E * E);
+    const int bx = (int)(b % nbx), by = (int)((b / nbx) % nby), bz = (int)(b / ((long long)nbx * nby));
+    const int x = min(B * bx + lx, nx - 1), y = min(B * by + ly, ny - 1), z = min(B * bz + lz, nz - 1);
+    out[k] = data[(long long)x + (long long)nx * y + (long long)nx * ny * z];
+  }
+}
+
+hipError_t launch_brickify(const float *data, const int n[3], int B, const int nb[3], float *out, hipStream_t s) {
+  const long long total = (long long)nb[0] * nb[1] * nb[2] * brick_floats(B);
+  long long g = (total + 255) / 256;
+  if (g > 65536) g = 65536;
+  if (B == 4)
+    hipLaunchKernelGGL(k_brickify<4>, dim3((unsigned)g), dim3(256), 0, s, data, n[0], n[1], n[2], nb[0], nb[1], total,
+                       out);
+  else if (B == 8)
+    hipLaunchKernelGGL(k_brickify<8>, dim3((unsigned)g), dim3(256), 0, s, data, n[0], n[1], n[2], nb[0], nb[1], total,
+                       out);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
 }  // namespace gxy

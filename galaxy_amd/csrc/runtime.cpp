@@ -253,8 +253,18 @@ static int ensure_init() {
 struct VolumeObj {
   DVolume d;
   float *data = nullptr;
+  float *bdata = nullptr;   // bricked copy (d.brick > 0)
   size_t count = 0;
 };
+
+// (re)builds the bricked copy from the linear voxels
+static int volume_brickify(VolumeObj *v) {
+  if (v->d.brick == 0) return 0;
+  const int B = v->d.brick;
+  HIPCHK(launch_brickify(v->data, v->d.n, B, v->d.nb, v->bdata, S()));
+  HIPCHK(hipStreamSynchronize(S()));
+  return 0;
+}
 
 struct TFObj {
   DTF d;
@@ -717,6 +727,7 @@ GXY_EXPORT void *gxy_volume_create(const int dims[3], const float origin[3], con
       return nullptr;
     }
   VolumeObj *v = new VolumeObj;
+  memset(&v->d, 0, sizeof v->d);
   v->count = (size_t)dims[0] * dims[1] * dims[2];
   v->data = (float *)dalloc(v->count * 4);
   if (!v->data) {
@@ -758,8 +769,40 @@ GXY_EXPORT int gxy_volume_destroy(void *vol) {
     G.volumes.erase(v);
   }
   dfree(v->data);
+  dfree(v->bdata);
   delete v;
   return 0;
+}
+GXY_EXPORT int gxy_volume_set_layout(void *vol, int brick) {
+  INIT_OR(-1);
+  VolumeObj *v = lookup<VolumeObj>(G.volumes, vol);
+  if (!v) return fail("not a volume handle");
+  if (brick != 0 && brick != 4 && brick != 8) return fail("volume layout %d: 0 (linear), 4 or 8 (bricks)", brick);
+  if (brick == v->d.brick) return 0;
+  dfree(v->bdata);
+  v->bdata = nullptr;
+  v->d.bdata = nullptr;
+  v->d.brick = 0;
+  if (brick == 0) return 0;
+  int nb[3];
+  size_t nbr = 1;
+  for (int k = 0; k < 3; k++) {
+    nb[k] = (v->d.n[k] - 1 + brick - 1) / brick;
+    nbr *= (size_t)nb[k];
+  }
+  if ((size_t)nb[0] * nb[1] >= ((size_t)1 << 24) || nb[1] >= (1 << 24) || nb[2] >= (1 << 24))
+    return fail("volume too large for the bricked layout");
+  v->bdata = (float *)dalloc(nbr * brick_floats(brick) * sizeof(float));
+  if (!v->bdata) return -1;
+  for (int k = 0; k < 3; k++) v->d.nb[k] = nb[k];
+  v->d.bdata = v->bdata;
+  v->d.brick = brick;
+  return volume_brickify(v);
+}
+GXY_EXPORT int gxy_volume_get_layout(void *vol) {
+  VolumeObj *v = lookup<VolumeObj>(G.volumes, vol);
+  if (!v) return fail("not a volume handle");
+  return v->d.brick;
 }
 GXY_EXPORT float *gxy_volume_device_data(void *vol) {
   VolumeObj *v = lookup<VolumeObj>(G.volumes, vol);
@@ -797,7 +840,7 @@ GXY_EXPORT int gxy_volume_fill_noise(void *vol, uint32_t seed, int octaves, floa
   HIPCHK(launch_noise(v->data, v->d.n[0], v->d.n[1], v->d.n[2], dperm, octaves, base_frequency, mm, S()));
   HIPCHK(hipStreamSynchronize(S()));
   dfree(dperm);
-  return 0;
+  return volume_brickify(v);
 }
 GXY_EXPORT int gxy_volume_download(void *vol, float *host) {
   VolumeObj *v = lookup<VolumeObj>(G.volumes, vol);
@@ -2270,11 +2313,25 @@ GXY_EXPORT int gxy_batch_destroy(void *b) {
   return 0;
 }
 
-GXY_EXPORT int gxy_batch_add_dataset(void *b, const Visualization_ispc *vis, float eps) {
+static int batch_dataset_params(BatchObj *B, const Visualization_ispc *vis, float eps, TraceParams &P, int &depth);
+
+GXY_EXPORT int gxy_batch_update_dataset(void *b, int ds, const Visualization_ispc *vis, float eps) {
   INIT_OR(-1);
   BatchObj *B = lookup<BatchObj>(g_batches, b);
   if (!B) return fail("not a batch handle");
+  if (ds < 0 || ds >= (int)B->ds.size()) return fail("dataset %d out of range", ds);
   TraceParams P;
+  int depth = 0;
+  if (batch_dataset_params(B, vis, eps, P, depth) != 0) return -1;
+  if ((P.nvv == 1) != (B->ds[ds].nvv == 1)) return fail("dataset %d changes kind", ds);
+  B->ds[ds] = P;
+  B->ds_depth[ds] = depth;
+  B->dirty_tables = B->dirty_graph = true;
+  return 0;
+}
+
+static int batch_dataset_params(BatchObj *B, const Visualization_ispc *vis, float eps, TraceParams &P, int &depth) {
+  (void)B;
   if (make_trace_params(vis, eps, P) != 0) return -1;
   const bool volume = P.nvv == 1 && !P.has_model, geometry = P.nvv == 0 && P.has_model;
   if (!volume && !geometry)
@@ -2283,11 +2340,21 @@ GXY_EXPORT int gxy_batch_add_dataset(void *b, const Visualization_ispc *vis, flo
   P.hit_geom = nullptr;
   P.hit_prim = nullptr;
   P.sample_partials = nullptr;
-  int depth = 0;
+  depth = 0;
   if (geometry) {
     ModelObj *M = lookup<ModelObj>(G.models, vis->model);
     depth = M ? std::max(M->tdepth, M->sdepth) : 64;
   }
+  return 0;
+}
+
+GXY_EXPORT int gxy_batch_add_dataset(void *b, const Visualization_ispc *vis, float eps) {
+  INIT_OR(-1);
+  BatchObj *B = lookup<BatchObj>(g_batches, b);
+  if (!B) return fail("not a batch handle");
+  TraceParams P;
+  int depth = 0;
+  if (batch_dataset_params(B, vis, eps, P, depth) != 0) return -1;
   B->ds.push_back(P);
   B->ds_depth.push_back(depth);
   B->dirty_tables = B->dirty_graph = true;

@@ -389,6 +389,12 @@ static hipError_t launch_trace_nv(const TraceParams &P, const RayList_ispc &R, i
   else if (NV == 0 && P.has_model && stack == 24) hipLaunchKernelGGL((k_trace<0, true, 0, -1, 24>), g, b, 0, s, P, R, n);
   else if (NV == 0 && P.has_model && stack == 32) hipLaunchKernelGGL((k_trace<0, true, 0, -1, 32>), g, b, 0, s, P, R, n);
   else if (P.has_model) hipLaunchKernelGGL((k_trace<NV, true, 0, -1>), g, b, 0, s, P, R, n);
+  else if (NV == 1 && P.vv[0].vol.brick == 8 && wpe == 5) hipLaunchKernelGGL((k_trace<NV, false, 5, 8>), g, b, 0, s, P, R, n);
+  else if (NV == 1 && P.vv[0].vol.brick == 8 && wpe == 6) hipLaunchKernelGGL((k_trace<NV, false, 6, 8>), g, b, 0, s, P, R, n);
+  else if (NV == 1 && P.vv[0].vol.brick == 8) hipLaunchKernelGGL((k_trace<NV, false, 0, 8>), g, b, 0, s, P, R, n);
+  else if (NV == 1 && P.vv[0].vol.brick == 4 && wpe == 5) hipLaunchKernelGGL((k_trace<NV, false, 5, 4>), g, b, 0, s, P, R, n);
+  else if (NV == 1 && P.vv[0].vol.brick == 4 && wpe == 6) hipLaunchKernelGGL((k_trace<NV, false, 6, 4>), g, b, 0, s, P, R, n);
+  else if (NV == 1 && P.vv[0].vol.brick == 4) hipLaunchKernelGGL((k_trace<NV, false, 0, 4>), g, b, 0, s, P, R, n);
   else if (small && wpe == 5) hipLaunchKernelGGL((k_trace<NV, false, 5, 1>), g, b, 0, s, P, R, n);
   else if (small && wpe == 6) hipLaunchKernelGGL((k_trace<NV, false, 6, 1>), g, b, 0, s, P, R, n);
   else if (small) hipLaunchKernelGGL((k_trace<NV, false, 0, 1>), g, b, 0, s, P, R, n);
@@ -415,6 +421,14 @@ hipError_t launch_sampler(const TraceParams &P, const RayList_ispc &R, int n, hi
   dim3 g((n + GXY_BLOCK - 1) / GXY_BLOCK), b(GXY_BLOCK);
   bool small = P.nvv > 0;
   for (int v = 0; v < P.nvv && v < GXY_MAX_VOLVIS; v++) small = small && P.vv[v].vol.small;
+  if (P.nvv == 1 && P.vv[0].vol.brick == 8) {
+    hipLaunchKernelGGL((k_sampler<1, 8>), g, b, 0, s, P, R, n);
+    return hipGetLastError();
+  }
+  if (P.nvv == 1 && P.vv[0].vol.brick == 4) {
+    hipLaunchKernelGGL((k_sampler<1, 4>), g, b, 0, s, P, R, n);
+    return hipGetLastError();
+  }
   switch (P.nvv * 2 + (small ? 1 : 0)) {
     case 2: hipLaunchKernelGGL((k_sampler<1, -1>), g, b, 0, s, P, R, n); break;
     case 3: hipLaunchKernelGGL((k_sampler<1, 1>), g, b, 0, s, P, R, n); break;

@@ -85,10 +85,13 @@ def _declare(L):
                                           vp]),
         "gxy_sample_raylist": (C.c_int, [vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, vp]),
         "gxy_set_geometry_stack": (C.c_int, [C.c_int]),
+        "gxy_volume_set_layout": (C.c_int, [vp, C.c_int]),
+        "gxy_volume_get_layout": (C.c_int, [vp]),
         "gxy_get_geometry_stack": (C.c_int, []),
         "gxy_batch_create": (vp, [C.c_int]),
         "gxy_batch_destroy": (C.c_int, [vp]),
         "gxy_batch_add_dataset": (C.c_int, [vp, vp, C.c_float]),
+        "gxy_batch_update_dataset": (C.c_int, [vp, C.c_int, vp, C.c_float]),
         "gxy_batch_set_lighting": (C.c_int, [vp, vp, C.c_float]),
         "gxy_batch_add_list": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, vp, C.c_int, C.c_int, vp, C.c_int,
                                          vp]),

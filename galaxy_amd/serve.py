@@ -118,12 +118,15 @@ def latency_summary(reqs, res):
                 mean_ms=round(float(lat.mean()) * 1e3, 3), max_ms=round(float(lat.max()) * 1e3, 3))
 
 
-def build_server(hip, descs, budget, slots=64, chunk=256, lighting=None):
-    """Datasets from scene descriptions; returns (Server, Built list)."""
+def build_server(hip, descs, budget, slots=64, chunk=256, lighting=None, layout=0):
+    """Datasets from scene descriptions (volumes in the given voxel layout);
+    returns (Server, Built list)."""
     from . import workloads
     S = Server(budget, slots, chunk)
     built = [scenes.build_scene(hip, sc) for sc in descs]
-    for b in built:
+    for b, sc in zip(built, descs):
+        if layout and sc.volume_vis:
+            glib.check(S.L.gxy_volume_set_layout(b.handles[0], layout), "volume_set_layout")
         S.add_dataset(b)
     keep = []
     S.set_lighting(scenes.make_lighting(lighting or workloads.batch_lighting(), keep))

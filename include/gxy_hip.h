@@ -227,6 +227,15 @@ int   gxy_volume_set_sampling_rate(void *vol, float rate);
 /* Fills the volume with seeded 4-octave gradient noise normalised to [0,1]. */
 int   gxy_volume_fill_noise(void *vol, uint32_t seed, int octaves, float base_frequency);
 int   gxy_volume_download(void *vol, float *host);
+/* Voxel layout the trace kernels read (no reference counterpart): 0 = the
+ * linear brick only; 4 / 8 = also a bricked copy (B^3 cells per brick stored
+ * with a one-voxel overlap, (B+1)^3 voxels padded to 128 bytes: 1.4x the
+ * voxels for B = 8, 2x for B = 4) that single-volume traces and samplers read.
+ * The copy is rebuilt from the linear voxels by this call and by
+ * gxy_volume_fill_noise; call it again after writing through
+ * gxy_volume_device_data.  Samples are identical in every layout. */
+int   gxy_volume_set_layout(void *vol, int brick);
+int   gxy_volume_get_layout(void *vol);
 
 /* Transfer function: the 256-entry resample of MappedVis::local_commit
  * (MappedVis.cpp:277-338).  colormap: n x [d, r, g, b]; opacitymap: m x [d, o];
@@ -377,6 +386,9 @@ int   gxy_batch_destroy(void *batch);
  * model and no VolumeVis (handles as for TraceRays_TraceRays).  Returns its
  * index.  epsilon: Renderer epsilon of the march step (0.001). */
 int   gxy_batch_add_dataset(void *batch, const Visualization_ispc *vis, float epsilon);
+/* Re-reads dataset ds from vis (after a change of its objects, e.g. a new
+ * volume layout); the kind (volume or geometry) must stay the same. */
+int   gxy_batch_update_dataset(void *batch, int ds, const Visualization_ispc *vis, float epsilon);
 /* Lighting shared by every render list; epsilon offsets spawned rays. */
 int   gxy_batch_set_lighting(void *batch, const Lighting_ispc *lights, float epsilon);
 /* A list of n device rays (consumed: traced in place) over dataset ds.

@@ -227,3 +227,30 @@ def test_continuous_batching_alone_equals_mix(datasets, hip):
         assert steps < 100
     for k in range(len(plan)):
         assert np.abs(fbs[k].to_host(f32) - alone[k]).max() <= 1e-5, k
+
+
+def test_bricked_volume_datasets_give_the_same_results(datasets, hip):
+    from galaxy_amd import lib as glib
+    L = glib.lib()
+    descs, built = datasets
+    keep = []
+    lights = lights_struct(keep)
+    plan = plan_of(SIZES, len(built), 1500)
+    B, lists = make_batch(built, lights, plan)
+    st0 = B.run(0)
+    ref = [fb.to_host(f32) for _, fb in lists]
+    vols = [b.handles[0] for b, d in zip(built, descs) if d.volume_vis]
+    try:
+        for lay in (8, 4):
+            for v in vols:
+                glib.check(L.gxy_volume_set_layout(v, lay))
+            B.refresh_datasets(built)
+            for gp in (0, 2):
+                st = B.run(gp)
+                for k in ("rays", "samples", "spawned", "terminated", "dropped", "kept"):
+                    assert getattr(st, k) == getattr(st0, k), (lay, gp, k)
+                for (l, fb), r in zip(lists, ref):
+                    assert np.abs(fb.to_host(f32) - r).max() <= 1e-5, (lay, gp, l)
+    finally:
+        for v in vols:
+            L.gxy_volume_set_layout(v, 0)

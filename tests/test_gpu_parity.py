@@ -379,3 +379,42 @@ def test_every_geometry_stack_gives_identical_hits(hip):
     finally:
         L_.gxy_set_geometry_stack(saved)
         L_.gxy_model_set_hit_output(Bh.vis.model, None, None)
+
+
+def test_every_volume_layout_gives_identical_rays(radial_vols_small, hip):
+    """The bricked copies (4^3 and 8^3 cells per brick) hold the same voxels:
+    traced rays and sampler particles are bit-identical to the linear layout."""
+    from galaxy_amd import lib as glib
+    L_ = glib.lib()
+    sc = _dvr_scene(radial_vols_small, 1)
+    Bh = scenes.build_scene(hip, sc)
+    vol = Bh.handles[0]
+    hr = random_rays(50000, seed=83, tmax=1e30)
+    vv = scenes.VolumeVisDesc(sc.volume_vis[0].volume, isovalues=np.array([0.3], f32))
+    scs = scenes.SceneDesc(volume_vis=[vv], local_box=sc.local_box, global_box=sc.global_box)
+    Bs = scenes.build_scene(hip, scs)
+    vols = Bs.handles[0]
+    hs = random_rays(20000, seed=84, typ_mix=(1.0, 0, 0))
+    cap = 200000
+    ref = None
+    for lay in (0, 8, 4, 0):
+        glib.check(L_.gxy_volume_set_layout(vol, lay))
+        glib.check(L_.gxy_volume_set_layout(vols, lay))
+        assert L_.gxy_volume_get_layout(vol) == lay
+        d = glib.DeviceRays.from_host(hr)
+        hip.trace(Bh.vis, d)
+        g = d.download()
+        ds = glib.DeviceRays.from_host(hs)
+        xyz = glib.DeviceArray(cap * 12)
+        cnt = C.c_int()
+        st = abi.FrameStats()
+        glib.check(L_.gxy_sample_raylist(C.byref(Bs.vis), hs.n, C.byref(ds.ispc), 1 << 30, xyz.ptr, cap,
+                                         C.byref(cnt), C.byref(st)))
+        parts = xyz.to_host(f32, 3 * cnt.value)
+        if ref is None:
+            ref = (g, parts, st.samples)
+            assert cnt.value > 3000
+            continue
+        for f in abi.FIELDS:
+            assert np.array_equal(g[f], ref[0][f]), (lay, f)
+        assert np.array_equal(parts, ref[1]) and st.samples == ref[2], lay
