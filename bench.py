@@ -453,13 +453,20 @@ def main():
                 t = time.perf_counter()
                 b2 = scenes.build_scene(hip, sc2)
                 build_s = time.perf_counter() - t
+                if not args.no_tune:
+                    from galaxy_amd import autotune
+                    fb2 = glib.DeviceArray(sc2.cameras[0].width * sc2.cameras[0].height * 16)
+                    autotune.tune_geometry_stack(L, frame_fn(L, b2, sc2.cameras[0], fb2, sc2.epsilon))
+                    fb2.free()
                 dt2, tot2, kt2, st2, _ = run_frames(L, hip, b2, sc2.cameras[0], steps, 1, sc2.epsilon)
                 extra[name] = dict(mrays_per_s=round(tot2["ray_passes"] / dt2 / 1e6, 2),
                                    ms_per_frame=round(dt2 / steps * 1e3, 3),
                                    rays_per_frame=tot2["ray_passes"] // steps,
                                    primary=tot2["primary"] // steps, secondary=tot2["secondary"] // steps,
                                    scene_build_s=round(build_s, 3),
-                                   trace_ms_per_frame=round(kt2["trace"][0] / steps, 3))
+                                   trace_ms_per_frame=round(kt2["trace"][0] / steps, 3),
+                                   kernel_ms_per_frame={k: round(v[0] / steps, 3) for k, v in kt2.items()},
+                                   geometry_stack=L.gxy_get_geometry_stack())
                 b2 = None
             except Exception as e:  # reported, not hidden
                 extra[name] = dict(error=str(e))

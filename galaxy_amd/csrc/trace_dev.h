@@ -285,16 +285,16 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
   constexpr int NVS = NV > 0 ? NV : 1;
   unsigned nsamp = 0;
   {
-    const bool shadeFlag = R.type[i] == RAY_PRIMARY;
-    f3 org = mk3(R.ox[i], R.oy[i], R.oz[i]);
-    f3 dir = mk3(R.dx[i], R.dy[i], R.dz[i]);
+    const bool shadeFlag = gptr(R.type)[i] == RAY_PRIMARY;
+    f3 org = mk3(gptr(R.ox)[i], gptr(R.oy)[i], gptr(R.oz)[i]);
+    f3 dir = mk3(gptr(R.dx)[i], gptr(R.dy)[i], gptr(R.dz)[i]);
     if (dir.x == 0.f) dir.x = 1e-6f;
     if (dir.y == 0.f) dir.y = 1e-6f;
     if (dir.z == 0.f) dir.z = 1e-6f;
-    float ray_t0 = R.t[i];
-    float ray_t = R.tMax[i];
+    float ray_t0 = gptr(R.t)[i];
+    float ray_t = gptr(R.tMax)[i];
     const float tTimeout = ray_t;
-    float cr = R.r[i], cg = R.g[i], cb = R.b[i], ca = R.o[i];
+    float cr = gptr(R.r)[i], cg = gptr(R.g)[i], cb = gptr(R.b)[i], ca = gptr(R.o)[i];
 
     // box clip (TraceRays.ispc:406-418)
     float tEntry, tExitVolume;
@@ -477,20 +477,20 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
       }
     }
 
-    R.r[i] = cr; R.g[i] = cg; R.b[i] = cb; R.o[i] = ca;
+    gptr(R.r)[i] = cr; gptr(R.g)[i] = cg; gptr(R.b)[i] = cb; gptr(R.o)[i] = ca;
     int term = opaque_of(cr, cg, cb, ca, P.opaque_alpha) ? RAY_OPAQUE : 0;
-    R.t[i] = ray_t;
+    gptr(R.t)[i] = ray_t;
     if (surface_hit) {
       term |= RAY_SURFACE;
       if (hit.opacity > 0.999f) term |= RAY_OPAQUE;
-      R.sr[i] = hit.color.x; R.sg[i] = hit.color.y; R.sb[i] = hit.color.z; R.so[i] = 1.0f;
-      R.nx[i] = hit.normal.x; R.ny[i] = hit.normal.y; R.nz[i] = hit.normal.z;
+      gptr(R.sr)[i] = hit.color.x; gptr(R.sg)[i] = hit.color.y; gptr(R.sb)[i] = hit.color.z; gptr(R.so)[i] = 1.0f;
+      gptr(R.nx)[i] = hit.normal.x; gptr(R.ny)[i] = hit.normal.y; gptr(R.nz)[i] = hit.normal.z;
     } else if (tTermination == tExitVolume) {
       term |= RAY_BOUNDARY;
     } else if (tTermination == tTimeout) {
       term |= RAY_TIMEOUT;
     }
-    R.term[i] = term;
+    gptr(R.term)[i] = term;
   }
   return nsamp;
 }
@@ -502,9 +502,9 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
 template <int NV, int SM = -1>
 __device__ __forceinline__ unsigned sample_one(const TraceParams &P, const RayList_ispc &R, const int i) {
   unsigned nsamp = 0;
-  f3 org = mk3(R.ox[i], R.oy[i], R.oz[i]);
-  f3 dir = mk3(R.dx[i], R.dy[i], R.dz[i]);
-  float rt = R.t[i];
+  f3 org = mk3(gptr(R.ox)[i], gptr(R.oy)[i], gptr(R.oz)[i]);
+  f3 dir = mk3(gptr(R.dx)[i], gptr(R.dy)[i], gptr(R.dz)[i]);
+  float rt = gptr(R.t)[i];
   if (dir.x == 0.f) dir.x = 1e-6f;
   if (dir.y == 0.f) dir.y = 1e-6f;
   if (dir.z == 0.f) dir.z = 1e-6f;
@@ -562,8 +562,8 @@ __device__ __forceinline__ unsigned sample_one(const TraceParams &P, const RayLi
     if (!(nt > tThis)) break;
     tThis = nt;
   }
-  R.t[i] = hit != -1 ? tThis + 0.001f : tThis;
-  R.term[i] = hit != -1 ? RAY_SURFACE : RAY_BOUNDARY;
+  gptr(R.t)[i] = hit != -1 ? tThis + 0.001f : tThis;
+  gptr(R.term)[i] = hit != -1 ? RAY_SURFACE : RAY_BOUNDARY;
   return nsamp;
 }
 
