@@ -388,15 +388,19 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_bwrite(BatchArgs A) {
           const int ab = base + prank * nsp;
           light_one(A.L, lmode, R, i, ab, nLs > 0 ? ab + A.L.nAO : -1, O);
         }
-        if (mode == BM_RENDER && f.term) {
-          const int x = R.x[i], y = R.y[i];
-          if (x >= 0 && y >= 0 && x < Ls.width && y < Ls.height) {
-            float *p = Ls.fb + 4 * ((int64_t)y * Ls.width + x);
-            atomicAdd(p + 0, R.r[i]);
-            atomicAdd(p + 1, R.g[i]);
-            atomicAdd(p + 2, R.b[i]);
-            atomicAdd(p + 3, R.o[i]);
+        if (mode == BM_RENDER) {
+          // after the lighting above (which updates the parent's colour)
+          int x = 0, y = 0;
+          float cr = 0.f, cg = 0.f, cb = 0.f, co = 0.f;
+          if (f.term) {
+            x = R.x[i];
+            y = R.y[i];
+            cr = R.r[i];
+            cg = R.g[i];
+            cb = R.b[i];
+            co = R.o[i];
           }
+          fb_accumulate_wave(Ls.fb, Ls.width, Ls.height, f.term, x, y, cr, cg, cb, co);
         }
         if (f.keep) copy_ray(R, i, O, keep_base + krank);
         if (mode == BM_SAMPLE) {

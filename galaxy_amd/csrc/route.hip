@@ -17,6 +17,9 @@ namespace gxy {
 
 __global__ __launch_bounds__(GXY_BLOCK) void k_route(RouteParams P, RayList_ispc R, int n) {
   int i = blockIdx.x * GXY_BLOCK + threadIdx.x;
+  bool acc = false;
+  int ax = 0, ay = 0;
+  float ar = 0.f, ag = 0.f, ab = 0.f, ao = 0.f;
   if (i < n) {
     const int typ = R.type[i];
     int c = R.classification[i];
@@ -29,16 +32,16 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_route(RouteParams P, RayList_ispc
     }
     if (P.mode & (RM_CLASSIFY | RM_ASSIGN)) R.classification[i] = c;
     if ((P.mode & RM_ACCUMULATE) && c == GXY_TERMINATED) {
-      int x = R.x[i], y = R.y[i];
-      if (x >= 0 && y >= 0 && x < P.width && y < P.height) {
-        float *p = P.fb + 4 * ((int64_t)y * P.width + x);
-        atomicAdd(p + 0, R.r[i]);
-        atomicAdd(p + 1, R.g[i]);
-        atomicAdd(p + 2, R.b[i]);
-        atomicAdd(p + 3, R.o[i]);
-      }
+      acc = true;
+      ax = R.x[i];
+      ay = R.y[i];
+      ar = R.r[i];
+      ag = R.g[i];
+      ab = R.b[i];
+      ao = R.o[i];
     }
   }
+  if (P.mode & RM_ACCUMULATE) fb_accumulate_wave(P.fb, P.width, P.height, acc, ax, ay, ar, ag, ab, ao);
 }
 
 hipError_t launch_route(const RouteParams &P, const RayList_ispc &R, int n, hipStream_t s) {

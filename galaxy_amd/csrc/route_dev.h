@@ -43,4 +43,37 @@ __device__ __forceinline__ int exit_face(const float *lo, const float *hi, float
   else return (dz < 0) ? 4 : 5;
 }
 
+// Rendering::AddLocalPixels (Rendering.cpp:125-153) for one wave: every lane
+// with f set adds (r, g, b, a) to pixel (x, y) of fb.  Lanes with the same
+// pixel in a contiguous run (the AO and shadow children of one parent are
+// adjacent) are summed first with a segmented suffix scan over shuffles, and
+// only the first lane of each run issues the four FP32 atomics, so children of
+// one pixel do not serialise on one address.  All 64 lanes must call it.
+__device__ __forceinline__ void fb_accumulate_wave(float *fb, int W, int H, bool f, int x, int y, float r, float g,
+                                                   float b, float a) {
+  const int lane = threadIdx.x & 63;
+  const int key = (f && x >= 0 && y >= 0 && x < W && y < H) ? y * W + x : -1;
+  if (!__any(key >= 0)) return;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int nk = __shfl_down(key, off, 64);
+    const float nr = __shfl_down(r, off, 64), ng = __shfl_down(g, off, 64);
+    const float nb = __shfl_down(b, off, 64), na = __shfl_down(a, off, 64);
+    if (lane + off < 64 && nk == key) {
+      r += nr;
+      g += ng;
+      b += nb;
+      a += na;
+    }
+  }
+  const int pk = __shfl_up(key, 1, 64);
+  if (key >= 0 && (lane == 0 || pk != key)) {
+    float *p = fb + 4 * (int64_t)key;
+    atomicAdd(p + 0, r);
+    atomicAdd(p + 1, g);
+    atomicAdd(p + 2, b);
+    atomicAdd(p + 3, a);
+  }
+}
+
 }  // namespace gxy
