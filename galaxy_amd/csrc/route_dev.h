@@ -44,26 +44,33 @@ __device__ __forceinline__ int exit_face(const float *lo, const float *hi, float
 }
 
 // Rendering::AddLocalPixels (Rendering.cpp:125-153) for one wave: every lane
-// with f set adds (r, g, b, a) to pixel (x, y) of fb.  Lanes with the same
-// pixel in a contiguous run (the AO and shadow children of one parent are
-// adjacent) are summed first with a segmented suffix scan over shuffles, and
-// only the first lane of each run issues the four FP32 atomics, so children of
-// one pixel do not serialise on one address.  All 64 lanes must call it.
+// with f set adds (r, g, b, a) to pixel (x, y) of fb.  A run of adjacent lanes
+// with the same pixel (the AO and shadow children of one parent are adjacent)
+// is summed first with a segmented suffix scan over shuffles that stops at the
+// end of each run, and only the first lane of the run issues the four FP32
+// atomics, so the children of one pixel do not serialise on one address.
+// Lanes without a pixel (key -1) form runs of their own that are never
+// written; equal pixels in separate runs are added separately.  All 64 lanes
+// must call it.
 __device__ __forceinline__ void fb_accumulate_wave(float *fb, int W, int H, bool f, int x, int y, float r, float g,
                                                    float b, float a) {
   const int lane = threadIdx.x & 63;
   const int key = (f && x >= 0 && y >= 0 && x < W && y < H) ? y * W + x : -1;
   if (!__any(key >= 0)) return;
+  // end: the window [lane, lane + off) summed so far reaches the end of the run
+  const int nkey = __shfl_down(key, 1, 64);
+  int end = (lane == 63 || nkey != key) ? 1 : 0;
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
-    const int nk = __shfl_down(key, off, 64);
     const float nr = __shfl_down(r, off, 64), ng = __shfl_down(g, off, 64);
     const float nb = __shfl_down(b, off, 64), na = __shfl_down(a, off, 64);
-    if (lane + off < 64 && nk == key) {
+    const int nend = __shfl_down(end, off, 64);
+    if (!end && lane + off < 64) {
       r += nr;
       g += ng;
       b += nb;
       a += na;
+      end = nend;
     }
   }
   const int pk = __shfl_up(key, 1, 64);
