@@ -356,14 +356,24 @@ __device__ __forceinline__ void copy_ray(const RayList_ispc &S, int i, const Ray
   D.type[j] = S.type[i]; D.term[j] = S.term[i]; D.classification[j] = S.classification[i];
 }
 
+#define BATCH_MAX_CHUNK 1024
+
+// Per chunk (one wave): (1) rank the parents and keep their ray offsets in
+// the wave's LDS slice; (2) write every child, one lane per child at
+// base + p * (nAO + nLights) + j (AO children first, then shadow children of
+// the p-th parent), from the parents' unmodified fields, so that consecutive
+// lanes store consecutive children; (3) per ray: the parent's ambient and
+// diffuse update, framebuffer accumulation of terminated rays (after that
+// update), keepers copied after the children, sampler particles.
 __global__ __launch_bounds__(GXY_BLOCK) void k_bwrite(BatchArgs A) {
-  const int lane = threadIdx.x & 63;
+  __shared__ int s_pl[BWAVES][BATCH_MAX_CHUNK];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int nc = *A.nin;
   const bool over = *A.overflow != 0;
-  const int gw = blockIdx.x * BWAVES + (threadIdx.x >> 6), nw = gridDim.x * BWAVES;
+  const int gw = blockIdx.x * BWAVES + w, nw = gridDim.x * BWAVES;
   const int nLs = A.L.doShadows ? A.L.nLights : 0;
-  const int nsp = A.L.nAO + nLs;
-  const int lmode = LM_AMBIENT | LM_DIFFUSE | (A.L.nAO > 0 ? LM_AO : 0) | (nLs > 0 ? LM_SHADOW : 0);
+  const int nAO = A.L.nAO;
+  const int nsp = nAO + nLs;
   const RayList_ispc O = rl_global(kconst(A.rls + A.out_pool));
   unsigned long long acc[BS_COUNT];
 #pragma unroll
@@ -375,24 +385,51 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_bwrite(BatchArgs A) {
     const RayList_ispc R = chunk_rays(A, ch);
     const int base = A.coff[c], npar = A.cpar[c];
     const int keep_base = base + npar * nsp;
-    int par_run = 0, keep_run = 0;
+    // (1) parents in rank order
+    int par_run = 0;
+    if (mode == BM_RENDER && nsp > 0) {
+      for (int k0 = 0; k0 < ch.count; k0 += 64) {
+        const int k = k0 + lane;
+        ChunkFlags f = chunk_flags(A, R, mode, ch.start + k, k < ch.count);
+        const unsigned long long mp = __ballot(f.par);
+        if (f.par) s_pl[w][par_run + __popcll(mp & bl_lanemask_lt())] = k;
+        par_run += __popcll(mp);
+      }
+      wave_lds_sync();
+      // (2) children
+      if (!over) {
+        for (int q = lane; q < npar * nsp; q += 64) {
+          const int p = q / nsp, j = q - p * nsp;
+          const ParentRay P = parent_load(R, ch.start + s_pl[w][p]);
+          if (j < nAO) light_ao_child(A.L, P, j, O, base + q);
+          else light_shadow_child(A.L, P, light_diffused_alpha(A.L, P.o), j - nAO, O, base + q);
+        }
+      }
+      wave_lds_sync();   // the slice is rewritten by the next chunk
+    }
+    // (3) parents' colours, accumulation, keepers, particles
+    int keep_run = 0;
     for (int k0 = 0; k0 < ch.count; k0 += 64) {
       const int k = k0 + lane, i = ch.start + k;
       ChunkFlags f = chunk_flags(A, R, mode, i, k < ch.count);
       const unsigned long long mp = __ballot(f.par), mk = __ballot(f.keep);
       const unsigned long long mt = __ballot(f.term), md = __ballot(f.drop);
-      const int prank = par_run + __popcll(mp & bl_lanemask_lt());
       const int krank = keep_run + __popcll(mk & bl_lanemask_lt());
       if (!over) {
-        if (f.par) {
-          const int ab = base + prank * nsp;
-          light_one(A.L, lmode, R, i, ab, nLs > 0 ? ab + A.L.nAO : -1, O);
-        }
         if (mode == BM_RENDER) {
-          // after the lighting above (which updates the parent's colour)
           int x = 0, y = 0;
           float cr = 0.f, cg = 0.f, cb = 0.f, co = 0.f;
-          if (f.term) {
+          if (f.par) {
+            // ambientLighting + diffuseLighting (TraceRays.cpp:124-131)
+            ParentRay P = parent_load(R, i);
+            light_ambient(A.L, P);
+            light_diffuse(A.L, P);
+            R.r[i] = P.r; R.g[i] = P.g; R.b[i] = P.b; R.o[i] = P.o;
+            if (f.term) {
+              x = P.px; y = P.py;
+              cr = P.r; cg = P.g; cb = P.b; co = P.o;
+            }
+          } else if (f.term) {
             x = R.x[i];
             y = R.y[i];
             cr = R.r[i];
@@ -424,7 +461,6 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_bwrite(BatchArgs A) {
       acc[BS_TERMINATED] += __popcll(mt);
       acc[BS_DROPPED] += __popcll(md);
       acc[BS_KEPT] += __popcll(mk);
-      par_run += __popcll(mp);
       keep_run += __popcll(mk);
     }
     if (!over) {

@@ -54,7 +54,12 @@ hipError_t launch_reduce_add(const unsigned long long *part, int nb, unsigned lo
 // Exclusive offsets of primary surface rays (flag scan).  block_counts needs
 // ceil(n/256)+1 ints of scratch; total (device int) receives the count.
 hipError_t launch_surface_scan(const RayList_ispc &R, int n, int *scratch, int *rank_out, int *total,
-                               hipStream_t s);
+                               hipStream_t s, int *pidx = nullptr);
+// Lighting with one work-item per spawned child (children at the offsets of
+// TraceRays.cpp:89-117 for the T parents whose ray indices pidx lists in rank
+// order), then the parents' ambient + diffuse update.
+hipError_t launch_lighting_children(const LightParams &L, int mode, const RayList_ispc &R, const int *pidx, int T,
+                                    const RayList_ispc &O, hipStream_t s);
 // Offsets as TraceRays.cpp:89-117 from the ranks: ao = nAO*rank, shadow =
 // nAO*total + nLights*rank (or -1).
 hipError_t launch_offsets_from_rank(const RayList_ispc &R, int n, const int *rank, const int *total, int nAO,

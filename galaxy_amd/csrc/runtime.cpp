@@ -1706,16 +1706,27 @@ static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
     C.st.passes++;
     C.st.lists++;
     if (!C.sampler) {
-      // TraceRays::Trace (TraceRays.cpp:68-146)
+      // TraceRays::Trace (TraceRays.cpp:68-146): the parents (primary surface
+      // rays) ranked by one flag scan; the AO block then the shadow block of
+      // their children, written one work-item per child
       Scratch sc;
-      int *ao_off = (int *)sc.get(sizeof(int) * n), *sh_off = (int *)sc.get(sizeof(int) * n);
-      if (!ao_off || !sh_off) return -1;
+      const int nb = (n + GXY_BLOCK - 1) / GXY_BLOCK;
+      int *scratch = (int *)sc.get(sizeof(int) * (nb + 2));
+      int *pidx = (int *)sc.get(sizeof(int) * std::max(1, n));
+      if (!scratch || !pidx) return -1;
+      int *total = scratch + nb + 1;
+      {
+        KTimer kt("spawn_scan");
+        HIPCHK(launch_surface_scan(R, n, scratch, nullptr, total, S(), pidx));
+      }
       int T = 0;
-      if (spawn_offsets(n, R, C.L.nAO, C.L.nLights, C.L.doShadows, ao_off, sh_off, &T) != 0) return -1;
-      const int nao = C.L.nAO * T, nsh = C.L.doShadows ? C.L.nLights * T : 0;
+      HIPCHK(hipMemcpyAsync(&T, total, sizeof(int), hipMemcpyDeviceToHost, S()));
+      HIPCHK(hipStreamSynchronize(S()));
+      const long long nao = (long long)C.L.nAO * T, nsh = C.L.doShadows ? (long long)C.L.nLights * T : 0;
+      if (nao + nsh > INT32_MAX - 16) return fail("%lld secondary rays exceed one RayList", nao + nsh);
       RayListObj *out = nullptr;
       if (nao + nsh > 0) {
-        out = raylist_new(nao + nsh, L->hdr.frame, 1);
+        out = raylist_new((int)(nao + nsh), L->hdr.frame, 1);
         if (!out) return -1;
       }
       int mode = LM_AMBIENT | LM_DIFFUSE | (nao ? LM_AO : 0) | (nsh ? LM_SHADOW : 0);
@@ -1724,7 +1735,7 @@ static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
       if (out) O = out->ispc;
       {
         KTimer kt("lighting");
-        HIPCHK(launch_lighting(C.L, mode, R, n, ao_off, sh_off, O, S()));
+        HIPCHK(launch_lighting_children(C.L, mode, R, pidx, T, O, S()));
       }
       if (out) {
         C.st.secondary_rays += nao + nsh;
@@ -2275,8 +2286,8 @@ static int batch_stats_read(BatchObj *B, gxy_batch_stats *st) {
 
 GXY_EXPORT void *gxy_batch_create(int chunk) {
   INIT_OR(nullptr);
-  if (chunk < 64 || chunk > 4096 || chunk % 64) {
-    fail("chunk %d: must be a multiple of 64 in [64, 4096]", chunk);
+  if (chunk < 64 || chunk > 1024 || chunk % 64) {
+    fail("chunk %d: must be a multiple of 64 in [64, 1024]", chunk);
     return nullptr;
   }
   BatchObj *B = new BatchObj;
@@ -2451,7 +2462,7 @@ GXY_EXPORT int gxy_batch_set_config(void *b, int chunk, int stack, int wpe, int 
   INIT_OR(-1);
   BatchObj *B = lookup<BatchObj>(g_batches, b);
   if (!B) return fail("not a batch handle");
-  if (chunk < 64 || chunk > 4096 || chunk % 64) return fail("chunk %d", chunk);
+  if (chunk < 64 || chunk > 1024 || chunk % 64) return fail("chunk %d: a multiple of 64 in [64, 1024]", chunk);
   if (stack != 0 && stack != 16 && stack != 24 && stack != 32) return fail("stack %d: 0, 16, 24 or 32", stack);
   if (wpe != 0 && wpe != 4 && wpe != 5) return fail("wpe %d: 0, 4 or 5", wpe);
   if (wpe == 5 && stack != 0) return fail("wpe 5 is built for the private-memory stack only");

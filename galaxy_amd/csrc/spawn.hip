@@ -59,7 +59,8 @@ __global__ __launch_bounds__(1024) void k_scan_single(int *a, int nb, int *total
   if (t == 1023) *total = part[1023];
 }
 
-__global__ __launch_bounds__(GXY_BLOCK) void k_flag_rank(RayList_ispc R, int n, const int *block_base, int *rank) {
+__global__ __launch_bounds__(GXY_BLOCK) void k_flag_rank(RayList_ispc R, int n, const int *block_base, int *rank,
+                                                         int *pidx) {
   __shared__ int wsum[GXY_BLOCK / 64];
   int i = blockIdx.x * GXY_BLOCK + threadIdx.x;
   bool f = i < n && is_parent(R, i);
@@ -69,12 +70,15 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_flag_rank(RayList_ispc R, int n, 
   __syncthreads();
   int base = block_base[blockIdx.x];
   for (int k = 0; k < w; k++) base += wsum[k];
-  // exclusive count of flagged rays before i (defined for every ray)
-  if (i < n) rank[i] = base + __popcll(m & lanemask_lt());
+  // exclusive count of flagged rays before i (defined for every ray); the
+  // flagged rays' indices in rank order when pidx is given
+  const int rk = base + __popcll(m & lanemask_lt());
+  if (i < n && rank) rank[i] = rk;
+  if (f && pidx) pidx[rk] = i;
 }
 
 hipError_t launch_surface_scan(const RayList_ispc &R, int n, int *scratch, int *rank_out, int *total,
-                               hipStream_t s) {
+                               hipStream_t s, int *pidx) {
   int nb = (n + GXY_BLOCK - 1) / GXY_BLOCK;
   if (nb == 0) {
     (void)hipMemsetAsync(total, 0, sizeof(int), s);
@@ -82,7 +86,7 @@ hipError_t launch_surface_scan(const RayList_ispc &R, int n, int *scratch, int *
   }
   hipLaunchKernelGGL(k_flag_count, dim3(nb), dim3(GXY_BLOCK), 0, s, R, n, scratch);
   hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, s, scratch, nb, total);
-  hipLaunchKernelGGL(k_flag_rank, dim3(nb), dim3(GXY_BLOCK), 0, s, R, n, scratch, rank_out);
+  hipLaunchKernelGGL(k_flag_rank, dim3(nb), dim3(GXY_BLOCK), 0, s, R, n, scratch, rank_out, pidx);
   return hipGetLastError();
 }
 
@@ -112,6 +116,56 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_lighting(LightParams L, int mode,
   int i = blockIdx.x * GXY_BLOCK + threadIdx.x;
   if (i >= n) return;
   light_one(L, mode, R, i, (mode & LM_AO) ? ao_off[i] : 0, (mode & LM_SHADOW) ? sh_off[i] : -1, O);
+}
+
+// The same sequence with one work-item per child, for the offsets of
+// TraceRays.cpp:89-117 (AO child j of the p-th parent at nAO*p + j, then its
+// shadow child k at nAO*T + nLights*p + k): consecutive work-items write
+// consecutive children, so every field store of a wave is one coalesced
+// transaction (one work-item per parent writes each field with a stride of
+// nAO entries).  pidx: the parents' ray indices in rank order.  The children
+// read the parents' unmodified colours, so this runs before k_parents_light.
+__global__ __launch_bounds__(GXY_BLOCK) void k_children(LightParams L, int mode, RayList_ispc R, const int *pidx,
+                                                        int T, RayList_ispc O) {
+  const long long t = (long long)blockIdx.x * GXY_BLOCK + threadIdx.x;
+  const int nAO = (mode & LM_AO) ? L.nAO : 0, nL = (mode & LM_SHADOW) ? L.nLights : 0;
+  const long long nao = (long long)nAO * T;
+  if (t < nao) {
+    const int p = (int)(t / nAO), j = (int)(t - (long long)p * nAO);
+    const ParentRay P = parent_load(R, pidx[p]);
+    light_ao_child(L, P, j, O, (int)t);
+  } else if (t < nao + (long long)nL * T) {
+    const long long t2 = t - nao;
+    const int p = (int)(t2 / nL), k = (int)(t2 - (long long)p * nL);
+    const ParentRay P = parent_load(R, pidx[p]);
+    const float o = (mode & LM_DIFFUSE) ? light_diffused_alpha(L, P.o) : P.o;
+    light_shadow_child(L, P, o, k, O, (int)t);
+  }
+}
+
+// ambientLighting and diffuseLighting of the T parents (after k_children)
+__global__ __launch_bounds__(GXY_BLOCK) void k_parents_light(LightParams L, int mode, RayList_ispc R, const int *pidx,
+                                                             int T) {
+  const int p = blockIdx.x * GXY_BLOCK + threadIdx.x;
+  if (p >= T) return;
+  const int i = pidx[p];
+  ParentRay P = parent_load(R, i);
+  if (mode & LM_AMBIENT) light_ambient(L, P);
+  if (mode & LM_DIFFUSE) light_diffuse(L, P);
+  R.r[i] = P.r; R.g[i] = P.g; R.b[i] = P.b; R.o[i] = P.o;
+}
+
+hipError_t launch_lighting_children(const LightParams &L, int mode, const RayList_ispc &R, const int *pidx, int T,
+                                    const RayList_ispc &O, hipStream_t s) {
+  if (T <= 0) return hipSuccess;
+  const long long nc = (long long)T * (((mode & LM_AO) ? L.nAO : 0) + ((mode & LM_SHADOW) ? L.nLights : 0));
+  if (nc > 0)
+    hipLaunchKernelGGL(k_children, dim3((unsigned)((nc + GXY_BLOCK - 1) / GXY_BLOCK)), dim3(GXY_BLOCK), 0, s, L,
+                       mode, R, pidx, T, O);
+  if (mode & (LM_AMBIENT | LM_DIFFUSE))
+    hipLaunchKernelGGL(k_parents_light, dim3((T + GXY_BLOCK - 1) / GXY_BLOCK), dim3(GXY_BLOCK), 0, s, L, mode, R,
+                       pidx, T);
+  return hipGetLastError();
 }
 
 hipError_t launch_lighting(const LightParams &L, int mode, const RayList_ispc &R, int n, const int *ao_off,

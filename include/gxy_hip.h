@@ -379,7 +379,7 @@ typedef struct gxy_batch_stats {
 
 enum { GXY_BATCH_RENDER = 0, GXY_BATCH_SAMPLE = 1 };
 
-/* chunk: rays per work-queue entry (64..4096, multiple of 64). */
+/* chunk: rays per work-queue entry (64..1024, multiple of 64). */
 void *gxy_batch_create(int chunk);
 int   gxy_batch_destroy(void *batch);
 /* A resident dataset: vis holds either one VolumeVis and no model, or a
@@ -403,7 +403,7 @@ int   gxy_batch_add_list(void *batch, int ds, int mode, int n, const RayList_isp
  * gxy_rays_random(seed, width, lo, hi). */
 int   gxy_batch_set_list_seed(void *batch, int list, unsigned seed, const float lo[3], const float hi[3]);
 int   gxy_batch_remove_list(void *batch, int list);
-/* Launch configuration: chunk (rays per queue entry), stack (BVH traversal
+/* Launch configuration: chunk (rays per queue entry, 64..1024), stack (BVH traversal
  * stack: 0 = 64 entries in private memory, 16/24/32 = entries in LDS; must
  * exceed the deepest tree by 2), wpe (waves per SIMD requested: 0, 4 or 5),
  * blocks_per_cu (persistent workgroups per compute unit, 1..16). */
