@@ -178,7 +178,7 @@ def bvh_depth(L, built):
     return dp.value
 
 
-def bench_batched(L, hip, steps, tune=True):
+def bench_batched(L, hip, steps, tune=True, fixed=None):
     """256 ray lists (1k-1M rays, log-uniform) over 64 resident datasets (blob
     meshes of 50k-5M triangles, 128^3-1024^3 noise volumes), shadow + 2 AO rays
     per surface hit: the batched form (one persistent launch per pass, passes
@@ -207,7 +207,12 @@ def bench_batched(L, hip, steps, tune=True):
                volume_voxels=int(sum(x[1] ** 3 for x in descs if x[0] == "volume")), max_bvh_depth=depth,
                build_s=round(build_s, 2))
     vols = [b.handles[0] for b, d in zip(built, descs) if d[0] == "volume"]
-    if tune:
+    if fixed is not None:
+        B.set_config(*fixed[:4])
+        for v in vols:
+            glib.check(L.gxy_volume_set_layout(v, fixed[4]))
+        B.refresh_datasets(built)
+    elif tune:
         best, table = gb.tune(B, max_depth=depth, graph_passes=2)
         res["tuned"] = dict(best=list(best), ms=[[list(c), round(m, 2)] for c, m in table])
         # voxel layout of every volume dataset: linear or 8^3 bricks (datasets re-registered)
@@ -272,7 +277,7 @@ def bench_batched(L, hip, steps, tune=True):
     return res
 
 
-def bench_served(L, hip, nreq, seed=5, layout=0):
+def bench_served(L, hip, nreq, seed=5, layout=0, config=None):
     """Poisson stream (200 requests/s) of render requests (256^2 to 3840x2160)
     and sampler requests (1M-50M rays) over resident 512^3-1024^3 noise volumes
     and 1M-20M triangle blob meshes, admitted under an HBM budget and served
@@ -287,6 +292,8 @@ def bench_served(L, hip, nreq, seed=5, layout=0):
                                   layout=layout)
     glib.check(L.gxy_synchronize())
     build_s = time.perf_counter() - t
+    if config is not None:
+        S.set_config(*config)
     rng = np.random.default_rng(seed)
     reqs, _ = serve.request_stream(nreq, 200.0, 0.5, rng, volume_ds=(0, 1, 2), any_ds=(0, 1, 2, 3, 4, 5))
     nrender = sum(1 for r in reqs if r.kind == 0)
@@ -346,6 +353,10 @@ def main():
                     help="kind,chunk,refill: use this trace launch configuration (implies --no-tune)")
     ap.add_argument("--volume-layout", type=int, default=None,
                     help="voxel layout of the volume: 0 linear, 8 or 4 bricks (implies --no-tune)")
+    ap.add_argument("--geometry-stack", type=int, default=None,
+                    help="BVH traversal stack of the mesh frames: 0 private, 16/24/32 LDS (no tuning)")
+    ap.add_argument("--batch-config", default=None,
+                    help="chunk,stack,wpe,blocks_per_cu,layout of the batched and served forms (no tuning)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device-to-device copy (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -453,7 +464,9 @@ def main():
                 t = time.perf_counter()
                 b2 = scenes.build_scene(hip, sc2)
                 build_s = time.perf_counter() - t
-                if not args.no_tune:
+                if args.geometry_stack is not None:
+                    glib.check(L.gxy_set_geometry_stack(args.geometry_stack))
+                elif not args.no_tune:
                     from galaxy_amd import autotune
                     fb2 = glib.DeviceArray(sc2.cameras[0].width * sc2.cameras[0].height * 16)
                     autotune.tune_geometry_stack(L, frame_fn(L, b2, sc2.cameras[0], fb2, sc2.epsilon))
@@ -476,13 +489,15 @@ def main():
     # ---------------------------------------------------------------- batched and served forms
     if not args.no_batched:
         try:
-            out["batched"] = bench_batched(L, hip, args.batch_steps)
+            fixed = [int(x) for x in args.batch_config.split(",")] if args.batch_config else None
+            out["batched"] = bench_batched(L, hip, args.batch_steps, tune=not args.no_tune, fixed=fixed)
         except Exception as e:
             out["batched"] = dict(error=str(e))
     if not args.no_served:
         try:
             lay = out.get("batched", {}).get("volume_layout", 0)
-            out["served"] = bench_served(L, hip, args.serve_requests, layout=lay)
+            cfg = out.get("batched", {}).get("config")
+            out["served"] = bench_served(L, hip, args.serve_requests, layout=lay, config=cfg)
         except Exception as e:
             out["served"] = dict(error=str(e))
 

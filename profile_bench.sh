@@ -10,7 +10,7 @@ OUT=${PROF_OUT:-$R/prof_raw}
 case $OUT in /*) ;; *) OUT=$R/$OUT ;; esac
 mkdir -p $OUT
 cd /tmp && export TMPDIR=/tmp
-BENCH="python3 $R/bench.py --steps 5 --warmup 1 --no-cpu --no-extra --trace-config 16,256,32 --copy-calibration"
+BENCH="python3 $R/bench.py --steps 5 --warmup 1 --no-cpu --no-extra --no-batched --no-served --trace-config 16,256,32 --volume-layout 4 --copy-calibration"
 echo "== kernel trace"
 timeout -s KILL 300 rocprofv3 --kernel-trace --stats --output-format csv -d $OUT/ktrace -o run -- $BENCH > $OUT/ktrace.log 2>&1 || exit 1
 echo "== FETCH_SIZE"

@@ -1,13 +1,16 @@
-"""Turns the rocprofv3 CSV output of profile_bench.sh into the short
-summaries kept in profiles/ (run by profile_bench.sh)."""
+"""Turns rocprofv3 CSV output into the short summaries kept in profiles/.
+
+  python3 profile_summary.py RAW DST                 (profile_bench.sh: volume-render frame)
+  python3 profile_summary.py --stats RAW OUT TITLE COMMAND   (profile_workloads.sh)
+"""
 import csv
 import glob
 import os
 import sys
 from collections import defaultdict
 
-RAW, DST = sys.argv[1], sys.argv[2]
-CMD = ("python3 bench.py --steps 5 --warmup 1 --no-cpu --no-extra --trace-config 16,256,32 --copy-calibration "
+RAW = DST = None
+CMD = ("python3 bench.py --steps 5 --warmup 1 --no-cpu --no-extra --no-batched --no-served --trace-config 16,256,32 --volume-layout 4 --copy-calibration "
        "(volume render 1920x1080, 512^3 noise, iso 0.5 + shadows)")
 
 
@@ -33,16 +36,16 @@ def main():
     cf = max(float(r["Counter_Value"]) for r in f if "copyBuffer" in r["Kernel_Name"])
     cw = max(float(r["Counter_Value"]) for r in w if "copyBuffer" in r["Kernel_Name"])
     sf, sw = 256 * 1024 / cf, 256 * 1024 / cw
-    kf = [float(r["Counter_Value"]) for r in f if "k_trace<" in r["Kernel_Name"]]
-    kw = [float(r["Counter_Value"]) for r in w if "k_trace<" in r["Kernel_Name"]]
+    kf = [float(r["Counter_Value"]) for r in f if "k_trace<1" in r["Kernel_Name"]]
+    kw = [float(r["Counter_Value"]) for r in w if "k_trace<1" in r["Kernel_Name"]]
     tb = [(a * sf + b * sw) * 1024 for a, b in zip(kf, kw)]
     sq = defaultdict(list)
     for r in rows("sq"):
-        if "k_trace<" in r["Kernel_Name"]:
+        if "k_trace<1" in r["Kernel_Name"]:
             sq[r["Counter_Name"]].append(float(r["Counter_Value"]))
     tcc = defaultdict(list)
     for r in rows("tcc"):
-        if "k_trace<" in r["Kernel_Name"]:
+        if "k_trace<1" in r["Kernel_Name"]:
             tcc[r["Counter_Name"]].append(float(r["Counter_Value"]))
     avg = lambda v: sum(v) / len(v) if v else float("nan")  # noqa: E731
     util = avg(sq["SQ_THREAD_CYCLES_VALU"]) / (64 * avg(sq["SQ_ACTIVE_INST_VALU"]))
@@ -67,5 +70,22 @@ def main():
     print("\n".join(lines))
 
 
+def stats_table(raw, out, title, cmd, top=24):
+    st = glob.glob(os.path.join(raw, "**", "*kernel_stats.csv"), recursive=True)
+    lines = ["# " + title, "", "Made by `bash profile_workloads.sh` (repository root), which runs",
+             "`rocprofv3 --kernel-trace --stats --output-format csv -- " + cmd + "`", "",
+             "| Kernel | Calls | Total ms | Average us | % |", "|---|---:|---:|---:|---:|"]
+    for r in sorted(csv.DictReader(open(st[0])), key=lambda r: -float(r["TotalDurationNs"]))[:top]:
+        lines.append("| `%s` | %s | %.3f | %.1f | %.1f |" % (r["Name"][:80], r["Calls"],
+                                                            float(r["TotalDurationNs"]) / 1e6,
+                                                            float(r["AverageNs"]) / 1e3, float(r["Percentage"])))
+    open(out, "w").write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1] == "--stats":
+        stats_table(*sys.argv[2:6])
+    else:
+        RAW, DST = sys.argv[1], sys.argv[2]
+        main()

@@ -1,0 +1,25 @@
+#!/bin/bash
+# Kernel-trace summaries of the mesh frames, the batched form and the served
+# form (run from the repository root on an MI355X; raw output goes to
+# $PROF_OUT, default ./prof_raw; summaries to profiles/).  Launch settings are
+# fixed to the values the tuners chose, so no tuning runs are profiled.
+set -o pipefail
+R=$(pwd)
+OUT=${PROF_OUT:-$R/prof_raw}
+case $OUT in /*) ;; *) OUT=$R/$OUT ;; esac
+mkdir -p $OUT
+cd /tmp && export TMPDIR=/tmp
+FIX="--trace-config 16,256,32 --volume-layout 4 --geometry-stack 32 --batch-config 256,0,0,4,8"
+MESH="python3 $R/bench.py --steps 5 --warmup 1 --no-cpu --no-batched --no-served $FIX"
+BATCH="python3 $R/bench.py --steps 2 --warmup 1 --no-cpu --no-extra --no-served --batch-steps 2 $FIX"
+SERVE="python3 $R/bench.py --steps 2 --warmup 1 --no-cpu --no-extra --no-batched --serve-requests 40 $FIX"
+echo "== mesh frames"
+timeout -s KILL 300 rocprofv3 --kernel-trace --stats --output-format csv -d $OUT/mesh -o run -- $MESH > $OUT/mesh.log 2>&1 || exit 1
+echo "== batched"
+timeout -s KILL 400 rocprofv3 --kernel-trace --stats --output-format csv -d $OUT/batched -o run -- $BATCH > $OUT/batched.log 2>&1 || exit 1
+echo "== served"
+timeout -s KILL 400 rocprofv3 --kernel-trace --stats --output-format csv -d $OUT/served -o run -- $SERVE > $OUT/served.log 2>&1 || exit 1
+cd $R
+python3 profile_summary.py --stats $OUT/mesh profiles/mesh_frames_kernels.md "Kernel trace, sphere-mesh and nine-icosphere frames (with the volume frame)" "${MESH/$R\//}"
+python3 profile_summary.py --stats $OUT/batched profiles/batched_c4_kernels.md "Kernel trace, batched form (256 lists, 64 datasets) and its per-list loop" "${BATCH/$R\//}"
+python3 profile_summary.py --stats $OUT/served profiles/served_c5_kernels.md "Kernel trace, served form (40 requests, continuous and one at a time)" "${SERVE/$R\//}"
