@@ -213,7 +213,7 @@ def bench_batched(L, hip, steps, tune=True, fixed=None):
             glib.check(L.gxy_volume_set_layout(v, fixed[4]))
         B.refresh_datasets(built)
     elif tune:
-        best, table = gb.tune(B, max_depth=depth, graph_passes=2)
+        best, table = gb.tune(B, max_depth=0, graph_passes=2)   # a stack too short for a model is skipped
         res["tuned"] = dict(best=list(best), ms=[[list(c), round(m, 2)] for c, m in table])
         # voxel layout of every volume dataset: linear or 8^3 bricks (datasets re-registered)
         lay_ms = {}
@@ -479,7 +479,7 @@ def main():
                                    scene_build_s=round(build_s, 3),
                                    trace_ms_per_frame=round(kt2["trace"][0] / steps, 3),
                                    kernel_ms_per_frame={k: round(v[0] / steps, 3) for k, v in kt2.items()},
-                                   geometry_stack=L.gxy_get_geometry_stack())
+                                   geometry_stack=L.gxy_get_geometry_stack(), bvh_width=L.gxy_get_bvh_width())
                 b2 = None
             except Exception as e:  # reported, not hidden
                 extra[name] = dict(error=str(e))

@@ -52,12 +52,15 @@ def tune_trace(L, render_once, candidates=None, reps=2):
 
 
 def tune_geometry_stack(L, render_once, stacks=None, reps=2):
-    """The same measurement over the geometry traversal stacks.  Returns
-    (best entries, [(entries, trace ms per frame), ...]) and leaves the best set."""
+    """The same measurement over the BVH width (binary or 4-wide) and the
+    geometry traversal stacks.  Returns (best (width, entries), [((width,
+    entries), trace ms per frame), ...]) and leaves the best set."""
     from . import lib as glib
     table = []
-    for st in stacks or GEOMETRY_STACKS:
-        glib.check(L.gxy_set_geometry_stack(st), "set_geometry_stack")
+    for wst in [(w, s) for w in (4, 2) for s in (stacks or GEOMETRY_STACKS)]:
+        glib.check(L.gxy_set_bvh_width(wst[0]), "set_bvh_width")
+        glib.check(L.gxy_set_geometry_stack(wst[1]), "set_geometry_stack")
+        st = wst
         render_once()
         glib.check(L.gxy_synchronize())
         L.gxy_timing_reset()
@@ -68,7 +71,8 @@ def tune_geometry_stack(L, render_once, stacks=None, reps=2):
         L.gxy_timing_enable(0)
         table.append((st, _trace_ms(L) / reps))
     best = min(table, key=lambda x: x[1])[0]
-    glib.check(L.gxy_set_geometry_stack(best), "set_geometry_stack")
+    glib.check(L.gxy_set_bvh_width(best[0]), "set_bvh_width")
+    glib.check(L.gxy_set_geometry_stack(best[1]), "set_geometry_stack")
     L.gxy_timing_reset()
     return best, table
 

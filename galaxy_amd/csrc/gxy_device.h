@@ -73,6 +73,16 @@ struct BVHNode {
   int pad_[2];
 };
 
+// 4-wide BVH node (128 bytes): the boxes of up to four children stored per
+// axis, then the child codes (inner node index >= 0, leaf code < 0 as in
+// BVHNode).  An empty slot has an empty box (lo = +inf, hi = -inf).
+struct BVH4Node {
+  float lox[4], loy[4], loz[4];
+  float hix[4], hiy[4], hiz[4];
+  int child[4];
+  int pad_[4];
+};
+
 struct DModel {
   // triangles, in BVH leaf order: 9 floats per triangle (v0 v1 v2)
   const BVHNode *tnodes;
@@ -94,6 +104,11 @@ struct DModel {
   int ngeom;
   const struct DGeomShade *shade;
   int depth;            // deepest leaf of either tree (traversal stack bound)
+  // the same trees collapsed to 4-wide nodes (null when not built)
+  const BVH4Node *tnodes4;
+  const BVH4Node *snodes4;
+  int depth4;           // deepest leaf of either 4-wide tree
+  int wide;             // traverse the 4-wide trees
 };
 
 // Per-geometry shading data (DataDrivenTriangleMesh_postIntersect,

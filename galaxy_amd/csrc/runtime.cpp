@@ -87,6 +87,7 @@ struct Runtime {
   // volume-only trace kernel: 0 one ray per work-item, 1 persistent lanes
   int trace_kind = 0, trace_chunk = 256, trace_refill = 32, trace_wpe = 0;
   int geom_stack = 0;           // BVH traversal stack of geometry-only traces (0 private, or LDS entries)
+  int bvh_width = 4;            // traverse the 4-wide trees (4) or the binary ones (2)
   float *uv = nullptr;          // AO direction tables U[256], V[256] (device)
 };
 static Runtime G;
@@ -297,7 +298,7 @@ struct ModelObj {
   std::vector<void *> owned;   // device buffers built at commit
   DGeomShade *shade = nullptr;
   int *hit_geom = nullptr, *hit_prim = nullptr;
-  int tnodes = 0, snodes = 0, tdepth = 0, sdepth = 0;
+  int tnodes = 0, snodes = 0, tdepth = 0, sdepth = 0, tdepth4 = 0, sdepth4 = 0;
   void free_built() {
     for (void *p : owned) dfree(p);
     owned.clear();
@@ -409,6 +410,7 @@ static int make_trace_params(const Visualization_ispc *vis, float global_eps, Tr
     if (!M->committed) return fail("model used before gxy_model_commit");
     P.has_model = (M->d.ntris > 0 || M->d.nsph > 0) ? 1 : 0;
     P.model = M->d;
+    P.model.wide = (G.bvh_width == 4 && (M->d.ntris == 0 || M->d.tnodes4) && (M->d.nsph == 0 || M->d.snodes4)) ? 1 : 0;
     P.hit_geom = M->hit_geom;
     P.hit_prim = M->hit_prim;
   }
@@ -587,6 +589,12 @@ GXY_EXPORT int gxy_set_geometry_stack(int entries) {
   return 0;
 }
 GXY_EXPORT int gxy_get_geometry_stack(void) { return G.geom_stack; }
+GXY_EXPORT int gxy_set_bvh_width(int width) {
+  if (width != 2 && width != 4) return fail("BVH width %d: 2 or 4", width);
+  G.bvh_width = width;
+  return 0;
+}
+GXY_EXPORT int gxy_get_bvh_width(void) { return G.bvh_width; }
 GXY_EXPORT int gxy_get_trace_config(int *kind, int *chunk, int *refill) {
   *kind = G.trace_kind == 0 && G.trace_wpe ? 10 + G.trace_wpe : G.trace_kind;
   *chunk = G.trace_chunk;
@@ -1088,6 +1096,103 @@ __global__ void k_gather_sph(const int *order, const int *fgeom, const int *fpri
 // Structural check of a built tree: every child index in range, every inner
 // node and every leaf reached exactly once from the root.  Run for trees of up
 // to GXY_BVH_VALIDATE_MAX nodes (default 4M) before the tree is used.
+// Collapses a binary BVH (device) into 4-wide nodes (device): each 4-wide
+// node takes a binary inner node's two children and repeatedly opens the
+// inner child with the largest box surface until it holds four children (or
+// only leaves remain); leaf codes are kept, inner children become 4-wide
+// nodes, depth first.  Returns the node buffer (allocated with dalloc) and the
+// 4-wide depth.
+static int collapse_bvh4(const BVHNode *dnodes, int nnodes, BVH4Node **out, int *depth4) {
+  std::vector<BVHNode> h(nnodes);
+  HIPCHK(hipMemcpyAsync(h.data(), dnodes, sizeof(BVHNode) * nnodes, hipMemcpyDeviceToHost, S()));
+  HIPCHK(hipStreamSynchronize(S()));
+  struct Ent {
+    int code;
+    float lo[3], hi[3];
+  };
+  auto area = [](const Ent &e) {
+    float d0 = e.hi[0] - e.lo[0], d1 = e.hi[1] - e.lo[1], d2 = e.hi[2] - e.lo[2];
+    return d0 * d1 + d1 * d2 + d2 * d0;
+  };
+  std::vector<BVH4Node> n4;
+  n4.reserve(nnodes / 2 + 4);
+  struct Job {
+    int bin;    // binary inner node
+    int slot;   // 4-wide node to fill
+    int depth;
+  };
+  std::vector<Job> jobs;
+  n4.push_back(BVH4Node());
+  jobs.push_back({0, 0, 1});
+  int maxd = 1;
+  while (!jobs.empty()) {
+    Job j = jobs.back();
+    jobs.pop_back();
+    maxd = std::max(maxd, j.depth);
+    const BVHNode &b = h[j.bin];
+    std::vector<Ent> es(2);
+    es[0].code = b.c0;
+    es[1].code = b.c1;
+    for (int k = 0; k < 3; k++) {
+      es[0].lo[k] = b.lo0[k];
+      es[0].hi[k] = b.hi0[k];
+      es[1].lo[k] = b.lo1[k];
+      es[1].hi[k] = b.hi1[k];
+    }
+    while (es.size() < 4) {
+      int best = -1;
+      for (int k = 0; k < (int)es.size(); k++)
+        if (es[k].code >= 0 && (best < 0 || area(es[k]) > area(es[best]))) best = k;
+      if (best < 0) break;
+      const BVHNode &c = h[es[best].code];
+      Ent a, d;
+      a.code = c.c0;
+      d.code = c.c1;
+      for (int k = 0; k < 3; k++) {
+        a.lo[k] = c.lo0[k];
+        a.hi[k] = c.hi0[k];
+        d.lo[k] = c.lo1[k];
+        d.hi[k] = c.hi1[k];
+      }
+      es[best] = a;
+      es.push_back(d);
+    }
+    BVH4Node nd;
+    memset(&nd, 0, sizeof nd);
+    for (int k = 0; k < 4; k++) {
+      float *lo[3] = {nd.lox, nd.loy, nd.loz}, *hi[3] = {nd.hix, nd.hiy, nd.hiz};
+      if (k < (int)es.size()) {
+        for (int a = 0; a < 3; a++) {
+          lo[a][k] = es[k].lo[a];
+          hi[a][k] = es[k].hi[a];
+        }
+        if (es[k].code >= 0) {
+          int idx = (int)n4.size();
+          n4.push_back(BVH4Node());
+          jobs.push_back({es[k].code, idx, j.depth + 1});
+          nd.child[k] = idx;
+        } else {
+          nd.child[k] = es[k].code;
+        }
+      } else {
+        for (int a = 0; a < 3; a++) {
+          lo[a][k] = INFINITY;
+          hi[a][k] = -INFINITY;
+        }
+        nd.child[k] = 0x7fffffff;
+      }
+    }
+    n4[j.slot] = nd;
+  }
+  BVH4Node *d4 = (BVH4Node *)dalloc(sizeof(BVH4Node) * n4.size());
+  if (!d4) return -1;
+  HIPCHK(hipMemcpyAsync(d4, n4.data(), sizeof(BVH4Node) * n4.size(), hipMemcpyHostToDevice, S()));
+  HIPCHK(hipStreamSynchronize(S()));
+  *out = d4;
+  *depth4 = maxd;
+  return 0;
+}
+
 static int validate_bvh(const BVHNode *dnodes, int nnodes, int nprim, const char *what) {
   size_t lim = getenv("GXY_BVH_VALIDATE_MAX") ? (size_t)atol(getenv("GXY_BVH_VALIDATE_MAX")) : (size_t)(4u << 20);
   if ((size_t)nnodes > lim) return 0;
@@ -1192,6 +1297,12 @@ GXY_EXPORT int gxy_model_commit(void *model) {
     M->tdepth = B.depth;
     if (B.depth > 60) return fail("triangle BVH depth %d exceeds the traversal stack", B.depth);
     if (validate_bvh(B.nodes, B.nnodes, (int)ntri, "triangle") != 0) return -1;
+    BVH4Node *n4 = nullptr;
+    int d4 = 0;
+    if (collapse_bvh4(B.nodes, B.nnodes, &n4, &d4) != 0) return -1;
+    M->owned.push_back(n4);
+    M->d.tnodes4 = n4;
+    M->tdepth4 = d4;
   }
   if (nsph > 0) {
     std::vector<int> fg(nsph), fp(nsph);
@@ -1233,8 +1344,15 @@ GXY_EXPORT int gxy_model_commit(void *model) {
     M->sdepth = B.depth;
     if (B.depth > 60) return fail("sphere BVH depth %d exceeds the traversal stack", B.depth);
     if (validate_bvh(B.nodes, B.nnodes, (int)nsph, "sphere") != 0) return -1;
+    BVH4Node *n4 = nullptr;
+    int d4 = 0;
+    if (collapse_bvh4(B.nodes, B.nnodes, &n4, &d4) != 0) return -1;
+    M->owned.push_back(n4);
+    M->d.snodes4 = n4;
+    M->sdepth4 = d4;
   }
   M->d.depth = std::max(M->tdepth, M->sdepth);
+  M->d.depth4 = std::max(M->tdepth4, M->sdepth4);
   M->committed = true;
   return 0;
 }
@@ -2354,7 +2472,7 @@ static int batch_dataset_params(BatchObj *B, const Visualization_ispc *vis, floa
   depth = 0;
   if (geometry) {
     ModelObj *M = lookup<ModelObj>(G.models, vis->model);
-    depth = M ? std::max(M->tdepth, M->sdepth) : 64;
+    depth = M ? (P.model.wide ? M->d.depth4 : std::max(M->tdepth, M->sdepth)) : 64;
   }
   return 0;
 }

@@ -384,7 +384,7 @@ static hipError_t launch_trace_nv(const TraceParams &P, const RayList_ispc &R, i
   bool small = true;
   for (int v = 0; v < NV; v++) small = small && P.vv[v].vol.small;
   // an LDS stack only when it holds the deepest path of the model
-  if (NV == 0 && P.has_model && stack > 0 && P.model.depth + 2 > stack) stack = 0;
+  if (NV == 0 && P.has_model && stack > 0 && (P.model.wide ? P.model.depth4 : P.model.depth) + 2 > stack) stack = 0;
   if (NV == 0 && P.has_model && stack == 16) hipLaunchKernelGGL((k_trace<0, true, 0, -1, 16>), g, b, 0, s, P, R, n);
   else if (NV == 0 && P.has_model && stack == 24) hipLaunchKernelGGL((k_trace<0, true, 0, -1, 24>), g, b, 0, s, P, R, n);
   else if (NV == 0 && P.has_model && stack == 32) hipLaunchKernelGGL((k_trace<0, true, 0, -1, 32>), g, b, 0, s, P, R, n);

@@ -185,6 +185,59 @@ __device__ __forceinline__ void bvh_walk(const BVHNode *nodes, f3 o, f3 inv, flo
   }
 }
 
+// Nearest-first walk of a 4-wide BVH.  A node's children are box-tested
+// against [t0, current hit t]; the walk descends into the nearest hit child
+// and pushes one entry (node << 4 | mask of the other hit children), which is
+// re-tested against the then-current hit t when popped, so the stack never
+// holds more than one entry per level (depth4 entries).
+template <class Stack, class LeafFn>
+__device__ __forceinline__ void bvh4_walk(const BVH4Node *nodes, f3 o, f3 inv, float t0, const GeoHit &h,
+                                          Stack &stack, LeafFn &&leaf) {
+  int sp = 0;
+  int cur = 0, mask = 0xF;
+  const BVH4Node *gn = gptr(nodes);
+  for (;;) {
+    const float4 *np = (const float4 *)(gn + cur);
+    const float4 lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5];
+    const int4 ch = ((const int4 *)np)[6];
+    const float lox[4] = {lx.x, lx.y, lx.z, lx.w}, loy[4] = {ly.x, ly.y, ly.z, ly.w}, loz[4] = {lz.x, lz.y, lz.z, lz.w};
+    const float hix[4] = {hx.x, hx.y, hx.z, hx.w}, hiy[4] = {hy.x, hy.y, hy.z, hy.w}, hiz[4] = {hz.x, hz.y, hz.z, hz.w};
+    const int cc[4] = {ch.x, ch.y, ch.z, ch.w};
+    int kn = -1;
+    float tn = 0.f;
+    int hitm = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      if ((mask >> k) & 1) {
+        const float lo[3] = {lox[k], loy[k], loz[k]}, hi[3] = {hix[k], hiy[k], hiz[k]};
+        float te;
+        if (child_hit(lo, hi, o, inv, t0, h.t, te)) {
+          hitm |= 1 << k;
+          if (kn < 0 || te < tn) { kn = k; tn = te; }
+        }
+      }
+    }
+    if (kn >= 0) {
+      const int rest = hitm & ~(1 << kn);
+      if (rest && sp < Stack::DEPTH) stack.put(sp++, (cur << 4) | rest);
+      int c = cc[0];
+#pragma unroll
+      for (int k = 1; k < 4; k++) c = (k == kn) ? cc[k] : c;
+      if (c >= 0) {
+        cur = c;
+        mask = 0xF;
+        continue;
+      }
+      const int code = ~c;
+      leaf(code >> 4, (code & 15) + 1);
+    }
+    if (sp == 0) return;
+    const int e = stack.get(--sp);
+    cur = e >> 4;
+    mask = e & 15;
+  }
+}
+
 template <class Stack>
 __device__ __forceinline__ void model_trace(const DModel &M, f3 org, f3 dir, float t0, float tfar, GeoHit &h,
                                             Stack &stack) {
@@ -196,7 +249,7 @@ __device__ __forceinline__ void model_trace(const DModel &M, f3 org, f3 dir, flo
   f3 inv = mk3(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
   if (M.ntris > 0) {
     WRay wr = make_wray(org, dir);
-    bvh_walk(M.tnodes, org, inv, t0, h, stack, [&](int start, int count) {
+    auto tleaf = [&](int start, int count) {
       for (int k = start; k < start + count; k++) {
         const float *p = gptr(M.tverts) + 9 * (int64_t)k;
         f3 p0 = mk3(p[0], p[1], p[2]), p1 = mk3(p[3], p[4], p[5]), p2 = mk3(p[6], p[7], p[8]);
@@ -206,10 +259,12 @@ __device__ __forceinline__ void model_trace(const DModel &M, f3 org, f3 dir, flo
           if (h.geom < 0 || better(t, g, pr, h)) { h.t = t; h.geom = g; h.prim = pr; h.u = u; h.v = v; }
         }
       }
-    });
+    };
+    if (M.wide) bvh4_walk(M.tnodes4, org, inv, t0, h, stack, tleaf);
+    else bvh_walk(M.tnodes, org, inv, t0, h, stack, tleaf);
   }
   if (M.nsph > 0) {
-    bvh_walk(M.snodes, org, inv, t0, h, stack, [&](int start, int count) {
+    auto sleaf = [&](int start, int count) {
       for (int k = start; k < start + count; k++) {
         const float *sc = gptr(M.scenter);
         f3 c = mk3(sc[3 * k], sc[3 * k + 1], sc[3 * k + 2]);
@@ -220,7 +275,9 @@ __device__ __forceinline__ void model_trace(const DModel &M, f3 org, f3 dir, flo
           if (h.geom < 0 || better(t, g, pr, h)) { h.t = t; h.geom = g; h.prim = pr; h.u = h.v = 0.f; }
         }
       }
-    });
+    };
+    if (M.wide) bvh4_walk(M.snodes4, org, inv, t0, h, stack, sleaf);
+    else bvh_walk(M.snodes, org, inv, t0, h, stack, sleaf);
   }
 }
 

@@ -85,6 +85,8 @@ def _declare(L):
                                           vp]),
         "gxy_sample_raylist": (C.c_int, [vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, vp]),
         "gxy_set_geometry_stack": (C.c_int, [C.c_int]),
+        "gxy_set_bvh_width": (C.c_int, [C.c_int]),
+        "gxy_get_bvh_width": (C.c_int, []),
         "gxy_volume_set_layout": (C.c_int, [vp, C.c_int]),
         "gxy_volume_get_layout": (C.c_int, [vp]),
         "gxy_get_geometry_stack": (C.c_int, []),

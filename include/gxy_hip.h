@@ -187,6 +187,11 @@ int         gxy_set_trace_config(int kind, int chunk, int refill);
  * stack).  Results are identical for every setting. */
 int         gxy_set_geometry_stack(int entries);
 int         gxy_get_geometry_stack(void);
+/* BVH traversed by geometry traces (no reference counterpart): 4 = the
+ * 4-wide trees every model commit also builds (128-byte nodes, about half the
+ * depth), 2 = the binary LBVH.  Hits are identical either way. */
+int         gxy_set_bvh_width(int width);
+int         gxy_get_bvh_width(void);
 int         gxy_get_trace_config(int *kind, int *chunk, int *refill);
 size_t      gxy_device_bytes_total(void);
 

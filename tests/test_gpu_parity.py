@@ -348,8 +348,9 @@ def test_every_trace_launch_configuration_gives_identical_results(radial_vols_sm
 
 
 def test_every_geometry_stack_gives_identical_hits(hip):
-    """The LDS and private traversal stacks visit the same nodes in the same
-    order: hits, t, colours and side-band geometry/primitive IDs are identical."""
+    """Binary and 4-wide trees, LDS and private traversal stacks: hits, t,
+    colours and side-band geometry/primitive IDs are identical (closest hit
+    with the deterministic tie rule does not depend on the visiting order)."""
     from galaxy_amd import autotune, lib as glib, workloads
     L_ = glib.lib()
     sc = workloads.blob_scene(60000, 4)
@@ -362,8 +363,10 @@ def test_every_geometry_stack_gives_identical_hits(hip):
     glib.check(L_.gxy_model_set_hit_output(Bh.vis.model, geom.ptr, prim.ptr))
     ref = None
     saved = L_.gxy_get_geometry_stack()
+    saved_w = L_.gxy_get_bvh_width()
     try:
-        for st in autotune.GEOMETRY_STACKS:
+        for width, st in [(w, s) for w in (2, 4) for s in autotune.GEOMETRY_STACKS]:
+            glib.check(L_.gxy_set_bvh_width(width))
             glib.check(L_.gxy_set_geometry_stack(st))
             d = glib.DeviceRays.from_host(hr)
             hip.trace(Bh.vis, d)
@@ -374,10 +377,11 @@ def test_every_geometry_stack_gives_identical_hits(hip):
                 assert (ids[0] >= 0).sum() > 10000
                 continue
             for f in abi.FIELDS:
-                assert np.array_equal(g[f], ref[0][f]), (st, f)
-            assert np.array_equal(ids[0], ref[1][0]) and np.array_equal(ids[1], ref[1][1]), st
+                assert np.array_equal(g[f], ref[0][f]), (width, st, f)
+            assert np.array_equal(ids[0], ref[1][0]) and np.array_equal(ids[1], ref[1][1]), (width, st)
     finally:
         L_.gxy_set_geometry_stack(saved)
+        L_.gxy_set_bvh_width(saved_w)
         L_.gxy_model_set_hit_output(Bh.vis.model, None, None)
 
 
