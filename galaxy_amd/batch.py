@@ -79,13 +79,14 @@ class Batch:
         glib.check(self.L.gxy_batch_remove_list(self.h, l), "batch_remove_list")
         self.lists.pop(l, None)
 
-    def set_config(self, chunk, stack, wpe, blocks_per_cu):
+    def set_config(self, chunk, stack, wpe, blocks_per_cu, split=0):
         glib.check(self.L.gxy_batch_set_config(self.h, chunk, stack, wpe, blocks_per_cu), "batch_set_config")
+        glib.check(self.L.gxy_batch_set_split(self.h, split), "batch_set_split")
 
     def config(self):
         v = [C.c_int() for _ in range(4)]
         glib.check(self.L.gxy_batch_get_config(self.h, *[C.byref(x) for x in v]), "batch_get_config")
-        return tuple(x.value for x in v)
+        return tuple(x.value for x in v) + (self.L.gxy_batch_get_split(self.h),)
 
     # ------------------------------------------------------------ running
     def run(self, graph_passes=0):
@@ -127,9 +128,10 @@ def build_datasets(hip, descs):
     return [scenes.build_scene(hip, sc) for sc in descs]
 
 
-# candidate launch configurations: (chunk, stack, wpe, blocks per CU)
-DEFAULT_CANDIDATES = [(256, 0, 0, 4), (256, 0, 4, 4), (128, 0, 4, 8), (512, 0, 4, 4), (256, 0, 5, 5),
-                      (256, 16, 4, 4), (256, 24, 0, 3), (256, 32, 0, 3)]
+# candidate launch configurations: (chunk, stack, wpe, blocks per CU, split)
+DEFAULT_CANDIDATES = [(256, 0, 0, 4, 0), (256, 0, 4, 4, 0), (128, 0, 4, 8, 0), (512, 0, 4, 4, 0),
+                      (256, 0, 5, 5, 0), (256, 16, 4, 4, 0), (256, 24, 0, 3, 0), (256, 32, 0, 3, 0),
+                      (256, 0, 0, 4, 1), (256, 0, 5, 4, 1), (256, 24, 5, 4, 1), (128, 0, 5, 8, 1)]
 
 
 def tune(batch, candidates=None, max_depth=64, graph_passes=2, reps=1):

@@ -24,6 +24,9 @@
 namespace gxy {
 
 #define BWAVES (GXY_BLOCK / 64)
+// isovalue crossings a sampler ray follows within one pass before it is
+// re-queued as a KEEP_HERE ray
+#define BATCH_MAX_SEGMENTS 4096
 
 __device__ __forceinline__ unsigned long long bl_lanemask_lt() {
   const int lane = threadIdx.x & 63;
@@ -75,7 +78,7 @@ __device__ __forceinline__ RayList_ispc chunk_rays(const BatchArgs &A, const Bat
 
 // per-workgroup totals of the BS_* statistics, one slot per workgroup
 __device__ __forceinline__ void block_partials(unsigned long long *v, int k0, int k1, unsigned long long *part,
-                                               int nblocks) {
+                                               int nblocks, int q = 0) {
   __shared__ unsigned long long ws[BWAVES][BS_COUNT];
   const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0)
@@ -85,7 +88,7 @@ __device__ __forceinline__ void block_partials(unsigned long long *v, int k0, in
     const int k = k0 + threadIdx.x;
     unsigned long long t = 0;
     for (int q = 0; q < BWAVES; q++) t += ws[q][k];
-    part[(size_t)k * nblocks + blockIdx.x] = t;
+    part[(size_t)k * 4 * nblocks + (size_t)q * nblocks + blockIdx.x] = t;
   }
 }
 
@@ -98,7 +101,38 @@ __global__ __launch_bounds__(1024) void k_bbegin(BatchArgs A) {
     *A.nout = 0;
     *A.nrays_out = 0;
   }
+  if (threadIdx.x < 3) {
+    A.kcount[threadIdx.x] = 0;
+    A.khead[threadIdx.x] = 0;
+  }
   for (int l = threadIdx.x; l < A.nlists; l += 1024) A.lpend[l] = 0;
+  for (long long k = threadIdx.x; k < (long long)BS_COUNT * 4 * A.nblocks; k += 1024) A.part[k] = 0;
+}
+
+// kind of a chunk: 0 volume render, 1 geometry render, 2 sampler
+__device__ __forceinline__ int chunk_kind(const BatchArgs &A, const BatchChunk &ch) {
+  const BatchList &Ls = A.lists[ch.list];
+  if (Ls.mode == BM_SAMPLE) return 2;
+  return A.ds[Ls.ds].has_model ? 1 : 0;
+}
+
+// per-kind lists of the input chunks (order within a list is irrelevant)
+__global__ __launch_bounds__(GXY_BLOCK) void k_bsplit(BatchArgs A) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nin = *A.nin;
+  for (int base = (blockIdx.x * BWAVES + w) * 64; base < nin; base += gridDim.x * BWAVES * 64) {
+    const int c = base + lane;
+    const int kind = c < nin ? chunk_kind(A, A.tin[c]) : -1;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      const unsigned long long m = __ballot(kind == q);
+      if (!m) continue;
+      int b0 = 0;
+      if (lane == 0) b0 = atomicAdd(A.kcount + q, __popcll(m));
+      b0 = first_lane_i(b0);
+      if (kind == q) A.kidx[(size_t)q * A.kcap + b0 + __popcll(m & bl_lanemask_lt())] = c;
+    }
+  }
 }
 
 // appends n host-built chunks (already on the device) to the input table
@@ -148,22 +182,28 @@ void k_btrace(BatchArgs A) {
     const int ds = Ls.ds, mode = Ls.mode;
     const TraceParams &P = kconst(A.ds + ds);
     const RayList_ispc R = chunk_rays(A, ch);
-    acc[BS_RAYS] += ch.count;
     // voxel addressing of the dataset's volume: bricked (8), small linear (1), large linear (0)
     const int lay = P.vv[0].vol.brick == 8 ? 8 : (P.vv[0].vol.small ? 1 : 0);
     if (mode == BM_SAMPLE) {
-      if (lay == 8)
-        for (int k = lane; k < ch.count; k += 64) acc[BS_SAMPLES] += sample_one<1, 8>(P, R, ch.start + k);
-      else if (lay == 1)
-        for (int k = lane; k < ch.count; k += 64) acc[BS_SAMPLES] += sample_one<1, 1>(P, R, ch.start + k);
-      else
-        for (int k = lane; k < ch.count; k += 64) acc[BS_SAMPLES] += sample_one<1, 0>(P, R, ch.start + k);
+      // every crossing of the ray in this launch (segments counted as ray passes)
+      float *xyz = gptr(Ls.xyz);
+      int *pc = Ls.pcount;
+      const int cap = Ls.pcap;
+      unsigned segs = 0;
+      for (int k = lane; k < ch.count; k += 64) {
+        if (lay == 8) acc[BS_SAMPLES] += sample_all<1, 8>(P, R, ch.start + k, xyz, pc, cap, BATCH_MAX_SEGMENTS, segs);
+        else if (lay == 1) acc[BS_SAMPLES] += sample_all<1, 1>(P, R, ch.start + k, xyz, pc, cap, BATCH_MAX_SEGMENTS, segs);
+        else acc[BS_SAMPLES] += sample_all<1, 0>(P, R, ch.start + k, xyz, pc, cap, BATCH_MAX_SEGMENTS, segs);
+        acc[BS_RAYS] += segs;
+      }
     } else if (P.has_model) {
+      acc[BS_RAYS] += lane < ch.count ? (ch.count - 1 - lane) / 64 + 1 : 0;
       for (int k = lane; k < ch.count; k += 64) {
         if (STK > 0) acc[BS_SAMPLES] += trace_one<0, true>(P, R, ch.start + k, s_col[w], s_op[w], lstack);
         else acc[BS_SAMPLES] += trace_one<0, true>(P, R, ch.start + k, s_col[w], s_op[w], pstack);
       }
     } else {
+      acc[BS_RAYS] += lane < ch.count ? (ch.count - 1 - lane) / 64 + 1 : 0;
       if (ds != cur_tf) {
         const DTF &T = P.vv[0].tf;
         for (int k = lane; k < 3 * T.nc; k += 64) s_col[w][0][k] = T.col[k];
@@ -185,9 +225,89 @@ void k_btrace(BatchArgs A) {
                                                                             pstack);
     }
   }
-  // lane sums of the sample counts (ray counts are per wave already)
+  // lane sums of the sample and ray-pass counts
   acc[BS_SAMPLES] = wave_sum_u64(acc[BS_SAMPLES]);
+  acc[BS_RAYS] = wave_sum_u64(acc[BS_RAYS]);
   block_partials(acc, BS_RAYS, BS_SAMPLES + 1, A.part, A.nblocks);
+}
+
+// ------------------------------------------------------------------ trace, one kernel per kind
+// The same per-chunk work as k_btrace for the chunks of one kind (the list
+// k_bsplit built), so that each kind gets its own register allocation and
+// occupancy; the three kernels of a pass run concurrently on three streams.
+template <int KIND, int STK, int WPE>
+__global__ __launch_bounds__(GXY_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, 8)))
+void k_btrace_kind(BatchArgs A) {
+  constexpr int TFW = KIND == 0 ? BWAVES : 1;
+  __shared__ float s_col[TFW][1][3 * GXY_TF_MAX];
+  __shared__ float s_op[TFW][1][GXY_TF_MAX];
+  __shared__ int s_stack[(KIND == 1 && STK > 0) ? STK : 1][GXY_BLOCK];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nk = A.kcount[KIND];
+  const int *idx = A.kidx + (size_t)KIND * A.kcap;
+  unsigned long long acc[BS_COUNT];
+#pragma unroll
+  for (int k = 0; k < BS_COUNT; k++) acc[k] = 0;
+  int cur_tf = -1;
+  PrivStack<GXY_BVH_STACK> pstack;
+  LdsStack<((KIND == 1 && STK > 0) ? STK : 1)> lstack;
+  lstack.base = &s_stack[0][threadIdx.x];
+  lstack.stride = GXY_BLOCK;
+  for (;;) {
+    int q = 0;
+    if (lane == 0) q = atomicAdd(A.khead + KIND, 1);
+    q = first_lane_i(q);
+    if (q >= nk) break;
+    const int c = first_lane_i(idx[q]);
+    const BatchChunk ch = kconst(A.tin + c);
+    const BatchList &Ls = kconst(A.lists + ch.list);
+    const int ds = Ls.ds;
+    const TraceParams &P = kconst(A.ds + ds);
+    const RayList_ispc R = chunk_rays(A, ch);
+    const int lay = P.vv[0].vol.brick == 8 ? 8 : (P.vv[0].vol.small ? 1 : 0);
+    if (KIND == 2) {
+      float *xyz = gptr(Ls.xyz);
+      int *pc = Ls.pcount;
+      const int cap = Ls.pcap;
+      unsigned segs = 0;
+      for (int k = lane; k < ch.count; k += 64) {
+        if (lay == 8) acc[BS_SAMPLES] += sample_all<1, 8>(P, R, ch.start + k, xyz, pc, cap, BATCH_MAX_SEGMENTS, segs);
+        else if (lay == 1) acc[BS_SAMPLES] += sample_all<1, 1>(P, R, ch.start + k, xyz, pc, cap, BATCH_MAX_SEGMENTS, segs);
+        else acc[BS_SAMPLES] += sample_all<1, 0>(P, R, ch.start + k, xyz, pc, cap, BATCH_MAX_SEGMENTS, segs);
+        acc[BS_RAYS] += segs;
+      }
+    } else if (KIND == 1) {
+      acc[BS_RAYS] += lane < ch.count ? (ch.count - 1 - lane) / 64 + 1 : 0;
+      for (int k = lane; k < ch.count; k += 64) {
+        if (STK > 0) acc[BS_SAMPLES] += trace_one<0, true>(P, R, ch.start + k, s_col[0], s_op[0], lstack);
+        else acc[BS_SAMPLES] += trace_one<0, true>(P, R, ch.start + k, s_col[0], s_op[0], pstack);
+      }
+    } else {
+      acc[BS_RAYS] += lane < ch.count ? (ch.count - 1 - lane) / 64 + 1 : 0;
+      if (ds != cur_tf) {
+        const DTF &T = P.vv[0].tf;
+        for (int k = lane; k < 3 * T.nc; k += 64) s_col[w % TFW][0][k] = T.col[k];
+        for (int k = lane; k < T.no; k += 64) s_op[w % TFW][0][k] = T.op[k];
+        wave_lds_sync();
+        cur_tf = ds;
+      }
+      if (lay == 8)
+        for (int k = lane; k < ch.count; k += 64)
+          acc[BS_SAMPLES] += trace_one<1, false, PrivStack<GXY_BVH_STACK>, 8>(P, R, ch.start + k, s_col[w % TFW],
+                                                                            s_op[w % TFW], pstack);
+      else if (lay == 1)
+        for (int k = lane; k < ch.count; k += 64)
+          acc[BS_SAMPLES] += trace_one<1, false, PrivStack<GXY_BVH_STACK>, 1>(P, R, ch.start + k, s_col[w % TFW],
+                                                                            s_op[w % TFW], pstack);
+      else
+        for (int k = lane; k < ch.count; k += 64)
+          acc[BS_SAMPLES] += trace_one<1, false, PrivStack<GXY_BVH_STACK>, 0>(P, R, ch.start + k, s_col[w % TFW],
+                                                                            s_op[w % TFW], pstack);
+    }
+  }
+  acc[BS_SAMPLES] = wave_sum_u64(acc[BS_SAMPLES]);
+  acc[BS_RAYS] = wave_sum_u64(acc[BS_RAYS]);
+  block_partials(acc, BS_RAYS, BS_SAMPLES + 1, A.part, A.nblocks, 1 + KIND);
 }
 
 // ------------------------------------------------------------------ post-pass: counts
@@ -484,7 +604,7 @@ __global__ __launch_bounds__(1024) void k_breduce(BatchArgs A, int k0, int k1) {
   __shared__ unsigned long long ws[16];
   for (int k = k0; k < k1; k++) {
     unsigned long long v = 0;
-    for (int b = threadIdx.x; b < A.nblocks; b += 1024) v += A.part[(size_t)k * A.nblocks + b];
+    for (int b = threadIdx.x; b < 4 * A.nblocks; b += 1024) v += A.part[(size_t)k * 4 * A.nblocks + b];
     v = wave_sum_u64(v);
     if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
     __syncthreads();
@@ -550,10 +670,39 @@ static void btrace_as(const BatchArgs &A, int blocks, hipStream_t s) {
   hipLaunchKernelGGL((k_btrace<STK, WPE>), dim3(blocks), dim3(GXY_BLOCK), 0, s, A);
 }
 
-hipError_t launch_batch_pass(const BatchArgs &A, const BatchLaunch &K, hipStream_t s) {
+template <int KIND, int STK, int WPE>
+static void bkind_as(const BatchArgs &A, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL((k_btrace_kind<KIND, STK, WPE>), dim3(blocks), dim3(GXY_BLOCK), 0, s, A);
+}
+
+static void launch_split_trace(const BatchArgs &A, const BatchLaunch &K, hipStream_t s, const hipStream_t *ks,
+                               const hipEvent_t *ev) {
+  const int tb = K.trace_blocks;
+  hipLaunchKernelGGL(k_bsplit, dim3(K.post_blocks), dim3(GXY_BLOCK), 0, s, A);
+  (void)hipEventRecord(ev[0], s);
+  for (int q = 0; q < 3; q++) (void)hipStreamWaitEvent(ks[q], ev[0], 0);
+  // volume render: occupancy request 5 or 6 waves per SIMD when asked (LDS TF slices only)
+  if (K.wpe == 5) bkind_as<0, 0, 5>(A, tb, ks[0]);
+  else if (K.wpe == 4) bkind_as<0, 0, 4>(A, tb, ks[0]);
+  else bkind_as<0, 0, 0>(A, tb, ks[0]);
+  if (K.stack == 16) bkind_as<1, 16, 0>(A, tb, ks[1]);
+  else if (K.stack == 24) bkind_as<1, 24, 0>(A, tb, ks[1]);
+  else if (K.stack == 32) bkind_as<1, 32, 0>(A, tb, ks[1]);
+  else bkind_as<1, 0, 0>(A, tb, ks[1]);
+  bkind_as<2, 0, 0>(A, tb, ks[2]);
+  for (int q = 0; q < 3; q++) {
+    (void)hipEventRecord(ev[1 + q], ks[q]);
+    (void)hipStreamWaitEvent(s, ev[1 + q], 0);
+  }
+}
+
+hipError_t launch_batch_pass(const BatchArgs &A, const BatchLaunch &K, hipStream_t s, const hipStream_t *ks,
+                             const hipEvent_t *ev) {
   hipLaunchKernelGGL(k_bbegin, dim3(1), dim3(1024), 0, s, A);
   const int tb = K.trace_blocks;
-  if (K.stack == 16) {
+  if (K.split && ks && ev) {
+    launch_split_trace(A, K, s, ks, ev);
+  } else if (K.stack == 16) {
     if (K.wpe == 4) btrace_as<16, 4>(A, tb, s); else btrace_as<16, 0>(A, tb, s);
   } else if (K.stack == 24) {
     if (K.wpe == 4) btrace_as<24, 4>(A, tb, s); else btrace_as<24, 0>(A, tb, s);

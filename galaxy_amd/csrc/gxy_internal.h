@@ -134,6 +134,10 @@ struct BatchArgs {
   int nblocks;
   unsigned long long *stats;  // BS_COUNT totals
   int *overflow;              // bit 0: pool or table full; bit 1: append overflow
+  int *kidx;                  // per kind (volume render, geometry, sampler): input chunk indices, kcap each
+  int *kcount;                // 3 counts
+  int *khead;                 // 3 work-queue heads
+  int kcap;
   LightParams L;
 };
 
@@ -142,10 +146,14 @@ struct BatchLaunch {
   int post_blocks;   // count / write grid
   int stack;         // traversal stack: 0 private (64 entries), or 16/24/32 entries in LDS
   int wpe;           // occupancy request (waves per SIMD), 0 none
+  int split;         // 0: one fused trace kernel; 1: one kernel per ray kind on three streams
 };
 
-// nblocks of A must equal max(trace_blocks, post_blocks)
-hipError_t launch_batch_pass(const BatchArgs &A, const BatchLaunch &K, hipStream_t s);
+// Partial sums: A.part holds BS_COUNT x (4 * nblocks) slots; nblocks equals
+// trace_blocks and post_blocks.  ks / ev: three streams and four events for the
+// split trace (may be null when split is 0).
+hipError_t launch_batch_pass(const BatchArgs &A, const BatchLaunch &K, hipStream_t s, const hipStream_t *ks,
+                             const hipEvent_t *ev);
 hipError_t launch_batch_reduce(const BatchArgs &A, hipStream_t s);
 hipError_t launch_batch_append(BatchChunk *tab, int *ntab, int cap, const BatchChunk *add, int n, int *overflow,
                                hipStream_t s);

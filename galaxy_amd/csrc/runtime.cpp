@@ -2082,8 +2082,12 @@ namespace gxy {
 
 struct BatchObj {
   // launch configuration
-  int chunk = 256, stack = 0, wpe = 0, bpc = 4;
+  int chunk = 256, stack = 0, wpe = 0, bpc = 4, split = 0;
   int cus = 0;
+  hipStream_t ks[3] = {nullptr, nullptr, nullptr};   // streams of the per-kind trace kernels
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  int *kidx = nullptr;                                // per-kind chunk lists, 3 x kcap
+  int kcap = 0;
   // datasets and lighting
   std::vector<TraceParams> ds;
   std::vector<int> ds_depth;
@@ -2130,8 +2134,12 @@ struct BatchObj {
     for (int k = 0; k < 2; k++)
       if (pool[k]) raylist_delete(pool[k]);
     pool[0] = pool[1] = nullptr;
-    void *bufs[] = {d_ds, d_lists, d_rls, tab[0], tab[1], d_init, d_fresh, pc, seg, lpend, part, stats, ints};
+    void *bufs[] = {d_ds, d_lists, d_rls, tab[0], tab[1], d_init, d_fresh, pc, seg, lpend, part, stats, ints, kidx};
     for (void *p : bufs) dfree(p);
+    for (int q = 0; q < 3; q++)
+      if (ks[q]) hipStreamDestroy(ks[q]);
+    for (int q = 0; q < 4; q++)
+      if (ev[q]) hipEventDestroy(ev[q]);
   }
 };
 
@@ -2239,6 +2247,10 @@ static BatchArgs batch_args(BatchObj *B, int pass) {
   A.part = B->part;
   A.nblocks = B->part_blocks;
   A.stats = B->stats;
+  A.kidx = B->kidx;
+  A.kcount = B->ints + 8;
+  A.khead = B->ints + 11;
+  A.kcap = B->kcap;
   A.L = B->L;
   return A;
 }
@@ -2249,6 +2261,7 @@ static BatchLaunch batch_launch(BatchObj *B) {
   K.post_blocks = B->part_blocks;   // equal to trace_blocks: every partial slot is written
   K.stack = B->stack;
   K.wpe = B->wpe;
+  K.split = B->split;
   return K;
 }
 
@@ -2272,6 +2285,10 @@ static int batch_reserve_pass(BatchObj *B, int pass, long long in_rays, long lon
     if (p) dfree(p);
     B->pc = (int *)q;
     B->pc_cap = (int)want;
+    dfree(B->kidx);
+    B->kidx = (int *)dalloc((size_t)want * 3 * sizeof(int));
+    if (!B->kidx) return -1;
+    B->kcap = (int)want;
     B->dirty_graph = true;
   }
   return 0;
@@ -2279,17 +2296,19 @@ static int batch_reserve_pass(BatchObj *B, int pass, long long in_rays, long lon
 
 static int batch_ensure_base(BatchObj *B) {
   if (!B->ints) {
-    B->ints = (int *)dalloc(8 * sizeof(int));
+    B->ints = (int *)dalloc(16 * sizeof(int));
     B->seg = (long long *)dalloc(512 * sizeof(long long));
     B->stats = (unsigned long long *)dalloc(BS_COUNT * sizeof(unsigned long long));
     if (!B->ints || !B->seg || !B->stats) return -1;
-    HIPCHK(hipMemsetAsync(B->ints, 0, 8 * sizeof(int), S()));
+    HIPCHK(hipMemsetAsync(B->ints, 0, 16 * sizeof(int), S()));
+    for (int q = 0; q < 3; q++) HIPCHK(hipStreamCreateWithFlags(&B->ks[q], hipStreamNonBlocking));
+    for (int q = 0; q < 4; q++) HIPCHK(hipEventCreateWithFlags(&B->ev[q], hipEventDisableTiming));
     HIPCHK(hipMemsetAsync(B->stats, 0, BS_COUNT * sizeof(unsigned long long), S()));
   }
   const int nb = batch_blocks(B);
   if (nb != B->part_blocks) {
     dfree(B->part);
-    B->part = (unsigned long long *)dalloc((size_t)nb * BS_COUNT * sizeof(unsigned long long));
+    B->part = (unsigned long long *)dalloc((size_t)nb * 4 * BS_COUNT * sizeof(unsigned long long));
     if (!B->part) return -1;
     B->part_blocks = nb;
     B->dirty_graph = true;
@@ -2311,7 +2330,8 @@ static int batch_check(BatchObj *B) {
 // regenerated, every list's primary chunks queued as the input of pass 0
 // everything a captured run bakes into its kernel arguments
 static std::vector<uint64_t> batch_graph_sig(BatchObj *B, int graph_passes) {
-  std::vector<uint64_t> v = {(uint64_t)graph_passes, (uint64_t)(uintptr_t)B->tab[0], (uint64_t)(uintptr_t)B->tab[1],
+  std::vector<uint64_t> v = {(uint64_t)graph_passes, (uint64_t)B->split, (uint64_t)(uintptr_t)B->kidx,
+                             (uint64_t)(uintptr_t)B->tab[0], (uint64_t)(uintptr_t)B->tab[1],
                              (uint64_t)B->tcap[0], (uint64_t)B->tcap[1], (uint64_t)(uintptr_t)B->pc,
                              (uint64_t)B->pc_cap, (uint64_t)(uintptr_t)B->d_init, (uint64_t)B->ninit,
                              (uint64_t)(uintptr_t)B->d_ds, (uint64_t)(uintptr_t)B->d_lists,
@@ -2367,7 +2387,7 @@ static int batch_setup_kernels(BatchObj *B) {
 
 static int batch_pass_kernels(BatchObj *B, int pass) {
   BatchArgs A = batch_args(B, pass);
-  HIPCHK(launch_batch_pass(A, batch_launch(B), S()));
+  HIPCHK(launch_batch_pass(A, batch_launch(B), S(), B->ks, B->ev));
   HIPCHK(launch_batch_reduce(A, S()));
   return 0;
 }
@@ -2602,6 +2622,20 @@ GXY_EXPORT int gxy_batch_set_config(void *b, int chunk, int stack, int wpe, int 
   return batch_ensure_base(B);
 }
 
+GXY_EXPORT int gxy_batch_set_split(void *b, int split) {
+  INIT_OR(-1);
+  BatchObj *B = lookup<BatchObj>(g_batches, b);
+  if (!B) return fail("not a batch handle");
+  if (split != 0 && split != 1) return fail("split %d: 0 or 1", split);
+  B->split = split;
+  B->dirty_graph = true;
+  return 0;
+}
+GXY_EXPORT int gxy_batch_get_split(void *b) {
+  BatchObj *B = lookup<BatchObj>(g_batches, b);
+  if (!B) return fail("not a batch handle");
+  return B->split;
+}
 GXY_EXPORT int gxy_batch_get_config(void *b, int *chunk, int *stack, int *wpe, int *bpc) {
   INIT_OR(-1);
   BatchObj *B = lookup<BatchObj>(g_batches, b);
@@ -2767,7 +2801,7 @@ GXY_EXPORT long long gxy_batch_bytes(void *b) {
     if (B->pool[k]) t += (long long)B->pool[k]->bytes;
   t += (long long)(B->tcap[0] + B->tcap[1] + B->init_cap + B->fresh_cap) * sizeof(BatchChunk);
   t += (long long)B->pc_cap * 5 * sizeof(int);
-  t += (long long)B->part_blocks * BS_COUNT * 8;
+  t += (long long)B->part_blocks * 4 * BS_COUNT * 8 + (long long)B->kcap * 3 * 4;
   return t;
 }
 

@@ -624,4 +624,35 @@ __device__ __forceinline__ unsigned sample_one(const TraceParams &P, const RayLi
   return nsamp;
 }
 
+// The sampler loop of one ray without re-queueing: SamplerTraceRays marches
+// to the next isovalue crossing, Sampler::HandleTerminatedRays appends the
+// particle at o + t*d (t = crossing + 0.001, Sampler.cpp:71-84), and the ray
+// continues from there (the KEEP_HERE re-trace of Renderer.cpp:659-691) until
+// it leaves the volume.  Every segment is one ray pass with the samples of
+// sample_one, so counts and particles equal those of the per-list loop (the
+// particles' order differs).  After max_segments segments the ray is left as
+// a SURFACE ray for the caller to re-queue (its particle not yet emitted).
+template <int NV, int SM>
+__device__ __forceinline__ unsigned sample_all(const TraceParams &P, const RayList_ispc &R, const int i, float *xyz,
+                                               int *pcount, int pcap, int max_segments, unsigned &segments) {
+  unsigned nsamp = 0;
+  segments = 0;
+  for (;;) {
+    nsamp += sample_one<NV, SM>(P, R, i);
+    segments++;
+    if (!(gptr(R.term)[i] & RAY_SURFACE) || (int)segments >= max_segments) break;
+    const float t = gptr(R.t)[i];
+    const float px = gptr(R.ox)[i] + t * gptr(R.dx)[i];
+    const float py = gptr(R.oy)[i] + t * gptr(R.dy)[i];
+    const float pz = gptr(R.oz)[i] + t * gptr(R.dz)[i];
+    const int off = atomicAdd(pcount, 1);
+    if (off < pcap) {
+      xyz[3 * (int64_t)off + 0] = px;
+      xyz[3 * (int64_t)off + 1] = py;
+      xyz[3 * (int64_t)off + 2] = pz;
+    }
+  }
+  return nsamp;
+}
+
 }  // namespace gxy

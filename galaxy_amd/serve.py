@@ -39,9 +39,10 @@ class Server:
         self.keep.append(lights)
         glib.check(self.L.gxy_server_set_lighting(self.h, C.byref(lights), eps), "server_set_lighting")
 
-    def set_config(self, chunk, stack, wpe, blocks_per_cu):
+    def set_config(self, chunk, stack, wpe, blocks_per_cu, split=0):
         b = self.L.gxy_server_batch(self.h)
         glib.check(self.L.gxy_batch_set_config(b, chunk, stack, wpe, blocks_per_cu), "batch_set_config")
+        glib.check(self.L.gxy_batch_set_split(b, split), "batch_set_split")
 
     def run(self, reqs, mode=abi.SERVE_CONTINUOUS):
         """reqs: a ctypes array of abi.Request sorted by arrival.  Returns

@@ -377,7 +377,8 @@ typedef struct gxy_batch_stats {
   unsigned long long spawned;     /* shadow + AO rays generated */
   unsigned long long terminated;
   unsigned long long dropped;
-  unsigned long long kept;        /* KEEP_HERE re-traces */
+  unsigned long long kept;        /* rays re-queued as KEEP_HERE (sampler rays follow up to 4096
+                                     crossings within one pass before they are re-queued) */
   int passes;                     /* passes run */
   int overflow;                   /* nonzero: a pass ran out of pool or table space (results incomplete) */
 } gxy_batch_stats;
@@ -414,6 +415,11 @@ int   gxy_batch_remove_list(void *batch, int list);
  * blocks_per_cu (persistent workgroups per compute unit, 1..16). */
 int   gxy_batch_set_config(void *batch, int chunk, int stack, int wpe, int blocks_per_cu);
 int   gxy_batch_get_config(void *batch, int *chunk, int *stack, int *wpe, int *blocks_per_cu);
+/* split 0: one persistent trace kernel takes every chunk; 1: three persistent
+ * kernels (volume render, geometry, sampler chunks), each with its own
+ * register budget, run concurrently on three streams. */
+int   gxy_batch_set_split(void *batch, int split);
+int   gxy_batch_get_split(void *batch);
 /* Runs every list from its own rays to completion (clears framebuffers and
  * particle counts first).  graph_passes > 0: the set-up and that many passes
  * are captured once in a HIP graph and replayed on later runs (re-captured

@@ -167,7 +167,8 @@ def test_sample_lists_equal_sampler_loop(datasets, hip):
     lr, fbr = B.add_render_list(ids[3], d, 3000, 1)
     B.set_seed(lr, 950)
     st = B.run(0)
-    assert st.passes > 3
+    # sampler rays follow all their crossings within a pass; the render list needs its shadow/AO pass
+    assert st.passes == 2
     for n, ds, seed, xyz, cnt in runs:
         c = int(cnt.to_host(np.int32, 1)[0])
         d = gb.random_rays(n, seed, width=n)
