@@ -145,11 +145,11 @@ def tune(batch, candidates=None, max_depth=64, graph_passes=2, reps=1):
     for cfg in candidates or DEFAULT_CANDIDATES:
         if cfg[1] and max_depth + 2 > cfg[1]:
             continue
-        batch.set_config(*cfg)
         try:
+            batch.set_config(*cfg)
             batch.run(graph_passes)     # capture / warm
         except glib.GxyError:
-            continue                    # a traversal stack too short for some model
+            continue                    # e.g. a traversal stack too short for some model
         glib.check(L.gxy_synchronize())
         t = time.perf_counter()
         for _ in range(reps):

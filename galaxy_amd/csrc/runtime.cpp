@@ -2603,7 +2603,6 @@ GXY_EXPORT int gxy_batch_set_config(void *b, int chunk, int stack, int wpe, int 
   if (chunk < 64 || chunk > 1024 || chunk % 64) return fail("chunk %d: a multiple of 64 in [64, 1024]", chunk);
   if (stack != 0 && stack != 16 && stack != 24 && stack != 32) return fail("stack %d: 0, 16, 24 or 32", stack);
   if (wpe != 0 && wpe != 4 && wpe != 5) return fail("wpe %d: 0, 4 or 5", wpe);
-  if (wpe == 5 && stack != 0) return fail("wpe 5 is built for the private-memory stack only");
   if (bpc < 1 || bpc > 16) return fail("blocks per CU %d", bpc);
   if (chunk != B->chunk) {
     if (B->queued > 0) return fail("chunk size changes only with no rays queued");
