@@ -315,6 +315,18 @@ def bench_served(L, hip, nreq, seed=5, layout=0, config=None):
         res[name] = d
     res["throughput_ratio"] = round(res["continuous"]["mrays_per_s"] / max(1e-9, res["one_at_a_time"]["mrays_per_s"]),
                                     3)
+    # the same mix at 5 requests/s, which one GPU keeps up with: latency without queueing behind a backlog
+    rng = np.random.default_rng(seed + 1)
+    reqs, _ = serve.request_stream(24, 5.0, 0.5, rng, volume_ds=(0, 1, 2), any_ds=(0, 1, 2, 3, 4, 5))
+    light = dict(requests=24, arrival_rate=5.0, arrival_span_s=round(reqs[23].arrival, 3))
+    for mode, name in ((abi.SERVE_CONTINUOUS, "continuous"), (abi.SERVE_ONE_AT_A_TIME, "one_at_a_time")):
+        log("serving at 5/s", name)
+        r, st = S.run(reqs, mode)
+        d = serve.latency_summary(reqs, r)
+        d.update(mrays_per_s=round(st.rays / st.wall / 1e6, 2), wall_s=round(st.wall, 3),
+                 ray_passes=int(st.rays), max_in_flight=int(st.max_in_flight))
+        light[name] = d
+    res["at_5_per_s"] = light
     S.destroy()
     for b in built:
         release(L, b)
