@@ -1099,13 +1099,56 @@ __global__ void k_gather_sph(const int *order, const int *fgeom, const int *fpri
 // Collapses a binary BVH (device) into 4-wide nodes (device): each 4-wide
 // node takes a binary inner node's two children and repeatedly opens the
 // inner child with the largest box surface until it holds four children (or
-// only leaves remain); leaf codes are kept, inner children become 4-wide
-// nodes, depth first.  Returns the node buffer (allocated with dalloc) and the
+// only leaves remain); a subtree of at most BVH4_LEAF primitives (a
+// contiguous range of the sorted primitives) becomes one leaf, other inner
+// children become 4-wide nodes, depth first.  Returns the node buffer (allocated with dalloc) and the
 // 4-wide depth.
+#define BVH4_LEAF 4
 static int collapse_bvh4(const BVHNode *dnodes, int nnodes, BVH4Node **out, int *depth4) {
   std::vector<BVHNode> h(nnodes);
   HIPCHK(hipMemcpyAsync(h.data(), dnodes, sizeof(BVHNode) * nnodes, hipMemcpyDeviceToHost, S()));
   HIPCHK(hipStreamSynchronize(S()));
+  // sorted-primitive range of every binary subtree (a Karras subtree covers a
+  // contiguous range), so that a subtree of at most BVH4_LEAF primitives
+  // becomes one leaf of the 4-wide tree
+  std::vector<int> first(nnodes, 0), last(nnodes, -1);
+  {
+    std::vector<std::pair<int, int>> st;   // (node, state)
+    st.push_back({0, 0});
+    while (!st.empty()) {
+      auto [n, state] = st.back();
+      st.pop_back();
+      const int c[2] = {h[n].c0, h[n].c1};
+      if (state == 0) {
+        st.push_back({n, 1});
+        for (int k = 0; k < 2; k++)
+          if (c[k] >= 0) st.push_back({c[k], 0});
+        continue;
+      }
+      int f = INT32_MAX, l = -1;
+      for (int k = 0; k < 2; k++) {
+        int cf, cl;
+        if (c[k] >= 0) {
+          cf = first[c[k]];
+          cl = last[c[k]];
+        } else {
+          const int code = ~c[k];
+          cf = code >> 4;
+          cl = cf + (code & 15);
+        }
+        f = std::min(f, cf);
+        l = std::max(l, cl);
+      }
+      first[n] = f;
+      last[n] = l;
+    }
+  }
+  // an inner entry whose subtree holds at most BVH4_LEAF primitives becomes a leaf
+  auto as_leaf = [&](int code) {
+    if (code < 0) return code;
+    const int cnt = last[code] - first[code] + 1;
+    return cnt <= BVH4_LEAF ? ~((first[code] << 4) | (cnt - 1)) : code;
+  };
   struct Ent {
     int code;
     float lo[3], hi[3];
@@ -1131,8 +1174,8 @@ static int collapse_bvh4(const BVHNode *dnodes, int nnodes, BVH4Node **out, int 
     maxd = std::max(maxd, j.depth);
     const BVHNode &b = h[j.bin];
     std::vector<Ent> es(2);
-    es[0].code = b.c0;
-    es[1].code = b.c1;
+    es[0].code = as_leaf(b.c0);
+    es[1].code = as_leaf(b.c1);
     for (int k = 0; k < 3; k++) {
       es[0].lo[k] = b.lo0[k];
       es[0].hi[k] = b.hi0[k];
@@ -1146,8 +1189,8 @@ static int collapse_bvh4(const BVHNode *dnodes, int nnodes, BVH4Node **out, int 
       if (best < 0) break;
       const BVHNode &c = h[es[best].code];
       Ent a, d;
-      a.code = c.c0;
-      d.code = c.c1;
+      a.code = as_leaf(c.c0);
+      d.code = as_leaf(c.c1);
       for (int k = 0; k < 3; k++) {
         a.lo[k] = c.lo0[k];
         a.hi[k] = c.hi0[k];
@@ -1183,6 +1226,36 @@ static int collapse_bvh4(const BVHNode *dnodes, int nnodes, BVH4Node **out, int 
       }
     }
     n4[j.slot] = nd;
+  }
+  // structure check: child indices in range, every node reached once, every
+  // primitive in exactly one leaf (leaf ranges inside the primitive arrays)
+  {
+    int nprim = 0;
+    for (int k = 0; k < nnodes; k++) nprim = std::max(nprim, last[k] + 1);
+    std::vector<char> seen_n(n4.size(), 0), seen_p(nprim, 0);
+    std::vector<int> st{0};
+    while (!st.empty()) {
+      const int n = st.back();
+      st.pop_back();
+      if (n < 0 || n >= (int)n4.size() || seen_n[n]) return fail("4-wide BVH: node %d invalid or shared", n);
+      seen_n[n] = 1;
+      for (int k = 0; k < 4; k++) {
+        if (!(n4[n].lox[k] <= n4[n].hix[k])) continue;   // empty slot
+        const int c = n4[n].child[k];
+        if (c >= 0) {
+          st.push_back(c);
+        } else {
+          const int code = ~c, a = code >> 4, cnt = (code & 15) + 1;
+          if (a < 0 || a + cnt > nprim) return fail("4-wide BVH: leaf range %d+%d outside %d", a, cnt, nprim);
+          for (int q = a; q < a + cnt; q++) seen_p[q]++;
+        }
+      }
+    }
+    const bool single = nnodes == 1 && h[0].c0 == h[0].c1;   // one primitive: both children are its leaf
+    for (int q = 0; q < nprim; q++)
+      if (seen_p[q] != (single ? 2 : 1)) return fail("4-wide BVH: primitive %d in %d leaves", q, (int)seen_p[q]);
+    for (size_t k = 0; k < n4.size(); k++)
+      if (!seen_n[k]) return fail("4-wide BVH: node %zu unreachable", k);
   }
   BVH4Node *d4 = (BVH4Node *)dalloc(sizeof(BVH4Node) * n4.size());
   if (!d4) return -1;

@@ -185,11 +185,13 @@ __device__ __forceinline__ void bvh_walk(const BVHNode *nodes, f3 o, f3 inv, flo
   }
 }
 
-// Nearest-first walk of a 4-wide BVH.  A node's children are box-tested
-// against [t0, current hit t]; the walk descends into the nearest hit child
-// and pushes one entry (node << 4 | mask of the other hit children), which is
-// re-tested against the then-current hit t when popped, so the stack never
-// holds more than one entry per level (depth4 entries).
+// Nearest-first walk of a 4-wide BVH.  At a node, the children's boxes are
+// tested against [t0, current hit t]; leaf children that are hit are
+// intersected at once (the LBVH leaves hold one primitive), then the walk
+// descends into the nearest inner child that is still hit and pushes one
+// entry (node << 4 | mask of the other hit inner children), re-tested against
+// the then-current hit t when popped, so the stack holds at most one entry per
+// level (depth4 entries).
 template <class Stack, class LeafFn>
 __device__ __forceinline__ void bvh4_walk(const BVH4Node *nodes, f3 o, f3 inv, float t0, const GeoHit &h,
                                           Stack &stack, LeafFn &&leaf) {
@@ -203,33 +205,41 @@ __device__ __forceinline__ void bvh4_walk(const BVH4Node *nodes, f3 o, f3 inv, f
     const float lox[4] = {lx.x, lx.y, lx.z, lx.w}, loy[4] = {ly.x, ly.y, ly.z, ly.w}, loz[4] = {lz.x, lz.y, lz.z, lz.w};
     const float hix[4] = {hx.x, hx.y, hx.z, hx.w}, hiy[4] = {hy.x, hy.y, hy.z, hy.w}, hiz[4] = {hz.x, hz.y, hz.z, hz.w};
     const int cc[4] = {ch.x, ch.y, ch.z, ch.w};
-    int kn = -1;
-    float tn = 0.f;
-    int hitm = 0;
+    float te[4];
+    int inner = 0;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
+      te[k] = 0.f;
       if ((mask >> k) & 1) {
         const float lo[3] = {lox[k], loy[k], loz[k]}, hi[3] = {hix[k], hiy[k], hiz[k]};
-        float te;
-        if (child_hit(lo, hi, o, inv, t0, h.t, te)) {
-          hitm |= 1 << k;
-          if (kn < 0 || te < tn) { kn = k; tn = te; }
+        if (child_hit(lo, hi, o, inv, t0, h.t, te[k])) {
+          if (cc[k] >= 0) {
+            inner |= 1 << k;
+          } else {
+            const int code = ~cc[k];
+            leaf(code >> 4, (code & 15) + 1);
+          }
         }
       }
     }
+    // the nearest inner child whose box still starts before the current hit
+    int kn = -1;
+    float tn = 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (((inner >> k) & 1) && te[k] <= h.t * 1.0000005f + 1e-30f && (kn < 0 || te[k] < tn)) {
+        kn = k;
+        tn = te[k];
+      }
     if (kn >= 0) {
-      const int rest = hitm & ~(1 << kn);
+      const int rest = inner & ~(1 << kn);
       if (rest && sp < Stack::DEPTH) stack.put(sp++, (cur << 4) | rest);
       int c = cc[0];
 #pragma unroll
       for (int k = 1; k < 4; k++) c = (k == kn) ? cc[k] : c;
-      if (c >= 0) {
-        cur = c;
-        mask = 0xF;
-        continue;
-      }
-      const int code = ~c;
-      leaf(code >> 4, (code & 15) + 1);
+      cur = c;
+      mask = 0xF;
+      continue;
     }
     if (sp == 0) return;
     const int e = stack.get(--sp);

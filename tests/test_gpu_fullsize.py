@@ -43,7 +43,7 @@ def test_full_size_volume_frame_matches_oracle_on_sampled_rows(hip, oracle):
                                         C.byref(so))
         g = fb1.reshape(cam.height, cam.width, 4)[::8]
         o = fbo.reshape(cam.height, cam.width, 4)[::8]
-        assert so.primary_rays > 100_000
+        assert so.ray_passes > 200_000
         assert np.abs(g - o).max() <= 1e-4
         assert np.sqrt(np.mean((g - o) ** 2)) <= 1e-5
     finally:
