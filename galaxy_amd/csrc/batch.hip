@@ -198,9 +198,15 @@ void k_btrace(BatchArgs A) {
       }
     } else if (P.has_model) {
       acc[BS_RAYS] += lane < ch.count ? (ch.count - 1 - lane) / 64 + 1 : 0;
+      const bool wide = P.model.wide != 0;
       for (int k = lane; k < ch.count; k += 64) {
-        if (STK > 0) acc[BS_SAMPLES] += trace_one<0, true>(P, R, ch.start + k, s_col[w], s_op[w], lstack);
-        else acc[BS_SAMPLES] += trace_one<0, true>(P, R, ch.start + k, s_col[w], s_op[w], pstack);
+        if (STK > 0) {
+          if (wide) acc[BS_SAMPLES] += trace_one<0, true, decltype(lstack), -1, 4>(P, R, ch.start + k, s_col[w], s_op[w], lstack);
+          else acc[BS_SAMPLES] += trace_one<0, true, decltype(lstack), -1, 2>(P, R, ch.start + k, s_col[w], s_op[w], lstack);
+        } else {
+          if (wide) acc[BS_SAMPLES] += trace_one<0, true, decltype(pstack), -1, 4>(P, R, ch.start + k, s_col[w], s_op[w], pstack);
+          else acc[BS_SAMPLES] += trace_one<0, true, decltype(pstack), -1, 2>(P, R, ch.start + k, s_col[w], s_op[w], pstack);
+        }
       }
     } else {
       acc[BS_RAYS] += lane < ch.count ? (ch.count - 1 - lane) / 64 + 1 : 0;
@@ -278,9 +284,15 @@ void k_btrace_kind(BatchArgs A) {
       }
     } else if (KIND == 1) {
       acc[BS_RAYS] += lane < ch.count ? (ch.count - 1 - lane) / 64 + 1 : 0;
+      const bool wide = P.model.wide != 0;
       for (int k = lane; k < ch.count; k += 64) {
-        if (STK > 0) acc[BS_SAMPLES] += trace_one<0, true>(P, R, ch.start + k, s_col[0], s_op[0], lstack);
-        else acc[BS_SAMPLES] += trace_one<0, true>(P, R, ch.start + k, s_col[0], s_op[0], pstack);
+        if (STK > 0) {
+          if (wide) acc[BS_SAMPLES] += trace_one<0, true, decltype(lstack), -1, 4>(P, R, ch.start + k, s_col[0], s_op[0], lstack);
+          else acc[BS_SAMPLES] += trace_one<0, true, decltype(lstack), -1, 2>(P, R, ch.start + k, s_col[0], s_op[0], lstack);
+        } else {
+          if (wide) acc[BS_SAMPLES] += trace_one<0, true, decltype(pstack), -1, 4>(P, R, ch.start + k, s_col[0], s_op[0], pstack);
+          else acc[BS_SAMPLES] += trace_one<0, true, decltype(pstack), -1, 2>(P, R, ch.start + k, s_col[0], s_op[0], pstack);
+        }
       }
     } else {
       acc[BS_RAYS] += lane < ch.count ? (ch.count - 1 - lane) / 64 + 1 : 0;

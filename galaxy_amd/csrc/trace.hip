@@ -55,7 +55,8 @@ hipError_t launch_reduce_add(const unsigned long long *part, int nb, unsigned lo
 // SM = 1 when every volume is small (32-bit voxel offsets), -1 otherwise.
 // STK > 0: the BVH traversal stack holds STK entries per work-item in LDS
 // (instead of 64 in private memory).
-template <int NV, bool GEOM, int WPE, int SM, int STK = 0>
+// W: BVH walked by geometry variants (2 binary, 4 four-wide, 0 per model).
+template <int NV, bool GEOM, int WPE, int SM, int STK = 0, int W = 0>
 __global__ __launch_bounds__(GXY_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, 8)))
 void k_trace(TraceParams P, RayList_ispc R, int n) {
   constexpr int NVS = NV > 0 ? NV : 1;
@@ -75,10 +76,10 @@ void k_trace(TraceParams P, RayList_ispc R, int n) {
     LdsStack<(STK > 0 ? STK : 1)> stack;
     stack.base = &s_stack[0][threadIdx.x];
     stack.stride = GXY_BLOCK;
-    if (i < n) nsamp = trace_one<NV, GEOM, LdsStack<(STK > 0 ? STK : 1)>, SM>(P, R, i, s_col, s_op, stack);
+    if (i < n) nsamp = trace_one<NV, GEOM, LdsStack<(STK > 0 ? STK : 1)>, SM, W>(P, R, i, s_col, s_op, stack);
   } else {
     PrivStack<GXY_BVH_STACK> stack;
-    if (i < n) nsamp = trace_one<NV, GEOM, PrivStack<GXY_BVH_STACK>, SM>(P, R, i, s_col, s_op, stack);
+    if (i < n) nsamp = trace_one<NV, GEOM, PrivStack<GXY_BVH_STACK>, SM, W>(P, R, i, s_col, s_op, stack);
   }
   if (P.sample_partials) block_sum_store(nsamp, P.sample_partials);
 }
@@ -385,10 +386,18 @@ static hipError_t launch_trace_nv(const TraceParams &P, const RayList_ispc &R, i
   for (int v = 0; v < NV; v++) small = small && P.vv[v].vol.small;
   // an LDS stack only when it holds the deepest path of the model
   if (NV == 0 && P.has_model && stack > 0 && (P.model.wide ? P.model.depth4 : P.model.depth) + 2 > stack) stack = 0;
-  if (NV == 0 && P.has_model && stack == 16) hipLaunchKernelGGL((k_trace<0, true, 0, -1, 16>), g, b, 0, s, P, R, n);
-  else if (NV == 0 && P.has_model && stack == 24) hipLaunchKernelGGL((k_trace<0, true, 0, -1, 24>), g, b, 0, s, P, R, n);
-  else if (NV == 0 && P.has_model && stack == 32) hipLaunchKernelGGL((k_trace<0, true, 0, -1, 32>), g, b, 0, s, P, R, n);
-  else if (P.has_model) hipLaunchKernelGGL((k_trace<NV, true, 0, -1>), g, b, 0, s, P, R, n);
+  const bool wide = P.model.wide != 0;
+  if (NV == 0 && P.has_model && wide) {
+    if (stack == 16) hipLaunchKernelGGL((k_trace<0, true, 0, -1, 16, 4>), g, b, 0, s, P, R, n);
+    else if (stack == 24) hipLaunchKernelGGL((k_trace<0, true, 0, -1, 24, 4>), g, b, 0, s, P, R, n);
+    else if (stack == 32) hipLaunchKernelGGL((k_trace<0, true, 0, -1, 32, 4>), g, b, 0, s, P, R, n);
+    else hipLaunchKernelGGL((k_trace<0, true, 0, -1, 0, 4>), g, b, 0, s, P, R, n);
+  } else if (NV == 0 && P.has_model) {
+    if (stack == 16) hipLaunchKernelGGL((k_trace<0, true, 0, -1, 16, 2>), g, b, 0, s, P, R, n);
+    else if (stack == 24) hipLaunchKernelGGL((k_trace<0, true, 0, -1, 24, 2>), g, b, 0, s, P, R, n);
+    else if (stack == 32) hipLaunchKernelGGL((k_trace<0, true, 0, -1, 32, 2>), g, b, 0, s, P, R, n);
+    else hipLaunchKernelGGL((k_trace<0, true, 0, -1, 0, 2>), g, b, 0, s, P, R, n);
+  } else if (P.has_model) hipLaunchKernelGGL((k_trace<NV, true, 0, -1>), g, b, 0, s, P, R, n);
   else if (NV == 1 && P.vv[0].vol.brick == 8 && wpe == 5) hipLaunchKernelGGL((k_trace<NV, false, 5, 8>), g, b, 0, s, P, R, n);
   else if (NV == 1 && P.vv[0].vol.brick == 8 && wpe == 6) hipLaunchKernelGGL((k_trace<NV, false, 6, 8>), g, b, 0, s, P, R, n);
   else if (NV == 1 && P.vv[0].vol.brick == 8) hipLaunchKernelGGL((k_trace<NV, false, 0, 8>), g, b, 0, s, P, R, n);

@@ -248,7 +248,9 @@ __device__ __forceinline__ void bvh4_walk(const BVH4Node *nodes, f3 o, f3 inv, f
   }
 }
 
-template <class Stack>
+// W = 2 or 4: the tree the walk uses; W = 0: chosen per call from M.wide
+// (both walks compiled into the caller).
+template <class Stack, int W = 0>
 __device__ __forceinline__ void model_trace(const DModel &M, f3 org, f3 dir, float t0, float tfar, GeoHit &h,
                                             Stack &stack) {
   h.t = tfar;
@@ -270,7 +272,7 @@ __device__ __forceinline__ void model_trace(const DModel &M, f3 org, f3 dir, flo
         }
       }
     };
-    if (M.wide) bvh4_walk(M.tnodes4, org, inv, t0, h, stack, tleaf);
+    if (W == 4 || (W == 0 && M.wide)) bvh4_walk(M.tnodes4, org, inv, t0, h, stack, tleaf);
     else bvh_walk(M.tnodes, org, inv, t0, h, stack, tleaf);
   }
   if (M.nsph > 0) {
@@ -286,7 +288,7 @@ __device__ __forceinline__ void model_trace(const DModel &M, f3 org, f3 dir, flo
         }
       }
     };
-    if (M.wide) bvh4_walk(M.snodes4, org, inv, t0, h, stack, sleaf);
+    if (W == 4 || (W == 0 && M.wide)) bvh4_walk(M.snodes4, org, inv, t0, h, stack, sleaf);
     else bvh_walk(M.snodes, org, inv, t0, h, stack, sleaf);
   }
 }
@@ -345,7 +347,7 @@ __device__ __forceinline__ bool opaque_of(float r, float g, float b, float a, fl
 // One ray of TraceRays_TraceRays (TraceRays.ispc:371-622): reads ray i of R,
 // writes its colour, t, term and surface fields back.  s_col / s_op hold the
 // NV transfer-function tables (LDS).  Returns the number of volume samples.
-template <int NV, bool GEOM, class Stack, int SM = -1>
+template <int NV, bool GEOM, class Stack, int SM = -1, int W = 0>
 __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayList_ispc &R, const int i,
                                               const float (*s_col)[3 * GXY_TF_MAX],
                                               const float (*s_op)[GXY_TF_MAX], Stack &stack) {
@@ -411,7 +413,7 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
     // LookForGeometryHit (TraceRays.ispc:108-141)
     if (GEOM) {
       GeoHit gh;
-      model_trace(P.model, org, dir, ray_t0, ray_t, gh, stack);
+      model_trace<Stack, W>(P.model, org, dir, ray_t0, ray_t, gh, stack);
       if (P.hit_geom) { P.hit_geom[i] = gh.geom; P.hit_prim[i] = gh.prim; }
       if (gh.geom >= 0) {
         ray_t = gh.t;
