@@ -60,6 +60,10 @@ hipError_t launch_surface_scan(const RayList_ispc &R, int n, int *scratch, int *
 // order), then the parents' ambient + diffuse update.
 hipError_t launch_lighting_children(const LightParams &L, int mode, const RayList_ispc &R, const int *pidx, int T,
                                     const RayList_ispc &O, hipStream_t s);
+// The same with the parent count T on the device (at most n parents): the
+// grids cover the upper bound and work-items beyond T exit.
+hipError_t launch_lighting_children_dev(const LightParams &L, int mode, const RayList_ispc &R, const int *pidx,
+                                        const int *T, int n, const RayList_ispc &O, hipStream_t s);
 // Offsets as TraceRays.cpp:89-117 from the ranks: ao = nAO*rank, shadow =
 // nAO*total + nLights*rank (or -1).
 hipError_t launch_offsets_from_rank(const RayList_ispc &R, int n, const int *rank, const int *total, int nAO,
@@ -80,6 +84,8 @@ hipError_t launch_accumulate_sampler(const RayList_ispc &R, int n, float *xyz, i
                                      int *scratch, hipStream_t s);
 hipError_t launch_camera(const CameraParams &C, const RayList_ispc &R, int capacity, int *scratch, int *total,
                          hipStream_t s);
+// writes the 64-byte RayList header hdr64 (host memory, copied into the launch) to block
+hipError_t launch_write_header(void *block, const void *hdr64, hipStream_t s);
 hipError_t launch_copy_rays(const RayList_ispc &src, int start, int count, const RayList_ispc &dst, int dst_start,
                             hipStream_t s);
 

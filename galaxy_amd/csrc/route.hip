@@ -374,4 +374,21 @@ hipError_t launch_copy_rays(const RayList_ispc &src, int start, int count, const
   return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ RayList header
+// the 64-byte block header written by a kernel (stream ordered, no host wait)
+struct HdrWords {
+  unsigned w[16];
+};
+
+__global__ void k_write_header(unsigned *dst, HdrWords h) {
+  if (threadIdx.x < 16) dst[threadIdx.x] = h.w[threadIdx.x];
+}
+
+hipError_t launch_write_header(void *block, const void *hdr64, hipStream_t s) {
+  HdrWords h;
+  memcpy(&h, hdr64, sizeof h);
+  hipLaunchKernelGGL(k_write_header, dim3(1), dim3(64), 0, s, (unsigned *)block, h);
+  return hipGetLastError();
+}
+
 }  // namespace gxy

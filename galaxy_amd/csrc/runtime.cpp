@@ -342,8 +342,12 @@ static RayListObj *raylist_new(int nrays, int frame, int type) {
     return nullptr;
   }
   set_pointers(r);
-  hipMemcpyAsync(r->block, &r->hdr, sizeof r->hdr, hipMemcpyHostToDevice, S());
-  hipStreamSynchronize(S());
+  if (launch_write_header(r->block, &r->hdr, S()) != hipSuccess) {
+    fail("RayList header write failed");
+    dfree(r->block);
+    delete r;
+    return nullptr;
+  }
   std::lock_guard<std::recursive_mutex> lk(G.mu);
   G.raylists.insert(r);
   return r;
@@ -1638,6 +1642,16 @@ struct Scratch {
   }
 };
 
+struct OwnedList {
+  RayListObj *p = nullptr;
+  RayListObj *release() {
+    RayListObj *r = p;
+    p = nullptr;
+    return r;
+  }
+  ~OwnedList() { raylist_delete(p, false); }
+};
+
 // TraceRays.cpp:89-117 on the device; n_ao / n_shadow through one readback
 static int spawn_offsets(int n, const RayList_ispc &R, int nAO, int nL, int doSh, int *ao_off, int *sh_off,
                          int *T_out) {
@@ -1896,42 +1910,41 @@ static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
     C.st.ray_passes += n;
     C.st.passes++;
     C.st.lists++;
+    Scratch spawn_sc;
+    int *scr_spawn = nullptr, *dT = nullptr, nsp = 0;
+    OwnedList out;   // the children, freed on an early return
     if (!C.sampler) {
       // TraceRays::Trace (TraceRays.cpp:68-146): the parents (primary surface
       // rays) ranked by one flag scan; the AO block then the shadow block of
       // their children, written one work-item per child
-      Scratch sc;
+      // the parent count stays on the device until the partition counts are
+      // read (one host wait per list): the children list is allocated for
+      // every ray being a parent and its size set afterwards
       const int nb = (n + GXY_BLOCK - 1) / GXY_BLOCK;
-      int *scratch = (int *)sc.get(sizeof(int) * (nb + 2));
-      int *pidx = (int *)sc.get(sizeof(int) * std::max(1, n));
-      if (!scratch || !pidx) return -1;
-      int *total = scratch + nb + 1;
+      scr_spawn = (int *)spawn_sc.get(sizeof(int) * (nb + 2));
+      int *pidx = (int *)spawn_sc.get(sizeof(int) * std::max(1, n));
+      if (!scr_spawn || !pidx) return -1;
+      int *total = scr_spawn + nb + 1;
       {
         KTimer kt("spawn_scan");
-        HIPCHK(launch_surface_scan(R, n, scratch, nullptr, total, S(), pidx));
+        HIPCHK(launch_surface_scan(R, n, scr_spawn, nullptr, total, S(), pidx));
       }
-      int T = 0;
-      HIPCHK(hipMemcpyAsync(&T, total, sizeof(int), hipMemcpyDeviceToHost, S()));
-      HIPCHK(hipStreamSynchronize(S()));
-      const long long nao = (long long)C.L.nAO * T, nsh = C.L.doShadows ? (long long)C.L.nLights * T : 0;
-      if (nao + nsh > INT32_MAX - 16) return fail("%lld secondary rays exceed one RayList", nao + nsh);
-      RayListObj *out = nullptr;
-      if (nao + nsh > 0) {
-        out = raylist_new((int)(nao + nsh), L->hdr.frame, 1);
-        if (!out) return -1;
+      nsp = C.L.nAO + (C.L.doShadows ? C.L.nLights : 0);
+      if ((long long)n * nsp > INT32_MAX - 16) return fail("%lld secondary rays exceed one RayList", (long long)n * nsp);
+      if (nsp > 0) {
+        out.p = raylist_new(n * nsp, L->hdr.frame, 1);
+        if (!out.p) return -1;
       }
-      int mode = LM_AMBIENT | LM_DIFFUSE | (nao ? LM_AO : 0) | (nsh ? LM_SHADOW : 0);
+      const int mode = LM_AMBIENT | LM_DIFFUSE | (C.L.nAO > 0 ? LM_AO : 0) |
+                       ((C.L.doShadows && C.L.nLights > 0) ? LM_SHADOW : 0);
       RayList_ispc O;
       memset(&O, 0, sizeof O);
-      if (out) O = out->ispc;
+      if (out.p) O = out.p->ispc;
       {
         KTimer kt("lighting");
-        HIPCHK(launch_lighting_children(C.L, mode, R, pidx, T, O, S()));
+        HIPCHK(launch_lighting_children_dev(C.L, mode, R, pidx, total, n, O, S()));
       }
-      if (out) {
-        C.st.secondary_rays += nao + nsh;
-        if (enqueue_split(C, out, 2) != 0) return -1;
-      }
+      dT = total;
     }
     // Classify, AssignDestinations, HandleTerminatedRays (Renderer.cpp:550-556)
     if (route(n, R, C.sampler ? (RM_CLASSIFY | RM_ASSIGN) : (RM_CLASSIFY | RM_ASSIGN | RM_ACCUMULATE), 1,
@@ -1971,7 +1984,18 @@ static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
       HIPCHK(launch_partition(R, n, table, ntable, K, nullptr, scr, dcounts, S()));
     }
     HIPCHK(hipMemcpyAsync(counts, dcounts, sizeof counts, hipMemcpyDeviceToHost, S()));
+    int T = 0;
+    if (dT) HIPCHK(hipMemcpyAsync(&T, dT, sizeof T, hipMemcpyDeviceToHost, S()));
     HIPCHK(hipStreamSynchronize(S()));
+    if (out.p && T > 0) {
+      // the children of the T parents (TraceRays.cpp:89-117 layout); the
+      // arrays stay laid out for the capacity (hdr.aligned_size)
+      RayListObj *o = out.release();
+      o->hdr.size = T * nsp;
+      HIPCHK(launch_write_header(o->block, &o->hdr, S()));
+      C.st.secondary_rays += (long long)T * nsp;
+      if (enqueue_split(C, o, 2) != 0) return -1;
+    }
     C.st.terminated += counts[7];
     C.st.dropped += counts[8];
     RayListObj *keep = nullptr;

@@ -126,8 +126,9 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_lighting(LightParams L, int mode,
 // nAO entries).  pidx: the parents' ray indices in rank order.  The children
 // read the parents' unmodified colours, so this runs before k_parents_light.
 __global__ __launch_bounds__(GXY_BLOCK) void k_children(LightParams L, int mode, RayList_ispc R, const int *pidx,
-                                                        int T, RayList_ispc O) {
+                                                        int Th, const int *Td, RayList_ispc O) {
   const long long t = (long long)blockIdx.x * GXY_BLOCK + threadIdx.x;
+  const int T = Td ? *Td : Th;
   const int nAO = (mode & LM_AO) ? L.nAO : 0, nL = (mode & LM_SHADOW) ? L.nLights : 0;
   const long long nao = (long long)nAO * T;
   if (t < nao) {
@@ -145,8 +146,9 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_children(LightParams L, int mode,
 
 // ambientLighting and diffuseLighting of the T parents (after k_children)
 __global__ __launch_bounds__(GXY_BLOCK) void k_parents_light(LightParams L, int mode, RayList_ispc R, const int *pidx,
-                                                             int T) {
+                                                             int Th, const int *Td) {
   const int p = blockIdx.x * GXY_BLOCK + threadIdx.x;
+  const int T = Td ? *Td : Th;
   if (p >= T) return;
   const int i = pidx[p];
   ParentRay P = parent_load(R, i);
@@ -161,10 +163,23 @@ hipError_t launch_lighting_children(const LightParams &L, int mode, const RayLis
   const long long nc = (long long)T * (((mode & LM_AO) ? L.nAO : 0) + ((mode & LM_SHADOW) ? L.nLights : 0));
   if (nc > 0)
     hipLaunchKernelGGL(k_children, dim3((unsigned)((nc + GXY_BLOCK - 1) / GXY_BLOCK)), dim3(GXY_BLOCK), 0, s, L,
-                       mode, R, pidx, T, O);
+                       mode, R, pidx, T, (const int *)nullptr, O);
   if (mode & (LM_AMBIENT | LM_DIFFUSE))
     hipLaunchKernelGGL(k_parents_light, dim3((T + GXY_BLOCK - 1) / GXY_BLOCK), dim3(GXY_BLOCK), 0, s, L, mode, R,
-                       pidx, T);
+                       pidx, T, (const int *)nullptr);
+  return hipGetLastError();
+}
+
+hipError_t launch_lighting_children_dev(const LightParams &L, int mode, const RayList_ispc &R, const int *pidx,
+                                        const int *T, int n, const RayList_ispc &O, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const long long nc = (long long)n * (((mode & LM_AO) ? L.nAO : 0) + ((mode & LM_SHADOW) ? L.nLights : 0));
+  if (nc > 0)
+    hipLaunchKernelGGL(k_children, dim3((unsigned)((nc + GXY_BLOCK - 1) / GXY_BLOCK)), dim3(GXY_BLOCK), 0, s, L,
+                       mode, R, pidx, 0, T, O);
+  if (mode & (LM_AMBIENT | LM_DIFFUSE))
+    hipLaunchKernelGGL(k_parents_light, dim3((n + GXY_BLOCK - 1) / GXY_BLOCK), dim3(GXY_BLOCK), 0, s, L, mode, R,
+                       pidx, 0, T);
   return hipGetLastError();
 }
 
