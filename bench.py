@@ -277,7 +277,7 @@ def bench_batched(L, hip, steps, tune=True, fixed=None):
     return res
 
 
-def bench_served(L, hip, nreq, seed=5, layout=0, config=None):
+def bench_served(L, hip, nreq, seed=5, layout=0, config=None, step_rays=4_000_000):
     """Poisson stream (200 requests/s) of render requests (256^2 to 3840x2160)
     and sampler requests (1M-50M rays) over resident 512^3-1024^3 noise volumes
     and 1M-20M triangle blob meshes, admitted under an HBM budget and served
@@ -308,11 +308,13 @@ def bench_served(L, hip, nreq, seed=5, layout=0, config=None):
         tune_ms.append((c, stb.wall * 1e3))
     best = min(tune_ms, key=lambda x: x[1])[0]
     S.set_config(*best)
+    # continuous batching steps take at most step_rays primary rays of admitted requests, fewest left first
+    S.set_step_rays(step_rays)
     nrender = sum(1 for r in reqs if r.kind == 0)
     res = dict(requests=nreq, render=nrender, sample=nreq - nrender, arrival_rate=200.0,
                arrival_span_s=round(reqs[nreq - 1].arrival, 3), budget_bytes=160 << 30, build_s=round(build_s, 2),
                datasets=["%s %d" % (k, s) for k, s, _ in descs], volume_layout=layout, config=list(best),
-               config_ms=[[list(c), round(m, 1)] for c, m in tune_ms])
+               config_ms=[[list(c), round(m, 1)] for c, m in tune_ms], step_rays=step_rays)
     # warm-up: one request of each kind
     w = (abi.Request * 2)()
     w[0] = serve.render_request(0, 256, 256, (0.5, 0.5, 2.4))
@@ -332,8 +334,11 @@ def bench_served(L, hip, nreq, seed=5, layout=0, config=None):
     rng = np.random.default_rng(seed + 1)
     reqs, _ = serve.request_stream(24, 5.0, 0.5, rng, volume_ds=(0, 1, 2), any_ds=(0, 1, 2, 3, 4, 5))
     light = dict(requests=24, arrival_rate=5.0, arrival_span_s=round(reqs[23].arrival, 3))
-    for mode, name in ((abi.SERVE_CONTINUOUS, "continuous"), (abi.SERVE_ONE_AT_A_TIME, "one_at_a_time")):
+    for mode, name, sr in ((abi.SERVE_CONTINUOUS, "continuous", step_rays),
+                           (abi.SERVE_CONTINUOUS, "continuous_whole_requests", 0),
+                           (abi.SERVE_ONE_AT_A_TIME, "one_at_a_time", step_rays)):
         log("serving at 5/s", name)
+        S.set_step_rays(sr)
         r, st = S.run(reqs, mode)
         d = serve.latency_summary(reqs, r)
         d.update(mrays_per_s=round(st.rays / st.wall / 1e6, 2), wall_s=round(st.wall, 3),
@@ -373,6 +378,8 @@ def main():
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--serve-requests", type=int, default=160)
+    ap.add_argument("--serve-step-rays", type=int, default=4_000_000,
+                    help="primary rays per continuous-batching step of the served form (0 = whole requests)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default trace launch configuration")
     ap.add_argument("--trace-config", default=None,
                     help="kind,chunk,refill: use this trace launch configuration (implies --no-tune)")
@@ -522,7 +529,8 @@ def main():
         try:
             lay = out.get("batched", {}).get("volume_layout", 0)
             cfg = out.get("batched", {}).get("config")
-            out["served"] = bench_served(L, hip, args.serve_requests, layout=lay, config=cfg)
+            out["served"] = bench_served(L, hip, args.serve_requests, layout=lay, config=cfg,
+                                         step_rays=args.serve_step_rays)
         except Exception as e:
             out["served"] = dict(error=str(e))
 

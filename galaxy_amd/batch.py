@@ -88,6 +88,10 @@ class Batch:
         glib.check(self.L.gxy_batch_get_config(self.h, *[C.byref(x) for x in v]), "batch_get_config")
         return tuple(x.value for x in v) + (self.L.gxy_batch_get_split(self.h),)
 
+    def set_step_rays(self, rays):
+        """Primary rays of added lists per step (0 = all); fewest left first."""
+        glib.check(self.L.gxy_batch_set_step_rays(self.h, int(rays)), "batch_set_step_rays")
+
     # ------------------------------------------------------------ running
     def run(self, graph_passes=0):
         st = abi.BatchStats()

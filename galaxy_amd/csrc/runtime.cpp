@@ -2219,6 +2219,7 @@ struct BatchObj {
   int *ints = nullptr;               // 0,1: table counts; 2 nrays_out; 3 head; 4 overflow
   int pass = 0;
   long long queued = 0;              // rays queued for the next pass (host view)
+  long long step_rays = 0;           // primary rays of added lists per step (0 = all)
   int passes_run = 0;
   bool dirty_tables = true;          // list / dataset tables need upload
   bool dirty_graph = true;
@@ -2302,9 +2303,9 @@ static int batch_pool(BatchObj *B, int k, long long n) {
   return 0;
 }
 
-static void batch_chunks_of(BatchObj *B, int l, std::vector<BatchChunk> &out) {
+static void batch_chunks_of(BatchObj *B, int l, std::vector<BatchChunk> &out, int from = 0) {
   const int n = B->lists[l].n;
-  for (int s = 0; s < n; s += B->chunk) {
+  for (int s = from; s < n; s += B->chunk) {
     BatchChunk c;
     c.src = 2 + l;
     c.start = s;
@@ -2703,12 +2704,13 @@ GXY_EXPORT int gxy_batch_set_config(void *b, int chunk, int stack, int wpe, int 
   if (bpc < 1 || bpc > 16) return fail("blocks per CU %d", bpc);
   if (chunk != B->chunk) {
     if (B->queued > 0) return fail("chunk size changes only with no rays queued");
-    std::vector<int> ls;
+    // re-cut the waiting primary rays of each list from its first waiting ray
+    std::vector<std::pair<int, int>> ls;
     for (const BatchChunk &c : B->fresh)
-      if (ls.empty() || ls.back() != c.list) ls.push_back(c.list);
+      if (ls.empty() || ls.back().first != c.list) ls.push_back({c.list, c.start});
     B->fresh.clear();
     B->chunk = chunk;
-    for (int l : ls) batch_chunks_of(B, l, B->fresh);
+    for (auto &lf : ls) batch_chunks_of(B, lf.first, B->fresh, lf.second);
   }
   B->chunk = chunk;
   B->stack = stack;
@@ -2726,6 +2728,19 @@ GXY_EXPORT int gxy_batch_set_split(void *b, int split) {
   B->split = split;
   B->dirty_graph = true;
   return 0;
+}
+GXY_EXPORT int gxy_batch_set_step_rays(void *b, long long rays) {
+  INIT_OR(-1);
+  BatchObj *B = lookup<BatchObj>(g_batches, b);
+  if (!B) return fail("not a batch handle");
+  if (rays < 0) return fail("step budget %lld", rays);
+  B->step_rays = rays;
+  return 0;
+}
+GXY_EXPORT long long gxy_batch_get_step_rays(void *b) {
+  BatchObj *B = lookup<BatchObj>(g_batches, b);
+  if (!B) return fail("not a batch handle");
+  return B->step_rays;
 }
 GXY_EXPORT int gxy_batch_get_split(void *b) {
   BatchObj *B = lookup<BatchObj>(g_batches, b);
@@ -2833,9 +2848,41 @@ GXY_EXPORT int gxy_batch_step(void *b, long long *next_rays) {
   HIPCHK(hipMemcpyAsync(&nin, B->ints + in, sizeof nin, hipMemcpyDeviceToHost, S()));
   HIPCHK(hipStreamSynchronize(S()));
   if (B->queued == 0) nin = 0;
+  // the primary chunks joining this step: all, or up to the step budget with
+  // the lists that have the fewest waiting rays first (each list's chunks stay
+  // in order, so its remaining rays are a suffix)
+  std::vector<BatchChunk> take;
+  if (B->step_rays <= 0) {
+    take.swap(B->fresh);
+  } else {
+    std::map<int, long long> left;
+    for (const BatchChunk &c : B->fresh) left[c.list] += c.count;
+    std::vector<std::pair<long long, int>> order;
+    for (auto &kv : left) order.push_back({kv.second, kv.first});
+    std::stable_sort(order.begin(), order.end());
+    std::map<int, long long> quota;
+    long long budget = B->step_rays;
+    for (auto &o : order) {
+      if (budget <= 0) break;
+      const long long q = std::min(o.first, budget);
+      quota[o.second] = q;
+      budget -= q;
+    }
+    std::vector<BatchChunk> rest;
+    for (const BatchChunk &c : B->fresh) {
+      auto it = quota.find(c.list);
+      if (it != quota.end() && it->second > 0) {
+        take.push_back(c);
+        it->second -= c.count;   // a whole chunk: the budget rounds up to the chunk
+      } else {
+        rest.push_back(c);
+      }
+    }
+    B->fresh.swap(rest);
+  }
   long long in_rays = B->queued;
-  for (const BatchChunk &c : B->fresh) in_rays += c.count;
-  const long long in_chunks = (long long)nin + (long long)B->fresh.size();
+  for (const BatchChunk &c : take) in_rays += c.count;
+  const long long in_chunks = (long long)nin + (long long)take.size();
   if (grow((void **)&B->tab[in], &B->tcap[in], in_chunks + 16, sizeof(BatchChunk), nin) != 0) return -1;
   long long prim = 0;
   for (size_t l = 0; l < B->lists.size(); l++)
@@ -2843,13 +2890,12 @@ GXY_EXPORT int gxy_batch_step(void *b, long long *next_rays) {
   if (batch_reserve_pass(B, p, in_rays, in_chunks, prim) != 0) return -1;
   if (batch_upload_tables(B) != 0) return -1;
   if (B->queued == 0) HIPCHK(launch_set_count(B->ints + in, 0, S()));
-  if (!B->fresh.empty()) {
-    const int nf = (int)B->fresh.size();
+  if (!take.empty()) {
+    const int nf = (int)take.size();
     if (grow((void **)&B->d_fresh, &B->fresh_cap, nf, sizeof(BatchChunk)) != 0) return -1;
-    HIPCHK(hipMemcpyAsync(B->d_fresh, B->fresh.data(), nf * sizeof(BatchChunk), hipMemcpyHostToDevice, S()));
+    HIPCHK(hipMemcpyAsync(B->d_fresh, take.data(), nf * sizeof(BatchChunk), hipMemcpyHostToDevice, S()));
     HIPCHK(launch_batch_append(B->tab[in], B->ints + in, B->tcap[in], B->d_fresh, nf, B->ints + 4, S()));
     HIPCHK(hipStreamSynchronize(S()));
-    B->fresh.clear();
   }
   {
     KTimer kt("batch_pass");
@@ -2862,7 +2908,9 @@ GXY_EXPORT int gxy_batch_step(void *b, long long *next_rays) {
   B->pass = p + 1;
   B->passes_run++;
   B->queued = rays;
-  if (next_rays) *next_rays = rays;
+  long long waiting = 0;
+  for (const BatchChunk &c : B->fresh) waiting += c.count;
+  if (next_rays) *next_rays = rays + waiting;
   return 0;
 }
 
@@ -2874,10 +2922,12 @@ GXY_EXPORT int gxy_batch_pending(void *b, int *per_list, int n) {
   if (nl <= 0) return 0;
   if (B->passes_run == 0) {
     for (int l = 0; l < nl; l++) per_list[l] = 0;
-    return 0;
+  } else {
+    HIPCHK(hipMemcpyAsync(per_list, B->lpend, nl * sizeof(int), hipMemcpyDeviceToHost, S()));
+    HIPCHK(hipStreamSynchronize(S()));
   }
-  HIPCHK(hipMemcpyAsync(per_list, B->lpend, nl * sizeof(int), hipMemcpyDeviceToHost, S()));
-  HIPCHK(hipStreamSynchronize(S()));
+  for (const BatchChunk &c : B->fresh)
+    if (c.list < nl) per_list[c.list] += c.count;
   return 0;
 }
 

@@ -127,34 +127,37 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_lighting(LightParams L, int mode,
 // read the parents' unmodified colours, so this runs before k_parents_light.
 __global__ __launch_bounds__(GXY_BLOCK) void k_children(LightParams L, int mode, RayList_ispc R, const int *pidx,
                                                         int Th, const int *Td, RayList_ispc O) {
-  const long long t = (long long)blockIdx.x * GXY_BLOCK + threadIdx.x;
   const int T = Td ? *Td : Th;
   const int nAO = (mode & LM_AO) ? L.nAO : 0, nL = (mode & LM_SHADOW) ? L.nLights : 0;
-  const long long nao = (long long)nAO * T;
-  if (t < nao) {
-    const int p = (int)(t / nAO), j = (int)(t - (long long)p * nAO);
-    const ParentRay P = parent_load(R, pidx[p]);
-    light_ao_child(L, P, j, O, (int)t);
-  } else if (t < nao + (long long)nL * T) {
-    const long long t2 = t - nao;
-    const int p = (int)(t2 / nL), k = (int)(t2 - (long long)p * nL);
-    const ParentRay P = parent_load(R, pidx[p]);
-    const float o = (mode & LM_DIFFUSE) ? light_diffused_alpha(L, P.o) : P.o;
-    light_shadow_child(L, P, o, k, O, (int)t);
+  const long long nao = (long long)nAO * T, nc = nao + (long long)nL * T;
+  // grid-stride: with T on the device the grid is bounded, not sized for every ray being a parent
+  const long long stride = (long long)gridDim.x * GXY_BLOCK;
+  for (long long t = (long long)blockIdx.x * GXY_BLOCK + threadIdx.x; t < nc; t += stride) {
+    if (t < nao) {
+      const int p = (int)(t / nAO), j = (int)(t - (long long)p * nAO);
+      const ParentRay P = parent_load(R, pidx[p]);
+      light_ao_child(L, P, j, O, (int)t);
+    } else {
+      const long long t2 = t - nao;
+      const int p = (int)(t2 / nL), k = (int)(t2 - (long long)p * nL);
+      const ParentRay P = parent_load(R, pidx[p]);
+      const float o = (mode & LM_DIFFUSE) ? light_diffused_alpha(L, P.o) : P.o;
+      light_shadow_child(L, P, o, k, O, (int)t);
+    }
   }
 }
 
 // ambientLighting and diffuseLighting of the T parents (after k_children)
 __global__ __launch_bounds__(GXY_BLOCK) void k_parents_light(LightParams L, int mode, RayList_ispc R, const int *pidx,
                                                              int Th, const int *Td) {
-  const int p = blockIdx.x * GXY_BLOCK + threadIdx.x;
   const int T = Td ? *Td : Th;
-  if (p >= T) return;
-  const int i = pidx[p];
-  ParentRay P = parent_load(R, i);
-  if (mode & LM_AMBIENT) light_ambient(L, P);
-  if (mode & LM_DIFFUSE) light_diffuse(L, P);
-  R.r[i] = P.r; R.g[i] = P.g; R.b[i] = P.b; R.o[i] = P.o;
+  for (int p = blockIdx.x * GXY_BLOCK + threadIdx.x; p < T; p += gridDim.x * GXY_BLOCK) {
+    const int i = pidx[p];
+    ParentRay P = parent_load(R, i);
+    if (mode & LM_AMBIENT) light_ambient(L, P);
+    if (mode & LM_DIFFUSE) light_diffuse(L, P);
+    R.r[i] = P.r; R.g[i] = P.g; R.b[i] = P.b; R.o[i] = P.o;
+  }
 }
 
 hipError_t launch_lighting_children(const LightParams &L, int mode, const RayList_ispc &R, const int *pidx, int T,
@@ -173,13 +176,15 @@ hipError_t launch_lighting_children(const LightParams &L, int mode, const RayLis
 hipError_t launch_lighting_children_dev(const LightParams &L, int mode, const RayList_ispc &R, const int *pidx,
                                         const int *T, int n, const RayList_ispc &O, hipStream_t s) {
   if (n <= 0) return hipSuccess;
+  // grids for the upper bound (n parents), at most 4096 workgroups (16 per CU)
+  const long long cap = 4096;
   const long long nc = (long long)n * (((mode & LM_AO) ? L.nAO : 0) + ((mode & LM_SHADOW) ? L.nLights : 0));
   if (nc > 0)
-    hipLaunchKernelGGL(k_children, dim3((unsigned)((nc + GXY_BLOCK - 1) / GXY_BLOCK)), dim3(GXY_BLOCK), 0, s, L,
-                       mode, R, pidx, 0, T, O);
+    hipLaunchKernelGGL(k_children, dim3((unsigned)std::min(cap, (nc + GXY_BLOCK - 1) / GXY_BLOCK)), dim3(GXY_BLOCK),
+                       0, s, L, mode, R, pidx, 0, T, O);
   if (mode & (LM_AMBIENT | LM_DIFFUSE))
-    hipLaunchKernelGGL(k_parents_light, dim3((n + GXY_BLOCK - 1) / GXY_BLOCK), dim3(GXY_BLOCK), 0, s, L, mode, R,
-                       pidx, 0, T);
+    hipLaunchKernelGGL(k_parents_light, dim3((unsigned)std::min(cap, (long long)(n + GXY_BLOCK - 1) / GXY_BLOCK)),
+                       dim3(GXY_BLOCK), 0, s, L, mode, R, pidx, 0, T);
   return hipGetLastError();
 }
 

@@ -44,6 +44,11 @@ class Server:
         glib.check(self.L.gxy_batch_set_config(b, chunk, stack, wpe, blocks_per_cu), "batch_set_config")
         glib.check(self.L.gxy_batch_set_split(b, split), "batch_set_split")
 
+    def set_step_rays(self, rays):
+        """Primary rays of admitted requests per continuous-batching step
+        (0 = all of them): the requests with the fewest rays left go first."""
+        glib.check(self.L.gxy_batch_set_step_rays(self.L.gxy_server_batch(self.h), int(rays)), "batch_set_step_rays")
+
     def run(self, reqs, mode=abi.SERVE_CONTINUOUS):
         """reqs: a ctypes array of abi.Request sorted by arrival.  Returns
         (results array, ServeStats)."""
@@ -119,11 +124,12 @@ def latency_summary(reqs, res):
                 mean_ms=round(float(lat.mean()) * 1e3, 3), max_ms=round(float(lat.max()) * 1e3, 3))
 
 
-def build_server(hip, descs, budget, slots=64, chunk=256, lighting=None, layout=0):
+def build_server(hip, descs, budget, slots=64, chunk=256, lighting=None, layout=0, step_rays=0):
     """Datasets from scene descriptions (volumes in the given voxel layout);
     returns (Server, Built list)."""
     from . import workloads
     S = Server(budget, slots, chunk)
+    S.set_step_rays(step_rays)
     built = [scenes.build_scene(hip, sc) for sc in descs]
     for b, sc in zip(built, descs):
         if layout and sc.volume_vis:

@@ -427,10 +427,19 @@ int   gxy_batch_get_split(void *batch);
  * afterwards run one by one. */
 int   gxy_batch_run(void *batch, int graph_passes, gxy_batch_stats *stats);
 /* Continuous form: one pass over the rays now queued, including the primary
- * rays of lists added since the last pass.  *next_rays receives the number
- * of rays queued for the following pass (synchronises). */
+ * rays of lists added since the last pass (up to the step budget, below).
+ * *next_rays receives the number of rays queued for the following pass,
+ * primary rays still waiting for a step included (synchronises). */
 int   gxy_batch_step(void *batch, long long *next_rays);
-/* per_list[l] = rays of list l queued for the next pass (l < n). */
+/* Step budget: at most `rays` primary rays of added lists join one step
+ * (0 = all of them, the default).  The lists with the fewest primary rays
+ * left go first; a list's remaining rays join later steps, so a step stays
+ * short and a list added meanwhile starts at the next step.  Results do not
+ * depend on the budget. */
+int   gxy_batch_set_step_rays(void *batch, long long rays);
+long long gxy_batch_get_step_rays(void *batch);
+/* per_list[l] = rays of list l queued for the next pass, with its primary
+ * rays still waiting for a step (l < n). */
 int   gxy_batch_pending(void *batch, int *per_list, int n);
 int   gxy_batch_get_stats(void *batch, gxy_batch_stats *stats);
 /* Device bytes held by the batch's ray pools and tables. */

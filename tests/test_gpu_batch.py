@@ -186,9 +186,11 @@ def test_sample_lists_equal_sampler_loop(datasets, hip):
         assert np.array_equal(a, b)
 
 
-def test_continuous_batching_alone_equals_mix(datasets, hip):
+@pytest.mark.parametrize("step_rays", [0, 1500])
+def test_continuous_batching_alone_equals_mix(datasets, hip, step_rays):
     """Lists added while others are in flight, with passes run one at a time,
-    get the results they get in a batch of their own."""
+    get the results they get in a batch of their own; with a step budget the
+    primary rays of the lists join over several steps, fewest left first."""
     from galaxy_amd import batch as gb
     descs, built = datasets
     keep = []
@@ -201,10 +203,13 @@ def test_continuous_batching_alone_equals_mix(datasets, hip):
         alone.append(ls[0][1].to_host(f32))
         B1.destroy()
     B = gb.Batch(256)
+    B.set_step_rays(step_rays)
+    assert B.L.gxy_batch_get_step_rays(B.h) == step_rays
     ids = [B.add_dataset(b) for b in built]
     B.set_lighting(lights)
     fbs = {}
     pending_plan = list(enumerate(plan))
+    queued = []
     steps = 0
     done = set()
     started = {}
@@ -217,9 +222,10 @@ def test_continuous_batching_alone_equals_mix(datasets, hip):
                 l, fb = B.add_render_list(ids[ds], d, n, 1)
                 fbs[k] = fb
                 started[k] = l
-        B.step()
+        queued.append(B.step())
         steps += 1
         pend = B.pending(len(B.lists) + 8)
+        assert queued[-1] == pend.sum()
         for k, l in list(started.items()):
             if pend[l] == 0:
                 done.add(k)
@@ -228,6 +234,9 @@ def test_continuous_batching_alone_equals_mix(datasets, hip):
         assert steps < 100
     for k in range(len(plan)):
         assert np.abs(fbs[k].to_host(f32) - alone[k]).max() <= 1e-5, k
+    if step_rays:
+        # 16169 primary rays, at most 1500 + 255 (a whole last chunk) per step
+        assert steps >= 9
 
 
 def test_bricked_volume_datasets_give_the_same_results(datasets, hip):

@@ -68,6 +68,26 @@ def test_stream_continuous_equals_one_at_a_time(server):
     assert lat["p99_ms"] >= lat["p50_ms"] > 0
 
 
+def test_step_budget_gives_the_same_outputs(server):
+    """Short steps (a few thousand primary rays, requests with the fewest rays
+    left first) change when requests finish, not what they return."""
+    reqs, outs = stream(seed=11, n=12)
+    res, st = server.run(reqs, abi.SERVE_CONTINUOUS)
+    a = outputs(reqs, res, outs)
+    for o in outs:
+        o[:] = 0
+    try:
+        server.set_step_rays(3000)
+        res1, st1 = server.run(reqs, abi.SERVE_CONTINUOUS)
+    finally:
+        server.set_step_rays(0)
+    same(a, outputs(reqs, res1, outs))
+    assert st.rays == st1.rays and st.samples == st1.samples
+    assert st1.passes > st.passes
+    for k in range(len(reqs)):
+        assert res1[k].status == 0
+
+
 def test_each_request_alone_equals_the_mix(server):
     reqs, outs = stream(seed=5, n=10)
     res, _ = server.run(reqs, abi.SERVE_CONTINUOUS)
