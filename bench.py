@@ -83,7 +83,8 @@ def run_frames(L, hip, built, cam, steps, warmup, eps=0.001):
         frame()
     glib.check(L.gxy_synchronize())
     L.gxy_timing_reset()
-    L.gxy_timing_enable(1)
+    # the timed frames record HIP events around the trace launches only (mode 2)
+    L.gxy_timing_enable(2)
     totals = dict(ray_passes=0, samples=0, primary=0, secondary=0, trace_launches=0)
     glib.check(L.gxy_synchronize())
     t0 = time.perf_counter()
@@ -96,10 +97,20 @@ def run_frames(L, hip, built, cam, steps, warmup, eps=0.001):
     glib.check(L.gxy_synchronize())
     t1 = time.perf_counter()
     L.gxy_timing_enable(0)
-    kt = {k: kernel_time(L, k) for k in ("trace", "lighting", "route", "partition", "spawn_scan", "camera")}
+    kt = {"trace": kernel_time(L, "trace")}
+    # per-kernel breakdown from a few more frames with every launch timed
+    nb_frames = min(steps, 5)
+    L.gxy_timing_reset()
+    L.gxy_timing_enable(1)
+    for _ in range(nb_frames):
+        frame()
+    glib.check(L.gxy_synchronize())
+    L.gxy_timing_enable(0)
+    breakdown = {k: round(kernel_time(L, k)[0] / nb_frames, 4)
+                 for k in ("trace", "lighting", "route", "partition", "spawn_scan", "camera")}
     image = fb.to_host(np.float32)
     fb.free()
-    return (t1 - t0), totals, kt, st, image
+    return (t1 - t0), totals, kt, st, image, breakdown
 
 
 def cpu_baseline(hip, sc, cam, alpha, target_s=15.0):
@@ -429,7 +440,7 @@ def main():
         tuned = dict(best=list(best), volume_layout=lay,
                      trace_ms=[[l_, list(c), round(t, 4)] for l_, c, t in table])
         log("autotune", tuned)
-    dt, tot, kt, st, image = run_frames(L, hip, built, cam, args.steps, args.warmup, sc.epsilon)
+    dt, tot, kt, st, image, breakdown = run_frames(L, hip, built, cam, args.steps, args.warmup, sc.epsilon)
     ms_step = dt / args.steps * 1e3
     mrays = tot["ray_passes"] / dt / 1e6
     gsamp = tot["samples"] / dt / 1e9
@@ -479,7 +490,7 @@ def main():
                    "shadow_per_frame": tot["secondary"] // args.steps},
         "gsamples_per_s": round(gsamp, 3),
         "roofline": roof,
-        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt.items()},
+        "kernel_ms_per_step": breakdown,
         "trace_config": tuned,
     }
     built = None
@@ -503,14 +514,14 @@ def main():
                     fb2 = glib.DeviceArray(sc2.cameras[0].width * sc2.cameras[0].height * 16)
                     autotune.tune_geometry_stack(L, frame_fn(L, b2, sc2.cameras[0], fb2, sc2.epsilon))
                     fb2.free()
-                dt2, tot2, kt2, st2, _ = run_frames(L, hip, b2, sc2.cameras[0], steps, 1, sc2.epsilon)
+                dt2, tot2, kt2, st2, _, bd2 = run_frames(L, hip, b2, sc2.cameras[0], steps, 1, sc2.epsilon)
                 extra[name] = dict(mrays_per_s=round(tot2["ray_passes"] / dt2 / 1e6, 2),
                                    ms_per_frame=round(dt2 / steps * 1e3, 3),
                                    rays_per_frame=tot2["ray_passes"] // steps,
                                    primary=tot2["primary"] // steps, secondary=tot2["secondary"] // steps,
                                    scene_build_s=round(build_s, 3),
                                    trace_ms_per_frame=round(kt2["trace"][0] / steps, 3),
-                                   kernel_ms_per_frame={k: round(v[0] / steps, 3) for k, v in kt2.items()},
+                                   kernel_ms_per_frame=bd2,
                                    geometry_stack=L.gxy_get_geometry_stack(), bvh_width=L.gxy_get_bvh_width())
                 b2 = None
             except Exception as e:  # reported, not hidden

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <array>
 #include <atomic>
 #include <chrono>
@@ -76,8 +77,9 @@ struct Runtime {
   std::map<void *, size_t> live;  // live block -> size class
   size_t bytes_live = 0;
   // timing
-  bool timing = false;
+  int timing = 0;                 // 0 off, 1 every kernel, 2 trace kernels only
   std::vector<PendingEv> pending;
+  std::vector<hipEvent_t> free_events;
   std::map<std::string, Timing> timings;
   // registries for handle validation
   std::set<void *> volumes, tfs, models, raylists, tracers, lightings, contexts;
@@ -171,13 +173,31 @@ void dfree(void *p) {
 }
 
 // ------------------------------------------------------------------ timing
+static hipEvent_t timing_event() {
+  if (!G.free_events.empty()) {
+    hipEvent_t e = G.free_events.back();
+    G.free_events.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
 struct KTimer {
   const char *name;
   hipEvent_t a = nullptr, b = nullptr;
   explicit KTimer(const char *n) : name(n) {
-    if (G.timing) {
-      hipEventCreate(&a);
-      hipEventCreate(&b);
+    if (G.timing == 1 || (G.timing == 2 && strcmp(n, "trace") == 0)) {
+      std::lock_guard<std::recursive_mutex> lk(G.mu);
+      a = timing_event();
+      b = timing_event();
+      if (!a || !b) {
+        if (a) G.free_events.push_back(a);
+        if (b) G.free_events.push_back(b);
+        a = b = nullptr;
+        return;
+      }
       hipEventRecord(a, S());
     }
   }
@@ -198,8 +218,8 @@ static void resolve_timings() {
     hipEventElapsedTime(&ms, p.a, p.b);
     G.timings[p.name].ms += ms;
     G.timings[p.name].launches++;
-    hipEventDestroy(p.a);
-    hipEventDestroy(p.b);
+    G.free_events.push_back(p.a);
+    G.free_events.push_back(p.b);
   }
   G.pending.clear();
 }
@@ -614,7 +634,8 @@ GXY_EXPORT size_t gxy_device_bytes_total(void) {
 
 GXY_EXPORT int gxy_timing_enable(int on) {
   INIT_OR(-1);
-  G.timing = on != 0;
+  if (on < 0 || on > 2) return fail("timing mode %d: 0, 1 or 2", on);
+  G.timing = on;
   return 0;
 }
 GXY_EXPORT int gxy_timing_reset(void) {
@@ -1870,6 +1891,8 @@ struct LoopCtx {
   float *xyz = nullptr;
   int capacity = 0;
   int *dcount = nullptr;
+  unsigned long long samples_read = 0;   // the sample counter as of the last host wait
+  bool have_samples = false;
 };
 
 // RayQManager + ThreadPool order (RayQManager.cpp:138-194, Threading.cpp:176-233):
@@ -1986,7 +2009,10 @@ static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
     HIPCHK(hipMemcpyAsync(counts, dcounts, sizeof counts, hipMemcpyDeviceToHost, S()));
     int T = 0;
     if (dT) HIPCHK(hipMemcpyAsync(&T, dT, sizeof T, hipMemcpyDeviceToHost, S()));
+    // the sample counter rides along: after the loop's last list no trace follows
+    HIPCHK(hipMemcpyAsync(&C.samples_read, CX().counters, sizeof C.samples_read, hipMemcpyDeviceToHost, S()));
     HIPCHK(hipStreamSynchronize(S()));
+    C.have_samples = true;
     if (out.p && T > 0) {
       // the children of the T parents (TraceRays.cpp:89-117 layout); the
       // arrays stay laid out for the capacity (hdr.aligned_size)
@@ -2059,10 +2085,11 @@ static int run_loop(LoopCtx &C, int n, const RayList_ispc &rays, RayListObj *ado
       return -1;
     }
   }
-  unsigned long long cnt[8];
-  HIPCHK(hipMemcpyAsync(cnt, CX().counters, sizeof cnt, hipMemcpyDeviceToHost, S()));
-  HIPCHK(hipStreamSynchronize(S()));
-  C.st.samples = (long long)cnt[0];
+  if (!C.have_samples) {
+    HIPCHK(hipMemcpyAsync(&C.samples_read, CX().counters, sizeof C.samples_read, hipMemcpyDeviceToHost, S()));
+    HIPCHK(hipStreamSynchronize(S()));
+  }
+  C.st.samples = (long long)C.samples_read;
   return 0;
 }
 
@@ -2196,7 +2223,10 @@ struct BatchObj {
   std::vector<unsigned> seed;
   std::vector<std::array<float, 6>> seed_box;
   std::vector<char> has_seed;
-  std::vector<BatchChunk> fresh;     // primary chunks of lists added since the last pass
+  struct Waiting {
+    int list, start, end;            // primary rays [start, end) of the list not yet in a step
+  };
+  std::vector<Waiting> fresh;        // in the order the lists were added
   // device state
   TraceParams *d_ds = nullptr;
   int d_ds_cap = 0;
@@ -2303,9 +2333,9 @@ static int batch_pool(BatchObj *B, int k, long long n) {
   return 0;
 }
 
-static void batch_chunks_of(BatchObj *B, int l, std::vector<BatchChunk> &out, int from = 0) {
+static void batch_chunks_of(BatchObj *B, int l, std::vector<BatchChunk> &out) {
   const int n = B->lists[l].n;
-  for (int s = from; s < n; s += B->chunk) {
+  for (int s = 0; s < n; s += B->chunk) {
     BatchChunk c;
     c.src = 2 + l;
     c.start = s;
@@ -2661,7 +2691,7 @@ GXY_EXPORT int gxy_batch_add_list(void *b, int ds, int mode, int n, const RayLis
   }
   if (mode == BM_RENDER) HIPCHK(launch_fill_f32(fb, 4LL * width * height, 0.f, S()));
   else HIPCHK(launch_set_count(count, 0, S()));
-  batch_chunks_of(B, l, B->fresh);
+  if (n > 0) B->fresh.push_back({l, 0, n});
   B->dirty_tables = B->dirty_graph = true;
   return l;
 }
@@ -2686,8 +2716,8 @@ GXY_EXPORT int gxy_batch_remove_list(void *b, int l) {
   BatchObj *B = lookup<BatchObj>(g_batches, b);
   if (!B) return fail("not a batch handle");
   if (l < 0 || l >= (int)B->lists.size() || !B->live[l]) return fail("no list %d", l);
-  for (const BatchChunk &c : B->fresh)
-    if (c.list == l) return fail("list %d has not been traced yet", l);
+  for (const BatchObj::Waiting &w : B->fresh)
+    if (w.list == l) return fail("list %d has not been traced yet", l);
   B->live[l] = 0;
   B->lists[l].n = 0;
   B->dirty_tables = B->dirty_graph = true;
@@ -2702,16 +2732,7 @@ GXY_EXPORT int gxy_batch_set_config(void *b, int chunk, int stack, int wpe, int 
   if (stack != 0 && stack != 16 && stack != 24 && stack != 32) return fail("stack %d: 0, 16, 24 or 32", stack);
   if (wpe != 0 && wpe != 4 && wpe != 5) return fail("wpe %d: 0, 4 or 5", wpe);
   if (bpc < 1 || bpc > 16) return fail("blocks per CU %d", bpc);
-  if (chunk != B->chunk) {
-    if (B->queued > 0) return fail("chunk size changes only with no rays queued");
-    // re-cut the waiting primary rays of each list from its first waiting ray
-    std::vector<std::pair<int, int>> ls;
-    for (const BatchChunk &c : B->fresh)
-      if (ls.empty() || ls.back().first != c.list) ls.push_back({c.list, c.start});
-    B->fresh.clear();
-    B->chunk = chunk;
-    for (auto &lf : ls) batch_chunks_of(B, lf.first, B->fresh, lf.second);
-  }
+  if (chunk != B->chunk && B->queued > 0) return fail("chunk size changes only with no rays queued");
   B->chunk = chunk;
   B->stack = stack;
   B->wpe = wpe;
@@ -2852,33 +2873,34 @@ GXY_EXPORT int gxy_batch_step(void *b, long long *next_rays) {
   // the lists that have the fewest waiting rays first (each list's chunks stay
   // in order, so its remaining rays are a suffix)
   std::vector<BatchChunk> take;
-  if (B->step_rays <= 0) {
-    take.swap(B->fresh);
-  } else {
-    std::map<int, long long> left;
-    for (const BatchChunk &c : B->fresh) left[c.list] += c.count;
-    std::vector<std::pair<long long, int>> order;
-    for (auto &kv : left) order.push_back({kv.second, kv.first});
-    std::stable_sort(order.begin(), order.end());
-    std::map<int, long long> quota;
-    long long budget = B->step_rays;
-    for (auto &o : order) {
+  {
+    // waiting ranges by rays left, fewest first (stable: ties in the order added)
+    std::vector<int> order(B->fresh.size());
+    for (size_t k = 0; k < order.size(); k++) order[k] = (int)k;
+    if (B->step_rays > 0)
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return B->fresh[a].end - B->fresh[a].start < B->fresh[b].end - B->fresh[b].start;
+      });
+    long long budget = B->step_rays > 0 ? B->step_rays : LLONG_MAX;
+    for (int k : order) {
       if (budget <= 0) break;
-      const long long q = std::min(o.first, budget);
-      quota[o.second] = q;
-      budget -= q;
-    }
-    std::vector<BatchChunk> rest;
-    for (const BatchChunk &c : B->fresh) {
-      auto it = quota.find(c.list);
-      if (it != quota.end() && it->second > 0) {
+      BatchObj::Waiting &w = B->fresh[k];
+      // whole chunks: the last list taken may exceed the budget by less than a chunk
+      while (w.start < w.end && budget > 0) {
+        BatchChunk c;
+        c.src = 2 + w.list;
+        c.start = w.start;
+        c.count = std::min(B->chunk, w.end - w.start);
+        c.list = w.list;
         take.push_back(c);
-        it->second -= c.count;   // a whole chunk: the budget rounds up to the chunk
-      } else {
-        rest.push_back(c);
+        w.start += c.count;
+        budget -= c.count;
       }
     }
-    B->fresh.swap(rest);
+    size_t o = 0;
+    for (size_t k = 0; k < B->fresh.size(); k++)
+      if (B->fresh[k].start < B->fresh[k].end) B->fresh[o++] = B->fresh[k];
+    B->fresh.resize(o);
   }
   long long in_rays = B->queued;
   for (const BatchChunk &c : take) in_rays += c.count;
@@ -2909,7 +2931,7 @@ GXY_EXPORT int gxy_batch_step(void *b, long long *next_rays) {
   B->passes_run++;
   B->queued = rays;
   long long waiting = 0;
-  for (const BatchChunk &c : B->fresh) waiting += c.count;
+  for (const BatchObj::Waiting &w : B->fresh) waiting += w.end - w.start;
   if (next_rays) *next_rays = rays + waiting;
   return 0;
 }
@@ -2926,8 +2948,8 @@ GXY_EXPORT int gxy_batch_pending(void *b, int *per_list, int n) {
     HIPCHK(hipMemcpyAsync(per_list, B->lpend, nl * sizeof(int), hipMemcpyDeviceToHost, S()));
     HIPCHK(hipStreamSynchronize(S()));
   }
-  for (const BatchChunk &c : B->fresh)
-    if (c.list < nl) per_list[c.list] += c.count;
+  for (const BatchObj::Waiting &w : B->fresh)
+    if (w.list < nl) per_list[w.list] += w.end - w.start;
   return 0;
 }
 

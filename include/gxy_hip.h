@@ -195,7 +195,8 @@ int         gxy_get_bvh_width(void);
 int         gxy_get_trace_config(int *kind, int *chunk, int *refill);
 size_t      gxy_device_bytes_total(void);
 
-/* Kernel timing: every launch records HIP events on its stream when enabled.
+/* Kernel timing: launches record HIP events on their stream when enabled
+ * (on = 1: every kernel; 2: the trace kernels only, "trace"; 0: off).
  * gxy_timing_get returns total ms and launch count for the named kernel. */
 int gxy_timing_enable(int on);
 int gxy_timing_reset(void);
