@@ -449,12 +449,13 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
 #pragma unroll
         for (int v = 0; v < NV; v++) sThis[v] = vol_finish(F[v]);
         nsamp += NV;
+        // the next position when neither an iso hit nor opacity ends the march
+        // here (the only cases in which the loop continues, and then
+        // tTermination is still the value used here)
+        const float nx0 = (tThis == tEntry) ? (tEntry + epsilon)
+                          : (((tThis + step) > tTermination) && (tThis < tTermination)) ? tTermination
+                                                                                        : tThis + step;
         {
-          // the next position when neither an iso hit nor opacity ends the
-          // march here (the only cases in which the loop continues)
-          const float nx0 = (tThis == tEntry) ? (tEntry + epsilon)
-                            : (((tThis + step) > tTermination) && (tThis < tTermination)) ? tTermination
-                                                                                          : tThis + step;
           f3 c1 = add3(org, scl3(nx0, dir));
 #pragma unroll
           for (int v = 0; v < NV; v++) F[v] = vol_fetch<SM>(P.vv[v].vol, c1);
@@ -499,10 +500,17 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
             if (P.vv[major].dvr) {
               const DTF &T = P.vv[major].tf;
               float sVolume = (sLast[major] + sThis[major]) / 2;
-              float sop = tf_opacity(s_op[major], T.no, T.lo, T.hi, T.so, sVolume);
+              f3 sc = mk3(0.f, 0.f, 0.f);
+              float sop;
+              if (T.nc == T.no && T.sc == T.so) {
+                // colour and opacity maps on the same control points: one index
+                sop = tf_opacity_color(s_op[major], s_col[major], T.no, T.lo, T.hi, T.so, sVolume, shadeFlag, sc);
+              } else {
+                sop = tf_opacity(s_op[major], T.no, T.lo, T.hi, T.so, sVolume);
+                if (sop > 0 && shadeFlag) sc = tf_color(s_col[major], T, sVolume);
+              }
               if (sop > 0) {
                 if (shadeFlag) {
-                  f3 sc = tf_color(s_col[major], T, sVolume);
                   float wo = clamp01f(sop * P.vv[major].vol.irate);
                   float a = 1.0f - ca;
                   cr = cr + a * (wo * sc.x);
@@ -523,11 +531,9 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
         opaque = opaque_of(cr, cg, cb, ca, P.opaque_alpha);
         if (opaque) tTermination = tThis;
         tLast = tThis;
-        float nxt = (tThis == tEntry) ? (tEntry + epsilon)
-                    : (((tThis + step) > tTermination) && (tThis < tTermination)) ? tTermination : tThis + step;
         // a step that no longer advances t ends the march (the reference would spin)
-        if (!(nxt > tThis) && !opaque && !hit_iso) break;
-        tThis = nxt;
+        if (!(nx0 > tThis) && !opaque && !hit_iso) break;
+        tThis = nx0;
       }
       ray_t = tTermination;
       if (hit_iso) {
@@ -601,9 +607,10 @@ __device__ __forceinline__ unsigned sample_one(const TraceParams &P, const RayLi
     float sCur[NV];
 #pragma unroll
     for (int k = 0; k < NV; k++) sCur[k] = vol_finish(F[k]);
+    // the next position when no crossing ends the march here
+    float nt0 = tThis + step;
+    if (nt0 > tExit) nt0 = tExit;
     {
-      float nt0 = tThis + step;
-      if (nt0 > tExit) nt0 = tExit;
       f3 c2 = add3(org, scl3(nt0, dir));
 #pragma unroll
       for (int k = 0; k < NV; k++) F[k] = vol_fetch<SM>(P.vv[k].vol, c2);
@@ -626,10 +633,8 @@ __device__ __forceinline__ unsigned sample_one(const TraceParams &P, const RayLi
       }
     }
     if (hit != -1 || tThis == tExit) break;
-    float nt = tThis + step;
-    if (nt > tExit) nt = tExit;
-    if (!(nt > tThis)) break;
-    tThis = nt;
+    if (!(nt0 > tThis)) break;
+    tThis = nt0;
   }
   gptr(R.t)[i] = hit != -1 ? tThis + 0.001f : tThis;
   gptr(R.term)[i] = hit != -1 ? RAY_SURFACE : RAY_BOUNDARY;
