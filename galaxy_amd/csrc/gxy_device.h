@@ -301,39 +301,6 @@ __device__ __forceinline__ float tf_opacity(const float *op, int n, float lo, fl
   int j = i + 1 < n - 1 ? i + 1 : n - 1;
   return (1.0f - r) * op[i] + r * op[j];
 }
-// tf_opacity of v and, when want and the opacity is positive, tf_color of v
-// into c, for colour and opacity maps with the same n, lo, hi and scale: the
-// control-point index and weight are computed once; both results equal the
-// separate lookups.
-__device__ __forceinline__ float tf_opacity_color(const float *op, const float *col, int n, float lo, float hi,
-                                                  float s, float v, bool want, f3 &c) {
-  if (v != v) return 0.f;
-  if (n == 0) {
-    c = mk3(1.f, 1.f, 1.f);
-    return 1.f;
-  }
-  float o;
-  if (v <= lo) {
-    o = op[0];
-    if (want && o > 0) c = mk3(col[0], col[1], col[2]);
-  } else if (v >= hi) {
-    o = op[n - 1];
-    if (want && o > 0) c = mk3(col[3 * n - 3], col[3 * n - 2], col[3 * n - 1]);
-  } else {
-    float x = (v - lo) * s;
-    int i = (int)x;
-    i = i < n - 1 ? i : n - 1;
-    float r = x - (float)i;
-    int j = i + 1 < n - 1 ? i + 1 : n - 1;
-    float q = 1.0f - r;
-    o = q * op[i] + r * op[j];
-    if (want && o > 0)
-      c = mk3(q * col[3 * i] + r * col[3 * j], q * col[3 * i + 1] + r * col[3 * j + 1],
-              q * col[3 * i + 2] + r * col[3 * j + 2]);
-  }
-  return o;
-}
-
 __device__ __forceinline__ f3 tf_color(const float *col, const DTF &T, float v) {
   return tf_color(col, T.nc, T.lo, T.hi, T.sc, v);
 }

@@ -500,17 +500,10 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
             if (P.vv[major].dvr) {
               const DTF &T = P.vv[major].tf;
               float sVolume = (sLast[major] + sThis[major]) / 2;
-              f3 sc = mk3(0.f, 0.f, 0.f);
-              float sop;
-              if (T.nc == T.no && T.sc == T.so) {
-                // colour and opacity maps on the same control points: one index
-                sop = tf_opacity_color(s_op[major], s_col[major], T.no, T.lo, T.hi, T.so, sVolume, shadeFlag, sc);
-              } else {
-                sop = tf_opacity(s_op[major], T.no, T.lo, T.hi, T.so, sVolume);
-                if (sop > 0 && shadeFlag) sc = tf_color(s_col[major], T, sVolume);
-              }
+              float sop = tf_opacity(s_op[major], T.no, T.lo, T.hi, T.so, sVolume);
               if (sop > 0) {
                 if (shadeFlag) {
+                  f3 sc = tf_color(s_col[major], T, sVolume);
                   float wo = clamp01f(sop * P.vv[major].vol.irate);
                   float a = 1.0f - ca;
                   cr = cr + a * (wo * sc.x);
