@@ -2882,11 +2882,10 @@ GXY_EXPORT int gxy_batch_step(void *b, long long *next_rays) {
         return B->fresh[a].end - B->fresh[a].start < B->fresh[b].end - B->fresh[b].start;
       });
     long long budget = B->step_rays > 0 ? B->step_rays : LLONG_MAX;
-    for (int k : order) {
-      if (budget <= 0) break;
-      BatchObj::Waiting &w = B->fresh[k];
-      // whole chunks: the last list taken may exceed the budget by less than a chunk
-      while (w.start < w.end && budget > 0) {
+    // whole chunks: a list's share may exceed its limit by less than a chunk
+    auto take_from = [&](BatchObj::Waiting &w, long long limit) {
+      long long got = 0;
+      while (w.start < w.end && budget > 0 && got < limit) {
         BatchChunk c;
         c.src = 2 + w.list;
         c.start = w.start;
@@ -2895,7 +2894,15 @@ GXY_EXPORT int gxy_batch_step(void *b, long long *next_rays) {
         take.push_back(c);
         w.start += c.count;
         budget -= c.count;
+        got += c.count;
       }
+    };
+    // the oldest waiting list keeps a quarter of every step, so that a stream
+    // of small lists cannot hold a large one back indefinitely
+    if (B->step_rays > 0 && !B->fresh.empty()) take_from(B->fresh[0], B->step_rays / 4);
+    for (int k : order) {
+      if (budget <= 0) break;
+      take_from(B->fresh[k], LLONG_MAX);
     }
     size_t o = 0;
     for (size_t k = 0; k < B->fresh.size(); k++)

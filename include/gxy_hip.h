@@ -433,10 +433,11 @@ int   gxy_batch_run(void *batch, int graph_passes, gxy_batch_stats *stats);
  * primary rays still waiting for a step included (synchronises). */
 int   gxy_batch_step(void *batch, long long *next_rays);
 /* Step budget: at most `rays` primary rays of added lists join one step
- * (0 = all of them, the default).  The lists with the fewest primary rays
- * left go first; a list's remaining rays join later steps, so a step stays
- * short and a list added meanwhile starts at the next step.  Results do not
- * depend on the budget. */
+ * (0 = all of them, the default; a list's last chunk may overshoot).  The
+ * oldest waiting list gets a quarter of the budget, then the lists with the
+ * fewest primary rays left go first; a list's remaining rays join later
+ * steps, so a step stays short and a list added meanwhile starts at the next
+ * step.  Results do not depend on the budget. */
 int   gxy_batch_set_step_rays(void *batch, long long rays);
 long long gxy_batch_get_step_rays(void *batch);
 /* per_list[l] = rays of list l queued for the next pass, with its primary
