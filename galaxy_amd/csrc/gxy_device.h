@@ -19,6 +19,8 @@
 #define GXY_MAX_VOLVIS 4
 #define GXY_MAX_SLICES 8
 #define GXY_MAX_ISO 8
+// round-cone pieces per path-line curve segment
+#define GXY_CURVE_PIECES 8
 #define GXY_TF_MAX 256
 #define GXY_BLOCK 256
 // upper bound on march steps per ray and pass (a 4096^3 diagonal at half-voxel
@@ -100,6 +102,15 @@ struct DModel {
   const int *sprim;
   int nsph;
   int sroot_leaf;
+  // path-line curve pieces (round cones), in BVH leaf order: sphere a and
+  // sphere b (x y z radius each) and the curve parameters (u0, u1) they sit at
+  const BVHNode *cnodes;
+  const float *ca;        // 4 per piece
+  const float *cb;        // 4 per piece
+  const float *cu;        // 2 per piece
+  const int *cgeom;
+  const int *cprim;       // curve segment (Embree primID) of each piece
+  int ncurve;
   // shading data, indexed by geomID
   int ngeom;
   const struct DGeomShade *shade;
@@ -107,20 +118,22 @@ struct DModel {
   // the same trees collapsed to 4-wide nodes (null when not built)
   const BVH4Node *tnodes4;
   const BVH4Node *snodes4;
+  const BVH4Node *cnodes4;
   int depth4;           // deepest leaf of either 4-wide tree
   int wide;             // traverse the 4-wide trees
 };
 
 // Per-geometry shading data (DataDrivenTriangleMesh_postIntersect,
-// DataDrivenSpheres_postIntersect).
+// DataDrivenSpheres_postIntersect, DataDrivenPathLines_postIntersect).
 struct DGeomShade {
-  int kind;               // 0 triangles, 1 spheres
+  int kind;               // 0 triangles, 1 spheres, 2 path lines
   int has_normals, has_data, has_tf;
-  const float *vertices;  // triangles: 3 per vertex
-  const uint32_t *indices;
+  const float *vertices;  // triangles: 3 per vertex; path lines: control points, 4 each (x y z radius)
+  const uint32_t *indices;  // path lines: first control point of each curve segment
   const float *normals;
   const float *vdata;     // per vertex (triangles) or per sphere
   const float *centers;
+  float r0, r1, v0, v1;   // path lines: radius and value range
   DTF tf;
 };
 

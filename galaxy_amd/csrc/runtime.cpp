@@ -309,9 +309,25 @@ struct SpheresObj {
   TFObj *tf = nullptr;
 };
 
+// Path lines as round Bezier curves (DataDrivenPathLines.cpp:46-160): the
+// control points (x y z radius) of every curve segment and, on the host, the
+// round-cone pieces each segment is traced as.
+struct PathObj {
+  int geomID;
+  int nseg = 0;
+  float *CP = nullptr;         // device: control points, 4 floats each
+  uint32_t *IC = nullptr;      // device: first control point of each segment
+  std::vector<float> pa, pb;   // pieces: sphere a and sphere b, 4 floats each
+  std::vector<float> pu;       // pieces: (u0, u1)
+  std::vector<int> pseg;       // pieces: curve segment
+  float r0, r1, v0, v1;
+  TFObj *tf = nullptr;
+};
+
 struct ModelObj {
   std::vector<MeshObj> meshes;
   std::vector<SpheresObj> spheres;
+  std::vector<PathObj> paths;
   int ngeom = 0;
   bool committed = false;
   DModel d;
@@ -319,6 +335,7 @@ struct ModelObj {
   DGeomShade *shade = nullptr;
   int *hit_geom = nullptr, *hit_prim = nullptr;
   int tnodes = 0, snodes = 0, tdepth = 0, sdepth = 0, tdepth4 = 0, sdepth4 = 0;
+  int cnodes = 0, cdepth = 0, cdepth4 = 0;
   void free_built() {
     for (void *p : owned) dfree(p);
     owned.clear();
@@ -1000,6 +1017,7 @@ GXY_EXPORT int gxy_model_destroy(void *model) {
   M->free_built();
   for (auto &m : M->meshes) { dfree(m.V); dfree(m.I); dfree(m.N); dfree(m.D); }
   for (auto &s : M->spheres) { dfree(s.C); dfree(s.D); }
+  for (auto &p : M->paths) { dfree(p.CP); dfree(p.IC); }
   delete M;
   return 0;
 }
@@ -1049,6 +1067,93 @@ GXY_EXPORT int gxy_model_add_spheres(void *model, const float *C, const float *D
   M->spheres.push_back(s);
   M->free_built();
   return s.geomID;
+}
+
+// MAP_RADIUS of DataDrivenPathLines.cpp:11-19
+static float path_radius(float d, float r0, float r1, float v0, float v1) {
+  if (v0 == v1) return r0;
+  const float R = (d - v0) / (v1 - v0);
+  return (R <= 0.0f) ? r0 : (R >= 1.0f) ? r1 : r0 + R * (r1 - r0);
+}
+
+GXY_EXPORT int gxy_model_add_pathlines(void *model, const float *V, const float *D, int nv, const int *I, int nseg,
+                                       float r0, float r1, float v0, float v1, void *tf) {
+  ModelObj *M = lookup<ModelObj>(G.models, model);
+  if (!M) return fail("not a model handle");
+  if (nv <= 0 || !V || !D || nseg <= 0 || !I) return fail("empty path-line set");
+  for (int i = 0; i < nseg; i++)
+    if (I[i] < 0 || I[i] + 1 >= nv) return fail("path-line segment %d starts at vertex %d of %d", i, I[i], nv);
+  PathObj P;
+  P.r0 = r0; P.r1 = r1; P.v0 = v0; P.v1 = v1;
+  P.tf = tf ? lookup<TFObj>(G.tfs, tf) : nullptr;
+  if (tf && !P.tf) return fail("path-line TF is not a TF handle");
+  // control points: a segment continuing the previous one starts with the
+  // previous segment's outgoing tangent, one followed by a continuation ends
+  // with a tangent through the next vertex split in the ratio of the two
+  // segment lengths; curve ends repeat their end point
+  // (DataDrivenPathLines.cpp:97-157)
+  std::vector<float> cp;
+  std::vector<uint32_t> ic;
+  auto v3 = [&](int i, int k) { return V[3 * (size_t)i + k]; };
+  auto push = [&](float x, float y, float z, float r) { cp.insert(cp.end(), {x, y, z, r}); };
+  bool middle = false;
+  float tan[3] = {0.f, 0.f, 0.f};
+  for (int i = 0; i < nseg; i++) {
+    const int idx = I[i];
+    const float st[3] = {v3(idx, 0), v3(idx, 1), v3(idx, 2)}, en[3] = {v3(idx + 1, 0), v3(idx + 1, 1), v3(idx + 1, 2)};
+    const float dx = st[0] - en[0], dy = st[1] - en[1], dz = st[2] - en[2];
+    const float lseg = std::sqrt(dx * dx + dy * dy + dz * dz);
+    const float rs = path_radius(D[idx], r0, r1, v0, v1), re = path_radius(D[idx + 1], r0, r1, v0, v1);
+    ic.push_back((uint32_t)(cp.size() / 4));
+    push(st[0], st[1], st[2], rs);
+    if (middle) push(st[0] + tan[0], st[1] + tan[1], st[2] + tan[2], (1.f - 1.f / 3) * rs + (1.f / 3) * re);
+    else push(st[0], st[1], st[2], rs);
+    middle = i + 1 < nseg && I[i + 1] == idx + 1;
+    if (middle) {
+      if (idx + 2 >= nv) return fail("path-line vertex %d out of range", idx + 2);
+      const float nx[3] = {v3(idx + 2, 0), v3(idx + 2, 1), v3(idx + 2, 2)};
+      float dl[3], bl2 = 0.f;
+      for (int k = 0; k < 3; k++) {
+        dl[k] = (1.f / 3) * (nx[k] - st[k]);
+        bl2 += (nx[k] - en[k]) * (nx[k] - en[k]);
+      }
+      const float bl = std::sqrt(bl2);
+      const float r = lseg / (lseg + bl);
+      push(en[0] - r * dl[0], en[1] - r * dl[1], en[2] - r * dl[2], (1.f - 2.f / 3) * rs + (2.f / 3) * re);
+      for (int k = 0; k < 3; k++) tan[k] = (1.f - r) * dl[k];
+    } else {
+      push(en[0], en[1], en[2], re);
+    }
+    push(en[0], en[1], en[2], re);
+  }
+  // pieces: the curve sampled at u = j / GXY_CURVE_PIECES (Bernstein weights)
+  for (int i = 0; i < nseg; i++) {
+    const float *c = &cp[4 * (size_t)ic[i]];
+    float prev[4] = {0, 0, 0, 0}, prevu = 0.f;
+    for (int j = 0; j <= GXY_CURVE_PIECES; j++) {
+      const float u = (float)j / (float)GXY_CURVE_PIECES, q = 1.0f - u;
+      const float w0 = q * q * q, w1 = 3.0f * q * q * u, w2 = 3.0f * q * u * u, w3 = u * u * u;
+      float x[4];
+      for (int k = 0; k < 4; k++) x[k] = w0 * c[k] + w1 * c[4 + k] + w2 * c[8 + k] + w3 * c[12 + k];
+      if (j > 0) {
+        P.pa.insert(P.pa.end(), prev, prev + 4);
+        P.pb.insert(P.pb.end(), x, x + 4);
+        P.pu.insert(P.pu.end(), {prevu, u});
+        P.pseg.push_back(i);
+      }
+      for (int k = 0; k < 4; k++) prev[k] = x[k];
+      prevu = u;
+    }
+  }
+  P.geomID = M->ngeom++;
+  P.nseg = nseg;
+  P.CP = upload(cp.data(), cp.size());
+  P.IC = upload(ic.data(), ic.size());
+  if (!P.CP || !P.IC) return -1;
+  HIPCHK(hipStreamSynchronize(S()));
+  M->paths.push_back(std::move(P));
+  M->free_built();
+  return M->paths.back().geomID;
 }
 
 namespace gxy {
@@ -1346,6 +1451,17 @@ GXY_EXPORT int gxy_model_commit(void *model) {
     if (m.tf) s.tf = m.tf->d;
     ntri += m.nt;
   }
+  int64_t ncur = 0;
+  for (auto &p : M->paths) {
+    DGeomShade &s = sh[p.geomID];
+    s.kind = 2;
+    s.vertices = p.CP;
+    s.indices = p.IC;
+    s.r0 = p.r0; s.r1 = p.r1; s.v0 = p.v0; s.v1 = p.v1;
+    s.has_tf = p.tf != nullptr;
+    if (p.tf) s.tf = p.tf->d;
+    ncur += (int64_t)p.pseg.size();
+  }
   for (auto &p : M->spheres) {
     DGeomShade &s = sh[p.geomID];
     s.kind = 1;
@@ -1356,7 +1472,7 @@ GXY_EXPORT int gxy_model_commit(void *model) {
     if (p.tf) s.tf = p.tf->d;
     nsph += p.n;
   }
-  if (ntri > (1 << 27) || nsph > (1 << 27)) return fail("more than 2^27 primitives of one kind");
+  if (ntri > (1 << 27) || nsph > (1 << 27) || ncur > (1 << 27)) return fail("more than 2^27 primitives of one kind");
   DGeomShade *dsh = upload(sh.data(), sh.size());
   M->owned.push_back(dsh);
   M->d.shade = dsh;
@@ -1449,17 +1565,77 @@ GXY_EXPORT int gxy_model_commit(void *model) {
     M->d.snodes4 = n4;
     M->sdepth4 = d4;
   }
-  M->d.depth = std::max(M->tdepth, M->sdepth);
-  M->d.depth4 = std::max(M->tdepth4, M->sdepth4);
+  if (ncur > 0) {
+    // round-cone pieces: boxes on the host, tree on the device, pieces
+    // gathered into leaf order on the host
+    std::vector<float> fa, fb, fu, boxes;
+    std::vector<int> fg, fp;
+    for (auto &p : M->paths)
+      for (size_t k = 0; k < p.pseg.size(); k++) {
+        const float *a = &p.pa[4 * k], *b = &p.pb[4 * k];
+        for (int c = 0; c < 3; c++) boxes.push_back(std::min(a[c] - a[3], b[c] - b[3]));
+        for (int c = 0; c < 3; c++) boxes.push_back(std::max(a[c] + a[3], b[c] + b[3]));
+        fa.insert(fa.end(), a, a + 4);
+        fb.insert(fb.end(), b, b + 4);
+        fu.insert(fu.end(), {p.pu[2 * k], p.pu[2 * k + 1]});
+        fg.push_back(p.geomID);
+        fp.push_back(p.pseg[k]);
+      }
+    float *dbox = upload(boxes.data(), boxes.size());
+    if (!dbox) return -1;
+    BVHBuildResult B;
+    HIPCHK(build_lbvh(dbox, (int)ncur, dalloc, dfree, &B, s));
+    if (!B.nodes) return fail("model commit: out of device memory");
+    std::vector<int> order(ncur);
+    HIPCHK(hipMemcpyAsync(order.data(), B.order, 4 * (size_t)ncur, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    dfree(dbox);
+    dfree(B.order);
+    std::vector<float> sa(4 * ncur), sb(4 * ncur), su(2 * ncur);
+    std::vector<int> sg(ncur), sp(ncur);
+    for (int64_t k = 0; k < ncur; k++) {
+      const int o = order[k];
+      if (o < 0 || o >= ncur) return fail("curve BVH order entry %d out of range", o);
+      for (int c = 0; c < 4; c++) { sa[4 * k + c] = fa[4 * (size_t)o + c]; sb[4 * k + c] = fb[4 * (size_t)o + c]; }
+      su[2 * k] = fu[2 * (size_t)o];
+      su[2 * k + 1] = fu[2 * (size_t)o + 1];
+      sg[k] = fg[o];
+      sp[k] = fp[o];
+    }
+    float *da = upload(sa.data(), sa.size()), *db = upload(sb.data(), sb.size()), *du = upload(su.data(), su.size());
+    int *dg = upload(sg.data(), sg.size()), *dp = upload(sp.data(), sp.size());
+    if (!da || !db || !du || !dg || !dp) return fail("model commit: out of device memory");
+    M->owned.insert(M->owned.end(), {(void *)B.nodes, (void *)da, (void *)db, (void *)du, (void *)dg, (void *)dp});
+    M->d.cnodes = B.nodes;
+    M->d.ca = da;
+    M->d.cb = db;
+    M->d.cu = du;
+    M->d.cgeom = dg;
+    M->d.cprim = dp;
+    M->d.ncurve = (int)ncur;
+    M->cnodes = B.nnodes;
+    M->cdepth = B.depth;
+    if (B.depth > 60) return fail("curve BVH depth %d exceeds the traversal stack", B.depth);
+    if (validate_bvh(B.nodes, B.nnodes, (int)ncur, "curve") != 0) return -1;
+    BVH4Node *n4 = nullptr;
+    int d4 = 0;
+    if (collapse_bvh4(B.nodes, B.nnodes, &n4, &d4) != 0) return -1;
+    M->owned.push_back(n4);
+    M->d.cnodes4 = n4;
+    M->cdepth4 = d4;
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  M->d.depth = std::max(std::max(M->tdepth, M->sdepth), M->cdepth);
+  M->d.depth4 = std::max(std::max(M->tdepth4, M->sdepth4), M->cdepth4);
   M->committed = true;
   return 0;
 }
 GXY_EXPORT int gxy_model_bvh_stats(void *model, int *nodes, int *leaves, int *depth, float *sah) {
   ModelObj *M = lookup<ModelObj>(G.models, model);
   if (!M) return fail("not a model handle");
-  if (nodes) *nodes = M->tnodes + M->snodes;
-  if (leaves) *leaves = M->d.ntris + M->d.nsph;
-  if (depth) *depth = std::max(M->tdepth, M->sdepth);
+  if (nodes) *nodes = M->tnodes + M->snodes + M->cnodes;
+  if (leaves) *leaves = M->d.ntris + M->d.nsph + M->d.ncurve;
+  if (depth) *depth = std::max(std::max(M->tdepth, M->sdepth), M->cdepth);
   if (sah) *sah = 0.f;
   return 0;
 }
@@ -2614,6 +2790,8 @@ static int batch_dataset_params(BatchObj *B, const Visualization_ispc *vis, floa
   if (!volume && !geometry)
     return fail("a batch dataset is one volume without geometry or geometry without volumes (got %d volumes%s)",
                 P.nvv, P.has_model ? " and a model" : "");
+  if (geometry && P.model.ncurve > 0)
+    return fail("a batch dataset with path lines: path lines are traced by the per-list loop only");
   P.hit_geom = nullptr;
   P.hit_prim = nullptr;
   P.sample_partials = nullptr;

@@ -102,6 +102,77 @@ __device__ __forceinline__ bool sphere_hit(f3 center, float radius, float eps, f
   return false;
 }
 
+// Path-line pieces.  A piece is the round cone of spheres (a, ra) and
+// (b, rb): their convex hull, which is the surface a sphere sweeps when its
+// centre and radius move linearly from one to the other.  Its boundary is the
+// cap of sphere a behind the tangent circle at axial coordinate ra*sin(phi),
+// the cone band up to the tangent circle on sphere b at l + rb*sin(phi), and
+// the cap of sphere b beyond it (l = |b - a|, sin(phi) = (ra - rb) / l).  On
+// the band, with y the axial coordinate from a and h the distance from the
+// axis, h*cos(phi) + y*sin(phi) = ra, which along the ray is one quadratic.
+
+// The two crossings of sphere (c, r), solved about the ray point nearest c.
+__device__ __forceinline__ bool sphere_roots(f3 c, float r, f3 org, f3 dir, float dd, float &t1, float &t2) {
+  const float tc = dot3(sub3(c, org), dir) / dd;
+  const f3 q = sub3(add3(org, scl3(tc, dir)), c);
+  const float disc = dd * (r * r - dot3(q, q));
+  if (disc < 0.f) return false;
+  const float h = sqrtf(disc) / dd;
+  t1 = tc - h;
+  t2 = tc + h;
+  return true;
+}
+
+// Entry and exit of the ray through the round cone (a, ra, b, rb): the least
+// and greatest boundary crossings.  t = the entry when it lies in (t0, tfar),
+// else the exit when that does (the rule of the reference's cylinder kernel,
+// DataDrivenPathLines.ispc:183-193).
+__device__ __forceinline__ bool roundcone_hit(f3 a, float ra, f3 b, float rb, f3 org, f3 dir, float t0, float tfar,
+                                              float &t) {
+  const float dd = dot3(dir, dir);
+  const f3 ba = sub3(b, a);
+  const float l2 = dot3(ba, ba), rd = ra - rb;
+  float tin = 3e38f, tout = -3e38f;
+  float t1, t2;
+  if (l2 <= rd * rd) {
+    // one sphere holds the other: the larger one
+    if (sphere_roots(ra >= rb ? a : b, ra >= rb ? ra : rb, org, dir, dd, t1, t2)) { tin = t1; tout = t2; }
+  } else {
+    const float l = sqrtf(l2);
+    const f3 w = scl3(1.0f / l, ba);
+    const float sn = rd / l, cs2 = 1.0f - sn * sn;
+    const float ya = ra * sn, yb = l + rb * sn;
+    if (sphere_roots(a, ra, org, dir, dd, t1, t2)) {
+      if (dot3(sub3(add3(org, scl3(t1, dir)), a), w) <= ya) { tin = mnf(tin, t1); tout = mxf(tout, t1); }
+      if (dot3(sub3(add3(org, scl3(t2, dir)), a), w) <= ya) { tin = mnf(tin, t2); tout = mxf(tout, t2); }
+    }
+    if (sphere_roots(b, rb, org, dir, dd, t1, t2)) {
+      if (dot3(sub3(add3(org, scl3(t1, dir)), a), w) >= yb) { tin = mnf(tin, t1); tout = mxf(tout, t1); }
+      if (dot3(sub3(add3(org, scl3(t2, dir)), a), w) >= yb) { tin = mnf(tin, t2); tout = mxf(tout, t2); }
+    }
+    // the band, solved about the ray point nearest a
+    const float tc = dot3(sub3(a, org), dir) / dd;
+    const f3 oa = sub3(add3(org, scl3(tc, dir)), a);
+    const float y0 = dot3(oa, w), yd = dot3(dir, w);
+    const float k = ra - sn * y0;
+    const float A = cs2 * (dd - yd * yd) - sn * sn * (yd * yd);
+    const float B = 2.0f * (cs2 * (dot3(oa, dir) - y0 * yd) + sn * yd * k);
+    const float Cc = cs2 * (dot3(oa, oa) - y0 * y0) - k * k;
+    const float disc = B * B - 4.0f * A * Cc;
+    if (A != 0.f && disc >= 0.f) {
+      const float sq = sqrtf(disc), i2a = 0.5f / A;
+      const float r1 = (-B - sq) * i2a, r2 = (-B + sq) * i2a;
+      const float y1 = y0 + r1 * yd, y2 = y0 + r2 * yd;
+      if (y1 > ya && y1 < yb && k - sn * (r1 * yd) >= 0.f) { tin = mnf(tin, tc + r1); tout = mxf(tout, tc + r1); }
+      if (y2 > ya && y2 < yb && k - sn * (r2 * yd) >= 0.f) { tin = mnf(tin, tc + r2); tout = mxf(tout, tc + r2); }
+    }
+  }
+  if (tin > tout) return false;
+  if (tin > t0 && tin < tfar) { t = tin; return true; }
+  if (tout > t0 && tout < tfar) { t = tout; return true; }
+  return false;
+}
+
 __device__ __forceinline__ bool better(float t, int g, int p, const GeoHit &h) {
   if (t < h.t) return true;
   if (t == h.t && (g < h.geom || (g == h.geom && p < h.prim))) return true;
@@ -291,6 +362,50 @@ __device__ __forceinline__ void model_trace(const DModel &M, f3 org, f3 dir, flo
     if (W == 4 || (W == 0 && M.wide)) bvh4_walk(M.snodes4, org, inv, t0, h, stack, sleaf);
     else bvh_walk(M.snodes, org, inv, t0, h, stack, sleaf);
   }
+  if (W == 0 && M.ncurve > 0) {
+    // path-line pieces, in the per-model variant only (models with path lines
+    // are dispatched to it); h.v carries the piece (its sorted index) to the
+    // shading
+    auto cleaf = [&](int start, int count) {
+      for (int k = start; k < start + count; k++) {
+        const float *A = gptr(M.ca) + 4 * (int64_t)k, *B = gptr(M.cb) + 4 * (int64_t)k;
+        float lim = h.geom < 0 ? h.t : nextafterf(h.t, 3e38f);
+        float t;
+        if (roundcone_hit(mk3(A[0], A[1], A[2]), A[3], mk3(B[0], B[1], B[2]), B[3], org, dir, t0, lim, t)) {
+          int g = gptr(M.cgeom)[k], pr = gptr(M.cprim)[k];
+          if (h.geom < 0 || better(t, g, pr, h)) {
+            h.t = t; h.geom = g; h.prim = pr; h.u = 0.f; h.v = __int_as_float(k);
+          }
+        }
+      }
+    };
+    if (M.wide) bvh4_walk(M.cnodes4, org, inv, t0, h, stack, cleaf);
+    else bvh_walk(M.cnodes, org, inv, t0, h, stack, cleaf);
+  }
+}
+
+// The surface normal of round-cone piece k at P (unnormalised) and the curve
+// parameter there: the piece's (u0, u1) at the projection of P on its axis.
+__device__ __forceinline__ f3 roundcone_normal(const DModel &M, int k, f3 P, float &u) {
+  const float *A = gptr(M.ca) + 4 * (int64_t)k, *B = gptr(M.cb) + 4 * (int64_t)k, *U = gptr(M.cu) + 2 * (int64_t)k;
+  const f3 a = mk3(A[0], A[1], A[2]), b = mk3(B[0], B[1], B[2]);
+  const float ra = A[3], rb = B[3];
+  const f3 ba = sub3(b, a), pa = sub3(P, a);
+  const float l2 = dot3(ba, ba), rd = ra - rb;
+  float s = l2 > 0.f ? dot3(pa, ba) / l2 : 0.f;
+  s = s < 0.f ? 0.f : (s > 1.f ? 1.f : s);
+  u = s >= 1.f ? U[1] : U[0] + s * (U[1] - U[0]);
+  if (l2 <= rd * rd) return ra >= rb ? pa : sub3(P, b);
+  const float l = sqrtf(l2);
+  const f3 w = scl3(1.0f / l, ba);
+  const float sn = rd / l;
+  const float y = dot3(pa, w);
+  if (y <= ra * sn) return pa;
+  if (y >= l + rb * sn) return sub3(P, b);
+  const f3 q = sub3(pa, scl3(y, w));
+  const float hq = sqrtf(dot3(q, q));
+  if (!(hq > 0.f)) return w;
+  return add3(scl3(sqrtf(1.0f - sn * sn) / hq, q), scl3(sn, w));
 }
 
 // postIntersect for the hit geometry, then DG_NORMALIZE | DG_FACEFORWARD.
@@ -320,12 +435,33 @@ __device__ void geom_shade(const DModel &M, const GeoHit &gh, f3 org, f3 dir, Hi
       float dv = b0 * vdat[i0] + b1 * vdat[i1] + b2 * vdat[i2];
       col = tf_color(gptr(S.tf.col), S.tf, dv);
     }
-  } else {
+  } else if (S.kind == 1) {
     const float *cs = gptr(S.centers);
     f3 c = mk3(cs[3 * gh.prim], cs[3 * gh.prim + 1], cs[3 * gh.prim + 2]);
     Ng = sub3(P, c);
     Ns = Ng;
     if (S.has_tf && S.has_data) col = tf_color(gptr(S.tf.col), S.tf, gptr(S.vdata)[gh.prim]);
+  } else {
+    // DataDrivenPathLines_postIntersect (DataDrivenPathLines.ispc:212-269):
+    // the radius interpolated between the segment's first two control points
+    // at the curve parameter, mapped back to a data value for the TF
+    float u;
+    Ng = roundcone_normal(M, __float_as_int(gh.v), P, u);
+    Ns = Ng;
+    if (S.has_tf) {
+      const uint32_t i0 = gptr(S.indices)[gh.prim];
+      const float *cp = gptr(S.vertices);
+      const float radius = ((1.f - u) * cp[4 * (int64_t)i0 + 3]) + (u * cp[4 * ((int64_t)i0 + 1) + 3]);
+      const float r0 = S.r0, r1 = S.r1, v0 = S.v0, v1 = S.v1;
+      float dv;
+      if (r0 == r1) dv = v0;
+      else if (r0 < r1 && radius < r0) dv = v0;
+      else if (r0 < r1 && radius > r1) dv = v1;
+      else if (r0 > r1 && radius < r1) dv = v1;
+      else if (r0 > r1 && radius > r0) dv = v0;
+      else dv = v0 + ((radius - r0) / (r1 - r0)) * (v1 - v0);
+      col = tf_color(gptr(S.tf.col), S.tf, dv);
+    }
   }
   Ng = normalize3(Ng);
   Ns = normalize3(Ns);

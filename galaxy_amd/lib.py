@@ -68,6 +68,8 @@ def _declare(L):
         "gxy_model_destroy": (C.c_int, [vp]),
         "gxy_model_add_triangles": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, vp, vp, vp]),
         "gxy_model_add_spheres": (C.c_int, [vp, vp, vp, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, vp]),
+        "gxy_model_add_pathlines": (C.c_int, [vp, vp, vp, C.c_int, vp, C.c_int, C.c_float, C.c_float, C.c_float,
+                                              C.c_float, vp]),
         "gxy_model_commit": (C.c_int, [vp]),
         "gxy_model_bvh_stats": (C.c_int, [vp, vp, vp, vp, vp]),
         "gxy_model_set_hit_output": (C.c_int, [vp, vp, vp]),
@@ -310,6 +312,11 @@ class HipBackend:
         D = None if D is None else a3(D)
         return check(self.L.gxy_model_add_spheres(m, Cn.ctypes.data, None if D is None else D.ctypes.data,
                                                   len(Cn) // 3, r0, r1, v0, v1, tf), "add_spheres")
+
+    def model_add_pathlines(self, m, V, D, I, r0, r1, v0, v1, tf):
+        V, D, I = a3(V), a3(D), a3(I, np.int32)
+        return check(self.L.gxy_model_add_pathlines(m, V.ctypes.data, D.ctypes.data, len(V) // 3, I.ctypes.data,
+                                                    len(I), r0, r1, v0, v1, tf), "add_pathlines")
 
     def model_commit(self, m, use_bvh=True):
         return check(self.L.gxy_model_commit(m), "model_commit")

@@ -177,6 +177,18 @@ class SpheresDesc:
 
 
 @dataclass
+class PathLinesDesc:
+    vertices: np.ndarray                       # (nv, 3)
+    data: np.ndarray                           # (nv,) per-vertex value -> radius
+    indices: np.ndarray                        # (nseg,) segment i joins indices[i], indices[i] + 1
+    radius0: float = 0.01                      # DataDrivenPathLines.cpp:58-63
+    radius1: float = 0.0
+    value0: float = 0.0
+    value1: float = 0.0
+    colormap: Optional[np.ndarray] = None
+
+
+@dataclass
 class LightingDesc:
     sources: np.ndarray = field(default_factory=lambda: np.array([[1, 1, 1]], f32))   # Lighting.ispc:39-43
     types: np.ndarray = field(default_factory=lambda: np.array([2], np.int32))
@@ -202,6 +214,7 @@ class SceneDesc:
     volume_vis: List[VolumeVisDesc] = field(default_factory=list)
     meshes: List[MeshDesc] = field(default_factory=list)
     spheres: List[SpheresDesc] = field(default_factory=list)
+    pathlines: List[PathLinesDesc] = field(default_factory=list)
     local_box: tuple = ((-1, -1, -1), (1, 1, 1))
     global_box: tuple = ((-1, -1, -1), (1, 1, 1))
     neighbors: tuple = (-1, -1, -1, -1, -1, -1)
@@ -476,7 +489,7 @@ def build_scene(be, sc: SceneDesc, lights: Optional[LightingDesc] = None):
         arr[i] = C.pointer(s)
     B.keep += [vvs, arr]
     model = None
-    if sc.meshes or sc.spheres:
+    if sc.meshes or sc.spheres or sc.pathlines:
         model = be.model_create()
         for m in sc.meshes:
             tf = None
@@ -490,6 +503,13 @@ def build_scene(be, sc: SceneDesc, lights: Optional[LightingDesc] = None):
                 tf = be.tf_create_from_maps(np.ascontiguousarray(s.colormap, f32), DEFAULT_OPACITYMAP.copy(), None)
                 B.handles.append(tf)
             be.model_add_spheres(model, s.centers, s.data, s.radius0, s.radius1, s.value0, s.value1, tf)
+        for p in sc.pathlines:
+            tf = None
+            if p.colormap is not None:
+                tf = be.tf_create_from_maps(np.ascontiguousarray(p.colormap, f32), DEFAULT_OPACITYMAP.copy(), None)
+                B.handles.append(tf)
+            be.model_add_pathlines(model, p.vertices, p.data, p.indices, p.radius0, p.radius1, p.value0, p.value1,
+                                   tf)
         be.model_commit(model)
         B.handles.append(model)
     vis = abi.Visualization_ispc()

@@ -265,6 +265,18 @@ int   gxy_model_add_triangles(void *model, const float *vertices, int nvertices,
                               const float *normals, const float *vertex_data, void *tf);
 int   gxy_model_add_spheres(void *model, const float *centers, const float *data, int n,
                             float radius0, float radius1, float value0, float value1, void *tf);
+/* Data-driven path lines (DataDrivenPathLines.cpp:46-160,
+ * DataDrivenPathLines.ispc:212-269): segment i joins vertex indices[i] and
+ * indices[i] + 1; per-vertex data gives the radius (radius0..radius1 over
+ * value0..value1, MAP_RADIUS); consecutive segments form round cubic Bezier
+ * curves with the reference's control points; the hit colour is TF(value)
+ * of the radius at the curve parameter.  Each curve segment is traced as 8
+ * round cones between curve samples (the reference intersects the curves in
+ * Embree, which is not available: that surface is approximated).  Returns
+ * the geomID; primID of a hit = segment index. */
+int   gxy_model_add_pathlines(void *model, const float *vertices, const float *data, int nvertices,
+                              const int *indices, int nsegments, float radius0, float radius1,
+                              float value0, float value1, void *tf);
 int   gxy_model_commit(void *model);
 /* BVH statistics after commit: nodes, leaves, max depth, SAH cost. */
 int   gxy_model_bvh_stats(void *model, int *nodes, int *leaves, int *depth, float *sah);

@@ -234,6 +234,20 @@ struct OSpheres {
   OTF *tf;
 };
 
+// Path lines (DataDrivenPathLines.cpp:46-160): control points of round cubic
+// Bezier segments (x y z radius) and the round-cone pieces each segment is
+// traced as (8 per segment, between curve samples at u = j/8).
+const int kCurvePieces = 8;
+struct OPaths {
+  int geomID;
+  std::vector<float> cp;          // control points, 4 floats each
+  std::vector<uint32_t> ic;       // first control point of each segment
+  std::vector<float> pa, pb, pu;  // pieces: sphere a, sphere b (4 each), (u0, u1)
+  std::vector<int> pseg;          // pieces: segment
+  float r0, r1, v0, v1;
+  OTF *tf;
+};
+
 struct Prim { int geom; int prim; };
 
 // Binary BVH over bounding boxes (binned SAH, 16 bins, leaves <= 4 primitives).
@@ -339,11 +353,13 @@ void bvh_build(OBVH &B, const std::vector<Box3> &boxes) {
 struct OModel {
   std::vector<std::unique_ptr<OMeshPart>> meshes;
   std::vector<std::unique_ptr<OSpheres>> spheres;
+  std::vector<std::unique_ptr<OPaths>> paths;
   int ngeom = 0;
   // flattened triangle list (all meshes) for brute force / BVH
   std::vector<Prim> tris;
   std::vector<Prim> sphs;
-  OBVH tbvh, sbvh;
+  std::vector<Prim> curs;       // (index into paths, piece)
+  OBVH tbvh, sbvh, cbvh;
   bool use_bvh = true;
   bool committed = false;
 };
@@ -442,7 +458,94 @@ struct GeoHit {
   float t;
   int geom, prim;
   float u, v;
+  int piece;    // path lines: (index into paths, piece) of the hit
+  int path;
 };
+
+// Round cone of spheres (a, ra), (b, rb): their convex hull.  Boundary: the
+// cap of sphere a behind the tangent circle at axial coordinate ra*sin(phi),
+// the cone band (h*cos(phi) + y*sin(phi) = ra, y axial from a, h from the
+// axis) up to the tangent circle on sphere b at l + rb*sin(phi), the cap of
+// sphere b beyond it.  Entry = least crossing, exit = greatest; the entry is
+// taken when it lies in (t0, tfar), else the exit when that does (the rule of
+// DataDrivenPathLines.ispc:183-193).  Same operation order as the HIP kernel.
+inline bool sphere_roots(v3 c, float r, v3 org, v3 dir, float dd, float &t1, float &t2) {
+  const float tc = dot(c - org, dir) / dd;
+  const v3 q = (org + tc * dir) - c;
+  const float disc = dd * (r * r - dot(q, q));
+  if (disc < 0.f) return false;
+  const float h = std::sqrt(disc) / dd;
+  t1 = tc - h;
+  t2 = tc + h;
+  return true;
+}
+inline float fmn(float a, float b) { return a < b ? a : b; }
+inline float fmx(float a, float b) { return a > b ? a : b; }
+inline bool roundcone_hit(v3 a, float ra, v3 b, float rb, v3 org, v3 dir, float t0, float tfar, float &t) {
+  const float dd = dot(dir, dir);
+  const v3 ba = b - a;
+  const float l2 = dot(ba, ba), rd = ra - rb;
+  float tin = 3e38f, tout = -3e38f;
+  float t1, t2;
+  if (l2 <= rd * rd) {
+    if (sphere_roots(ra >= rb ? a : b, ra >= rb ? ra : rb, org, dir, dd, t1, t2)) { tin = t1; tout = t2; }
+  } else {
+    const float l = std::sqrt(l2);
+    const v3 w = (1.0f / l) * ba;
+    const float sn = rd / l, cs2 = 1.0f - sn * sn;
+    const float ya = ra * sn, yb = l + rb * sn;
+    if (sphere_roots(a, ra, org, dir, dd, t1, t2)) {
+      if (dot((org + t1 * dir) - a, w) <= ya) { tin = fmn(tin, t1); tout = fmx(tout, t1); }
+      if (dot((org + t2 * dir) - a, w) <= ya) { tin = fmn(tin, t2); tout = fmx(tout, t2); }
+    }
+    if (sphere_roots(b, rb, org, dir, dd, t1, t2)) {
+      if (dot((org + t1 * dir) - a, w) >= yb) { tin = fmn(tin, t1); tout = fmx(tout, t1); }
+      if (dot((org + t2 * dir) - a, w) >= yb) { tin = fmn(tin, t2); tout = fmx(tout, t2); }
+    }
+    const float tc = dot(a - org, dir) / dd;
+    const v3 oa = (org + tc * dir) - a;
+    const float y0 = dot(oa, w), yd = dot(dir, w);
+    const float k = ra - sn * y0;
+    const float A = cs2 * (dd - yd * yd) - sn * sn * (yd * yd);
+    const float B = 2.0f * (cs2 * (dot(oa, dir) - y0 * yd) + sn * yd * k);
+    const float Cc = cs2 * (dot(oa, oa) - y0 * y0) - k * k;
+    const float disc = B * B - 4.0f * A * Cc;
+    if (A != 0.f && disc >= 0.f) {
+      const float sq = std::sqrt(disc), i2a = 0.5f / A;
+      const float r1 = (-B - sq) * i2a, r2 = (-B + sq) * i2a;
+      const float y1 = y0 + r1 * yd, y2 = y0 + r2 * yd;
+      if (y1 > ya && y1 < yb && k - sn * (r1 * yd) >= 0.f) { tin = fmn(tin, tc + r1); tout = fmx(tout, tc + r1); }
+      if (y2 > ya && y2 < yb && k - sn * (r2 * yd) >= 0.f) { tin = fmn(tin, tc + r2); tout = fmx(tout, tc + r2); }
+    }
+  }
+  if (tin > tout) return false;
+  if (tin > t0 && tin < tfar) { t = tin; return true; }
+  if (tout > t0 && tout < tfar) { t = tout; return true; }
+  return false;
+}
+// Surface normal (unnormalised) of piece k of P at point X, and the curve
+// parameter there (the piece's u range at the projection of X on its axis).
+inline v3 roundcone_normal(const OPaths &Pt, int k, v3 X, float &u) {
+  const float *A = &Pt.pa[4 * (size_t)k], *B = &Pt.pb[4 * (size_t)k], *U = &Pt.pu[2 * (size_t)k];
+  const v3 a = mk(A[0], A[1], A[2]), b = mk(B[0], B[1], B[2]);
+  const float ra = A[3], rb = B[3];
+  const v3 ba = b - a, pa = X - a;
+  const float l2 = dot(ba, ba), rd = ra - rb;
+  float s = l2 > 0.f ? dot(pa, ba) / l2 : 0.f;
+  s = s < 0.f ? 0.f : (s > 1.f ? 1.f : s);
+  u = s >= 1.f ? U[1] : U[0] + s * (U[1] - U[0]);
+  if (l2 <= rd * rd) return ra >= rb ? pa : X - b;
+  const float l = std::sqrt(l2);
+  const v3 w = (1.0f / l) * ba;
+  const float sn = rd / l;
+  const float y = dot(pa, w);
+  if (y <= ra * sn) return pa;
+  if (y >= l + rb * sn) return X - b;
+  const v3 q = pa - y * w;
+  const float hq = std::sqrt(dot(q, q));
+  if (!(hq > 0.f)) return w;
+  return (std::sqrt(1.0f - sn * sn) / hq) * q + sn * w;
+}
 
 // Closest hit: candidate replaces the current one when it is nearer, or equal
 // in t with a lower (geomID, primID) — the documented deterministic tie rule.
@@ -508,8 +611,21 @@ void model_trace(const OModel &M, v3 org, v3 dir, float t0, float tfar, GeoHit &
       }
     }
   };
+  auto test_cur = [&](int idx) {
+    const Prim &P = M.curs[idx];
+    const OPaths &pl = *M.paths[P.geom];
+    const float *A = &pl.pa[4 * (size_t)P.prim], *B = &pl.pb[4 * (size_t)P.prim];
+    float t;
+    float lim = h.geom < 0 ? h.t : std::nextafter(h.t, 3e38f);
+    const int seg = pl.pseg[P.prim];
+    if (roundcone_hit(mk(A[0], A[1], A[2]), A[3], mk(B[0], B[1], B[2]), B[3], org, dir, t0, lim, t) &&
+        (h.geom < 0 || better(t, pl.geomID, seg, h))) {
+      h.t = t; h.geom = pl.geomID; h.prim = seg; h.u = h.v = 0.f; h.path = P.geom; h.piece = P.prim;
+    }
+  };
   walk(M.tbvh, (int)M.tris.size(), test_tri);
   walk(M.sbvh, (int)M.sphs.size(), test_sph);
+  walk(M.cbvh, (int)M.curs.size(), test_cur);
 }
 
 // ------------------------------------------------------------------ vis
@@ -656,6 +772,27 @@ void trace_one(const Visualization_ispc &vis, const std::vector<OVolumeVis> &vv,
               Ng = P - c;
               Ns = Ng;
               if (s->tf && !s->D.empty()) col = tf_color(*s->tf, s->D[gh.prim]);
+              break;
+            }
+          for (auto &pl : model->paths)
+            if (pl->geomID == gh.geom) {
+              // DataDrivenPathLines_postIntersect (DataDrivenPathLines.ispc:212-269)
+              float u;
+              Ng = roundcone_normal(*pl, gh.piece, P, u);
+              Ns = Ng;
+              if (pl->tf) {
+                const uint32_t i0 = pl->ic[gh.prim];
+                const float radius = ((1.f - u) * pl->cp[4 * (size_t)i0 + 3]) + (u * pl->cp[4 * ((size_t)i0 + 1) + 3]);
+                const float r0 = pl->r0, r1 = pl->r1, v0 = pl->v0, v1 = pl->v1;
+                float dv;
+                if (r0 == r1) dv = v0;
+                else if (r0 < r1 && radius < r0) dv = v0;
+                else if (r0 < r1 && radius > r1) dv = v1;
+                else if (r0 > r1 && radius < r1) dv = v1;
+                else if (r0 > r1 && radius > r0) dv = v0;
+                else dv = v0 + ((radius - r0) / (r1 - r0)) * (v1 - v0);
+                col = tf_color(*pl->tf, dv);
+              }
               break;
             }
         }
@@ -973,6 +1110,87 @@ OEXPORT int oracle_model_add_spheres(void *mp, const float *C, const float *D, i
   M->spheres.push_back(std::move(s));
   return M->spheres.back()->geomID;
 }
+// MAP_RADIUS (DataDrivenPathLines.cpp:11-19)
+inline float path_radius(float d, float r0, float r1, float v0, float v1) {
+  if (v0 == v1) return r0;
+  const float R = (d - v0) / (v1 - v0);
+  return (R <= 0.0f) ? r0 : (R >= 1.0f) ? r1 : r0 + R * (r1 - r0);
+}
+// DataDrivenPathLines::finalize (DataDrivenPathLines.cpp:97-157): per segment
+// four control points; a segment that continues the previous one starts with
+// the previous outgoing tangent (radius lerp 1/3), one that is continued ends
+// with the tangent (next - start)/3 split in the ratio of the two segment
+// lengths (radius lerp 2/3); curve ends repeat the end point.  Then the
+// round-cone pieces between Bernstein samples of each segment.
+OEXPORT int oracle_model_add_pathlines(void *mp, const float *V, const float *D, int nv, const int *I, int nseg,
+                                       float r0, float r1, float v0, float v1, void *tf) {
+  OModel *M = (OModel *)mp;
+  auto P = std::unique_ptr<OPaths>(new OPaths);
+  P->r0 = r0; P->r1 = r1; P->v0 = v0; P->v1 = v1;
+  P->tf = (OTF *)tf;
+  bool middle = false;
+  v3 tangent = mk(0.f, 0.f, 0.f);
+  auto vert = [&](int i) { return mk(V[3 * (size_t)i], V[3 * (size_t)i + 1], V[3 * (size_t)i + 2]); };
+  auto push = [&](v3 p, float r) { P->cp.insert(P->cp.end(), {p.x, p.y, p.z, r}); };
+  for (int i = 0; i < nseg; i++) {
+    const int idx = I[i];
+    if (idx < 0 || idx + 1 >= nv) return -1;
+    const v3 start = vert(idx), end = vert(idx + 1);
+    const v3 se = start - end;
+    const float lseg = std::sqrt(dot(se, se));
+    const float rs = path_radius(D[idx], r0, r1, v0, v1), re = path_radius(D[idx + 1], r0, r1, v0, v1);
+    P->ic.push_back((uint32_t)(P->cp.size() / 4));
+    push(start, rs);
+    if (middle) push(start + tangent, (1.f - 1.f / 3) * rs + (1.f / 3) * re);
+    else push(start, rs);
+    middle = i + 1 < nseg && I[i + 1] == idx + 1;
+    if (middle) {
+      if (idx + 2 >= nv) return -1;
+      const v3 next = vert(idx + 2);
+      const v3 delta = (1.f / 3) * (next - start);
+      const v3 ne = next - end;
+      const float bl = std::sqrt(dot(ne, ne));
+      const float r = lseg / (lseg + bl);
+      push(end - r * delta, (1.f - 2.f / 3) * rs + (2.f / 3) * re);
+      tangent = (1.f - r) * delta;
+    } else {
+      push(end, re);
+    }
+    push(end, re);
+  }
+  for (int i = 0; i < nseg; i++) {
+    const float *c = &P->cp[4 * (size_t)P->ic[i]];
+    float prev[4] = {0, 0, 0, 0}, prevu = 0.f;
+    for (int j = 0; j <= kCurvePieces; j++) {
+      const float u = (float)j / (float)kCurvePieces, q = 1.0f - u;
+      const float w0 = q * q * q, w1 = 3.0f * q * q * u, w2 = 3.0f * q * u * u, w3 = u * u * u;
+      float x[4];
+      for (int k = 0; k < 4; k++) x[k] = w0 * c[k] + w1 * c[4 + k] + w2 * c[8 + k] + w3 * c[12 + k];
+      if (j > 0) {
+        P->pa.insert(P->pa.end(), prev, prev + 4);
+        P->pb.insert(P->pb.end(), x, x + 4);
+        P->pu.insert(P->pu.end(), {prevu, u});
+        P->pseg.push_back(i);
+      }
+      for (int k = 0; k < 4; k++) prev[k] = x[k];
+      prevu = u;
+    }
+  }
+  P->geomID = M->ngeom++;
+  M->paths.push_back(std::move(P));
+  return M->paths.back()->geomID;
+}
+// Control points (4 floats each) and segment starts of path-line set `which`
+// (in the order added); returns the number of control points.
+OEXPORT int oracle_pathline_controls(void *mp, int which, float *cp, int cap, int *ic, int icap) {
+  OModel *M = (OModel *)mp;
+  if (which < 0 || which >= (int)M->paths.size()) return -1;
+  const OPaths &P = *M->paths[which];
+  const int n = (int)P.cp.size() / 4;
+  for (int k = 0; k < std::min(cap, n) * 4; k++) cp[k] = P.cp[k];
+  for (int k = 0; k < std::min(icap, (int)P.ic.size()); k++) ic[k] = (int)P.ic[k];
+  return n;
+}
 OEXPORT int oracle_model_commit(void *mp, int use_bvh) {
   OModel *M = (OModel *)mp;
   M->use_bvh = use_bvh != 0;
@@ -1001,9 +1219,25 @@ OEXPORT int oracle_model_commit(void *mp, int use_bvh) {
       sbx.push_back(b);
     }
   }
+  M->curs.clear();
+  std::vector<Box3> cbx;
+  for (int g = 0; g < (int)M->paths.size(); g++) {
+    auto &pl = *M->paths[g];
+    for (int k = 0; k < (int)pl.pseg.size(); k++) {
+      M->curs.push_back({g, k});
+      const float *a = &pl.pa[4 * (size_t)k], *b = &pl.pb[4 * (size_t)k];
+      Box3 bx;
+      for (int c = 0; c < 3; c++) {
+        bx.lo[c] = std::min(a[c] - a[3], b[c] - b[3]);
+        bx.hi[c] = std::max(a[c] + a[3], b[c] + b[3]);
+      }
+      cbx.push_back(bx);
+    }
+  }
   if (M->use_bvh) {
     bvh_build(M->tbvh, tb);
     bvh_build(M->sbvh, sbx);
+    bvh_build(M->cbvh, cbx);
   }
   M->committed = true;
   return 0;

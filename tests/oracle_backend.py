@@ -49,6 +49,9 @@ def lib():
         L.oracle_model_add_spheres.argtypes = [vp, vp, vp, C.c_int, C.c_float, C.c_float, C.c_float,
                                                C.c_float, vp]
         L.oracle_model_commit.argtypes = [vp, C.c_int]
+        L.oracle_model_add_pathlines.argtypes = [vp, vp, vp, C.c_int, vp, C.c_int, C.c_float, C.c_float,
+                                                 C.c_float, C.c_float, vp]
+        L.oracle_pathline_controls.argtypes = [vp, C.c_int, vp, C.c_int, vp, C.c_int]
         L.oracle_model_intersect.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.oracle_triangles_bruteforce64.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]
         L.oracle_TraceRays.restype = C.c_longlong
@@ -131,6 +134,12 @@ class OracleBackend:
         self.keep += [Cn, D]
         return self.L.oracle_model_add_spheres(m, Cn.ctypes.data, None if D is None else D.ctypes.data,
                                                len(Cn) // 3, r0, r1, v0, v1, tf)
+
+    def model_add_pathlines(self, m, V, D, I, r0, r1, v0, v1, tf):
+        V, D, I = a3(V), a3(D), a3(I, np.int32)
+        self.keep += [V, D, I]
+        return self.L.oracle_model_add_pathlines(m, V.ctypes.data, D.ctypes.data, len(V) // 3, I.ctypes.data,
+                                                 len(I), r0, r1, v0, v1, tf)
 
     def model_commit(self, m, use_bvh=True):
         return self.L.oracle_model_commit(m, 1 if use_bvh else 0)

@@ -1,0 +1,105 @@
+"""Pins of the oracle's path lines (DataDrivenPathLines.cpp:11-157, .ispc:212-269):
+the round-Bezier control points through the properties the reference's
+construction fixes, MAP_RADIUS on its cases, a straight constant-radius path
+line against the closed-form capsule, and every hit of a curved scene on the
+FP64 swept-sphere surface with no crossing before it."""
+import numpy as np
+import pytest
+
+from pathlines_common import (PIECES, capsule_hits, controls, helix_lines, oracle_intersect, pieces64, rays_toward,
+                              swept_distance)
+
+f32 = np.float32
+
+
+def _model(oracle, V, D, I, r0=0.02, r1=0.06, v0=0.0, v1=1.0):
+    M = oracle.model_create()
+    oracle.model_add_pathlines(M, V, D, I, r0, r1, v0, v1, None)
+    oracle.model_commit(M, True)
+    return M
+
+
+def test_control_points_follow_the_construction_rules(oracle):
+    V, D, I = helix_lines(seed=1, nlines=2, nverts=6)
+    M = _model(oracle, V, D, I)
+    cp, ic = controls(oracle, M, 0, len(I))
+    assert list(ic) == list(range(0, 4 * len(I), 4))
+    P = cp.reshape(-1, 4, 4).astype(np.float64)
+    rad = np.clip(D.astype(np.float64), 0, 1) * 0.04 + 0.02        # MAP_RADIUS with r0 0.02, r1 0.06 over [0, 1]
+    for i, idx in enumerate(I):
+        first = i == 0 or I[i - 1] != idx - 1
+        last = i == len(I) - 1 or I[i + 1] != idx + 1
+        # end points and their radii
+        assert np.allclose(P[i, 0, :3], V[idx], atol=1e-7) and np.allclose(P[i, 3, :3], V[idx + 1], atol=1e-7)
+        assert np.isclose(P[i, 0, 3], rad[idx], atol=1e-7) and np.isclose(P[i, 3, 3], rad[idx + 1], atol=1e-7)
+        if first:
+            assert np.array_equal(P[i, 1], P[i, 0])                         # a curve starts with a repeated point
+        else:
+            assert np.isclose(P[i, 1, 3], (2 * rad[idx] + rad[idx + 1]) / 3, atol=1e-6)
+        if last:
+            assert np.array_equal(P[i, 2], P[i, 3])                         # and ends with one
+        else:
+            assert np.isclose(P[i, 2, 3], (rad[idx] + 2 * rad[idx + 1]) / 3, atol=1e-6)
+            # the tangent through the shared vertex: the chord (next - start) / 3, split between the two
+            # segments in the ratio of their lengths, the same direction on both sides (G1 joint)
+            out_, in_ = P[i, 3, :3] - P[i, 2, :3], P[i + 1, 1, :3] - P[i + 1, 0, :3]
+            chord = (V[idx + 2].astype(np.float64) - V[idx]) / 3
+            assert np.allclose(out_ + in_, chord, atol=1e-6)
+            l1 = np.linalg.norm(V[idx + 1].astype(np.float64) - V[idx])
+            l2 = np.linalg.norm(V[idx + 2].astype(np.float64) - V[idx + 1])
+            assert np.allclose(out_, chord * l1 / (l1 + l2), atol=1e-6)
+            assert np.array_equal(P[i + 1, 0, :3], P[i, 3, :3])
+
+
+@pytest.mark.parametrize("r0,r1,v0,v1", [(0.05, 0.05, 0.0, 0.0), (0.02, 0.08, 0.2, 0.6), (0.08, 0.02, 0.2, 0.6)])
+def test_map_radius_cases(oracle, r0, r1, v0, v1):
+    V = np.array([[0, 0, 0], [1, 0, 0], [2, 0, 0], [3, 0, 0], [4, 0, 0]], f32)
+    D = np.array([-1.0, 0.2, 0.4, 0.6, 2.0], f32)
+    I = np.array([0, 1, 2, 3], np.int32)
+    M = _model(oracle, V, D, I, r0, r1, v0, v1)
+    cp, _ = controls(oracle, M, 0, 4)
+    ends = np.r_[cp[0::4, 3], cp[-1, 3]]
+    if v0 == v1:
+        want = np.full(5, r0)
+    else:
+        R = (D.astype(np.float64) - v0) / (v1 - v0)
+        want = np.where(R <= 0, r0, np.where(R >= 1, r1, r0 + R * (r1 - r0)))
+    assert np.allclose(ends, want, atol=1e-7)
+
+
+def test_straight_pathline_is_a_capsule(oracle):
+    V = np.array([[0.25 * k, 0.0, 0.0] for k in range(5)], f32)
+    D = np.zeros(5, f32)
+    I = np.arange(4, dtype=np.int32)
+    M = _model(oracle, V, D, I, 0.1, 0.3, 0.0, 0.0)        # value0 == value1: constant radius 0.1
+    org, d = rays_toward(6000, 5, (-0.15, -0.15, -0.15), (1.15, 0.15, 0.15))
+    t, g, p = oracle_intersect(oracle, M, org, d)
+    ref = capsule_hits(org, d, (0, 0, 0), (1, 0, 0), 0.1)
+    hit = np.isfinite(ref)
+    assert hit.sum() > 2000 and (~hit).sum() > 500
+    got = g >= 0
+    # grazing rays may fall either way
+    assert (got == hit).mean() >= 0.999
+    both = got & hit
+    assert np.allclose(t[both], ref[both], rtol=2e-5, atol=2e-6)
+    assert set(np.unique(p[got])) <= {0, 1, 2, 3}
+
+
+def test_hits_lie_on_the_swept_surface(oracle):
+    V, D, I = helix_lines(seed=2, nlines=4, nverts=10)
+    M = _model(oracle, V, D, I)
+    org, d = rays_toward(3000, 9, (-0.9, -0.9, -0.7), (0.9, 0.9, 0.7))
+    t, g, p = oracle_intersect(oracle, M, org, d)
+    hit = g >= 0
+    assert hit.sum() > 300
+    cp, ic = controls(oracle, M, 0, len(I))
+    a, b = pieces64(cp, ic)
+    x = org[hit].astype(np.float64) + t[hit, None].astype(np.float64) * d[hit]
+    # on the surface of the hit segment's pieces
+    assert np.abs(swept_distance(x, a[p[hit]], b[p[hit]])).max() <= 2e-5
+    # nothing of any segment before the hit: points at 0.2 .. 0.98 of the way are outside every piece
+    allA = np.broadcast_to(a.reshape(1, -1, 4), (hit.sum(), a.shape[0] * PIECES, 4))
+    allB = np.broadcast_to(b.reshape(1, -1, 4), (hit.sum(), a.shape[0] * PIECES, 4))
+    for f in (0.2, 0.6, 0.98):
+        xf = org[hit].astype(np.float64) + (f * t[hit])[:, None] * d[hit]
+        assert swept_distance(xf, allA, allB, grid=257).min() > -1e-6
