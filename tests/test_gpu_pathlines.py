@@ -45,7 +45,8 @@ def test_pathlines_match_oracle(hip, oracle):
     o = hr.copy()
     res = oracle.trace(Bo.vis, o, geom_ids=True)
     hits = res["geom"] >= 0
-    assert (res["geom"] == 0).sum() > 2000 and (res["geom"] == 1).sum() > 100
+    # geomID 0 is the mesh (meshes are added first), 1 the path lines
+    assert (res["geom"] == 1).sum() > 500 and (res["geom"] == 0).sum() > 100
     agree = (gg == res["geom"]) & (gp == res["prim"])
     assert agree.mean() >= 0.9999
     assert np.allclose(g["t"][hits & agree], o["t"][hits & agree], rtol=1e-5)
