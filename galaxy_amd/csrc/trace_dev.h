@@ -364,17 +364,19 @@ __device__ __forceinline__ void model_trace(const DModel &M, f3 org, f3 dir, flo
   }
   if (W == 0 && M.ncurve > 0) {
     // path-line pieces, in the per-model variant only (models with path lines
-    // are dispatched to it); h.v carries the piece (its sorted index) to the
-    // shading
+    // are dispatched to it); h.u holds the hit piece's first curve parameter
+    // (pieces of one segment that tie in t: the lower one wins, whatever the
+    // traversal order), h.v the piece (its sorted index) for the shading
     auto cleaf = [&](int start, int count) {
       for (int k = start; k < start + count; k++) {
         const float *A = gptr(M.ca) + 4 * (int64_t)k, *B = gptr(M.cb) + 4 * (int64_t)k;
         float lim = h.geom < 0 ? h.t : nextafterf(h.t, 3e38f);
         float t;
         if (roundcone_hit(mk3(A[0], A[1], A[2]), A[3], mk3(B[0], B[1], B[2]), B[3], org, dir, t0, lim, t)) {
-          int g = gptr(M.cgeom)[k], pr = gptr(M.cprim)[k];
-          if (h.geom < 0 || better(t, g, pr, h)) {
-            h.t = t; h.geom = g; h.prim = pr; h.u = 0.f; h.v = __int_as_float(k);
+          const int g = gptr(M.cgeom)[k], pr = gptr(M.cprim)[k];
+          const float u0 = gptr(M.cu)[2 * (int64_t)k];
+          if (h.geom < 0 || better(t, g, pr, h) || (t == h.t && g == h.geom && pr == h.prim && u0 < h.u)) {
+            h.t = t; h.geom = g; h.prim = pr; h.u = u0; h.v = __int_as_float(k);
           }
         }
       }
@@ -385,23 +387,28 @@ __device__ __forceinline__ void model_trace(const DModel &M, f3 org, f3 dir, flo
 }
 
 // The surface normal of round-cone piece k at P (unnormalised) and the curve
-// parameter there: the piece's (u0, u1) at the projection of P on its axis.
+// parameter there: u0 or u1 on the caps, on the band the piece's (u0, u1) at
+// the projection of P on its axis.
 __device__ __forceinline__ f3 roundcone_normal(const DModel &M, int k, f3 P, float &u) {
   const float *A = gptr(M.ca) + 4 * (int64_t)k, *B = gptr(M.cb) + 4 * (int64_t)k, *U = gptr(M.cu) + 2 * (int64_t)k;
   const f3 a = mk3(A[0], A[1], A[2]), b = mk3(B[0], B[1], B[2]);
   const float ra = A[3], rb = B[3];
   const f3 ba = sub3(b, a), pa = sub3(P, a);
   const float l2 = dot3(ba, ba), rd = ra - rb;
-  float s = l2 > 0.f ? dot3(pa, ba) / l2 : 0.f;
-  s = s < 0.f ? 0.f : (s > 1.f ? 1.f : s);
-  u = s >= 1.f ? U[1] : U[0] + s * (U[1] - U[0]);
-  if (l2 <= rd * rd) return ra >= rb ? pa : sub3(P, b);
+  if (l2 <= rd * rd) {
+    u = ra >= rb ? U[0] : U[1];
+    return ra >= rb ? pa : sub3(P, b);
+  }
   const float l = sqrtf(l2);
   const f3 w = scl3(1.0f / l, ba);
   const float sn = rd / l;
   const float y = dot3(pa, w);
-  if (y <= ra * sn) return pa;
-  if (y >= l + rb * sn) return sub3(P, b);
+  // caps: the end parameters; the band: the projection on the axis
+  if (y <= ra * sn) { u = U[0]; return pa; }
+  if (y >= l + rb * sn) { u = U[1]; return sub3(P, b); }
+  float s = dot3(pa, ba) / l2;
+  s = s < 0.f ? 0.f : (s > 1.f ? 1.f : s);
+  u = s >= 1.f ? U[1] : U[0] + s * (U[1] - U[0]);
   const f3 q = sub3(pa, scl3(y, w));
   const float hq = sqrtf(dot3(q, q));
   if (!(hq > 0.f)) return w;

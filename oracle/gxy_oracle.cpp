@@ -524,23 +524,27 @@ inline bool roundcone_hit(v3 a, float ra, v3 b, float rb, v3 org, v3 dir, float 
   return false;
 }
 // Surface normal (unnormalised) of piece k of P at point X, and the curve
-// parameter there (the piece's u range at the projection of X on its axis).
+// parameter there (u0 or u1 on the caps, on the band the piece's u range at
+// the projection of X on its axis).
 inline v3 roundcone_normal(const OPaths &Pt, int k, v3 X, float &u) {
   const float *A = &Pt.pa[4 * (size_t)k], *B = &Pt.pb[4 * (size_t)k], *U = &Pt.pu[2 * (size_t)k];
   const v3 a = mk(A[0], A[1], A[2]), b = mk(B[0], B[1], B[2]);
   const float ra = A[3], rb = B[3];
   const v3 ba = b - a, pa = X - a;
   const float l2 = dot(ba, ba), rd = ra - rb;
-  float s = l2 > 0.f ? dot(pa, ba) / l2 : 0.f;
-  s = s < 0.f ? 0.f : (s > 1.f ? 1.f : s);
-  u = s >= 1.f ? U[1] : U[0] + s * (U[1] - U[0]);
-  if (l2 <= rd * rd) return ra >= rb ? pa : X - b;
+  if (l2 <= rd * rd) {
+    u = ra >= rb ? U[0] : U[1];
+    return ra >= rb ? pa : X - b;
+  }
   const float l = std::sqrt(l2);
   const v3 w = (1.0f / l) * ba;
   const float sn = rd / l;
   const float y = dot(pa, w);
-  if (y <= ra * sn) return pa;
-  if (y >= l + rb * sn) return X - b;
+  if (y <= ra * sn) { u = U[0]; return pa; }
+  if (y >= l + rb * sn) { u = U[1]; return X - b; }
+  float s = dot(pa, ba) / l2;
+  s = s < 0.f ? 0.f : (s > 1.f ? 1.f : s);
+  u = s >= 1.f ? U[1] : U[0] + s * (U[1] - U[0]);
   const v3 q = pa - y * w;
   const float hq = std::sqrt(dot(q, q));
   if (!(hq > 0.f)) return w;
@@ -618,9 +622,12 @@ void model_trace(const OModel &M, v3 org, v3 dir, float t0, float tfar, GeoHit &
     float t;
     float lim = h.geom < 0 ? h.t : std::nextafter(h.t, 3e38f);
     const int seg = pl.pseg[P.prim];
+    const float u0 = pl.pu[2 * (size_t)P.prim];
+    // pieces of one segment that tie in t: the lower curve parameter wins
     if (roundcone_hit(mk(A[0], A[1], A[2]), A[3], mk(B[0], B[1], B[2]), B[3], org, dir, t0, lim, t) &&
-        (h.geom < 0 || better(t, pl.geomID, seg, h))) {
-      h.t = t; h.geom = pl.geomID; h.prim = seg; h.u = h.v = 0.f; h.path = P.geom; h.piece = P.prim;
+        (h.geom < 0 || better(t, pl.geomID, seg, h) ||
+         (t == h.t && pl.geomID == h.geom && seg == h.prim && u0 < h.u))) {
+      h.t = t; h.geom = pl.geomID; h.prim = seg; h.u = u0; h.v = 0.f; h.path = P.geom; h.piece = P.prim;
     }
   };
   walk(M.tbvh, (int)M.tris.size(), test_tri);

@@ -103,3 +103,30 @@ def test_hits_lie_on_the_swept_surface(oracle):
     for f in (0.2, 0.6, 0.98):
         xf = org[hit].astype(np.float64) + (f * t[hit])[:, None] * d[hit]
         assert swept_distance(xf, allA, allB, grid=257).min() > -1e-6
+
+
+def test_traversal_order_does_not_change_results(oracle):
+    """Pieces of one segment can reach the same depth (a shared sphere): the
+    tie rule (lower curve parameter) makes normals and colours independent of
+    the order the tree visits them, so the BVH and brute force agree."""
+    from conftest import random_rays
+    from galaxy_amd import scenes
+    from pathlines_common import helix_desc
+    b = ((-1.5,) * 3, (1.5,) * 3)
+    sc = scenes.SceneDesc(pathlines=[helix_desc(seed=3)], local_box=b, global_box=b)
+    outs = []
+    for use_bvh in (True, False):
+        orig = oracle.model_commit
+        oracle.model_commit = lambda m, use_bvh_=True, _b=use_bvh: orig(m, _b)
+        try:
+            Bo = scenes.build_scene(oracle, sc)
+        finally:
+            oracle.model_commit = orig
+        hr = random_rays(30000, seed=12, lo=-1.4, hi=1.4)
+        res = oracle.trace(Bo.vis, hr, geom_ids=True)
+        outs.append((hr, res))
+    (h1, r1), (h2, r2) = outs
+    assert (r1["geom"] >= 0).sum() > 1000
+    assert np.array_equal(r1["geom"], r2["geom"]) and np.array_equal(r1["prim"], r2["prim"])
+    for f in ("t", "nx", "ny", "nz", "sr", "sg", "sb", "r", "g", "b", "o"):
+        assert np.array_equal(h1[f], h2[f]), f
