@@ -449,7 +449,7 @@ static int make_trace_params(const Visualization_ispc *vis, float global_eps, Tr
     ModelObj *M = lookup<ModelObj>(G.models, vis->model);
     if (!M) return fail("Visualization model is not a gxy model handle");
     if (!M->committed) return fail("model used before gxy_model_commit");
-    P.has_model = (M->d.ntris > 0 || M->d.nsph > 0) ? 1 : 0;
+    P.has_model = (M->d.ntris > 0 || M->d.nsph > 0 || M->d.ncurve > 0) ? 1 : 0;
     P.model = M->d;
     P.model.wide = (G.bvh_width == 4 && (M->d.ntris == 0 || M->d.tnodes4) && (M->d.nsph == 0 || M->d.snodes4)) ? 1 : 0;
     P.hit_geom = M->hit_geom;
