@@ -507,12 +507,14 @@ def main():
                 t = time.perf_counter()
                 b2 = scenes.build_scene(hip, sc2)
                 build_s = time.perf_counter() - t
+                gtab = []
                 if args.geometry_stack is not None:
                     glib.check(L.gxy_set_geometry_stack(args.geometry_stack))
                 elif not args.no_tune:
                     from galaxy_amd import autotune
                     fb2 = glib.DeviceArray(sc2.cameras[0].width * sc2.cameras[0].height * 16)
-                    autotune.tune_geometry_stack(L, frame_fn(L, b2, sc2.cameras[0], fb2, sc2.epsilon))
+                    _, gtab = autotune.tune_geometry_stack(L, frame_fn(L, b2, sc2.cameras[0], fb2, sc2.epsilon),
+                                                           reps=4)
                     fb2.free()
                 dt2, tot2, kt2, st2, _, bd2 = run_frames(L, hip, b2, sc2.cameras[0], steps, 1, sc2.epsilon)
                 extra[name] = dict(mrays_per_s=round(tot2["ray_passes"] / dt2 / 1e6, 2),
@@ -522,7 +524,8 @@ def main():
                                    scene_build_s=round(build_s, 3),
                                    trace_ms_per_frame=round(kt2["trace"][0] / steps, 3),
                                    kernel_ms_per_frame=bd2,
-                                   geometry_stack=L.gxy_get_geometry_stack(), bvh_width=L.gxy_get_bvh_width())
+                                   geometry_stack=L.gxy_get_geometry_stack(), bvh_width=L.gxy_get_bvh_width(),
+                                   geometry_tune=[[list(c), round(ms, 4)] for c, ms in gtab])
                 b2 = None
             except Exception as e:  # reported, not hidden
                 extra[name] = dict(error=str(e))
