@@ -499,7 +499,8 @@ def main():
     if not args.no_extra:
         extra = {}
         for name, fn, steps in (("c1_sphere_mesh_512", workloads.c1, args.steps),
-                                ("c3_nine_icospheres_4k_ao16", workloads.c3, args.c3_steps)):
+                                ("c3_nine_icospheres_4k_ao16", workloads.c3, args.c3_steps),
+                                ("pathlines_4000x63_segments_1024", workloads.pathlines, args.steps)):
             try:
                 sc2, a2 = fn()
                 glib.check(L.gxy_set_opacity_threshold(a2))

@@ -95,6 +95,32 @@ C2_TF = (np.array([[0.23, 0.30, 0.75], [0.34, 0.51, 0.95], [0.55, 0.69, 1.00], [
 UNIT_BOX = ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
 
 
+def pathlines(width=1024, height=1024, nlines=4000, nverts=64, seed=7):
+    """Data-driven path lines: seeded smooth random curves (a walk with
+    momentum) through the unit cube, per-vertex data mapping to radii
+    0.002-0.006 and to a two-colour TF, one point light with shadows."""
+    rng = np.random.default_rng(seed)
+    p = rng.uniform(0.1, 0.9, (nlines, 3))
+    v = rng.normal(size=(nlines, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    pts = np.empty((nlines, nverts, 3))
+    for k in range(nverts):
+        pts[:, k] = p
+        v = v + 0.35 * rng.normal(size=v.shape)
+        v /= np.linalg.norm(v, axis=1, keepdims=True)
+        p = np.clip(p + 0.012 * v, 0.02, 0.98)
+    V = pts.reshape(-1, 3).astype(f32)
+    D = np.repeat(rng.uniform(0, 1, nlines), nverts).astype(f32) * np.tile(np.linspace(0.3, 1.0, nverts), nlines)
+    I = (np.arange(nlines)[:, None] * nverts + np.arange(nverts - 1)[None, :]).reshape(-1).astype(np.int32)
+    pl = scenes.PathLinesDesc(V, D.astype(f32), I, radius0=0.002, radius1=0.006, value0=0.0, value1=1.0,
+                              colormap=np.array([[0.0, 0.2, 0.4, 0.9], [1.0, 0.95, 0.5, 0.1]], f32))
+    box = UNIT_BOX
+    L = scenes.LightingDesc(np.array([[2.0, 3.0, 4.0]], f32), np.array([2], np.int32), Ka=0.3, Kd=0.7,
+                            shadows=True)
+    cam = scenes.CameraDesc((0.5, 0.6, 2.3), (0.5, 0.5, 0.5), (0.0, 1.0, 0.0), 35.0, width, height)
+    return scenes.SceneDesc(pathlines=[pl], local_box=box, global_box=box, lighting=L, cameras=[cam]), 0.999
+
+
 def unit_volume_scene(edge, seed, octaves=4, base_frequency=4.0):
     """A noise volume of edge^3 voxels spanning the unit cube, with the volume
     render's 8-point TF, iso 0.5 and compositing on; step 0.5 voxel."""
