@@ -1118,7 +1118,8 @@ GXY_EXPORT int gxy_model_add_pathlines(void *model, const float *V, const float 
         bl2 += (nx[k] - en[k]) * (nx[k] - en[k]);
       }
       const float bl = std::sqrt(bl2);
-      const float r = lseg / (lseg + bl);
+      // two zero-length segments in a row: the reference's 0/0 taken as 1/2
+      const float r = lseg + bl > 0.f ? lseg / (lseg + bl) : 0.5f;
       push(en[0] - r * dl[0], en[1] - r * dl[1], en[2] - r * dl[2], (1.f - 2.f / 3) * rs + (2.f / 3) * re);
       for (int k = 0; k < 3; k++) tan[k] = (1.f - r) * dl[k];
     } else {

@@ -1157,7 +1157,8 @@ OEXPORT int oracle_model_add_pathlines(void *mp, const float *V, const float *D,
       const v3 delta = (1.f / 3) * (next - start);
       const v3 ne = next - end;
       const float bl = std::sqrt(dot(ne, ne));
-      const float r = lseg / (lseg + bl);
+      // two zero-length segments in a row: the reference's 0/0 taken as 1/2
+      const float r = lseg + bl > 0.f ? lseg / (lseg + bl) : 0.5f;
       push(end - r * delta, (1.f - 2.f / 3) * rs + (2.f / 3) * re);
       tangent = (1.f - r) * delta;
     } else {

@@ -130,3 +130,19 @@ def test_traversal_order_does_not_change_results(oracle):
     assert np.array_equal(r1["geom"], r2["geom"]) and np.array_equal(r1["prim"], r2["prim"])
     for f in ("t", "nx", "ny", "nz", "sr", "sg", "sb", "r", "g", "b", "o"):
         assert np.array_equal(h1[f], h2[f]), f
+
+
+def test_repeated_vertices_give_finite_pieces(oracle):
+    """Zero-length segments (a polyline that stops): the split ratio of two
+    zero lengths in a row (0/0 in the reference) is taken as 1/2, so every
+    control point stays finite and rays still hit the line."""
+    V = np.array([[0, 0, 0], [0.5, 0, 0], [0.5, 0, 0], [0.5, 0, 0], [1, 0, 0]], f32)
+    D = np.zeros(5, f32)
+    I = np.arange(4, dtype=np.int32)
+    M = _model(oracle, V, D, I, 0.1, 0.1, 0.0, 0.0)
+    cp, ic = controls(oracle, M, 0, 4)
+    assert np.isfinite(cp).all()
+    org, d = rays_toward(2000, 3, (-0.15, -0.15, -0.15), (1.15, 0.15, 0.15))
+    t, g, p = oracle_intersect(oracle, M, org, d)
+    ref = capsule_hits(org, d, (0, 0, 0), (1, 0, 0), 0.1)
+    assert ((g >= 0) == np.isfinite(ref)).mean() >= 0.999
