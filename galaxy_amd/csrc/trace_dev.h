@@ -416,10 +416,13 @@ __device__ __forceinline__ f3 roundcone_normal(const DModel &M, int k, f3 P, flo
 }
 
 // postIntersect for the hit geometry, then DG_NORMALIZE | DG_FACEFORWARD.
+// W: the BVH width of the kernel variant; path lines (traced by the
+// per-model variant, W = 0, only) are shaded there only.
+template <int W>
 __device__ void geom_shade(const DModel &M, const GeoHit &gh, f3 org, f3 dir, Hit &hit) {
   const DGeomShade &S = gptr(M.shade)[gh.geom];
   f3 P = add3(org, scl3(gh.t, dir));
-  f3 Ng, Ns, col = mk3(1.f, 1.f, 1.f);
+  f3 Ng = mk3(0.f, 0.f, 1.f), Ns = Ng, col = mk3(1.f, 1.f, 1.f);
   if (S.kind == 0) {
     const uint32_t *I = gptr(S.indices) + 3 * (int64_t)gh.prim;
     uint32_t i0 = I[0], i1 = I[1], i2 = I[2];
@@ -448,7 +451,7 @@ __device__ void geom_shade(const DModel &M, const GeoHit &gh, f3 org, f3 dir, Hi
     Ng = sub3(P, c);
     Ns = Ng;
     if (S.has_tf && S.has_data) col = tf_color(gptr(S.tf.col), S.tf, gptr(S.vdata)[gh.prim]);
-  } else {
+  } else if (W == 0) {
     // DataDrivenPathLines_postIntersect (DataDrivenPathLines.ispc:212-269):
     // the radius interpolated between the segment's first two control points
     // at the curve parameter, mapped back to a data value for the TF
@@ -561,7 +564,7 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
       if (gh.geom >= 0) {
         ray_t = gh.t;
         surface_hit = true;
-        if (shadeFlag) geom_shade(P.model, gh, org, dir, hit);
+        if (shadeFlag) geom_shade<W>(P.model, gh, org, dir, hit);
       }
     }
 
