@@ -362,6 +362,20 @@ def bench_served(L, hip, nreq, seed=5, layout=0, config=None, step_rays=4_000_00
     return res
 
 
+def valu_from_profile(name):
+    """(VALU instructions per wave, waves per launch) of the trace kernel from
+    the SQ counter line of the committed profile summary."""
+    p = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(p):
+        return None
+    import re
+    for line in open(p):
+        m = re.search(r"VALU instructions per wave ([0-9.]+), waves per launch ([0-9.]+)", line)
+        if m:
+            return float(m.group(1)), float(m.group(2))
+    return None
+
+
 def traffic_from_profile(name):
     """HBM bytes per trace launch measured by rocprofv3 --pmc (FETCH_SIZE and
     WRITE_SIZE, copy-kernel calibrated) and committed under profiles/."""
@@ -459,6 +473,14 @@ def main():
                     frac=round(tfl / FP32_PEAK_TFLOPS, 5),
                     note="FP32 VALU work (no MFMA on this path); MI355X FP32 peak")
     roof["traffic"] = traffic_from_profile("trace_c2_traffic.txt")
+    # the bound the march actually meets: VALU issue.  A wave64 VALU instruction
+    # occupies a 16-lane SIMD for 4 cycles; 1024 SIMDs at 2.4 GHz
+    vp = valu_from_profile("trace_c2_traffic.txt")
+    if vp and tr_n:
+        ms = tr_ms / tr_n
+        roof["valu_issue"] = dict(instr_per_launch=int(vp[0] * vp[1]),
+                                  frac=round(vp[0] * vp[1] * 4 / (1024 * 2.4e9 * ms * 1e-3), 4),
+                                  note="SQ_INSTS_VALU/SQ_WAVES from the committed profile x this run's launch time")
     kcfg = (C.c_int(), C.c_int(), C.c_int())
     L.gxy_get_trace_config(*[C.byref(x) for x in kcfg])
     lay = L.gxy_volume_get_layout(built.handles[0])
