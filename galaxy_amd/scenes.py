@@ -281,6 +281,49 @@ def resolve_lights(L, cam):
     return out
 
 
+def camera_from_json(c, width=512, height=512):
+    """One camera object of a state file or a viewer "Camera" command
+    (Camera.cpp LoadFromJSON: viewpoint, viewcenter or viewdirection, viewup,
+    aov)."""
+    eye = tuple(float(x) for x in c["viewpoint"])
+    if "viewcenter" in c:
+        center = tuple(float(x) for x in c["viewcenter"])
+    else:
+        center = tuple(eye[k] + float(c["viewdirection"][k]) for k in range(3))
+    return CameraDesc(eye, center, tuple(float(x) for x in c["viewup"]), float(c.get("aov", 30.0)), width, height)
+
+
+def vis_from_json(v, dsets, eps=0.001, cams=()):
+    """One Visualization object (operators + Lighting) over the datasets
+    `dsets` (name -> VolumeDesc) as a SceneDesc (Visualization.cpp
+    LoadFromJSON / local_commit)."""
+    sc = SceneDesc(epsilon=eps, cameras=list(cams))
+    sc.lighting = load_lighting(v.get("Lighting", {}))
+    first = True
+    for op in v.get("operators", []):
+        if op["type"] not in ("Volume", "VolumeVis"):
+            continue
+        vol = dsets[op["dataset"]]
+        vv = VolumeVisDesc(vol)
+        if "colormap" in op:
+            vv.colormap = np.array(op["colormap"], f32)
+        if "opacitymap" in op:
+            vv.opacitymap = np.array(op["opacitymap"], f32)
+        if "data range" in op:
+            vv.data_range = tuple(op["data range"])
+        vv.isovalues = np.array(op.get("isovalues", []), f32)
+        if "slices" in op:
+            vv.slices = np.array(op["slices"], f32).reshape(-1, 4)
+        elif "plane" in op:
+            vv.slices = np.array(op["plane"], f32).reshape(1, 4)
+        vv.volume_render = bool(op.get("volume rendering", False))
+        sc.volume_vis.append(vv)
+        if first:   # Visualization::local_commit takes the first vis's boxes (Visualization.cpp:150-162)
+            sc.local_box, sc.global_box, sc.neighbors = vol.local_box, vol.global_box, vol.neighbors
+            first = False
+    return sc
+
+
 def load_state(path_or_dict, volumes, width=512, height=512):
     """Loads a Galaxy state file.  `volumes` maps a dataset file name or
     dataset name to a VolumeDesc.  Returns a list of (SceneDesc, [CameraDesc])
@@ -291,44 +334,9 @@ def load_state(path_or_dict, volumes, width=512, height=512):
         key = d.get("filename", d["name"])
         vol = volumes.get(key, volumes.get(d["name"]))
         dsets[d["name"]] = vol
-    cams = []
-    for c in doc.get("Cameras", []):
-        eye = tuple(float(x) for x in c["viewpoint"])
-        if "viewcenter" in c:
-            center = tuple(float(x) for x in c["viewcenter"])
-        else:
-            center = tuple(eye[k] + float(c["viewdirection"][k]) for k in range(3))
-        cams.append(CameraDesc(eye, center, tuple(float(x) for x in c["viewup"]),
-                               float(c.get("aov", 30.0)), width, height))
+    cams = [camera_from_json(c, width, height) for c in doc.get("Cameras", [])]
     eps = float(doc.get("Renderer", {}).get("epsilon", 0.001))
-    out = []
-    for v in doc.get("Visualizations", []):
-        sc = SceneDesc(epsilon=eps, cameras=cams)
-        sc.lighting = load_lighting(v.get("Lighting", {}))
-        first = True
-        for op in v.get("operators", []):
-            if op["type"] not in ("Volume", "VolumeVis"):
-                continue
-            vol = dsets[op["dataset"]]
-            vv = VolumeVisDesc(vol)
-            if "colormap" in op:
-                vv.colormap = np.array(op["colormap"], f32)
-            if "opacitymap" in op:
-                vv.opacitymap = np.array(op["opacitymap"], f32)
-            if "data range" in op:
-                vv.data_range = tuple(op["data range"])
-            vv.isovalues = np.array(op.get("isovalues", []), f32)
-            if "slices" in op:
-                vv.slices = np.array(op["slices"], f32).reshape(-1, 4)
-            elif "plane" in op:
-                vv.slices = np.array(op["plane"], f32).reshape(1, 4)
-            vv.volume_render = bool(op.get("volume rendering", False))
-            sc.volume_vis.append(vv)
-            if first:   # Visualization::local_commit takes the first vis's boxes (Visualization.cpp:150-162)
-                sc.local_box, sc.global_box, sc.neighbors = vol.local_box, vol.global_box, vol.neighbors
-                first = False
-        out.append(sc)
-    return out
+    return [vis_from_json(v, dsets, eps, cams) for v in doc.get("Visualizations", [])]
 
 
 # --------------------------------------------------------------------- images
