@@ -7,9 +7,8 @@ import json
 import os
 
 import numpy as np
-import pytest
 
-from conftest import GOLDEN, gold_image, image_error
+from conftest import GOLDEN
 from galaxy_amd import frontend, scenes
 
 
