@@ -24,7 +24,7 @@ def smooth_volume(n, seed):
 
 def dataset_descs():
     d = np.linspace(0.0, 1.0, 8)
-    rgb, op = workloads.C2_TF
+    rgb, op = workloads.VOLUME_RENDER_TF
     out = []
     for k, n in enumerate((40, 56)):
         vol = smooth_volume(n, 10 + k)
