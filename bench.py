@@ -195,8 +195,8 @@ def bench_batched(L, hip, steps, tune=True, fixed=None):
     per surface hit: the batched form (one persistent launch per pass, passes
     captured in a HIP graph) against the per-list loop over the same lists."""
     from galaxy_amd import abi, batch as gb, lib as glib, scenes, workloads
-    descs, lists, Ld = workloads.c4()
-    log("building c4: %d datasets, %d lists, %d rays" % (len(descs), len(lists), sum(n for n, _, _ in lists)))
+    descs, lists, Ld = workloads.batched_lists()
+    log("building the batched lists: %d datasets, %d lists, %d rays" % (len(descs), len(lists), sum(n for n, _, _ in lists)))
     t = time.perf_counter()
     built = [scenes.build_scene(hip, workloads.dataset_scene(d)) for d in descs]
     glib.check(L.gxy_synchronize())
@@ -244,7 +244,7 @@ def bench_batched(L, hip, steps, tune=True, fixed=None):
         B.refresh_datasets(built)
         res["tuned"]["volume_layout"] = layout
         res["tuned"]["layout_ms"] = {str(k): round(v, 2) for k, v in lay_ms.items()}
-        log("c4 tuned", res["tuned"])
+        log("batched lists tuned", res["tuned"])
     st = B.run(2)                      # warm (graph captured)
     glib.check(L.gxy_synchronize())
     t = time.perf_counter()
@@ -398,7 +398,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
-    ap.add_argument("--c3-steps", type=int, default=3)
+    ap.add_argument("--icosphere-steps", type=int, default=3)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=3)
@@ -435,10 +435,10 @@ def main():
         b.free()
 
     # ---------------------------------------------------------------- c2
-    sc, alpha = workloads.c2()
+    sc, alpha = workloads.volume_render()
     cam = sc.cameras[0]
     glib.check(L.gxy_set_opacity_threshold(alpha))
-    log("building c2 (512^3 noise volume)")
+    log("building the volume render (512^3 noise volume)")
     built = scenes.build_scene(hip, sc)
     tuned = None
     if args.trace_config:
@@ -472,10 +472,10 @@ def main():
         roof = dict(bound="mfma", achieved=round(tfl, 4), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
                     frac=round(tfl / FP32_PEAK_TFLOPS, 5),
                     note="FP32 VALU work (no MFMA on this path); MI355X FP32 peak")
-    roof["traffic"] = traffic_from_profile("trace_c2_traffic.txt")
+    roof["traffic"] = traffic_from_profile("trace_volume_render_traffic.txt")
     # the bound the march actually meets: VALU issue.  A wave64 VALU instruction
     # occupies a 16-lane SIMD for 4 cycles; 1024 SIMDs at 2.4 GHz
-    vp = valu_from_profile("trace_c2_traffic.txt")
+    vp = valu_from_profile("trace_volume_render_traffic.txt")
     if vp and tr_n:
         ms = tr_ms / tr_n
         roof["valu_issue"] = dict(instr_per_launch=int(vp[0] * vp[1]),
@@ -520,8 +520,8 @@ def main():
     # ---------------------------------------------------------------- extras
     if not args.no_extra:
         extra = {}
-        for name, fn, steps in (("c1_sphere_mesh_512", workloads.c1, args.steps),
-                                ("c3_nine_icospheres_4k_ao16", workloads.c3, args.c3_steps),
+        for name, fn, steps in (("sphere_mesh_512", workloads.sphere_mesh, args.steps),
+                                ("nine_icospheres_4k_ao16", workloads.nine_icospheres, args.icosphere_steps),
                                 ("pathlines_4000x63_segments_1024", workloads.pathlines, args.steps)):
             try:
                 sc2, a2 = fn()
@@ -575,7 +575,7 @@ def main():
     if not args.no_cpu:
         try:
             log("cpu baseline")
-            sc, alpha = workloads.c2()
+            sc, alpha = workloads.volume_render()
             out["cpu_baseline"] = cpu_baseline(hip, sc, sc.cameras[0], alpha)
         except Exception as e:
             out["cpu_baseline"] = dict(error=str(e))

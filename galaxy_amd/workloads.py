@@ -1,17 +1,17 @@
 """The benchmark workloads (synthetic, seeded), as scene descriptions.
 
-  c1  512x512 pinhole camera, primary + one hard shadow ray per pixel from one
+  sphere_mesh        512x512 pinhole camera, primary + one hard shadow ray per pixel from one
       point light, one UV-sphere mesh (radius 1, 128x80 segments, 20,480 FP32
       triangles, Lambertian shading), camera on a +z orbit at distance 3
-  c2  1920x1080 rays through a 512^3 FP32 field of seeded 4-octave gradient
+  volume_render      1920x1080 rays through a 512^3 FP32 field of seeded 4-octave gradient
       noise in [0,1]; 8-point piecewise-linear RGBA transfer function; step 0.5
       voxel; front-to-back compositing with early termination at alpha 0.99;
       isosurface at 0.5 with shadow rays
-  c3  3x3 grid of icospheres at subdivision 8 (1,310,720 triangles each,
+  nine_icospheres    3x3 grid of icospheres at subdivision 8 (1,310,720 triangles each,
       11,796,480 total), 3840x2160 primary rays, 1 shadow ray and 16
       cosine-weighted AO rays per hit, every secondary ray spawned through the
       device RayList queue
-  c4  batched: 256 ray lists of log-uniform sizes in [1k, 1M] (origins uniform
+  batched_lists      256 ray lists of log-uniform sizes in [1k, 1M] (origins uniform
       in the unit cube, isotropic directions) over 64 resident datasets, each a
       random blob mesh of 50k-5M triangles or a 128^3-1024^3 FP32 noise volume
       (log-uniform sizes), traced together pass by pass with shadow and AO
@@ -36,7 +36,7 @@ def volume_boxes(n, origin, delta):
     return box, box
 
 
-def c2(n=512, width=1920, height=1080, seed=1234, octaves=4, base_frequency=4.0):
+def volume_render(n=512, width=1920, height=1080, seed=1234, octaves=4, base_frequency=4.0):
     """Volume render: noise volume + DVR + iso 0.5 + shadows."""
     delta = 2.0 / (n - 1)
     origin = (-1.0, -1.0, -1.0)
@@ -57,7 +57,7 @@ def c2(n=512, width=1920, height=1080, seed=1234, octaves=4, base_frequency=4.0)
     return scenes.SceneDesc(volume_vis=[vv], local_box=lbox, global_box=gbox, lighting=L, cameras=[cam]), 0.99
 
 
-def c1(width=512, height=512, seed=1):
+def sphere_mesh(width=512, height=512, seed=1):
     """Sphere mesh with one point light and hard shadows."""
     m = scenes.uv_sphere(1.0, 128, 80)
     m.colormap = np.array([[-1.0, 0.8, 0.3, 0.2], [1.0, 0.2, 0.5, 0.9]], f32)
@@ -71,7 +71,7 @@ def c1(width=512, height=512, seed=1):
     return scenes.SceneDesc(meshes=[m], local_box=box, global_box=box, lighting=L, cameras=[cam]), 0.999
 
 
-def c3(width=3840, height=2160, subdiv=8, n_ao=16, ao_radius=0.5):
+def nine_icospheres(width=3840, height=2160, subdiv=8, n_ao=16, ao_radius=0.5):
     """Nine icospheres, shadows and 16 AO rays per hit."""
     base = scenes.icosphere(subdiv, 0.45)
     meshes = []
@@ -88,7 +88,7 @@ def c3(width=3840, height=2160, subdiv=8, n_ao=16, ao_radius=0.5):
     return scenes.SceneDesc(meshes=meshes, local_box=box, global_box=box, lighting=L, cameras=[cam]), 0.999
 
 
-C2_TF = (np.array([[0.23, 0.30, 0.75], [0.34, 0.51, 0.95], [0.55, 0.69, 1.00], [0.75, 0.82, 0.93],
+VOLUME_RENDER_TF = (np.array([[0.23, 0.30, 0.75], [0.34, 0.51, 0.95], [0.55, 0.69, 1.00], [0.75, 0.82, 0.93],
                    [0.93, 0.82, 0.76], [0.97, 0.66, 0.54], [0.90, 0.45, 0.34], [0.71, 0.02, 0.15]]),
          np.array([0.02, 0.0, 0.04, 0.0, 0.0, 0.04, 0.0, 0.03]))
 
@@ -125,7 +125,7 @@ def unit_volume_scene(edge, seed, octaves=4, base_frequency=4.0):
     """A noise volume of edge^3 voxels spanning the unit cube, with the volume
     render's 8-point TF, iso 0.5 and compositing on; step 0.5 voxel."""
     d = np.linspace(0.0, 1.0, 8)
-    rgb, op = C2_TF
+    rgb, op = VOLUME_RENDER_TF
     delta = f32(1.0 / (edge - 1))
     vol = scenes.VolumeDesc(None, (f32(0.0),) * 3, (delta,) * 3, sampling_rate=0.5, local_box=UNIT_BOX,
                             global_box=UNIT_BOX, shape=(edge, edge, edge), noise=(seed, octaves, base_frequency))
@@ -145,7 +145,7 @@ def batch_lighting():
                                n_ao=2, ao_radius=0.1, shadows=True)
 
 
-def c4(nlists=256, ndatasets=64, rays=(1_000, 1_000_000), edges=(128, 1024), tris=(50_000, 5_000_000), seed=4):
+def batched_lists(nlists=256, ndatasets=64, rays=(1_000, 1_000_000), edges=(128, 1024), tris=(50_000, 5_000_000), seed=4):
     """Returns (dataset scene descriptions, [(rays, dataset, seed), ...] per
     list, lighting).  Dataset k is a volume when k is even, a blob mesh when
     odd; sizes and list sizes are log-uniform in the given ranges."""

@@ -670,7 +670,7 @@ template <typename Acc>
 struct Col4 { Acc x, y, z, w; };
 
 // opaque test of TraceRays.ispc:466,550,581
-float g_opaque_alpha = 0.999f;  // reference value; the C2 workload uses 0.99
+float g_opaque_alpha = 0.999f;  // reference value; the 1920x1080 512^3 volume render uses 0.99
 
 template <typename Acc>
 inline bool opaque_of(const Col4<Acc> &c) {

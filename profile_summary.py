@@ -31,7 +31,7 @@ def main():
             lines.append("| `%s` | %s | %.3f | %.1f | %.1f |" % (r["Name"][:70], r["Calls"],
                                                                 float(r["TotalDurationNs"]) / 1e6,
                                                                 float(r["AverageNs"]) / 1e3, float(r["Percentage"])))
-    open(os.path.join(DST, "trace_c2_kernels.md"), "w").write("\n".join(lines) + "\n")
+    open(os.path.join(DST, "trace_volume_render_kernels.md"), "w").write("\n".join(lines) + "\n")
     f, w = rows("fetch"), rows("write")
     cf = max(float(r["Counter_Value"]) for r in f if "copyBuffer" in r["Kernel_Name"])
     cw = max(float(r["Counter_Value"]) for r in w if "copyBuffer" in r["Kernel_Name"])
@@ -66,7 +66,7 @@ def main():
         "waves per launch %.0f" % (util, avg(sq["SQ_INSTS_VALU"]) / max(1, avg(sq["SQ_WAVES"])), avg(sq["SQ_WAVES"])),
         "L2 (TCC) hit rate %.3f" % hit,
     ]
-    open(os.path.join(DST, "trace_c2_traffic.txt"), "w").write("\n".join(lines) + "\n")
+    open(os.path.join(DST, "trace_volume_render_traffic.txt"), "w").write("\n".join(lines) + "\n")
     print("\n".join(lines))
 
 

@@ -18,7 +18,7 @@ def test_full_size_volume_frame_matches_oracle_on_sampled_rows(hip, oracle):
     from galaxy_amd import lib as glib
     from oracle_backend import lib as olib
     L = glib.lib()
-    sc, alpha = workloads.c2()
+    sc, alpha = workloads.volume_render()
     cam = sc.cameras[0]
     saved = L.gxy_get_opacity_threshold()
     glib.check(L.gxy_set_opacity_threshold(alpha))

@@ -213,7 +213,7 @@ def test_geometry_meshes_and_spheres(hip, oracle):
 
 def test_mesh_frame_with_shadows_matches_oracle(hip, oracle):
     from galaxy_amd import workloads
-    sc, alpha = workloads.c1(256, 256)
+    sc, alpha = workloads.sphere_mesh(256, 256)
     c = sc.cameras[0]
     fb_g, st_g = hip.render_frame(scenes.build_scene(hip, sc), c)
     fb_o, st_o = oracle.render_frame(scenes.build_scene(oracle, sc), c)

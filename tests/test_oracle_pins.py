@@ -287,9 +287,9 @@ def test_triangle_hits_against_fp64_bruteforce(oracle):
 
 def test_sphere_hit_closed_form(oracle):
     M = oracle.model_create()
-    C3 = np.array([[0.0, 0.0, 0.0], [3.0, 0.0, 0.0]], f32)
+    cen = np.array([[0.0, 0.0, 0.0], [3.0, 0.0, 0.0]], f32)
     D = np.array([0.0, 1.0], f32)
-    oracle.model_add_spheres(M, C3, D, 0.5, 1.0, 0.0, 1.0, None)   # radii 0.5 and 1.0 by data
+    oracle.model_add_spheres(M, cen, D, 0.5, 1.0, 0.0, 1.0, None)   # radii 0.5 and 1.0 by data
     oracle.model_commit(M, True)
     org = np.array([[0, 0, 5], [3, 0, 5], [0, 0, 0]], f32)
     d = np.array([[0, 0, -1], [0, 0, -1], [0, 0, 1]], f32)
