@@ -141,7 +141,7 @@ struct TraceParams {
   int nvv;
   int integrate;
   int has_model;
-  int pad_;
+  int occlusion;       // 1: non-primary rays end their geometry search at the first hit (no volumes)
   float blo[3], bhi[3];
   float step, eps;
   float istep;         // 1 / step

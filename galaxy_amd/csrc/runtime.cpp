@@ -89,6 +89,7 @@ struct Runtime {
   // volume-only trace kernel: 0 one ray per work-item, 1 persistent lanes
   int trace_kind = 0, trace_chunk = 256, trace_refill = 32, trace_wpe = 0;
   int geom_stack = 0;           // BVH traversal stack of geometry-only traces (0 private, or LDS entries)
+  int occlusion = 1;            // rank loops: shadow/AO rays of volume-free scenes stop at their first hit
   int bvh_width = 4;            // traverse the 4-wide trees (4) or the binary ones (2)
   float *uv = nullptr;          // AO direction tables U[256], V[256] (device)
 };
@@ -630,6 +631,12 @@ GXY_EXPORT int gxy_set_geometry_stack(int entries) {
   return 0;
 }
 GXY_EXPORT int gxy_get_geometry_stack(void) { return G.geom_stack; }
+GXY_EXPORT int gxy_set_occlusion_rays(int on) {
+  INIT_OR(-1);
+  G.occlusion = on ? 1 : 0;
+  return 0;
+}
+GXY_EXPORT int gxy_get_occlusion_rays(void) { return G.occlusion; }
 GXY_EXPORT int gxy_set_bvh_width(int width) {
   if (width != 2 && width != 4) return fail("BVH width %d: 2 or 4", width);
   G.bvh_width = width;
@@ -2273,6 +2280,8 @@ static int run_loop(LoopCtx &C, int n, const RayList_ispc &rays, RayListObj *ado
 static int loop_setup(LoopCtx &C, void *vis, void *lights, const int nb[6], float eps, int maxlist) {
   memset(&C.st, 0, sizeof C.st);
   if (make_trace_params((const Visualization_ispc *)vis, eps, C.P) != 0) return -1;
+  // the loop reads only the classification of shadow and AO rays, never their t
+  C.P.occlusion = (G.occlusion && C.P.nvv == 0) ? 1 : 0;
   C.box = ((const Visualization_ispc *)vis)->local_bb;
   if (lights) {
     if (make_light_params((const Lighting_ispc *)lights, 0.001f, C.L) != 0) return -1;
@@ -2793,6 +2802,7 @@ static int batch_dataset_params(BatchObj *B, const Visualization_ispc *vis, floa
                 P.nvv, P.has_model ? " and a model" : "");
   if (geometry && P.model.ncurve > 0)
     return fail("a batch dataset with path lines: path lines are traced by the per-list loop only");
+  P.occlusion = (G.occlusion && geometry) ? 1 : 0;
   P.hit_geom = nullptr;
   P.hit_prim = nullptr;
   P.sample_partials = nullptr;

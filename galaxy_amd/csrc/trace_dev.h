@@ -220,9 +220,10 @@ __device__ __forceinline__ bool child_hit(const float *lo, const float *hi, f3 o
   return tn <= tf * 1.0000005f + 1e-30f;
 }
 
+// any: the walk ends at the first leaf that records a hit (occlusion query).
 template <class Stack, class LeafFn>
 __device__ __forceinline__ void bvh_walk(const BVHNode *nodes, f3 o, f3 inv, float t0, const GeoHit &h,
-                                         Stack &stack, LeafFn &&leaf) {
+                                         Stack &stack, LeafFn &&leaf, bool any = false) {
   int sp = 0;
   stack.put(sp++, GXY_SENTINEL);
   int cur = 0;
@@ -251,6 +252,7 @@ __device__ __forceinline__ void bvh_walk(const BVHNode *nodes, f3 o, f3 inv, flo
     } else {
       int code = ~cur;
       leaf(code >> 4, (code & 15) + 1);
+      if (any && h.geom >= 0) return;
       cur = stack.get(--sp);
     }
   }
@@ -265,7 +267,7 @@ __device__ __forceinline__ void bvh_walk(const BVHNode *nodes, f3 o, f3 inv, flo
 // level (depth4 entries).
 template <class Stack, class LeafFn>
 __device__ __forceinline__ void bvh4_walk(const BVH4Node *nodes, f3 o, f3 inv, float t0, const GeoHit &h,
-                                          Stack &stack, LeafFn &&leaf) {
+                                          Stack &stack, LeafFn &&leaf, bool any = false) {
   int sp = 0;
   int cur = 0, mask = 0xF;
   const BVH4Node *gn = gptr(nodes);
@@ -293,6 +295,7 @@ __device__ __forceinline__ void bvh4_walk(const BVH4Node *nodes, f3 o, f3 inv, f
         }
       }
     }
+    if (any && h.geom >= 0) return;
     // the nearest inner child whose box still starts before the current hit
     int kn = -1;
     float tn = 0.f;
@@ -322,8 +325,9 @@ __device__ __forceinline__ void bvh4_walk(const BVH4Node *nodes, f3 o, f3 inv, f
 // W = 2 or 4: the tree the walk uses; W = 0: chosen per call from M.wide
 // (both walks compiled into the caller).
 template <class Stack, int W = 0>
+// any: an occlusion query; the search ends at the first hit in (t0, tfar].
 __device__ __forceinline__ void model_trace(const DModel &M, f3 org, f3 dir, float t0, float tfar, GeoHit &h,
-                                            Stack &stack) {
+                                            Stack &stack, bool any = false) {
   h.t = tfar;
   h.geom = -1;
   h.prim = -1;
@@ -343,9 +347,10 @@ __device__ __forceinline__ void model_trace(const DModel &M, f3 org, f3 dir, flo
         }
       }
     };
-    if (W == 4 || (W == 0 && M.wide)) bvh4_walk(M.tnodes4, org, inv, t0, h, stack, tleaf);
-    else bvh_walk(M.tnodes, org, inv, t0, h, stack, tleaf);
+    if (W == 4 || (W == 0 && M.wide)) bvh4_walk(M.tnodes4, org, inv, t0, h, stack, tleaf, any);
+    else bvh_walk(M.tnodes, org, inv, t0, h, stack, tleaf, any);
   }
+  if (any && h.geom >= 0) return;
   if (M.nsph > 0) {
     auto sleaf = [&](int start, int count) {
       for (int k = start; k < start + count; k++) {
@@ -359,9 +364,10 @@ __device__ __forceinline__ void model_trace(const DModel &M, f3 org, f3 dir, flo
         }
       }
     };
-    if (W == 4 || (W == 0 && M.wide)) bvh4_walk(M.snodes4, org, inv, t0, h, stack, sleaf);
-    else bvh_walk(M.snodes, org, inv, t0, h, stack, sleaf);
+    if (W == 4 || (W == 0 && M.wide)) bvh4_walk(M.snodes4, org, inv, t0, h, stack, sleaf, any);
+    else bvh_walk(M.snodes, org, inv, t0, h, stack, sleaf, any);
   }
+  if (any && h.geom >= 0) return;
   if (W == 0 && M.ncurve > 0) {
     // path-line pieces, in the per-model variant only (models with path lines
     // are dispatched to it); h.u holds the hit piece's first curve parameter
@@ -381,8 +387,8 @@ __device__ __forceinline__ void model_trace(const DModel &M, f3 org, f3 dir, flo
         }
       }
     };
-    if (M.wide) bvh4_walk(M.cnodes4, org, inv, t0, h, stack, cleaf);
-    else bvh_walk(M.cnodes, org, inv, t0, h, stack, cleaf);
+    if (M.wide) bvh4_walk(M.cnodes4, org, inv, t0, h, stack, cleaf, any);
+    else bvh_walk(M.cnodes, org, inv, t0, h, stack, cleaf, any);
   }
 }
 
@@ -559,7 +565,11 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
     // LookForGeometryHit (TraceRays.ispc:108-141)
     if (GEOM) {
       GeoHit gh;
-      model_trace<Stack, W>(P.model, org, dir, ray_t0, ray_t, gh, stack);
+      // a shadow or AO ray in a scene without volumes needs only whether it
+      // is blocked (any hit terminates it with its own colour) when the host
+      // does not read its t (the rank loops; not TraceRays_TraceRays)
+      const bool any = NV == 0 && P.occlusion && !shadeFlag;
+      model_trace<Stack, W>(P.model, org, dir, ray_t0, ray_t, gh, stack, any);
       if (P.hit_geom) { P.hit_geom[i] = gh.geom; P.hit_prim[i] = gh.prim; }
       if (gh.geom >= 0) {
         ray_t = gh.t;

@@ -68,6 +68,8 @@ def _declare(L):
         "gxy_model_destroy": (C.c_int, [vp]),
         "gxy_model_add_triangles": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, vp, vp, vp]),
         "gxy_model_add_spheres": (C.c_int, [vp, vp, vp, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, vp]),
+        "gxy_set_occlusion_rays": (C.c_int, [C.c_int]),
+        "gxy_get_occlusion_rays": (C.c_int, []),
         "gxy_model_add_pathlines": (C.c_int, [vp, vp, vp, C.c_int, vp, C.c_int, C.c_float, C.c_float, C.c_float,
                                               C.c_float, vp]),
         "gxy_model_commit": (C.c_int, [vp]),

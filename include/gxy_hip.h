@@ -187,6 +187,14 @@ int         gxy_set_trace_config(int kind, int chunk, int refill);
  * stack).  Results are identical for every setting. */
 int         gxy_set_geometry_stack(int entries);
 int         gxy_get_geometry_stack(void);
+/* Occlusion rays (default 1): in the rank loops (gxy_render_frame,
+ * gxy_process_raylist, the batched and served forms) a shadow or AO ray of a
+ * scene without volumes ends its geometry search at its first hit: Classify
+ * terminates it for any hit and its colour does not depend on which, and the
+ * loops never read its t.  TraceRays_TraceRays always finds the closest hit.
+ * 0: closest hit everywhere (same frames and counts). */
+int gxy_set_occlusion_rays(int on);
+int gxy_get_occlusion_rays(void);
 /* BVH traversed by geometry traces (no reference counterpart): 4 = the
  * 4-wide trees every model commit also builds (128-byte nodes, about half the
  * depth), 2 = the binary LBVH.  Hits are identical either way. */

@@ -422,3 +422,25 @@ def test_every_volume_layout_gives_identical_rays(radial_vols_small, hip):
         for f in abi.FIELDS:
             assert np.array_equal(g[f], ref[0][f]), (lay, f)
         assert np.array_equal(parts, ref[1]) and st.samples == ref[2], lay
+
+
+def test_occlusion_rays_give_the_same_frames(hip):
+    """Shadow and AO rays of a volume-free scene stop at their first hit in
+    the rank loop (gxy_set_occlusion_rays): the frame and every count equal
+    the closest-hit loop's."""
+    from galaxy_amd import lib as glib
+    from galaxy_amd import workloads
+    L_ = glib.lib()
+    sc, alpha = workloads.nine_icospheres(480, 270, subdiv=5, n_ao=4, ao_radius=0.5)
+    b = scenes.build_scene(hip, sc)
+    c = sc.cameras[0]
+    out = []
+    try:
+        for on in (0, 1):
+            glib.check(L_.gxy_set_occlusion_rays(on))
+            out.append(hip.render_frame(b, c))
+    finally:
+        glib.check(L_.gxy_set_occlusion_rays(1))
+    (fb0, st0), (fb1, st1) = out
+    assert st0.as_dict() == st1.as_dict() and st0.secondary_rays > 100000
+    assert np.abs(fb0 - fb1).max() <= 1e-6
