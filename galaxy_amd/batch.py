@@ -88,6 +88,10 @@ class Batch:
         glib.check(self.L.gxy_batch_get_config(self.h, *[C.byref(x) for x in v]), "batch_get_config")
         return tuple(x.value for x in v) + (self.L.gxy_batch_get_split(self.h),)
 
+    def set_max_segments(self, n):
+        """Isovalue crossings a sampler ray follows within one pass (1..4096)."""
+        glib.check(self.L.gxy_batch_set_max_segments(self.h, int(n)), "batch_set_max_segments")
+
     def set_step_rays(self, rays):
         """Primary rays of added lists per step (0 = all); fewest left first."""
         glib.check(self.L.gxy_batch_set_step_rays(self.h, int(rays)), "batch_set_step_rays")

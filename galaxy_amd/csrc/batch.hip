@@ -26,7 +26,6 @@ namespace gxy {
 #define BWAVES (GXY_BLOCK / 64)
 // isovalue crossings a sampler ray follows within one pass before it is
 // re-queued as a KEEP_HERE ray
-#define BATCH_MAX_SEGMENTS 4096
 
 __device__ __forceinline__ unsigned long long bl_lanemask_lt() {
   const int lane = threadIdx.x & 63;
@@ -191,9 +190,9 @@ void k_btrace(BatchArgs A) {
       const int cap = Ls.pcap;
       unsigned segs = 0;
       for (int k = lane; k < ch.count; k += 64) {
-        if (lay == 8) acc[BS_SAMPLES] += sample_all<1, 8>(P, R, ch.start + k, xyz, pc, cap, BATCH_MAX_SEGMENTS, segs);
-        else if (lay == 1) acc[BS_SAMPLES] += sample_all<1, 1>(P, R, ch.start + k, xyz, pc, cap, BATCH_MAX_SEGMENTS, segs);
-        else acc[BS_SAMPLES] += sample_all<1, 0>(P, R, ch.start + k, xyz, pc, cap, BATCH_MAX_SEGMENTS, segs);
+        if (lay == 8) acc[BS_SAMPLES] += sample_all<1, 8>(P, R, ch.start + k, xyz, pc, cap, A.max_segments, segs);
+        else if (lay == 1) acc[BS_SAMPLES] += sample_all<1, 1>(P, R, ch.start + k, xyz, pc, cap, A.max_segments, segs);
+        else acc[BS_SAMPLES] += sample_all<1, 0>(P, R, ch.start + k, xyz, pc, cap, A.max_segments, segs);
         acc[BS_RAYS] += segs;
       }
     } else if (P.has_model) {
@@ -277,9 +276,9 @@ void k_btrace_kind(BatchArgs A) {
       const int cap = Ls.pcap;
       unsigned segs = 0;
       for (int k = lane; k < ch.count; k += 64) {
-        if (lay == 8) acc[BS_SAMPLES] += sample_all<1, 8>(P, R, ch.start + k, xyz, pc, cap, BATCH_MAX_SEGMENTS, segs);
-        else if (lay == 1) acc[BS_SAMPLES] += sample_all<1, 1>(P, R, ch.start + k, xyz, pc, cap, BATCH_MAX_SEGMENTS, segs);
-        else acc[BS_SAMPLES] += sample_all<1, 0>(P, R, ch.start + k, xyz, pc, cap, BATCH_MAX_SEGMENTS, segs);
+        if (lay == 8) acc[BS_SAMPLES] += sample_all<1, 8>(P, R, ch.start + k, xyz, pc, cap, A.max_segments, segs);
+        else if (lay == 1) acc[BS_SAMPLES] += sample_all<1, 1>(P, R, ch.start + k, xyz, pc, cap, A.max_segments, segs);
+        else acc[BS_SAMPLES] += sample_all<1, 0>(P, R, ch.start + k, xyz, pc, cap, A.max_segments, segs);
         acc[BS_RAYS] += segs;
       }
     } else if (KIND == 1) {

@@ -144,6 +144,7 @@ struct BatchArgs {
   int *kcount;                // 3 counts
   int *khead;                 // 3 work-queue heads
   int kcap;
+  int max_segments;           // sampler rays: isovalue crossings followed per pass (the rest re-queued)
   LightParams L;
 };
 

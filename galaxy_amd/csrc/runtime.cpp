@@ -2436,6 +2436,7 @@ struct BatchObj {
   int pass = 0;
   long long queued = 0;              // rays queued for the next pass (host view)
   long long step_rays = 0;           // primary rays of added lists per step (0 = all)
+  int max_segments = 4096;           // sampler rays: crossings followed per pass
   int passes_run = 0;
   bool dirty_tables = true;          // list / dataset tables need upload
   bool dirty_graph = true;
@@ -2565,6 +2566,7 @@ static BatchArgs batch_args(BatchObj *B, int pass) {
   A.kcount = B->ints + 8;
   A.khead = B->ints + 11;
   A.kcap = B->kcap;
+  A.max_segments = B->max_segments;
   A.L = B->L;
   return A;
 }
@@ -2651,7 +2653,8 @@ static std::vector<uint64_t> batch_graph_sig(BatchObj *B, int graph_passes) {
                              (uint64_t)(uintptr_t)B->d_ds, (uint64_t)(uintptr_t)B->d_lists,
                              (uint64_t)(uintptr_t)B->d_rls, (uint64_t)(uintptr_t)B->part, (uint64_t)B->part_blocks,
                              (uint64_t)(uintptr_t)B->lpend, (uint64_t)B->chunk, (uint64_t)B->stack,
-                             (uint64_t)B->wpe, (uint64_t)B->bpc, (uint64_t)B->lists.size()};
+                             (uint64_t)B->wpe, (uint64_t)B->bpc, (uint64_t)B->lists.size(),
+                             (uint64_t)B->max_segments};
   for (int k = 0; k < 2; k++) {
     v.push_back((uint64_t)(uintptr_t)B->pool[k]);
     v.push_back(B->pool[k] ? (uint64_t)(uintptr_t)B->pool[k]->block : 0);
@@ -2945,6 +2948,15 @@ GXY_EXPORT int gxy_batch_set_step_rays(void *b, long long rays) {
   if (!B) return fail("not a batch handle");
   if (rays < 0) return fail("step budget %lld", rays);
   B->step_rays = rays;
+  return 0;
+}
+GXY_EXPORT int gxy_batch_set_max_segments(void *b, int n) {
+  INIT_OR(-1);
+  BatchObj *B = lookup<BatchObj>(g_batches, b);
+  if (!B) return fail("not a batch handle");
+  if (n < 1 || n > 4096) return fail("max segments %d: 1..4096", n);
+  B->max_segments = n;
+  B->dirty_graph = true;
   return 0;
 }
 GXY_EXPORT long long gxy_batch_get_step_rays(void *b) {

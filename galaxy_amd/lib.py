@@ -108,6 +108,7 @@ def _declare(L):
         "gxy_batch_set_split": (C.c_int, [vp, C.c_int]),
         "gxy_batch_get_split": (C.c_int, [vp]),
         "gxy_batch_set_step_rays": (C.c_int, [vp, C.c_longlong]),
+        "gxy_batch_set_max_segments": (C.c_int, [vp, C.c_int]),
         "gxy_batch_get_step_rays": (C.c_longlong, [vp]),
         "gxy_batch_run": (C.c_int, [vp, C.c_int, vp]),
         "gxy_batch_step": (C.c_int, [vp, vp]),

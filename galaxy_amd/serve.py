@@ -49,6 +49,10 @@ class Server:
         (0 = all of them): the requests with the fewest rays left go first."""
         glib.check(self.L.gxy_batch_set_step_rays(self.L.gxy_server_batch(self.h), int(rays)), "batch_set_step_rays")
 
+    def set_max_segments(self, n):
+        """Isovalue crossings a sampler ray follows within one step (1..4096)."""
+        glib.check(self.L.gxy_batch_set_max_segments(self.L.gxy_server_batch(self.h), int(n)), "set_max_segments")
+
     def run(self, reqs, mode=abi.SERVE_CONTINUOUS):
         """reqs: a ctypes array of abi.Request sorted by arrival.  Returns
         (results array, ServeStats)."""

@@ -459,6 +459,10 @@ int   gxy_batch_step(void *batch, long long *next_rays);
  * steps, so a step stays short and a list added meanwhile starts at the next
  * step.  Results do not depend on the budget. */
 int   gxy_batch_set_step_rays(void *batch, long long rays);
+/* Sample lists: isovalue crossings one ray follows within a pass (1..4096,
+ * default 4096); a ray with more continues in the next pass.  Results do not
+ * depend on it; a small value shortens the longest rays of a pass. */
+int   gxy_batch_set_max_segments(void *batch, int segments);
 long long gxy_batch_get_step_rays(void *batch);
 /* per_list[l] = rays of list l queued for the next pass, with its primary
  * rays still waiting for a step (l < n). */

@@ -264,3 +264,38 @@ def test_bricked_volume_datasets_give_the_same_results(datasets, hip):
     finally:
         for v in vols:
             L.gxy_volume_set_layout(v, 0)
+
+
+def test_sampler_segment_cap_gives_the_same_particles(datasets, hip):
+    """A sampler ray that reaches the per-pass crossing cap continues in the
+    next pass: particle sets, ray passes and samples do not depend on the cap."""
+    from galaxy_amd import batch as gb
+    descs, built = datasets
+    ref = None
+    for cap_seg in (4096, 3, 1):
+        B = gb.Batch(128)
+        B.set_max_segments(cap_seg)
+        ids = [B.add_dataset(b) for b in built]
+        keep = []
+        B.set_lighting(lights_struct(keep))
+        lists = []
+        for k, n in enumerate([5000, 20000]):
+            d = gb.random_rays(n, 1900 + k, width=n)
+            l, xyz, cnt = B.add_sample_list(ids[k % 2], d, 400000)
+            B.set_seed(l, 1900 + k)
+            lists.append((xyz, cnt))
+        st = B.run(2)
+        got = []
+        for xyz, cnt in lists:
+            c = int(cnt.to_host(np.int32, 1)[0])
+            a = xyz.to_host(f32, 3 * c).reshape(-1, 3)
+            got.append(a[np.lexsort(a.T)])
+        if ref is None:
+            ref = (got, st.rays, st.samples, st.passes)
+            assert st.passes == 1
+        else:
+            assert st.rays == ref[1] and st.samples == ref[2], cap_seg
+            assert st.passes > ref[3], cap_seg
+            for a, b in zip(got, ref[0]):
+                assert np.array_equal(a, b), cap_seg
+        B.destroy()
