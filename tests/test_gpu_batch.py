@@ -291,11 +291,12 @@ def test_sampler_segment_cap_gives_the_same_particles(datasets, hip):
             a = xyz.to_host(f32, 3 * c).reshape(-1, 3)
             got.append(a[np.lexsort(a.T)])
         if ref is None:
-            ref = (got, st.rays, st.samples, st.passes)
-            assert st.passes == 1
+            ref = (got, st.rays, st.samples, st.passes)     # the graph's two passes, the second empty
         else:
             assert st.rays == ref[1] and st.samples == ref[2], cap_seg
-            assert st.passes > ref[3], cap_seg
+            assert st.passes >= ref[3], cap_seg
+            if cap_seg == 1:
+                assert st.passes > ref[3]
             for a, b in zip(got, ref[0]):
                 assert np.array_equal(a, b), cap_seg
         B.destroy()
