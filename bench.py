@@ -307,7 +307,8 @@ def bench_served(L, hip, nreq, seed=5, layout=0, config=None, step_rays=4_000_00
     reqs, _ = serve.request_stream(nreq, 200.0, 0.5, rng, volume_ds=(0, 1, 2), any_ds=(0, 1, 2, 3, 4, 5))
     # launch configuration for this mix: the batched form's choice against the split kernels, timed on a
     # burst of the stream's first 12 requests arriving at once
-    cands = ([tuple(config)] if config is not None else []) + [(256, 0, 0, 4, 1), (256, 0, 5, 4, 1)]
+    cands = ([tuple(config)] if config is not None else []) + [(256, 0, 0, 4, 1), (256, 0, 5, 4, 1),
+                                                               (128, 0, 5, 4, 1), (64, 0, 5, 4, 1)]
     burst = (abi.Request * 12)()
     for k in range(12):
         burst[k] = reqs[k]
