@@ -305,8 +305,10 @@ def bench_served(L, hip, nreq, seed=5, layout=0, config=None, step_rays=4_000_00
     build_s = time.perf_counter() - t
     rng = np.random.default_rng(seed)
     reqs, _ = serve.request_stream(nreq, 200.0, 0.5, rng, volume_ds=(0, 1, 2), any_ds=(0, 1, 2, 3, 4, 5))
-    # launch configuration for this mix: the batched form's choice against the split kernels, timed on a
-    # burst of the stream's first 12 requests arriving at once
+    # launch configuration for this mix: the batched form's choice against the split kernels and smaller
+    # chunks, timed on a burst of the stream's first 12 requests arriving at once, served in steps
+    # continuous batching steps take at most step_rays primary rays of admitted requests, fewest left first
+    S.set_step_rays(step_rays)
     cands = ([tuple(config)] if config is not None else []) + [(256, 0, 0, 4, 1), (256, 0, 5, 4, 1),
                                                                (128, 0, 5, 4, 1), (64, 0, 5, 4, 1)]
     burst = (abi.Request * 12)()
@@ -320,8 +322,6 @@ def bench_served(L, hip, nreq, seed=5, layout=0, config=None, step_rays=4_000_00
         tune_ms.append((c, stb.wall * 1e3))
     best = min(tune_ms, key=lambda x: x[1])[0]
     S.set_config(*best)
-    # continuous batching steps take at most step_rays primary rays of admitted requests, fewest left first
-    S.set_step_rays(step_rays)
     nrender = sum(1 for r in reqs if r.kind == 0)
     res = dict(requests=nreq, render=nrender, sample=nreq - nrender, arrival_rate=200.0,
                arrival_span_s=round(reqs[nreq - 1].arrival, 3), budget_bytes=160 << 30, build_s=round(build_s, 2),
