@@ -1157,8 +1157,11 @@ GXY_EXPORT int gxy_model_add_pathlines(void *model, const float *V, const float 
   P.nseg = nseg;
   P.CP = upload(cp.data(), cp.size());
   P.IC = upload(ic.data(), ic.size());
-  if (!P.CP || !P.IC) return -1;
-  HIPCHK(hipStreamSynchronize(S()));
+  if (!P.CP || !P.IC || hipStreamSynchronize(S()) != hipSuccess) {
+    dfree(P.CP);
+    dfree(P.IC);
+    return fail("path lines: device upload failed");
+  }
   M->paths.push_back(std::move(P));
   M->free_built();
   return M->paths.back().geomID;
@@ -1595,15 +1598,22 @@ GXY_EXPORT int gxy_model_commit(void *model) {
     HIPCHK(build_lbvh(dbox, (int)ncur, dalloc, dfree, &B, s));
     if (!B.nodes) return fail("model commit: out of device memory");
     std::vector<int> order(ncur);
-    HIPCHK(hipMemcpyAsync(order.data(), B.order, 4 * (size_t)ncur, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    const hipError_t e1 = hipMemcpyAsync(order.data(), B.order, 4 * (size_t)ncur, hipMemcpyDeviceToHost, s);
+    const hipError_t e2 = hipStreamSynchronize(s);
     dfree(dbox);
     dfree(B.order);
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+      dfree(B.nodes);
+      return fail("curve BVH: %s", hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+    }
     std::vector<float> sa(4 * ncur), sb(4 * ncur), su(2 * ncur);
     std::vector<int> sg(ncur), sp(ncur);
     for (int64_t k = 0; k < ncur; k++) {
       const int o = order[k];
-      if (o < 0 || o >= ncur) return fail("curve BVH order entry %d out of range", o);
+      if (o < 0 || o >= ncur) {
+        dfree(B.nodes);
+        return fail("curve BVH order entry %d out of range", o);
+      }
       for (int c = 0; c < 4; c++) { sa[4 * k + c] = fa[4 * (size_t)o + c]; sb[4 * k + c] = fb[4 * (size_t)o + c]; }
       su[2 * k] = fu[2 * (size_t)o];
       su[2 * k + 1] = fu[2 * (size_t)o + 1];
@@ -1612,8 +1622,9 @@ GXY_EXPORT int gxy_model_commit(void *model) {
     }
     float *da = upload(sa.data(), sa.size()), *db = upload(sb.data(), sb.size()), *du = upload(su.data(), su.size());
     int *dg = upload(sg.data(), sg.size()), *dp = upload(sp.data(), sp.size());
-    if (!da || !db || !du || !dg || !dp) return fail("model commit: out of device memory");
+    // owned from here on, so that the model frees them on any failure below
     M->owned.insert(M->owned.end(), {(void *)B.nodes, (void *)da, (void *)db, (void *)du, (void *)dg, (void *)dp});
+    if (!da || !db || !du || !dg || !dp) return fail("model commit: out of device memory");
     M->d.cnodes = B.nodes;
     M->d.ca = da;
     M->d.cb = db;
