@@ -28,6 +28,7 @@ struct RouteParams {
   float *fb;
   int width, height;
   int mode;
+  int unique;   // 1: no two rays of the list carry the same pixel (plain add, no atomics)
 };
 
 struct CameraParams {

@@ -41,7 +41,7 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_route(RouteParams P, RayList_ispc
       ao = R.o[i];
     }
   }
-  if (P.mode & RM_ACCUMULATE) fb_accumulate_wave(P.fb, P.width, P.height, acc, ax, ay, ar, ag, ab, ao);
+  if (P.mode & RM_ACCUMULATE) fb_accumulate_wave(P.fb, P.width, P.height, acc, ax, ay, ar, ag, ab, ao, P.unique != 0);
 }
 
 hipError_t launch_route(const RouteParams &P, const RayList_ispc &R, int n, hipStream_t s) {

@@ -52,10 +52,22 @@ __device__ __forceinline__ int exit_face(const float *lo, const float *hi, float
 // Lanes without a pixel (key -1) form runs of their own that are never
 // written; equal pixels in separate runs are added separately.  All 64 lanes
 // must call it.
+// unique: no other ray of the launch adds to the same pixel, so a plain
+// read-add-write gives the atomic's sum (one addition per pixel).
 __device__ __forceinline__ void fb_accumulate_wave(float *fb, int W, int H, bool f, int x, int y, float r, float g,
-                                                   float b, float a) {
+                                                   float b, float a, bool unique = false) {
   const int lane = threadIdx.x & 63;
   const int key = (f && x >= 0 && y >= 0 && x < W && y < H) ? y * W + x : -1;
+  if (unique) {
+    if (key >= 0) {
+      float *p = fb + 4 * (int64_t)key;
+      p[0] = p[0] + r;
+      p[1] = p[1] + g;
+      p[2] = p[2] + b;
+      p[3] = p[3] + a;
+    }
+    return;
+  }
   if (!__any(key >= 0)) return;
   // end: the window [lane, lane + off) summed so far reaches the end of the run
   const int nkey = __shfl_down(key, 1, 64);

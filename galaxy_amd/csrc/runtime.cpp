@@ -347,6 +347,10 @@ struct ModelObj {
 struct RayListObj {
   void *block = nullptr;
   size_t bytes = 0;
+  // what the rank loop knows of the lists it makes itself (0 for others):
+  // kinds 1 every ray primary, 2 none primary; unique: each pixel at most once
+  int kinds = 0;
+  int unique = 0;
   gxy_raylist_hdr hdr;
   RayList_ispc ispc;
 };
@@ -1906,9 +1910,10 @@ GXY_EXPORT int gxy_spawn_offsets(int n, RayList_ispc *R, int nAO, int nL, int do
 }
 
 static int route(int n, const RayList_ispc &R, int mode, int reverse, const gxy_box3f *box, const int nb[6],
-                 float *fb, int w, int h) {
+                 float *fb, int w, int h, int unique = 0) {
   RouteParams P;
   memset(&P, 0, sizeof P);
+  P.unique = unique;
   P.reverse = reverse;
   for (int k = 0; k < 6; k++) P.neighbors[k] = nb ? nb[k] : -1;
   if (box)
@@ -2112,6 +2117,8 @@ static int enqueue_split(LoopCtx &C, RayListObj *out, int prio) {
     int c = std::min(C.maxlist, n - s);
     RayListObj *p = raylist_new(c, out->hdr.frame, out->hdr.type);
     if (!p) return -1;
+    p->kinds = out->kinds;
+    p->unique = out->unique;
     HIPCHK(launch_copy_rays(out->ispc, s, c, p->ispc, 0, S()));
     C.q.push_back({p, prio});
   }
@@ -2131,7 +2138,8 @@ static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
     Scratch spawn_sc;
     int *scr_spawn = nullptr, *dT = nullptr, nsp = 0;
     OwnedList out;   // the children, freed on an early return
-    if (!C.sampler) {
+    // a list without primary rays has no parents: no spawn stage
+    if (!C.sampler && L->kinds != 2) {
       // TraceRays::Trace (TraceRays.cpp:68-146): the parents (primary surface
       // rays) ranked by one flag scan; the AO block then the shadow block of
       // their children, written one work-item per child
@@ -2152,6 +2160,9 @@ static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
       if (nsp > 0) {
         out.p = raylist_new(n * nsp, L->hdr.frame, 1);
         if (!out.p) return -1;
+        // AO and shadow rays only; one child per parent keeps the parents' pixels unique
+        out.p->kinds = 2;
+        out.p->unique = (L->unique && nsp == 1) ? 1 : 0;
       }
       const int mode = LM_AMBIENT | LM_DIFFUSE | (C.L.nAO > 0 ? LM_AO : 0) |
                        ((C.L.doShadows && C.L.nLights > 0) ? LM_SHADOW : 0);
@@ -2166,7 +2177,7 @@ static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
     }
     // Classify, AssignDestinations, HandleTerminatedRays (Renderer.cpp:550-556)
     if (route(n, R, C.sampler ? (RM_CLASSIFY | RM_ASSIGN) : (RM_CLASSIFY | RM_ASSIGN | RM_ACCUMULATE), 1,
-              &C.box, C.nb, C.fb, C.w, C.h) != 0)
+              &C.box, C.nb, C.fb, C.w, C.h, L->unique) != 0)
       return -1;
     if (C.sampler) {
       // Sampler::HandleTerminatedRays (Sampler.cpp:51-89)
@@ -2226,6 +2237,8 @@ static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
     if (counts[0] > 0) {
       keep = raylist_new(counts[0], L->hdr.frame, L->hdr.type);
       if (!keep) return -1;
+      keep->kinds = L->kinds;
+      keep->unique = L->unique;
       lists[0] = keep->ispc;
       C.st.kept += counts[0];
       any = true;
@@ -2355,6 +2368,8 @@ GXY_EXPORT int gxy_render_frame(void *vis, void *lights, const float eye[3], con
   if (!V) return fail("null visualization");
   RayListObj *prim = raylist_new(w * h, 0, 0);
   if (!prim) return -1;
+  prim->kinds = 1;    // camera rays: all primary, one per pixel
+  prim->unique = 1;
   int np = 0;
   if (camera_generate(eye, center, 0, up, aov, w, h, &V->local_bb, &V->global_bb, prim->ispc, w * h, 1, &np) != 0) {
     raylist_delete(prim);
