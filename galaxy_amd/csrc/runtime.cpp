@@ -351,6 +351,7 @@ struct RayListObj {
   // kinds 1 every ray primary, 2 none primary; unique: each pixel at most once
   int kinds = 0;
   int unique = 0;
+  bool hdr_dirty = false;   // the device copy of hdr not written yet (lists only the loop sees)
   gxy_raylist_hdr hdr;
   RayList_ispc ispc;
 };
@@ -364,7 +365,9 @@ static void set_pointers(RayListObj *r) {
   for (int k = 0; k < 5; k++) ip[k] = (int *)(base + (size_t)(20 + k) * nn * 4);
 }
 
-static RayListObj *raylist_new(int nrays, int frame, int type) {
+// write_header false: the 64-byte block header is written only when the
+// block leaves the library (lists the rank loop makes and consumes itself)
+static RayListObj *raylist_new(int nrays, int frame, int type, bool write_header = true) {
   if (nrays < 0) {
     fail("negative ray count");
     return nullptr;
@@ -384,7 +387,9 @@ static RayListObj *raylist_new(int nrays, int frame, int type) {
     return nullptr;
   }
   set_pointers(r);
-  if (launch_write_header(r->block, &r->hdr, S()) != hipSuccess) {
+  if (!write_header) {
+    r->hdr_dirty = true;
+  } else if (launch_write_header(r->block, &r->hdr, S()) != hipSuccess) {
     fail("RayList header write failed");
     dfree(r->block);
     delete r;
@@ -393,6 +398,14 @@ static RayListObj *raylist_new(int nrays, int frame, int type) {
   std::lock_guard<std::recursive_mutex> lk(G.mu);
   G.raylists.insert(r);
   return r;
+}
+
+// writes the block header of a list whose header was left to be written
+static int raylist_sync_header(RayListObj *r) {
+  if (!r->hdr_dirty) return 0;
+  HIPCHK(launch_write_header(r->block, &r->hdr, S()));
+  r->hdr_dirty = false;
+  return 0;
 }
 
 static void raylist_delete(RayListObj *r, bool sync = true) {
@@ -754,6 +767,7 @@ GXY_EXPORT size_t gxy_raylist_bytes(void *rl) {
 GXY_EXPORT int gxy_raylist_download(void *rl, void *host_block) {
   RayListObj *r = lookup<RayListObj>(G.raylists, rl);
   if (!r) return fail("not a raylist handle");
+  if (raylist_sync_header(r) != 0) return -1;
   HIPCHK(hipMemcpyAsync(host_block, r->block, r->bytes, hipMemcpyDeviceToHost, S()));
   HIPCHK(hipStreamSynchronize(S()));
   return 0;
@@ -2115,7 +2129,7 @@ static int enqueue_split(LoopCtx &C, RayListObj *out, int prio) {
   }
   for (int s = 0; s < n; s += C.maxlist) {
     int c = std::min(C.maxlist, n - s);
-    RayListObj *p = raylist_new(c, out->hdr.frame, out->hdr.type);
+    RayListObj *p = raylist_new(c, out->hdr.frame, out->hdr.type, false);
     if (!p) return -1;
     p->kinds = out->kinds;
     p->unique = out->unique;
@@ -2158,7 +2172,7 @@ static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
       nsp = C.L.nAO + (C.L.doShadows ? C.L.nLights : 0);
       if ((long long)n * nsp > INT32_MAX - 16) return fail("%lld secondary rays exceed one RayList", (long long)n * nsp);
       if (nsp > 0) {
-        out.p = raylist_new(n * nsp, L->hdr.frame, 1);
+        out.p = raylist_new(n * nsp, L->hdr.frame, 1, false);
         if (!out.p) return -1;
         // AO and shadow rays only; one child per parent keeps the parents' pixels unique
         out.p->kinds = 2;
@@ -2224,7 +2238,6 @@ static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
       // arrays stay laid out for the capacity (hdr.aligned_size)
       RayListObj *o = out.release();
       o->hdr.size = T * nsp;
-      HIPCHK(launch_write_header(o->block, &o->hdr, S()));
       C.st.secondary_rays += (long long)T * nsp;
       if (enqueue_split(C, o, 2) != 0) return -1;
     }
@@ -2235,7 +2248,7 @@ static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
     memset(lists, 0, sizeof lists);
     bool any = false;
     if (counts[0] > 0) {
-      keep = raylist_new(counts[0], L->hdr.frame, L->hdr.type);
+      keep = raylist_new(counts[0], L->hdr.frame, L->hdr.type, false);
       if (!keep) return -1;
       keep->kinds = L->kinds;
       keep->unique = L->unique;
@@ -2278,7 +2291,7 @@ static int run_loop(LoopCtx &C, int n, const RayList_ispc &rays, RayListObj *ado
     // initial lists, at most maxlist rays each (primary priority)
     for (int s = 0; s < n; s += C.maxlist) {
       int c = std::min(C.maxlist, n - s);
-      RayListObj *p = raylist_new(c, 0, 0);
+      RayListObj *p = raylist_new(c, 0, 0, false);
       if (!p) return -1;
       HIPCHK(launch_copy_rays(rays, s, c, p->ispc, 0, S()));
       C.q.push_back({p, 3});
@@ -2366,7 +2379,7 @@ GXY_EXPORT int gxy_render_frame(void *vis, void *lights, const float eye[3], con
   INIT_OR(-1);
   const Visualization_ispc *V = (const Visualization_ispc *)vis;
   if (!V) return fail("null visualization");
-  RayListObj *prim = raylist_new(w * h, 0, 0);
+  RayListObj *prim = raylist_new(w * h, 0, 0, false);
   if (!prim) return -1;
   prim->kinds = 1;    // camera rays: all primary, one per pixel
   prim->unique = 1;
@@ -2537,7 +2550,7 @@ static int batch_pool(BatchObj *B, int k, long long n) {
   if (n > INT32_MAX - 16) return fail("ray pool of %lld rays exceeds 2^31", n);
   long long c = std::max<long long>(n, B->pool[k] ? (long long)B->pool[k]->hdr.aligned_size * 3 / 2 : 0);
   if (c > INT32_MAX - 16) c = n;
-  RayListObj *r = raylist_new((int)c, 0, 1);
+  RayListObj *r = raylist_new((int)c, 0, 1, false);
   if (!r) return -1;
   if (B->pool[k]) raylist_delete(B->pool[k]);
   B->pool[k] = r;
@@ -3267,7 +3280,7 @@ static int flight_make(ServerObj *S_, const gxy_request &r, Flight &f) {
     if (r.width <= 0 || r.height <= 0 || (long long)r.width * r.height > INT32_MAX / 2)
       return fail("render request of %dx%d", r.width, r.height);
     const int px = r.width * r.height;
-    f.rays = raylist_new(px, 0, 0);
+    f.rays = raylist_new(px, 0, 0, false);
     f.fb = (float *)dalloc((size_t)px * 16);
     if (!f.rays || !f.fb) return -1;
     int nout = 0;
@@ -3279,7 +3292,7 @@ static int flight_make(ServerObj *S_, const gxy_request &r, Flight &f) {
   } else {
     if (r.nrays <= 0 || r.nrays > INT32_MAX / 2 || r.capacity < 0) return fail("sample request of %lld rays", r.nrays);
     f.n = (int)r.nrays;
-    f.rays = raylist_new(f.n, 0, 0);
+    f.rays = raylist_new(f.n, 0, 0, false);
     f.xyz = (float *)dalloc((size_t)std::max<long long>(1, r.capacity) * 12);
     f.cnt = (int *)dalloc(sizeof(int));
     if (!f.rays || !f.xyz || !f.cnt) return -1;
