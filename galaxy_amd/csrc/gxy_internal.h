@@ -52,8 +52,19 @@ hipError_t launch_trace_pl(const TraceParams &P, const RayList_ispc &R, int n, i
 int trace_pl_blocks(int n, int chunk);
 hipError_t launch_reduce_add(const unsigned long long *part, int nb, unsigned long long *acc, hipStream_t s);
 
-// Exclusive offsets of primary surface rays (flag scan).  block_counts needs
-// ceil(n/256)+1 ints of scratch; total (device int) receives the count.
+// Exclusive sum of m ints by the device-wide scan; temp holds scan_temp_bytes(m).
+size_t scan_temp_bytes(int m);
+hipError_t launch_exclusive_sum(const int *in, int *out, int m, void *temp, hipStream_t s);
+// Offsets from per-workgroup counts: scratch holds block_scan_scratch_bytes(nb)
+// bytes, the count kernel writes nb counts and a zero after them at scratch,
+// and launch_block_scan writes their exclusive sum (nb + 1 entries, the total
+// last) at block_scan_offsets(scratch, nb).
+size_t block_scan_scratch_bytes(int nb);
+int *block_scan_offsets(int *scratch, int nb);
+hipError_t launch_block_scan(int *scratch, int nb, hipStream_t s);
+// Exclusive offsets of primary surface rays (flag scan).  scratch holds
+// surface_scan_scratch_bytes(n) bytes; total (device int) receives the count.
+size_t surface_scan_scratch_bytes(int n);
 hipError_t launch_surface_scan(const RayList_ispc &R, int n, int *scratch, int *rank_out, int *total,
                                hipStream_t s, int *pidx = nullptr);
 // Lighting with one work-item per spawned child (children at the offsets of
@@ -75,12 +86,15 @@ hipError_t launch_lighting(const LightParams &L, int mode, const RayList_ispc &R
 hipError_t launch_route(const RouteParams &P, const RayList_ispc &R, int n, hipStream_t s);
 // Stable partition by classification into K categories: the category of a ray
 // is cat_table[class + 4] (host table, ntable <= 64 entries, -1 = not copied).
-// counts: K ints (device).  scratch: K*ceil(n/256) ints, kept intact between
-// the count phase (lists == nullptr) and launch_partition_scatter.
+// counts: K ints (device).  scratch: partition_scratch_bytes(n, K) bytes, kept
+// intact between the count phase (lists == nullptr) and launch_partition_scatter.
+size_t partition_scratch_bytes(int n, int K);
 hipError_t launch_partition(const RayList_ispc &R, int n, const int *cat_table, int ntable, int K,
                             const RayList_ispc *lists, int *scratch, int *counts, hipStream_t s);
 hipError_t launch_partition_scatter(const RayList_ispc &R, int n, const int *cat_table, int ntable, int K,
                                     const RayList_ispc *lists, const int *scratch, hipStream_t s);
+// scratch: block_scan_scratch_bytes(ceil(n/256)) bytes (camera: of the pixel
+// order's ray slots)
 hipError_t launch_accumulate_sampler(const RayList_ispc &R, int n, float *xyz, int capacity, int *count,
                                      int *scratch, hipStream_t s);
 hipError_t launch_camera(const CameraParams &C, const RayList_ispc &R, int capacity, int *scratch, int *total,

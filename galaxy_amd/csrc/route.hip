@@ -87,32 +87,14 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_part_count(RayList_ispc R, int n,
     for (int q = 0; q < GXY_BLOCK / 64; q++) s += wc[q][threadIdx.x];
     counts[threadIdx.x * nb + blockIdx.x] = s;
   }
+  // a zero after the K*nb counts: their exclusive sum then ends in the total
+  if (blockIdx.x == 0 && threadIdx.x == 0) counts[K * nb] = 0;
 }
 
-// one workgroup per category: exclusive scan over workgroups; total -> tot[k]
-__global__ __launch_bounds__(1024) void k_part_scan(int *counts, int nb, int *tot) {
-  __shared__ int part[1024];
-  int *a = counts + (int64_t)blockIdx.x * nb;
-  const int t = threadIdx.x;
-  const int per = (nb + 1023) / 1024;
-  const int s0 = t * per, s1 = min(nb, s0 + per);
-  int sum = 0;
-  for (int k = s0; k < s1; k++) sum += a[k];
-  part[t] = sum;
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    int v = t >= o ? part[t - o] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  int base = t > 0 ? part[t - 1] : 0;
-  for (int k = s0; k < s1; k++) {
-    int v = a[k];
-    a[k] = base;
-    base += v;
-  }
-  if (t == 1023) tot[blockIdx.x] = part[1023];
+// category totals from the exclusive sum over the category-major counts
+__global__ void k_part_totals(const int *off, int nb, int K, int *tot) {
+  const int k = threadIdx.x;
+  if (k < K) tot[k] = off[(k + 1) * nb] - off[k * nb];
 }
 
 // destination lists are passed by value in the kernel arguments
@@ -146,7 +128,7 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_part_scatter(RayList_ispc R, int 
   }
   __syncthreads();
   if (cat >= 0 && cat < K && cat < GXY_MAX_SCATTER) {
-    int off = base[cat * nb + blockIdx.x] + rank;
+    int off = base[cat * nb + blockIdx.x] - base[cat * nb] + rank;
     for (int q = 0; q < w; q++) off += wc[q][cat];
     const RayList_ispc &D = lists.l[cat];
     if (D.ox) copy_one(R, i, D, off);
@@ -160,17 +142,37 @@ static CatTable make_table(const int *cat_table, int ntable) {
   return T;
 }
 
+// scratch layout: [K*nb+1 counts][their exclusive sum][scan temp], each array
+// rounded to 64 ints
+static inline size_t part_pad(int n, int K) {
+  const size_t m = (size_t)K * ((n + GXY_BLOCK - 1) / GXY_BLOCK) + 1;
+  return (m + 63) / 64 * 64;
+}
+
+size_t partition_scratch_bytes(int n, int K) {
+  const int m = K * ((n + GXY_BLOCK - 1) / GXY_BLOCK) + 1;
+  return 2 * sizeof(int) * part_pad(n, K) + scan_temp_bytes(m);
+}
+
+__global__ void k_zero_ints(int *p, int m) {
+  if ((int)threadIdx.x < m) p[threadIdx.x] = 0;
+}
+
 hipError_t launch_partition(const RayList_ispc &R, int n, const int *cat_table, int ntable, int K,
                             const RayList_ispc *lists, int *scratch, int *counts, hipStream_t s) {
   if (K > GXY_MAX_CAT || ntable > GXY_CAT_TABLE) return hipErrorInvalidValue;
   int nb = (n + GXY_BLOCK - 1) / GXY_BLOCK;
   if (nb == 0) {
-    (void)hipMemsetAsync(counts, 0, K * sizeof(int), s);
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, counts, K);
     return hipGetLastError();
   }
   CatTable T = make_table(cat_table, ntable);
+  const size_t pad = part_pad(n, K);
+  int *off = scratch + pad;
   hipLaunchKernelGGL(k_part_count, dim3(nb), dim3(GXY_BLOCK), 0, s, R, n, T, K, nb, scratch);
-  hipLaunchKernelGGL(k_part_scan, dim3(K), dim3(1024), 0, s, scratch, nb, counts);
+  hipError_t e = launch_exclusive_sum(scratch, off, K * nb + 1, scratch + 2 * pad, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_part_totals, dim3(1), dim3(64), 0, s, off, nb, K, counts);
   if (lists) return launch_partition_scatter(R, n, cat_table, ntable, K, lists, scratch, s);
   return hipGetLastError();
 }
@@ -184,7 +186,7 @@ hipError_t launch_partition_scatter(const RayList_ispc &R, int n, const int *cat
   ListTable L;
   memset(&L, 0, sizeof L);
   for (int k = 0; k < K; k++) L.l[k] = lists[k];
-  hipLaunchKernelGGL(k_part_scatter, dim3(nb), dim3(GXY_BLOCK), 0, s, R, n, T, K, nb, scratch, L);
+  hipLaunchKernelGGL(k_part_scatter, dim3(nb), dim3(GXY_BLOCK), 0, s, R, n, T, K, nb, scratch + part_pad(n, K), L);
   return hipGetLastError();
 }
 
@@ -198,7 +200,10 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_sampler_count(RayList_ispc R, int
   unsigned long long m = __ballot(f);
   if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = __popcll(m);
   __syncthreads();
-  if (threadIdx.x == 0) bc[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+  if (threadIdx.x == 0) {
+    bc[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+    if (blockIdx.x == 0) bc[gridDim.x] = 0;
+  }
 }
 
 __global__ __launch_bounds__(GXY_BLOCK) void k_sampler_write(RayList_ispc R, int n, const int *bbase, const int *count,
@@ -222,16 +227,18 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_sampler_write(RayList_ispc R, int
 }
 
 __global__ void k_add_count(int *count, const int *add) { *count += *add; }
+__global__ void k_copy_total(const int *src, int *dst) { *dst = *src; }
 
 hipError_t launch_accumulate_sampler(const RayList_ispc &R, int n, float *xyz, int capacity, int *count,
                                      int *scratch, hipStream_t s) {
   int nb = (n + GXY_BLOCK - 1) / GXY_BLOCK;
   if (nb == 0) return hipSuccess;
-  int *tot = scratch + nb + 1;
+  int *off = block_scan_offsets(scratch, nb);
   hipLaunchKernelGGL(k_sampler_count, dim3(nb), dim3(GXY_BLOCK), 0, s, R, n, scratch);
-  hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, scratch, nb, tot);
-  hipLaunchKernelGGL(k_sampler_write, dim3(nb), dim3(GXY_BLOCK), 0, s, R, n, scratch, count, xyz, capacity);
-  hipLaunchKernelGGL(k_add_count, dim3(1), dim3(1), 0, s, count, tot);
+  hipError_t e = launch_block_scan(scratch, nb, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_sampler_write, dim3(nb), dim3(GXY_BLOCK), 0, s, R, n, off, count, xyz, capacity);
+  hipLaunchKernelGGL(k_add_count, dim3(1), dim3(1), 0, s, count, off + nb);
   return hipGetLastError();
 }
 
@@ -316,7 +323,10 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_camera_count(CameraParams C, int 
   unsigned long long m = __ballot(f);
   if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = __popcll(m);
   __syncthreads();
-  if (threadIdx.x == 0) bc[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+  if (threadIdx.x == 0) {
+    bc[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+    if (blockIdx.x == 0) bc[gridDim.x] = 0;
+  }
 }
 
 __global__ __launch_bounds__(GXY_BLOCK) void k_camera_write(CameraParams C, int npix, const int *bbase,
@@ -352,9 +362,16 @@ hipError_t launch_camera(const CameraParams &C, const RayList_ispc &R, int capac
   if (C.order == 0) npix = C.width * C.height;
   else npix = ((C.width + 7) / 8) * ((C.height + 7) / 8) * 64;
   int nb = (npix + GXY_BLOCK - 1) / GXY_BLOCK;
+  if (nb <= 0) {
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, total, 1);
+    return hipGetLastError();
+  }
+  int *off = block_scan_offsets(scratch, nb);
   hipLaunchKernelGGL(k_camera_count, dim3(nb), dim3(GXY_BLOCK), 0, s, C, npix, scratch);
-  hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(1024), 0, s, scratch, nb, total);
-  hipLaunchKernelGGL(k_camera_write, dim3(nb), dim3(GXY_BLOCK), 0, s, C, npix, scratch, R, capacity);
+  hipError_t e = launch_block_scan(scratch, nb, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_camera_write, dim3(nb), dim3(GXY_BLOCK), 0, s, C, npix, off, R, capacity);
+  hipLaunchKernelGGL(k_copy_total, dim3(1), dim3(1), 0, s, off + nb, total);
   return hipGetLastError();
 }
 

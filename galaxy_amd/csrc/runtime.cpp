@@ -1890,11 +1890,10 @@ struct OwnedList {
 static int spawn_offsets(int n, const RayList_ispc &R, int nAO, int nL, int doSh, int *ao_off, int *sh_off,
                          int *T_out) {
   Scratch sc;
-  int nb = (n + GXY_BLOCK - 1) / GXY_BLOCK;
-  int *scratch = (int *)sc.get(sizeof(int) * (nb + 2));
+  int *scratch = (int *)sc.get(surface_scan_scratch_bytes(n));
   int *rank = (int *)sc.get(sizeof(int) * std::max(1, n));
-  if (!scratch || !rank) return -1;
-  int *total = scratch + nb + 1;
+  int *total = (int *)sc.get(sizeof(int));
+  if (!scratch || !rank || !total) return -1;
   {
     KTimer kt("spawn_scan");
     HIPCHK(launch_surface_scan(R, n, scratch, rank, total, S()));
@@ -1963,8 +1962,7 @@ GXY_EXPORT int gxy_partition_rays(int n, RayList_ispc *R, int ndest, RayList_isp
   table[GXY_KEEP_HERE + 4] = ndest;
   for (int d = 0; d < ndest; d++) table[d + 4] = d;
   Scratch sc;
-  int nb = (n + GXY_BLOCK - 1) / GXY_BLOCK;
-  int *scratch = (int *)sc.get(sizeof(int) * ((size_t)K * std::max(1, nb) + 1));
+  int *scratch = (int *)sc.get(partition_scratch_bytes(n, K));
   int *dcounts = (int *)sc.get(sizeof(int) * K);
   if (!scratch || !dcounts) return -1;
   {
@@ -2049,9 +2047,9 @@ static int camera_generate(const float eye[3], const float cdir[3], int is_dir, 
   int npix = order == 0 ? w * h : ((w + 7) / 8) * ((h + 7) / 8) * 64;
   Scratch sc;
   int nb = (npix + GXY_BLOCK - 1) / GXY_BLOCK;
-  int *scratch = (int *)sc.get(sizeof(int) * (nb + 2));
-  if (!scratch) return -1;
-  int *total = scratch + nb + 1;
+  int *scratch = (int *)sc.get(block_scan_scratch_bytes(nb));
+  int *total = (int *)sc.get(sizeof(int));
+  if (!scratch || !total) return -1;
   {
     KTimer kt("camera");
     HIPCHK(launch_camera(C, R, capacity, scratch, total, S()));
@@ -2075,7 +2073,7 @@ GXY_EXPORT int gxy_sampler_collect(int n, RayList_ispc *R, float *xyz, int capac
   INIT_OR(-1);
   Scratch sc;
   int nb = (n + GXY_BLOCK - 1) / GXY_BLOCK;
-  int *scratch = (int *)sc.get(sizeof(int) * (nb + 4));
+  int *scratch = (int *)sc.get(block_scan_scratch_bytes(nb));
   if (!scratch) return -1;
   KTimer kt("sampler_collect");
   HIPCHK(launch_accumulate_sampler(*R, n, xyz, capacity, count, scratch, S()));
@@ -2160,11 +2158,10 @@ static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
       // the parent count stays on the device until the partition counts are
       // read (one host wait per list): the children list is allocated for
       // every ray being a parent and its size set afterwards
-      const int nb = (n + GXY_BLOCK - 1) / GXY_BLOCK;
-      scr_spawn = (int *)spawn_sc.get(sizeof(int) * (nb + 2));
+      scr_spawn = (int *)spawn_sc.get(surface_scan_scratch_bytes(n));
       int *pidx = (int *)spawn_sc.get(sizeof(int) * std::max(1, n));
-      if (!scr_spawn || !pidx) return -1;
-      int *total = scr_spawn + nb + 1;
+      int *total = (int *)spawn_sc.get(sizeof(int));
+      if (!scr_spawn || !pidx || !total) return -1;
       {
         KTimer kt("spawn_scan");
         HIPCHK(launch_surface_scan(R, n, scr_spawn, nullptr, total, S(), pidx));
@@ -2197,7 +2194,7 @@ static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
       // Sampler::HandleTerminatedRays (Sampler.cpp:51-89)
       Scratch sc;
       int nbk = (n + GXY_BLOCK - 1) / GXY_BLOCK;
-      int *scr = (int *)sc.get(sizeof(int) * (nbk + 4));
+      int *scr = (int *)sc.get(block_scan_scratch_bytes(nbk));
       if (!scr) return -1;
       KTimer kt("sampler_collect");
       HIPCHK(launch_accumulate_sampler(R, n, C.xyz, C.capacity, C.dcount, scr, S()));
@@ -2217,8 +2214,7 @@ static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
       }
     const int K = 9, KL = 7;
     Scratch sc;
-    int nbk = (n + GXY_BLOCK - 1) / GXY_BLOCK;
-    int *scr = (int *)sc.get(sizeof(int) * ((size_t)K * nbk + 1));
+    int *scr = (int *)sc.get(partition_scratch_bytes(n, K));
     int *dcounts = (int *)sc.get(sizeof(int) * K);
     if (!scr || !dcounts) return -1;
     int counts[K];

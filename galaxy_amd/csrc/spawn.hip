@@ -9,6 +9,8 @@
 // gives the global base.  The lighting kernel then applies, per parent and in
 // the reference order, ambientLighting -> generateAORays -> diffuseLighting ->
 // generateShadowRays (TraceRays.cpp:124-131, TraceRays.ispc:625-923).
+#include <hipcub/hipcub.hpp>
+
 #include "light_dev.h"
 
 namespace gxy {
@@ -18,7 +20,8 @@ __device__ __forceinline__ unsigned long long lanemask_lt() {
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-// Per-workgroup count of flagged rays.
+// Per-workgroup count of flagged rays; a zero after the nb counts, so that
+// their exclusive sum ends in the total.
 __global__ __launch_bounds__(GXY_BLOCK) void k_flag_count(RayList_ispc R, int n, int *block_counts) {
   __shared__ int wsum[GXY_BLOCK / 64];
   int i = blockIdx.x * GXY_BLOCK + threadIdx.x;
@@ -30,33 +33,8 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_flag_count(RayList_ispc R, int n,
     int s = 0;
     for (int w = 0; w < GXY_BLOCK / 64; w++) s += wsum[w];
     block_counts[blockIdx.x] = s;
+    if (blockIdx.x == 0) block_counts[gridDim.x] = 0;
   }
-}
-
-// Exclusive scan of nb ints in place (single workgroup, 1024 work-items);
-// total written to *total.
-__global__ __launch_bounds__(1024) void k_scan_single(int *a, int nb, int *total) {
-  __shared__ int part[1024];
-  const int t = threadIdx.x;
-  const int per = (nb + 1023) / 1024;
-  const int s0 = t * per, s1 = min(nb, s0 + per);
-  int sum = 0;
-  for (int k = s0; k < s1; k++) sum += a[k];
-  part[t] = sum;
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    int v = t >= o ? part[t - o] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  int base = t > 0 ? part[t - 1] : 0;
-  for (int k = s0; k < s1; k++) {
-    int v = a[k];
-    a[k] = base;
-    base += v;
-  }
-  if (t == 1023) *total = part[1023];
 }
 
 __global__ __launch_bounds__(GXY_BLOCK) void k_flag_rank(RayList_ispc R, int n, const int *block_base, int *rank,
@@ -77,16 +55,54 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_flag_rank(RayList_ispc R, int n, 
   if (f && pidx) pidx[rk] = i;
 }
 
+// ------------------------------------------------------------------ device-wide scans
+// Exclusive sum of m ints (in -> out) by the device-wide single-pass scan of
+// rocPRIM (hipCUB); temp holds scan_temp_bytes(m) bytes.
+size_t scan_temp_bytes(int m) {
+  size_t bytes = 0;
+  hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const int *)nullptr, (int *)nullptr, m, (hipStream_t)0);
+  return (bytes + 255) & ~(size_t)255;
+}
+
+hipError_t launch_exclusive_sum(const int *in, int *out, int m, void *temp, hipStream_t s) {
+  size_t bytes = scan_temp_bytes(m);
+  return hipcub::DeviceScan::ExclusiveSum(temp, bytes, in, out, m, s);
+}
+
+__global__ void k_set_int(int *dst, const int *src, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = src ? *src : v;
+}
+
+// ints of one scan array of m entries, rounded to 64 (256-byte alignment)
+static inline size_t scan_pad(size_t m) { return (m + 63) / 64 * 64; }
+
+size_t block_scan_scratch_bytes(int nb) {
+  return 2 * sizeof(int) * scan_pad((size_t)nb + 1) + scan_temp_bytes(nb + 1);
+}
+
+int *block_scan_offsets(int *scratch, int nb) { return scratch + scan_pad((size_t)nb + 1); }
+
+hipError_t launch_block_scan(int *scratch, int nb, hipStream_t s) {
+  return launch_exclusive_sum(scratch, block_scan_offsets(scratch, nb), nb + 1,
+                              scratch + 2 * scan_pad((size_t)nb + 1), s);
+}
+
+size_t surface_scan_scratch_bytes(int n) { return block_scan_scratch_bytes((n + GXY_BLOCK - 1) / GXY_BLOCK); }
+
 hipError_t launch_surface_scan(const RayList_ispc &R, int n, int *scratch, int *rank_out, int *total,
                                hipStream_t s, int *pidx) {
   int nb = (n + GXY_BLOCK - 1) / GXY_BLOCK;
   if (nb == 0) {
-    (void)hipMemsetAsync(total, 0, sizeof(int), s);
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, total, (const int *)nullptr, 0);
     return hipGetLastError();
   }
+  // block counts with a zero after them, and their exclusive sum (the total in its last slot)
+  int *off = block_scan_offsets(scratch, nb);
   hipLaunchKernelGGL(k_flag_count, dim3(nb), dim3(GXY_BLOCK), 0, s, R, n, scratch);
-  hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, s, scratch, nb, total);
-  hipLaunchKernelGGL(k_flag_rank, dim3(nb), dim3(GXY_BLOCK), 0, s, R, n, scratch, rank_out, pidx);
+  hipError_t e = launch_block_scan(scratch, nb, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_flag_rank, dim3(nb), dim3(GXY_BLOCK), 0, s, R, n, off, rank_out, pidx);
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, total, (const int *)(off + nb), 0);
   return hipGetLastError();
 }
 
