@@ -87,7 +87,8 @@ hipError_t launch_route(const RouteParams &P, const RayList_ispc &R, int n, hipS
 // Stable partition by classification into K categories: the category of a ray
 // is cat_table[class + 4] (host table, ntable <= 64 entries, -1 = not copied).
 // counts: K ints (device).  scratch: partition_scratch_bytes(n, K) bytes, kept
-// intact between the count phase (lists == nullptr) and launch_partition_scatter.
+// intact between the count phase (lists == nullptr) and launch_partition_scatter,
+// which takes the same n and K (lists: K entries, those with a null ox not copied).
 size_t partition_scratch_bytes(int n, int K);
 hipError_t launch_partition(const RayList_ispc &R, int n, const int *cat_table, int ntable, int K,
                             const RayList_ispc *lists, int *scratch, int *counts, hipStream_t s);

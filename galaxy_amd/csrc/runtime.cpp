@@ -2212,7 +2212,7 @@ static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
         table[C.nb[f] + 4] = 1 + f;
         ntable = std::max(ntable, C.nb[f] + 5);
       }
-    const int K = 9, KL = 7;
+    const int K = 9;
     Scratch sc;
     int *scr = (int *)sc.get(partition_scratch_bytes(n, K));
     int *dcounts = (int *)sc.get(sizeof(int) * K);
@@ -2240,7 +2240,8 @@ static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
     C.st.terminated += counts[7];
     C.st.dropped += counts[8];
     RayListObj *keep = nullptr;
-    RayList_ispc lists[KL];
+    // categories 7 and 8 (terminated, dropped) are counted, not copied
+    RayList_ispc lists[K];
     memset(lists, 0, sizeof lists);
     bool any = false;
     if (counts[0] > 0) {
@@ -2267,7 +2268,7 @@ static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
     }
     if (any) {
       KTimer kt("partition");
-      HIPCHK(launch_partition_scatter(R, n, table, ntable, KL, lists, scr, S()));
+      HIPCHK(launch_partition_scatter(R, n, table, ntable, K, lists, scr, S()));
     }
     if (owned) raylist_delete(L, false);
     L = keep;
