@@ -100,6 +100,11 @@ hipError_t launch_accumulate_sampler(const RayList_ispc &R, int n, float *xyz, i
                                      int *scratch, hipStream_t s);
 hipError_t launch_camera(const CameraParams &C, const RayList_ispc &R, int capacity, int *scratch, int *total,
                          hipStream_t s);
+// one kernel copying up to 16 ints (a), one int (b, optional) and one 64-bit
+// counter (c, optional) into the mapped host area out_dev (20 ints: a at 0,
+// b at 16, c at 18)
+hipError_t launch_readback(const int *a, int na, const int *b, const unsigned long long *c, int *out_dev,
+                           hipStream_t s);
 // writes the 64-byte RayList header hdr64 (host memory, copied into the launch) to block
 hipError_t launch_write_header(void *block, const void *hdr64, hipStream_t s);
 hipError_t launch_copy_rays(const RayList_ispc &src, int start, int count, const RayList_ispc &dst, int dst_start,

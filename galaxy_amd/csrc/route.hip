@@ -227,6 +227,23 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_sampler_write(RayList_ispc R, int
 }
 
 __global__ void k_add_count(int *count, const int *add) { *count += *add; }
+
+// The values the host waits for after a list, written straight into mapped
+// host memory: out[0..na) = a, out[16] = *b, out[18..19] = *c (64-bit).
+__global__ void k_readback(const int *a, int na, const int *b, const unsigned long long *c, int *out) {
+  const int t = threadIdx.x;
+  if (t < na) out[t] = a[t];
+  if (t == 16 && b) out[16] = *b;
+  if (t == 18 && c) *(unsigned long long *)(out + 18) = *c;
+  __threadfence_system();
+}
+
+hipError_t launch_readback(const int *a, int na, const int *b, const unsigned long long *c, int *out_dev,
+                           hipStream_t s) {
+  if (na < 0 || na > 16) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_readback, dim3(1), dim3(64), 0, s, a, na, b, c, out_dev);
+  return hipGetLastError();
+}
 __global__ void k_copy_total(const int *src, int *dst) { *dst = *src; }
 
 hipError_t launch_accumulate_sampler(const RayList_ispc &R, int n, float *xyz, int capacity, int *count,

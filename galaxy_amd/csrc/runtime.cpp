@@ -65,6 +65,7 @@ struct PendingEv {
 struct Ctx {
   hipStream_t stream = nullptr;
   unsigned long long *counters = nullptr;
+  int *hread = nullptr, *dread = nullptr;   // mapped host readback area (host / device address)
   std::multimap<size_t, void *> free_blocks;
   size_t bytes_cached = 0;
 };
@@ -86,6 +87,7 @@ struct Runtime {
   std::atomic<int> raylist_id{0};
   float opaque_alpha = 0.999f;
   std::vector<void *> outbound[6];   // RayListObj* per exit face, last frames
+  std::vector<std::pair<int *, int *>> free_reads;   // readback areas of destroyed contexts (kept mapped)
   // volume-only trace kernel: 0 one ray per work-item, 1 persistent lanes
   int trace_kind = 0, trace_chunk = 256, trace_refill = 32, trace_wpe = 0;
   int geom_stack = 0;           // BVH traversal stack of geometry-only traces (0 private, or LDS entries)
@@ -171,6 +173,58 @@ void dfree(void *p) {
   if (G.contexts.empty()) hipStreamSynchronize(S());
   else hipDeviceSynchronize();   // another context's stream may still read it
   dfree_stream_ordered(p);
+}
+
+// ------------------------------------------------------------------ readback
+// A context's readback area: 64 ints of pinned, mapped, coherent host memory
+// that kernels write directly, so a list's counts reach the host with one
+// kernel and one stream wait.  Areas outlive their contexts (reused, never
+// unmapped while the process runs).
+static int *readback_area(Ctx &c) {
+  if (c.hread) return c.dread;
+  std::lock_guard<std::recursive_mutex> lk(G.mu);
+  if (!G.free_reads.empty()) {
+    c.hread = G.free_reads.back().first;
+    c.dread = G.free_reads.back().second;
+    G.free_reads.pop_back();
+    return c.dread;
+  }
+  void *h = nullptr, *d = nullptr;
+  if (hipHostMalloc(&h, 64 * sizeof(int), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+    (void)hipGetLastError();
+    fail("pinned readback allocation failed");
+    return nullptr;
+  }
+  if (hipHostGetDevicePointer(&d, h, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    hipHostFree(h);
+    fail("no device address for the readback area");
+    return nullptr;
+  }
+  c.hread = (int *)h;
+  c.dread = (int *)d;
+  return c.dread;
+}
+
+struct Readback {
+  int ints[16];
+  int one;
+  unsigned long long u64;
+};
+
+// Waits for the stream and returns up to 16 device ints, one device int and
+// one device 64-bit counter (each optional).
+static int readback(const int *ints, int nints, const int *one, const unsigned long long *u64, Readback &out) {
+  Ctx &cx = CX();
+  int *dev = readback_area(cx);
+  if (!dev) return -1;
+  HIPCHK(launch_readback(ints, nints, one, u64, dev, S()));
+  HIPCHK(hipStreamSynchronize(S()));
+  const volatile int *h = cx.hread;
+  for (int k = 0; k < nints; k++) out.ints[k] = h[k];
+  out.one = one ? h[16] : 0;
+  out.u64 = u64 ? ((unsigned long long)(unsigned)h[19] << 32) | (unsigned)h[18] : 0;
+  return 0;
 }
 
 // ------------------------------------------------------------------ timing
@@ -569,6 +623,7 @@ GXY_EXPORT int gxy_context_destroy(void *ctx) {
   {
     std::lock_guard<std::recursive_mutex> lk(G.mu);
     release_cache(*c);
+    if (c->hread) G.free_reads.push_back({c->hread, c->dread});
   }
   hipStreamDestroy(c->stream);
   if (t_ctx == c) t_ctx = nullptr;
@@ -1899,10 +1954,9 @@ static int spawn_offsets(int n, const RayList_ispc &R, int nAO, int nL, int doSh
     HIPCHK(launch_surface_scan(R, n, scratch, rank, total, S()));
     HIPCHK(launch_offsets_from_rank(R, n, rank, total, nAO, nL, doSh, ao_off, sh_off, S()));
   }
-  int T = 0;
-  HIPCHK(hipMemcpyAsync(&T, total, sizeof(int), hipMemcpyDeviceToHost, S()));
-  HIPCHK(hipStreamSynchronize(S()));
-  *T_out = T;
+  Readback rb;
+  if (readback(nullptr, 0, total, nullptr, rb) != 0) return -1;
+  *T_out = rb.one;
   return 0;
 }
 }  // namespace gxy
@@ -2054,9 +2108,9 @@ static int camera_generate(const float eye[3], const float cdir[3], int is_dir, 
     KTimer kt("camera");
     HIPCHK(launch_camera(C, R, capacity, scratch, total, S()));
   }
-  int T = 0;
-  HIPCHK(hipMemcpyAsync(&T, total, sizeof(int), hipMemcpyDeviceToHost, S()));
-  HIPCHK(hipStreamSynchronize(S()));
+  Readback rb;
+  if (readback(nullptr, 0, total, nullptr, rb) != 0) return -1;
+  const int T = rb.one;
   if (T > capacity) return fail("camera produced %d rays for a list of %d", T, capacity);
   *n_out = T;
   return 0;
@@ -2222,12 +2276,13 @@ static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
       KTimer kt("partition");
       HIPCHK(launch_partition(R, n, table, ntable, K, nullptr, scr, dcounts, S()));
     }
-    HIPCHK(hipMemcpyAsync(counts, dcounts, sizeof counts, hipMemcpyDeviceToHost, S()));
-    int T = 0;
-    if (dT) HIPCHK(hipMemcpyAsync(&T, dT, sizeof T, hipMemcpyDeviceToHost, S()));
-    // the sample counter rides along: after the loop's last list no trace follows
-    HIPCHK(hipMemcpyAsync(&C.samples_read, CX().counters, sizeof C.samples_read, hipMemcpyDeviceToHost, S()));
-    HIPCHK(hipStreamSynchronize(S()));
+    // one wait per list: the partition counts, the parent count and the
+    // sample counter (after the loop's last list no trace follows)
+    Readback rb;
+    if (readback(dcounts, K, dT, CX().counters, rb) != 0) return -1;
+    for (int k = 0; k < K; k++) counts[k] = rb.ints[k];
+    const int T = rb.one;
+    C.samples_read = rb.u64;
     C.have_samples = true;
     if (out.p && T > 0) {
       // the children of the T parents (TraceRays.cpp:89-117 layout); the
@@ -2304,8 +2359,9 @@ static int run_loop(LoopCtx &C, int n, const RayList_ispc &rays, RayListObj *ado
     }
   }
   if (!C.have_samples) {
-    HIPCHK(hipMemcpyAsync(&C.samples_read, CX().counters, sizeof C.samples_read, hipMemcpyDeviceToHost, S()));
-    HIPCHK(hipStreamSynchronize(S()));
+    Readback rb;
+    if (readback(nullptr, 0, nullptr, CX().counters, rb) != 0) return -1;
+    C.samples_read = rb.u64;
   }
   C.st.samples = (long long)C.samples_read;
   return 0;
