@@ -92,7 +92,7 @@ struct Runtime {
   int trace_kind = 0, trace_chunk = 256, trace_refill = 32, trace_wpe = 0;
   int geom_stack = 0;           // BVH traversal stack of geometry-only traces (0 private, or LDS entries)
   int occlusion = 1;            // rank loops: shadow/AO rays of volume-free scenes stop at their first hit
-  int bvh_width = 4;            // traverse the 4-wide trees (4) or the binary ones (2)
+  int bvh_width = 2;            // traverse the binary trees (2) or the 4-wide ones (4)
   float *uv = nullptr;          // AO direction tables U[256], V[256] (device)
 };
 static Runtime G;

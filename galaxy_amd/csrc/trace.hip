@@ -388,8 +388,9 @@ static hipError_t launch_trace_nv(const TraceParams &P, const RayList_ispc &R, i
   if (NV == 0 && P.has_model && stack > 0 && (P.model.wide ? P.model.depth4 : P.model.depth) + 2 > stack) stack = 0;
   const bool wide = P.model.wide != 0;
   if (NV == 0 && P.has_model && P.model.ncurve > 0) {
-    // path lines are traced by the per-model variant (private stack) only
-    hipLaunchKernelGGL((k_trace<0, true, 0, -1, 0, 0>), g, b, 0, s, P, R, n);
+    // path lines: the variants that trace curve pieces too (private stack)
+    if (wide) hipLaunchKernelGGL((k_trace<0, true, 0, -1, 0, 20>), g, b, 0, s, P, R, n);
+    else hipLaunchKernelGGL((k_trace<0, true, 0, -1, 0, 18>), g, b, 0, s, P, R, n);
   } else if (NV == 0 && P.has_model && wide) {
     if (stack == 16) hipLaunchKernelGGL((k_trace<0, true, 0, -1, 16, 4>), g, b, 0, s, P, R, n);
     else if (stack == 24) hipLaunchKernelGGL((k_trace<0, true, 0, -1, 24, 4>), g, b, 0, s, P, R, n);

@@ -267,7 +267,7 @@ __device__ __forceinline__ void bvh_walk(const BVHNode *nodes, f3 o, f3 inv, flo
 // level (depth4 entries).
 template <class Stack, class LeafFn>
 __device__ __forceinline__ void bvh4_walk(const BVH4Node *nodes, f3 o, f3 inv, float t0, const GeoHit &h,
-                                          Stack &stack, LeafFn &&leaf, bool any = false) {
+                                          Stack &stack, LeafFn &&leaf, bool any, int dlim) {
   int sp = 0;
   int cur = 0, mask = 0xF;
   const BVH4Node *gn = gptr(nodes);
@@ -279,23 +279,30 @@ __device__ __forceinline__ void bvh4_walk(const BVH4Node *nodes, f3 o, f3 inv, f
     const float hix[4] = {hx.x, hx.y, hx.z, hx.w}, hiy[4] = {hy.x, hy.y, hy.z, hy.w}, hiz[4] = {hz.x, hz.y, hz.z, hz.w};
     const int cc[4] = {ch.x, ch.y, ch.z, ch.w};
     float te[4];
-    int inner = 0;
+    int inner = 0, leaves = 0;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
       te[k] = 0.f;
       if ((mask >> k) & 1) {
         const float lo[3] = {lox[k], loy[k], loz[k]}, hi[3] = {hix[k], hiy[k], hiz[k]};
         if (child_hit(lo, hi, o, inv, t0, h.t, te[k])) {
-          if (cc[k] >= 0) {
-            inner |= 1 << k;
-          } else {
-            const int code = ~cc[k];
-            leaf(code >> 4, (code & 15) + 1);
-          }
+          if (cc[k] >= 0) inner |= 1 << k;
+          else leaves |= 1 << k;
         }
       }
     }
-    if (any && h.geom >= 0) return;
+    // the hit leaf children in slot order, through one copy of the leaf code
+    // (a wave runs it as often as its lanes' largest leaf count)
+    while (leaves) {
+      const int k = __builtin_ctz(leaves);
+      leaves &= leaves - 1;
+      int c = cc[0];
+#pragma unroll
+      for (int q = 1; q < 4; q++) c = (q == k) ? cc[q] : c;
+      const int code = ~c;
+      leaf(code >> 4, (code & 15) + 1);
+      if (any && h.geom >= 0) return;
+    }
     // the nearest inner child whose box still starts before the current hit
     int kn = -1;
     float tn = 0.f;
@@ -306,8 +313,41 @@ __device__ __forceinline__ void bvh4_walk(const BVH4Node *nodes, f3 o, f3 inv, f
         tn = te[k];
       }
     if (kn >= 0) {
-      const int rest = inner & ~(1 << kn);
-      if (rest && sp < Stack::DEPTH) stack.put(sp++, (cur << 4) | rest);
+      int rest = inner & ~(1 << kn);
+      if (rest && sp <= dlim) {
+        // the other children still hit, each its own entry, the nearest on top
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+          if (((rest >> k) & 1) && !(te[k] <= h.t * 1.0000005f + 1e-30f)) rest &= ~(1 << k);
+        while (rest) {
+          int kf = __builtin_ctz(rest);
+          float tf = te[0];
+#pragma unroll
+          for (int k = 1; k < 4; k++) tf = (k == kf) ? te[k] : tf;
+#pragma unroll
+          for (int k = 0; k < 4; k++)
+            if (((rest >> k) & 1) && te[k] > tf) {
+              kf = k;
+              tf = te[k];
+            }
+          rest &= ~(1 << kf);
+          int c2 = cc[0];
+#pragma unroll
+          for (int k = 1; k < 4; k++) c2 = (k == kf) ? cc[k] : c2;
+          stack.put(sp++, c2 << 4);
+        }
+      } else if (rest && sp < Stack::DEPTH) {
+        if ((rest & (rest - 1)) == 0) {
+          // one other child: its own node, whole (mask 0)
+          const int k2 = __builtin_ctz(rest);
+          int c2 = cc[0];
+#pragma unroll
+          for (int k = 1; k < 4; k++) c2 = (k == k2) ? cc[k] : c2;
+          stack.put(sp++, c2 << 4);
+        } else {
+          stack.put(sp++, (cur << 4) | rest);
+        }
+      }
       int c = cc[0];
 #pragma unroll
       for (int k = 1; k < 4; k++) c = (k == kn) ? cc[k] : c;
@@ -318,12 +358,16 @@ __device__ __forceinline__ void bvh4_walk(const BVH4Node *nodes, f3 o, f3 inv, f
     if (sp == 0) return;
     const int e = stack.get(--sp);
     cur = e >> 4;
-    mask = e & 15;
+    mask = (e & 15) ? (e & 15) : 0xF;
   }
 }
 
-// W = 2 or 4: the tree the walk uses; W = 0: chosen per call from M.wide
-// (both walks compiled into the caller).
+// W: 2 or 4, the tree width the walks use; 0, either (chosen per call from
+// M.wide, both walks compiled into the caller); 16 + width: that width with
+// the path-line pieces traced too.  Path lines are traced (and shaded) by the
+// W = 0 and W > 16 variants only.
+__host__ __device__ constexpr bool with_curves(int W) { return W == 0 || W > 16; }
+
 template <class Stack, int W = 0>
 // any: an occlusion query; the search ends at the first hit in (t0, tfar].
 __device__ __forceinline__ void model_trace(const DModel &M, f3 org, f3 dir, float t0, float tfar, GeoHit &h,
@@ -333,7 +377,10 @@ __device__ __forceinline__ void model_trace(const DModel &M, f3 org, f3 dir, flo
   h.prim = -1;
   h.u = h.v = 0.f;
   if (tfar < t0) return;
+  constexpr int TW = W & 15;
   f3 inv = mk3(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
+  // 4-wide walks: stack depth up to which other hit children get entries of their own
+  const int dlim = Stack::DEPTH - M.depth4 - 3;
   if (M.ntris > 0) {
     WRay wr = make_wray(org, dir);
     auto tleaf = [&](int start, int count) {
@@ -347,7 +394,7 @@ __device__ __forceinline__ void model_trace(const DModel &M, f3 org, f3 dir, flo
         }
       }
     };
-    if (W == 4 || (W == 0 && M.wide)) bvh4_walk(M.tnodes4, org, inv, t0, h, stack, tleaf, any);
+    if (TW == 4 || (TW == 0 && M.wide)) bvh4_walk(M.tnodes4, org, inv, t0, h, stack, tleaf, any, dlim);
     else bvh_walk(M.tnodes, org, inv, t0, h, stack, tleaf, any);
   }
   if (any && h.geom >= 0) return;
@@ -364,13 +411,13 @@ __device__ __forceinline__ void model_trace(const DModel &M, f3 org, f3 dir, flo
         }
       }
     };
-    if (W == 4 || (W == 0 && M.wide)) bvh4_walk(M.snodes4, org, inv, t0, h, stack, sleaf, any);
+    if (TW == 4 || (TW == 0 && M.wide)) bvh4_walk(M.snodes4, org, inv, t0, h, stack, sleaf, any, dlim);
     else bvh_walk(M.snodes, org, inv, t0, h, stack, sleaf, any);
   }
   if (any && h.geom >= 0) return;
-  if (W == 0 && M.ncurve > 0) {
-    // path-line pieces, in the per-model variant only (models with path lines
-    // are dispatched to it); h.u holds the hit piece's first curve parameter
+  if (with_curves(W) && M.ncurve > 0) {
+    // path-line pieces (models with path lines are dispatched to the variants
+    // that trace them); h.u holds the hit piece's first curve parameter
     // (pieces of one segment that tie in t: the lower one wins, whatever the
     // traversal order), h.v the piece (its sorted index) for the shading
     auto cleaf = [&](int start, int count) {
@@ -387,7 +434,7 @@ __device__ __forceinline__ void model_trace(const DModel &M, f3 org, f3 dir, flo
         }
       }
     };
-    if (M.wide) bvh4_walk(M.cnodes4, org, inv, t0, h, stack, cleaf, any);
+    if (TW == 4 || (TW == 0 && M.wide)) bvh4_walk(M.cnodes4, org, inv, t0, h, stack, cleaf, any, dlim);
     else bvh_walk(M.cnodes, org, inv, t0, h, stack, cleaf, any);
   }
 }
@@ -422,8 +469,8 @@ __device__ __forceinline__ f3 roundcone_normal(const DModel &M, int k, f3 P, flo
 }
 
 // postIntersect for the hit geometry, then DG_NORMALIZE | DG_FACEFORWARD.
-// W: the BVH width of the kernel variant; path lines (traced by the
-// per-model variant, W = 0, only) are shaded there only.
+// W: the walk variant of the kernel (model_trace); path lines are shaded by
+// the variants that trace them only.
 template <int W>
 __device__ void geom_shade(const DModel &M, const GeoHit &gh, f3 org, f3 dir, Hit &hit) {
   const DGeomShade &S = gptr(M.shade)[gh.geom];
@@ -457,7 +504,7 @@ __device__ void geom_shade(const DModel &M, const GeoHit &gh, f3 org, f3 dir, Hi
     Ng = sub3(P, c);
     Ns = Ng;
     if (S.has_tf && S.has_data) col = tf_color(gptr(S.tf.col), S.tf, gptr(S.vdata)[gh.prim]);
-  } else if (W == 0) {
+  } else if (with_curves(W)) {
     // DataDrivenPathLines_postIntersect (DataDrivenPathLines.ispc:212-269):
     // the radius interpolated between the segment's first two control points
     // at the curve parameter, mapped back to a data value for the TF

@@ -195,9 +195,11 @@ int         gxy_get_geometry_stack(void);
  * 0: closest hit everywhere (same frames and counts). */
 int gxy_set_occlusion_rays(int on);
 int gxy_get_occlusion_rays(void);
-/* BVH traversed by geometry traces (no reference counterpart): 4 = the
- * 4-wide trees every model commit also builds (128-byte nodes, about half the
- * depth), 2 = the binary LBVH.  Hits are identical either way. */
+/* BVH traversed by geometry traces (no reference counterpart): 2 (default) =
+ * the binary LBVH, 4 = the 4-wide trees every model commit also builds
+ * (128-byte nodes, about half the depth).  Hits are identical either way.
+ * The binary walk is faster on scenes of several objects and on path lines
+ * (1.3-1.8x on MI355X), the 4-wide one by about 9% on one closed mesh. */
 int         gxy_set_bvh_width(int width);
 int         gxy_get_bvh_width(void);
 int         gxy_get_trace_config(int *kind, int *chunk, int *refill);

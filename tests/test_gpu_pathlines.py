@@ -31,12 +31,23 @@ def gpu_hits(hip, built, hr):
         glib.check(L_.gxy_model_set_hit_output(built.vis.model, None, None))
 
 
+@pytest.fixture(params=[2, 4], ids=["binary", "wide"])
+def bvh_width(request):
+    """Runs the test with path lines walked in the binary and the 4-wide trees."""
+    from galaxy_amd import lib as glib
+    L_ = glib.lib()
+    saved = L_.gxy_get_bvh_width()
+    glib.check(L_.gxy_set_bvh_width(request.param))
+    yield request.param
+    L_.gxy_set_bvh_width(saved)
+
+
 def scene_with(pl, meshes=(), box=1.5):
     b = ((-box,) * 3, (box,) * 3)
     return scenes.SceneDesc(pathlines=[pl], meshes=list(meshes), local_box=b, global_box=b)
 
 
-def test_pathlines_match_oracle(hip, oracle):
+def test_pathlines_match_oracle(hip, oracle, bvh_width):
     mesh = scenes.icosphere(3, 0.25, (0.9, -0.9, 0.0))
     sc = scene_with(helix_desc(seed=3), [mesh])
     Bh, Bo = scenes.build_scene(hip, sc), scenes.build_scene(oracle, sc)
@@ -94,7 +105,7 @@ def test_hits_lie_on_the_swept_surface(hip, oracle):
     assert (np.einsum("ij,ij->i", nrm, d[hit]) < 0).all()
 
 
-def test_pathline_frame_with_shadows_and_ao_matches_oracle(hip, oracle):
+def test_pathline_frame_with_shadows_and_ao_matches_oracle(hip, oracle, bvh_width):
     box = ((-1.2,) * 3, (1.2,) * 3)
     L = scenes.LightingDesc(np.array([[3.0, 4.0, 5.0]], f32), np.array([2], np.int32), Ka=0.3, Kd=0.7, n_ao=2,
                             ao_radius=0.3, shadows=True)
