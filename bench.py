@@ -521,6 +521,7 @@ def main():
     # ---------------------------------------------------------------- extras
     if not args.no_extra:
         extra = {}
+        saved_width = L.gxy_get_bvh_width()
         for name, fn, steps in (("sphere_mesh_512", workloads.sphere_mesh, args.steps),
                                 ("nine_icospheres_4k_ao16", workloads.nine_icospheres, args.icosphere_steps),
                                 ("pathlines_4000x63_segments_1024", workloads.pathlines, args.steps)):
@@ -554,6 +555,8 @@ def main():
             except Exception as e:  # reported, not hidden
                 extra[name] = dict(error=str(e))
         out["extra_workloads"] = extra
+        # the batched and served forms start from the library's BVH width, not the last frame's tuned one
+        glib.check(L.gxy_set_bvh_width(saved_width))
         glib.check(L.gxy_set_opacity_threshold(alpha))
 
     # ---------------------------------------------------------------- batched and served forms

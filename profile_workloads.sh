@@ -20,6 +20,6 @@ timeout -s KILL 400 rocprofv3 --kernel-trace --stats --output-format csv -d $OUT
 echo "== served"
 timeout -s KILL 400 rocprofv3 --kernel-trace --stats --output-format csv -d $OUT/served -o run -- $SERVE > $OUT/served.log 2>&1 || exit 1
 cd $R
-python3 profile_summary.py --stats $OUT/mesh profiles/mesh_frames_kernels.md "Kernel trace, sphere-mesh and nine-icosphere frames (with the volume frame)" "${MESH/$R\//}"
+python3 profile_summary.py --stats $OUT/mesh profiles/mesh_frames_kernels.md "Kernel trace, sphere-mesh, nine-icosphere and path-line frames (with the volume frame)" "${MESH/$R\//}"
 python3 profile_summary.py --stats $OUT/batched profiles/batched_lists_kernels.md "Kernel trace, batched form (256 lists, 64 datasets) and its per-list loop" "${BATCH/$R\//}"
 python3 profile_summary.py --stats $OUT/served profiles/served_stream_kernels.md "Kernel trace, served form (40 requests, continuous and one at a time)" "${SERVE/$R\//}"
