@@ -105,6 +105,15 @@ hipError_t launch_camera(const CameraParams &C, const RayList_ispc &R, int capac
 // b at 16, c at 18)
 hipError_t launch_readback(const int *a, int na, const int *b, const unsigned long long *c, int *out_dev,
                            hipStream_t s);
+// SendPixelsMsg contents: count of TERMINATED rays (device int), then the
+// header and pixels into msg (scratch: block_scan_scratch_bytes(ceil(n/256))
+// bytes, kept between the two calls); applying a message of count pixels
+// (inside: device int, pixels within the framebuffer)
+hipError_t launch_pixels_count(const RayList_ispc &R, int n, int *scratch, int *total, hipStream_t s);
+hipError_t launch_pixels_write(const RayList_ispc &R, int n, int *scratch, const gxy_pixels_header &h, void *msg,
+                               hipStream_t s);
+hipError_t launch_pixels_apply(const void *msg, int count, int frame, float *fb, int width, int height, int *kbuf,
+                               int *inside, hipStream_t s);
 // writes the 64-byte RayList header hdr64 (host memory, copied into the launch) to block
 hipError_t launch_write_header(void *block, const void *hdr64, hipStream_t s);
 hipError_t launch_copy_rays(const RayList_ispc &src, int start, int count, const RayList_ispc &dst, int dst_start,

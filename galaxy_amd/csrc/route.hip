@@ -259,6 +259,119 @@ hipError_t launch_accumulate_sampler(const RayList_ispc &R, int n, float *xyz, i
   return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ pixel messages
+// HandleTerminatedRays' remote path (Renderer.cpp:490-499): the pixels of the
+// TERMINATED rays in ray order, after the SendPixelsMsg header.
+__global__ __launch_bounds__(GXY_BLOCK) void k_pix_count(RayList_ispc R, int n, int *bc) {
+  __shared__ int ws[GXY_BLOCK / 64];
+  int i = blockIdx.x * GXY_BLOCK + threadIdx.x;
+  bool f = i < n && R.classification[i] == GXY_TERMINATED;
+  unsigned long long m = __ballot(f);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = __popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    bc[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+    if (blockIdx.x == 0) bc[gridDim.x] = 0;
+  }
+}
+
+__global__ __launch_bounds__(GXY_BLOCK) void k_pix_write(RayList_ispc R, int n, const int *bbase, gxy_pixel *out) {
+  __shared__ int ws[GXY_BLOCK / 64];
+  int i = blockIdx.x * GXY_BLOCK + threadIdx.x;
+  bool f = i < n && R.classification[i] == GXY_TERMINATED;
+  unsigned long long m = __ballot(f);
+  int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) ws[w] = __popcll(m);
+  __syncthreads();
+  if (!f) return;
+  int off = bbase[blockIdx.x] + __popcll(m & lanemask_lt64());
+  for (int q = 0; q < w; q++) off += ws[q];
+  gxy_pixel p;
+  p.x = R.x[i];
+  p.y = R.y[i];
+  p.r = R.r[i];
+  p.g = R.g[i];
+  p.b = R.b[i];
+  p.o = R.o[i];
+  out[off] = p;
+}
+
+__global__ void k_pix_header(gxy_pixels_header h, gxy_pixels_header *dst) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = h;
+}
+
+hipError_t launch_pixels_count(const RayList_ispc &R, int n, int *scratch, int *total, hipStream_t s) {
+  int nb = (n + GXY_BLOCK - 1) / GXY_BLOCK;
+  if (nb == 0) {
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, total, 1);
+    return hipGetLastError();
+  }
+  int *off = block_scan_offsets(scratch, nb);
+  hipLaunchKernelGGL(k_pix_count, dim3(nb), dim3(GXY_BLOCK), 0, s, R, n, scratch);
+  hipError_t e = launch_block_scan(scratch, nb, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_copy_total, dim3(1), dim3(1), 0, s, off + nb, total);
+  return hipGetLastError();
+}
+
+hipError_t launch_pixels_write(const RayList_ispc &R, int n, int *scratch, const gxy_pixels_header &h, void *msg,
+                               hipStream_t s) {
+  int nb = (n + GXY_BLOCK - 1) / GXY_BLOCK;
+  gxy_pixel *px = (gxy_pixel *)((char *)msg + sizeof(gxy_pixels_header));
+  if (nb > 0)
+    hipLaunchKernelGGL(k_pix_write, dim3(nb), dim3(GXY_BLOCK), 0, s, R, n, block_scan_offsets(scratch, nb), px);
+  hipLaunchKernelGGL(k_pix_header, dim3(1), dim3(64), 0, s, h, (gxy_pixels_header *)msg);
+  return hipGetLastError();
+}
+
+// Rendering.cpp:104-121 (ACCUMULATE_PIXEL with frame stamps): the one
+// work-item that raises a stale stamp clears the pixel; the additions follow
+// in a second launch.
+__global__ __launch_bounds__(GXY_BLOCK) void k_pix_stamp(const gxy_pixel *p, int count, int frame, float *fb,
+                                                         int width, int height, int *kbuf) {
+  int j = blockIdx.x * GXY_BLOCK + threadIdx.x;
+  if (j >= count) return;
+  const int x = p[j].x, y = p[j].y;
+  if (x < 0 || y < 0 || x >= width || y >= height) return;
+  const int64_t off = (int64_t)y * width + x;
+  if (kbuf[off] < frame && atomicMax(&kbuf[off], frame) < frame) {
+    fb[4 * off + 0] = 0.f;
+    fb[4 * off + 1] = 0.f;
+    fb[4 * off + 2] = 0.f;
+    fb[4 * off + 3] = 0.f;
+  }
+}
+
+__global__ __launch_bounds__(GXY_BLOCK) void k_pix_add(const gxy_pixel *p, int count, float *fb, int width,
+                                                       int height, int *inside) {
+  int j = blockIdx.x * GXY_BLOCK + threadIdx.x;
+  bool in = false;
+  if (j < count) {
+    const gxy_pixel q = p[j];
+    in = q.x >= 0 && q.y >= 0 && q.x < width && q.y < height;
+    if (in) {
+      float *d = fb + 4 * ((int64_t)q.y * width + q.x);
+      atomicAdd(d + 0, q.r);
+      atomicAdd(d + 1, q.g);
+      atomicAdd(d + 2, q.b);
+      atomicAdd(d + 3, q.o);
+    }
+  }
+  unsigned long long m = __ballot(in);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(inside, __popcll(m));
+}
+
+hipError_t launch_pixels_apply(const void *msg, int count, int frame, float *fb, int width, int height, int *kbuf,
+                               int *inside, hipStream_t s) {
+  const gxy_pixel *px = (const gxy_pixel *)((const char *)msg + sizeof(gxy_pixels_header));
+  hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, inside, 1);
+  if (count <= 0) return hipGetLastError();
+  const int nb = (count + GXY_BLOCK - 1) / GXY_BLOCK;
+  if (kbuf) hipLaunchKernelGGL(k_pix_stamp, dim3(nb), dim3(GXY_BLOCK), 0, s, px, count, frame, fb, width, height, kbuf);
+  hipLaunchKernelGGL(k_pix_add, dim3(nb), dim3(GXY_BLOCK), 0, s, px, count, fb, width, height, inside);
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ camera
 // Box::intersect (Box.cpp:107-150)
 __device__ __forceinline__ bool box_intersect(const float *lo, const float *hi, f3 org, f3 dir, float &tmin,

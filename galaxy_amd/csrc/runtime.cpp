@@ -2007,6 +2007,74 @@ GXY_EXPORT int gxy_accumulate_terminated(int n, RayList_ispc *R, float *fb, int 
   return route(n, *R, RM_ACCUMULATE, 1, nullptr, nullptr, fb, w, h);
 }
 
+// SendPixelsMsg (Renderer.h:246-305) from HandleTerminatedRays' remote path
+// (Renderer.cpp:490-499): count, check the capacity, then header + pixels
+GXY_EXPORT int gxy_pixels_pack(int n, RayList_ispc *R, int64_t rkey, int64_t rskey, int frame, int source,
+                               void *msg, size_t capacity_bytes, int *count_out) {
+  INIT_OR(-1);
+  if (n < 0 || !R) return fail("gxy_pixels_pack: %d rays", n);
+  if (!msg || ((uintptr_t)msg & 7)) return fail("gxy_pixels_pack: message buffer null or not 8-byte aligned");
+  if (capacity_bytes < sizeof(gxy_pixels_header)) return fail("gxy_pixels_pack: %zu bytes hold no header", capacity_bytes);
+  Scratch sc;
+  const int nb = (n + GXY_BLOCK - 1) / GXY_BLOCK;
+  int *scratch = (int *)sc.get(block_scan_scratch_bytes(nb));
+  int *total = (int *)sc.get(sizeof(int));
+  if (!scratch || !total) return -1;
+  {
+    KTimer kt("pixels");
+    HIPCHK(launch_pixels_count(*R, n, scratch, total, S()));
+  }
+  Readback rb;
+  if (readback(nullptr, 0, total, nullptr, rb) != 0) return -1;
+  const int count = rb.one;
+  const size_t need = sizeof(gxy_pixels_header) + sizeof(gxy_pixel) * (size_t)count;
+  if (need > capacity_bytes) return fail("gxy_pixels_pack: %d pixels need %zu bytes, capacity %zu", count, need,
+                                         capacity_bytes);
+  gxy_pixels_header h;
+  memset(&h, 0, sizeof h);
+  h.rkey = rkey;
+  h.rskey = rskey;
+  h.count = count;
+  h.frame = frame;
+  h.source = source;
+  {
+    KTimer kt("pixels");
+    HIPCHK(launch_pixels_write(*R, n, scratch, h, msg, S()));
+  }
+  HIPCHK(hipStreamSynchronize(S()));
+  if (count_out) *count_out = count;
+  return 0;
+}
+
+// SendPixelsMsg::Action + Rendering::AddLocalPixels (Renderer.h:313-336,
+// Rendering.cpp:126-153): the header is read from the device message
+GXY_EXPORT int gxy_pixels_apply(const void *msg, size_t bytes, float *fb, int w, int h, int *kbuf, int *fb_frame,
+                                int *applied) {
+  INIT_OR(-1);
+  if (applied) *applied = 0;
+  if (!msg || ((uintptr_t)msg & 7)) return fail("gxy_pixels_apply: message null or not 8-byte aligned");
+  if (bytes < sizeof(gxy_pixels_header)) return fail("gxy_pixels_apply: %zu bytes hold no header", bytes);
+  if (!fb || !fb_frame || w <= 0 || h <= 0) return fail("gxy_pixels_apply: framebuffer %dx%d", w, h);
+  Readback rb;
+  // count, frame, source: the header's ints 4..6
+  if (readback((const int *)msg + 4, 3, nullptr, nullptr, rb) != 0) return -1;
+  const int count = rb.ints[0], frame = rb.ints[1];
+  if (count < 0 || sizeof(gxy_pixels_header) + sizeof(gxy_pixel) * (size_t)count > bytes)
+    return fail("gxy_pixels_apply: header count %d does not fit %zu bytes", count, bytes);
+  if (frame < *fb_frame) return 0;   // an older frame's pixels are dropped
+  *fb_frame = frame;
+  Scratch sc;
+  int *inside = (int *)sc.get(sizeof(int));
+  if (!inside) return -1;
+  {
+    KTimer kt("pixels");
+    HIPCHK(launch_pixels_apply(msg, count, frame, fb, w, h, kbuf, inside, S()));
+  }
+  if (readback(nullptr, 0, inside, nullptr, rb) != 0) return -1;
+  if (applied) *applied = rb.one;
+  return 0;
+}
+
 GXY_EXPORT int gxy_partition_rays(int n, RayList_ispc *R, int ndest, RayList_ispc *const *lists,
                                   RayList_ispc *keepers, int *counts) {
   INIT_OR(-1);

@@ -79,6 +79,8 @@ def _declare(L):
         "gxy_classify": (C.c_int, [C.c_int, vp, C.c_int]),
         "gxy_assign_destinations": (C.c_int, [C.c_int, vp, vp, vp, C.c_int]),
         "gxy_accumulate_terminated": (C.c_int, [C.c_int, vp, vp, C.c_int, C.c_int]),
+        "gxy_pixels_pack": (C.c_int, [C.c_int, vp, C.c_int64, C.c_int64, C.c_int, C.c_int, vp, C.c_size_t, vp]),
+        "gxy_pixels_apply": (C.c_int, [vp, C.c_size_t, vp, C.c_int, C.c_int, vp, vp, vp]),
         "gxy_partition_rays": (C.c_int, [C.c_int, vp, C.c_int, vp, vp, vp]),
         "gxy_camera_generate": (C.c_int, [vp, vp, C.c_int, vp, C.c_float, C.c_int, C.c_int, vp, vp, vp, C.c_int,
                                           C.c_int, vp]),

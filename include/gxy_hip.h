@@ -73,6 +73,20 @@ typedef struct gxy_raylist_hdr {
   int32_t pad_[5];
 } gxy_raylist_hdr;
 
+/* Renderer::SendPixelsMsg contents (src/renderer/Renderer.h:246-262,
+ * src/renderer/Pixel.h:33-37): this 32-byte header (two 8-byte Keys, count,
+ * frame, sending rank, tail padding to 8-byte alignment), then count pixels of
+ * 24 bytes. */
+typedef struct gxy_pixels_header {
+  int64_t rkey, rskey;
+  int32_t count, frame, source;
+  int32_t pad_;
+} gxy_pixels_header;
+typedef struct gxy_pixel {
+  int32_t x, y;
+  float r, g, b, o;
+} gxy_pixel;
+
 /* ---- Lighting_ispc: src/renderer/Lighting.ih:23-32 ------------------------ */
 typedef struct Lighting_ispc {
   int      n_ao_rays;
@@ -313,6 +327,24 @@ int gxy_assign_destinations(int nRays, RayList_ispc *rays, const gxy_box3f *loca
  * Rendering.cpp:125-153): fb[(y*width+x)*4 + c] += rgba for every TERMINATED
  * ray.  fb is a device float array of width*height*4. */
 int gxy_accumulate_terminated(int nRays, RayList_ispc *rays, float *fb, int width, int height);
+/* HandleTerminatedRays' remote path (Renderer.cpp:490-499) with
+ * SendPixelsMsg::SendPixelsMsg and StashPixel (Renderer.h:267-305): writes the
+ * message contents into msg (device memory, 8-byte aligned, capacity_bytes):
+ * the header {rkey, rskey, count, frame, source}, then the pixels of the
+ * TERMINATED rays in ray order.  *count_out = pixels.  Fails with nothing
+ * written when 32 + 24*count exceeds capacity_bytes.  Synchronises. */
+int gxy_pixels_pack(int nRays, RayList_ispc *rays, int64_t rkey, int64_t rskey, int frame, int source,
+                    void *msg, size_t capacity_bytes, int *count_out);
+/* SendPixelsMsg::Action + Rendering::AddLocalPixels (Renderer.h:313-336,
+ * Rendering.cpp:104-153) for a message in device memory (bytes long): when
+ * its frame is not older than *fb_frame, *fb_frame takes it and every pixel
+ * is added to fb (width*height*4 floats, FP32 atomics); with kbuffer
+ * (width*height device ints of per-pixel frame stamps, may be NULL) a pixel
+ * whose stamp is older than the frame is cleared and stamped first.  Pixels
+ * outside the framebuffer are skipped (the reference does not check them).
+ * *applied = pixels added (0 for an older frame).  Synchronises. */
+int gxy_pixels_apply(const void *msg, size_t bytes, float *fb, int width, int height, int *kbuffer,
+                     int *fb_frame, int *applied);
 /* processRays_task::work's split (Renderer.cpp:561-634): stable partition of the
  * rays by classification into ndest per-destination device lists plus keepers.
  * counts[0..ndest-1] = rays for destination rank d, counts[ndest] = keepers.

@@ -26,6 +26,9 @@
 //                                   brute force (Embree's own tie order unpinned)
 //   * classify / route / counts   - hand-enumerated cases of Renderer.cpp rules
 //   * camera                      - gold images; FUZZ / box rules by cases
+//   * pixel messages              - byte layout against the reference struct
+//                                   declarations; pack + apply equal to direct
+//                                   accumulation; frame stamps by a hand case
 //
 // Numerics: compiled with -ffp-contract=off; min/max are the a<b?a:b forms of
 // ISPC; the ISPC fast-math rcp/rsqrt are taken as correctly rounded 1/x and
@@ -1524,6 +1527,71 @@ OEXPORT int oracle_accumulate(int n, RayList_ispc *R, float *fb, int w, int h) {
       p[0] += R->r[i]; p[1] += R->g[i]; p[2] += R->b[i]; p[3] += R->o[i];
       k++;
     }
+  return k;
+}
+
+// SendPixelsMsg contents (Renderer.h:246-305, Pixel.h:33-37) as
+// HandleTerminatedRays' remote path fills them (Renderer.cpp:490-499): a
+// 32-byte header (int64 rkey, int64 rskey, int32 count, frame, source, 4 bytes
+// of padding) then, per TERMINATED ray in ray order, int32 x, y and float
+// r, g, b, o.  Returns the bytes written, or -1 when cap is too small.
+OEXPORT long long oracle_pixels_pack(int n, RayList_ispc *R, long long rkey, long long rskey, int frame,
+                                     int source, unsigned char *out, long long cap) {
+  int count = 0;
+  for (int i = 0; i < n; i++)
+    if (R->classification[i] == GXY_TERMINATED) count++;
+  const long long need = 32 + 24LL * count;
+  if (need > cap) return -1;
+  memset(out, 0, 32);
+  memcpy(out + 0, &rkey, 8);
+  memcpy(out + 8, &rskey, 8);
+  memcpy(out + 16, &count, 4);
+  memcpy(out + 20, &frame, 4);
+  memcpy(out + 24, &source, 4);
+  unsigned char *p = out + 32;
+  for (int i = 0; i < n; i++)
+    if (R->classification[i] == GXY_TERMINATED) {
+      memcpy(p + 0, &R->x[i], 4);
+      memcpy(p + 4, &R->y[i], 4);
+      memcpy(p + 8, &R->r[i], 4);
+      memcpy(p + 12, &R->g[i], 4);
+      memcpy(p + 16, &R->b[i], 4);
+      memcpy(p + 20, &R->o[i], 4);
+      p += 24;
+    }
+  return need;
+}
+
+// SendPixelsMsg::Action + Rendering::AddLocalPixels (Renderer.h:313-336,
+// Rendering.cpp:104-153): a message of an older frame than *frame is
+// dropped; otherwise *frame takes the message's frame and each pixel, in
+// order, clears its framebuffer entry when its stamp (kbuf, may be null) is
+// older and is then added.  Pixels outside w x h are skipped.  Returns the
+// pixels added.
+OEXPORT int oracle_pixels_apply(const unsigned char *msg, float *fb, int w, int h, int *kbuf, int *frame) {
+  int count, f;
+  memcpy(&count, msg + 16, 4);
+  memcpy(&f, msg + 20, 4);
+  if (f < *frame) return 0;
+  *frame = f;
+  int k = 0;
+  const unsigned char *p = msg + 32;
+  for (int j = 0; j < count; j++, p += 24) {
+    int x, y;
+    float v[4];
+    memcpy(&x, p, 4);
+    memcpy(&y, p + 4, 4);
+    memcpy(v, p + 8, 16);
+    if (x < 0 || y < 0 || x >= w || y >= h) continue;
+    const size_t off = (size_t)y * w + x;
+    float *d = fb + 4 * off;
+    if (kbuf && kbuf[off] < f) {
+      d[0] = d[1] = d[2] = d[3] = 0.f;
+      kbuf[off] = f;
+    }
+    for (int c = 0; c < 4; c++) d[c] += v[c];
+    k++;
+  }
   return k;
 }
 

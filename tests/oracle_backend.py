@@ -68,6 +68,11 @@ def lib():
         L.oracle_assign_destinations.argtypes = [C.c_int, vp, vp, vp, C.c_int]
         L.oracle_accumulate.restype = C.c_int
         L.oracle_accumulate.argtypes = [C.c_int, vp, vp, C.c_int, C.c_int]
+        L.oracle_pixels_pack.restype = C.c_longlong
+        L.oracle_pixels_pack.argtypes = [C.c_int, vp, C.c_longlong, C.c_longlong, C.c_int, C.c_int, vp,
+                                         C.c_longlong]
+        L.oracle_pixels_apply.restype = C.c_int
+        L.oracle_pixels_apply.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp]
         L.oracle_partition_counts.restype = C.c_int
         L.oracle_partition_counts.argtypes = [C.c_int, vp, C.c_int, vp]
         L.oracle_SamplerTraceRays.argtypes = [vp, C.c_int, vp]
