@@ -59,12 +59,17 @@ hipError_t launch_exclusive_sum(const int *in, int *out, int m, void *temp, hipS
 // bytes, the count kernel writes nb counts and a zero after them at scratch,
 // and launch_block_scan writes their exclusive sum (nb + 1 entries, the total
 // last) at block_scan_offsets(scratch, nb).
+// (one workgroup for count arrays of at most GXY_SMALL_SCAN entries, the
+// device-wide scan above that)
+#define GXY_SMALL_SCAN 16384
 size_t block_scan_scratch_bytes(int nb);
 int *block_scan_offsets(int *scratch, int nb);
 hipError_t launch_block_scan(int *scratch, int nb, hipStream_t s);
 // Exclusive offsets of primary surface rays (flag scan).  scratch holds
-// surface_scan_scratch_bytes(n) bytes; total (device int) receives the count.
+// surface_scan_scratch_bytes(n) bytes; total (device int, may be null)
+// receives the count, which also stays at surface_scan_total(scratch, n).
 size_t surface_scan_scratch_bytes(int n);
+int *surface_scan_total(int *scratch, int n);
 hipError_t launch_surface_scan(const RayList_ispc &R, int n, int *scratch, int *rank_out, int *total,
                                hipStream_t s, int *pidx = nullptr);
 // Lighting with one work-item per spawned child (children at the offsets of
@@ -95,15 +100,16 @@ hipError_t launch_partition(const RayList_ispc &R, int n, const int *cat_table, 
 hipError_t launch_partition_scatter(const RayList_ispc &R, int n, const int *cat_table, int ntable, int K,
                                     const RayList_ispc *lists, const int *scratch, hipStream_t s);
 // scratch: block_scan_scratch_bytes(ceil(n/256)) bytes (camera: of the pixel
-// order's ray slots)
+// order's ray slots; total may be null: the count stays at
+// block_scan_offsets(scratch, nb)[nb])
 hipError_t launch_accumulate_sampler(const RayList_ispc &R, int n, float *xyz, int capacity, int *count,
                                      int *scratch, hipStream_t s);
 hipError_t launch_camera(const CameraParams &C, const RayList_ispc &R, int capacity, int *scratch, int *total,
                          hipStream_t s);
 // one kernel copying up to 16 ints (a), one int (b, optional) and one 64-bit
-// counter (c, optional) into the mapped host area out_dev (20 ints: a at 0,
-// b at 16, c at 18)
-hipError_t launch_readback(const int *a, int na, const int *b, const unsigned long long *c, int *out_dev,
+// counter (c, optional) into the mapped host area out_dev (a at 0, b at 16,
+// c at 18), then seq at 32 once those are visible
+hipError_t launch_readback(const int *a, int na, const int *b, const unsigned long long *c, int *out_dev, int seq,
                            hipStream_t s);
 // SendPixelsMsg contents: count of TERMINATED rays (device int), then the
 // header and pixels into msg (scratch: block_scan_scratch_bytes(ceil(n/256))

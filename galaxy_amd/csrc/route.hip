@@ -91,6 +91,32 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_part_count(RayList_ispc R, int n,
   if (blockIdx.x == 0 && threadIdx.x == 0) counts[K * nb] = 0;
 }
 
+// one workgroup per category: exclusive sum of its nb counts (relative to the
+// category's start) and the category's total, for nb <= GXY_SMALL_SCAN
+__global__ __launch_bounds__(1024) void k_part_scan_small(const int *counts, int nb, int *off, int *tot) {
+  __shared__ int part[1024];
+  const int64_t base0 = (int64_t)blockIdx.x * nb;
+  const int t = threadIdx.x, per = (nb + 1023) / 1024;
+  const int s0 = min(nb, t * per), s1 = min(nb, s0 + per);
+  int sum = 0;
+  for (int k = s0; k < s1; k++) sum += counts[base0 + k];
+  part[t] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int base = t > 0 ? part[t - 1] : 0;
+  for (int k = s0; k < s1; k++) {
+    const int v = counts[base0 + k];
+    off[base0 + k] = base;
+    base += v;
+  }
+  if (t == 1023) tot[blockIdx.x] = part[1023];
+}
+
 // category totals from the exclusive sum over the category-major counts
 __global__ void k_part_totals(const int *off, int nb, int K, int *tot) {
   const int k = threadIdx.x;
@@ -170,9 +196,14 @@ hipError_t launch_partition(const RayList_ispc &R, int n, const int *cat_table, 
   const size_t pad = part_pad(n, K);
   int *off = scratch + pad;
   hipLaunchKernelGGL(k_part_count, dim3(nb), dim3(GXY_BLOCK), 0, s, R, n, T, K, nb, scratch);
-  hipError_t e = launch_exclusive_sum(scratch, off, K * nb + 1, scratch + 2 * pad, s);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_part_totals, dim3(1), dim3(64), 0, s, off, nb, K, counts);
+  if (nb <= GXY_SMALL_SCAN) {
+    // offsets relative to each category's start (the scatter subtracts off[cat * nb] = 0)
+    hipLaunchKernelGGL(k_part_scan_small, dim3(K), dim3(1024), 0, s, scratch, nb, off, counts);
+  } else {
+    hipError_t e = launch_exclusive_sum(scratch, off, K * nb + 1, scratch + 2 * pad, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_part_totals, dim3(1), dim3(64), 0, s, off, nb, K, counts);
+  }
   if (lists) return launch_partition_scatter(R, n, cat_table, ntable, K, lists, scratch, s);
   return hipGetLastError();
 }
@@ -229,19 +260,25 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_sampler_write(RayList_ispc R, int
 __global__ void k_add_count(int *count, const int *add) { *count += *add; }
 
 // The values the host waits for after a list, written straight into mapped
-// host memory: out[0..na) = a, out[16] = *b, out[18..19] = *c (64-bit).
-__global__ void k_readback(const int *a, int na, const int *b, const unsigned long long *c, int *out) {
+// host memory: out[0..na) = a, out[16] = *b, out[18..19] = *c (64-bit), and
+// out[32] = seq once the others are visible to the host.
+__global__ void k_readback(const int *a, int na, const int *b, const unsigned long long *c, int *out, int seq) {
   const int t = threadIdx.x;
   if (t < na) out[t] = a[t];
   if (t == 16 && b) out[16] = *b;
   if (t == 18 && c) *(unsigned long long *)(out + 18) = *c;
   __threadfence_system();
+  __syncthreads();
+  if (t == 0) {
+    *(volatile int *)(out + 32) = seq;
+    __threadfence_system();
+  }
 }
 
-hipError_t launch_readback(const int *a, int na, const int *b, const unsigned long long *c, int *out_dev,
+hipError_t launch_readback(const int *a, int na, const int *b, const unsigned long long *c, int *out_dev, int seq,
                            hipStream_t s) {
   if (na < 0 || na > 16) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_readback, dim3(1), dim3(64), 0, s, a, na, b, c, out_dev);
+  hipLaunchKernelGGL(k_readback, dim3(1), dim3(64), 0, s, a, na, b, c, out_dev, seq);
   return hipGetLastError();
 }
 __global__ void k_copy_total(const int *src, int *dst) { *dst = *src; }
@@ -492,16 +529,18 @@ hipError_t launch_camera(const CameraParams &C, const RayList_ispc &R, int capac
   if (C.order == 0) npix = C.width * C.height;
   else npix = ((C.width + 7) / 8) * ((C.height + 7) / 8) * 64;
   int nb = (npix + GXY_BLOCK - 1) / GXY_BLOCK;
-  if (nb <= 0) {
-    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, total, 1);
+  if (nb <= 0) nb = 0;
+  int *off = block_scan_offsets(scratch, nb);
+  if (nb == 0) {
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, off, 1);
+    if (total) hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, total, 1);
     return hipGetLastError();
   }
-  int *off = block_scan_offsets(scratch, nb);
   hipLaunchKernelGGL(k_camera_count, dim3(nb), dim3(GXY_BLOCK), 0, s, C, npix, scratch);
   hipError_t e = launch_block_scan(scratch, nb, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_camera_write, dim3(nb), dim3(GXY_BLOCK), 0, s, C, npix, off, R, capacity);
-  hipLaunchKernelGGL(k_copy_total, dim3(1), dim3(1), 0, s, off + nb, total);
+  if (total) hipLaunchKernelGGL(k_copy_total, dim3(1), dim3(1), 0, s, off + nb, total);
   return hipGetLastError();
 }
 

@@ -66,6 +66,7 @@ struct Ctx {
   hipStream_t stream = nullptr;
   unsigned long long *counters = nullptr;
   int *hread = nullptr, *dread = nullptr;   // mapped host readback area (host / device address)
+  int rb_seq = 0;                            // sequence number of the last readback
   std::multimap<size_t, void *> free_blocks;
   size_t bytes_cached = 0;
 };
@@ -187,6 +188,8 @@ static int *readback_area(Ctx &c) {
     c.hread = G.free_reads.back().first;
     c.dread = G.free_reads.back().second;
     G.free_reads.pop_back();
+    memset(c.hread, 0, 64 * sizeof(int));   // no sequence number of its last owner
+    c.rb_seq = 0;
     return c.dread;
   }
   void *h = nullptr, *d = nullptr;
@@ -203,6 +206,8 @@ static int *readback_area(Ctx &c) {
   }
   c.hread = (int *)h;
   c.dread = (int *)d;
+  memset(c.hread, 0, 64 * sizeof(int));
+  c.rb_seq = 0;
   return c.dread;
 }
 
@@ -212,15 +217,29 @@ struct Readback {
   unsigned long long u64;
 };
 
-// Waits for the stream and returns up to 16 device ints, one device int and
-// one device 64-bit counter (each optional).
+// Returns up to 16 device ints, one device int and one device 64-bit counter
+// (each optional) once the stream has reached them.  The host polls the
+// sequence number the readback kernel writes last (the kernel runs after all
+// work queued before it); every 256 polls it asks the stream, so an error or
+// a kernel that never wrote ends the wait through hipStreamSynchronize.
 static int readback(const int *ints, int nints, const int *one, const unsigned long long *u64, Readback &out) {
   Ctx &cx = CX();
   int *dev = readback_area(cx);
   if (!dev) return -1;
-  HIPCHK(launch_readback(ints, nints, one, u64, dev, S()));
-  HIPCHK(hipStreamSynchronize(S()));
+  const int seq = ++cx.rb_seq;
+  HIPCHK(launch_readback(ints, nints, one, u64, dev, seq, S()));
   const volatile int *h = cx.hread;
+  for (unsigned spins = 1; h[32] != seq; spins++) {
+    if ((spins & 255) == 0) {
+      const hipError_t q = hipStreamQuery(S());
+      if (q == hipErrorNotReady) continue;
+      (void)hipGetLastError();
+      HIPCHK(hipStreamSynchronize(S()));
+      if (h[32] != seq) return fail("readback %d did not arrive", seq);
+      break;
+    }
+    __builtin_ia32_pause();
+  }
   for (int k = 0; k < nints; k++) out.ints[k] = h[k];
   out.one = one ? h[16] : 0;
   out.u64 = u64 ? ((unsigned long long)(unsigned)h[19] << 32) | (unsigned)h[18] : 0;
@@ -1947,11 +1966,11 @@ static int spawn_offsets(int n, const RayList_ispc &R, int nAO, int nL, int doSh
   Scratch sc;
   int *scratch = (int *)sc.get(surface_scan_scratch_bytes(n));
   int *rank = (int *)sc.get(sizeof(int) * std::max(1, n));
-  int *total = (int *)sc.get(sizeof(int));
-  if (!scratch || !rank || !total) return -1;
+  if (!scratch || !rank) return -1;
+  int *total = surface_scan_total(scratch, n);
   {
     KTimer kt("spawn_scan");
-    HIPCHK(launch_surface_scan(R, n, scratch, rank, total, S()));
+    HIPCHK(launch_surface_scan(R, n, scratch, rank, nullptr, S()));
     HIPCHK(launch_offsets_from_rank(R, n, rank, total, nAO, nL, doSh, ao_off, sh_off, S()));
   }
   Readback rb;
@@ -2170,11 +2189,11 @@ static int camera_generate(const float eye[3], const float cdir[3], int is_dir, 
   Scratch sc;
   int nb = (npix + GXY_BLOCK - 1) / GXY_BLOCK;
   int *scratch = (int *)sc.get(block_scan_scratch_bytes(nb));
-  int *total = (int *)sc.get(sizeof(int));
-  if (!scratch || !total) return -1;
+  if (!scratch) return -1;
+  int *total = block_scan_offsets(scratch, nb) + nb;
   {
     KTimer kt("camera");
-    HIPCHK(launch_camera(C, R, capacity, scratch, total, S()));
+    HIPCHK(launch_camera(C, R, capacity, scratch, nullptr, S()));
   }
   Readback rb;
   if (readback(nullptr, 0, total, nullptr, rb) != 0) return -1;
@@ -2282,11 +2301,11 @@ static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
       // every ray being a parent and its size set afterwards
       scr_spawn = (int *)spawn_sc.get(surface_scan_scratch_bytes(n));
       int *pidx = (int *)spawn_sc.get(sizeof(int) * std::max(1, n));
-      int *total = (int *)spawn_sc.get(sizeof(int));
-      if (!scr_spawn || !pidx || !total) return -1;
+      if (!scr_spawn || !pidx) return -1;
+      int *total = surface_scan_total(scr_spawn, n);
       {
         KTimer kt("spawn_scan");
-        HIPCHK(launch_surface_scan(R, n, scr_spawn, nullptr, total, S(), pidx));
+        HIPCHK(launch_surface_scan(R, n, scr_spawn, nullptr, nullptr, S(), pidx));
       }
       nsp = C.L.nAO + (C.L.doShadows ? C.L.nLights : 0);
       if ((long long)n * nsp > INT32_MAX - 16) return fail("%lld secondary rays exceed one RayList", (long long)n * nsp);

@@ -82,27 +82,62 @@ size_t block_scan_scratch_bytes(int nb) {
 
 int *block_scan_offsets(int *scratch, int nb) { return scratch + scan_pad((size_t)nb + 1); }
 
+// Exclusive sum of m ints (in -> out) by one workgroup of 1024 work-items,
+// each summing a run of ceil(m / 1024): one launch where the device-wide scan
+// takes two, for the count arrays of lists up to a few million rays.
+__global__ __launch_bounds__(1024) void k_scan_small(const int *in, int *out, int m) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x, per = (m + 1023) / 1024;
+  const int s0 = min(m, t * per), s1 = min(m, s0 + per);
+  int sum = 0;
+  for (int k = s0; k < s1; k++) sum += in[k];
+  part[t] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int base = t > 0 ? part[t - 1] : 0;
+  for (int k = s0; k < s1; k++) {
+    const int v = in[k];
+    out[k] = base;
+    base += v;
+  }
+}
+
 hipError_t launch_block_scan(int *scratch, int nb, hipStream_t s) {
-  return launch_exclusive_sum(scratch, block_scan_offsets(scratch, nb), nb + 1,
-                              scratch + 2 * scan_pad((size_t)nb + 1), s);
+  int *off = block_scan_offsets(scratch, nb);
+  if (nb + 1 <= GXY_SMALL_SCAN) {
+    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, s, scratch, off, nb + 1);
+    return hipGetLastError();
+  }
+  return launch_exclusive_sum(scratch, off, nb + 1, scratch + 2 * scan_pad((size_t)nb + 1), s);
 }
 
 size_t surface_scan_scratch_bytes(int n) { return block_scan_scratch_bytes((n + GXY_BLOCK - 1) / GXY_BLOCK); }
 
+int *surface_scan_total(int *scratch, int n) {
+  const int nb = (n + GXY_BLOCK - 1) / GXY_BLOCK;
+  return block_scan_offsets(scratch, nb) + nb;
+}
+
 hipError_t launch_surface_scan(const RayList_ispc &R, int n, int *scratch, int *rank_out, int *total,
                                hipStream_t s, int *pidx) {
   int nb = (n + GXY_BLOCK - 1) / GXY_BLOCK;
-  if (nb == 0) {
-    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, total, (const int *)nullptr, 0);
-    return hipGetLastError();
-  }
   // block counts with a zero after them, and their exclusive sum (the total in its last slot)
   int *off = block_scan_offsets(scratch, nb);
+  if (nb == 0) {
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, off, (const int *)nullptr, 0);
+    if (total) hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, total, (const int *)nullptr, 0);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(k_flag_count, dim3(nb), dim3(GXY_BLOCK), 0, s, R, n, scratch);
   hipError_t e = launch_block_scan(scratch, nb, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_flag_rank, dim3(nb), dim3(GXY_BLOCK), 0, s, R, n, off, rank_out, pidx);
-  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, total, (const int *)(off + nb), 0);
+  if (total) hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, total, (const int *)(off + nb), 0);
   return hipGetLastError();
 }
 
