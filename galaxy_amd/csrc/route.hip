@@ -176,8 +176,9 @@ static inline size_t part_pad(int n, int K) {
 }
 
 size_t partition_scratch_bytes(int n, int K) {
-  const int m = K * ((n + GXY_BLOCK - 1) / GXY_BLOCK) + 1;
-  return 2 * sizeof(int) * part_pad(n, K) + scan_temp_bytes(m);
+  const int nb = (n + GXY_BLOCK - 1) / GXY_BLOCK;
+  // temp storage of the device-wide scan only on the path that runs it
+  return 2 * sizeof(int) * part_pad(n, K) + (nb > GXY_SMALL_SCAN ? scan_temp_bytes(K * nb + 1) : 0);
 }
 
 __global__ void k_zero_ints(int *p, int m) {

@@ -2180,9 +2180,11 @@ static void camera_params(const float eye[3], const float cdir[3], int is_dir, c
   C.order = order;
 }
 
+// counter_out (may be null): the context's sample counter as of the camera
+// rays, read in the same wait
 static int camera_generate(const float eye[3], const float cdir[3], int is_dir, const float up[3], float aov,
                            int w, int h, const gxy_box3f *lb, const gxy_box3f *gb, const RayList_ispc &R,
-                           int capacity, int order, int *n_out) {
+                           int capacity, int order, int *n_out, unsigned long long *counter_out = nullptr) {
   CameraParams C;
   camera_params(eye, cdir, is_dir, up, aov, w, h, lb, gb, order, C);
   int npix = order == 0 ? w * h : ((w + 7) / 8) * ((h + 7) / 8) * 64;
@@ -2196,8 +2198,9 @@ static int camera_generate(const float eye[3], const float cdir[3], int is_dir, 
     HIPCHK(launch_camera(C, R, capacity, scratch, nullptr, S()));
   }
   Readback rb;
-  if (readback(nullptr, 0, total, nullptr, rb) != 0) return -1;
+  if (readback(nullptr, 0, total, counter_out ? CX().counters : nullptr, rb) != 0) return -1;
   const int T = rb.one;
+  if (counter_out) *counter_out = rb.u64;
   if (T > capacity) return fail("camera produced %d rays for a list of %d", T, capacity);
   *n_out = T;
   return 0;
@@ -2245,6 +2248,8 @@ struct LoopCtx {
   int capacity = 0;
   int *dcount = nullptr;
   unsigned long long samples_read = 0;   // the sample counter as of the last host wait
+  bool have_base = false;                // the counter was read at the loop's start (not cleared)
+  unsigned long long samples_base = 0;
   bool have_samples = false;
 };
 
@@ -2422,7 +2427,7 @@ static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
 }
 
 static int run_loop(LoopCtx &C, int n, const RayList_ispc &rays, RayListObj *adopt = nullptr) {
-  HIPCHK(hipMemsetAsync(CX().counters, 0, 8 * sizeof(unsigned long long), S()));
+  if (!C.have_base) HIPCHK(hipMemsetAsync(CX().counters, 0, 8 * sizeof(unsigned long long), S()));
   if (adopt && n <= C.maxlist) {
     // the caller's list is processed in place (it is the loop's to consume)
     C.q.push_back({adopt, 3});
@@ -2450,7 +2455,7 @@ static int run_loop(LoopCtx &C, int n, const RayList_ispc &rays, RayListObj *ado
     if (readback(nullptr, 0, nullptr, CX().counters, rb) != 0) return -1;
     C.samples_read = rb.u64;
   }
-  C.st.samples = (long long)C.samples_read;
+  C.st.samples = (long long)(C.samples_read - (C.have_base ? C.samples_base : 0));
   return 0;
 }
 
@@ -2524,11 +2529,16 @@ GXY_EXPORT int gxy_render_frame(void *vis, void *lights, const float eye[3], con
   prim->kinds = 1;    // camera rays: all primary, one per pixel
   prim->unique = 1;
   int np = 0;
-  if (camera_generate(eye, center, 0, up, aov, w, h, &V->local_bb, &V->global_bb, prim->ispc, w * h, 1, &np) != 0) {
+  unsigned long long base = 0;
+  if (camera_generate(eye, center, 0, up, aov, w, h, &V->local_bb, &V->global_bb, prim->ispc, w * h, 1, &np,
+                      &base) != 0) {
     raylist_delete(prim);
     return -1;
   }
   LoopCtx C;
+  // the sample counter is not cleared: the frame's samples are counted from its value here
+  C.have_base = true;
+  C.samples_base = base;
   if (loop_setup(C, vis, lights, nb, eps, maxlist) != 0 || !C.have_lights) {
     raylist_delete(prim);
     return C.have_lights ? -1 : fail("gxy_render_frame needs a Lighting_ispc");

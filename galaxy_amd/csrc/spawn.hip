@@ -77,7 +77,9 @@ __global__ void k_set_int(int *dst, const int *src, int v) {
 static inline size_t scan_pad(size_t m) { return (m + 63) / 64 * 64; }
 
 size_t block_scan_scratch_bytes(int nb) {
-  return 2 * sizeof(int) * scan_pad((size_t)nb + 1) + scan_temp_bytes(nb + 1);
+  // the device-wide scan's temp storage only where it is used (its size
+  // query costs device-property calls on the host)
+  return 2 * sizeof(int) * scan_pad((size_t)nb + 1) + (nb + 1 > GXY_SMALL_SCAN ? scan_temp_bytes(nb + 1) : 0);
 }
 
 int *block_scan_offsets(int *scratch, int nb) { return scratch + scan_pad((size_t)nb + 1); }
