@@ -166,10 +166,22 @@ def cpu_baseline(hip, sc, cam, alpha, target_s=15.0):
             samples += s2.samples
     vol.voxels = None
     return dict(value=round(passes / dt / 1e6, 4), unit="Mrays/s", cores=threads, kind="port",
-                gsamples_per_s=round(samples / dt / 1e9, 5),
+                gsamples_per_s=round(samples / dt / 1e9, 5), host_cores=os.cpu_count(), cpu_model=cpu_model(),
                 sample="%d x primary rays of every %d-th image row (%d primary, %d ray passes in all, %.1f s), "
                        "same scene and seeds, host CPU %s" % (reps, stride, n, passes, dt,
                                                                platform.processor() or "x86_64"))
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or the platform's machine type."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.machine()
 
 
 def release(L, built):
