@@ -2,6 +2,7 @@
 
   python3 profile_summary.py RAW DST                 (profile_bench.sh: volume-render frame)
   python3 profile_summary.py --stats RAW OUT TITLE COMMAND   (profile_workloads.sh)
+  python3 profile_summary.py --counters RAW OUT TITLE COMMAND (profile_workloads.sh: geometry kernels)
 """
 import csv
 import glob
@@ -83,9 +84,48 @@ def stats_table(raw, out, title, cmd, top=24):
     print("\n".join(lines))
 
 
+def counters_table(raw, out, title, cmd):
+    """Per geometry trace kernel (k_trace<0, true, ...>): SQ, FETCH_SIZE and
+    TCC counters of its launches (three runs of the same command), FETCH_SIZE
+    scaled by what a 256 MiB device copy in the same run reads."""
+    def load(d):
+        f = glob.glob(os.path.join(raw, d, "**", "*counter_collection.csv"), recursive=True)
+        return list(csv.DictReader(open(f[0]))) if f else []
+    fetch = load("mesh_fetch")
+    cf = max((float(r["Counter_Value"]) for r in fetch if "copyBuffer" in r["Kernel_Name"]), default=float("nan"))
+    scale = 256 * 1024 / cf
+    per = defaultdict(lambda: defaultdict(list))
+    for d in ("mesh_sq", "mesh_fetch", "mesh_tcc"):
+        for r in load(d):
+            k = r["Kernel_Name"]
+            if "k_trace<0, true" in k:
+                per[k.split("(")[0]][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    avg = lambda v: sum(v) / len(v) if v else float("nan")  # noqa: E731
+    lines = ["# " + title, "", "Made by `bash profile_workloads.sh` (repository root); one counter group per run:"]
+    for grp in ("SQ_WAVES SQ_INSTS_VALU SQ_ACTIVE_INST_VALU SQ_THREAD_CYCLES_VALU SQ_INSTS_VMEM_RD SQ_WAVE_CYCLES "
+                "SQ_BUSY_CYCLES SQ_INSTS_SALU", "FETCH_SIZE", "TCC_HIT_sum TCC_MISS_sum"):
+        lines.append("`rocprofv3 --pmc %s --output-format csv -- %s`" % (grp, cmd))
+    lines += ["", "FETCH_SIZE scaled by %.4f (a 256 MiB device-to-device copy in the same run reads %.0f KiB)."
+              % (scale, cf), "",
+              "| Kernel | Launches | VALU lane utilisation | VALU instr / wave | VMEM reads / wave | Waves / launch "
+              "| L2 hit rate | HBM fetch MB / launch |", "|---|---:|---:|---:|---:|---:|---:|---:|"]
+    for k in sorted(per):
+        c = per[k]
+        waves = avg(c["SQ_WAVES"])
+        util = avg(c["SQ_THREAD_CYCLES_VALU"]) / (64 * avg(c["SQ_ACTIVE_INST_VALU"]))
+        hit = avg(c["TCC_HIT_sum"]) / (avg(c["TCC_HIT_sum"]) + avg(c["TCC_MISS_sum"]))
+        lines.append("| `%s` | %d | %.3f | %.0f | %.0f | %.0f | %.3f | %.1f |" % (
+            k, len(c["SQ_WAVES"]), util, avg(c["SQ_INSTS_VALU"]) / waves, avg(c["SQ_INSTS_VMEM_RD"]) / waves,
+            waves, hit, avg(c["FETCH_SIZE"]) * scale * 1024 / 1e6))
+    open(out, "w").write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 if __name__ == "__main__":
     if sys.argv[1] == "--stats":
         stats_table(*sys.argv[2:6])
+    elif sys.argv[1] == "--counters":
+        counters_table(*sys.argv[2:6])
     else:
         RAW, DST = sys.argv[1], sys.argv[2]
         main()
