@@ -229,8 +229,11 @@ hipError_t launch_lighting_children(const LightParams &L, int mode, const RayLis
 hipError_t launch_lighting_children_dev(const LightParams &L, int mode, const RayList_ispc &R, const int *pidx,
                                         const int *T, int n, const RayList_ispc &O, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  // grids for the upper bound (n parents), at most 4096 workgroups (16 per CU)
-  const long long cap = 4096;
+  // grids for the upper bound (n parents), at most 32768 workgroups (128 per
+  // CU): the child writes wait on gathers of their parents' fields, and more
+  // work-items in flight hide more of that latency (0.727 -> 0.675 ms for the
+  // 29 M children of a 4K nine-icosphere frame against 4096 workgroups)
+  const long long cap = 32768;
   const long long nc = (long long)n * (((mode & LM_AO) ? L.nAO : 0) + ((mode & LM_SHADOW) ? L.nLights : 0));
   if (nc > 0)
     hipLaunchKernelGGL(k_children, dim3((unsigned)std::min(cap, (nc + GXY_BLOCK - 1) / GXY_BLOCK)), dim3(GXY_BLOCK),
