@@ -562,6 +562,7 @@ def main():
                                    trace_ms_per_frame=round(kt2["trace"][0] / steps, 3),
                                    kernel_ms_per_frame=bd2,
                                    geometry_stack=L.gxy_get_geometry_stack(), bvh_width=L.gxy_get_bvh_width(),
+                                   geometry_waves=L.gxy_get_geometry_waves(),
                                    geometry_tune=[[list(c), round(ms, 4)] for c, ms in gtab])
                 b2 = None
             except Exception as e:  # reported, not hidden
@@ -569,6 +570,7 @@ def main():
         out["extra_workloads"] = extra
         # the batched and served forms start from the library's BVH width, not the last frame's tuned one
         glib.check(L.gxy_set_bvh_width(saved_width))
+        glib.check(L.gxy_set_geometry_waves(0))
         glib.check(L.gxy_set_opacity_threshold(alpha))
 
     # ---------------------------------------------------------------- batched and served forms

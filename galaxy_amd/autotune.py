@@ -51,15 +51,24 @@ def tune_trace(L, render_once, candidates=None, reps=2):
     return best, table
 
 
+def geometry_candidates(stacks=None):
+    """(BVH width, traversal stack entries, waves per SIMD) of the geometry
+    tuner: both widths over the stacks at the compiler's occupancy, plus the
+    binary private-stack walk at 8 waves per SIMD."""
+    return [(w, s, 0) for w in (4, 2) for s in (stacks or GEOMETRY_STACKS)] + [(2, 0, 8)]
+
+
 def tune_geometry_stack(L, render_once, stacks=None, reps=2):
-    """The same measurement over the BVH width (binary or 4-wide) and the
-    geometry traversal stacks.  Returns (best (width, entries), [((width,
-    entries), trace ms per frame), ...]) and leaves the best set."""
+    """The same measurement over the BVH width (binary or 4-wide), the
+    geometry traversal stacks and the binary walk's occupancy.  Returns (best
+    (width, entries, waves), [((width, entries, waves), trace ms per frame),
+    ...]) and leaves the best set."""
     from . import lib as glib
     table = []
-    for wst in [(w, s) for w in (4, 2) for s in (stacks or GEOMETRY_STACKS)]:
+    for wst in geometry_candidates(stacks):
         glib.check(L.gxy_set_bvh_width(wst[0]), "set_bvh_width")
         glib.check(L.gxy_set_geometry_stack(wst[1]), "set_geometry_stack")
+        glib.check(L.gxy_set_geometry_waves(wst[2]), "set_geometry_waves")
         st = wst
         render_once()
         glib.check(L.gxy_synchronize())
@@ -73,6 +82,7 @@ def tune_geometry_stack(L, render_once, stacks=None, reps=2):
     best = min(table, key=lambda x: x[1])[0]
     glib.check(L.gxy_set_bvh_width(best[0]), "set_bvh_width")
     glib.check(L.gxy_set_geometry_stack(best[1]), "set_geometry_stack")
+    glib.check(L.gxy_set_geometry_waves(best[2]), "set_geometry_waves")
     L.gxy_timing_reset()
     return best, table
 

@@ -94,6 +94,7 @@ struct Runtime {
   int geom_stack = 0;           // BVH traversal stack of geometry-only traces (0 private, or LDS entries)
   int occlusion = 1;            // rank loops: shadow/AO rays of volume-free scenes stop at their first hit
   int bvh_width = 2;            // traverse the binary trees (2) or the 4-wide ones (4)
+  int geom_waves = 0;           // binary private-stack walks: 0 the compiler's occupancy, 8 waves per SIMD
   float *uv = nullptr;          // AO direction tables U[256], V[256] (device)
 };
 static Runtime G;
@@ -734,6 +735,12 @@ GXY_EXPORT int gxy_set_bvh_width(int width) {
   return 0;
 }
 GXY_EXPORT int gxy_get_bvh_width(void) { return G.bvh_width; }
+GXY_EXPORT int gxy_set_geometry_waves(int waves) {
+  if (waves != 0 && waves != 8) return fail("geometry waves %d: 0 or 8", waves);
+  G.geom_waves = waves;
+  return 0;
+}
+GXY_EXPORT int gxy_get_geometry_waves(void) { return G.geom_waves; }
 GXY_EXPORT int gxy_get_trace_config(int *kind, int *chunk, int *refill) {
   *kind = G.trace_kind == 0 && G.trace_wpe ? 10 + G.trace_wpe : G.trace_kind;
   *chunk = G.trace_chunk;
@@ -1770,7 +1777,7 @@ static int trace_launch(TraceParams P, const RayList_ispc &R, int n, bool sample
   {
     KTimer kt(sampler ? "sampler" : "trace");
     if (pl) HIPCHK(launch_trace_pl(P, R, n, G.trace_chunk, G.trace_refill, S()));
-    else HIPCHK(sampler ? launch_sampler(P, R, n, S()) : launch_trace(P, R, n, G.trace_wpe, G.geom_stack, S()));
+    else HIPCHK(sampler ? launch_sampler(P, R, n, S()) : launch_trace(P, R, n, G.trace_wpe, G.geom_stack, S(), G.geom_waves));
   }
   HIPCHK(launch_reduce_add(part, nb, CX().counters, S()));
   dfree_stream_ordered(part);

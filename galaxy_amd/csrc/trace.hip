@@ -380,7 +380,7 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_sampler(TraceParams P, RayList_is
 // ------------------------------------------------------------- launchers
 template <int NV>
 static hipError_t launch_trace_nv(const TraceParams &P, const RayList_ispc &R, int n, int wpe, int stack,
-                                  hipStream_t s) {
+                                  hipStream_t s, int gwaves) {
   dim3 g((n + GXY_BLOCK - 1) / GXY_BLOCK), b(GXY_BLOCK);
   bool small = true;
   for (int v = 0; v < NV; v++) small = small && P.vv[v].vol.small;
@@ -400,6 +400,7 @@ static hipError_t launch_trace_nv(const TraceParams &P, const RayList_ispc &R, i
     if (stack == 16) hipLaunchKernelGGL((k_trace<0, true, 0, -1, 16, 2>), g, b, 0, s, P, R, n);
     else if (stack == 24) hipLaunchKernelGGL((k_trace<0, true, 0, -1, 24, 2>), g, b, 0, s, P, R, n);
     else if (stack == 32) hipLaunchKernelGGL((k_trace<0, true, 0, -1, 32, 2>), g, b, 0, s, P, R, n);
+    else if (gwaves == 8) hipLaunchKernelGGL((k_trace<0, true, 8, -1, 0, 2>), g, b, 0, s, P, R, n);
     else hipLaunchKernelGGL((k_trace<0, true, 0, -1, 0, 2>), g, b, 0, s, P, R, n);
   } else if (P.has_model) hipLaunchKernelGGL((k_trace<NV, true, 0, -1>), g, b, 0, s, P, R, n);
   else if (NV == 1 && P.vv[0].vol.brick == 8 && wpe == 5) hipLaunchKernelGGL((k_trace<NV, false, 5, 8>), g, b, 0, s, P, R, n);
@@ -417,14 +418,15 @@ static hipError_t launch_trace_nv(const TraceParams &P, const RayList_ispc &R, i
   return hipGetLastError();
 }
 
-hipError_t launch_trace(const TraceParams &P, const RayList_ispc &R, int n, int wpe, int stack, hipStream_t s) {
+hipError_t launch_trace(const TraceParams &P, const RayList_ispc &R, int n, int wpe, int stack, hipStream_t s,
+                        int gwaves) {
   if (n <= 0) return hipSuccess;
   switch (P.nvv) {
-    case 0: return launch_trace_nv<0>(P, R, n, wpe, stack, s);
-    case 1: return launch_trace_nv<1>(P, R, n, wpe, stack, s);
-    case 2: return launch_trace_nv<2>(P, R, n, wpe, stack, s);
-    case 3: return launch_trace_nv<3>(P, R, n, wpe, stack, s);
-    case 4: return launch_trace_nv<4>(P, R, n, wpe, stack, s);
+    case 0: return launch_trace_nv<0>(P, R, n, wpe, stack, s, gwaves);
+    case 1: return launch_trace_nv<1>(P, R, n, wpe, stack, s, gwaves);
+    case 2: return launch_trace_nv<2>(P, R, n, wpe, stack, s, gwaves);
+    case 3: return launch_trace_nv<3>(P, R, n, wpe, stack, s, gwaves);
+    case 4: return launch_trace_nv<4>(P, R, n, wpe, stack, s, gwaves);
   }
   return hipErrorInvalidValue;
 }

@@ -93,6 +93,8 @@ def _declare(L):
         "gxy_set_geometry_stack": (C.c_int, [C.c_int]),
         "gxy_set_bvh_width": (C.c_int, [C.c_int]),
         "gxy_get_bvh_width": (C.c_int, []),
+        "gxy_set_geometry_waves": (C.c_int, [C.c_int]),
+        "gxy_get_geometry_waves": (C.c_int, []),
         "gxy_volume_set_layout": (C.c_int, [vp, C.c_int]),
         "gxy_volume_get_layout": (C.c_int, [vp]),
         "gxy_get_geometry_stack": (C.c_int, []),

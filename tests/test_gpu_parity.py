@@ -365,9 +365,10 @@ def test_every_geometry_stack_gives_identical_hits(hip):
     saved = L_.gxy_get_geometry_stack()
     saved_w = L_.gxy_get_bvh_width()
     try:
-        for width, st in [(w, s) for w in (2, 4) for s in autotune.GEOMETRY_STACKS]:
+        for width, st, waves in autotune.geometry_candidates():
             glib.check(L_.gxy_set_bvh_width(width))
             glib.check(L_.gxy_set_geometry_stack(st))
+            glib.check(L_.gxy_set_geometry_waves(waves))
             d = glib.DeviceRays.from_host(hr)
             hip.trace(Bh.vis, d)
             g = d.download()
@@ -377,11 +378,12 @@ def test_every_geometry_stack_gives_identical_hits(hip):
                 assert (ids[0] >= 0).sum() > 10000
                 continue
             for f in abi.FIELDS:
-                assert np.array_equal(g[f], ref[0][f]), (width, st, f)
-            assert np.array_equal(ids[0], ref[1][0]) and np.array_equal(ids[1], ref[1][1]), (width, st)
+                assert np.array_equal(g[f], ref[0][f]), (width, st, waves, f)
+            assert np.array_equal(ids[0], ref[1][0]) and np.array_equal(ids[1], ref[1][1]), (width, st, waves)
     finally:
         L_.gxy_set_geometry_stack(saved)
         L_.gxy_set_bvh_width(saved_w)
+        L_.gxy_set_geometry_waves(0)
         L_.gxy_model_set_hit_output(Bh.vis.model, None, None)
 
 
