@@ -42,7 +42,8 @@ struct CameraParams {
 // wpe: occupancy variant of the volume-only kernel (0 none, 5 or 6 waves/SIMD);
 // stack: BVH traversal stack of geometry-only launches (0: 64 entries in
 // private memory; 16, 24, 32: entries in LDS, used when the tree fits)
-// gwaves: 8 asks for 8 waves per SIMD (64 VGPRs) in the binary private-stack geometry walk, 0 the compiler's choice
+// gwaves: 8 asks for 8 waves per SIMD (64 VGPRs) in the binary private-stack geometry walks (with or without
+// path lines), 0 the compiler's choice
 hipError_t launch_trace(const TraceParams &P, const RayList_ispc &R, int n, int wpe, int stack, hipStream_t s,
                         int gwaves = 0);
 hipError_t launch_sampler(const TraceParams &P, const RayList_ispc &R, int n, hipStream_t s);

@@ -390,6 +390,7 @@ static hipError_t launch_trace_nv(const TraceParams &P, const RayList_ispc &R, i
   if (NV == 0 && P.has_model && P.model.ncurve > 0) {
     // path lines: the variants that trace curve pieces too (private stack)
     if (wide) hipLaunchKernelGGL((k_trace<0, true, 0, -1, 0, 20>), g, b, 0, s, P, R, n);
+    else if (gwaves == 8) hipLaunchKernelGGL((k_trace<0, true, 8, -1, 0, 18>), g, b, 0, s, P, R, n);
     else hipLaunchKernelGGL((k_trace<0, true, 0, -1, 0, 18>), g, b, 0, s, P, R, n);
   } else if (NV == 0 && P.has_model && wide) {
     if (stack == 16) hipLaunchKernelGGL((k_trace<0, true, 0, -1, 16, 4>), g, b, 0, s, P, R, n);

@@ -216,11 +216,12 @@ int gxy_get_occlusion_rays(void);
  * (1.3-1.8x on MI355X), the 4-wide one by about 9% on one closed mesh. */
 int         gxy_set_bvh_width(int width);
 int         gxy_get_bvh_width(void);
-/* Occupancy of the binary walk with the private stack (no reference
- * counterpart): 0 (default) = the compiler's register allocation (70 VGPRs,
- * 7 waves per SIMD), 8 = 8 waves per SIMD (64 VGPRs, some spills; 4 % faster
- * on the 11.8 M-triangle nine-icosphere frame, 5 % slower on a 20 K-triangle
- * mesh).  Results are identical. */
+/* Occupancy of the binary walks with the private stack, path lines included
+ * (no reference counterpart): 0 (default) = the compiler's register
+ * allocation (70 VGPRs, 7 waves per SIMD; 76 and 6 with path lines), 8 = 8
+ * waves per SIMD (64 VGPRs, some spills; 4 % faster on the 11.8 M-triangle
+ * nine-icosphere frame, 5 % on the path-line frame, 5 % slower on a
+ * 20 K-triangle mesh).  Results are identical. */
 int         gxy_set_geometry_waves(int waves);
 int         gxy_get_geometry_waves(void);
 int         gxy_get_trace_config(int *kind, int *chunk, int *refill);

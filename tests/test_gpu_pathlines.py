@@ -31,15 +31,18 @@ def gpu_hits(hip, built, hr):
         glib.check(L_.gxy_model_set_hit_output(built.vis.model, None, None))
 
 
-@pytest.fixture(params=[2, 4], ids=["binary", "wide"])
+@pytest.fixture(params=[(2, 0), (4, 0), (2, 8)], ids=["binary", "wide", "binary-8-waves"])
 def bvh_width(request):
-    """Runs the test with path lines walked in the binary and the 4-wide trees."""
+    """Runs the test with path lines walked in the binary and the 4-wide trees,
+    and in the binary tree at 8 waves per SIMD."""
     from galaxy_amd import lib as glib
     L_ = glib.lib()
     saved = L_.gxy_get_bvh_width()
-    glib.check(L_.gxy_set_bvh_width(request.param))
+    glib.check(L_.gxy_set_bvh_width(request.param[0]))
+    glib.check(L_.gxy_set_geometry_waves(request.param[1]))
     yield request.param
     L_.gxy_set_bvh_width(saved)
+    L_.gxy_set_geometry_waves(0)
 
 
 def scene_with(pl, meshes=(), box=1.5):
