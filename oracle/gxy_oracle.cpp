@@ -24,6 +24,11 @@
 //                                   against the reference's table values
 //   * triangle / sphere hits      - FP64 brute force on small scenes; BVH vs
 //                                   brute force (Embree's own tie order unpinned)
+//   * triangle / sphere shading   - closed forms: affine vertex data through a
+//                                   linear colormap, radial vertex normals of an
+//                                   inscribed mesh (exact radial Ns), sphere
+//                                   normal and datum colour, face-forwarding;
+//                                   planted mistakes (g_fault) must fail them
 //   * classify / route / counts   - hand-enumerated cases of Renderer.cpp rules
 //   * camera                      - gold images; FUZZ / box rules by cases
 //   * pixel messages              - byte layout against the reference struct
@@ -675,6 +680,13 @@ struct Col4 { Acc x, y, z, w; };
 // opaque test of TraceRays.ispc:466,550,581
 float g_opaque_alpha = 0.999f;  // reference value; the 1920x1080 512^3 volume render uses 0.99
 
+// Planted mistakes in the geometry shading, switched on only by the tests that
+// show the shading pins fail on them (0 = none, the restatement itself):
+//   1 barycentric weights of vertices 1 and 2 swapped   2 geometric normal for Ns
+//   3 vertex data read at the first vertex only         4 Ns not face-forwarded
+//   5 sphere data read at the next sphere's index
+int g_fault = 0;
+
 template <typename Acc>
 inline bool opaque_of(const Col4<Acc> &c) {
   Acc m = c.x < c.y ? c.x : c.y;
@@ -763,14 +775,15 @@ void trace_one(const Visualization_ispc &vis, const std::vector<OVolumeVis> &vv,
             Ng = cross(p0 - p1, p2 - p0);
             Ns = Ng;
             float b0 = 1.0f - gh.u - gh.v, b1 = gh.u, b2 = gh.v;
-            if (!m->N.empty()) {
+            if (g_fault == 1) std::swap(b1, b2);
+            if (!m->N.empty() && g_fault != 2) {
               v3 a = mk(m->N[3 * I[0]], m->N[3 * I[0] + 1], m->N[3 * I[0] + 2]);
               v3 b = mk(m->N[3 * I[1]], m->N[3 * I[1] + 1], m->N[3 * I[1] + 2]);
               v3 c = mk(m->N[3 * I[2]], m->N[3 * I[2] + 1], m->N[3 * I[2] + 2]);
               Ns = b0 * a + b1 * b + b2 * c;
             }
             if (!m->D.empty() && m->tf) {
-              float dv = b0 * m->D[I[0]] + b1 * m->D[I[1]] + b2 * m->D[I[2]];
+              float dv = g_fault == 3 ? m->D[I[0]] : b0 * m->D[I[0]] + b1 * m->D[I[1]] + b2 * m->D[I[2]];
               col = tf_color(*m->tf, dv);
             }
             break;
@@ -781,7 +794,8 @@ void trace_one(const Visualization_ispc &vis, const std::vector<OVolumeVis> &vv,
               v3 c = mk(s->C[3 * gh.prim], s->C[3 * gh.prim + 1], s->C[3 * gh.prim + 2]);
               Ng = P - c;
               Ns = Ng;
-              if (s->tf && !s->D.empty()) col = tf_color(*s->tf, s->D[gh.prim]);
+              if (s->tf && !s->D.empty())
+                col = tf_color(*s->tf, s->D[g_fault == 5 ? (gh.prim + 1) % s->D.size() : (size_t)gh.prim]);
               break;
             }
           for (auto &pl : model->paths)
@@ -809,7 +823,7 @@ void trace_one(const Visualization_ispc &vis, const std::vector<OVolumeVis> &vv,
         Ng = normalize(Ng);
         Ns = normalize(Ns);
         if (dot(dir, Ng) >= 0.f) Ng = neg(Ng);
-        if (dot(dir, Ns) >= 0.f) Ns = neg(Ns);
+        if (dot(dir, Ns) >= 0.f && g_fault != 4) Ns = neg(Ns);
         hit.color = col;
         hit.opacity = alpha;
         hit.point = P;
@@ -1028,6 +1042,7 @@ const float kTwoPi = 6.2831855f;  // 2.f*M_PI in float
 
 OEXPORT void oracle_set_threads(int n) { g_threads = n; }
 OEXPORT void oracle_set_opacity_threshold(float a) { g_opaque_alpha = a; }
+OEXPORT void oracle_plant_fault(int code) { g_fault = code; }
 OEXPORT int oracle_get_threads(void) { return nthreads(); }
 
 OEXPORT void oracle_uv_tables(float *U, float *V) {
