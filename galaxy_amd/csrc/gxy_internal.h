@@ -88,6 +88,15 @@ hipError_t launch_lighting_children_dev(const LightParams &L, int mode, const Ra
 // nAO*total + nLights*rank (or -1).
 hipError_t launch_offsets_from_rank(const RayList_ispc &R, int n, const int *rank, const int *total, int nAO,
                                     int nLights, int doShadows, int *ao_off, int *sh_off, hipStream_t s);
+// The same offsets with every temporary in the static scratch array of
+// spawn.hip (no allocation).  noalloc_scratch_address gives that array's
+// address on the current device (0, or -1 with *ptr null); lists of at most
+// GXY_NOALLOC_MAX_RAYS rays; *total_out receives the device address of the
+// parent total.
+static_assert(GXY_NOALLOC_MAX_RAYS == (GXY_SMALL_SCAN - 1) * GXY_BLOCK, "static scratch limit");
+int noalloc_scratch_address(int **ptr);
+hipError_t launch_spawn_offsets_noalloc(const RayList_ispc &R, int n, int *scr, int nAO, int nL, int doShadows,
+                                        int *ao_off, int *sh_off, const int **total_out, hipStream_t s);
 hipError_t launch_lighting(const LightParams &L, int mode, const RayList_ispc &R, int n, const int *ao_off,
                            const int *sh_off, const RayList_ispc &O, hipStream_t s);
 

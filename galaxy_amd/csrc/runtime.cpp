@@ -96,6 +96,9 @@ struct Runtime {
   int bvh_width = 2;            // traverse the binary trees (2) or the 4-wide ones (4)
   int geom_waves = 0;           // binary private-stack walks: 0 the compiler's occupancy, 8 waves per SIMD
   float *uv = nullptr;          // AO direction tables U[256], V[256] (device)
+  // device address of the static scratch array of the allocation-free spawn
+  // offsets, per device (taken once, with that device current, at set-up)
+  int *noalloc_scratch[64] = {};
 };
 static Runtime G;
 static thread_local Ctx *t_ctx = nullptr;
@@ -329,7 +332,13 @@ static int ensure_init() {
   if (getenv("GXY_DEVICE")) dev = atoi(getenv("GXY_DEVICE"));
   HIPCHK(hipSetDevice(dev));
   HIPCHK(hipStreamCreateWithFlags(&G.def.stream, hipStreamNonBlocking));
+  if (dev >= 0 && dev < 64 && !G.noalloc_scratch[dev] && noalloc_scratch_address(&G.noalloc_scratch[dev]) != 0) {
+    (void)hipGetLastError();   // gxy_spawn_offsets_noalloc reports it; nothing else needs the array
+    G.noalloc_scratch[dev] = nullptr;
+  }
   G.device = dev;
+  // the default context's readback area now, so that no later call allocates it
+  if (!readback_area(G.def)) return -1;
   float UV[512];
   make_uv(UV, UV + 256);
   G.uv = (float *)dalloc(sizeof UV);
@@ -609,8 +618,20 @@ GXY_EXPORT void *gxy_context_create(void) {
   Ctx *prev = t_ctx;
   t_ctx = c;
   c->counters = (unsigned long long *)dalloc(8 * sizeof(unsigned long long));
+  // the readback area now, so that no later call on this context allocates it
+  if (!c->counters || !readback_area(*c)) {
+    dfree_stream_ordered(c->counters);
+    {
+      std::lock_guard<std::recursive_mutex> lk(G.mu);
+      release_cache(*c);
+      if (c->hread) G.free_reads.push_back({c->hread, c->dread});
+    }
+    t_ctx = prev;
+    hipStreamDestroy(c->stream);
+    delete c;
+    return nullptr;
+  }
   t_ctx = prev;
-  if (!c->counters) return nullptr;
   hipMemsetAsync(c->counters, 0, 8 * sizeof(unsigned long long), c->stream);
   hipStreamSynchronize(c->stream);
   std::lock_guard<std::recursive_mutex> lk(G.mu);
@@ -1999,6 +2020,33 @@ GXY_EXPORT int gxy_spawn_offsets(int n, RayList_ispc *R, int nAO, int nL, int do
   if (spawn_offsets(n, *R, nAO, nL, doSh, ao_off, sh_off, &T) != 0) return -1;
   *n_ao = nAO * T;
   *n_sh = doSh ? nL * T : 0;
+  return 0;
+}
+
+// gxy_spawn_offsets without any allocation: the scan's temporaries are the
+// static scratch array of spawn.hip, by the device address taken at set-up;
+// the parent total comes back through the context's readback area (allocated
+// when the library or the context was set up).  Calls that overlap on one GPU
+// share that array and race.
+GXY_EXPORT int gxy_spawn_offsets_noalloc(int n, RayList_ispc *R, int nAO, int nL, int doSh, int *ao_off,
+                                         int *sh_off, int *n_ao, int *n_sh) {
+  INIT_OR(-1);
+  if (n < 0) return fail("negative ray count");
+  if (n > GXY_NOALLOC_MAX_RAYS)
+    return fail("gxy_spawn_offsets_noalloc: %d rays exceed the static scratch limit of %d rays", n,
+                (int)GXY_NOALLOC_MAX_RAYS);
+  int *scr = (G.device >= 0 && G.device < 64) ? G.noalloc_scratch[G.device] : nullptr;
+  if (!scr) return fail("gxy_spawn_offsets_noalloc: no device address for the static scratch array");
+  if (n == 0) {
+    *n_ao = *n_sh = 0;
+    return 0;
+  }
+  const int *total = nullptr;
+  HIPCHK(launch_spawn_offsets_noalloc(*R, n, scr, nAO, nL, doSh, ao_off, sh_off, &total, S()));
+  Readback rb;
+  if (readback(nullptr, 0, total, nullptr, rb) != 0) return -1;
+  *n_ao = nAO * rb.one;
+  *n_sh = doSh ? nL * rb.one : 0;
   return 0;
 }
 

@@ -163,6 +163,67 @@ hipError_t launch_offsets_from_rank(const RayList_ispc &R, int n, const int *ran
   return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ allocation-free spawn offsets
+// The per-workgroup parent counts and their exclusive sums of the
+// allocation-free offsets live in this file-scope array: one copy per GPU,
+// held until the process ends (2 * GXY_SMALL_SCAN ints = 128 KiB).  Lists of
+// up to GXY_NOALLOC_MAX_RAYS rays fit, so their count array (at most
+// GXY_SMALL_SCAN entries with the trailing zero) is scanned by the
+// one-workgroup scan, which needs no temporary storage.  Host code never
+// passes the array's name to a kernel: it uses the address
+// hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) int scratch[2 * GXY_SMALL_SCAN];
+
+int noalloc_scratch_address(int **ptr) {
+  *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress((void **)ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) {
+    *ptr = nullptr;
+    return -1;
+  }
+  return 0;
+}
+
+// Offsets straight from the scanned workgroup counts: each work-item
+// recomputes its rank among the parents (ballot + the waves before it), so no
+// per-ray rank array is needed.  The parent total is block_base[nb].
+__global__ __launch_bounds__(GXY_BLOCK) void k_offsets_direct(RayList_ispc R, int n, const int *block_base, int nb,
+                                                              int nAO, int nL, int doShadows, int *ao_off,
+                                                              int *sh_off) {
+  __shared__ int wsum[GXY_BLOCK / 64];
+  const int i = blockIdx.x * GXY_BLOCK + threadIdx.x;
+  const bool f = i < n && is_parent(R, i);
+  const unsigned long long m = __ballot(f);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) wsum[w] = __popcll(m);
+  __syncthreads();
+  if (i >= n) return;
+  int r = block_base[blockIdx.x];
+  for (int k = 0; k < w; k++) r += wsum[k];
+  r += __popcll(m & lanemask_lt());
+  const int T = block_base[nb];
+  if (ao_off) ao_off[i] = nAO * r;
+  if (sh_off) sh_off[i] = (doShadows && f) ? nAO * T + nL * r : -1;
+}
+
+// Three launches on s over scr (the array above, by its device address); the
+// parent total is left at *total_out (a device address inside scr).  Every
+// scratch entry read is written first in the same call (the count kernel
+// writes nb counts and the trailing zero; the scan writes nb + 1 sums), so
+// nothing needs clearing between calls.
+hipError_t launch_spawn_offsets_noalloc(const RayList_ispc &R, int n, int *scr, int nAO, int nL, int doShadows,
+                                        int *ao_off, int *sh_off, const int **total_out, hipStream_t s) {
+  const int nb = (n + GXY_BLOCK - 1) / GXY_BLOCK;
+  if (n <= 0 || nb + 1 > GXY_SMALL_SCAN || !scr) return hipErrorInvalidValue;
+  int *counts = scr, *base = scr + GXY_SMALL_SCAN;
+  hipLaunchKernelGGL(k_flag_count, dim3(nb), dim3(GXY_BLOCK), 0, s, R, n, counts);
+  hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, s, (const int *)counts, base, nb + 1);
+  hipLaunchKernelGGL(k_offsets_direct, dim3(nb), dim3(GXY_BLOCK), 0, s, R, n, (const int *)base, nb, nAO, nL,
+                     doShadows, ao_off, sh_off);
+  *total_out = base + nb;
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ lighting
 __global__ __launch_bounds__(GXY_BLOCK) void k_lighting(LightParams L, int mode, RayList_ispc R, int n,
                                                         const int *ao_off, const int *sh_off, RayList_ispc O) {

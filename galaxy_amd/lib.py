@@ -76,6 +76,7 @@ def _declare(L):
         "gxy_model_bvh_stats": (C.c_int, [vp, vp, vp, vp, vp]),
         "gxy_model_set_hit_output": (C.c_int, [vp, vp, vp]),
         "gxy_spawn_offsets": (C.c_int, [C.c_int, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp]),
+        "gxy_spawn_offsets_noalloc": (C.c_int, [C.c_int, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp]),
         "gxy_classify": (C.c_int, [C.c_int, vp, C.c_int]),
         "gxy_assign_destinations": (C.c_int, [C.c_int, vp, vp, vp, C.c_int]),
         "gxy_accumulate_terminated": (C.c_int, [C.c_int, vp, vp, C.c_int, C.c_int]),

@@ -323,6 +323,17 @@ int   gxy_model_set_hit_output(void *model, int *geom_ids, int *prim_ids);
  * nRays; n_ao / n_shadow receive the counts (this call synchronises). */
 int gxy_spawn_offsets(int nRays, RayList_ispc *raysIn, int nAO, int nLights, int do_shadows,
                       int *ao_offsets, int *shadow_offsets, int *n_ao, int *n_shadow);
+/* No counterpart in the reference.  gxy_spawn_offsets for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): same arguments and results, but every temporary lives in
+ * one static device array per GPU (128 KiB, held until the process ends) and
+ * the call allocates nothing once gxy_init (or the first call) has set the
+ * library up.  At most GXY_NOALLOC_MAX_RAYS rays; larger lists fail with an
+ * error.  Calls that overlap on one GPU share the array and race.
+ * Synchronises, like gxy_spawn_offsets. */
+#define GXY_NOALLOC_MAX_RAYS 4194048   /* 16383 workgroups of 256 rays */
+int gxy_spawn_offsets_noalloc(int nRays, RayList_ispc *raysIn, int nAO, int nLights, int do_shadows,
+                              int *ao_offsets, int *shadow_offsets, int *n_ao, int *n_shadow);
 
 /* Renderer::Classify (Renderer.cpp:304-421), reverse lighting on when
  * reverse_lighting != 0 (the reference default, CMakeLists.txt:66). */
