@@ -335,9 +335,10 @@ __device__ __forceinline__ ChunkFlags chunk_flags(const BatchArgs &A, const RayL
   f.cls = 0;
   if (!valid) return f;
   const int typ = R.type[i], term = R.term[i];
-  int c = classify_ray(typ, term, R.classification[i], 1);
+  int c = classify_ray(typ, term, R.classification[i], A.L.reverse);
   // AssignDestinations with no neighbouring partition (Renderer.cpp:423-454)
-  if (c == RAY_BOUNDARY) c = (typ == RAY_SHADOW || typ == RAY_AO) ? GXY_DROP_ON_FLOOR : GXY_TERMINATED;
+  if (c == RAY_BOUNDARY)
+    c = ((typ == RAY_SHADOW || typ == RAY_AO) && A.L.reverse) ? GXY_DROP_ON_FLOOR : GXY_TERMINATED;
   f.cls = c;
   f.par = mode == BM_RENDER && typ == RAY_PRIMARY && (term & RAY_SURFACE);
   f.keep = c == GXY_KEEP_HERE;
@@ -533,7 +534,7 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_bwrite(BatchArgs A) {
           const int p = q / nsp, j = q - p * nsp;
           const ParentRay P = parent_load(R, ch.start + s_pl[w][p]);
           if (j < nAO) light_ao_child(A.L, P, j, O, base + q);
-          else light_shadow_child(A.L, P, light_diffused_alpha(A.L, P.o), j - nAO, O, base + q);
+          else light_shadow_child(A.L, P, A.L.reverse ? light_diffused_alpha(A.L, P.o) : P.o, j - nAO, O, base + q);
         }
       }
       wave_lds_sync();   // the slice is rewritten by the next chunk
@@ -551,10 +552,11 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_bwrite(BatchArgs A) {
           int x = 0, y = 0;
           float cr = 0.f, cg = 0.f, cb = 0.f, co = 0.f;
           if (f.par) {
-            // ambientLighting + diffuseLighting (TraceRays.cpp:124-131)
+            // ambientLighting + diffuseLighting (TraceRays.cpp:124-131); without
+            // reverse lighting only those whose rays are not cast (:133-141)
             ParentRay P = parent_load(R, i);
-            light_ambient(A.L, P);
-            light_diffuse(A.L, P);
+            if (A.L.reverse || A.L.nAO <= 0) light_ambient(A.L, P);
+            if (A.L.reverse || !(A.L.doShadows && A.L.nLights > 0)) light_diffuse(A.L, P);
             R.r[i] = P.r; R.g[i] = P.g; R.b[i] = P.b; R.o[i] = P.o;
             if (f.term) {
               x = P.px; y = P.py;

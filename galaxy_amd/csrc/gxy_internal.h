@@ -11,7 +11,8 @@ namespace gxy {
 enum { LM_AMBIENT = 1, LM_AO = 2, LM_DIFFUSE = 4, LM_SHADOW = 8 };
 
 struct LightParams {
-  int nAO, nLights, doShadows, pad_;
+  int nAO, nLights, doShadows;
+  int reverse;         // Galaxy's reverse lighting model (1, its default) or the direct one (0)
   float aoRadius, Ka, Kd, eps;
   float L[3 * 16];
   int types[16];

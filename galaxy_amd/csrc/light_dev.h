@@ -40,11 +40,12 @@ __device__ __forceinline__ void light_ambient(const LightParams &L, ParentRay &P
   P.b += s * P.sb;
 }
 
-// TraceRays_generateAORays (TraceRays.ispc:640-730), reverse lighting: AO
-// child j of P (P.o before diffuseLighting), written to O[off].
+// TraceRays_generateAORays (TraceRays.ispc:640-730): AO child j of P (P.o
+// before diffuseLighting), written to O[off]; its colour is negative under
+// reverse lighting (:639-643).
 __device__ __forceinline__ void light_ao_child(const LightParams &L, const ParentRay &P, int j, const RayList_ispc &O,
                                                int off) {
-  const float Ka = -L.Ka / L.nAO;
+  const float Ka = (L.reverse ? -L.Ka : L.Ka) / L.nAO;
   float as = Ka * (1.0f - P.o);
   float ar = as * P.sr, ag = as * P.sg, ab = as * P.sb;
   const f3 nrm = P.nrm;
@@ -106,11 +107,12 @@ __device__ __forceinline__ float light_diffused_alpha(const LightParams &L, floa
   return o + Kd * (1 - o) * o;
 }
 
-// TraceRays_generateShadowRays (TraceRays.ispc:778-855), reverse lighting:
-// shadow child k of P for parent alpha o (after diffuseLighting), to O[off].
+// TraceRays_generateShadowRays (TraceRays.ispc:778-855): shadow child k of P
+// for parent alpha o (after diffuseLighting under reverse lighting, which also
+// makes its colour negative, :778-782), to O[off].
 __device__ __forceinline__ void light_shadow_child(const LightParams &L, const ParentRay &P, float o, int k,
                                                    const RayList_ispc &O, int off) {
-  const float Kd = -L.Kd / L.nLights;
+  const float Kd = (L.reverse ? -L.Kd : L.Kd) / L.nLights;
   const f3 nrm = P.nrm;
   f3 sp = mk3(P.ox0 + P.t * P.dx0 + L.eps * nrm.x, P.oy0 + P.t * P.dy0 + L.eps * nrm.y,
               P.oz0 + P.t * P.dz0 + L.eps * nrm.z);

@@ -87,6 +87,7 @@ struct Runtime {
   std::set<void *> volumes, tfs, models, raylists, tracers, lightings, contexts;
   std::atomic<int> raylist_id{0};
   float opaque_alpha = 0.999f;
+  int reverse_lighting = 1;       // CMakeLists.txt:66 GXY_REVERSE_LIGHTING (default ON)
   std::vector<void *> outbound[6];   // RayListObj* per exit face, last frames
   std::vector<std::pair<int *, int *>> free_reads;   // readback areas of destroyed contexts (kept mapped)
   // volume-only trace kernel: 0 one ray per work-item, 1 persistent lanes
@@ -572,6 +573,7 @@ static int make_light_params(const Lighting_ispc *l, float eps, LightParams &L) 
   L.Ka = l->Ka;
   L.Kd = l->Kd;
   L.eps = eps;
+  L.reverse = G.reverse_lighting;
   L.U = G.uv;
   L.V = G.uv + 256;
   for (int k = 0; k < L.nLights; k++) {
@@ -2068,12 +2070,31 @@ static int route(int n, const RayList_ispc &R, int mode, int reverse, const gxy_
   return 0;
 }
 
+// The lighting model is one library setting (gxy_set_reverse_lighting), as
+// Galaxy's is one build option: a flag that disagrees with it is an error,
+// since reverse-signed children classified by the direct rules (or the
+// converse) give results no Galaxy build produces.
+static int check_reverse(int reverse, const char *what) {
+  if ((reverse != 0) != (G.reverse_lighting != 0))
+    return fail("%s: reverse_lighting=%d but the library's lighting model is %s (gxy_set_reverse_lighting)", what,
+                reverse, G.reverse_lighting ? "reverse" : "direct");
+  return 0;
+}
+GXY_EXPORT int gxy_set_reverse_lighting(int on) {
+  INIT_OR(-1);
+  G.reverse_lighting = on ? 1 : 0;
+  return 0;
+}
+GXY_EXPORT int gxy_get_reverse_lighting(void) { return G.reverse_lighting; }
+
 GXY_EXPORT int gxy_classify(int n, RayList_ispc *R, int reverse) {
   INIT_OR(-1);
+  if (check_reverse(reverse, "gxy_classify") != 0) return -1;
   return route(n, *R, RM_CLASSIFY, reverse, nullptr, nullptr, nullptr, 0, 0);
 }
 GXY_EXPORT int gxy_assign_destinations(int n, RayList_ispc *R, const gxy_box3f *box, const int nb[6], int reverse) {
   INIT_OR(-1);
+  if (check_reverse(reverse, "gxy_assign_destinations") != 0) return -1;
   return route(n, *R, RM_ASSIGN, reverse, box, nb, nullptr, 0, 0);
 }
 GXY_EXPORT int gxy_accumulate_terminated(int n, RayList_ispc *R, float *fb, int w, int h) {
@@ -2376,8 +2397,11 @@ static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
         out.p->kinds = 2;
         out.p->unique = (L->unique && nsp == 1) ? 1 : 0;
       }
-      const int mode = LM_AMBIENT | LM_DIFFUSE | (C.L.nAO > 0 ? LM_AO : 0) |
-                       ((C.L.doShadows && C.L.nLights > 0) ? LM_SHADOW : 0);
+      // TraceRays.cpp:124-131 (reverse lighting: every step) or :133-141 (the
+      // cast rays carry ambient / diffuse instead of the parent)
+      const bool ao = C.L.nAO > 0, sh = C.L.doShadows && C.L.nLights > 0;
+      const int mode = C.L.reverse ? (LM_AMBIENT | LM_DIFFUSE | (ao ? LM_AO : 0) | (sh ? LM_SHADOW : 0))
+                                   : ((ao ? LM_AO : LM_AMBIENT) | (sh ? LM_SHADOW : LM_DIFFUSE));
       RayList_ispc O;
       memset(&O, 0, sizeof O);
       if (out.p) O = out.p->ispc;
@@ -2388,7 +2412,8 @@ static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
       dT = total;
     }
     // Classify, AssignDestinations, HandleTerminatedRays (Renderer.cpp:550-556)
-    if (route(n, R, C.sampler ? (RM_CLASSIFY | RM_ASSIGN) : (RM_CLASSIFY | RM_ASSIGN | RM_ACCUMULATE), 1,
+    if (route(n, R, C.sampler ? (RM_CLASSIFY | RM_ASSIGN) : (RM_CLASSIFY | RM_ASSIGN | RM_ACCUMULATE),
+              G.reverse_lighting,
               &C.box, C.nb, C.fb, C.w, C.h, L->unique) != 0)
       return -1;
     if (C.sampler) {
@@ -2673,6 +2698,7 @@ struct BatchObj {
   long long *seg = nullptr;          // 2 * 256
   int *lpend = nullptr;
   int lpend_cap = 0;
+  int lpend_n = 0;    // lists whose lpend entries the passes reset (the last pass arguments built)
   unsigned long long *part = nullptr;
   int part_blocks = 0;
   unsigned long long *stats = nullptr;  // BS_COUNT
@@ -2782,6 +2808,7 @@ static BatchArgs batch_args(BatchObj *B, int pass) {
   const int in = (pass + 1) & 1, out = pass & 1;
   A.lists = B->d_lists;
   A.nlists = (int)B->lists.size();
+  B->lpend_n = A.nlists;
   A.chunk = B->chunk;
   A.ds = B->d_ds;
   A.rls = B->d_rls;
@@ -3394,10 +3421,12 @@ GXY_EXPORT int gxy_batch_pending(void *b, int *per_list, int n) {
   if (!B) return fail("not a batch handle");
   const int nl = std::min(n, (int)B->lists.size());
   if (nl <= 0) return 0;
-  if (B->passes_run == 0) {
-    for (int l = 0; l < nl; l++) per_list[l] = 0;
-  } else {
-    HIPCHK(hipMemcpyAsync(per_list, B->lpend, nl * sizeof(int), hipMemcpyDeviceToHost, S()));
+  // lpend holds the counts of the lists the last passes covered; lists added
+  // since then have only waiting rays
+  const int nc = B->passes_run == 0 ? 0 : std::min(nl, std::min(B->lpend_n, B->lpend_cap));
+  for (int l = nc; l < nl; l++) per_list[l] = 0;
+  if (nc > 0) {
+    HIPCHK(hipMemcpyAsync(per_list, B->lpend, nc * sizeof(int), hipMemcpyDeviceToHost, S()));
     HIPCHK(hipStreamSynchronize(S()));
   }
   for (const BatchObj::Waiting &w : B->fresh)

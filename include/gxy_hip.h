@@ -335,8 +335,20 @@ int gxy_spawn_offsets(int nRays, RayList_ispc *raysIn, int nAO, int nLights, int
 int gxy_spawn_offsets_noalloc(int nRays, RayList_ispc *raysIn, int nAO, int nLights, int do_shadows,
                               int *ao_offsets, int *shadow_offsets, int *n_ao, int *n_shadow);
 
+/* Galaxy's lighting model, a build option there (GXY_REVERSE_LIGHTING,
+ * CMakeLists.txt:66, default ON) and one library setting here: 1 = reverse
+ * lighting (the default), 0 = the direct model of TraceRays.cpp:133-141,
+ * TraceRays.ispc:639-643,778-782, Renderer.cpp:365-447.  The TraceRays_*
+ * lighting entry points, the frame loops and the batched form follow it (a
+ * batch takes it when its lighting is set); gxy_classify and
+ * gxy_assign_destinations fail when their reverse_lighting argument disagrees
+ * with it. */
+int gxy_set_reverse_lighting(int on);
+int gxy_get_reverse_lighting(void);
+
 /* Renderer::Classify (Renderer.cpp:304-421), reverse lighting on when
- * reverse_lighting != 0 (the reference default, CMakeLists.txt:66). */
+ * reverse_lighting != 0 (the reference default, CMakeLists.txt:66); it must
+ * equal the library setting above. */
 int gxy_classify(int nRays, RayList_ispc *rays, int reverse_lighting);
 /* Renderer::AssignDestinations (Renderer.cpp:423-454) + Box::exit_face
  * (Box.cpp:84-97); neighbors: 6 ranks or -1. */

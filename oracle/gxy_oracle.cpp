@@ -687,6 +687,11 @@ float g_opaque_alpha = 0.999f;  // reference value; the 1920x1080 512^3 volume r
 //   5 sphere data read at the next sphere's index
 int g_fault = 0;
 
+// Galaxy's lighting model (CMakeLists.txt:66 GXY_REVERSE_LIGHTING, default ON):
+// 1 = reverse lighting, 0 = the #else branches of TraceRays.cpp:133-141,
+// TraceRays.ispc:639-643,778-782 and Renderer.cpp:365-447.
+int g_reverse = 1;
+
 template <typename Acc>
 inline bool opaque_of(const Col4<Acc> &c) {
   Acc m = c.x < c.y ? c.x : c.y;
@@ -1043,6 +1048,7 @@ const float kTwoPi = 6.2831855f;  // 2.f*M_PI in float
 OEXPORT void oracle_set_threads(int n) { g_threads = n; }
 OEXPORT void oracle_set_opacity_threshold(float a) { g_opaque_alpha = a; }
 OEXPORT void oracle_plant_fault(int code) { g_fault = code; }
+OEXPORT void oracle_set_reverse_lighting(int on) { g_reverse = on ? 1 : 0; }
 OEXPORT int oracle_get_threads(void) { return nthreads(); }
 
 OEXPORT void oracle_uv_tables(float *U, float *V) {
@@ -1375,11 +1381,12 @@ OEXPORT void oracle_ambientLighting(void *lp, int n, RayList_ispc *R) {
     }
 }
 
-// TraceRays_generateAORays (TraceRays.ispc:625-733), reverse lighting.
+// TraceRays_generateAORays (TraceRays.ispc:625-733); the colour is negative
+// under reverse lighting (:639-643).
 OEXPORT void oracle_generateAORays(void *lp, int n, RayList_ispc *R, const int *offsets, RayList_ispc *O, float eps) {
   make_uv();
   OLights L = lights_of(lp);
-  const float Ka = -L.Ka / L.nAO;
+  const float Ka = (g_reverse ? -L.Ka : L.Ka) / L.nAO;
   for (int i = 0; i < n; i++) {
     if (!(R->type[i] == RAY_PRIMARY && (R->term[i] & RAY_SURFACE))) continue;
     v3 nrm = mk(R->nx[i], R->ny[i], R->nz[i]);
@@ -1446,11 +1453,12 @@ OEXPORT void oracle_diffuseLighting(void *lp, int n, RayList_ispc *R) {
   }
 }
 
-// TraceRays_generateShadowRays (TraceRays.ispc:763-857), reverse lighting.
+// TraceRays_generateShadowRays (TraceRays.ispc:763-857); the colour is
+// negative under reverse lighting (:778-782).
 OEXPORT void oracle_generateShadowRays(void *lp, int n, RayList_ispc *R, const int *offsets, RayList_ispc *O,
                                        float eps) {
   OLights L = lights_of(lp);
-  const float Kd = -L.Kd / L.n;
+  const float Kd = (g_reverse ? -L.Kd : L.Kd) / L.n;
   for (int i = 0; i < n; i++) {
     int off = offsets[i];
     if (off == -1) continue;
@@ -1837,10 +1845,17 @@ void process_list(FrameCtx &C, std::unique_ptr<HRayList> rl, std::deque<std::uni
     int nout = oracle_spawn_offsets(n, &R, L.nAO, L.n, L.doShadows, aoo.data(), sho.data(), &nao, &nsh);
     std::unique_ptr<HRayList> out;
     if (nout) out.reset(new HRayList(nout, 1));
-    oracle_ambientLighting(C.lights, n, &R);
-    if (nao) oracle_generateAORays(C.lights, n, &R, aoo.data(), &out->r, C.eps);
-    oracle_diffuseLighting(C.lights, n, &R);
-    if (nsh) oracle_generateShadowRays(C.lights, n, &R, sho.data(), &out->r, C.eps);
+    if (g_reverse) {   // TraceRays.cpp:124-131
+      oracle_ambientLighting(C.lights, n, &R);
+      if (nao) oracle_generateAORays(C.lights, n, &R, aoo.data(), &out->r, C.eps);
+      oracle_diffuseLighting(C.lights, n, &R);
+      if (nsh) oracle_generateShadowRays(C.lights, n, &R, sho.data(), &out->r, C.eps);
+    } else {           // TraceRays.cpp:133-141
+      if (nao) oracle_generateAORays(C.lights, n, &R, aoo.data(), &out->r, C.eps);
+      else oracle_ambientLighting(C.lights, n, &R);
+      if (nsh) oracle_generateShadowRays(C.lights, n, &R, sho.data(), &out->r, C.eps);
+      else oracle_diffuseLighting(C.lights, n, &R);
+    }
     if (out) {
       C.st.secondary_rays += nout;
       if (out->n > C.maxlist) {
@@ -1852,8 +1867,8 @@ void process_list(FrameCtx &C, std::unique_ptr<HRayList> rl, std::deque<std::uni
         }
       } else queue.push_back(std::move(out));
     }
-    oracle_classify(n, &R, 1);
-    oracle_assign_destinations(n, &R, &C.V.ispc->local_bb, C.nb, 1);
+    oracle_classify(n, &R, g_reverse);
+    oracle_assign_destinations(n, &R, &C.V.ispc->local_bb, C.nb, g_reverse);
     oracle_accumulate(n, &R, C.fb, C.w, C.h);
     int nk = 0;
     for (int i = 0; i < n; i++) {

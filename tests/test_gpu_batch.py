@@ -300,3 +300,43 @@ def test_sampler_segment_cap_gives_the_same_particles(datasets, hip):
             for a, b in zip(got, ref[0]):
                 assert np.array_equal(a, b), cap_seg
         B.destroy()
+
+
+def test_batch_direct_lighting_equals_per_list_loop(datasets, hip):
+    """The direct lighting model (gxy_set_reverse_lighting(0), taken by the
+    batch when its lighting is set) gives the per-list loop's frames."""
+    from galaxy_amd import lib as glib
+    L = glib.lib()
+    descs, built = datasets
+    keep = []
+    glib.check(L.gxy_set_reverse_lighting(0))
+    try:
+        lights = lights_struct(keep)
+        plan = plan_of([64, 777, 3001, 129, 2048, 5], len(built), 500)
+        B, lists = make_batch(built, lights, plan)
+        st = B.run(0)
+        assert st.overflow == 0
+        for (n, ds, seed), (l, fb) in zip(plan, lists):
+            ref, s = plain_list(L, built[ds], lights, n, seed, ds)
+            assert np.abs(fb.to_host(f32).reshape(-1, 4) - ref).max() <= 1e-5, (n, ds)
+    finally:
+        glib.check(L.gxy_set_reverse_lighting(1))
+
+
+def test_pending_counts_lists_added_after_a_step(datasets, hip):
+    """Lists added after a step report their waiting rays only: the pass
+    counters cover the lists of the last passes and no entry beyond them."""
+    from galaxy_amd import batch as gb
+    descs, built = datasets
+    keep = []
+    lights = lights_struct(keep)
+    B, lists = make_batch(built, lights, plan_of([500, 700, 900], len(built), 700))
+    B.step()
+    extra = []
+    for k, n in enumerate((1000, 3000, 50, 4096, 7)):
+        d = gb.random_rays(n, 800 + k, width=n)
+        l, fb = B.add_render_list(k % len(built), d, n, 1)
+        extra.append((l, n))
+    p = B.pending(3 + len(extra))
+    assert all(0 <= v <= 50 * 900 for v in p[:3])
+    assert [int(p[l]) for l, _ in extra] == [n for _, n in extra]
