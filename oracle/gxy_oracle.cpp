@@ -277,6 +277,129 @@ struct Box3 {
   }
 };
 
+// Threads over [0, n) in nt fixed contiguous chunks (chunk k to thread k), so
+// that per-chunk partial results merged in chunk order are deterministic.
+void par_chunks(int n, int nt, const std::function<void(int, int, int)> &body) {
+  if (nt <= 1) { body(0, 0, n); return; }
+  std::vector<std::thread> th;
+  for (int k = 0; k < nt; k++) {
+    const int s = (int)((long long)n * k / nt), e = (int)((long long)n * (k + 1) / nt);
+    th.emplace_back([&body, k, s, e]() { body(k, s, e); });
+  }
+  for (auto &t : th) t.join();
+}
+
+// One node of the binned-SAH build over order[start, start + count): sets its
+// bounds, and returns the split point (or -1 for a leaf of <= 4 primitives).
+// With nt > 1 the bounds and bins are gathered by nt threads and merged in
+// chunk order, which gives exactly the serial result (min, max and counts).
+int bvh_split(const std::vector<Box3> &boxes, const std::vector<float> &cen, std::vector<int> &order, int start,
+              int count, BNode &nd, int nt) {
+  const int NB = 16;
+  Box3 bb, cb;
+  bb.empty();
+  cb.empty();
+  {
+    std::vector<Box3> pb(nt), pc(nt);
+    par_chunks(count, nt, [&](int k, int s, int e) {
+      Box3 a, c;
+      a.empty();
+      c.empty();
+      for (int i = start + s; i < start + e; i++) {
+        a.grow(boxes[order[i]]);
+        c.growp(&cen[3 * (size_t)order[i]]);
+      }
+      pb[k] = a;
+      pc[k] = c;
+    });
+    for (int k = 0; k < nt; k++) { bb.grow(pb[k]); cb.grow(pc[k]); }
+  }
+  for (int k = 0; k < 3; k++) { nd.lo[k] = bb.lo[k]; nd.hi[k] = bb.hi[k]; }
+  if (count <= 4) { nd.left = -1; nd.start = start; nd.count = count; return -1; }
+  float best = 1e38f;
+  int bestAxis = -1, bestSplit = -1;
+  for (int ax = 0; ax < 3; ax++) {
+    float ext = cb.hi[ax] - cb.lo[ax];
+    if (!(ext > 0.f)) continue;
+    std::vector<Box3> pbin((size_t)nt * NB);
+    std::vector<int> pcnt((size_t)nt * NB, 0);
+    par_chunks(count, nt, [&](int k, int s, int e) {
+      Box3 *bin = &pbin[(size_t)k * NB];
+      int *cnt = &pcnt[(size_t)k * NB];
+      for (int b = 0; b < NB; b++) bin[b].empty();
+      for (int i = start + s; i < start + e; i++) {
+        int p = order[i];
+        int b = (int)((cen[3 * (size_t)p + ax] - cb.lo[ax]) / ext * NB);
+        b = b < 0 ? 0 : (b >= NB ? NB - 1 : b);
+        cnt[b]++;
+        bin[b].grow(boxes[p]);
+      }
+    });
+    Box3 bin[NB];
+    int cnt[NB] = {0};
+    for (int b = 0; b < NB; b++) bin[b].empty();
+    for (int k = 0; k < nt; k++)
+      for (int b = 0; b < NB; b++) { bin[b].grow(pbin[(size_t)k * NB + b]); cnt[b] += pcnt[(size_t)k * NB + b]; }
+    float ra[NB];
+    int rc[NB];
+    Box3 acc;
+    acc.empty();
+    int c = 0;
+    for (int b = NB - 1; b > 0; b--) { acc.grow(bin[b]); c += cnt[b]; ra[b] = acc.area(); rc[b] = c; }
+    acc.empty();
+    c = 0;
+    for (int b = 0; b < NB - 1; b++) {
+      acc.grow(bin[b]);
+      c += cnt[b];
+      if (c == 0 || rc[b + 1] == 0) continue;
+      float cost = acc.area() * c + ra[b + 1] * rc[b + 1];
+      if (cost < best) { best = cost; bestAxis = ax; bestSplit = b; }
+    }
+  }
+  int mid;
+  if (bestAxis < 0) {
+    mid = start + count / 2;
+  } else {
+    float ext = cb.hi[bestAxis] - cb.lo[bestAxis];
+    auto it = std::partition(order.begin() + start, order.begin() + start + count, [&](int p) {
+      int b = (int)((cen[3 * (size_t)p + bestAxis] - cb.lo[bestAxis]) / ext * NB);
+      b = b < 0 ? 0 : (b >= NB ? NB - 1 : b);
+      return b <= bestSplit;
+    });
+    mid = (int)(it - order.begin());
+    if (mid == start || mid == start + count) mid = start + count / 2;
+  }
+  nd.count = 0;
+  return mid;
+}
+
+// Builds the subtree of order[start, start + count) into nodes (root at nodes[0]).
+void bvh_build_serial(std::vector<BNode> &nodes, const std::vector<Box3> &boxes, const std::vector<float> &cen,
+                      std::vector<int> &order, int start, int count) {
+  struct Job { int node, start, count; };
+  std::vector<Job> stack;
+  nodes.assign(1, BNode());
+  stack.push_back({0, start, count});
+  while (!stack.empty()) {
+    Job j = stack.back();
+    stack.pop_back();
+    BNode nd;
+    const int mid = bvh_split(boxes, cen, order, j.start, j.count, nd, 1);
+    nodes[j.node] = nd;
+    if (mid < 0) continue;
+    int l = (int)nodes.size();
+    nodes.push_back(BNode());
+    nodes.push_back(BNode());
+    nodes[j.node].left = l;
+    stack.push_back({l, j.start, mid - j.start});
+    stack.push_back({l + 1, mid, j.start + j.count - mid});
+  }
+}
+
+// Binned-SAH build.  The top of the tree is split with every thread working
+// on each node's bounds and bins; once the open ranges are small, whole
+// subtrees are built in parallel and appended in range order.  The splits do
+// not depend on the thread count.
 void bvh_build(OBVH &B, const std::vector<Box3> &boxes) {
   int n = (int)boxes.size();
   B.nodes.clear();
@@ -286,75 +409,50 @@ void bvh_build(OBVH &B, const std::vector<Box3> &boxes) {
   std::vector<float> cen(3 * (size_t)n);
   for (int i = 0; i < n; i++)
     for (int k = 0; k < 3; k++) cen[3 * (size_t)i + k] = 0.5f * (boxes[i].lo[k] + boxes[i].hi[k]);
+  const int nt = nthreads();
+  const int grain = std::max(4096, n / (8 * std::max(1, nt)));
   struct Job { int node, start, count; };
-  std::vector<Job> stack;
+  std::vector<Job> open, small;
   B.nodes.push_back(BNode());
-  stack.push_back({0, 0, n});
-  while (!stack.empty()) {
-    Job j = stack.back();
-    stack.pop_back();
-    Box3 bb, cb;
-    bb.empty();
-    cb.empty();
-    for (int i = j.start; i < j.start + j.count; i++) {
-      bb.grow(boxes[B.order[i]]);
-      cb.growp(&cen[3 * (size_t)B.order[i]]);
-    }
-    BNode &nd = B.nodes[j.node];
-    for (int k = 0; k < 3; k++) { nd.lo[k] = bb.lo[k]; nd.hi[k] = bb.hi[k]; }
-    if (j.count <= 4) { nd.left = -1; nd.start = j.start; nd.count = j.count; continue; }
-    const int NB = 16;
-    float best = 1e38f;
-    int bestAxis = -1, bestSplit = -1;
-    for (int ax = 0; ax < 3; ax++) {
-      float ext = cb.hi[ax] - cb.lo[ax];
-      if (!(ext > 0.f)) continue;
-      Box3 bin[NB];
-      int cnt[NB] = {0};
-      for (int b = 0; b < NB; b++) bin[b].empty();
-      for (int i = j.start; i < j.start + j.count; i++) {
-        int p = B.order[i];
-        int b = (int)((cen[3 * (size_t)p + ax] - cb.lo[ax]) / ext * NB);
-        b = b < 0 ? 0 : (b >= NB ? NB - 1 : b);
-        cnt[b]++;
-        bin[b].grow(boxes[p]);
-      }
-      float ra[NB];
-      int rc[NB];
-      Box3 acc;
-      acc.empty();
-      int c = 0;
-      for (int b = NB - 1; b > 0; b--) { acc.grow(bin[b]); c += cnt[b]; ra[b] = acc.area(); rc[b] = c; }
-      acc.empty();
-      c = 0;
-      for (int b = 0; b < NB - 1; b++) {
-        acc.grow(bin[b]);
-        c += cnt[b];
-        if (c == 0 || rc[b + 1] == 0) continue;
-        float cost = acc.area() * c + ra[b + 1] * rc[b + 1];
-        if (cost < best) { best = cost; bestAxis = ax; bestSplit = b; }
-      }
-    }
-    int mid;
-    if (bestAxis < 0) {
-      mid = j.start + j.count / 2;
-    } else {
-      float ext = cb.hi[bestAxis] - cb.lo[bestAxis];
-      auto it = std::partition(B.order.begin() + j.start, B.order.begin() + j.start + j.count, [&](int p) {
-        int b = (int)((cen[3 * (size_t)p + bestAxis] - cb.lo[bestAxis]) / ext * NB);
-        b = b < 0 ? 0 : (b >= NB ? NB - 1 : b);
-        return b <= bestSplit;
-      });
-      mid = (int)(it - B.order.begin());
-      if (mid == j.start || mid == j.start + j.count) mid = j.start + j.count / 2;
-    }
+  open.push_back({0, 0, n});
+  while (!open.empty()) {
+    Job j = open.back();
+    open.pop_back();
+    if (j.count <= grain) { small.push_back(j); continue; }
+    BNode nd;
+    const int mid = bvh_split(boxes, cen, B.order, j.start, j.count, nd, j.count >= (1 << 16) ? nt : 1);
+    B.nodes[j.node] = nd;
+    if (mid < 0) continue;
     int l = (int)B.nodes.size();
     B.nodes.push_back(BNode());
     B.nodes.push_back(BNode());
     B.nodes[j.node].left = l;
-    B.nodes[j.node].count = 0;
-    stack.push_back({l, j.start, mid - j.start});
-    stack.push_back({l + 1, mid, j.start + j.count - mid});
+    open.push_back({l, j.start, mid - j.start});
+    open.push_back({l + 1, mid, j.start + j.count - mid});
+  }
+  std::sort(small.begin(), small.end(), [](const Job &a, const Job &b) { return a.start < b.start; });
+  std::vector<std::vector<BNode>> sub(small.size());
+  {
+    std::atomic<int> next(0);
+    std::vector<std::thread> th;
+    for (int k = 0; k < std::max(1, nt); k++)
+      th.emplace_back([&]() {
+        for (;;) {
+          const int q = next.fetch_add(1);
+          if (q >= (int)small.size()) break;
+          bvh_build_serial(sub[q], boxes, cen, B.order, small[q].start, small[q].count);
+        }
+      });
+    for (auto &t : th) t.join();
+  }
+  for (size_t q = 0; q < small.size(); q++) {
+    const int base = (int)B.nodes.size() - 1;   // local node i >= 1 goes to base + i
+    for (size_t i = 0; i < sub[q].size(); i++) {
+      BNode nd = sub[q][i];
+      if (nd.left >= 0) nd.left += base;
+      if (i == 0) B.nodes[small[q].node] = nd;
+      else B.nodes.push_back(nd);
+    }
   }
 }
 
@@ -567,6 +665,12 @@ inline bool better(float t, int g, int p, const GeoHit &h) {
   return false;
 }
 
+// Work counters of the BVH walks (box tests, primitive tests, walks), kept
+// while g_count_walks is set: the algorithmic work per ray of a binned-SAH
+// tree with leaves of <= 4 primitives, walked nearest first.
+bool g_count_walks = false;
+std::atomic<long long> g_node_tests(0), g_prim_tests(0), g_walks(0);
+
 void model_trace(const OModel &M, v3 org, v3 dir, float t0, float tfar, GeoHit &h) {
   h.t = tfar;
   h.geom = -1;
@@ -594,32 +698,51 @@ void model_trace(const OModel &M, v3 org, v3 dir, float t0, float tfar, GeoHit &
       h.t = t; h.geom = s.geomID; h.prim = P.prim; h.u = h.v = 0.f;
     }
   };
+  // nearest-first walk: a node's two child boxes are tested at the node, the
+  // hit ones pushed farther first with their entry distance, and a popped
+  // entry beyond the current hit is skipped (the closest hit and its tie rule
+  // do not depend on the order)
+  long long node_tests = 0, prim_tests = 0;
   auto walk = [&](const OBVH &B, int nprim, const std::function<void(int)> &test) {
-    if (!M.use_bvh) { for (int i = 0; i < nprim; i++) test(i); return; }
+    if (!M.use_bvh) { for (int i = 0; i < nprim; i++) test(i); prim_tests += nprim; return; }
     if (B.nodes.empty()) return;
     float inv[3] = {1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z};
     float o[3] = {org.x, org.y, org.z};
-    int stack[128];
-    int sp = 0;
-    stack[sp++] = 0;
-    while (sp) {
-      const BNode &nd = B.nodes[stack[--sp]];
+    // entry distance of a box within [t0, current hit], or +inf for a miss;
+    // a slack of a few ulps lets a hit at exactly the current t reach its leaf
+    auto enter = [&](const BNode &nd) {
       float tmin = t0, tmax = h.t;
-      bool miss = false;
       for (int k = 0; k < 3; k++) {
         float a = (nd.lo[k] - o[k]) * inv[k], b = (nd.hi[k] - o[k]) * inv[k];
         float lo = a < b ? a : b, hi = a < b ? b : a;
         tmin = lo > tmin ? lo : tmin;
         tmax = hi < tmax ? hi : tmax;
       }
-      // slack of a few ulps so that a hit at exactly the current t still reaches its leaf
-      if (tmin > tmax * 1.0000005f + 1e-30f) miss = true;
-      if (miss) continue;
+      node_tests++;
+      return tmin > tmax * 1.0000005f + 1e-30f ? INFINITY : tmin;
+    };
+    int stack[128];
+    float sent[128];
+    int sp = 0;
+    const float e0 = enter(B.nodes[0]);
+    if (e0 == INFINITY) return;
+    stack[sp] = 0;
+    sent[sp++] = e0;
+    while (sp) {
+      --sp;
+      if (sent[sp] > h.t * 1.0000005f + 1e-30f) continue;
+      const BNode &nd = B.nodes[stack[sp]];
       if (nd.left < 0) {
         for (int i = nd.start; i < nd.start + nd.count; i++) test(B.order[i]);
+        prim_tests += nd.count;
       } else {
-        stack[sp++] = nd.left;
-        stack[sp++] = nd.left + 1;
+        const float ea = enter(B.nodes[nd.left]), eb = enter(B.nodes[nd.left + 1]);
+        const bool near_a = ea <= eb;
+        const int nn = near_a ? nd.left : nd.left + 1, fn = near_a ? nd.left + 1 : nd.left;
+        const float en = near_a ? ea : eb, ef = near_a ? eb : ea;
+        if (sp + 2 > 128) { fprintf(stderr, "oracle: BVH walk stack overflow\n"); abort(); }
+        if (ef != INFINITY) { stack[sp] = fn; sent[sp++] = ef; }
+        if (en != INFINITY) { stack[sp] = nn; sent[sp++] = en; }
       }
     }
   };
@@ -641,6 +764,11 @@ void model_trace(const OModel &M, v3 org, v3 dir, float t0, float tfar, GeoHit &
   walk(M.tbvh, (int)M.tris.size(), test_tri);
   walk(M.sbvh, (int)M.sphs.size(), test_sph);
   walk(M.cbvh, (int)M.curs.size(), test_cur);
+  if (g_count_walks) {
+    g_node_tests.fetch_add(node_tests, std::memory_order_relaxed);
+    g_prim_tests.fetch_add(prim_tests, std::memory_order_relaxed);
+    g_walks.fetch_add(1, std::memory_order_relaxed);
+  }
 }
 
 // ------------------------------------------------------------------ vis
@@ -1048,6 +1176,14 @@ const float kTwoPi = 6.2831855f;  // 2.f*M_PI in float
 OEXPORT void oracle_set_threads(int n) { g_threads = n; }
 OEXPORT void oracle_set_opacity_threshold(float a) { g_opaque_alpha = a; }
 OEXPORT void oracle_plant_fault(int code) { g_fault = code; }
+// counts: {box tests, primitive tests, walks} since the last reset; on != 0 counts
+OEXPORT void oracle_walk_counters(int on, long long *counts) {
+  if (counts) { counts[0] = g_node_tests.load(); counts[1] = g_prim_tests.load(); counts[2] = g_walks.load(); }
+  g_node_tests = 0;
+  g_prim_tests = 0;
+  g_walks = 0;
+  g_count_walks = on != 0;
+}
 OEXPORT void oracle_set_reverse_lighting(int on) { g_reverse = on ? 1 : 0; }
 OEXPORT int oracle_get_threads(void) { return nthreads(); }
 

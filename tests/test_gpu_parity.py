@@ -211,6 +211,35 @@ def test_geometry_meshes_and_spheres(hip, oracle):
     assert leaves.value == 20480 + 20480 + 3000 and depth.value <= 60
 
 
+def test_nine_icosphere_frame_under_every_geometry_setting(hip, oracle):
+    """The nine-icosphere scene (subdivision 3, 16 AO + 1 shadow ray per hit)
+    at 128x96 against the oracle under each walk the geometry tuner may pick
+    (BVH width, traversal stack, waves per SIMD): counts exact, pixels within
+    the FP32 accumulation tolerance."""
+    from galaxy_amd import autotune, workloads
+    from galaxy_amd import lib as glib
+    L_ = glib.lib()
+    sc, _ = workloads.nine_icospheres(128, 96, subdiv=3)
+    c = sc.cameras[0]
+    lights = scenes.resolve_lights(sc.lighting, c)
+    fb_o, st_o = oracle.render_frame(scenes.build_scene(oracle, sc, lights), c)
+    assert st_o.secondary_rays > 20000
+    Bh = scenes.build_scene(hip, sc, lights)
+    saved = (L_.gxy_get_bvh_width(), L_.gxy_get_geometry_stack(), L_.gxy_get_geometry_waves())
+    try:
+        for w, stk, wv in autotune.geometry_candidates():
+            glib.check(L_.gxy_set_bvh_width(w))
+            glib.check(L_.gxy_set_geometry_stack(stk))
+            glib.check(L_.gxy_set_geometry_waves(wv))
+            fb_g, st_g = hip.render_frame(Bh, c)
+            assert same_stats(st_g, st_o), (w, stk, wv)
+            assert np.abs(fb_g - fb_o).max() <= 1e-4, (w, stk, wv)
+    finally:
+        glib.check(L_.gxy_set_bvh_width(saved[0]))
+        glib.check(L_.gxy_set_geometry_stack(saved[1]))
+        glib.check(L_.gxy_set_geometry_waves(saved[2]))
+
+
 def test_mesh_frame_with_shadows_matches_oracle(hip, oracle):
     from galaxy_amd import workloads
     sc, alpha = workloads.sphere_mesh(256, 256)
