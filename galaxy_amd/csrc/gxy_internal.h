@@ -57,8 +57,8 @@ int trace_pl_blocks(int n, int chunk);
 hipError_t launch_reduce_add(const unsigned long long *part, int nb, unsigned long long *acc, hipStream_t s);
 
 // Exclusive sum of m ints by the device-wide scan; temp holds scan_temp_bytes(m).
-size_t scan_temp_bytes(int m);
-hipError_t launch_exclusive_sum(const int *in, int *out, int m, void *temp, hipStream_t s);
+size_t scan_temp_bytes(long long m);
+hipError_t launch_exclusive_sum(const int *in, int *out, long long m, void *temp, hipStream_t s);
 // Offsets from per-workgroup counts: scratch holds block_scan_scratch_bytes(nb)
 // bytes, the count kernel writes nb counts and a zero after them at scratch,
 // and launch_block_scan writes their exclusive sum (nb + 1 entries, the total
@@ -222,14 +222,23 @@ hipError_t launch_random_rays(const RayList_ispc &R, int n, unsigned seed, int w
 // (device, allocated through alloc_fn) and the sorted primitive order.
 typedef void *(*alloc_fn_t)(size_t);
 typedef void (*free_fn_t)(void *);
+// Leaves of up to GXY_BVH_LEAF primitives; trees of height below GXY_BVH_MAXH.
+#define GXY_BVH_LEAF 8
+#define GXY_BVH_MAXH 256
 struct BVHBuildResult {
-  BVHNode *nodes;
-  int *order;
+  BVHNode *nodes;     // binary tree, root 0
+  int *order;         // primitive of each leaf slot (leaf ranges index it)
   int nnodes;
-  int depth;
-  float sah;
+  int depth;          // levels of the binary tree, leaves included
+  float sah;          // SAH cost of the binary tree over its root's surface area
+  BVH4Node *nodes4;   // the same tree collapsed to 4-wide nodes, root 0
+  int nnodes4;
+  int depth4;         // levels of 4-wide inner nodes
+  int leaves;
+  int heights;        // height of the tree before the last treelet round
 };
-hipError_t build_lbvh(const float *boxes_dev, int n, alloc_fn_t alloc, free_fn_t dfree, BVHBuildResult *out,
-                      hipStream_t s);
+// prim_cost: SAH cost of one primitive test against one node visit's 1.2
+hipError_t build_lbvh(const float *boxes_dev, int n, float prim_cost, alloc_fn_t alloc, free_fn_t dfree,
+                      BVHBuildResult *out, hipStream_t s);
 
 }  // namespace gxy

@@ -88,6 +88,10 @@ struct Runtime {
   std::atomic<int> raylist_id{0};
   float opaque_alpha = 0.999f;
   int reverse_lighting = 1;       // CMakeLists.txt:66 GXY_REVERSE_LIGHTING (default ON)
+  // SAH cost of one triangle, sphere and round-cone test (a node visit costs
+  // 1.2): the trace times of the mesh and path-line frames fall as the cost
+  // rises to these values and are flat beyond them
+  float bvh_cost[3] = {2.0f, 1.0f, 8.0f};
   std::vector<void *> outbound[6];   // RayListObj* per exit face, last frames
   std::vector<std::pair<int *, int *>> free_reads;   // readback areas of destroyed contexts (kept mapped)
   // volume-only trace kernel: 0 one ray per work-item, 1 persistent lanes
@@ -421,6 +425,7 @@ struct ModelObj {
   int *hit_geom = nullptr, *hit_prim = nullptr;
   int tnodes = 0, snodes = 0, tdepth = 0, sdepth = 0, tdepth4 = 0, sdepth4 = 0;
   int cnodes = 0, cdepth = 0, cdepth4 = 0;
+  float tsah = 0.f, ssah = 0.f, csah = 0.f;   // SAH cost of each tree over its root's surface
   void free_built() {
     for (void *p : owned) dfree(p);
     owned.clear();
@@ -1359,173 +1364,39 @@ __global__ void k_gather_sph(const int *order, const int *fgeom, const int *fpri
 // Structural check of a built tree: every child index in range, every inner
 // node and every leaf reached exactly once from the root.  Run for trees of up
 // to GXY_BVH_VALIDATE_MAX nodes (default 4M) before the tree is used.
-// Collapses a binary BVH (device) into 4-wide nodes (device): each 4-wide
-// node takes a binary inner node's two children and repeatedly opens the
-// inner child with the largest box surface until it holds four children (or
-// only leaves remain); a subtree of at most BVH4_LEAF primitives (a
-// contiguous range of the sorted primitives) becomes one leaf, other inner
-// children become 4-wide nodes, depth first.  Returns the node buffer (allocated with dalloc) and the
-// 4-wide depth.
-#define BVH4_LEAF 4
-static int collapse_bvh4(const BVHNode *dnodes, int nnodes, BVH4Node **out, int *depth4) {
-  std::vector<BVHNode> h(nnodes);
-  HIPCHK(hipMemcpyAsync(h.data(), dnodes, sizeof(BVHNode) * nnodes, hipMemcpyDeviceToHost, S()));
+// Structural check of a 4-wide tree built on the device: child indices in
+// range, every node reached once from the root, every primitive in exactly one
+// leaf.  Run for trees of up to GXY_BVH_VALIDATE_MAX nodes (default 4M).
+static int validate_bvh4(const BVH4Node *d4, int nn4, int nprim, const char *what) {
+  size_t lim = getenv("GXY_BVH_VALIDATE_MAX") ? (size_t)atol(getenv("GXY_BVH_VALIDATE_MAX")) : (size_t)(4u << 20);
+  if ((size_t)nn4 > lim) return 0;
+  std::vector<BVH4Node> n4(nn4);
+  HIPCHK(hipMemcpyAsync(n4.data(), d4, sizeof(BVH4Node) * nn4, hipMemcpyDeviceToHost, S()));
   HIPCHK(hipStreamSynchronize(S()));
-  // sorted-primitive range of every binary subtree (a Karras subtree covers a
-  // contiguous range), so that a subtree of at most BVH4_LEAF primitives
-  // becomes one leaf of the 4-wide tree
-  std::vector<int> first(nnodes, 0), last(nnodes, -1);
-  {
-    std::vector<std::pair<int, int>> st;   // (node, state)
-    st.push_back({0, 0});
-    while (!st.empty()) {
-      auto [n, state] = st.back();
-      st.pop_back();
-      const int c[2] = {h[n].c0, h[n].c1};
-      if (state == 0) {
-        st.push_back({n, 1});
-        for (int k = 0; k < 2; k++)
-          if (c[k] >= 0) st.push_back({c[k], 0});
-        continue;
-      }
-      int f = INT32_MAX, l = -1;
-      for (int k = 0; k < 2; k++) {
-        int cf, cl;
-        if (c[k] >= 0) {
-          cf = first[c[k]];
-          cl = last[c[k]];
-        } else {
-          const int code = ~c[k];
-          cf = code >> 4;
-          cl = cf + (code & 15);
-        }
-        f = std::min(f, cf);
-        l = std::max(l, cl);
-      }
-      first[n] = f;
-      last[n] = l;
-    }
-  }
-  // an inner entry whose subtree holds at most BVH4_LEAF primitives becomes a leaf
-  auto as_leaf = [&](int code) {
-    if (code < 0) return code;
-    const int cnt = last[code] - first[code] + 1;
-    return cnt <= BVH4_LEAF ? ~((first[code] << 4) | (cnt - 1)) : code;
-  };
-  struct Ent {
-    int code;
-    float lo[3], hi[3];
-  };
-  auto area = [](const Ent &e) {
-    float d0 = e.hi[0] - e.lo[0], d1 = e.hi[1] - e.lo[1], d2 = e.hi[2] - e.lo[2];
-    return d0 * d1 + d1 * d2 + d2 * d0;
-  };
-  std::vector<BVH4Node> n4;
-  n4.reserve(nnodes / 2 + 4);
-  struct Job {
-    int bin;    // binary inner node
-    int slot;   // 4-wide node to fill
-    int depth;
-  };
-  std::vector<Job> jobs;
-  n4.push_back(BVH4Node());
-  jobs.push_back({0, 0, 1});
-  int maxd = 1;
-  while (!jobs.empty()) {
-    Job j = jobs.back();
-    jobs.pop_back();
-    maxd = std::max(maxd, j.depth);
-    const BVHNode &b = h[j.bin];
-    std::vector<Ent> es(2);
-    es[0].code = as_leaf(b.c0);
-    es[1].code = as_leaf(b.c1);
-    for (int k = 0; k < 3; k++) {
-      es[0].lo[k] = b.lo0[k];
-      es[0].hi[k] = b.hi0[k];
-      es[1].lo[k] = b.lo1[k];
-      es[1].hi[k] = b.hi1[k];
-    }
-    while (es.size() < 4) {
-      int best = -1;
-      for (int k = 0; k < (int)es.size(); k++)
-        if (es[k].code >= 0 && (best < 0 || area(es[k]) > area(es[best]))) best = k;
-      if (best < 0) break;
-      const BVHNode &c = h[es[best].code];
-      Ent a, d;
-      a.code = as_leaf(c.c0);
-      d.code = as_leaf(c.c1);
-      for (int k = 0; k < 3; k++) {
-        a.lo[k] = c.lo0[k];
-        a.hi[k] = c.hi0[k];
-        d.lo[k] = c.lo1[k];
-        d.hi[k] = c.hi1[k];
-      }
-      es[best] = a;
-      es.push_back(d);
-    }
-    BVH4Node nd;
-    memset(&nd, 0, sizeof nd);
+  std::vector<char> seen_n(nn4, 0);
+  std::vector<int> seen_p(nprim, 0);
+  std::vector<int> st{0};
+  while (!st.empty()) {
+    const int n = st.back();
+    st.pop_back();
+    if (n < 0 || n >= nn4 || seen_n[n]) return fail("%s 4-wide BVH: node %d invalid or shared", what, n);
+    seen_n[n] = 1;
     for (int k = 0; k < 4; k++) {
-      float *lo[3] = {nd.lox, nd.loy, nd.loz}, *hi[3] = {nd.hix, nd.hiy, nd.hiz};
-      if (k < (int)es.size()) {
-        for (int a = 0; a < 3; a++) {
-          lo[a][k] = es[k].lo[a];
-          hi[a][k] = es[k].hi[a];
-        }
-        if (es[k].code >= 0) {
-          int idx = (int)n4.size();
-          n4.push_back(BVH4Node());
-          jobs.push_back({es[k].code, idx, j.depth + 1});
-          nd.child[k] = idx;
-        } else {
-          nd.child[k] = es[k].code;
-        }
+      if (!(n4[n].lox[k] <= n4[n].hix[k])) continue;   // empty slot
+      const int c = n4[n].child[k];
+      if (c >= 0) {
+        st.push_back(c);
       } else {
-        for (int a = 0; a < 3; a++) {
-          lo[a][k] = INFINITY;
-          hi[a][k] = -INFINITY;
-        }
-        nd.child[k] = 0x7fffffff;
+        const int code = ~c, a = code >> 4, cnt = (code & 15) + 1;
+        if (a < 0 || a + cnt > nprim) return fail("%s 4-wide BVH: leaf range %d+%d outside %d", what, a, cnt, nprim);
+        for (int q = a; q < a + cnt; q++) seen_p[q]++;
       }
     }
-    n4[j.slot] = nd;
   }
-  // structure check: child indices in range, every node reached once, every
-  // primitive in exactly one leaf (leaf ranges inside the primitive arrays)
-  {
-    int nprim = 0;
-    for (int k = 0; k < nnodes; k++) nprim = std::max(nprim, last[k] + 1);
-    std::vector<char> seen_n(n4.size(), 0), seen_p(nprim, 0);
-    std::vector<int> st{0};
-    while (!st.empty()) {
-      const int n = st.back();
-      st.pop_back();
-      if (n < 0 || n >= (int)n4.size() || seen_n[n]) return fail("4-wide BVH: node %d invalid or shared", n);
-      seen_n[n] = 1;
-      for (int k = 0; k < 4; k++) {
-        if (!(n4[n].lox[k] <= n4[n].hix[k])) continue;   // empty slot
-        const int c = n4[n].child[k];
-        if (c >= 0) {
-          st.push_back(c);
-        } else {
-          const int code = ~c, a = code >> 4, cnt = (code & 15) + 1;
-          if (a < 0 || a + cnt > nprim) return fail("4-wide BVH: leaf range %d+%d outside %d", a, cnt, nprim);
-          for (int q = a; q < a + cnt; q++) seen_p[q]++;
-        }
-      }
-    }
-    const bool single = nnodes == 1 && h[0].c0 == h[0].c1;   // one primitive: both children are its leaf
-    for (int q = 0; q < nprim; q++)
-      if (seen_p[q] != (single ? 2 : 1)) return fail("4-wide BVH: primitive %d in %d leaves", q, (int)seen_p[q]);
-    for (size_t k = 0; k < n4.size(); k++)
-      if (!seen_n[k]) return fail("4-wide BVH: node %zu unreachable", k);
-  }
-  BVH4Node *d4 = (BVH4Node *)dalloc(sizeof(BVH4Node) * n4.size());
-  if (!d4) return -1;
-  HIPCHK(hipMemcpyAsync(d4, n4.data(), sizeof(BVH4Node) * n4.size(), hipMemcpyHostToDevice, S()));
-  HIPCHK(hipStreamSynchronize(S()));
-  *out = d4;
-  *depth4 = maxd;
+  for (int q = 0; q < nprim; q++)
+    if (seen_p[q] != 1) return fail("%s 4-wide BVH: primitive %d in %d leaves", what, q, seen_p[q]);
+  for (int k = 0; k < nn4; k++)
+    if (!seen_n[k]) return fail("%s 4-wide BVH: node %d unreachable", what, k);
   return 0;
 }
 
@@ -1623,7 +1494,7 @@ GXY_EXPORT int gxy_model_commit(void *model) {
     }
     int *dfg = upload(fg.data(), fg.size()), *dfp = upload(fp.data(), fp.size());
     BVHBuildResult B;
-    HIPCHK(build_lbvh(boxes, (int)ntri, dalloc, dfree, &B, s));
+    HIPCHK(build_lbvh(boxes, (int)ntri, G.bvh_cost[0], dalloc, dfree, &B, s));
     float *tverts = (float *)dalloc(9 * 4 * (size_t)ntri);
     int *tg = (int *)dalloc(4 * (size_t)ntri), *tp = (int *)dalloc(4 * (size_t)ntri);
     if (!tverts || !tg || !tp || !B.nodes) return fail("model commit: out of device memory");
@@ -1644,12 +1515,11 @@ GXY_EXPORT int gxy_model_commit(void *model) {
     M->tdepth = B.depth;
     if (B.depth > 60) return fail("triangle BVH depth %d exceeds the traversal stack", B.depth);
     if (validate_bvh(B.nodes, B.nnodes, (int)ntri, "triangle") != 0) return -1;
-    BVH4Node *n4 = nullptr;
-    int d4 = 0;
-    if (collapse_bvh4(B.nodes, B.nnodes, &n4, &d4) != 0) return -1;
-    M->owned.push_back(n4);
-    M->d.tnodes4 = n4;
-    M->tdepth4 = d4;
+    M->owned.push_back(B.nodes4);
+    if (validate_bvh4(B.nodes4, B.nnodes4, (int)ntri, "triangle") != 0) return -1;
+    M->d.tnodes4 = B.nodes4;
+    M->tdepth4 = B.depth4;
+    M->tsah = B.sah;
   }
   if (nsph > 0) {
     std::vector<int> fg(nsph), fp(nsph);
@@ -1669,7 +1539,7 @@ GXY_EXPORT int gxy_model_commit(void *model) {
     int *dfg = upload(fg.data(), fg.size()), *dfp = upload(fp.data(), fp.size());
     float *dfe = upload(feps.data(), feps.size());
     BVHBuildResult B;
-    HIPCHK(build_lbvh(boxes, (int)nsph, dalloc, dfree, &B, s));
+    HIPCHK(build_lbvh(boxes, (int)nsph, G.bvh_cost[1], dalloc, dfree, &B, s));
     float *sc = (float *)dalloc(12 * (size_t)nsph), *sr = (float *)dalloc(4 * (size_t)nsph);
     float *se = (float *)dalloc(4 * (size_t)nsph);
     int *sg = (int *)dalloc(4 * (size_t)nsph), *sp = (int *)dalloc(4 * (size_t)nsph);
@@ -1691,12 +1561,11 @@ GXY_EXPORT int gxy_model_commit(void *model) {
     M->sdepth = B.depth;
     if (B.depth > 60) return fail("sphere BVH depth %d exceeds the traversal stack", B.depth);
     if (validate_bvh(B.nodes, B.nnodes, (int)nsph, "sphere") != 0) return -1;
-    BVH4Node *n4 = nullptr;
-    int d4 = 0;
-    if (collapse_bvh4(B.nodes, B.nnodes, &n4, &d4) != 0) return -1;
-    M->owned.push_back(n4);
-    M->d.snodes4 = n4;
-    M->sdepth4 = d4;
+    M->owned.push_back(B.nodes4);
+    if (validate_bvh4(B.nodes4, B.nnodes4, (int)nsph, "sphere") != 0) return -1;
+    M->d.snodes4 = B.nodes4;
+    M->sdepth4 = B.depth4;
+    M->ssah = B.sah;
   }
   if (ncur > 0) {
     // round-cone pieces: boxes on the host, tree on the device, pieces
@@ -1717,7 +1586,7 @@ GXY_EXPORT int gxy_model_commit(void *model) {
     float *dbox = upload(boxes.data(), boxes.size());
     if (!dbox) return -1;
     BVHBuildResult B;
-    HIPCHK(build_lbvh(dbox, (int)ncur, dalloc, dfree, &B, s));
+    HIPCHK(build_lbvh(dbox, (int)ncur, G.bvh_cost[2], dalloc, dfree, &B, s));
     if (!B.nodes) return fail("model commit: out of device memory");
     std::vector<int> order(ncur);
     const hipError_t e1 = hipMemcpyAsync(order.data(), B.order, 4 * (size_t)ncur, hipMemcpyDeviceToHost, s);
@@ -1758,12 +1627,11 @@ GXY_EXPORT int gxy_model_commit(void *model) {
     M->cdepth = B.depth;
     if (B.depth > 60) return fail("curve BVH depth %d exceeds the traversal stack", B.depth);
     if (validate_bvh(B.nodes, B.nnodes, (int)ncur, "curve") != 0) return -1;
-    BVH4Node *n4 = nullptr;
-    int d4 = 0;
-    if (collapse_bvh4(B.nodes, B.nnodes, &n4, &d4) != 0) return -1;
-    M->owned.push_back(n4);
-    M->d.cnodes4 = n4;
-    M->cdepth4 = d4;
+    M->owned.push_back(B.nodes4);
+    if (validate_bvh4(B.nodes4, B.nnodes4, (int)ncur, "curve") != 0) return -1;
+    M->d.cnodes4 = B.nodes4;
+    M->cdepth4 = B.depth4;
+    M->csah = B.sah;
     HIPCHK(hipStreamSynchronize(s));
   }
   M->d.depth = std::max(std::max(M->tdepth, M->sdepth), M->cdepth);
@@ -1771,13 +1639,28 @@ GXY_EXPORT int gxy_model_commit(void *model) {
   M->committed = true;
   return 0;
 }
+GXY_EXPORT int gxy_set_bvh_prim_cost(int kind, float cost) {
+  INIT_OR(-1);
+  if (kind < 0 || kind > 2) return fail("BVH primitive kind %d: 0 triangles, 1 spheres, 2 path-line pieces", kind);
+  if (!(cost > 0.f && cost < 1000.f)) return fail("BVH primitive cost %g out of (0, 1000)", cost);
+  G.bvh_cost[kind] = cost;
+  return 0;
+}
+GXY_EXPORT float gxy_get_bvh_prim_cost(int kind) { return (kind >= 0 && kind <= 2) ? G.bvh_cost[kind] : -1.f; }
+
 GXY_EXPORT int gxy_model_bvh_stats(void *model, int *nodes, int *leaves, int *depth, float *sah) {
   ModelObj *M = lookup<ModelObj>(G.models, model);
   if (!M) return fail("not a model handle");
   if (nodes) *nodes = M->tnodes + M->snodes + M->cnodes;
   if (leaves) *leaves = M->d.ntris + M->d.nsph + M->d.ncurve;
   if (depth) *depth = std::max(std::max(M->tdepth, M->sdepth), M->cdepth);
-  if (sah) *sah = 0.f;
+  // the SAH costs of the triangle, sphere and curve trees, weighted by their primitives
+  if (sah) {
+    const double np = (double)M->d.ntris + M->d.nsph + M->d.ncurve;
+    *sah = np > 0 ? (float)((M->tsah * (double)M->d.ntris + M->ssah * (double)M->d.nsph +
+                             M->csah * (double)M->d.ncurve) / np)
+                  : 0.f;
+  }
   return 0;
 }
 GXY_EXPORT int gxy_model_set_hit_output(void *model, int *g, int *p) {

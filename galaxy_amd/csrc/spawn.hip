@@ -5,12 +5,10 @@
 // (nLights slots each).  Since every parent contributes 0 or a fixed count,
 // both offset arrays follow from one exclusive rank of the "primary & surface"
 // flag: a wave ballot + popcount gives the rank inside a wave, LDS combines the
-// four waves of a workgroup, and a single-workgroup scan over workgroup totals
-// gives the global base.  The lighting kernel then applies, per parent and in
+// four waves of a workgroup, and a scan over workgroup totals (one workgroup,
+// or the device-wide scan below for long lists) gives the global base.  The lighting kernel then applies, per parent and in
 // the reference order, ambientLighting -> generateAORays -> diffuseLighting ->
 // generateShadowRays (TraceRays.cpp:124-131, TraceRays.ispc:625-923).
-#include <hipcub/hipcub.hpp>
-
 #include "light_dev.h"
 
 namespace gxy {
@@ -56,33 +54,13 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_flag_rank(RayList_ispc R, int n, 
 }
 
 // ------------------------------------------------------------------ device-wide scans
-// Exclusive sum of m ints (in -> out) by the device-wide single-pass scan of
-// rocPRIM (hipCUB); temp holds scan_temp_bytes(m) bytes.
-size_t scan_temp_bytes(int m) {
-  size_t bytes = 0;
-  hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const int *)nullptr, (int *)nullptr, m, (hipStream_t)0);
-  return (bytes + 255) & ~(size_t)255;
-}
-
-hipError_t launch_exclusive_sum(const int *in, int *out, int m, void *temp, hipStream_t s) {
-  size_t bytes = scan_temp_bytes(m);
-  return hipcub::DeviceScan::ExclusiveSum(temp, bytes, in, out, m, s);
-}
-
-__global__ void k_set_int(int *dst, const int *src, int v) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = src ? *src : v;
-}
-
-// ints of one scan array of m entries, rounded to 64 (256-byte alignment)
-static inline size_t scan_pad(size_t m) { return (m + 63) / 64 * 64; }
-
-size_t block_scan_scratch_bytes(int nb) {
-  // the device-wide scan's temp storage only where it is used (its size
-  // query costs device-property calls on the host)
-  return 2 * sizeof(int) * scan_pad((size_t)nb + 1) + (nb + 1 > GXY_SMALL_SCAN ? scan_temp_bytes(nb + 1) : 0);
-}
-
-int *block_scan_offsets(int *scratch, int nb) { return scratch + scan_pad((size_t)nb + 1); }
+// Exclusive sum of m ints (in -> out), reduce-then-scan over tiles of
+// SCAN_TILE ints (256 work-items x 8): one launch sums each tile, the tile
+// sums are scanned (one workgroup when they fit, else the same scheme one
+// level up), and one launch scans each tile from its base.  temp holds
+// scan_temp_bytes(m) bytes.  m up to 2^31 - 1.
+#define SCAN_ITEMS 8
+#define SCAN_TILE (GXY_BLOCK * SCAN_ITEMS)
 
 // Exclusive sum of m ints (in -> out) by one workgroup of 1024 work-items,
 // each summing a run of ceil(m / 1024): one launch where the device-wide scan
@@ -108,6 +86,91 @@ __global__ __launch_bounds__(1024) void k_scan_small(const int *in, int *out, in
     base += v;
   }
 }
+
+__device__ __forceinline__ int block_sum256(int v, int *red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ __launch_bounds__(GXY_BLOCK) void k_scan_tile_sums(const int *in, long long m, int *sums) {
+  __shared__ int red[4];
+  const long long base = (long long)blockIdx.x * SCAN_TILE;
+  int v = 0;
+  for (int k = 0; k < SCAN_ITEMS; k++) {
+    const long long i = base + (long long)k * GXY_BLOCK + threadIdx.x;
+    if (i < m) v += in[i];
+  }
+  v = block_sum256(v, red);
+  if (threadIdx.x == 0) sums[blockIdx.x] = v;
+}
+
+// Each work-item scans 8 consecutive ints; the work-items' totals are scanned
+// across the workgroup (wave shuffles, then the four wave totals).
+__global__ __launch_bounds__(GXY_BLOCK) void k_scan_tiles(const int *in, int *out, long long m, const int *tile_base) {
+  __shared__ int wtot[4];
+  const long long base = (long long)blockIdx.x * SCAN_TILE + (long long)threadIdx.x * SCAN_ITEMS;
+  int v[SCAN_ITEMS];
+  int t = 0;
+  for (int k = 0; k < SCAN_ITEMS; k++) {
+    v[k] = base + k < m ? in[base + k] : 0;
+    t += v[k];
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int incl = t;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  if (lane == 63) wtot[w] = incl;
+  __syncthreads();
+  int run = tile_base[blockIdx.x] + incl - t;
+  for (int k = 0; k < w; k++) run += wtot[k];
+  for (int k = 0; k < SCAN_ITEMS; k++) {
+    if (base + k < m) out[base + k] = run;
+    run += v[k];
+  }
+}
+
+static inline size_t scan_pad_ll(long long m) { return (size_t)((m + 63) / 64 * 64); }
+
+size_t scan_temp_bytes(long long m) {
+  if (m <= GXY_SMALL_SCAN) return 256;
+  const long long nt = (m + SCAN_TILE - 1) / SCAN_TILE;
+  return 2 * sizeof(int) * scan_pad_ll(nt) + scan_temp_bytes(nt);
+}
+
+hipError_t launch_exclusive_sum(const int *in, int *out, long long m, void *temp, hipStream_t s) {
+  if (m <= 0) return hipSuccess;
+  if (m <= GXY_SMALL_SCAN) {
+    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, s, in, out, (int)m);
+    return hipGetLastError();
+  }
+  const long long nt = (m + SCAN_TILE - 1) / SCAN_TILE;
+  int *sums = (int *)temp, *sbase = sums + scan_pad_ll(nt);
+  hipLaunchKernelGGL(k_scan_tile_sums, dim3((unsigned)nt), dim3(GXY_BLOCK), 0, s, in, m, sums);
+  hipError_t e = launch_exclusive_sum(sums, sbase, nt, sbase + scan_pad_ll(nt), s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)nt), dim3(GXY_BLOCK), 0, s, in, out, m, (const int *)sbase);
+  return hipGetLastError();
+}
+
+__global__ void k_set_int(int *dst, const int *src, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = src ? *src : v;
+}
+
+// ints of one scan array of m entries, rounded to 64 (256-byte alignment)
+static inline size_t scan_pad(size_t m) { return (m + 63) / 64 * 64; }
+
+size_t block_scan_scratch_bytes(int nb) {
+  // the device-wide scan's temp storage only where it is used (its size
+  // query costs device-property calls on the host)
+  return 2 * sizeof(int) * scan_pad((size_t)nb + 1) + (nb + 1 > GXY_SMALL_SCAN ? scan_temp_bytes(nb + 1) : 0);
+}
+
+int *block_scan_offsets(int *scratch, int nb) { return scratch + scan_pad((size_t)nb + 1); }
 
 hipError_t launch_block_scan(int *scratch, int nb, hipStream_t s) {
   int *off = block_scan_offsets(scratch, nb);

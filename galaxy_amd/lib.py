@@ -74,6 +74,8 @@ def _declare(L):
                                               C.c_float, vp]),
         "gxy_model_commit": (C.c_int, [vp]),
         "gxy_model_bvh_stats": (C.c_int, [vp, vp, vp, vp, vp]),
+        "gxy_set_bvh_prim_cost": (C.c_int, [C.c_int, C.c_float]),
+        "gxy_get_bvh_prim_cost": (C.c_float, [C.c_int]),
         "gxy_model_set_hit_output": (C.c_int, [vp, vp, vp]),
         "gxy_spawn_offsets": (C.c_int, [C.c_int, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp]),
         "gxy_spawn_offsets_noalloc": (C.c_int, [C.c_int, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp]),

@@ -310,8 +310,17 @@ int   gxy_model_add_pathlines(void *model, const float *vertices, const float *d
                               const int *indices, int nsegments, float radius0, float radius1,
                               float value0, float value1, void *tf);
 int   gxy_model_commit(void *model);
-/* BVH statistics after commit: nodes, leaves, max depth, SAH cost. */
+/* BVH statistics after commit: binary inner nodes, primitives, levels of the
+ * deepest tree (leaves included), and the SAH cost of the trees (each over its
+ * root's surface area: node visits 1.2, primitive tests their set cost),
+ * averaged with the primitive counts as weights. */
 int   gxy_model_bvh_stats(void *model, int *nodes, int *leaves, int *depth, float *sah);
+/* SAH cost of one primitive test, per kind (0 triangles, 1 spheres, 2
+ * path-line pieces; defaults 2, 1, 8) against 1.2 for a node visit, used by
+ * the next commits: it sets how readily the build makes leaves of several
+ * primitives (up to 8).  Any value gives the same hits. */
+int   gxy_set_bvh_prim_cost(int kind, float cost);
+float gxy_get_bvh_prim_cost(int kind);
 /* Side-band hit record of the last TraceRays_TraceRays on this model:
  * per ray (geomID, primID); device arrays of n ints each, or NULL to disable. */
 int   gxy_model_set_hit_output(void *model, int *geom_ids, int *prim_ids);
