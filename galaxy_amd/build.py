@@ -1,8 +1,9 @@
 """Builds libgxy_hip.so (gfx950) in-tree from galaxy_amd/csrc/*.hip|*.cpp.
 
 Every source is compiled by hipcc for gfx950 only, with FMA contraction off
-(the kernels keep the operation order of the CPU restatement they are checked
-against) and packed FP32 instructions disabled.
+(the volume marches write their FMAs explicitly; everything else keeps the
+operation order of the CPU restatement it is checked against) and packed FP32
+instructions disabled.
 """
 import concurrent.futures as cf
 import os

@@ -259,6 +259,90 @@ __device__ __forceinline__ VFetch vol_fetch(const DVolume &V, f3 p) {
   return F;
 }
 
+// The march's fetch: the position is given in voxel coordinates (a ray's
+// l(t) = A + t * B, one FMA per axis, A = (org - origin) / spacing and
+// B = dir / spacing), clamped into the grid with one v_med3_f32 per axis.  A
+// NaN coordinate (which a finite ray never produces) would still address voxel
+// 0: v_cvt_i32_f32 maps NaN to 0.
+template <int SM>
+__device__ __forceinline__ VFetch vol_fetch_vox(const DVolume &V, float l0, float l1, float l2) {
+  l0 = __builtin_amdgcn_fmed3f(l0, 0.f, V.ub[0]);
+  l1 = __builtin_amdgcn_fmed3f(l1, 0.f, V.ub[1]);
+  l2 = __builtin_amdgcn_fmed3f(l2, 0.f, V.ub[2]);
+  const int i0 = (int)l0, i1 = (int)l1, i2 = (int)l2;
+  VFetch F;
+  F.f0 = l0 - (float)i0;
+  F.f1 = l1 - (float)i1;
+  F.f2 = l2 - (float)i2;
+  if (SM == 4 || SM == 8) {
+    constexpr int B = SM == 4 ? 4 : 8, LB = SM == 4 ? 2 : 3, E = B + 1, PL = E * E;
+    const unsigned brick = (unsigned)(i0 >> LB) + __umul24((unsigned)(i1 >> LB), (unsigned)V.nb[0]) +
+                           __umul24((unsigned)(i2 >> LB), (unsigned)V.nb[0] * (unsigned)V.nb[1]);
+    const unsigned loc = (unsigned)(i0 & (B - 1)) + (unsigned)(i1 & (B - 1)) * E + (unsigned)(i2 & (B - 1)) * PL;
+    const float *d = gptr(V.bdata) + ((size_t)brick * brick_floats(B) + loc);
+    F.p00 = *(const float2u *)(d);
+    F.p10 = *(const float2u *)(d + E);
+    F.p01 = *(const float2u *)(d + PL);
+    F.p11 = *(const float2u *)(d + PL + E);
+    return F;
+  }
+  const float *vd = gptr(V.data);
+  const bool small = SM == 1 || (SM < 0 && V.small);
+  if (small) {
+    const unsigned sy = (unsigned)V.n[0], sz = (unsigned)V.n[0] * (unsigned)V.n[1];
+    const unsigned off = (unsigned)i0 + __umul24((unsigned)i1, sy) + __umul24((unsigned)i2, sz);
+    const size_t bo = (size_t)(off * 4u);
+    F.p00 = *(const float2u *)((const char *)vd + bo);
+    F.p10 = *(const float2u *)((const char *)(vd + sy) + bo);
+    F.p01 = *(const float2u *)((const char *)(vd + sz) + bo);
+    F.p11 = *(const float2u *)((const char *)(vd + sz + sy) + bo);
+  } else {
+    const int64_t sy = V.n[0], SZ = (int64_t)V.n[0] * V.n[1];
+    const float *d = vd + ((int64_t)i0 + sy * i1 + SZ * i2);
+    F.p00 = *(const float2u *)(d);
+    F.p10 = *(const float2u *)(d + sy);
+    F.p01 = *(const float2u *)(d + SZ);
+    F.p11 = *(const float2u *)(d + SZ + sy);
+  }
+  return F;
+}
+
+// a + f * (b - a): one subtraction and one FMA
+__device__ __forceinline__ float lerp_fma(float a, float b, float f) { return __builtin_fmaf(f, b - a, a); }
+
+// Trilinear interpolation of a fetch with FMA lerps (x, then y, then z).
+__device__ __forceinline__ float vol_finish_fma(const VFetch &F) {
+  const float a00 = lerp_fma(F.p00.x, F.p00.y, F.f0);
+  const float a10 = lerp_fma(F.p10.x, F.p10.y, F.f0);
+  const float a01 = lerp_fma(F.p01.x, F.p01.y, F.f0);
+  const float a11 = lerp_fma(F.p11.x, F.p11.y, F.f0);
+  return lerp_fma(lerp_fma(a00, a10, F.f1), lerp_fma(a01, a11, F.f1), F.f2);
+}
+
+// A ray's position in one volume's voxel coordinates, l(t) = A + t * B.
+struct VoxRay {
+  float a0, a1, a2, b0, b1, b2;
+};
+__device__ __forceinline__ VoxRay vox_ray(const DVolume &V, f3 org, f3 dir) {
+  VoxRay r;
+  r.a0 = (org.x - V.org[0]) * V.isp[0];
+  r.a1 = (org.y - V.org[1]) * V.isp[1];
+  r.a2 = (org.z - V.org[2]) * V.isp[2];
+  r.b0 = dir.x * V.isp[0];
+  r.b1 = dir.y * V.isp[1];
+  r.b2 = dir.z * V.isp[2];
+  return r;
+}
+template <int SM>
+__device__ __forceinline__ VFetch vox_fetch(const DVolume &V, const VoxRay &r, float t) {
+  return vol_fetch_vox<SM>(V, __builtin_fmaf(t, r.b0, r.a0), __builtin_fmaf(t, r.b1, r.a1),
+                           __builtin_fmaf(t, r.b2, r.a2));
+}
+template <int SM>
+__device__ __forceinline__ float vox_sample(const DVolume &V, const VoxRay &r, float t) {
+  return vol_finish_fma(vox_fetch<SM>(V, r, t));
+}
+
 __device__ __forceinline__ float vol_finish(const VFetch &F) {
   float g0 = 1.0f - F.f0, g1 = 1.0f - F.f1, g2 = 1.0f - F.f2;
   float a00 = g0 * F.p00.x + F.f0 * F.p00.y;
@@ -288,31 +372,29 @@ __device__ __forceinline__ f3 vol_gradient(const DVolume &V, f3 p) {
 
 // LinearTransferFunction lookups over tables that may live in LDS; the
 // division by (hi-lo) is a multiplication by the precomputed (n-1)/(hi-lo).
+// The table coordinate x = (v - lo) * (n-1)/(hi-lo) is clamped to [0, n-1]
+// (one v_med3_f32), which gives the end entries outside [lo, hi]; entry
+// i = min(floor(x), n-2) and its successor are blended with one FMA per
+// channel (at x = n-1 the blend reaches the last entry within an ulp).
 __device__ __forceinline__ f3 tf_color(const float *col, int n, float lo, float hi, float sc, float v) {
   if (v != v) return mk3(0.f, 0.f, 0.f);
   if (n == 0) return mk3(1.f, 1.f, 1.f);
-  if (v <= lo) return mk3(col[0], col[1], col[2]);
-  if (v >= hi) return mk3(col[3 * n - 3], col[3 * n - 2], col[3 * n - 1]);
-  float x = (v - lo) * sc;
+  if (n == 1) return mk3(col[0], col[1], col[2]);
+  const float x = __builtin_amdgcn_fmed3f((v - lo) * sc, 0.f, (float)(n - 1));
   int i = (int)x;
-  i = i < n - 1 ? i : n - 1;
-  float r = x - (float)i;
-  int j = i + 1 < n - 1 ? i + 1 : n - 1;
-  float q = 1.0f - r;
-  return mk3(q * col[3 * i] + r * col[3 * j], q * col[3 * i + 1] + r * col[3 * j + 1],
-             q * col[3 * i + 2] + r * col[3 * j + 2]);
+  i = i < n - 2 ? i : n - 2;
+  const float r = x - (float)i;
+  const float *c = col + 3 * i;
+  return mk3(lerp_fma(c[0], c[3], r), lerp_fma(c[1], c[4], r), lerp_fma(c[2], c[5], r));
 }
 __device__ __forceinline__ float tf_opacity(const float *op, int n, float lo, float hi, float so, float v) {
   if (v != v) return 0.f;
   if (n == 0) return 1.f;
-  if (v <= lo) return op[0];
-  if (v >= hi) return op[n - 1];
-  float x = (v - lo) * so;
+  if (n == 1) return op[0];
+  const float x = __builtin_amdgcn_fmed3f((v - lo) * so, 0.f, (float)(n - 1));
   int i = (int)x;
-  i = i < n - 1 ? i : n - 1;
-  float r = x - (float)i;
-  int j = i + 1 < n - 1 ? i + 1 : n - 1;
-  return (1.0f - r) * op[i] + r * op[j];
+  i = i < n - 2 ? i : n - 2;
+  return lerp_fma(op[i], op[i + 1], x - (float)i);
 }
 __device__ __forceinline__ f3 tf_color(const float *col, const DTF &T, float v) {
   return tf_color(col, T.nc, T.lo, T.hi, T.sc, v);

@@ -224,9 +224,9 @@ __device__ __forceinline__ bool pl_step(const TraceParams &P, MarchState &m, flo
   const float step = P.step, epsilon = P.eps;
   float tThis = m.tThis;
   float sThis[NV];
-  f3 coord = add3(m.org, scl3(tThis, m.dir));
+  // the arithmetic of the one-ray-per-item march (voxel-coordinate positions)
 #pragma unroll
-  for (int v = 0; v < NV; v++) sThis[v] = vol_sample(P.vv[v].vol, coord);
+  for (int v = 0; v < NV; v++) sThis[v] = vox_sample<-1>(P.vv[v].vol, vox_ray(P.vv[v].vol, m.org, m.dir), tThis);
   nsamp += NV;
   bool hit_iso = false;
   if (tThis > m.tEntry && m.tLast >= epsilon) {
@@ -258,9 +258,8 @@ __device__ __forceinline__ bool pl_step(const TraceParams &P, MarchState &m, flo
       tThis = ht;
       m.flags |= 2 | 4;
       hit_iso = true;
-      f3 c2 = add3(m.org, scl3(tThis, m.dir));
 #pragma unroll
-      for (int v = 0; v < NV; v++) sThis[v] = vol_sample(P.vv[v].vol, c2);
+      for (int v = 0; v < NV; v++) sThis[v] = vox_sample<-1>(P.vv[v].vol, vox_ray(P.vv[v].vol, m.org, m.dir), tThis);
       nsamp += NV;
     }
 #pragma unroll
@@ -272,12 +271,7 @@ __device__ __forceinline__ bool pl_step(const TraceParams &P, MarchState &m, flo
         if (sop > 0) {
           if (m.flags & 1) {
             f3 sc = tf_color(s_col[major], T, sVolume);
-            float wo = clamp01f(sop * P.vv[major].vol.irate);
-            float a = 1.0f - m.ca;
-            m.cr = m.cr + a * (wo * sc.x);
-            m.cg = m.cg + a * (wo * sc.y);
-            m.cb = m.cb + a * (wo * sc.z);
-            m.ca = m.ca + a * wo;
+            composite(m.cr, m.cg, m.cb, m.ca, sc, clamp01f(sop * P.vv[major].vol.irate));
           } else {
             float wop = ((tThis - m.tLast) * P.istep) * clamp01f(sop * P.vv[major].vol.irate);
             float f = 1.0f - wop;

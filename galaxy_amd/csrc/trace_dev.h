@@ -543,6 +543,17 @@ __device__ __forceinline__ bool opaque_of(float r, float g, float b, float a, fl
   return mnf(mnf(r, g), b) >= 1.0f || a > thr;
 }
 
+// Front-to-back compositing of one weighted sample (TraceRays.ispc:512-533):
+// C += (1 - C.a) * w * (rgb, 1), with (1 - C.a) * w formed once and one FMA
+// per channel.
+__device__ __forceinline__ void composite(float &cr, float &cg, float &cb, float &ca, f3 sc, float wo) {
+  const float a = 1.0f - ca, aw = a * wo;
+  cr = __builtin_fmaf(aw, sc.x, cr);
+  cg = __builtin_fmaf(aw, sc.y, cg);
+  cb = __builtin_fmaf(aw, sc.z, cb);
+  ca = __builtin_fmaf(a, wo, ca);
+}
+
 // One ray of TraceRays_TraceRays (TraceRays.ispc:371-622): reads ray i of R,
 // writes its colour, t, term and surface fields back.  s_col / s_op hold the
 // NV transfer-function tables (LDS).  Returns the number of volume samples.
@@ -642,15 +653,16 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
       // the gathers of the sample at tThis are issued one iteration ahead
       // (every fetch position is clamped into the volume, so a fetch that the
       // march does not use is harmless)
-      VFetch F[NVS];
-      {
-        f3 c0 = add3(org, scl3(tThis, dir));
+      // positions in each volume's voxel coordinates: one FMA per axis and sample
+      VoxRay VR[NVS];
 #pragma unroll
-        for (int v = 0; v < NV; v++) F[v] = vol_fetch<SM>(P.vv[v].vol, c0);
-      }
+      for (int v = 0; v < NV; v++) VR[v] = vox_ray(P.vv[v].vol, org, dir);
+      VFetch F[NVS];
+#pragma unroll
+      for (int v = 0; v < NV; v++) F[v] = vox_fetch<SM>(P.vv[v].vol, VR[v], tThis);
       while (tThis <= tTermination && !opaque && !hit_iso && iter++ < GXY_MAX_MARCH) {
 #pragma unroll
-        for (int v = 0; v < NV; v++) sThis[v] = vol_finish(F[v]);
+        for (int v = 0; v < NV; v++) sThis[v] = vol_finish_fma(F[v]);
         nsamp += NV;
         // the next position when neither an iso hit nor opacity ends the march
         // here (the only cases in which the loop continues, and then
@@ -658,11 +670,8 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
         const float nx0 = (tThis == tEntry) ? (tEntry + epsilon)
                           : (((tThis + step) > tTermination) && (tThis < tTermination)) ? tTermination
                                                                                         : tThis + step;
-        {
-          f3 c1 = add3(org, scl3(nx0, dir));
 #pragma unroll
-          for (int v = 0; v < NV; v++) F[v] = vol_fetch<SM>(P.vv[v].vol, c1);
-        }
+        for (int v = 0; v < NV; v++) F[v] = vox_fetch<SM>(P.vv[v].vol, VR[v], nx0);
         if (tThis > tEntry && tLast >= epsilon) {
           // LookForIsoHit: the last matching isovalue in list order wins
           bool ih = false;
@@ -693,9 +702,8 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
             tThis = hit.t;
             surface_hit = true;
             hit_iso = true;
-            f3 c2 = add3(org, scl3(tThis, dir));
 #pragma unroll
-            for (int v = 0; v < NV; v++) sThis[v] = vol_sample_t<SM>(P.vv[v].vol, c2);
+            for (int v = 0; v < NV; v++) sThis[v] = vox_sample<SM>(P.vv[v].vol, VR[v], tThis);
             nsamp += NV;
           }
 #pragma unroll
@@ -707,12 +715,7 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
               if (sop > 0) {
                 if (shadeFlag) {
                   f3 sc = tf_color(s_col[major], T, sVolume);
-                  float wo = clamp01f(sop * P.vv[major].vol.irate);
-                  float a = 1.0f - ca;
-                  cr = cr + a * (wo * sc.x);
-                  cg = cg + a * (wo * sc.y);
-                  cb = cb + a * (wo * sc.z);
-                  ca = ca + a * wo;
+                  composite(cr, cg, cb, ca, sc, clamp01f(sop * P.vv[major].vol.irate));
                 } else {
                   float wop = ((tThis - tLast) * P.istep) * clamp01f(sop * P.vv[major].vol.irate);
                   float f = 1.0f - wop;
@@ -786,31 +789,30 @@ __device__ __forceinline__ unsigned sample_one(const TraceParams &P, const RayLi
   float tThis = tEntry + step;
   int hit = -1;
   float sLast[NV], tLast[NV];
-  f3 c0 = add3(org, scl3(tEntry, dir));
+  // positions in each volume's voxel coordinates: one FMA per axis and sample
+  VoxRay VR[NV];
 #pragma unroll
-  for (int k = 0; k < NV; k++) { sLast[k] = vol_sample_t<SM>(P.vv[k].vol, c0); tLast[k] = tEntry; }
+  for (int k = 0; k < NV; k++) {
+    VR[k] = vox_ray(P.vv[k].vol, org, dir);
+    sLast[k] = vox_sample<SM>(P.vv[k].vol, VR[k], tEntry);
+    tLast[k] = tEntry;
+  }
   nsamp += NV;
   int iter = 0;
   // gathers issued one step ahead, as in the trace march (positions clamp
   // into the volume, so an unused fetch is harmless)
   VFetch F[NV];
-  {
-    f3 c1 = add3(org, scl3(tThis, dir));
 #pragma unroll
-    for (int k = 0; k < NV; k++) F[k] = vol_fetch<SM>(P.vv[k].vol, c1);
-  }
+  for (int k = 0; k < NV; k++) F[k] = vox_fetch<SM>(P.vv[k].vol, VR[k], tThis);
   while (tThis <= tExit && hit == -1 && iter++ < GXY_MAX_MARCH) {
     float sCur[NV];
 #pragma unroll
-    for (int k = 0; k < NV; k++) sCur[k] = vol_finish(F[k]);
+    for (int k = 0; k < NV; k++) sCur[k] = vol_finish_fma(F[k]);
     // the next position when no crossing ends the march here
     float nt0 = tThis + step;
     if (nt0 > tExit) nt0 = tExit;
-    {
-      f3 c2 = add3(org, scl3(nt0, dir));
 #pragma unroll
-      for (int k = 0; k < NV; k++) F[k] = vol_fetch<SM>(P.vv[k].vol, c2);
-    }
+    for (int k = 0; k < NV; k++) F[k] = vox_fetch<SM>(P.vv[k].vol, VR[k], nt0);
 #pragma unroll
     for (int k = 0; k < NV; k++) {
       if (hit == -1) {
