@@ -100,6 +100,7 @@ class ServeStats(C.Structure):
 
 
 SERVE_CONTINUOUS, SERVE_ONE_AT_A_TIME = 0, 1
+RQ_DONE, RQ_FAILED, RQ_REJECTED, RQ_CANCELLED, RQ_QUEUED, RQ_RUNNING = 0, -1, -2, -3, 1, 2
 
 
 def make_box(lo, hi):

@@ -20,7 +20,7 @@ FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-ffp-contract=
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-munsafe-fp-atomics", "-fvisibility=hidden", "-Wno-unused-result",
          "-I" + os.path.join(HERE, "..", "include")]
-SOURCES = ["trace.hip", "spawn.hip", "route.hip", "noise.hip", "bvh.hip", "batch.hip", "runtime.cpp"]
+SOURCES = ["trace.hip", "spawn.hip", "route.hip", "noise.hip", "bvh.hip", "batch.hip", "runtime.cpp", "viewer.cpp"]
 
 
 def _compile(src):

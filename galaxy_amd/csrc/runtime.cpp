@@ -16,7 +16,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -3328,12 +3330,18 @@ GXY_EXPORT long long gxy_batch_bytes(void *b) {
   INIT_OR(-1);
   BatchObj *B = lookup<BatchObj>(g_batches, b);
   if (!B) return fail("not a batch handle");
+  // the device blocks the batch holds, at their allocated (size-class) sizes
+  std::lock_guard<std::recursive_mutex> lk(G.mu);
+  auto held = [](const void *p) -> long long {
+    auto it = G.live.find((void *)p);
+    return it == G.live.end() ? 0 : (long long)it->second;
+  };
   long long t = 0;
   for (int k = 0; k < 2; k++)
-    if (B->pool[k]) t += (long long)B->pool[k]->bytes;
-  t += (long long)(B->tcap[0] + B->tcap[1] + B->init_cap + B->fresh_cap) * sizeof(BatchChunk);
-  t += (long long)B->pc_cap * 5 * sizeof(int);
-  t += (long long)B->part_blocks * 4 * BS_COUNT * 8 + (long long)B->kcap * 3 * 4;
+    if (B->pool[k]) t += held(B->pool[k]->block);
+  const void *bufs[] = {B->d_ds, B->d_lists, B->d_rls, B->tab[0], B->tab[1], B->d_init, B->d_fresh,
+                        B->pc,   B->seg,     B->lpend, B->part,   B->stats,  B->ints,    B->kidx};
+  for (const void *p : bufs) t += held(p);
   return t;
 }
 
@@ -3349,18 +3357,8 @@ GXY_EXPORT int gxy_rays_random(const RayList_ispc *rays, int n, unsigned seed, i
 // ====================================================================== served form
 namespace gxy {
 
-struct ServerObj {
-  long long budget = 0;
-  int slots = 1;
-  void *batch = nullptr;
-  std::vector<const Visualization_ispc *> vis;
-  std::vector<float> eps;
-  const Lighting_ispc *lights = nullptr;
-  float leps = 0.001f;
-  int nsp = 0;
-};
-
-static std::set<void *> g_servers;
+// Request states (gxy_request_result.status)
+enum { RQ_DONE = 0, RQ_FAILED = -1, RQ_REJECTED = -2, RQ_CANCELLED = -3, RQ_QUEUED = 1, RQ_RUNNING = 2 };
 
 struct Flight {
   int req = -1, list = -1;
@@ -3371,12 +3369,69 @@ struct Flight {
   int n = 0;
 };
 
+// One request of a server: the caller's description (its out buffer must stay
+// valid until the request is final), its result, and whether a cancellation
+// is asked for.
+struct ReqRecord {
+  gxy_request req;
+  gxy_request_result res;
+  bool cancel = false;
+};
+
+struct ServerObj {
+  long long budget = 0;
+  int slots = 1;
+  void *batch = nullptr;
+  std::vector<const Visualization_ispc *> vis;
+  std::vector<float> eps;
+  const Lighting_ispc *lights = nullptr;
+  float leps = 0.001f;
+  int nsp = 0;
+  // the requests, the queue of those not yet admitted (ids in arrival order)
+  // and the admitted ones; guarded by mu.  A record's request is not changed
+  // after it is added (the deque keeps references to records valid), so the
+  // service thread reads it without the lock.
+  std::mutex mu;
+  std::condition_variable cv;        // a request arrived or a cancellation was asked for
+  std::condition_variable idle_cv;   // a step ended or the queue changed
+  std::deque<ReqRecord> recs;
+  std::deque<int> queue;
+  std::vector<Flight> fl;
+  long long base_bytes = 0;          // device bytes in use when serving started
+  long long peak = 0;
+  int max_in_flight = 0;
+  std::chrono::steady_clock::time_point t0;
+  // the live service thread (gxy_server_start)
+  std::thread th;
+  bool running = false, stop = false;
+  int loop_error = 0;
+  std::string loop_msg;
+};
+
+static std::set<void *> g_servers;
+
+static double since(const ServerObj *S_) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - S_->t0).count();
+}
+
 static long long request_bytes(const ServerObj *S_, const gxy_request &r) {
   if (r.kind == 0) {
     const long long px = (long long)r.width * r.height;
     return px * (100 + 16) + 2 * px * (1 + S_->nsp) * 100;
   }
   return r.nrays * 100 * 3 + r.capacity * 12;
+}
+
+// a request the server cannot take at all (before any device work)
+static const char *request_invalid(const ServerObj *S_, const gxy_request &r) {
+  if (r.kind != 0 && r.kind != 1) return "request kind must be 0 (render) or 1 (sample)";
+  if (r.dataset < 0 || r.dataset >= (int)S_->vis.size()) return "request dataset out of range";
+  if (r.kind == 0 && (r.width <= 0 || r.height <= 0 || (long long)r.width * r.height > INT32_MAX / 2))
+    return "render request size out of range";
+  if (r.kind == 1 && (r.nrays <= 0 || r.nrays > INT32_MAX / 2 || r.capacity < 0))
+    return "sample request size out of range";
+  if (request_bytes(S_, r) > S_->budget) return "request needs more device memory than the server's budget";
+  return nullptr;
 }
 
 static void flight_free(Flight &f) {
@@ -3391,11 +3446,8 @@ static void flight_free(Flight &f) {
 
 // the request's rays and outputs on the device
 static int flight_make(ServerObj *S_, const gxy_request &r, Flight &f) {
-  if (r.dataset < 0 || r.dataset >= (int)S_->vis.size()) return fail("request dataset %d out of range", r.dataset);
   const Visualization_ispc *v = S_->vis[r.dataset];
   if (r.kind == 0) {
-    if (r.width <= 0 || r.height <= 0 || (long long)r.width * r.height > INT32_MAX / 2)
-      return fail("render request of %dx%d", r.width, r.height);
     const int px = r.width * r.height;
     f.rays = raylist_new(px, 0, 0, false);
     f.fb = (float *)dalloc((size_t)px * 16);
@@ -3407,7 +3459,6 @@ static int flight_make(ServerObj *S_, const gxy_request &r, Flight &f) {
     f.n = nout;
     HIPCHK(launch_fill_f32(f.fb, 4LL * px, 0.f, S()));
   } else {
-    if (r.nrays <= 0 || r.nrays > INT32_MAX / 2 || r.capacity < 0) return fail("sample request of %lld rays", r.nrays);
     f.n = (int)r.nrays;
     f.rays = raylist_new(f.n, 0, 0, false);
     f.xyz = (float *)dalloc((size_t)std::max<long long>(1, r.capacity) * 12);
@@ -3434,8 +3485,201 @@ static int flight_finish(const gxy_request &r, Flight &f, gxy_request_result &re
     if (r.out && k > 0) HIPCHK(hipMemcpyAsync(r.out, f.xyz, (size_t)k * 12, hipMemcpyDeviceToHost, S()));
   }
   HIPCHK(hipStreamSynchronize(S()));
-  res.status = 0;
   return 0;
+}
+
+// Ends an admitted request: its batch list removed, its buffers freed, its
+// status set (under the server lock).
+static void flight_end(ServerObj *S_, size_t k, int status) {
+  Flight f = S_->fl[k];
+  if (f.list >= 0) gxy_batch_remove_list(S_->batch, f.list);
+  flight_free(f);
+  std::lock_guard<std::mutex> lk(S_->mu);
+  ReqRecord &rr = S_->recs[f.req];
+  rr.res.status = status;
+  rr.res.finished = since(S_);
+  S_->fl.erase(S_->fl.begin() + k);
+  S_->idle_cv.notify_all();
+}
+
+// Every admitted request ended as failed (after an error of the batch), so
+// that no list of a failed step is stepped again and no buffer leaks.
+static void serve_abort(ServerObj *S_) {
+  while (!S_->fl.empty()) flight_end(S_, S_->fl.size() - 1, RQ_FAILED);
+}
+
+// Admission: queued requests whose arrival has come, in order, while a slot is
+// free and the device bytes held since serving started plus the request's
+// estimate fit the budget (an estimate over the budget alone was rejected at
+// submission).  A request whose set-up fails is failed alone.
+static void serve_admit(ServerObj *S_) {
+  BatchObj *B = (BatchObj *)S_->batch;
+  for (;;) {
+    int id;
+    {
+      std::lock_guard<std::mutex> lk(S_->mu);
+      if (S_->queue.empty() || (int)S_->fl.size() >= S_->slots) return;
+      id = S_->queue.front();
+      ReqRecord &rr = S_->recs[id];
+      if (rr.req.arrival > since(S_)) return;
+      const long long est = request_bytes(S_, rr.req);
+      if (!S_->fl.empty() && (long long)G.bytes_live - S_->base_bytes + est > S_->budget) return;   // wait for memory
+      S_->queue.pop_front();
+      if (rr.cancel) {
+        rr.res.status = RQ_CANCELLED;
+        rr.res.finished = since(S_);
+        S_->idle_cv.notify_all();
+        continue;
+      }
+      rr.res.status = RQ_RUNNING;
+      rr.res.admitted = since(S_);
+    }
+    const gxy_request r = S_->recs[id].req;
+    Flight f;
+    f.req = id;
+    f.bytes = request_bytes(S_, r);
+    bool ok = flight_make(S_, r, f) == 0;
+    if (ok) {
+      f.list = r.kind == 0 ? gxy_batch_add_list(B, r.dataset, BM_RENDER, f.n, &f.rays->ispc, f.fb, r.width, r.height,
+                                                nullptr, 0, nullptr)
+                           : gxy_batch_add_list(B, r.dataset, BM_SAMPLE, f.n, &f.rays->ispc, nullptr, 0, 0, f.xyz,
+                                                (int)std::min<long long>(r.capacity, INT32_MAX), f.cnt);
+      ok = f.list >= 0;
+    }
+    std::lock_guard<std::mutex> lk(S_->mu);
+    S_->fl.push_back(f);
+    if (!ok) {
+      S_->fl.pop_back();
+      flight_free(f);
+      ReqRecord &rr = S_->recs[id];
+      rr.res.status = RQ_FAILED;
+      rr.res.finished = since(S_);
+      S_->idle_cv.notify_all();
+    }
+    S_->max_in_flight = std::max(S_->max_in_flight, (int)S_->fl.size());
+  }
+}
+
+// Cancellations asked for since the last step: admitted requests leave the
+// batch now (their partial outputs are dropped).
+static void serve_cancel(ServerObj *S_) {
+  for (size_t k = 0; k < S_->fl.size();) {
+    bool c;
+    {
+      std::lock_guard<std::mutex> lk(S_->mu);
+      c = S_->recs[S_->fl[k].req].cancel;
+    }
+    if (c) flight_end(S_, k, RQ_CANCELLED);
+    else k++;
+  }
+}
+
+// One continuous-batching step over the admitted requests, then every request
+// with nothing left in the batch is finished.  Returns -1 on a batch error
+// (every admitted request is then failed and released).
+static int serve_step(ServerObj *S_) {
+  BatchObj *B = (BatchObj *)S_->batch;
+  if (S_->fl.empty()) return 0;
+  long long q = 0;
+  if (gxy_batch_step(B, &q) != 0) {
+    serve_abort(S_);
+    return -1;
+  }
+  S_->peak = std::max(S_->peak, (long long)G.bytes_live - S_->base_bytes);
+  std::vector<int> pend(B->lists.size(), 0);
+  if (gxy_batch_pending(B, pend.data(), (int)pend.size()) != 0) {
+    serve_abort(S_);
+    return -1;
+  }
+  for (size_t k = 0; k < S_->fl.size();) {
+    Flight &f = S_->fl[k];
+    if (pend[f.list] == 0) {
+      gxy_request_result tmp;
+      {
+        std::lock_guard<std::mutex> lk(S_->mu);
+        tmp = S_->recs[f.req].res;
+      }
+      const int rc = flight_finish(S_->recs[f.req].req, f, tmp);
+      {
+        std::lock_guard<std::mutex> lk(S_->mu);
+        gxy_request_result &res = S_->recs[f.req].res;
+        res.rays = tmp.rays;
+        res.particles = tmp.particles;
+      }
+      flight_end(S_, k, rc == 0 ? RQ_DONE : RQ_FAILED);
+    } else {
+      k++;
+    }
+  }
+  return 0;
+}
+
+static int server_ready(ServerObj *S_) {
+  if (!S_->lights) return fail("gxy_server_set_lighting not called");
+  return 0;
+}
+
+// A new request record (status queued, or rejected when it cannot be taken).
+static int record_request(ServerObj *S_, const gxy_request &r, double arrival) {
+  ReqRecord rr;
+  rr.req = r;
+  rr.req.arrival = arrival;
+  memset(&rr.res, 0, sizeof rr.res);
+  rr.res.status = RQ_QUEUED;
+  const int id = (int)S_->recs.size();
+  if (request_invalid(S_, r)) {
+    rr.res.status = RQ_REJECTED;
+    rr.res.admitted = rr.res.finished = arrival;
+    S_->recs.push_back(rr);
+  } else {
+    S_->recs.push_back(rr);
+    S_->queue.push_back(id);
+  }
+  return id;
+}
+
+// The live service: its own execution context (stream), admission, cancel and
+// step until asked to stop; a batch error fails the admitted requests and the
+// ones still queued, and is reported by gxy_server_drain / gxy_server_stop.
+static void service_loop(ServerObj *S_) {
+  void *ctx = gxy_context_create();
+  if (!ctx || gxy_context_bind(ctx) != 0) {
+    std::lock_guard<std::mutex> lk(S_->mu);
+    S_->loop_error = -1;
+    S_->loop_msg = "service thread: no execution context";
+    S_->running = false;
+    S_->idle_cv.notify_all();
+    return;
+  }
+  for (;;) {
+    {
+      std::unique_lock<std::mutex> lk(S_->mu);
+      if (S_->stop && S_->queue.empty() && S_->fl.empty()) break;
+      if (S_->fl.empty()) {
+        // nothing admitted: sleep until a request arrives (or its arrival time)
+        bool wait = S_->queue.empty() || S_->recs[S_->queue.front()].req.arrival > since(S_);
+        if (wait && !S_->stop) S_->cv.wait_for(lk, std::chrono::milliseconds(2));
+      }
+    }
+    serve_cancel(S_);
+    serve_admit(S_);
+    if (serve_step(S_) != 0) {
+      std::lock_guard<std::mutex> lk(S_->mu);
+      S_->loop_error = -1;
+      S_->loop_msg = t_err;
+      for (int id : S_->queue) {
+        S_->recs[id].res.status = RQ_FAILED;
+        S_->recs[id].res.finished = since(S_);
+      }
+      S_->queue.clear();
+      S_->idle_cv.notify_all();
+    }
+  }
+  gxy_context_bind(nullptr);
+  gxy_context_destroy(ctx);
+  std::lock_guard<std::mutex> lk(S_->mu);
+  S_->running = false;
+  S_->idle_cv.notify_all();
 }
 
 }  // namespace gxy
@@ -3452,15 +3696,35 @@ GXY_EXPORT void *gxy_server_create(long long budget, int slots, int chunk) {
   S_->budget = budget;
   S_->slots = slots;
   S_->batch = b;
+  S_->t0 = std::chrono::steady_clock::now();
   std::lock_guard<std::recursive_mutex> lk(G.mu);
   g_servers.insert(S_);
   return S_;
+}
+
+// Stops the service thread: with cancel, every request still queued or
+// admitted is cancelled first; otherwise the thread finishes them.
+static int server_stop(ServerObj *S_, bool cancel) {
+  {
+    std::lock_guard<std::mutex> lk(S_->mu);
+    if (!S_->th.joinable()) return 0;
+    if (cancel)
+      for (auto &rr : S_->recs)
+        if (rr.res.status == RQ_QUEUED || rr.res.status == RQ_RUNNING) rr.cancel = true;
+    S_->stop = true;
+    S_->cv.notify_all();
+  }
+  S_->th.join();
+  std::lock_guard<std::mutex> lk(S_->mu);
+  S_->stop = false;
+  return S_->loop_error;
 }
 
 GXY_EXPORT int gxy_server_destroy(void *s) {
   INIT_OR(-1);
   ServerObj *S_ = lookup<ServerObj>(g_servers, s);
   if (!S_) return fail("not a server handle");
+  server_stop(S_, true);
   {
     std::lock_guard<std::recursive_mutex> lk(G.mu);
     g_servers.erase(S_);
@@ -3484,6 +3748,7 @@ GXY_EXPORT int gxy_server_add_dataset(void *s, const Visualization_ispc *vis, fl
   INIT_OR(-1);
   ServerObj *S_ = lookup<ServerObj>(g_servers, s);
   if (!S_) return fail("not a server handle");
+  if (S_->th.joinable()) return fail("add datasets before gxy_server_start");
   int k = gxy_batch_add_dataset(S_->batch, vis, eps);
   if (k < 0) return -1;
   S_->vis.push_back(vis);
@@ -3495,10 +3760,96 @@ GXY_EXPORT int gxy_server_set_lighting(void *s, const Lighting_ispc *lights, flo
   INIT_OR(-1);
   ServerObj *S_ = lookup<ServerObj>(g_servers, s);
   if (!S_) return fail("not a server handle");
+  if (S_->th.joinable()) return fail("set the lighting before gxy_server_start");
   if (gxy_batch_set_lighting(S_->batch, lights, eps) != 0) return -1;
   S_->lights = lights;
   S_->leps = eps;
   S_->nsp = (int)lights->n_ao_rays + (lights->do_shadows ? (int)lights->nLights : 0);
+  return 0;
+}
+
+GXY_EXPORT int gxy_server_start(void *s) {
+  INIT_OR(-1);
+  ServerObj *S_ = lookup<ServerObj>(g_servers, s);
+  if (!S_) return fail("not a server handle");
+  if (server_ready(S_) != 0) return -1;
+  std::lock_guard<std::mutex> lk(S_->mu);
+  if (S_->th.joinable()) return fail("server already started");
+  HIPCHK(hipDeviceSynchronize());   // datasets uploaded on other streams are complete
+  S_->t0 = std::chrono::steady_clock::now();
+  S_->base_bytes = (long long)G.bytes_live;
+  S_->loop_error = 0;
+  S_->loop_msg.clear();
+  S_->running = true;
+  S_->th = std::thread(service_loop, S_);
+  return 0;
+}
+
+GXY_EXPORT int gxy_server_submit(void *s, const gxy_request *req) {
+  INIT_OR(-1);
+  ServerObj *S_ = lookup<ServerObj>(g_servers, s);
+  if (!S_) return fail("not a server handle");
+  if (!req) return fail("null request");
+  std::lock_guard<std::mutex> lk(S_->mu);
+  if (!S_->th.joinable() || !S_->running) return fail("server not started");
+  const int id = record_request(S_, *req, since(S_));
+  S_->cv.notify_all();
+  return id;
+}
+
+GXY_EXPORT int gxy_server_poll(void *s, int id, gxy_request_result *res) {
+  INIT_OR(-1);
+  ServerObj *S_ = lookup<ServerObj>(g_servers, s);
+  if (!S_) return fail("not a server handle");
+  std::lock_guard<std::mutex> lk(S_->mu);
+  if (id < 0 || id >= (int)S_->recs.size()) return fail("request id %d unknown", id);
+  if (res) *res = S_->recs[id].res;
+  return S_->recs[id].res.status;
+}
+
+GXY_EXPORT int gxy_server_cancel(void *s, int id) {
+  INIT_OR(-1);
+  ServerObj *S_ = lookup<ServerObj>(g_servers, s);
+  if (!S_) return fail("not a server handle");
+  std::lock_guard<std::mutex> lk(S_->mu);
+  if (id < 0 || id >= (int)S_->recs.size()) return fail("request id %d unknown", id);
+  ReqRecord &rr = S_->recs[id];
+  if (rr.res.status != RQ_QUEUED && rr.res.status != RQ_RUNNING) return 0;   // already final
+  rr.cancel = true;
+  S_->cv.notify_all();
+  return 0;
+}
+
+GXY_EXPORT int gxy_server_drain(void *s, double timeout_s) {
+  INIT_OR(-1);
+  ServerObj *S_ = lookup<ServerObj>(g_servers, s);
+  if (!S_) return fail("not a server handle");
+  std::unique_lock<std::mutex> lk(S_->mu);
+  if (!S_->th.joinable()) return fail("server not started");
+  const auto until = std::chrono::steady_clock::now() + std::chrono::duration<double>(timeout_s);
+  const bool drained = S_->idle_cv.wait_until(lk, until, [&] {
+    return (S_->queue.empty() && S_->fl.empty()) || !S_->running;
+  });
+  if (S_->loop_error) return fail("server loop failed: %s", S_->loop_msg.c_str());
+  if (drained) return 0;
+  // the watchdog: cancel what is left and report the timeout
+  int left = (int)S_->queue.size() + (int)S_->fl.size();
+  for (auto &rr : S_->recs)
+    if (rr.res.status == RQ_QUEUED || rr.res.status == RQ_RUNNING) rr.cancel = true;
+  S_->cv.notify_all();
+  const bool emptied = S_->idle_cv.wait_for(lk, std::chrono::seconds(60), [&] {
+    return (S_->queue.empty() && S_->fl.empty()) || !S_->running;
+  });
+  return fail("drain timed out after %.3f s with %d requests left (%s)", timeout_s, left,
+              emptied ? "cancelled" : "cancellation still pending");
+}
+
+GXY_EXPORT int gxy_server_stop(void *s) {
+  INIT_OR(-1);
+  ServerObj *S_ = lookup<ServerObj>(g_servers, s);
+  if (!S_) return fail("not a server handle");
+  const int rc = server_stop(S_, false);
+  if (rc != 0) return fail("server loop failed: %s", S_->loop_msg.c_str());
   return 0;
 }
 
@@ -3507,127 +3858,98 @@ GXY_EXPORT int gxy_server_run(void *s, const gxy_request *reqs, int n, int mode,
   INIT_OR(-1);
   ServerObj *S_ = lookup<ServerObj>(g_servers, s);
   if (!S_) return fail("not a server handle");
-  if (!S_->lights) return fail("gxy_server_set_lighting not called");
+  if (server_ready(S_) != 0) return -1;
+  if (S_->th.joinable()) return fail("the service thread is running (gxy_server_stop first)");
   for (int k = 1; k < n; k++)
     if (reqs[k].arrival < reqs[k - 1].arrival) return fail("requests must be sorted by arrival");
-  for (int k = 0; k < n; k++) memset(&res[k], 0, sizeof res[k]), res[k].status = -1;
   gxy_serve_stats out;
   memset(&out, 0, sizeof out);
-  const auto t0 = std::chrono::steady_clock::now();
-  auto el = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+  S_->recs.clear();
+  S_->queue.clear();
+  S_->t0 = std::chrono::steady_clock::now();
+  S_->base_bytes = (long long)G.bytes_live;
+  S_->peak = 0;
+  S_->max_in_flight = 0;
+  for (int k = 0; k < n; k++) record_request(S_, reqs[k], reqs[k].arrival);
   auto wait_until = [&](double t) {
     for (;;) {
-      double d = t - el();
+      double d = t - since(S_);
       if (d <= 0) return;
       std::this_thread::sleep_for(std::chrono::microseconds(d > 0.002 ? 1000 : 50));
     }
   };
-  const long long base = (long long)G.bytes_live;
-  long long peak = 0;
+  int rc = 0;
   if (mode == 1) {
     // one request at a time through the per-list loop
     const int nb[6] = {-1, -1, -1, -1, -1, -1};
     for (int k = 0; k < n; k++) {
-      const gxy_request &r = reqs[k];
+      ReqRecord &rr = S_->recs[k];
+      if (rr.res.status != RQ_QUEUED) continue;
+      const gxy_request &r = rr.req;
       wait_until(r.arrival);
-      res[k].admitted = el();
+      rr.res.admitted = since(S_);
       Flight f;
-      if (flight_make(S_, r, f) != 0) {
-        flight_free(f);
-        return -1;
-      }
+      int frc = flight_make(S_, r, f);
       gxy_frame_stats fs;
       memset(&fs, 0, sizeof fs);
-      int rc;
-      if (r.kind == 0) {
-        rc = gxy_process_raylist((void *)S_->vis[r.dataset], (void *)S_->lights, f.n, &f.rays->ispc, nb,
-                                 S_->eps[r.dataset], 1 << 30, f.fb, r.width, r.height, &fs);
-      } else {
-        int c = 0;
-        rc = gxy_sample_raylist((void *)S_->vis[r.dataset], f.n, &f.rays->ispc, 1 << 30, f.xyz, (int)r.capacity, &c,
-                                &fs);
-        if (rc == 0) HIPCHK(hipMemcpyAsync(f.cnt, &c, sizeof c, hipMemcpyHostToDevice, S()));
+      if (frc == 0) {
+        if (r.kind == 0) {
+          frc = gxy_process_raylist((void *)S_->vis[r.dataset], (void *)S_->lights, f.n, &f.rays->ispc, nb,
+                                    S_->eps[r.dataset], 1 << 30, f.fb, r.width, r.height, &fs);
+        } else {
+          int c = 0;
+          frc = gxy_sample_raylist((void *)S_->vis[r.dataset], f.n, &f.rays->ispc, 1 << 30, f.xyz, (int)r.capacity,
+                                   &c, &fs);
+          if (frc == 0 && hipMemcpyAsync(f.cnt, &c, sizeof c, hipMemcpyHostToDevice, S()) != hipSuccess) frc = -1;
+        }
       }
-      peak = std::max(peak, (long long)G.bytes_live - base);
-      if (rc != 0 || flight_finish(r, f, res[k]) != 0) {
-        flight_free(f);
-        return -1;
-      }
+      S_->peak = std::max(S_->peak, (long long)G.bytes_live - S_->base_bytes);
+      if (frc == 0) frc = flight_finish(r, f, rr.res);
       flight_free(f);
-      res[k].finished = el();
-      out.rays += fs.ray_passes;
-      out.samples += fs.samples;
-      out.passes += fs.passes;
+      rr.res.status = frc == 0 ? RQ_DONE : RQ_FAILED;
+      rr.res.finished = since(S_);
+      if (frc == 0) {
+        out.rays += fs.ray_passes;
+        out.samples += fs.samples;
+        out.passes += fs.passes;
+      }
     }
-    out.max_in_flight = n > 0 ? 1 : 0;
+    S_->max_in_flight = n > 0 ? 1 : 0;
   } else {
-    // continuous batching over the batch's passes
+    // continuous batching: requests join the batch as they arrive
     BatchObj *B = (BatchObj *)S_->batch;
     gxy_batch_stats b0;
     if (gxy_batch_get_stats(B, &b0) != 0) return -1;
     const int passes0 = B->passes_run;
-    std::vector<Flight> fl;
-    std::vector<int> pend;
-    int next = 0, done = 0;
-    while (done < n) {
-      while (next < n && reqs[next].arrival <= el() && (int)fl.size() < S_->slots) {
-        const gxy_request &r = reqs[next];
-        const long long est = request_bytes(S_, r);
-        if (!fl.empty() && (long long)G.bytes_live - base + est > S_->budget) break;   // wait for memory
-        if (fl.empty() && est > S_->budget) return fail("request %d needs %lld bytes, over the budget", next, est);
-        Flight f;
-        f.req = next;
-        f.bytes = est;
-        if (flight_make(S_, r, f) != 0) {
-          flight_free(f);
-          return -1;
-        }
-        f.list = r.kind == 0 ? gxy_batch_add_list(B, r.dataset, BM_RENDER, f.n, &f.rays->ispc, f.fb, r.width,
-                                                  r.height, nullptr, 0, nullptr)
-                             : gxy_batch_add_list(B, r.dataset, BM_SAMPLE, f.n, &f.rays->ispc, nullptr, 0, 0, f.xyz,
-                                                  (int)std::min<long long>(r.capacity, INT32_MAX), f.cnt);
-        if (f.list < 0) {
-          flight_free(f);
-          return -1;
-        }
-        res[next].admitted = el();
-        fl.push_back(f);
-        next++;
-      }
-      out.max_in_flight = std::max(out.max_in_flight, (int)fl.size());
-      if (fl.empty()) {
-        if (next < n) wait_until(reqs[next].arrival);
+    for (;;) {
+      serve_admit(S_);
+      if (S_->fl.empty()) {
+        if (S_->queue.empty()) break;
+        wait_until(S_->recs[S_->queue.front()].req.arrival);
         continue;
       }
-      long long q = 0;
-      if (gxy_batch_step(B, &q) != 0) return -1;
-      peak = std::max(peak, (long long)G.bytes_live - base);
-      pend.assign(B->lists.size(), 0);
-      if (gxy_batch_pending(B, pend.data(), (int)pend.size()) != 0) return -1;
-      for (size_t k = 0; k < fl.size();) {
-        Flight &f = fl[k];
-        if (pend[f.list] == 0) {
-          if (flight_finish(reqs[f.req], f, res[f.req]) != 0) return -1;
-          res[f.req].finished = el();
-          if (gxy_batch_remove_list(B, f.list) != 0) return -1;
-          flight_free(f);
-          fl.erase(fl.begin() + k);
-          done++;
-        } else {
-          k++;
-        }
+      if (serve_step(S_) != 0) {
+        rc = -1;
+        for (int id : S_->queue) S_->recs[id].res.status = RQ_FAILED;
+        S_->queue.clear();
+        break;
       }
     }
     gxy_batch_stats b1;
-    if (gxy_batch_get_stats(B, &b1) != 0) return -1;
-    out.rays = b1.rays - b0.rays;
-    out.samples = b1.samples - b0.samples;
-    out.passes = B->passes_run - passes0;
+    if (gxy_batch_get_stats(B, &b1) == 0) {
+      out.rays = b1.rays - b0.rays;
+      out.samples = b1.samples - b0.samples;
+      out.passes = B->passes_run - passes0;
+    }
   }
   double first = n > 0 ? reqs[0].arrival : 0, last = 0;
-  for (int k = 0; k < n; k++) last = std::max(last, res[k].finished);
+  for (int k = 0; k < n; k++) {
+    res[k] = S_->recs[k].res;
+    last = std::max(last, res[k].finished);
+  }
   out.wall = last - first;
-  out.peak_bytes = peak;
+  out.peak_bytes = S_->peak;
+  out.max_in_flight = S_->max_in_flight;
   if (st) *st = out;
-  return 0;
+  return rc;
 }

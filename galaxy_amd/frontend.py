@@ -1,157 +1,110 @@
-"""The multiserver viewer's command front end over this library.
+"""The multiserver viewer's command front end (gxy_viewer_* in
+include/gxy_hip.h, galaxy_amd/csrc/viewer.cpp).
 
-`ViewerHandler.handle(line)` takes the command lines of Galaxy's viewer
+`ViewerHandler.handle(line)` passes the command lines of Galaxy's viewer
 handler (ViewerClientServer::handle, multiserver/ViewerClientServer.cpp:63-252)
-and answers with the same replies and error texts:
+to the C++ viewer and returns (handled, reply) with the reference's replies
+and error texts:
 
     permute_pixels [on|off]     ok / error non-default arg ...
-    max_rays_per_packet N       the list size limit of the rank loop
-    commit                      rebuilds the device scene
+    max_rays_per_packet N       kept and reported
+    commit                      the next frame rebuilds the served visualization
     json {...}                  Datasets / Visualization / Camera / Renderer
     window W H                  image size
-    render / renderMany N       one frame (or N) on the GPU
+    render / renderMany N       one frame (or N together) through the live server
 
-Datasets are resolved by name or file name through the `datasets` map the
-handler is given (volumes already described on the host, as Galaxy's
-Datasets::LoadFromJSON would load them); a `json` command holding Datasets
-returns after loading them, as the reference does.  A frame goes through
-gxy_render_frame (camera rays on the device, the per-list loop, accumulation
-into a device framebuffer); the handler keeps the last frame in `frame`
-(width*height*4 floats) and counts frames in `frames`, where the reference
-hands the pixels to its socket handler.  The socket transport itself is not
-part of this library."""
-import json
+Datasets are resolved by file name or name against the volumes the handler is
+given (VolumeDesc on the host, uploaded once when a GPU is present, as
+Galaxy's Datasets::LoadFromJSON would load them).  The last frame is `frame`
+(width*height*4 floats) and `frames` counts frames; the socket transport of
+the multiserver is not part of this library."""
+import ctypes as C
 
-from . import scenes
+import numpy as np
+
+from . import abi, scenes
+from . import lib as glib
 
 
 class ViewerHandler:
-    def __init__(self, backend, datasets=None):
-        self.be = backend
-        self.catalog = dict(datasets or {})   # name or file name -> VolumeDesc
-        self.dsets = {}                        # dataset name -> VolumeDesc (after a Datasets command)
-        self.vis = None                        # Visualization JSON object
-        self.cam = None                        # Camera JSON object
-        self.eps = 0.001
-        self.width, self.height = 512, 512
-        self.permute = False
-        self.maxlist = 1000000
-        self.built = None
-        self.scene = None
-        self.frame = None
-        self.frames = 0
-        self.stats = None
+    def __init__(self, datasets=None, budget=0, slots=0, hip=None):
+        """datasets: key (file name or name) -> scenes.VolumeDesc.  Volumes are
+        uploaded through `hip` (a lib.HipBackend) when given; without it only
+        the names are registered (commands can be answered, frames not)."""
+        self.L = glib.load()
+        self.h = glib.checkp(self.L.gxy_viewer_create(int(budget), int(slots)), "viewer_create")
+        self.keep = []
+        for key, vol in (datasets or {}).items():
+            handle = None
+            if hip is not None and vol is not None:
+                vox = np.ascontiguousarray(vol.voxels, np.float32)
+                handle = hip.volume_create(vol.dims, vol.origin, vol.spacing, vox, vol.sampling_rate)
+                self.keep += [vox, handle]
+            lb = abi.make_box(*(vol.local_box if vol is not None else ((-1,) * 3, (1,) * 3)))
+            gb = abi.make_box(*(vol.global_box if vol is not None else ((-1,) * 3, (1,) * 3)))
+            glib.check(self.L.gxy_viewer_add_volume(self.h, key.encode(), handle, C.byref(lb), C.byref(gb)),
+                       "viewer_add_volume")
 
-    # ------------------------------------------------------------------ commands
+    def destroy(self):
+        if self.h:
+            self.L.gxy_viewer_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
     def handle(self, line):
         """Returns (handled, reply); unknown commands are not handled."""
-        parts = line.strip().split(None, 1)
-        cmd = parts[0] if parts else ""
-        rest = parts[1] if len(parts) > 1 else ""
-        if cmd == "permute_pixels":
-            arg = rest.split()
-            if not arg or arg[0] == "on":
-                self.permute = True
-                return True, "ok"
-            if arg[0] == "off":
-                self.permute = False
-                return True, "ok"
-            return True, "error non-default arg to permute_pixels must be on or off"
-        if cmd == "max_rays_per_packet":
-            try:
-                self.maxlist = int(rest.split()[0])
-            except (IndexError, ValueError):
-                return True, "max_rays_per_packet requires integer arg"
-            return True, "ok"
-        if cmd == "commit":
-            self.commit()
-            return True, "ok"
-        if cmd == "json":
-            return True, self._json(rest)
-        if cmd == "window":
-            try:
-                w, h = (int(x) for x in rest.split()[:2])
-            except ValueError:
-                return True, "error window command needs width and height as integers"
-            self.width, self.height = w, h
-            return True, "ok"
-        if cmd == "render":
-            self.render()
-            return True, "ok"
-        if cmd == "renderMany":
-            try:
-                n = int(rest.split()[0])
-            except (IndexError, ValueError):
-                n = 0
-            for _ in range(n):
-                self.render()
-            return True, "ok"
-        return False, ""
+        buf = C.create_string_buffer(4096)
+        r = self.L.gxy_viewer_handle(self.h, line.encode(), buf, len(buf))
+        if r < 0:
+            raise glib.GxyError("viewer_handle failed")
+        return bool(r), buf.value.decode()
 
-    def _json(self, text):
-        try:
-            doc = json.loads(text)
-        except ValueError:
-            return "error parsing json command"
-        if not isinstance(doc, dict):
-            return "error parsing json command"
-        if "Datasets" in doc:
-            try:
-                for d in doc["Datasets"]:
-                    vol = self.catalog.get(d.get("filename", d["name"]), self.catalog.get(d["name"]))
-                    if vol is None:
-                        raise KeyError(d["name"])
-                    self.dsets[d["name"]] = vol
-            except (KeyError, TypeError):
-                return "error loading datasets from json command"
-            self.built = None
-            return "ok"
-        if "Visualization" in doc:
-            v = doc["Visualization"]
-            try:
-                scenes.vis_from_json(v, self.dsets, self.eps)
-            except (KeyError, TypeError, ValueError):
-                return "error loading visualization from json command"
-            self.vis = v
-            self.built = None
-        if "Camera" in doc:
-            try:
-                scenes.camera_from_json(doc["Camera"], self.width, self.height)
-            except (KeyError, TypeError, ValueError):
-                return "error loading camera from json command"
-            self.cam = doc["Camera"]
-        if "Renderer" in doc:
-            r = doc["Renderer"]
-            if isinstance(r, dict) and "epsilon" in r:
-                self.eps = float(r["epsilon"])
-                self.built = None
-        return "ok"
+    def _settings(self):
+        v = [C.c_int() for _ in range(4)]
+        self.L.gxy_viewer_settings(self.h, *[C.byref(x) for x in v])
+        return [x.value for x in v]
 
-    # ------------------------------------------------------------------ state
-    def commit(self):
-        """Builds the device scene of the current visualization."""
-        if self.vis is None:
-            return
-        self.scene = scenes.vis_from_json(self.vis, self.dsets, self.eps)
-        self.built = None
+    @property
+    def permute(self):
+        return bool(self._settings()[0])
 
-    def render(self):
-        if self.vis is None or self.cam is None:
-            return
-        cam = scenes.camera_from_json(self.cam, self.width, self.height)
-        if self.built is None:
-            self.scene = scenes.vis_from_json(self.vis, self.dsets, self.eps)
-            # camera-relative lights follow the camera of each frame (Rendering.cpp:157-216)
-            self.built = scenes.build_scene(self.be, self.scene, scenes.resolve_lights(self.scene.lighting, cam))
-        else:
-            # the device scene stays; only the lighting is re-resolved for this camera
-            self.built.lights = scenes.make_lighting(scenes.resolve_lights(self.scene.lighting, cam), self.built.keep)
-        self.frame, self.stats = self.be.render_frame(self.built, cam, self.scene.neighbors, self.scene.epsilon,
-                                                      self.maxlist)
-        self.frames += 1
+    @property
+    def maxlist(self):
+        return self._settings()[1]
+
+    @property
+    def width(self):
+        return self._settings()[2]
+
+    @property
+    def height(self):
+        return self._settings()[3]
+
+    @property
+    def frames(self):
+        n = C.c_int()
+        self.L.gxy_viewer_frame(self.h, None, None, None, C.byref(n))
+        return n.value
+
+    @property
+    def frame(self):
+        w, h = C.c_int(), C.c_int()
+        if self.L.gxy_viewer_frame(self.h, None, C.byref(w), C.byref(h), None) != 1:
+            return None
+        out = np.zeros(w.value * h.value * 4, np.float32)
+        self.L.gxy_viewer_frame(self.h, out.ctypes.data, None, None, None)
+        return out
 
     def image(self):
         """The last frame as 8-bit RGBA rows (ImageWriter's conversion)."""
-        if self.frame is None:
+        f = self.frame
+        if f is None:
             return None
-        return scenes.fb_to_rgba8(self.frame, self.width, self.height)
+        w, h = C.c_int(), C.c_int()
+        self.L.gxy_viewer_frame(self.h, None, C.byref(w), C.byref(h), None)
+        return scenes.fb_to_rgba8(f, w.value, h.value)

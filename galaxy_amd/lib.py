@@ -131,6 +131,18 @@ def _declare(L):
         "gxy_server_set_lighting": (C.c_int, [vp, vp, C.c_float]),
         "gxy_server_batch": (vp, [vp]),
         "gxy_server_run": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, vp]),
+        "gxy_server_start": (C.c_int, [vp]),
+        "gxy_server_submit": (C.c_int, [vp, vp]),
+        "gxy_server_poll": (C.c_int, [vp, C.c_int, vp]),
+        "gxy_server_cancel": (C.c_int, [vp, C.c_int]),
+        "gxy_server_drain": (C.c_int, [vp, C.c_double]),
+        "gxy_server_stop": (C.c_int, [vp]),
+        "gxy_viewer_create": (vp, [C.c_longlong, C.c_int]),
+        "gxy_viewer_destroy": (C.c_int, [vp]),
+        "gxy_viewer_add_volume": (C.c_int, [vp, C.c_char_p, vp, vp, vp]),
+        "gxy_viewer_handle": (C.c_int, [vp, C.c_char_p, vp, C.c_size_t]),
+        "gxy_viewer_frame": (C.c_int, [vp, vp, vp, vp, vp]),
+        "gxy_viewer_settings": (C.c_int, [vp, vp, vp, vp, vp]),
         "gxy_outbound_count": (C.c_int, [C.c_int]),
         "gxy_outbound_take": (vp, [C.c_int, vp, vp]),
         "gxy_outbound_clear": (C.c_int, []),

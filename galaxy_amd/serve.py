@@ -53,6 +53,33 @@ class Server:
         """Isovalue crossings a sampler ray follows within one step (1..4096)."""
         glib.check(self.L.gxy_batch_set_max_segments(self.L.gxy_server_batch(self.h), int(n)), "set_max_segments")
 
+    # ---------------------------------------------------------- live service
+    def start(self):
+        glib.check(self.L.gxy_server_start(self.h), "server_start")
+
+    def submit(self, req):
+        """Queues a copy of abi.Request req (its out buffer must outlive it);
+        returns the request id."""
+        return glib.check(self.L.gxy_server_submit(self.h, C.byref(req)), "server_submit")
+
+    def poll(self, rid):
+        """(status, RequestResult): 1 queued, 2 admitted, 0 done, -1 failed,
+        -2 rejected, -3 cancelled."""
+        res = abi.RequestResult()
+        st = self.L.gxy_server_poll(self.h, rid, C.byref(res))
+        return st, res
+
+    def cancel(self, rid):
+        glib.check(self.L.gxy_server_cancel(self.h, rid), "server_cancel")
+
+    def drain(self, timeout_s):
+        """0 when everything finished in time; otherwise the rest is cancelled
+        and the call fails (GxyError)."""
+        return glib.check(self.L.gxy_server_drain(self.h, float(timeout_s)), "server_drain")
+
+    def stop(self):
+        glib.check(self.L.gxy_server_stop(self.h), "server_stop")
+
     def run(self, reqs, mode=abi.SERVE_CONTINUOUS):
         """reqs: a ctypes array of abi.Request sorted by arrival.  Returns
         (results array, ServeStats)."""

@@ -577,7 +577,9 @@ typedef struct gxy_request_result {
   double admitted, finished;   /* seconds after the start of the stream */
   long long particles;         /* sample: isovalue crossings found */
   long long rays;              /* render: primary rays generated; sample: rays */
-  int status;                  /* 0 done, -1 failed */
+  int status;                  /* 0 done, -1 failed, -2 rejected (invalid, or over the HBM
+                                  budget by itself), -3 cancelled; while pending:
+                                  1 queued, 2 admitted */
   int pad_;
 } gxy_request_result;
 
@@ -605,6 +607,54 @@ void *gxy_server_batch(void *server);
  * k's admission and completion times. */
 int   gxy_server_run(void *server, const gxy_request *reqs, int n, int mode, gxy_request_result *res,
                      gxy_serve_stats *stats);
+
+/* The live served form (the request loop of the multiserver viewer,
+ * ViewerClientServer.cpp:63-250, on the GPU): gxy_server_start runs a service
+ * thread on its own stream that admits submitted requests as slots and the
+ * HBM budget allow and steps every admitted request together (continuous
+ * batching), until gxy_server_stop.  gxy_server_submit queues a copy of the
+ * request (its out buffer must stay valid until the request is final; its
+ * arrival is the time of submission) and returns its id; a request that is
+ * invalid or over the budget by itself is rejected at once (status -2) while
+ * the others go on.  gxy_server_poll returns the status (and fills res).
+ * gxy_server_cancel cancels a queued or admitted request (admitted ones leave
+ * the batch at the next step boundary).  gxy_server_drain waits until nothing
+ * is queued or admitted; after timeout_s seconds it cancels what is left and
+ * fails (the queue-drain watchdog).  A batch error fails the admitted and
+ * queued requests, releases their buffers, and is reported by the next
+ * gxy_server_drain or gxy_server_stop.  gxy_server_run needs the service
+ * thread stopped. */
+int   gxy_server_start(void *server);
+int   gxy_server_submit(void *server, const gxy_request *req);
+int   gxy_server_poll(void *server, int id, gxy_request_result *res);
+int   gxy_server_cancel(void *server, int id);
+int   gxy_server_drain(void *server, double timeout_s);
+int   gxy_server_stop(void *server);
+
+/* ==== the viewer's command front end (ViewerClientServer::handle,
+ * multiserver/ViewerClientServer.cpp:63-252) on the live served form ====== */
+/* A viewer takes the command lines of Galaxy's viewer (permute_pixels,
+ * max_rays_per_packet, commit, json, window, render, renderMany N) and writes
+ * the reference's reply ("ok" or its error text) into reply; returns 1 when
+ * the command is one of these, 0 otherwise.  json Datasets name volumes
+ * registered with gxy_viewer_add_volume (by file name, else name); json
+ * Visualization (VolumeVis operators: colormap, opacitymap, data range,
+ * isovalues, slices / plane, volume rendering; Lighting), Camera and Renderer
+ * epsilon are kept; render / renderMany submit render requests to a live
+ * server (gxy_server_start) holding the visualization, rebuilt when it or the
+ * camera-resolved lighting changes, and wait for them.  Creating a viewer and
+ * answering commands need no GPU until a frame is rendered.  permute_pixels
+ * and max_rays_per_packet are kept and reported (the served form's camera
+ * rays are tiled and its lists are stepped by its own step budget). */
+void *gxy_viewer_create(long long hbm_budget, int slots);   /* 0: 64 GiB, 64 slots */
+int   gxy_viewer_destroy(void *viewer);
+int   gxy_viewer_add_volume(void *viewer, const char *key, void *volume, const gxy_box3f *local_box,
+                            const gxy_box3f *global_box);
+int   gxy_viewer_handle(void *viewer, const char *line, char *reply, size_t capacity);
+/* The last frame (width*height*4 floats into out, when not NULL); returns 1
+ * when there is one. */
+int   gxy_viewer_frame(void *viewer, float *out, int *width, int *height, int *frames);
+int   gxy_viewer_settings(void *viewer, int *permute, int *max_rays_per_packet, int *width, int *height);
 
 #ifdef __cplusplus
 }
