@@ -124,16 +124,15 @@ def cpu_baseline(hip, sc, cam, alpha, target_s=15.0):
     olib().oracle_set_threads(threads)
     olib().oracle_set_opacity_threshold(alpha)
     ob = OracleBackend()
-    vv0 = sc.volume_vis[0]
-    vol = vv0.volume
-    if vol.voxels is None:
+    vol = sc.volume_vis[0].volume if sc.volume_vis else None
+    if vol is not None and vol.voxels is None:
         # the same voxels the GPU rendered (generated on the device, copied here)
         hv = hip.volume_create(vol.dims, vol.origin, vol.spacing, None, vol.sampling_rate)
         hip.volume_fill_noise(hv, *vol.noise)
         vox = hip.volume_download(hv, vol.dims)
         hip.L.gxy_volume_destroy(hv)
         vol.voxels = vox
-    Bo = scenes.build_scene(ob, sc)
+    Bo = scenes.build_scene(ob, sc, scenes.resolve_lights(sc.lighting, cam))
     rays = ob.camera_rays(cam, sc.local_box, sc.global_box)
     nb = np.full(6, -1, np.int32)
 
@@ -164,7 +163,8 @@ def cpu_baseline(hip, sc, cam, alpha, target_s=15.0):
             dt += d2
             passes += s2.ray_passes
             samples += s2.samples
-    vol.voxels = None
+    if vol is not None:
+        vol.voxels = None
     return dict(value=round(passes / dt / 1e6, 4), unit="Mrays/s", cores=threads, kind="port",
                 gsamples_per_s=round(samples / dt / 1e9, 5), host_cores=os.cpu_count(), cpu_model=cpu_model(),
                 sample="%d x primary rays of every %d-th image row (%d primary, %d ray passes in all, %.1f s), "
@@ -182,6 +182,44 @@ def cpu_model():
     except OSError:
         pass
     return platform.machine()
+
+
+WALK_COUNTS = os.path.join(ROOT, "tests", "golden", "walk_counts.json")
+
+
+def geometry_roofline(name, sc, tot, steps, trace_ms, trace_launches):
+    """Roofline of a geometry frame's trace launches: algorithmic bytes (104 B
+    per ray pass, the scene once per launch: 12 B per vertex and per
+    triangle) and algorithmic flops (the recorded BVH work per primary,
+    shadow and AO ray of the oracle's binned-SAH tree, 20 flops per child-box
+    test and 51 per triangle test) over the measured trace time."""
+    with open(WALK_COUNTS) as f:
+        wc = json.load(f).get(name)
+    if wc is None or trace_launches == 0 or trace_ms <= 0:
+        return None
+    per = {k: (v["box_tests"] * 20.0 + v["prim_tests"] * 51.0) / v["rays"] for k, v in wc.items() if k != "stride"}
+    L_ = sc.lighting
+    nao, nsh = int(L_.n_ao), (len(L_.types) if L_.shadows else 0)
+    prim, sec = tot["primary"], tot["secondary"]
+    ao = sec * nao // max(1, nao + nsh)
+    sh = sec - ao
+    flops = prim * per["primary"] + ao * per.get("ao", 0.0) + sh * per.get("shadow", 0.0)
+    scene_bytes = sum(12 * len(m.vertices) + 12 * len(m.indices) for m in sc.meshes)
+    launches = trace_launches
+    nbytes = 104.0 * tot["ray_passes"] + scene_bytes * launches
+    t = trace_ms * 1e-3
+    gbs, tfl = nbytes / t / 1e9, flops / t / 1e12
+    if gbs / HBM_PEAK_GBS >= tfl / FP32_PEAK_TFLOPS:
+        roof = dict(bound="hbm", achieved=round(gbs, 2), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(gbs / HBM_PEAK_GBS, 5))
+    else:
+        roof = dict(bound="mfma", achieved=round(tfl, 4), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+                    frac=round(tfl / FP32_PEAK_TFLOPS, 5), note="FP32 VALU work (no MFMA on this path); MI355X FP32 peak")
+    roof.update(traffic=None, hbm_frac=round(gbs / HBM_PEAK_GBS, 5), fp32_frac=round(tfl / FP32_PEAK_TFLOPS, 5),
+                kernel_ms_per_launch=round(trace_ms / launches, 4), alg_bytes_per_launch=int(nbytes / launches),
+                alg_flops_per_launch=int(flops / launches),
+                flops_per_ray={k: round(v, 1) for k, v in per.items()},
+                counts="tests/golden/walk_counts.json (oracle binned-SAH tree, leaves <= 4, nearest-first walk)")
+    return roof
 
 
 def release(L, built):
@@ -300,7 +338,7 @@ def bench_batched(L, hip, steps, tune=True, fixed=None):
     return res
 
 
-def bench_served(L, hip, nreq, seed=5, layout=0, config=None, step_rays=4_000_000):
+def bench_served(L, hip, nreq, seed=5, layout=0, config=None, step_rays=4_000_000, tune=True):
     """Poisson stream (200 requests/s) of render requests (256^2 to 3840x2160)
     and sampler requests (1M-50M rays) over resident 512^3-1024^3 noise volumes
     and 1M-20M triangle blob meshes, admitted under an HBM budget and served
@@ -323,6 +361,8 @@ def bench_served(L, hip, nreq, seed=5, layout=0, config=None, step_rays=4_000_00
     S.set_step_rays(step_rays)
     cands = ([tuple(config)] if config is not None else []) + [(256, 0, 0, 4, 1), (256, 0, 5, 4, 1),
                                                                (128, 0, 5, 4, 1), (64, 0, 5, 4, 1)]
+    if not tune and config is not None:
+        cands = [tuple(config)]
     burst = (abi.Request * 12)()
     for k in range(12):
         burst[k] = reqs[k]
@@ -423,6 +463,8 @@ def main():
                     help="kind,chunk,refill: use this trace launch configuration (implies --no-tune)")
     ap.add_argument("--volume-layout", type=int, default=None,
                     help="voxel layout of the volume: 0 linear, 8 or 4 bricks (implies --no-tune)")
+    ap.add_argument("--geometry-config", default=None,
+                    help="BVH width, traversal stack, waves per SIMD of the mesh frames (skips their tuner), e.g. 2,0,8")
     ap.add_argument("--geometry-stack", type=int, default=None,
                     help="BVH traversal stack of the mesh frames: 0 private, 16/24/32 LDS (no tuning)")
     ap.add_argument("--batch-config", default=None,
@@ -545,7 +587,12 @@ def main():
                 b2 = scenes.build_scene(hip, sc2)
                 build_s = time.perf_counter() - t
                 gtab = []
-                if args.geometry_stack is not None:
+                if args.geometry_config:
+                    w_, s_, v_ = (int(x) for x in args.geometry_config.split(","))
+                    glib.check(L.gxy_set_bvh_width(w_))
+                    glib.check(L.gxy_set_geometry_stack(s_))
+                    glib.check(L.gxy_set_geometry_waves(v_))
+                elif args.geometry_stack is not None:
                     glib.check(L.gxy_set_geometry_stack(args.geometry_stack))
                 elif not args.no_tune:
                     from galaxy_amd import autotune
@@ -564,6 +611,15 @@ def main():
                                    geometry_stack=L.gxy_get_geometry_stack(), bvh_width=L.gxy_get_bvh_width(),
                                    geometry_waves=L.gxy_get_geometry_waves(),
                                    geometry_tune=[[list(c), round(ms, 4)] for c, ms in gtab])
+                roof2 = geometry_roofline(name, sc2, tot2, steps, kt2["trace"][0], kt2["trace"][1])
+                if roof2:
+                    extra[name]["roofline"] = roof2
+                if not args.no_cpu and name in ("sphere_mesh_512", "nine_icospheres_4k_ao16"):
+                    log("cpu baseline", name)
+                    extra[name]["cpu_baseline"] = cpu_baseline(hip, sc2, sc2.cameras[0], a2,
+                                                               target_s=10.0 if name.startswith("sphere") else 15.0)
+                    glib.check(L.gxy_set_opacity_threshold(alpha))
+                release(L, b2)
                 b2 = None
             except Exception as e:  # reported, not hidden
                 extra[name] = dict(error=str(e))
@@ -584,8 +640,11 @@ def main():
         try:
             lay = out.get("batched", {}).get("volume_layout", 0)
             cfg = out.get("batched", {}).get("config")
+            if args.batch_config:
+                cfg = [int(x) for x in args.batch_config.split(",")]
+                lay = cfg[4]
             out["served"] = bench_served(L, hip, args.serve_requests, layout=lay, config=cfg,
-                                         step_rays=args.serve_step_rays)
+                                         step_rays=args.serve_step_rays, tune=not args.batch_config)
         except Exception as e:
             out["served"] = dict(error=str(e))
 

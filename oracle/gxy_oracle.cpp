@@ -669,9 +669,15 @@ inline bool better(float t, int g, int p, const GeoHit &h) {
 // while g_count_walks is set: the algorithmic work per ray of a binned-SAH
 // tree with leaves of <= 4 primitives, walked nearest first.
 bool g_count_walks = false;
+// Occlusion rays (shadow and AO rays of scenes without volumes) end their walk
+// at the first hit, as the GPU frame loops do; off by default (the reference
+// traces every ray to its closest hit, TraceRays.ispc:117).
+bool g_occlusion_any = false;
 std::atomic<long long> g_node_tests(0), g_prim_tests(0), g_walks(0);
 
-void model_trace(const OModel &M, v3 org, v3 dir, float t0, float tfar, GeoHit &h) {
+// any: the walk ends at the first primitive hit (an occlusion query: a shadow
+// or AO ray in a scene without volumes needs only whether it is blocked).
+void model_trace(const OModel &M, v3 org, v3 dir, float t0, float tfar, GeoHit &h, bool any = false) {
   h.t = tfar;
   h.geom = -1;
   h.prim = -1;
@@ -735,6 +741,7 @@ void model_trace(const OModel &M, v3 org, v3 dir, float t0, float tfar, GeoHit &
       if (nd.left < 0) {
         for (int i = nd.start; i < nd.start + nd.count; i++) test(B.order[i]);
         prim_tests += nd.count;
+        if (any && h.geom >= 0) return;
       } else {
         const float ea = enter(B.nodes[nd.left]), eb = enter(B.nodes[nd.left + 1]);
         const bool near_a = ea <= eb;
@@ -762,8 +769,8 @@ void model_trace(const OModel &M, v3 org, v3 dir, float t0, float tfar, GeoHit &
     }
   };
   walk(M.tbvh, (int)M.tris.size(), test_tri);
-  walk(M.sbvh, (int)M.sphs.size(), test_sph);
-  walk(M.cbvh, (int)M.curs.size(), test_cur);
+  if (!(any && h.geom >= 0)) walk(M.sbvh, (int)M.sphs.size(), test_sph);
+  if (!(any && h.geom >= 0)) walk(M.cbvh, (int)M.curs.size(), test_cur);
   if (g_count_walks) {
     g_node_tests.fetch_add(node_tests, std::memory_order_relaxed);
     g_prim_tests.fetch_add(prim_tests, std::memory_order_relaxed);
@@ -887,7 +894,7 @@ void trace_one(const Visualization_ispc &vis, const std::vector<OVolumeVis> &vv,
   // LookForGeometryHit (TraceRays.ispc:108-141)
   if (model) {
     GeoHit gh;
-    model_trace(*model, org, dir, ray_t0, ray_t, gh);
+    model_trace(*model, org, dir, ray_t0, ray_t, gh, g_occlusion_any && !shadeFlag && vv.empty());
     if (gh.geom >= 0) {
       ray_t = gh.t;
       surface_hit = true;
@@ -1185,6 +1192,7 @@ OEXPORT void oracle_walk_counters(int on, long long *counts) {
   g_count_walks = on != 0;
 }
 OEXPORT void oracle_set_reverse_lighting(int on) { g_reverse = on ? 1 : 0; }
+OEXPORT void oracle_set_occlusion_any(int on) { g_occlusion_any = on != 0; }
 OEXPORT int oracle_get_threads(void) { return nthreads(); }
 
 OEXPORT void oracle_uv_tables(float *U, float *V) {

@@ -94,6 +94,7 @@ def lib():
         L.oracle_plant_fault.argtypes = [C.c_int]
         L.oracle_set_reverse_lighting.argtypes = [C.c_int]
         L.oracle_walk_counters.argtypes = [C.c_int, vp]
+        L.oracle_set_occlusion_any.argtypes = [C.c_int]
         L.oracle_get_threads.restype = C.c_int
         _lib = L
     return _lib

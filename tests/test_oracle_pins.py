@@ -354,3 +354,20 @@ def test_factor_and_partition():
     assert parts[0]["offsets"] == (1, 1, 1) and parts[0]["counts"] == (5, 5, 5)
     assert parts[7]["offsets"] == (5, 5, 5) and parts[7]["counts"] == (4, 4, 4)
     assert parts[7]["gcounts"] == (6, 6, 6)
+
+
+def test_walk_counts_fixture_reproduces():
+    """The recorded BVH work per ray of the sphere-mesh frame (the geometry
+    roofline's algorithmic flops) is what the oracle's walk counts today."""
+    import sys
+    sys.path.insert(0, os.path.join(GOLDEN))
+    import make_walk_counts as mw
+    ref = json.load(open(os.path.join(GOLDEN, "walk_counts.json")))
+    make, stride = mw.scenes_to_count()["sphere_mesh_512"]
+    assert mw.walk_counts(make(), stride) == ref["sphere_mesh_512"]
+    # every recorded ray kind did some work: a box test per walk at least, no more primitive tests than boxes
+    for scene in ref.values():
+        for kind in ("primary", "shadow", "ao"):
+            if kind in scene:
+                c = scene[kind]
+                assert c["rays"] > 0 and c["box_tests"] >= c["rays"] and c["prim_tests"] <= c["box_tests"]
