@@ -10,9 +10,12 @@
 // 3. Karras topology (HPG 2012): one work-item per inner node finds its key
 //    range and split with the delta() prefix length (equal codes broken by
 //    index);
-// 4. bottom-up refit with an arrival counter per node: child boxes, primitive
-//    counts, heights, and the SAH cost of every subtree both as built and with
-//    the cheaper of leaf or node at each subtree that fits a leaf;
+// 4. bottom-up refit, one launch per depth (depths by pointer jumping over the
+//    parent links, then a counting sort of the inner nodes by depth): child
+//    boxes, primitive counts, heights, and the SAH cost of every subtree both
+//    as built and with the cheaper of leaf or node at each subtree that fits a
+//    leaf.  Kernel boundaries order the levels, so no work-item waits on
+//    another's stores;
 // 5. treelet restructuring (Karras and Aila, "Fast Parallel Construction of
 //    High-Quality Bounding Volume Hierarchies", HPG 2013): every node with at
 //    least gamma primitives below it roots a treelet of 7 leaves, grown from its
@@ -289,61 +292,135 @@ __device__ __forceinline__ void child_totals(const BuildArrays &A, int code, con
   }
 }
 
-// One work-item per primitive climbs from its leaf; the second arrival at a
-// node finishes it (the first returns), so every node is finished once, after
-// both its subtrees.
-__global__ __launch_bounds__(GXY_BLOCK) void k_refit(const float *boxes, const int *order, int n, BVHNode *nodes,
-                                                     const int *parent, int *flags, BuildArrays A) {
-  const int k = blockIdx.x * GXY_BLOCK + threadIdx.x;
-  if (k >= n) return;
-  const int ni = n - 1;
-  const float *pb = boxes + 6 * (int64_t)order[k];
-  float lo[3] = {pb[0], pb[1], pb[2]}, hi[3] = {pb[3], pb[4], pb[5]};
-  int code = leaf_code(k);
-  int p = parent[ni + k];
-  // a climb is at most the tree's height; the bound only guards a malformed tree
-  for (int climb = 0; p >= 0 && climb < 4 * GXY_BVH_MAXH; climb++) {
-    BVHNode *nd = nodes + p;
-    const int slot = ldi(&nd->c0) == code ? 0 : 1;
-    float *dl = slot ? nd->lo1 : nd->lo0, *dh = slot ? nd->hi1 : nd->hi0;
-    for (int a = 0; a < 3; a++) { stf(dl + a, lo[a]); stf(dh + a, hi[a]); }
-    __threadfence();
-    if (atomicAdd(flags + p, 1) == 0) return;   // the sibling subtree finishes this node
-    __threadfence();
-    float l0[3], h0[3], l1[3], h1[3];
-    slot_box(nd, 0, l0, h0);
-    slot_box(nd, 1, l1, h1);
-    int n0, n1, g0, g1;
-    float s0, s1, q0, q1;
-    child_totals(A, ldi(&nd->c0), l0, h0, n0, s0, q0, g0);
-    child_totals(A, ldi(&nd->c1), l1, h1, n1, s1, q1, g1);
-    for (int a = 0; a < 3; a++) { lo[a] = fminf(l0[a], l1[a]); hi[a] = fmaxf(h0[a], h1[a]); }
-    const float ar = half_area(lo, hi);
-    const int c = n0 + n1;
-    const float cost = BVH_CI * ar + s0 + s1;
-    const float inner = BVH_CI * ar + q0 + q1, asleaf = A.ct * ar * (float)c;
-    const bool lf = p != 0 && c <= GXY_BVH_LEAF && asleaf <= inner;
-    sti(A.cnt + p, c);
-    stf(A.cost + p, cost);
-    stf(A.costc + p, lf ? asleaf : inner);
-    sti(A.leaf + p, lf ? 1 : 0);
-    sti(A.height + p, max(g0, g1) + 1);
-    code = p;
-    p = parent[p];
+// Depth of every inner node by pointer jumping over the parent links: after
+// round r, anc is the ancestor 2^r levels up (-1 past the root) and dep the
+// distance to it (the full depth once anc is -1).  Eight rounds give exact
+// depths below 256; deeper nodes keep a depth >= 256.
+__global__ __launch_bounds__(GXY_BLOCK) void k_depth_init(const int *parent, int ni, int *anc, int *dep) {
+  const int i = blockIdx.x * GXY_BLOCK + threadIdx.x;
+  if (i >= ni) return;
+  const int a = parent[i];
+  anc[i] = a;
+  dep[i] = a >= 0 ? 1 : 0;
+}
+__global__ __launch_bounds__(GXY_BLOCK) void k_depth_jump(const int *anc, const int *dep, int ni, int *anc2,
+                                                          int *dep2) {
+  const int i = blockIdx.x * GXY_BLOCK + threadIdx.x;
+  if (i >= ni) return;
+  const int a = anc[i];
+  dep2[i] = a >= 0 ? dep[i] + dep[a] : dep[i];
+  anc2[i] = a >= 0 ? anc[a] : -1;
+}
+
+// Counting sort of node indices by a small key (depth or height, clamped to
+// GXY_BVH_MAXH - 1).  Equal keys are common within a wave (neighbouring nodes
+// sit at the same height), so a wave adds each distinct key once: the lowest
+// active lane's key, all lanes holding it, one atomic.  k_level_hist counts;
+// k_level_scatter reserves each key's range for its LEVEL_TILE nodes with one
+// global atomic per key and places them (order within a key is arbitrary).
+#define LEVEL_TILE (GXY_BLOCK * 8)
+__device__ __forceinline__ int level_key(const int *key, int64_t i, int n) {
+  return i < n ? min(max(key[i], 0), GXY_BVH_MAXH - 1) : -1;
+}
+__global__ __launch_bounds__(GXY_BLOCK) void k_level_hist(const int *key, int n, int *hist) {
+  __shared__ int h[GXY_BVH_MAXH];
+  for (int t = threadIdx.x; t < GXY_BVH_MAXH; t += GXY_BLOCK) h[t] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * LEVEL_TILE;
+  for (int r = 0; r < LEVEL_TILE / GXY_BLOCK; r++) {
+    const int k = level_key(key, base + (int64_t)r * GXY_BLOCK + threadIdx.x, n);
+    unsigned long long act = __ballot(k >= 0);
+    while (act) {
+      const int kk = __shfl(k, __ffsll((long long)act) - 1, 64);
+      const unsigned long long same = __ballot(k == kk) & act;
+      if (k == kk && __popcll(same & rs_lanemask_lt()) == 0) atomicAdd(&h[kk], __popcll(same));
+      act &= ~same;
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < GXY_BVH_MAXH; t += GXY_BLOCK)
+    if (h[t]) atomicAdd(hist + t, h[t]);
+}
+__global__ __launch_bounds__(GXY_BLOCK) void k_level_scatter(const int *key, int n, int *cursor, int *list) {
+  __shared__ int h[GXY_BVH_MAXH];
+  for (int t = threadIdx.x; t < GXY_BVH_MAXH; t += GXY_BLOCK) h[t] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * LEVEL_TILE;
+  for (int r = 0; r < LEVEL_TILE / GXY_BLOCK; r++) {
+    const int k = level_key(key, base + (int64_t)r * GXY_BLOCK + threadIdx.x, n);
+    unsigned long long act = __ballot(k >= 0);
+    while (act) {
+      const int kk = __shfl(k, __ffsll((long long)act) - 1, 64);
+      const unsigned long long same = __ballot(k == kk) & act;
+      if (k == kk && __popcll(same & rs_lanemask_lt()) == 0) atomicAdd(&h[kk], __popcll(same));
+      act &= ~same;
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < GXY_BVH_MAXH; t += GXY_BLOCK)
+    if (h[t]) h[t] = atomicAdd(cursor + t, h[t]);   // this tile's first position for key t
+  __syncthreads();
+  for (int r = 0; r < LEVEL_TILE / GXY_BLOCK; r++) {
+    const int64_t i = base + (int64_t)r * GXY_BLOCK + threadIdx.x;
+    const int k = level_key(key, i, n);
+    unsigned long long act = __ballot(k >= 0);
+    while (act) {
+      const int kk = __shfl(k, __ffsll((long long)act) - 1, 64);
+      const unsigned long long same = __ballot(k == kk) & act;
+      int first = 0;
+      if (k == kk && __popcll(same & rs_lanemask_lt()) == 0) first = atomicAdd(&h[kk], __popcll(same));
+      first = __shfl(first, __ffsll((long long)same) - 1, 64);
+      if (k == kk) list[first + __popcll(same & rs_lanemask_lt())] = (int)i;
+      act &= ~same;
+    }
   }
 }
 
-// ------------------------------------------------------------------ treelets
-// Node lists by height: a counting sort of the inner nodes.
-__global__ __launch_bounds__(GXY_BLOCK) void k_height_hist(const int *height, int ni, int *hh) {
+// Refit of the inner nodes of one depth (every deeper node done by earlier
+// launches): the children's boxes go into this node's slots, and the totals
+// of its subtree are formed from the children's.
+__global__ __launch_bounds__(GXY_BLOCK) void k_refit_level(const float *boxes, const int *order, BVHNode *nodes,
+                                                           const int *list, int count, BuildArrays A) {
   const int i = blockIdx.x * GXY_BLOCK + threadIdx.x;
-  if (i < ni) atomicAdd(hh + min(height[i], GXY_BVH_MAXH - 1), 1);
-}
-__global__ __launch_bounds__(GXY_BLOCK) void k_height_scatter(const int *height, int ni, int *pos, int *list) {
-  const int i = blockIdx.x * GXY_BLOCK + threadIdx.x;
-  if (i < ni) list[atomicAdd(pos + min(height[i], GXY_BVH_MAXH - 1), 1)] = i;
+  if (i >= count) return;
+  const int p = list[i];
+  BVHNode *nd = nodes + p;
+  float l[2][3], h[2][3], s[2], q[2];
+  int c[2], g[2];
+  for (int slot = 0; slot < 2; slot++) {
+    const int code = slot ? nd->c1 : nd->c0;
+    if (code >= 0) {
+      const BVHNode *ch = nodes + code;
+      for (int a = 0; a < 3; a++) {
+        l[slot][a] = fminf(ch->lo0[a], ch->lo1[a]);
+        h[slot][a] = fmaxf(ch->hi0[a], ch->hi1[a]);
+      }
+    } else {
+      const float *pb = boxes + 6 * (int64_t)order[(~code) >> 4];
+      for (int a = 0; a < 3; a++) { l[slot][a] = pb[a]; h[slot][a] = pb[3 + a]; }
+    }
+    child_totals(A, code, l[slot], h[slot], c[slot], s[slot], q[slot], g[slot]);
+  }
+  float lo[3], hi[3];
+  for (int a = 0; a < 3; a++) {
+    nd->lo0[a] = l[0][a]; nd->hi0[a] = h[0][a];
+    nd->lo1[a] = l[1][a]; nd->hi1[a] = h[1][a];
+    lo[a] = fminf(l[0][a], l[1][a]);
+    hi[a] = fmaxf(h[0][a], h[1][a]);
+  }
+  const float ar = half_area(lo, hi);
+  const int cc = c[0] + c[1];
+  const float cost = BVH_CI * ar + s[0] + s[1];
+  const float inner = BVH_CI * ar + q[0] + q[1], asleaf = A.ct * ar * (float)cc;
+  const bool lf = p != 0 && cc <= GXY_BVH_LEAF && asleaf <= inner;
+  A.cnt[p] = cc;
+  A.cost[p] = cost;
+  A.costc[p] = lf ? asleaf : inner;
+  A.leaf[p] = lf ? 1 : 0;
+  A.height[p] = max(g[0], g[1]) + 1;
 }
 
+// ------------------------------------------------------------------ treelets
 #define TL 7                 // treelet leaves
 #define TLS (1 << TL)        // subsets
 #define TB 64                // work-items per workgroup of the treelet kernel
@@ -764,6 +841,43 @@ static hipError_t read_ints(const int *dev, int *host, int n, hipStream_t s) {
   return hipStreamSynchronize(s);
 }
 
+// Node indices sorted by key (see k_level_hist): cnt[k] and start[k] of each
+// key on the host, the list on the device.  hh holds 2 * GXY_BVH_MAXH ints.
+static hipError_t sort_by_level(const int *key, int n, int *hh, int *cnt, int *start, int *list, hipStream_t s) {
+  const dim3 grid((n + LEVEL_TILE - 1) / LEVEL_TILE);
+  BVH_CHK(hipMemsetAsync(hh, 0, sizeof(int) * GXY_BVH_MAXH, s));
+  hipLaunchKernelGGL(k_level_hist, grid, dim3(GXY_BLOCK), 0, s, key, n, hh);
+  BVH_CHK(read_ints(hh, cnt, GXY_BVH_MAXH, s));
+  start[0] = 0;
+  for (int k = 0; k < GXY_BVH_MAXH; k++) start[k + 1] = start[k] + cnt[k];
+  if (cnt[GXY_BVH_MAXH - 1] > 0) return hipErrorInvalidValue;   // deeper than the level table
+  BVH_CHK(hipMemcpyAsync(hh + GXY_BVH_MAXH, start, sizeof(int) * GXY_BVH_MAXH, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_level_scatter, grid, dim3(GXY_BLOCK), 0, s, key, n, hh + GXY_BVH_MAXH, list);
+  return hipGetLastError();
+}
+
+// Bottom-up refit of the ni inner nodes, one launch per depth, deepest
+// first; t0..t3 are ni ints each of scratch, list ni ints.
+static hipError_t refit_levels(const float *boxes, const int *order, int ni, BVHNode *nodes, const int *parent,
+                               const BuildArrays &A, int *t0, int *t1, int *t2, int *t3, int *hh, int *list,
+                               hipStream_t s) {
+  const dim3 grid((ni + GXY_BLOCK - 1) / GXY_BLOCK);
+  hipLaunchKernelGGL(k_depth_init, grid, dim3(GXY_BLOCK), 0, s, parent, ni, t0, t1);
+  int *anc = t0, *dep = t1, *anc2 = t2, *dep2 = t3;
+  for (int r = 0; r < 8; r++) {
+    hipLaunchKernelGGL(k_depth_jump, grid, dim3(GXY_BLOCK), 0, s, (const int *)anc, (const int *)dep, ni, anc2, dep2);
+    int *t = anc; anc = anc2; anc2 = t;
+    t = dep; dep = dep2; dep2 = t;
+  }
+  int cnt[GXY_BVH_MAXH], start[GXY_BVH_MAXH + 1];
+  BVH_CHK(sort_by_level(dep, ni, hh, cnt, start, list, s));
+  for (int d = GXY_BVH_MAXH - 2; d >= 0; d--)
+    if (cnt[d] > 0)
+      hipLaunchKernelGGL(k_refit_level, dim3((cnt[d] + GXY_BLOCK - 1) / GXY_BLOCK), dim3(GXY_BLOCK), 0, s, boxes,
+                         order, nodes, (const int *)(list + start[d]), cnt[d], A);
+  return hipGetLastError();
+}
+
 hipError_t build_lbvh(const float *boxes_dev, int n, float prim_cost, alloc_fn_t alloc, free_fn_t dfree,
                       BVHBuildResult *out, hipStream_t s) {
   memset(out, 0, sizeof *out);
@@ -829,7 +943,8 @@ hipError_t build_lbvh(const float *boxes_dev, int n, float prim_cost, alloc_fn_t
   // 3: topology
   BVHNode *nodes = (BVHNode *)P.get(sizeof(BVHNode) * (size_t)ni);
   int *parent = (int *)P.get(sizeof(int) * (size_t)(ni + n));
-  int *flags = (int *)P.get(sizeof(int) * (size_t)ni);
+  int *t0 = (int *)P.get(sizeof(int) * (size_t)ni), *t1 = (int *)P.get(sizeof(int) * (size_t)ni);
+  int *t2 = (int *)P.get(sizeof(int) * (size_t)ni), *t3 = (int *)P.get(sizeof(int) * (size_t)ni);
   BuildArrays A;
   A.ct = prim_cost;
   A.cnt = (int *)P.get(sizeof(int) * (size_t)ni);
@@ -839,7 +954,7 @@ hipError_t build_lbvh(const float *boxes_dev, int n, float prim_cost, alloc_fn_t
   A.height = (int *)P.get(sizeof(int) * (size_t)ni);
   int *hh = (int *)P.get(sizeof(int) * 2 * GXY_BVH_MAXH);
   int *hlist = (int *)P.get(sizeof(int) * (size_t)ni);
-  if (!nodes || !parent || !flags || !A.cnt || !A.cost || !A.costc || !A.leaf || !A.height || !hh || !hlist)
+  if (!nodes || !parent || !t0 || !t1 || !t2 || !t3 || !A.cnt || !A.cost || !A.costc || !A.leaf || !A.height || !hh || !hlist)
     return hipErrorOutOfMemory;
   hipLaunchKernelGGL(k_karras, dim3((ni + GXY_BLOCK - 1) / GXY_BLOCK), dim3(GXY_BLOCK), 0, s, (const uint64_t *)keys,
                      n, nodes, parent);
@@ -848,24 +963,12 @@ hipError_t build_lbvh(const float *boxes_dev, int n, float prim_cost, alloc_fn_t
   // 4-5: refit, then treelet rounds (each followed by a refit for heights and totals)
   int maxh = 0;
   for (int round = 0; round <= 3; round++) {
-    BVH_CHK(hipMemsetAsync(flags, 0, sizeof(int) * (size_t)ni, s));
-    hipLaunchKernelGGL(k_refit, dim3(nb), dim3(GXY_BLOCK), 0, s, boxes_dev, (const int *)order, n, nodes,
-                       (const int *)parent, flags, A);
-    BVH_CHK(hipGetLastError());
+    BVH_CHK(refit_levels(boxes_dev, (const int *)order, ni, nodes, (const int *)parent, A, t0, t1, t2, t3, hh, hlist,
+                         s));
     if (round == 3 || ni < TL) break;
     // inner nodes by height
-    int hcount[GXY_BVH_MAXH];
-    BVH_CHK(hipMemsetAsync(hh, 0, sizeof(int) * GXY_BVH_MAXH, s));
-    hipLaunchKernelGGL(k_height_hist, dim3((ni + GXY_BLOCK - 1) / GXY_BLOCK), dim3(GXY_BLOCK), 0, s,
-                       (const int *)A.height, ni, hh);
-    BVH_CHK(read_ints(hh, hcount, GXY_BVH_MAXH, s));
-    int hstart[GXY_BVH_MAXH + 1];
-    hstart[0] = 0;
-    for (int h = 0; h < GXY_BVH_MAXH; h++) hstart[h + 1] = hstart[h] + hcount[h];
-    if (hcount[GXY_BVH_MAXH - 1] > 0) return hipErrorInvalidValue;   // deeper than the height table
-    BVH_CHK(hipMemcpyAsync(hh + GXY_BVH_MAXH, hstart, sizeof(int) * GXY_BVH_MAXH, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_height_scatter, dim3((ni + GXY_BLOCK - 1) / GXY_BLOCK), dim3(GXY_BLOCK), 0, s,
-                       (const int *)A.height, ni, hh + GXY_BVH_MAXH, hlist);
+    int hcount[GXY_BVH_MAXH], hstart[GXY_BVH_MAXH + 1];
+    BVH_CHK(sort_by_level((const int *)A.height, ni, hh, hcount, hstart, hlist, s));
     const int gamma = TL << round;
     for (int h = 1; h < GXY_BVH_MAXH; h++) {
       if (hcount[h] == 0) continue;
@@ -905,7 +1008,10 @@ hipError_t build_lbvh(const float *boxes_dev, int n, float prim_cost, alloc_fn_t
   }
   P.release(nodes);
   P.release(parent);
-  P.release(flags);
+  P.release(t0);
+  P.release(t1);
+  P.release(t2);
+  P.release(t3);
   // 7: 4-wide collapse
   BVH4Node *w4 = (BVH4Node *)P.get(sizeof(BVH4Node) * (size_t)nn);
   WideEnt *wa = (WideEnt *)P.get(sizeof(WideEnt) * (size_t)nn), *wb = (WideEnt *)P.get(sizeof(WideEnt) * (size_t)nn);
