@@ -276,9 +276,9 @@ def bench_batched(L, hip, steps, tune=True, fixed=None):
     elif tune:
         best, table = gb.tune(B, max_depth=0, graph_passes=2)   # a stack too short for a model is skipped
         res["tuned"] = dict(best=list(best), ms=[[list(c), round(m, 2)] for c, m in table])
-        # voxel layout of every volume dataset: linear or 8^3 bricks (datasets re-registered)
+        # voxel layout of every volume dataset: linear, 4^3 or 8^3 bricks (datasets re-registered)
         lay_ms = {}
-        for lay in (0, 8):
+        for lay in (0, 4, 8):
             for v in vols:
                 glib.check(L.gxy_volume_set_layout(v, lay))
             B.refresh_datasets(built)
@@ -360,7 +360,9 @@ def bench_served(L, hip, nreq, seed=5, layout=0, config=None, step_rays=4_000_00
     # continuous batching steps take at most step_rays primary rays of admitted requests, fewest left first
     S.set_step_rays(step_rays)
     cands = ([tuple(config)] if config is not None else []) + [(256, 0, 0, 4, 1), (256, 0, 5, 4, 1),
-                                                               (128, 0, 5, 4, 1), (64, 0, 5, 4, 1)]
+                                                               (128, 0, 5, 4, 1), (64, 0, 5, 4, 1),
+                                                               (64, 0, 5, 8, 1), (64, 0, 6, 8, 1),
+                                                               (256, 0, 6, 8, 1)]
     if not tune and config is not None:
         cands = [tuple(config)]
     burst = (abi.Request * 12)()
@@ -374,11 +376,20 @@ def bench_served(L, hip, nreq, seed=5, layout=0, config=None, step_rays=4_000_00
         tune_ms.append((c, stb.wall * 1e3))
     best = min(tune_ms, key=lambda x: x[1])[0]
     S.set_config(*best)
+    # isovalue crossings a sampler ray follows per pass (the rest re-queued): shorter tails, more passes
+    seg_ms = {}
+    for nseg in (4096, 16, 4) if tune else ():
+        S.set_max_segments(nseg)
+        _, stb = S.run(burst, abi.SERVE_CONTINUOUS)
+        seg_ms[nseg] = stb.wall * 1e3
+    max_segments = min(seg_ms, key=seg_ms.get) if seg_ms else 4096
+    S.set_max_segments(max_segments)
     nrender = sum(1 for r in reqs if r.kind == 0)
     res = dict(requests=nreq, render=nrender, sample=nreq - nrender, arrival_rate=200.0,
                arrival_span_s=round(reqs[nreq - 1].arrival, 3), budget_bytes=160 << 30, build_s=round(build_s, 2),
                datasets=["%s %d" % (k, s) for k, s, _ in descs], volume_layout=layout, config=list(best),
-               config_ms=[[list(c), round(m, 1)] for c, m in tune_ms], step_rays=step_rays)
+               config_ms=[[list(c), round(m, 1)] for c, m in tune_ms], step_rays=step_rays,
+               max_segments=max_segments, max_segments_ms={str(k): round(v, 1) for k, v in seg_ms.items()})
     # warm-up: one request of each kind
     w = (abi.Request * 2)()
     w[0] = serve.render_request(0, 256, 256, (0.5, 0.5, 2.4))
@@ -469,6 +480,9 @@ def main():
                     help="BVH traversal stack of the mesh frames: 0 private, 16/24/32 LDS (no tuning)")
     ap.add_argument("--batch-config", default=None,
                     help="chunk,stack,wpe,blocks_per_cu,layout of the batched and served forms (no tuning)")
+    ap.add_argument("--serve-config", default=None,
+                    help="chunk,stack,wpe,blocks_per_cu,split of the served form (no tuning); its voxel layout is "
+                         "the last field of --batch-config")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device-to-device copy (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -641,10 +655,11 @@ def main():
             lay = out.get("batched", {}).get("volume_layout", 0)
             cfg = out.get("batched", {}).get("config")
             if args.batch_config:
-                cfg = [int(x) for x in args.batch_config.split(",")]
-                lay = cfg[4]
+                lay = int(args.batch_config.split(",")[4])
+            if args.serve_config:
+                cfg = [int(x) for x in args.serve_config.split(",")]
             out["served"] = bench_served(L, hip, args.serve_requests, layout=lay, config=cfg,
-                                         step_rays=args.serve_step_rays, tune=not args.batch_config)
+                                         step_rays=args.serve_step_rays, tune=not args.serve_config)
         except Exception as e:
             out["served"] = dict(error=str(e))
 

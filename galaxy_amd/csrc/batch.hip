@@ -91,6 +91,54 @@ __device__ __forceinline__ void block_partials(unsigned long long *v, int k0, in
   }
 }
 
+// voxel addressing of a dataset's volume: bricked (8 or 4), small linear (1), large linear (0)
+__device__ __forceinline__ int chunk_layout(const TraceParams &P) {
+  const int b = P.vv[0].vol.brick;
+  return b == 8 ? 8 : (b == 4 ? 4 : (P.vv[0].vol.small ? 1 : 0));
+}
+
+// The render rays of one volume chunk by the chunk's layout, one ray per lane at a time.
+template <int SM>
+__device__ __forceinline__ unsigned long long render_chunk(const TraceParams &P, const RayList_ispc &R,
+                                                           const BatchChunk &ch, const float (*col)[3 * GXY_TF_MAX],
+                                                           const float (*op)[GXY_TF_MAX],
+                                                           PrivStack<GXY_BVH_STACK> &pstack) {
+  unsigned long long n = 0;
+  for (int k = threadIdx.x & 63; k < ch.count; k += 64)
+    n += trace_one<1, false, PrivStack<GXY_BVH_STACK>, SM>(P, R, ch.start + k, col, op, pstack);
+  return n;
+}
+__device__ __forceinline__ unsigned long long render_chunk_any(const TraceParams &P, const RayList_ispc &R,
+                                                               const BatchChunk &ch, int lay,
+                                                               const float (*col)[3 * GXY_TF_MAX],
+                                                               const float (*op)[GXY_TF_MAX],
+                                                               PrivStack<GXY_BVH_STACK> &pstack) {
+  if (lay == 8) return render_chunk<8>(P, R, ch, col, op, pstack);
+  if (lay == 4) return render_chunk<4>(P, R, ch, col, op, pstack);
+  if (lay == 1) return render_chunk<1>(P, R, ch, col, op, pstack);
+  return render_chunk<0>(P, R, ch, col, op, pstack);
+}
+
+// The sampler rays of one chunk (samples and ray passes into acc).
+template <int SM>
+__device__ __forceinline__ void sample_chunk_as(const BatchArgs &A, const TraceParams &P, const RayList_ispc &R,
+                                                const BatchChunk &ch, float *xyz, int *pc, int cap,
+                                                unsigned long long *acc) {
+  unsigned segs = 0;
+  for (int k = threadIdx.x & 63; k < ch.count; k += 64) {
+    acc[BS_SAMPLES] += sample_all<1, SM>(P, R, ch.start + k, xyz, pc, cap, A.max_segments, segs);
+    acc[BS_RAYS] += segs;
+  }
+}
+__device__ __forceinline__ void sample_chunk_any(const BatchArgs &A, const TraceParams &P, const RayList_ispc &R,
+                                                 const BatchChunk &ch, int lay, float *xyz, int *pc, int cap,
+                                                 unsigned long long *acc) {
+  if (lay == 8) sample_chunk_as<8>(A, P, R, ch, xyz, pc, cap, acc);
+  else if (lay == 4) sample_chunk_as<4>(A, P, R, ch, xyz, pc, cap, acc);
+  else if (lay == 1) sample_chunk_as<1>(A, P, R, ch, xyz, pc, cap, acc);
+  else sample_chunk_as<0>(A, P, R, ch, xyz, pc, cap, acc);
+}
+
 // ------------------------------------------------------------------ pass set-up
 // work-queue head, per-list pending counts and the output count cleared
 // (kernels, not memsets, so that the pass can be captured in a graph)
@@ -181,20 +229,14 @@ void k_btrace(BatchArgs A) {
     const int ds = Ls.ds, mode = Ls.mode;
     const TraceParams &P = kconst(A.ds + ds);
     const RayList_ispc R = chunk_rays(A, ch);
-    // voxel addressing of the dataset's volume: bricked (8), small linear (1), large linear (0)
-    const int lay = P.vv[0].vol.brick == 8 ? 8 : (P.vv[0].vol.small ? 1 : 0);
+    // voxel addressing of the dataset's volume: bricked (8 or 4), small linear (1), large linear (0)
+    const int lay = chunk_layout(P);
     if (mode == BM_SAMPLE) {
       // every crossing of the ray in this launch (segments counted as ray passes)
       float *xyz = gptr(Ls.xyz);
       int *pc = Ls.pcount;
       const int cap = Ls.pcap;
-      unsigned segs = 0;
-      for (int k = lane; k < ch.count; k += 64) {
-        if (lay == 8) acc[BS_SAMPLES] += sample_all<1, 8>(P, R, ch.start + k, xyz, pc, cap, A.max_segments, segs);
-        else if (lay == 1) acc[BS_SAMPLES] += sample_all<1, 1>(P, R, ch.start + k, xyz, pc, cap, A.max_segments, segs);
-        else acc[BS_SAMPLES] += sample_all<1, 0>(P, R, ch.start + k, xyz, pc, cap, A.max_segments, segs);
-        acc[BS_RAYS] += segs;
-      }
+      sample_chunk_any(A, P, R, ch, lay, xyz, pc, cap, acc);
     } else if (P.has_model) {
       acc[BS_RAYS] += lane < ch.count ? (ch.count - 1 - lane) / 64 + 1 : 0;
       const bool wide = P.model.wide != 0;
@@ -216,18 +258,7 @@ void k_btrace(BatchArgs A) {
         wave_lds_sync();
         cur_tf = ds;
       }
-      if (lay == 8)
-        for (int k = lane; k < ch.count; k += 64)
-          acc[BS_SAMPLES] += trace_one<1, false, PrivStack<GXY_BVH_STACK>, 8>(P, R, ch.start + k, s_col[w], s_op[w],
-                                                                            pstack);
-      else if (lay == 1)
-        for (int k = lane; k < ch.count; k += 64)
-          acc[BS_SAMPLES] += trace_one<1, false, PrivStack<GXY_BVH_STACK>, 1>(P, R, ch.start + k, s_col[w], s_op[w],
-                                                                            pstack);
-      else
-        for (int k = lane; k < ch.count; k += 64)
-          acc[BS_SAMPLES] += trace_one<1, false, PrivStack<GXY_BVH_STACK>, 0>(P, R, ch.start + k, s_col[w], s_op[w],
-                                                                            pstack);
+      acc[BS_SAMPLES] += render_chunk_any(P, R, ch, lay, s_col[w], s_op[w], pstack);
     }
   }
   // lane sums of the sample and ray-pass counts
@@ -269,18 +300,12 @@ void k_btrace_kind(BatchArgs A) {
     const int ds = Ls.ds;
     const TraceParams &P = kconst(A.ds + ds);
     const RayList_ispc R = chunk_rays(A, ch);
-    const int lay = P.vv[0].vol.brick == 8 ? 8 : (P.vv[0].vol.small ? 1 : 0);
+    const int lay = chunk_layout(P);
     if (KIND == 2) {
       float *xyz = gptr(Ls.xyz);
       int *pc = Ls.pcount;
       const int cap = Ls.pcap;
-      unsigned segs = 0;
-      for (int k = lane; k < ch.count; k += 64) {
-        if (lay == 8) acc[BS_SAMPLES] += sample_all<1, 8>(P, R, ch.start + k, xyz, pc, cap, A.max_segments, segs);
-        else if (lay == 1) acc[BS_SAMPLES] += sample_all<1, 1>(P, R, ch.start + k, xyz, pc, cap, A.max_segments, segs);
-        else acc[BS_SAMPLES] += sample_all<1, 0>(P, R, ch.start + k, xyz, pc, cap, A.max_segments, segs);
-        acc[BS_RAYS] += segs;
-      }
+      sample_chunk_any(A, P, R, ch, lay, xyz, pc, cap, acc);
     } else if (KIND == 1) {
       acc[BS_RAYS] += lane < ch.count ? (ch.count - 1 - lane) / 64 + 1 : 0;
       const bool wide = P.model.wide != 0;
@@ -302,18 +327,7 @@ void k_btrace_kind(BatchArgs A) {
         wave_lds_sync();
         cur_tf = ds;
       }
-      if (lay == 8)
-        for (int k = lane; k < ch.count; k += 64)
-          acc[BS_SAMPLES] += trace_one<1, false, PrivStack<GXY_BVH_STACK>, 8>(P, R, ch.start + k, s_col[w % TFW],
-                                                                            s_op[w % TFW], pstack);
-      else if (lay == 1)
-        for (int k = lane; k < ch.count; k += 64)
-          acc[BS_SAMPLES] += trace_one<1, false, PrivStack<GXY_BVH_STACK>, 1>(P, R, ch.start + k, s_col[w % TFW],
-                                                                            s_op[w % TFW], pstack);
-      else
-        for (int k = lane; k < ch.count; k += 64)
-          acc[BS_SAMPLES] += trace_one<1, false, PrivStack<GXY_BVH_STACK>, 0>(P, R, ch.start + k, s_col[w % TFW],
-                                                                            s_op[w % TFW], pstack);
+      acc[BS_SAMPLES] += render_chunk_any(P, R, ch, lay, s_col[w % TFW], s_op[w % TFW], pstack);
     }
   }
   acc[BS_SAMPLES] = wave_sum_u64(acc[BS_SAMPLES]);
@@ -695,14 +709,15 @@ static void launch_split_trace(const BatchArgs &A, const BatchLaunch &K, hipStre
   (void)hipEventRecord(ev[0], s);
   for (int q = 0; q < 3; q++) (void)hipStreamWaitEvent(ks[q], ev[0], 0);
   // volume render: occupancy request 5 or 6 waves per SIMD when asked (LDS TF slices only)
-  if (K.wpe == 5) bkind_as<0, 0, 5>(A, tb, ks[0]);
+  if (K.wpe == 6) bkind_as<0, 0, 6>(A, tb, ks[0]);
+  else if (K.wpe == 5) bkind_as<0, 0, 5>(A, tb, ks[0]);
   else if (K.wpe == 4) bkind_as<0, 0, 4>(A, tb, ks[0]);
   else bkind_as<0, 0, 0>(A, tb, ks[0]);
   if (K.stack == 16) bkind_as<1, 16, 0>(A, tb, ks[1]);
   else if (K.stack == 24) bkind_as<1, 24, 0>(A, tb, ks[1]);
   else if (K.stack == 32) bkind_as<1, 32, 0>(A, tb, ks[1]);
   else bkind_as<1, 0, 0>(A, tb, ks[1]);
-  bkind_as<2, 0, 0>(A, tb, ks[2]);
+  bkind_as<2, 0, 8>(A, tb, ks[2]);
   for (int q = 0; q < 3; q++) {
     (void)hipEventRecord(ev[1 + q], ks[q]);
     (void)hipStreamWaitEvent(s, ev[1 + q], 0);

@@ -19,8 +19,12 @@
 #define GXY_MAX_VOLVIS 4
 #define GXY_MAX_SLICES 8
 #define GXY_MAX_ISO 8
-// round-cone pieces per path-line curve segment
-#define GXY_CURVE_PIECES 8
+// path-line curve segments are traced as round cones between curve samples at
+// equal steps of u, as many per segment as keep the traced solid within
+// GXY_CURVE_TOL x (the segment's least radius) of the swept-sphere solid
+// (at most GXY_CURVE_MAX_PIECES)
+#define GXY_CURVE_TOL 0.01
+#define GXY_CURVE_MAX_PIECES 32
 #define GXY_TF_MAX 256
 #define GXY_BLOCK 256
 // upper bound on march steps per ray and pass (a 4096^3 diagonal at half-voxel

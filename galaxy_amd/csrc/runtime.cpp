@@ -402,7 +402,7 @@ struct SpheresObj {
 
 // Path lines as round Bezier curves (DataDrivenPathLines.cpp:46-160): the
 // control points (x y z radius) of every curve segment and, on the host, the
-// round-cone pieces each segment is traced as.
+// round-cone pieces each segment is traced as (curve_pieces of them).
 struct PathObj {
   int geomID;
   int nseg = 0;
@@ -1212,6 +1212,35 @@ static float path_radius(float d, float r0, float r1, float v0, float v1) {
   return (R <= 0.0f) ? r0 : (R >= 1.0f) ? r1 : r0 + R * (r1 - r0);
 }
 
+// Round cones per curve segment (control points c, x y z radius each).  With
+// n equal steps of u, the pieces' centres and radii are the linear
+// interpolants of p(u) and r(u) on each step, which differ from them by at
+// most (1/n)^2 / 8 times max|p''| and max|r''|; a ball moved by d and resized
+// by e stays within d + e of the original, so the union of the pieces (the
+// traced solid) and the union of the spheres (p(u), r(u)) (the swept-sphere
+// solid) lie within (1/n)^2 / 8 (max|p''| + max|r''|) of each other.  For a
+// cubic, p'' is linear in u: max|p''| <= 6 max(|c0 - 2c1 + c2|, |c1 - 2c2 + c3|),
+// likewise for r, and r(u) >= the least control radius.  The count is the
+// least n that brings the bound to GXY_CURVE_TOL times that radius (one piece
+// for a straight segment of constant radius), at most GXY_CURVE_MAX_PIECES.
+static int curve_pieces(const float *c) {
+  double dp = 0.0, dr = 0.0, rmin = c[3];
+  for (int k = 0; k < 2; k++) {
+    double d[4];
+    for (int a = 0; a < 4; a++)
+      d[a] = (double)c[4 * k + a] - 2.0 * (double)c[4 * (k + 1) + a] + (double)c[4 * (k + 2) + a];
+    const double l = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    dp = l > dp ? l : dp;
+    dr = std::fabs(d[3]) > dr ? std::fabs(d[3]) : dr;
+  }
+  for (int k = 1; k < 4; k++) rmin = (double)c[4 * k + 3] < rmin ? (double)c[4 * k + 3] : rmin;
+  const double m = 6.0 * (dp + dr);
+  if (!(m > 0.0)) return 1;
+  if (!(rmin > 0.0)) return GXY_CURVE_MAX_PIECES;
+  const double n = std::ceil(std::sqrt(m / (8.0 * GXY_CURVE_TOL * rmin)));
+  return n < 1.0 ? 1 : (n > (double)GXY_CURVE_MAX_PIECES ? GXY_CURVE_MAX_PIECES : (int)n);
+}
+
 GXY_EXPORT int gxy_model_add_pathlines(void *model, const float *V, const float *D, int nv, const int *I, int nseg,
                                        float r0, float r1, float v0, float v1, void *tf) {
   ModelObj *M = lookup<ModelObj>(G.models, model);
@@ -1263,12 +1292,13 @@ GXY_EXPORT int gxy_model_add_pathlines(void *model, const float *V, const float 
     }
     push(en[0], en[1], en[2], re);
   }
-  // pieces: the curve sampled at u = j / GXY_CURVE_PIECES (Bernstein weights)
+  // pieces: the curve sampled at u = j / n (Bernstein weights), n from curve_pieces
   for (int i = 0; i < nseg; i++) {
     const float *c = &cp[4 * (size_t)ic[i]];
+    const int np = curve_pieces(c);
     float prev[4] = {0, 0, 0, 0}, prevu = 0.f;
-    for (int j = 0; j <= GXY_CURVE_PIECES; j++) {
-      const float u = (float)j / (float)GXY_CURVE_PIECES, q = 1.0f - u;
+    for (int j = 0; j <= np; j++) {
+      const float u = (float)j / (float)np, q = 1.0f - u;
       const float w0 = q * q * q, w1 = 3.0f * q * q * u, w2 = 3.0f * q * u * u, w3 = u * u * u;
       float x[4];
       for (int k = 0; k < 4; k++) x[k] = w0 * c[k] + w1 * c[4 + k] + w2 * c[8 + k] + w3 * c[12 + k];
@@ -3078,7 +3108,7 @@ GXY_EXPORT int gxy_batch_set_config(void *b, int chunk, int stack, int wpe, int 
   if (!B) return fail("not a batch handle");
   if (chunk < 64 || chunk > 1024 || chunk % 64) return fail("chunk %d: a multiple of 64 in [64, 1024]", chunk);
   if (stack != 0 && stack != 16 && stack != 24 && stack != 32) return fail("stack %d: 0, 16, 24 or 32", stack);
-  if (wpe != 0 && wpe != 4 && wpe != 5) return fail("wpe %d: 0, 4 or 5", wpe);
+  if (wpe != 0 && wpe != 4 && wpe != 5 && wpe != 6) return fail("wpe %d: 0, 4, 5 or 6", wpe);
   if (bpc < 1 || bpc > 16) return fail("blocks per CU %d", bpc);
   if (chunk != B->chunk && B->queued > 0) return fail("chunk size changes only with no rays queued");
   B->chunk = chunk;
@@ -3397,6 +3427,7 @@ struct ServerObj {
   std::deque<ReqRecord> recs;
   std::deque<int> queue;
   std::vector<Flight> fl;
+  int admitting = 0;                   // requests taken off the queue, not yet in fl (admission under way)
   long long base_bytes = 0;          // device bytes in use when serving started
   long long peak = 0;
   int max_in_flight = 0;
@@ -3512,6 +3543,9 @@ static void serve_abort(ServerObj *S_) {
 // free and the device bytes held since serving started plus the request's
 // estimate fit the budget (an estimate over the budget alone was rejected at
 // submission).  A request whose set-up fails is failed alone.
+// nothing queued, being admitted or in flight (S_->mu held)
+static bool server_idle(const ServerObj *S_) { return S_->queue.empty() && S_->admitting == 0 && S_->fl.empty(); }
+
 static void serve_admit(ServerObj *S_) {
   BatchObj *B = (BatchObj *)S_->batch;
   for (;;) {
@@ -3533,6 +3567,7 @@ static void serve_admit(ServerObj *S_) {
       }
       rr.res.status = RQ_RUNNING;
       rr.res.admitted = since(S_);
+      S_->admitting++;
     }
     const gxy_request r = S_->recs[id].req;
     Flight f;
@@ -3547,6 +3582,7 @@ static void serve_admit(ServerObj *S_) {
       ok = f.list >= 0;
     }
     std::lock_guard<std::mutex> lk(S_->mu);
+    S_->admitting--;
     S_->fl.push_back(f);
     if (!ok) {
       S_->fl.pop_back();
@@ -3654,7 +3690,7 @@ static void service_loop(ServerObj *S_) {
   for (;;) {
     {
       std::unique_lock<std::mutex> lk(S_->mu);
-      if (S_->stop && S_->queue.empty() && S_->fl.empty()) break;
+      if (S_->stop && server_idle(S_)) break;
       if (S_->fl.empty()) {
         // nothing admitted: sleep until a request arrives (or its arrival time)
         bool wait = S_->queue.empty() || S_->recs[S_->queue.front()].req.arrival > since(S_);
@@ -3828,7 +3864,7 @@ GXY_EXPORT int gxy_server_drain(void *s, double timeout_s) {
   if (!S_->th.joinable()) return fail("server not started");
   const auto until = std::chrono::steady_clock::now() + std::chrono::duration<double>(timeout_s);
   const bool drained = S_->idle_cv.wait_until(lk, until, [&] {
-    return (S_->queue.empty() && S_->fl.empty()) || !S_->running;
+    return server_idle(S_) || !S_->running;
   });
   if (S_->loop_error) return fail("server loop failed: %s", S_->loop_msg.c_str());
   if (drained) return 0;
@@ -3838,7 +3874,7 @@ GXY_EXPORT int gxy_server_drain(void *s, double timeout_s) {
     if (rr.res.status == RQ_QUEUED || rr.res.status == RQ_RUNNING) rr.cancel = true;
   S_->cv.notify_all();
   const bool emptied = S_->idle_cv.wait_for(lk, std::chrono::seconds(60), [&] {
-    return (S_->queue.empty() && S_->fl.empty()) || !S_->running;
+    return server_idle(S_) || !S_->running;
   });
   return fail("drain timed out after %.3f s with %d requests left (%s)", timeout_s, left,
               emptied ? "cancelled" : "cancellation still pending");

@@ -244,8 +244,33 @@ struct OSpheres {
 
 // Path lines (DataDrivenPathLines.cpp:46-160): control points of round cubic
 // Bezier segments (x y z radius) and the round-cone pieces each segment is
-// traced as (8 per segment, between curve samples at u = j/8).
-const int kCurvePieces = 8;
+// traced as, between curve samples at u = j/n.  n per segment: the least
+// count whose interpolation bound (1/n)^2/8 * (max|p''| + max|r''|), with the
+// second derivatives of the cubic bounded by 6 x its largest second
+// difference of control points, is at most kCurveTol x the least control
+// radius; 1 when both second differences vanish, at most kCurveMaxPieces.
+// The reference hands the control points to Embree's round-curve intersector
+// instead; the pieces tessellate that swept-sphere surface, and the tests
+// measure the hits against it in FP64.
+const double kCurveTol = 0.01;
+const int kCurveMaxPieces = 32;
+static int pieces_for(const float *c) {
+  double sd[2][4];
+  for (int k = 0; k < 2; k++)
+    for (int a = 0; a < 4; a++) sd[k][a] = (double)c[4 * k + a] - 2.0 * (double)c[4 * k + 4 + a] + (double)c[4 * k + 8 + a];
+  double len[2], rad[2];
+  for (int k = 0; k < 2; k++) {
+    len[k] = std::sqrt(sd[k][0] * sd[k][0] + sd[k][1] * sd[k][1] + sd[k][2] * sd[k][2]);
+    rad[k] = std::fabs(sd[k][3]);
+  }
+  const double bound = 6.0 * (std::max(len[0], len[1]) + std::max(rad[0], rad[1]));
+  const double rlo = std::min(std::min((double)c[3], (double)c[7]), std::min((double)c[11], (double)c[15]));
+  if (!(bound > 0.0)) return 1;
+  if (!(rlo > 0.0)) return kCurveMaxPieces;
+  const double n = std::ceil(std::sqrt(bound / (8.0 * kCurveTol * rlo)));
+  if (n < 1.0) return 1;
+  return n > (double)kCurveMaxPieces ? kCurveMaxPieces : (int)n;
+}
 struct OPaths {
   int geomID;
   std::vector<float> cp;          // control points, 4 floats each
@@ -1336,9 +1361,10 @@ OEXPORT int oracle_model_add_pathlines(void *mp, const float *V, const float *D,
   }
   for (int i = 0; i < nseg; i++) {
     const float *c = &P->cp[4 * (size_t)P->ic[i]];
+    const int np = pieces_for(c);
     float prev[4] = {0, 0, 0, 0}, prevu = 0.f;
-    for (int j = 0; j <= kCurvePieces; j++) {
-      const float u = (float)j / (float)kCurvePieces, q = 1.0f - u;
+    for (int j = 0; j <= np; j++) {
+      const float u = (float)j / (float)np, q = 1.0f - u;
       const float w0 = q * q * q, w1 = 3.0f * q * q * u, w2 = 3.0f * q * u * u, w3 = u * u * u;
       float x[4];
       for (int k = 0; k < 4; k++) x[k] = w0 * c[k] + w1 * c[4 + k] + w2 * c[8 + k] + w3 * c[12 + k];

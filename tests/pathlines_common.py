@@ -1,14 +1,33 @@
 """Path-line scenes and FP64 references shared by the oracle and GPU tests:
 round-Bezier control points from the reference's construction rules
 (DataDrivenPathLines.cpp:97-157) checked through their defining properties,
-the closed-form capsule a straight constant-radius path line must be, and the
-signed distance to the swept-sphere surface that every hit must lie on."""
+the closed-form capsule a straight constant-radius path line must be, the
+signed distance to the round-cone pieces every hit must lie on, and the
+distance to the true swept-sphere surface of the cubic (the union of the
+spheres (p(u), r(u)), 0 <= u <= 1) that bounds the tessellation."""
 import numpy as np
 
 from galaxy_amd import scenes
 
 f32 = np.float32
-PIECES = 8          # round cones per curve segment in both implementations
+CURVE_TOL = 0.01    # traced solid within CURVE_TOL x the least control radius of the swept-sphere solid
+MAX_PIECES = 32
+
+
+def piece_counts(P):
+    """Round cones per segment from control points P (nseg, 4, 4): the least
+    n with (1/n)^2 / 8 * 6 (max|second difference of xyz| + max|second
+    difference of r|) <= CURVE_TOL * least control radius (the interpolation
+    bound of a cubic), 1 for a straight constant-radius segment."""
+    P = np.asarray(P, np.float64)
+    sd = np.stack([P[:, 0] - 2.0 * P[:, 1] + P[:, 2], P[:, 1] - 2.0 * P[:, 2] + P[:, 3]], 1)   # (nseg, 2, 4)
+    ln = np.sqrt(sd[..., 0] * sd[..., 0] + sd[..., 1] * sd[..., 1] + sd[..., 2] * sd[..., 2]).max(1)
+    bound = 6.0 * (ln + np.abs(sd[..., 3]).max(1))
+    rlo = P[:, :, 3].min(1)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        n = np.ceil(np.sqrt(bound / (8.0 * CURVE_TOL * rlo)))
+    n = np.where(bound > 0, np.where(rlo > 0, np.clip(n, 1, MAX_PIECES), MAX_PIECES), 1)
+    return n.astype(np.int64)
 
 
 def helix_lines(seed=3, nlines=6, nverts=12):
@@ -43,16 +62,75 @@ def controls(oracle, model, which, nseg):
     return cp, ic
 
 
+def segment_controls(cp, ic):
+    """Control points of every segment, (nseg, 4, 4) in FP64."""
+    return cp.astype(np.float64)[ic[:, None] + np.arange(4)[None, :]]
+
+
 def pieces64(cp, ic):
     """Round-cone pieces (a, ra, b, rb) of every segment in FP64 from the
-    control points: Bernstein samples at u = j / PIECES.  Returns (a, b) of
-    shape (nseg, PIECES, 4)."""
-    P = cp.astype(np.float64)[ic[:, None] + np.arange(4)[None, :]]          # (nseg, 4, 4)
-    u = np.arange(PIECES + 1) / PIECES
+    control points: Bernstein samples at u = j / n, n = piece_counts.
+    Returns (a, b) of shape (nseg, MAX_PIECES, 4); a segment with fewer
+    pieces repeats its last one."""
+    P = segment_controls(cp, ic)
+    n = piece_counts(P.astype(f32))
+    j = np.minimum(np.arange(MAX_PIECES + 1)[None, :], n[:, None])            # (nseg, MAX_PIECES+1)
+    u = j / n[:, None]
     q = 1 - u
-    W = np.stack([q ** 3, 3 * q * q * u, 3 * q * u * u, u ** 3], 1)          # (PIECES+1, 4)
-    X = np.einsum("jk,skc->sjc", W, P)                                        # (nseg, PIECES+1, 4)
-    return X[:, :-1], X[:, 1:]
+    W = np.stack([q ** 3, 3 * q * q * u, 3 * q * u * u, u ** 3], 2)          # (nseg, MAX_PIECES+1, 4)
+    X = np.einsum("sjk,skc->sjc", W, P)                                       # (nseg, MAX_PIECES+1, 4)
+    a, b = X[:, :-1].copy(), X[:, 1:].copy()
+    last = (np.arange(MAX_PIECES)[None, :] >= n[:, None])
+    idx = np.minimum(np.arange(MAX_PIECES)[None, :], n[:, None] - 1)
+    a[last] = np.take_along_axis(X, idx[..., None], 1)[last]
+    b[last] = np.take_along_axis(X, (idx + 1)[..., None], 1)[last]
+    return a, b
+
+
+def _bezier(P, u):
+    """Points (x y z r) of cubics P (n, 4, 4) at parameters u (n, k)."""
+    q = 1 - u
+    W = np.stack([q ** 3, 3 * q * q * u, 3 * q * u * u, u ** 3], -1)
+    return np.einsum("nkj,njc->nkc", W, P)
+
+
+def sweep_distance(x, P, grid=1025, iters=60):
+    """FP64 signed distance from points x (n, 3) to the swept-sphere solid of
+    each point's own cubic P (n, 4, 4): min over u of |x - p(u)| - r(u), by a
+    dense grid in u refined by ternary search around its best sample.  Outside
+    the solid this is the distance to it; just inside, minus the depth to first
+    order in that depth over the radius."""
+    x = np.asarray(x, np.float64)
+    u = np.broadcast_to(np.linspace(0, 1, grid), (len(x), grid))
+    X = _bezier(P, u)
+    d = np.linalg.norm(x[:, None] - X[..., :3], axis=2) - X[..., 3]
+    k = d.argmin(1)
+    lo = np.clip((k - 1) / (grid - 1), 0, 1)
+    hi = np.clip((k + 1) / (grid - 1), 0, 1)
+
+    def f(uu):
+        p = _bezier(P, uu[:, None])[:, 0]
+        return np.linalg.norm(x - p[:, :3], axis=1) - p[:, 3]
+    for _ in range(iters):
+        m1 = lo + (hi - lo) / 3
+        m2 = hi - (hi - lo) / 3
+        left = f(m1) < f(m2)
+        hi = np.where(left, m2, hi)
+        lo = np.where(left, lo, m1)
+    return np.minimum(f(0.5 * (lo + hi)), d.min(1))
+
+
+def curve_hit_error(x, seg, P, I):
+    """Distance of hit points x on segments seg to the true swept surface of
+    their path line near the hit (the hit segment and its neighbours on the
+    same polyline), and the tolerance CURVE_TOL x (least control radius of the
+    hit segment) the tessellation guarantees."""
+    best = np.full(len(x), np.inf)
+    for off in (-1, 0, 1):
+        s = np.clip(seg + off, 0, len(I) - 1)
+        same = (I[s] - I[seg]) == off
+        best = np.where(same, np.minimum(best, sweep_distance(x, P[s])), best)
+    return np.abs(best), CURVE_TOL * P[seg, :, 3].min(1)
 
 
 def swept_distance(x, a, b, grid=2049):

@@ -1,7 +1,8 @@
 """Path lines on the GPU (gxy_model_add_pathlines): hits, depths, normals and
 colours equal the oracle's on random rays; a straight constant-radius path
 line is the closed-form capsule; every hit lies on the FP64 swept-sphere
-surface; a lit frame with shadow and AO rays matches the oracle's."""
+surface and within the stated tolerance of the true swept-sphere surface of
+the curves; a lit frame with shadow and AO rays matches the oracle's."""
 import ctypes as C
 
 import numpy as np
@@ -9,7 +10,8 @@ import pytest
 
 from conftest import random_rays
 from galaxy_amd import abi, scenes
-from pathlines_common import capsule_hits, controls, helix_desc, pieces64, rays_toward, swept_distance
+from pathlines_common import (capsule_hits, controls, curve_hit_error, helix_desc, pieces64, rays_toward,
+                              segment_controls, swept_distance)
 
 pytestmark = pytest.mark.gpu
 f32 = np.float32
@@ -106,6 +108,35 @@ def test_hits_lie_on_the_swept_surface(hip, oracle):
     nrm = np.c_[g["nx"][hit], g["ny"][hit], g["nz"][hit]]
     assert np.allclose(np.linalg.norm(nrm, axis=1), 1, atol=1e-5)
     assert (np.einsum("ij,ij->i", nrm, d[hit]) < 0).all()
+
+
+def test_hits_within_tolerance_of_the_true_sweep(hip, oracle):
+    """GPU hits on the first 150 lines of the 4000-line path-line workload lie
+    within CURVE_TOL x (least control radius of the hit segment) of the
+    surface the spheres (p(u), r(u)) of the round Bezier segments sweep
+    (DataDrivenPathLines.ispc:319-331), measured in FP64."""
+    from galaxy_amd import workloads
+    full = workloads.pathlines()[0].pathlines[0]
+    nl, nv = 150, 64
+    pl = scenes.PathLinesDesc(full.vertices[:nl * nv], full.data[:nl * nv], full.indices[:nl * (nv - 1)],
+                              radius0=full.radius0, radius1=full.radius1, value0=full.value0, value1=full.value1,
+                              colormap=full.colormap)
+    Bh = scenes.build_scene(hip, scene_with(pl))
+    org, d = rays_toward(12000, 4, pl.vertices.min(0), pl.vertices.max(0), dist=2.0)
+    hr = abi.HostRays.from_arrays(org, d, tmax=np.full(len(org), 1e30, f32))
+    hr["type"] = np.full(hr.n, 2, np.int32)
+    g, gg, gp = gpu_hits(hip, Bh, hr)
+    hit = gg >= 0
+    assert hit.sum() > 3000
+    M = oracle.model_create()
+    oracle.model_add_pathlines(M, pl.vertices, pl.data, pl.indices, pl.radius0, pl.radius1, pl.value0, pl.value1,
+                               None)
+    cp, ic = controls(oracle, M, 0, len(pl.indices))
+    x = org[hit].astype(np.float64) + g["t"][hit, None].astype(np.float64) * d[hit].astype(np.float64)
+    err, tol = curve_hit_error(x, gp[hit], segment_controls(cp, ic), pl.indices)
+    print("%d GPU hits: distance to the swept surface max %.3g mean %.3g, max err/tol %.3f"
+          % (hit.sum(), err.max(), err.mean(), (err / tol).max()), flush=True)
+    assert (err <= tol).all()
 
 
 def test_pathline_frame_with_shadows_and_ao_matches_oracle(hip, oracle, bvh_width):

@@ -6,8 +6,8 @@ FP64 swept-sphere surface with no crossing before it."""
 import numpy as np
 import pytest
 
-from pathlines_common import (PIECES, capsule_hits, controls, helix_lines, oracle_intersect, pieces64, rays_toward,
-                              swept_distance)
+from pathlines_common import (MAX_PIECES, capsule_hits, controls, curve_hit_error, helix_lines, oracle_intersect,
+                              piece_counts, pieces64, rays_toward, segment_controls, swept_distance)
 
 f32 = np.float32
 
@@ -98,8 +98,9 @@ def test_hits_lie_on_the_swept_surface(oracle):
     # on the surface of the hit segment's pieces
     assert np.abs(swept_distance(x, a[p[hit]], b[p[hit]])).max() <= 2e-5
     # nothing of any segment before the hit: points at 0.2 .. 0.98 of the way are outside every piece
-    allA = np.broadcast_to(a.reshape(1, -1, 4), (hit.sum(), a.shape[0] * PIECES, 4))
-    allB = np.broadcast_to(b.reshape(1, -1, 4), (hit.sum(), a.shape[0] * PIECES, 4))
+    real = np.arange(MAX_PIECES)[None, :] < piece_counts(segment_controls(cp, ic).astype(f32))[:, None]
+    allA = np.broadcast_to(a[real][None], (hit.sum(), real.sum(), 4))
+    allB = np.broadcast_to(b[real][None], (hit.sum(), real.sum(), 4))
     for f in (0.2, 0.6, 0.98):
         xf = org[hit].astype(np.float64) + (f * t[hit])[:, None] * d[hit]
         assert swept_distance(xf, allA, allB, grid=257).min() > -1e-6
@@ -146,3 +147,50 @@ def test_repeated_vertices_give_finite_pieces(oracle):
     t, g, p = oracle_intersect(oracle, M, org, d)
     ref = capsule_hits(org, d, (0, 0, 0), (1, 0, 0), 0.1)
     assert ((g >= 0) == np.isfinite(ref)).mean() >= 0.999
+
+
+def test_piece_counts(oracle):
+    """The tessellation rule: one piece for an inner segment of an evenly
+    spaced straight line of constant radius (its control points are evenly
+    spaced too, so the cubic is the chord itself), more where the control
+    points are uneven (the repeated end points) or the curve bends, never more
+    than MAX_PIECES."""
+    V, D, I = helix_lines(seed=2, nlines=4, nverts=10)
+    M = _model(oracle, V, D, I)
+    cp, ic = controls(oracle, M, 0, len(I))
+    n = piece_counts(segment_controls(cp, ic).astype(f32))
+    assert n.min() >= 2 and n.max() <= MAX_PIECES
+    line = np.array([[0.25 * k, 0.0, 0.0] for k in range(5)], f32)
+    M2 = _model(oracle, line, np.zeros(5, f32), np.arange(4, dtype=np.int32), 0.1, 0.3, 0.0, 0.0)
+    cp2, ic2 = controls(oracle, M2, 0, 4)
+    n2 = piece_counts(segment_controls(cp2, ic2).astype(f32))
+    assert list(n2[1:3]) == [1, 1] and n2[0] > 1 and n2[3] > 1
+
+
+@pytest.mark.parametrize("case", ["helix", "workload_lines"])
+def test_hits_within_tolerance_of_the_true_sweep(oracle, case):
+    """The reference intersects round Bezier curves (DataDrivenPathLines.ispc:
+    319-331): the surface swept by the spheres (p(u), r(u)).  Every hit of the
+    tessellated curves lies within CURVE_TOL x (least control radius of its
+    segment) of that surface, measured in FP64."""
+    from galaxy_amd import workloads
+    if case == "helix":
+        V, D, I = helix_lines(seed=2, nlines=4, nverts=10)
+        r0, r1 = 0.02, 0.06
+        org, d = rays_toward(3000, 9, (-0.9, -0.9, -0.7), (0.9, 0.9, 0.7))
+    else:
+        pl = workloads.pathlines()[0].pathlines[0]
+        nl, nv = 150, 64                      # the first 150 lines of the 4000-line workload
+        V, D, I = pl.vertices[:nl * nv], pl.data[:nl * nv], pl.indices[:nl * (nv - 1)]
+        r0, r1 = pl.radius0, pl.radius1
+        org, d = rays_toward(12000, 4, V.min(0), V.max(0), dist=2.0)
+    M = _model(oracle, V, D, I, r0, r1)
+    t, g, p = oracle_intersect(oracle, M, org, d)
+    hit = g >= 0
+    assert hit.sum() > 300
+    cp, ic = controls(oracle, M, 0, len(I))
+    x = org[hit].astype(np.float64) + t[hit, None].astype(np.float64) * d[hit].astype(np.float64)
+    err, tol = curve_hit_error(x, p[hit], segment_controls(cp, ic), I)
+    print("%s: %d hits, distance to the swept surface max %.3g mean %.3g, max err/tol %.3f"
+          % (case, hit.sum(), err.max(), err.mean(), (err / tol).max()))
+    assert (err <= tol).all()
