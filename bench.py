@@ -463,6 +463,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--icosphere-steps", type=int, default=3)
+    ap.add_argument("--extra-only", default=None,
+                    help="run only this mesh-frame workload (sphere_mesh_512, nine_icospheres_4k_ao16 or "
+                         "pathlines_4000x63_segments_1024)")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=3)
@@ -593,6 +596,8 @@ def main():
         for name, fn, steps in (("sphere_mesh_512", workloads.sphere_mesh, args.steps),
                                 ("nine_icospheres_4k_ao16", workloads.nine_icospheres, args.icosphere_steps),
                                 ("pathlines_4000x63_segments_1024", workloads.pathlines, args.steps)):
+            if args.extra_only and name != args.extra_only:
+                continue
             try:
                 sc2, a2 = fn()
                 glib.check(L.gxy_set_opacity_threshold(a2))

@@ -91,10 +91,11 @@ __device__ __forceinline__ void block_partials(unsigned long long *v, int k0, in
   }
 }
 
-// voxel addressing of a dataset's volume: bricked (8 or 4), small linear (1), large linear (0)
+// voxel addressing of a dataset's volume: bricked (8 or 4, with 64-bit addresses: the batched
+// kernels are bound by HBM lines, not by address arithmetic), small linear (1), large linear (0)
 __device__ __forceinline__ int chunk_layout(const TraceParams &P) {
   const int b = P.vv[0].vol.brick;
-  return b == 8 ? 8 : (b == 4 ? 4 : (P.vv[0].vol.small ? 1 : 0));
+  return b == 8 ? 16 : (b == 4 ? 12 : (P.vv[0].vol.small ? 1 : 0));
 }
 
 // The render rays of one volume chunk by the chunk's layout, one ray per lane at a time.
@@ -113,8 +114,8 @@ __device__ __forceinline__ unsigned long long render_chunk_any(const TraceParams
                                                                const float (*col)[3 * GXY_TF_MAX],
                                                                const float (*op)[GXY_TF_MAX],
                                                                PrivStack<GXY_BVH_STACK> &pstack) {
-  if (lay == 8) return render_chunk<8>(P, R, ch, col, op, pstack);
-  if (lay == 4) return render_chunk<4>(P, R, ch, col, op, pstack);
+  if (lay == 16) return render_chunk<16>(P, R, ch, col, op, pstack);
+  if (lay == 12) return render_chunk<12>(P, R, ch, col, op, pstack);
   if (lay == 1) return render_chunk<1>(P, R, ch, col, op, pstack);
   return render_chunk<0>(P, R, ch, col, op, pstack);
 }
@@ -133,8 +134,8 @@ __device__ __forceinline__ void sample_chunk_as(const BatchArgs &A, const TraceP
 __device__ __forceinline__ void sample_chunk_any(const BatchArgs &A, const TraceParams &P, const RayList_ispc &R,
                                                  const BatchChunk &ch, int lay, float *xyz, int *pc, int cap,
                                                  unsigned long long *acc) {
-  if (lay == 8) sample_chunk_as<8>(A, P, R, ch, xyz, pc, cap, acc);
-  else if (lay == 4) sample_chunk_as<4>(A, P, R, ch, xyz, pc, cap, acc);
+  if (lay == 16) sample_chunk_as<16>(A, P, R, ch, xyz, pc, cap, acc);
+  else if (lay == 12) sample_chunk_as<12>(A, P, R, ch, xyz, pc, cap, acc);
   else if (lay == 1) sample_chunk_as<1>(A, P, R, ch, xyz, pc, cap, acc);
   else sample_chunk_as<0>(A, P, R, ch, xyz, pc, cap, acc);
 }

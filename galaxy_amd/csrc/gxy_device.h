@@ -44,6 +44,7 @@ struct DVolume {
   float irate;    // 1 / samplingRate
   int small;      // <= 2^30 voxels, x-y plane and z extent < 2^24: 32-bit byte offsets
   int brick;      // 0, or B = 4 / 8: a bricked copy of the voxels is in bdata
+  int bsmall;     // the bricked copy spans under 4 GiB: 32-bit byte offsets
   int nb[3];      // bricks per axis: ceil((n - 1) / B)
   const float *bdata;  // bricks of (B+1)^3 voxels (one-voxel overlap), x fastest
                        // inside a brick, each padded to a multiple of 128 bytes
@@ -209,6 +210,44 @@ struct VFetch {
   float f0, f1, f2;
 };
 
+// The four x-pair loads of cell (i0, i1, i2) from the bricked copy (B = 4 or
+// 8).  BIG = false when the whole copy spans under 4 GiB (DVolume::bsmall):
+// the byte offset is formed in 32 bits (24-bit multiplies) and the loads take
+// it beside the uniform base address; BIG = true forms a 64-bit address.
+template <int B, bool BIG>
+__device__ __forceinline__ void brick_fetch(const DVolume &V, int i0, int i1, int i2, VFetch &F) {
+  constexpr int LB = B == 4 ? 2 : 3, E = B + 1, PL = E * E;
+  const unsigned brick = (unsigned)(i0 >> LB) + __umul24((unsigned)(i1 >> LB), (unsigned)V.nb[0]) +
+                         __umul24((unsigned)(i2 >> LB), (unsigned)V.nb[0] * (unsigned)V.nb[1]);
+  const unsigned loc = (unsigned)(i0 & (B - 1)) + (unsigned)(i1 & (B - 1)) * E + (unsigned)(i2 & (B - 1)) * PL;
+  if (!BIG) {
+    const unsigned off = __umul24(brick, (unsigned)(4 * brick_floats(B))) + 4u * loc;
+    const char *d = (const char *)gptr(V.bdata) + (size_t)off;
+    F.p00 = *(const float2u *)(d);
+    F.p10 = *(const float2u *)(d + 4 * E);
+    F.p01 = *(const float2u *)(d + 4 * PL);
+    F.p11 = *(const float2u *)(d + 4 * (PL + E));
+  } else {
+    const float *d = gptr(V.bdata) + ((size_t)brick * brick_floats(B) + loc);
+    F.p00 = *(const float2u *)(d);
+    F.p10 = *(const float2u *)(d + E);
+    F.p01 = *(const float2u *)(d + PL);
+    F.p11 = *(const float2u *)(d + PL + E);
+  }
+}
+
+// Voxel addressing variants (SM): 4 / 8 the bricked copy with 32-bit offsets,
+// 12 / 16 the bricked copy (B = 4 / 8) with 64-bit addresses, 1 the linear
+// voxels with 32-bit offsets, 0 linear with 64-bit addresses, -1 chosen per
+// volume at run time (linear).
+__host__ __device__ constexpr bool sm_bricked(int SM) { return SM == 4 || SM == 8 || SM == 12 || SM == 16; }
+__host__ __device__ constexpr int sm_brick(int SM) { return (SM == 4 || SM == 12) ? 4 : 8; }
+__host__ __device__ constexpr bool sm_big(int SM) { return SM == 12 || SM == 16; }
+// the variant for a volume's current layout
+__host__ __device__ inline int sm_of(const DVolume &V) {
+  return V.brick ? V.brick + (V.bsmall ? 0 : 8) : (V.small ? 1 : 0);
+}
+
 // SM = 4 / 8: the bricked copy.  Cell (i0, i1, i2) lies in brick (i >> log2 B)
 // at local (i & (B-1)) and its eight corners are inside that brick (the
 // overlap voxel), so the four pair loads sit at constant offsets from one
@@ -230,16 +269,8 @@ __device__ __forceinline__ VFetch vol_fetch(const DVolume &V, f3 p) {
   F.f0 = l0 - (float)i0;
   F.f1 = l1 - (float)i1;
   F.f2 = l2 - (float)i2;
-  if (SM == 4 || SM == 8) {
-    constexpr int B = SM == 4 ? 4 : 8, LB = SM == 4 ? 2 : 3, E = B + 1, PL = E * E;
-    const unsigned brick = (unsigned)(i0 >> LB) + __umul24((unsigned)(i1 >> LB), (unsigned)V.nb[0]) +
-                           __umul24((unsigned)(i2 >> LB), (unsigned)V.nb[0] * (unsigned)V.nb[1]);
-    const unsigned loc = (unsigned)(i0 & (B - 1)) + (unsigned)(i1 & (B - 1)) * E + (unsigned)(i2 & (B - 1)) * PL;
-    const float *d = gptr(V.bdata) + ((size_t)brick * brick_floats(B) + loc);
-    F.p00 = *(const float2u *)(d);
-    F.p10 = *(const float2u *)(d + E);
-    F.p01 = *(const float2u *)(d + PL);
-    F.p11 = *(const float2u *)(d + PL + E);
+  if (sm_bricked(SM)) {
+    brick_fetch<sm_brick(SM), sm_big(SM)>(V, i0, i1, i2, F);
     return F;
   }
   const float *vd = gptr(V.data);
@@ -278,16 +309,8 @@ __device__ __forceinline__ VFetch vol_fetch_vox(const DVolume &V, float l0, floa
   F.f0 = l0 - (float)i0;
   F.f1 = l1 - (float)i1;
   F.f2 = l2 - (float)i2;
-  if (SM == 4 || SM == 8) {
-    constexpr int B = SM == 4 ? 4 : 8, LB = SM == 4 ? 2 : 3, E = B + 1, PL = E * E;
-    const unsigned brick = (unsigned)(i0 >> LB) + __umul24((unsigned)(i1 >> LB), (unsigned)V.nb[0]) +
-                           __umul24((unsigned)(i2 >> LB), (unsigned)V.nb[0] * (unsigned)V.nb[1]);
-    const unsigned loc = (unsigned)(i0 & (B - 1)) + (unsigned)(i1 & (B - 1)) * E + (unsigned)(i2 & (B - 1)) * PL;
-    const float *d = gptr(V.bdata) + ((size_t)brick * brick_floats(B) + loc);
-    F.p00 = *(const float2u *)(d);
-    F.p10 = *(const float2u *)(d + E);
-    F.p01 = *(const float2u *)(d + PL);
-    F.p11 = *(const float2u *)(d + PL + E);
+  if (sm_bricked(SM)) {
+    brick_fetch<sm_brick(SM), sm_big(SM)>(V, i0, i1, i2, F);
     return F;
   }
   const float *vd = gptr(V.data);

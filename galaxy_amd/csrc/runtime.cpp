@@ -969,6 +969,7 @@ GXY_EXPORT int gxy_volume_set_layout(void *vol, int brick) {
   v->bdata = nullptr;
   v->d.bdata = nullptr;
   v->d.brick = 0;
+  v->d.bsmall = 0;
   if (brick == 0) return 0;
   int nb[3];
   size_t nbr = 1;
@@ -983,6 +984,8 @@ GXY_EXPORT int gxy_volume_set_layout(void *vol, int brick) {
   for (int k = 0; k < 3; k++) v->d.nb[k] = nb[k];
   v->d.bdata = v->bdata;
   v->d.brick = brick;
+  // a cell's last load ends (PL + E + 2) floats past its offset, inside its own brick
+  v->d.bsmall = nbr * brick_floats(brick) * sizeof(float) < ((size_t)1 << 32) ? 1 : 0;
   return volume_brickify(v);
 }
 GXY_EXPORT int gxy_volume_get_layout(void *vol) {

@@ -372,6 +372,25 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_sampler(TraceParams P, RayList_is
 }
 
 // ------------------------------------------------------------- launchers
+// one volume with a bricked copy: the variant for its addressing (32-bit
+// offsets when the copy spans under 4 GiB) and the occupancy request
+template <int SM>
+static void launch_trace_sm(const TraceParams &P, const RayList_ispc &R, int n, int wpe, dim3 g, dim3 b,
+                            hipStream_t s) {
+  if (wpe == 5) hipLaunchKernelGGL((k_trace<1, false, 5, SM>), g, b, 0, s, P, R, n);
+  else if (wpe == 6) hipLaunchKernelGGL((k_trace<1, false, 6, SM>), g, b, 0, s, P, R, n);
+  else hipLaunchKernelGGL((k_trace<1, false, 0, SM>), g, b, 0, s, P, R, n);
+}
+static void launch_trace_bricked(const TraceParams &P, const RayList_ispc &R, int n, int wpe, int sm, dim3 g, dim3 b,
+                                 hipStream_t s) {
+  switch (sm) {
+    case 4: launch_trace_sm<4>(P, R, n, wpe, g, b, s); break;
+    case 8: launch_trace_sm<8>(P, R, n, wpe, g, b, s); break;
+    case 12: launch_trace_sm<12>(P, R, n, wpe, g, b, s); break;
+    default: launch_trace_sm<16>(P, R, n, wpe, g, b, s); break;
+  }
+}
+
 template <int NV>
 static hipError_t launch_trace_nv(const TraceParams &P, const RayList_ispc &R, int n, int wpe, int stack,
                                   hipStream_t s, int gwaves) {
@@ -398,12 +417,7 @@ static hipError_t launch_trace_nv(const TraceParams &P, const RayList_ispc &R, i
     else if (gwaves == 8) hipLaunchKernelGGL((k_trace<0, true, 8, -1, 0, 2>), g, b, 0, s, P, R, n);
     else hipLaunchKernelGGL((k_trace<0, true, 0, -1, 0, 2>), g, b, 0, s, P, R, n);
   } else if (P.has_model) hipLaunchKernelGGL((k_trace<NV, true, 0, -1>), g, b, 0, s, P, R, n);
-  else if (NV == 1 && P.vv[0].vol.brick == 8 && wpe == 5) hipLaunchKernelGGL((k_trace<NV, false, 5, 8>), g, b, 0, s, P, R, n);
-  else if (NV == 1 && P.vv[0].vol.brick == 8 && wpe == 6) hipLaunchKernelGGL((k_trace<NV, false, 6, 8>), g, b, 0, s, P, R, n);
-  else if (NV == 1 && P.vv[0].vol.brick == 8) hipLaunchKernelGGL((k_trace<NV, false, 0, 8>), g, b, 0, s, P, R, n);
-  else if (NV == 1 && P.vv[0].vol.brick == 4 && wpe == 5) hipLaunchKernelGGL((k_trace<NV, false, 5, 4>), g, b, 0, s, P, R, n);
-  else if (NV == 1 && P.vv[0].vol.brick == 4 && wpe == 6) hipLaunchKernelGGL((k_trace<NV, false, 6, 4>), g, b, 0, s, P, R, n);
-  else if (NV == 1 && P.vv[0].vol.brick == 4) hipLaunchKernelGGL((k_trace<NV, false, 0, 4>), g, b, 0, s, P, R, n);
+  else if (NV == 1 && P.vv[0].vol.brick) launch_trace_bricked(P, R, n, wpe, sm_of(P.vv[0].vol), g, b, s);
   else if (small && wpe == 5) hipLaunchKernelGGL((k_trace<NV, false, 5, 1>), g, b, 0, s, P, R, n);
   else if (small && wpe == 6) hipLaunchKernelGGL((k_trace<NV, false, 6, 1>), g, b, 0, s, P, R, n);
   else if (small) hipLaunchKernelGGL((k_trace<NV, false, 0, 1>), g, b, 0, s, P, R, n);
@@ -431,12 +445,13 @@ hipError_t launch_sampler(const TraceParams &P, const RayList_ispc &R, int n, hi
   dim3 g((n + GXY_BLOCK - 1) / GXY_BLOCK), b(GXY_BLOCK);
   bool small = P.nvv > 0;
   for (int v = 0; v < P.nvv && v < GXY_MAX_VOLVIS; v++) small = small && P.vv[v].vol.small;
-  if (P.nvv == 1 && P.vv[0].vol.brick == 8) {
-    hipLaunchKernelGGL((k_sampler<1, 8>), g, b, 0, s, P, R, n);
-    return hipGetLastError();
-  }
-  if (P.nvv == 1 && P.vv[0].vol.brick == 4) {
-    hipLaunchKernelGGL((k_sampler<1, 4>), g, b, 0, s, P, R, n);
+  if (P.nvv == 1 && P.vv[0].vol.brick) {
+    switch (sm_of(P.vv[0].vol)) {
+      case 4: hipLaunchKernelGGL((k_sampler<1, 4>), g, b, 0, s, P, R, n); break;
+      case 8: hipLaunchKernelGGL((k_sampler<1, 8>), g, b, 0, s, P, R, n); break;
+      case 12: hipLaunchKernelGGL((k_sampler<1, 12>), g, b, 0, s, P, R, n); break;
+      default: hipLaunchKernelGGL((k_sampler<1, 16>), g, b, 0, s, P, R, n); break;
+    }
     return hipGetLastError();
   }
   switch (P.nvv * 2 + (small ? 1 : 0)) {
