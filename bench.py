@@ -214,7 +214,9 @@ def geometry_roofline(name, sc, tot, steps, trace_ms, trace_launches):
     else:
         roof = dict(bound="mfma", achieved=round(tfl, 4), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
                     frac=round(tfl / FP32_PEAK_TFLOPS, 5), note="FP32 VALU work (no MFMA on this path); MI355X FP32 peak")
-    roof.update(traffic=None, hbm_frac=round(gbs / HBM_PEAK_GBS, 5), fp32_frac=round(tfl / FP32_PEAK_TFLOPS, 5),
+    prof = {"nine_icospheres_4k_ao16": "nine_icospheres_counters.md"}.get(name)
+    roof.update(traffic=traffic_from_profile(prof) if prof else None,
+                hbm_frac=round(gbs / HBM_PEAK_GBS, 5), fp32_frac=round(tfl / FP32_PEAK_TFLOPS, 5),
                 kernel_ms_per_launch=round(trace_ms / launches, 4), alg_bytes_per_launch=int(nbytes / launches),
                 alg_flops_per_launch=int(flops / launches),
                 flops_per_ray={k: round(v, 1) for k, v in per.items()},
