@@ -113,16 +113,18 @@ def run_frames(L, hip, built, cam, steps, warmup, eps=0.001):
     return (t1 - t0), totals, kt, st, image, breakdown
 
 
-def cpu_baseline(hip, sc, cam, alpha, target_s=15.0):
+def cpu_baseline(hip, sc, cam, alpha, target_s=15.0, ao=(0, 0)):
     """The CPU oracle (C++ restatement, std::thread on the host cores) on a
     bounded sample of the same frame: primary rays of every k-th image row,
-    traced, lit, spawned, classified and accumulated like the GPU frame."""
+    traced, lit, spawned, classified and accumulated like the GPU frame (AO
+    directions: ao = (mode, seed) as the GPU frame's)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_backend import OracleBackend, lib as olib
     from galaxy_amd import abi, scenes
     threads = min(16, os.cpu_count() or 1)
     olib().oracle_set_threads(threads)
     olib().oracle_set_opacity_threshold(alpha)
+    olib().oracle_set_ao_directions(*ao)
     ob = OracleBackend()
     vol = sc.volume_vis[0].volume if sc.volume_vis else None
     if vol is not None and vol.voxels is None:
@@ -165,6 +167,7 @@ def cpu_baseline(hip, sc, cam, alpha, target_s=15.0):
             samples += s2.samples
     if vol is not None:
         vol.voxels = None
+    olib().oracle_set_ao_directions(0, 0)
     return dict(value=round(passes / dt / 1e6, 4), unit="Mrays/s", cores=threads, kind="port",
                 gsamples_per_s=round(samples / dt / 1e9, 5), host_cores=os.cpu_count(), cpu_model=cpu_model(),
                 sample="%d x primary rays of every %d-th image row (%d primary, %d ray passes in all, %.1f s), "
@@ -603,6 +606,9 @@ def main():
             try:
                 sc2, a2 = fn()
                 glib.check(L.gxy_set_opacity_threshold(a2))
+                # the nine-icosphere frame's AO directions: seeded PCG32 streams
+                ao = (1, workloads.NINE_ICOSPHERES_AO_SEED) if name == "nine_icospheres_4k_ao16" else (0, 0)
+                glib.check(L.gxy_set_ao_directions(*ao))
                 log("building", name)
                 t = time.perf_counter()
                 b2 = scenes.build_scene(hip, sc2)
@@ -632,18 +638,22 @@ def main():
                                    geometry_stack=L.gxy_get_geometry_stack(), bvh_width=L.gxy_get_bvh_width(),
                                    geometry_waves=L.gxy_get_geometry_waves(),
                                    geometry_tune=[[list(c), round(ms, 4)] for c, ms in gtab])
+                if sc2.lighting is not None and sc2.lighting.n_ao:
+                    extra[name]["ao_directions"] = ("PCG32 streams, seed %d" % ao[1]) if ao[0] else "UV table"
                 roof2 = geometry_roofline(name, sc2, tot2, steps, kt2["trace"][0], kt2["trace"][1])
                 if roof2:
                     extra[name]["roofline"] = roof2
                 if not args.no_cpu and name in ("sphere_mesh_512", "nine_icospheres_4k_ao16"):
                     log("cpu baseline", name)
                     extra[name]["cpu_baseline"] = cpu_baseline(hip, sc2, sc2.cameras[0], a2,
-                                                               target_s=10.0 if name.startswith("sphere") else 15.0)
+                                                               target_s=10.0 if name.startswith("sphere") else 15.0,
+                                                               ao=ao)
                     glib.check(L.gxy_set_opacity_threshold(alpha))
                 release(L, b2)
                 b2 = None
             except Exception as e:  # reported, not hidden
                 extra[name] = dict(error=str(e))
+            glib.check(L.gxy_set_ao_directions(0, 0))
         out["extra_workloads"] = extra
         # the batched and served forms start from the library's BVH width, not the last frame's tuned one
         glib.check(L.gxy_set_bvh_width(saved_width))

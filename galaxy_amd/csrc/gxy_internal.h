@@ -17,6 +17,8 @@ struct LightParams {
   float L[3 * 16];
   int types[16];
   const float *U, *V;  // AO direction tables (256 entries each, device memory)
+  int ao_pcg;          // 1: AO directions from PCG32 streams per (pixel, AO index, seed) instead of the tables
+  unsigned ao_seed;
 };
 
 // route modes (bits)

@@ -43,6 +43,39 @@ __device__ __forceinline__ void light_ambient(const LightParams &L, ParentRay &P
 // TraceRays_generateAORays (TraceRays.ispc:640-730): AO child j of P (P.o
 // before diffuseLighting), written to O[off]; its colour is negative under
 // reverse lighting (:639-643).
+// PCG32 (O'Neill, "PCG: A Family of Simple Fast Space-Efficient Statistically
+// Good Algorithms for Random Number Generation", 2014; pcg32_random_r and
+// pcg32_srandom_r): a 64-bit LCG state, output by xorshift-high and a random
+// rotation.
+struct Pcg32 {
+  unsigned long long state, inc;
+};
+__device__ __forceinline__ unsigned pcg32_next(Pcg32 &g) {
+  const unsigned long long old = g.state;
+  g.state = old * 6364136223846793005ull + g.inc;
+  const unsigned xs = (unsigned)(((old >> 18u) ^ old) >> 27u);
+  const unsigned rot = (unsigned)(old >> 59u);
+  return (xs >> rot) | (xs << ((0u - rot) & 31u));
+}
+__device__ __forceinline__ Pcg32 pcg32_seeded(unsigned long long initstate, unsigned long long initseq) {
+  Pcg32 g;
+  g.state = 0ull;
+  g.inc = (initseq << 1u) | 1u;
+  pcg32_next(g);
+  g.state += initstate;
+  pcg32_next(g);
+  return g;
+}
+// The (U, V) pair of AO ray j of the pixel (px, py): the stream selected by the
+// pixel, started at (seed, j); each value is the top 24 bits of an output
+// over 2^24 (exact in FP32, in [0, 1)).
+__device__ __forceinline__ void ao_pcg_uv(unsigned seed, int px, int py, int j, float &u, float &v) {
+  Pcg32 g = pcg32_seeded(((unsigned long long)seed << 32) | (unsigned)j,
+                         ((unsigned long long)(unsigned)py << 32) | (unsigned)px);
+  u = (float)(pcg32_next(g) >> 8) * (1.0f / 16777216.0f);
+  v = (float)(pcg32_next(g) >> 8) * (1.0f / 16777216.0f);
+}
+
 __device__ __forceinline__ void light_ao_child(const LightParams &L, const ParentRay &P, int j, const RayList_ispc &O,
                                                int off) {
   const float Ka = (L.reverse ? -L.Ka : L.Ka) / L.nAO;
@@ -57,8 +90,14 @@ __device__ __forceinline__ void light_ao_child(const LightParams &L, const Paren
   ox = ox + L.eps * nrm.x;
   oy = oy + L.eps * nrm.y;
   oz = oz + L.eps * nrm.z;
-  int rr = ((P.px * 9949 + P.py * 9613 + j * 9151) >> 8) & 0xff;
-  float r0 = L.U[rr], r1 = L.V[rr];
+  float r0, r1;
+  if (L.ao_pcg) {
+    ao_pcg_uv(L.ao_seed, P.px, P.py, j, r0, r1);
+  } else {
+    int rr = ((P.px * 9949 + P.py * 9613 + j * 9151) >> 8) & 0xff;
+    r0 = L.U[rr];
+    r1 = L.V[rr];
+  }
   float w = sqrtf(1.f - r1);
   float x = cosf(6.2831855f * r0) * w;
   float y = sinf(6.2831855f * r0) * w;

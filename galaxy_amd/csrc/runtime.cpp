@@ -98,6 +98,8 @@ struct Runtime {
   std::vector<std::pair<int *, int *>> free_reads;   // readback areas of destroyed contexts (kept mapped)
   // volume-only trace kernel: 0 one ray per work-item, 1 persistent lanes
   int trace_kind = 0, trace_chunk = 256, trace_refill = 32, trace_wpe = 0;
+  int ao_pcg = 0;          // AO directions: 0 the UV table (reference), 1 PCG32 streams
+  unsigned ao_seed = 0;
   int geom_stack = 0;           // BVH traversal stack of geometry-only traces (0 private, or LDS entries)
   int occlusion = 1;            // rank loops: shadow/AO rays of volume-free scenes stop at their first hit
   int bvh_width = 2;            // traverse the binary trees (2) or the 4-wide ones (4)
@@ -583,6 +585,8 @@ static int make_light_params(const Lighting_ispc *l, float eps, LightParams &L) 
   L.reverse = G.reverse_lighting;
   L.U = G.uv;
   L.V = G.uv + 256;
+  L.ao_pcg = G.ao_pcg;
+  L.ao_seed = G.ao_seed;
   for (int k = 0; k < L.nLights; k++) {
     for (int a = 0; a < 3; a++) L.L[3 * k + a] = l->lights[3 * k + a];
     L.types[k] = l->types ? l->types[k] : 2;
@@ -2004,6 +2008,17 @@ GXY_EXPORT int gxy_set_reverse_lighting(int on) {
   return 0;
 }
 GXY_EXPORT int gxy_get_reverse_lighting(void) { return G.reverse_lighting; }
+GXY_EXPORT int gxy_set_ao_directions(int mode, unsigned seed) {
+  INIT_OR(-1);
+  if (mode != 0 && mode != 1) return fail("AO directions %d: 0 the UV table, 1 PCG32 streams", mode);
+  G.ao_pcg = mode;
+  G.ao_seed = seed;
+  return 0;
+}
+GXY_EXPORT int gxy_get_ao_directions(unsigned *seed) {
+  if (seed) *seed = G.ao_seed;
+  return G.ao_pcg;
+}
 
 GXY_EXPORT int gxy_classify(int n, RayList_ispc *R, int reverse) {
   INIT_OR(-1);

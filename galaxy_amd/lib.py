@@ -82,6 +82,8 @@ def _declare(L):
         "gxy_classify": (C.c_int, [C.c_int, vp, C.c_int]),
         "gxy_set_reverse_lighting": (C.c_int, [C.c_int]),
         "gxy_get_reverse_lighting": (C.c_int, []),
+        "gxy_set_ao_directions": (C.c_int, [C.c_int, C.c_uint]),
+        "gxy_get_ao_directions": (C.c_int, [vp]),
         "gxy_assign_destinations": (C.c_int, [C.c_int, vp, vp, vp, C.c_int]),
         "gxy_accumulate_terminated": (C.c_int, [C.c_int, vp, vp, C.c_int, C.c_int]),
         "gxy_pixels_pack": (C.c_int, [C.c_int, vp, C.c_int64, C.c_int64, C.c_int, C.c_int, vp, C.c_size_t, vp]),

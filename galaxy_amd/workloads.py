@@ -71,8 +71,15 @@ def sphere_mesh(width=512, height=512, seed=1):
     return scenes.SceneDesc(meshes=[m], local_box=box, global_box=box, lighting=L, cameras=[cam]), 0.999
 
 
+# seed of the nine-icosphere frame's AO directions (PCG32 streams per pixel,
+# gxy_set_ao_directions(1, seed)); parity runs of the frame also use the
+# reference's UV table
+NINE_ICOSPHERES_AO_SEED = 8
+
+
 def nine_icospheres(width=3840, height=2160, subdiv=8, n_ao=16, ao_radius=0.5):
-    """Nine icospheres, shadows and 16 AO rays per hit."""
+    """Nine icospheres, shadows and 16 AO rays per hit (AO directions seeded:
+    NINE_ICOSPHERES_AO_SEED)."""
     base = scenes.icosphere(subdiv, 0.45)
     meshes = []
     for i in range(3):

@@ -354,6 +354,17 @@ int gxy_spawn_offsets_noalloc(int nRays, RayList_ispc *raysIn, int nAO, int nLig
  * with it. */
 int gxy_set_reverse_lighting(int on);
 int gxy_get_reverse_lighting(void);
+/* AO ray directions (no counterpart in the reference's ABI; its
+ * generateAORays, TraceRays.ispc:625-733, always uses the UV table): mode 0
+ * (default) = the reference's 256-entry UV table indexed by its pixel hash
+ * ((x*9949 + y*9613 + j*9151) >> 8) & 0xff; mode 1 = (U, V) drawn from a PCG32
+ * stream (O'Neill 2014, pcg32_srandom_r(initstate = seed << 32 | j,
+ * initseq = y << 32 | x)), the top 24 bits of two outputs over 2^24, for
+ * workloads configured with seeded random AO directions.  The direction
+ * formula (cosine-weighted, not renormalised) is the reference's either way.
+ * Applies to every entry point that spawns AO rays. */
+int gxy_set_ao_directions(int mode, unsigned seed);
+int gxy_get_ao_directions(unsigned *seed);
 
 /* Renderer::Classify (Renderer.cpp:304-421), reverse lighting on when
  * reverse_lighting != 0 (the reference default, CMakeLists.txt:66); it must

@@ -21,7 +21,12 @@
 //                                   (forward difference: OSPRay stencil itself is
 //                                   parity unpinned)
 //   * lighting / AO / shadows     - gold images (shadow and AO states); AO table
-//                                   against the reference's table values
+//                                   against the reference's table values; the
+//                                   PCG32 AO streams (a non-reference option)
+//                                   against the published demo outputs and the
+//                                   moments of the cosine-weighted hemisphere
+//   * path-line tessellation      - hits within the stated bound of the FP64
+//                                   swept-sphere surface of the curve
 //   * triangle / sphere hits      - FP64 brute force on small scenes; BVH vs
 //                                   brute force (Embree's own tie order unpinned)
 //   * triangle / sphere shading   - closed forms: affine vertex data through a
@@ -1217,6 +1222,37 @@ OEXPORT void oracle_walk_counters(int on, long long *counts) {
   g_count_walks = on != 0;
 }
 OEXPORT void oracle_set_reverse_lighting(int on) { g_reverse = on ? 1 : 0; }
+
+// AO directions from PCG32 instead of the UV table (the library's
+// gxy_set_ao_directions mode 1).  PCG32 as published by O'Neill (2014):
+// pcg32_srandom_r / pcg32_random_r; pinned by the known answers of its demo
+// program in the tests.
+static int g_ao_pcg = 0;
+static uint32_t g_ao_seed = 0;
+struct OPcg { uint64_t state, inc; };
+static uint32_t opcg_next(OPcg &g) {
+  const uint64_t prev = g.state;
+  g.state = prev * 6364136223846793005ull + g.inc;
+  const uint32_t shifted = (uint32_t)(((prev >> 18) ^ prev) >> 27);
+  const uint32_t r = (uint32_t)(prev >> 59);
+  return (shifted >> r) | (shifted << ((32u - r) & 31u));
+}
+static OPcg opcg_init(uint64_t initstate, uint64_t initseq) {
+  OPcg g{0ull, (initseq << 1) | 1ull};
+  opcg_next(g);
+  g.state += initstate;
+  opcg_next(g);
+  return g;
+}
+OEXPORT void oracle_set_ao_directions(int mode, uint32_t seed) {
+  g_ao_pcg = mode ? 1 : 0;
+  g_ao_seed = seed;
+}
+// n outputs of the stream (initstate, initseq), for the known-answer test
+OEXPORT void oracle_pcg32(uint64_t initstate, uint64_t initseq, int n, uint32_t *out) {
+  OPcg g = opcg_init(initstate, initseq);
+  for (int k = 0; k < n; k++) out[k] = opcg_next(g);
+}
 OEXPORT void oracle_set_occlusion_any(int on) { g_occlusion_any = on != 0; }
 OEXPORT int oracle_get_threads(void) { return nthreads(); }
 
@@ -1574,8 +1610,18 @@ OEXPORT void oracle_generateAORays(void *lp, int n, RayList_ispc *R, const int *
     oz = oz + eps * nrm.z;
     for (int j = 0; j < L.nAO; j++) {
       int off = offsets[i] + j;
-      int r = ((R->x[i] * 9949 + R->y[i] * 9613 + j * 9151) >> 8) & 0xff;
-      float r0 = g_U[r], r1 = g_V[r];
+      float r0, r1;
+      if (g_ao_pcg) {
+        // stream of the pixel, started at (seed, j); top 24 bits over 2^24
+        OPcg g = opcg_init(((uint64_t)g_ao_seed << 32) | (uint32_t)j,
+                           ((uint64_t)(uint32_t)R->y[i] << 32) | (uint32_t)R->x[i]);
+        r0 = (float)(opcg_next(g) >> 8) / 16777216.0f;
+        r1 = (float)(opcg_next(g) >> 8) / 16777216.0f;
+      } else {
+        int r = ((R->x[i] * 9949 + R->y[i] * 9613 + j * 9151) >> 8) & 0xff;
+        r0 = g_U[r];
+        r1 = g_V[r];
+      }
       float w = std::sqrt(1.f - r1);
       float x = std::cos(kTwoPi * r0) * w;
       float y = std::sin(kTwoPi * r0) * w;

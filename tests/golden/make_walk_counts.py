@@ -3,7 +3,8 @@ the sphere-mesh frame (512x512, one shadow ray per hit) and the
 nine-icosphere frame (3840x2160, 16 AO + 1 shadow rays per hit), counted by
 the CPU oracle's binned-SAH tree (leaves of at most 4 primitives) walked
 nearest first: child-box tests and primitive tests per primary, shadow and AO
-ray (occlusion rays end at their first hit, as the frame loops do).  The
+ray (occlusion rays end at their first hit, as the frame loops do; the
+nine-icosphere frame's AO directions from its seeded PCG32 streams).  The
 bench prices them at 20 flops per box test and 51 per triangle test for the
 geometry roofline.  Run from the repository root:
     python tests/golden/make_walk_counts.py"""
@@ -23,9 +24,10 @@ for p in (ROOT, os.path.dirname(HERE)):
 OUT = os.path.join(HERE, "walk_counts.json")
 
 
-def walk_counts(sc, stride):
+def walk_counts(sc, stride, ao=(0, 0)):
     """Per-ray walk work of the camera rays of every stride-th image row and of
-    the AO and shadow rays they spawn."""
+    the AO and shadow rays they spawn (AO directions: ao = (mode, seed) of
+    gxy_set_ao_directions)."""
     from galaxy_amd import abi, scenes
     from oracle_backend import OracleBackend
     ob = OracleBackend()
@@ -44,6 +46,7 @@ def walk_counts(sc, stride):
         return dict(rays=int(cnt[2]), box_tests=int(cnt[0]), prim_tests=int(cnt[1]))
 
     L.oracle_set_occlusion_any(1)
+    L.oracle_set_ao_directions(*ao)
     try:
         o = sub.copy()
         out = {"stride": stride, "primary": traced(o)}
@@ -66,19 +69,21 @@ def walk_counts(sc, stride):
             out["shadow"] = traced(kids.subset(np.arange(na.value, nout)))
     finally:
         L.oracle_set_occlusion_any(0)
+        L.oracle_set_ao_directions(0, 0)
     return out
 
 
 def scenes_to_count():
     from galaxy_amd import workloads
-    return {"sphere_mesh_512": (lambda: workloads.sphere_mesh(512, 512)[0], 1),
-            "nine_icospheres_4k_ao16": (lambda: workloads.nine_icospheres()[0], 16)}
+    return {"sphere_mesh_512": (lambda: workloads.sphere_mesh(512, 512)[0], 1, (0, 0)),
+            "nine_icospheres_4k_ao16": (lambda: workloads.nine_icospheres()[0], 16,
+                                        (1, workloads.NINE_ICOSPHERES_AO_SEED))}
 
 
 def main():
     res = {}
-    for name, (make, stride) in scenes_to_count().items():
-        res[name] = walk_counts(make(), stride)
+    for name, (make, stride, ao) in scenes_to_count().items():
+        res[name] = walk_counts(make(), stride, ao)
         print(name, json.dumps(res[name]), flush=True)
     with open(OUT, "w") as f:
         json.dump(res, f, indent=1, sort_keys=True)

@@ -93,6 +93,8 @@ def lib():
         L.oracle_set_opacity_threshold.argtypes = [C.c_float]
         L.oracle_plant_fault.argtypes = [C.c_int]
         L.oracle_set_reverse_lighting.argtypes = [C.c_int]
+        L.oracle_set_ao_directions.argtypes = [C.c_int, C.c_uint32]
+        L.oracle_pcg32.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_void_p]
         L.oracle_walk_counters.argtypes = [C.c_int, vp]
         L.oracle_set_occlusion_any.argtypes = [C.c_int]
         L.oracle_get_threads.restype = C.c_int

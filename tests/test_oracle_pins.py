@@ -363,8 +363,8 @@ def test_walk_counts_fixture_reproduces():
     sys.path.insert(0, os.path.join(GOLDEN))
     import make_walk_counts as mw
     ref = json.load(open(os.path.join(GOLDEN, "walk_counts.json")))
-    make, stride = mw.scenes_to_count()["sphere_mesh_512"]
-    assert mw.walk_counts(make(), stride) == ref["sphere_mesh_512"]
+    make, stride, ao = mw.scenes_to_count()["sphere_mesh_512"]
+    assert mw.walk_counts(make(), stride, ao) == ref["sphere_mesh_512"]
     # every recorded ray kind did some work: a box test per walk at least, no more primitive tests than boxes
     for scene in ref.values():
         for kind in ("primary", "shadow", "ao"):
