@@ -1753,8 +1753,12 @@ GXY_EXPORT void TraceRays_destroy(void *self) {
   delete t;
 }
 
+// The reference's entry points return nothing; their errors are reported by
+// gxy_last_error(), which each of them clears first, so that an earlier
+// failure elsewhere is not taken for theirs.
 GXY_EXPORT void *TraceRays_TraceRays(void *self, void *vis, int nRaysIn, void *raysIn, float epsilon) {
   (void)self;
+  t_err.clear();
   if (ensure_init() != 0) return nullptr;
   TraceParams P;
   if (make_trace_params((const Visualization_ispc *)vis, epsilon, P) != 0) return nullptr;
@@ -1764,6 +1768,7 @@ GXY_EXPORT void *TraceRays_TraceRays(void *self, void *vis, int nRaysIn, void *r
 
 static void lighting_entry(void *lights, int n, void *raysIn, int *off, void *raysOut, float eps, int mode,
                            const char *name) {
+  t_err.clear();
   if (ensure_init() != 0) return;
   LightParams L;
   if (make_light_params((const Lighting_ispc *)lights, eps, L) != 0) return;
@@ -1796,6 +1801,7 @@ GXY_EXPORT void TraceRays_diffuseLighting(void *self, void *lights, int n, void 
 
 GXY_EXPORT void *SamplerTraceRays_SamplerTraceRays(void *self, void *vis, int n, void *raysIn) {
   (void)self;
+  t_err.clear();
   if (ensure_init() != 0) return nullptr;
   const Visualization_ispc *V = (const Visualization_ispc *)vis;
   if (!V || V->nVolumeVis < 1) return nullptr;  // SamplerTraceRays.ispc:139

@@ -131,7 +131,9 @@ void  TraceRays_destroy(void *self);
 /* TraceRays.ispc:326-623.  Traces nRaysIn device-resident rays in place:
  * box clip, slices, geometry (BVH), volume march with TF compositing and
  * isosurfaces; writes r g b o, t, term and, on a surface hit, sr sg sb so and
- * nx ny nz.  Returns NULL like the reference. */
+ * nx ny nz.  Returns NULL like the reference; errors are reported by
+ * gxy_last_error(), which this and the other TraceRays_* / SamplerTraceRays_*
+ * entry points clear on entry (empty after a call that succeeded). */
 void *TraceRays_TraceRays(void *self, void *vis, int nRaysIn, void *raysIn, float epsilon);
 
 /* TraceRays.ispc:625-733.  offsets: device int array, one per input ray. */

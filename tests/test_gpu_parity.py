@@ -281,8 +281,10 @@ def test_nine_icosphere_frame_under_every_geometry_setting(hip, oracle):
 
 
 def test_mesh_frame_with_shadows_matches_oracle(hip, oracle):
+    """The sphere-mesh frame at its configured 512x512 (20,480 triangles, one
+    point light with shadows)."""
     from galaxy_amd import workloads
-    sc, alpha = workloads.sphere_mesh(256, 256)
+    sc, alpha = workloads.sphere_mesh(512, 512)
     c = sc.cameras[0]
     fb_g, st_g = hip.render_frame(scenes.build_scene(hip, sc), c)
     fb_o, st_o = oracle.render_frame(scenes.build_scene(oracle, sc), c)
@@ -515,3 +517,16 @@ def test_occlusion_rays_give_the_same_frames(hip):
     (fb0, st0), (fb1, st1) = out
     assert st0.as_dict() == st1.as_dict() and st0.secondary_rays > 100000
     assert np.abs(fb0 - fb1).max() <= 1e-6
+
+
+def test_void_entry_points_clear_the_last_error(radial_vols_small, hip):
+    """TraceRays_TraceRays returns nothing (as the reference's export): a call
+    that succeeds leaves gxy_last_error() empty even after another call failed."""
+    from galaxy_amd import lib as glib
+    L_ = glib.lib()
+    sc = _dvr_scene(radial_vols_small, 1)
+    Bh = scenes.build_scene(hip, sc)
+    assert L_.gxy_tf_destroy(C.c_void_p(16)) < 0                  # not a handle: "not a TF handle"
+    assert L_.gxy_last_error().decode() != ""
+    d = glib.DeviceRays.from_host(random_rays(1000, seed=3))
+    assert hip.trace(Bh.vis, d) == ""
