@@ -562,8 +562,11 @@ def main():
     L.gxy_get_trace_config(*[C.byref(x) for x in kcfg])
     lay = L.gxy_volume_get_layout(built.handles[0])
     sm = lay if lay else 1
-    roof["kernel"] = {0: "k_trace<1,false,0,%d>" % sm, 1: "k_trace_pl<1>", 15: "k_trace<1,false,5,%d>" % sm,
-                      16: "k_trace<1,false,6,%d>" % sm}.get(kcfg[0].value, str(kcfg[0].value))
+    # the variant the launcher picks: one volume, bricked (32-bit offsets under 4 GiB), one isovalue
+    mi = ",0,0,1" if lay and len(sc.volume_vis[0].isovalues) <= 1 else ""
+    roof["kernel"] = {0: "k_trace<1,false,0,%d%s>" % (sm, mi), 1: "k_trace_pl<1>",
+                      15: "k_trace<1,false,5,%d%s>" % (sm, mi),
+                      16: "k_trace<1,false,6,%d%s>" % (sm, mi)}.get(kcfg[0].value, str(kcfg[0].value))
     roof["volume_layout"] = lay
     roof["kernel_ms_per_launch"] = round(tr_ms / max(1, tr_n), 4)
     roof["alg_bytes_per_launch"] = int(alg_bytes / max(1, tr_n))

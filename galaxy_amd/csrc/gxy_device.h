@@ -217,8 +217,14 @@ struct VFetch {
 template <int B, bool BIG>
 __device__ __forceinline__ void brick_fetch(const DVolume &V, int i0, int i1, int i2, VFetch &F) {
   constexpr int LB = B == 4 ? 2 : 3, E = B + 1, PL = E * E;
-  const unsigned brick = (unsigned)(i0 >> LB) + __umul24((unsigned)(i1 >> LB), (unsigned)V.nb[0]) +
-                         __umul24((unsigned)(i2 >> LB), (unsigned)V.nb[0] * (unsigned)V.nb[1]);
+  unsigned ty = __umul24((unsigned)(i1 >> LB), (unsigned)V.nb[0]);
+  unsigned tz = __umul24((unsigned)(i2 >> LB), (unsigned)V.nb[0] * (unsigned)V.nb[1]);
+  // (an empty statement that keeps the two products as they are: the
+  // compiler would otherwise merge them with the sums into 64-bit
+  // multiply-adds, which issue at a quarter of the rate of the 24-bit
+  // multiplies and the three-input add)
+  asm("" : "+v"(ty), "+v"(tz));
+  const unsigned brick = (unsigned)(i0 >> LB) + ty + tz;
   const unsigned loc = (unsigned)(i0 & (B - 1)) + (unsigned)(i1 & (B - 1)) * E + (unsigned)(i2 & (B - 1)) * PL;
   if (!BIG) {
     const unsigned off = __umul24(brick, (unsigned)(4 * brick_floats(B))) + 4u * loc;

@@ -56,7 +56,9 @@ hipError_t launch_reduce_add(const unsigned long long *part, int nb, unsigned lo
 // STK > 0: the BVH traversal stack holds STK entries per work-item in LDS
 // (instead of 64 in private memory).
 // W: BVH walked by geometry variants (2 binary, 4 four-wide, 0 per model).
-template <int NV, bool GEOM, int WPE, int SM, int STK = 0, int W = 0>
+// MI: isovalues per volume the iso test is unrolled for (1 when no volume
+// has more than one, fewer scalar branches per sample).
+template <int NV, bool GEOM, int WPE, int SM, int STK = 0, int W = 0, int MI = GXY_MAX_ISO>
 __global__ __launch_bounds__(GXY_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, 8)))
 void k_trace(TraceParams P, RayList_ispc R, int n) {
   constexpr int NVS = NV > 0 ? NV : 1;
@@ -76,10 +78,10 @@ void k_trace(TraceParams P, RayList_ispc R, int n) {
     LdsStack<(STK > 0 ? STK : 1)> stack;
     stack.base = &s_stack[0][threadIdx.x];
     stack.stride = GXY_BLOCK;
-    if (i < n) nsamp = trace_one<NV, GEOM, LdsStack<(STK > 0 ? STK : 1)>, SM, W>(P, R, i, s_col, s_op, stack);
+    if (i < n) nsamp = trace_one<NV, GEOM, LdsStack<(STK > 0 ? STK : 1)>, SM, W, MI>(P, R, i, s_col, s_op, stack);
   } else {
     PrivStack<GXY_BVH_STACK> stack;
-    if (i < n) nsamp = trace_one<NV, GEOM, PrivStack<GXY_BVH_STACK>, SM, W>(P, R, i, s_col, s_op, stack);
+    if (i < n) nsamp = trace_one<NV, GEOM, PrivStack<GXY_BVH_STACK>, SM, W, MI>(P, R, i, s_col, s_op, stack);
   }
   if (P.sample_partials) block_sum_store(nsamp, P.sample_partials);
 }
@@ -377,8 +379,12 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_sampler(TraceParams P, RayList_is
 template <int SM>
 static void launch_trace_sm(const TraceParams &P, const RayList_ispc &R, int n, int wpe, dim3 g, dim3 b,
                             hipStream_t s) {
-  if (wpe == 5) hipLaunchKernelGGL((k_trace<1, false, 5, SM>), g, b, 0, s, P, R, n);
+  const bool one_iso = P.vv[0].nIso <= 1;
+  if (wpe == 5 && one_iso) hipLaunchKernelGGL((k_trace<1, false, 5, SM, 0, 0, 1>), g, b, 0, s, P, R, n);
+  else if (wpe == 5) hipLaunchKernelGGL((k_trace<1, false, 5, SM>), g, b, 0, s, P, R, n);
+  else if (wpe == 6 && one_iso) hipLaunchKernelGGL((k_trace<1, false, 6, SM, 0, 0, 1>), g, b, 0, s, P, R, n);
   else if (wpe == 6) hipLaunchKernelGGL((k_trace<1, false, 6, SM>), g, b, 0, s, P, R, n);
+  else if (one_iso) hipLaunchKernelGGL((k_trace<1, false, 0, SM, 0, 0, 1>), g, b, 0, s, P, R, n);
   else hipLaunchKernelGGL((k_trace<1, false, 0, SM>), g, b, 0, s, P, R, n);
 }
 static void launch_trace_bricked(const TraceParams &P, const RayList_ispc &R, int n, int wpe, int sm, dim3 g, dim3 b,

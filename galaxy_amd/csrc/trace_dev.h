@@ -557,7 +557,9 @@ __device__ __forceinline__ void composite(float &cr, float &cg, float &cb, float
 // One ray of TraceRays_TraceRays (TraceRays.ispc:371-622): reads ray i of R,
 // writes its colour, t, term and surface fields back.  s_col / s_op hold the
 // NV transfer-function tables (LDS).  Returns the number of volume samples.
-template <int NV, bool GEOM, class Stack, int SM = -1, int W = 0>
+// MI: the isovalues per volume the iso test is unrolled for (the volume's
+// nIso must not exceed it).
+template <int NV, bool GEOM, class Stack, int SM = -1, int W = 0, int MI = GXY_MAX_ISO>
 __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayList_ispc &R, const int i,
                                               const float (*s_col)[3 * GXY_TF_MAX],
                                               const float (*s_op)[GXY_TF_MAX], Stack &stack) {
@@ -682,7 +684,7 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
             const int ni = P.vv[major].nIso;
             // unrolled to the fixed maximum: the isovalues stay in registers
 #pragma unroll
-            for (int m = 0; m < GXY_MAX_ISO; m++) {
+            for (int m = 0; m < MI; m++) {
               if (m < ni) {
                 float isoval = P.vv[major].iso[m];
                 if (((isoval >= sl) && (isoval < st)) || ((isoval <= sl) && (isoval > st))) {
