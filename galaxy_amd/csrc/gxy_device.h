@@ -171,7 +171,8 @@ __device__ __forceinline__ f3 cross3(f3 a, f3 b) {
 }
 __device__ __forceinline__ float mnf(float a, float b) { return a < b ? a : b; }
 __device__ __forceinline__ float mxf(float a, float b) { return a > b ? a : b; }
-__device__ __forceinline__ float clamp01f(float v) { return v < 0.f ? 0.f : (v > 1.f ? 1.f : v); }
+// [0, 1] clamp of a number (v_med3_f32; the march's opacities are never NaN)
+__device__ __forceinline__ float clamp01f(float v) { return __builtin_amdgcn_fmed3f(v, 0.f, 1.f); }
 __device__ __forceinline__ f3 normalize3(f3 v) { float s = 1.0f / sqrtf(dot3(v, v)); return scl3(s, v); }
 __device__ __forceinline__ f3 safe_normalize3(f3 v) {
   float d = dot3(v, v);
@@ -409,25 +410,27 @@ __device__ __forceinline__ f3 vol_gradient(const DVolume &V, f3 p) {
 // (one v_med3_f32), which gives the end entries outside [lo, hi]; entry
 // i = min(floor(x), n-2) and its successor are blended with one FMA per
 // channel (at x = n-1 the blend reaches the last entry within an ulp).
+// Tables hold two entries at least (an empty or one-entry table is stored as
+// two equal entries at scale 0: the same value everywhere).  A NaN value
+// looks up zero colour and zero opacity, selected at the end rather than
+// branched on.
 __device__ __forceinline__ f3 tf_color(const float *col, int n, float lo, float hi, float sc, float v) {
-  if (v != v) return mk3(0.f, 0.f, 0.f);
-  if (n == 0) return mk3(1.f, 1.f, 1.f);
-  if (n == 1) return mk3(col[0], col[1], col[2]);
-  const float x = __builtin_amdgcn_fmed3f((v - lo) * sc, 0.f, (float)(n - 1));
+  const bool nan = v != v;
+  const float x = nan ? 0.f : __builtin_amdgcn_fmed3f((v - lo) * sc, 0.f, (float)(n - 1));
   int i = (int)x;
   i = i < n - 2 ? i : n - 2;
   const float r = x - (float)i;
-  const float *c = col + 3 * i;
-  return mk3(lerp_fma(c[0], c[3], r), lerp_fma(c[1], c[4], r), lerp_fma(c[2], c[5], r));
+  const float *c = col + __umul24((unsigned)i, 3u);
+  return mk3(nan ? 0.f : lerp_fma(c[0], c[3], r), nan ? 0.f : lerp_fma(c[1], c[4], r),
+             nan ? 0.f : lerp_fma(c[2], c[5], r));
 }
 __device__ __forceinline__ float tf_opacity(const float *op, int n, float lo, float hi, float so, float v) {
-  if (v != v) return 0.f;
-  if (n == 0) return 1.f;
-  if (n == 1) return op[0];
-  const float x = __builtin_amdgcn_fmed3f((v - lo) * so, 0.f, (float)(n - 1));
+  const bool nan = v != v;
+  const float x = nan ? 0.f : __builtin_amdgcn_fmed3f((v - lo) * so, 0.f, (float)(n - 1));
   int i = (int)x;
   i = i < n - 2 ? i : n - 2;
-  return lerp_fma(op[i], op[i + 1], x - (float)i);
+  const float o = lerp_fma(op[i], op[i + 1], x - (float)i);
+  return nan ? 0.f : o;
 }
 __device__ __forceinline__ f3 tf_color(const float *col, const DTF &T, float v) {
   return tf_color(col, T.nc, T.lo, T.hi, T.sc, v);

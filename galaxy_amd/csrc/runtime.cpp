@@ -1052,23 +1052,33 @@ static void *tf_make(std::vector<float> col, std::vector<float> op, float lo, fl
   TFObj *t = new TFObj;
   t->hcol = std::move(col);
   t->hop = std::move(op);
-  t->col = (float *)dalloc(std::max<size_t>(4, t->hcol.size() * 4));
-  t->op = (float *)dalloc(std::max<size_t>(4, t->hop.size() * 4));
+  // The device lookups read two entries at least: no colours is white
+  // everywhere, no opacities opaque, and a one-entry table is that entry
+  // everywhere (two equal entries at scale 0), so the lookups need no cases.
+  std::vector<float> dcol = t->hcol, dop = t->hop;
+  if (dcol.empty()) dcol = {1.f, 1.f, 1.f};
+  if (dcol.size() == 3) dcol.insert(dcol.end(), {dcol[0], dcol[1], dcol[2]});
+  if (dop.empty()) dop = {1.f};
+  if (dop.size() == 1) dop.push_back(dop[0]);
+  t->col = (float *)dalloc(dcol.size() * 4);
+  t->op = (float *)dalloc(dop.size() * 4);
   if (!t->col || !t->op) {
+    dfree(t->col);
+    dfree(t->op);
     delete t;
     return nullptr;
   }
-  if (!t->hcol.empty()) hipMemcpyAsync(t->col, t->hcol.data(), t->hcol.size() * 4, hipMemcpyHostToDevice, S());
-  if (!t->hop.empty()) hipMemcpyAsync(t->op, t->hop.data(), t->hop.size() * 4, hipMemcpyHostToDevice, S());
+  hipMemcpyAsync(t->col, dcol.data(), dcol.size() * 4, hipMemcpyHostToDevice, S());
+  hipMemcpyAsync(t->op, dop.data(), dop.size() * 4, hipMemcpyHostToDevice, S());
   hipStreamSynchronize(S());
   t->d.col = t->col;
   t->d.op = t->op;
-  t->d.nc = (int)t->hcol.size() / 3;
-  t->d.no = (int)t->hop.size();
+  t->d.nc = (int)dcol.size() / 3;
+  t->d.no = (int)dop.size();
   t->d.lo = lo;
   t->d.hi = hi;
-  t->d.sc = t->d.nc > 1 ? (float)(t->d.nc - 1) / (hi - lo) : 0.f;
-  t->d.so = t->d.no > 1 ? (float)(t->d.no - 1) / (hi - lo) : 0.f;
+  t->d.sc = t->hcol.size() / 3 > 1 ? (float)(t->d.nc - 1) / (hi - lo) : 0.f;
+  t->d.so = t->hop.size() > 1 ? (float)(t->d.no - 1) / (hi - lo) : 0.f;
   std::lock_guard<std::recursive_mutex> lk(G.mu);
   G.tfs.insert(t);
   return t;
