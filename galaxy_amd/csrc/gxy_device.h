@@ -60,6 +60,7 @@ struct DTF {
   int nc, no;
   float lo, hi;
   float sc, so;   // (nc-1)/(hi-lo), (no-1)/(hi-lo)
+  float xc, xo;   // nc-1, no-1 as floats (the clamp bounds of the table coordinate)
 };
 
 struct DVolVis {
@@ -313,9 +314,11 @@ __device__ __forceinline__ VFetch vol_fetch_vox(const DVolume &V, float l0, floa
   l2 = __builtin_amdgcn_fmed3f(l2, 0.f, V.ub[2]);
   const int i0 = (int)l0, i1 = (int)l1, i2 = (int)l2;
   VFetch F;
-  F.f0 = l0 - (float)i0;
-  F.f1 = l1 - (float)i1;
-  F.f2 = l2 - (float)i2;
+  // l - floor(l) in one v_fract_f32: exact, and equal to l - (float)i for the
+  // clamped l >= 0
+  F.f0 = __builtin_amdgcn_fractf(l0);
+  F.f1 = __builtin_amdgcn_fractf(l1);
+  F.f2 = __builtin_amdgcn_fractf(l2);
   if (sm_bricked(SM)) {
     brick_fetch<sm_brick(SM), sm_big(SM)>(V, i0, i1, i2, F);
     return F;
@@ -414,26 +417,33 @@ __device__ __forceinline__ f3 vol_gradient(const DVolume &V, f3 p) {
 // two equal entries at scale 0: the same value everywhere).  A NaN value
 // looks up zero colour and zero opacity, selected at the end rather than
 // branched on.
-__device__ __forceinline__ f3 tf_color(const float *col, int n, float lo, float hi, float sc, float v) {
+__device__ __forceinline__ f3 tf_color(const float *col, const DTF &T, float v) {
   const bool nan = v != v;
-  const float x = nan ? 0.f : __builtin_amdgcn_fmed3f((v - lo) * sc, 0.f, (float)(n - 1));
+  const float x = nan ? 0.f : __builtin_amdgcn_fmed3f((v - T.lo) * T.sc, 0.f, T.xc);
   int i = (int)x;
-  i = i < n - 2 ? i : n - 2;
+  i = i < T.nc - 2 ? i : T.nc - 2;
   const float r = x - (float)i;
   const float *c = col + __umul24((unsigned)i, 3u);
   return mk3(nan ? 0.f : lerp_fma(c[0], c[3], r), nan ? 0.f : lerp_fma(c[1], c[4], r),
              nan ? 0.f : lerp_fma(c[2], c[5], r));
 }
-__device__ __forceinline__ float tf_opacity(const float *op, int n, float lo, float hi, float so, float v) {
+__device__ __forceinline__ float tf_opacity(const float *op, const DTF &T, float v) {
   const bool nan = v != v;
-  const float x = nan ? 0.f : __builtin_amdgcn_fmed3f((v - lo) * so, 0.f, (float)(n - 1));
+  const float x = nan ? 0.f : __builtin_amdgcn_fmed3f((v - T.lo) * T.so, 0.f, T.xo);
   int i = (int)x;
-  i = i < n - 2 ? i : n - 2;
+  i = i < T.no - 2 ? i : T.no - 2;
   const float o = lerp_fma(op[i], op[i + 1], x - (float)i);
   return nan ? 0.f : o;
 }
-__device__ __forceinline__ f3 tf_color(const float *col, const DTF &T, float v) {
-  return tf_color(col, T.nc, T.lo, T.hi, T.sc, v);
+// The colour lookup of a value known not to be NaN (the march composites a
+// sample only when its opacity lookup is positive, which a NaN's is not).
+__device__ __forceinline__ f3 tf_color_num(const float *col, const DTF &T, float v) {
+  const float x = __builtin_amdgcn_fmed3f((v - T.lo) * T.sc, 0.f, T.xc);
+  int i = (int)x;
+  i = i < T.nc - 2 ? i : T.nc - 2;
+  const float r = x - (float)i;
+  const float *c = col + __umul24((unsigned)i, 3u);
+  return mk3(lerp_fma(c[0], c[3], r), lerp_fma(c[1], c[4], r), lerp_fma(c[2], c[5], r));
 }
 
 // MyIntersectBox (TraceRays.ispc:90-106), rcp as 1/x.

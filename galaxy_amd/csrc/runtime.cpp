@@ -555,6 +555,7 @@ static int make_trace_params(const Visualization_ispc *vis, float global_eps, Tr
     P.blo[k] = vis->local_bb.lower[k];
     P.bhi[k] = vis->local_bb.upper[k];
   }
+  if (P.step != P.step) return fail("the volume step is NaN (check the volumes' sampling rates)");
   P.istep = P.step > 0 ? 1.0f / P.step : 0.f;
   if (vis->model) {
     ModelObj *M = lookup<ModelObj>(G.models, vis->model);
@@ -1079,6 +1080,8 @@ static void *tf_make(std::vector<float> col, std::vector<float> op, float lo, fl
   t->d.hi = hi;
   t->d.sc = t->hcol.size() / 3 > 1 ? (float)(t->d.nc - 1) / (hi - lo) : 0.f;
   t->d.so = t->hop.size() > 1 ? (float)(t->d.no - 1) / (hi - lo) : 0.f;
+  t->d.xc = (float)(t->d.nc - 1);
+  t->d.xo = (float)(t->d.no - 1);
   std::lock_guard<std::recursive_mutex> lk(G.mu);
   G.tfs.insert(t);
   return t;

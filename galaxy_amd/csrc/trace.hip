@@ -269,10 +269,10 @@ __device__ __forceinline__ bool pl_step(const TraceParams &P, MarchState &m, flo
       if (P.vv[major].dvr) {
         const DTF &T = P.vv[major].tf;
         float sVolume = (sLast[major] + sThis[major]) / 2;
-        float sop = tf_opacity(s_op[major], T.no, T.lo, T.hi, T.so, sVolume);
+        float sop = tf_opacity(s_op[major], T, sVolume);
         if (sop > 0) {
           if (m.flags & 1) {
-            f3 sc = tf_color(s_col[major], T, sVolume);
+            f3 sc = tf_color_num(s_col[major], T, sVolume);
             composite(m.cr, m.cg, m.cb, m.ca, sc, clamp01f(sop * P.vv[major].vol.irate));
           } else {
             float wop = ((tThis - m.tLast) * P.istep) * clamp01f(sop * P.vv[major].vol.irate);

@@ -662,19 +662,44 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
       VFetch F[NVS];
 #pragma unroll
       for (int v = 0; v < NV; v++) F[v] = vox_fetch<SM>(P.vv[v].vol, VR[v], tThis);
-      while (tThis <= tTermination && !opaque && !hit_iso && iter++ < GXY_MAX_MARCH) {
+      // The first sample (at tEntry) ends no interval: it only becomes sLast,
+      // and the next position is tEntry + epsilon.  Taken before the loop, it
+      // leaves every pass of the loop with tThis > tEntry (t strictly grows,
+      // or the march ends).
+      bool go = tThis <= tTermination && !opaque;
+      iter = 1;
+      // -inf held in a register the compiler cannot see into, so that the
+      // three-way median below stays one v_med3_f32 (folded to a minimum it
+      // would gain a NaN-quieting v_max_f32 per step)
+      float ninf = -__builtin_inff();
+      asm("" : "+s"(ninf));
+      if (go) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) sLast[v] = vol_finish_fma(F[v]);
+        nsamp += NV;
+        const float nx0 = tEntry + epsilon;
+#pragma unroll
+        for (int v = 0; v < NV; v++) F[v] = vox_fetch<SM>(P.vv[v].vol, VR[v], nx0);
+        tLast = tThis;
+        go = nx0 > tThis;
+        tThis = nx0;
+      }
+      while (go && tThis <= tTermination && !opaque && !hit_iso && iter++ < GXY_MAX_MARCH) {
 #pragma unroll
         for (int v = 0; v < NV; v++) sThis[v] = vol_finish_fma(F[v]);
         nsamp += NV;
         // the next position when neither an iso hit nor opacity ends the march
         // here (the only cases in which the loop continues, and then
-        // tTermination is still the value used here)
-        const float nx0 = (tThis == tEntry) ? (tEntry + epsilon)
-                          : (((tThis + step) > tTermination) && (tThis < tTermination)) ? tTermination
-                                                                                        : tThis + step;
+        // tTermination is still the value used here): tThis + step, cut at
+        // tTermination (the median of the two and -inf is their minimum, in
+        // one v_med3_f32).  (The reference steps past tTermination when tThis is
+        // already there; the march ends then either way, with the same
+        // samples, and the min ends it by the no-advance test below.  The
+        // step is a number: the host rejects a NaN step.)
+        const float nx0 = __builtin_amdgcn_fmed3f(tThis + step, tTermination, ninf);
 #pragma unroll
         for (int v = 0; v < NV; v++) F[v] = vox_fetch<SM>(P.vv[v].vol, VR[v], nx0);
-        if (tThis > tEntry && tLast >= epsilon) {
+        if (tLast >= epsilon) {
           // LookForIsoHit: the last matching isovalue in list order wins
           bool ih = false;
           int ivid = -1;
@@ -713,23 +738,25 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
             if (P.vv[major].dvr) {
               const DTF &T = P.vv[major].tf;
               float sVolume = (sLast[major] + sThis[major]) / 2;
-              float sop = tf_opacity(s_op[major], T.no, T.lo, T.hi, T.so, sVolume);
+              float sop = tf_opacity(s_op[major], T, sVolume);
               if (sop > 0) {
                 if (shadeFlag) {
-                  f3 sc = tf_color(s_col[major], T, sVolume);
+                  f3 sc = tf_color_num(s_col[major], T, sVolume);
                   composite(cr, cg, cb, ca, sc, clamp01f(sop * P.vv[major].vol.irate));
                 } else {
                   float wop = ((tThis - tLast) * P.istep) * clamp01f(sop * P.vv[major].vol.irate);
                   float f = 1.0f - wop;
                   cr *= f; cg *= f; cb *= f; ca *= f;
                 }
+                // only a sample that changes the colour can make the ray
+                // opaque (it was not when this pass began)
+                opaque = opaque_of(cr, cg, cb, ca, P.opaque_alpha);
               }
             }
           }
         }
 #pragma unroll
         for (int v = 0; v < NV; v++) sLast[v] = sThis[v];
-        opaque = opaque_of(cr, cg, cb, ca, P.opaque_alpha);
         if (opaque) tTermination = tThis;
         tLast = tThis;
         // a step that no longer advances t ends the march (the reference would spin)
