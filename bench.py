@@ -564,9 +564,11 @@ def main():
     sm = lay if lay else 1
     # the variant the launcher picks: one volume, bricked (32-bit offsets under 4 GiB), one isovalue
     mi = ",0,0,1" if lay and len(sc.volume_vis[0].isovalues) <= 1 else ""
-    roof["kernel"] = {0: "k_trace<1,false,0,%d%s>" % (sm, mi), 1: "k_trace_pl<1>",
-                      15: "k_trace<1,false,5,%d%s>" % (sm, mi),
-                      16: "k_trace<1,false,6,%d%s>" % (sm, mi)}.get(kcfg[0].value, str(kcfg[0].value))
+    k = kcfg[0].value
+    w = k - 10 if k >= 10 else 0
+    if w > 6 and not mi:
+        w = 6                               # 17 / 18 act as 16 beyond one bricked volume and isovalue
+    roof["kernel"] = "k_trace_pl<1>" if k == 1 else "k_trace<1,false,%d,%d%s>" % (w, sm, mi)
     roof["volume_layout"] = lay
     roof["kernel_ms_per_launch"] = round(tr_ms / max(1, tr_n), 4)
     roof["alg_bytes_per_launch"] = int(alg_bytes / max(1, tr_n))

@@ -44,7 +44,8 @@ struct DVolume {
   float irate;    // 1 / samplingRate
   int small;      // <= 2^30 voxels, x-y plane and z extent < 2^24: 32-bit byte offsets
   int brick;      // 0, or B = 4 / 8: a bricked copy of the voxels is in bdata
-  int bsmall;     // the bricked copy spans under 4 GiB: 32-bit byte offsets
+  int bsmall;     // the bricked copy spans under 4 GiB and a plane of bricks holds
+                  // under 2^24 floats: 32-bit byte offsets from 24-bit multiplies
   int nb[3];      // bricks per axis: ceil((n - 1) / B)
   const float *bdata;  // bricks of (B+1)^3 voxels (one-voxel overlap), x fastest
                        // inside a brick, each padded to a multiple of 128 bytes
@@ -213,12 +214,36 @@ struct VFetch {
 };
 
 // The four x-pair loads of cell (i0, i1, i2) from the bricked copy (B = 4 or
-// 8).  BIG = false when the whole copy spans under 4 GiB (DVolume::bsmall):
-// the byte offset is formed in 32 bits (24-bit multiplies) and the loads take
-// it beside the uniform base address; BIG = true forms a 64-bit address.
+// 8).  BIG = false when the whole copy spans under 4 GiB and a plane of
+// bricks holds under 2^24 floats (DVolume::bsmall): the byte offset is formed
+// in 32 bits (24-bit multiply-adds) and the loads take it beside the uniform
+// base address; BIG = true forms a 64-bit address.
 template <int B, bool BIG>
 __device__ __forceinline__ void brick_fetch(const DVolume &V, int i0, int i1, int i2, VFetch &F) {
   constexpr int LB = B == 4 ? 2 : 3, E = B + 1, PL = E * E;
+  if (!BIG) {
+    // The float offset BF * brick + loc, with brick = a0 + nb0 a1 + nb0 nb1 a2
+    // (a = i >> log2 B), loc = b0 + E b1 + E^2 b2 and b = i - B a, is
+    //   i0 + (BF - B) a0 + E i1 + (BF nb0 - E B) a1 + E^2 i2 + (BF nb0 nb1 - E^2 B) a2:
+    // one chain of 24-bit multiply-adds with no masks (every factor is under
+    // 2^24: DVolume::bsmall requires BF nb0 nb1 < 2^24), scaled to bytes at
+    // the end (the copy spans under 4 GiB).
+    constexpr unsigned BF = brick_floats(B);
+    const unsigned k1 = BF * (unsigned)V.nb[0] - E * B;
+    const unsigned k2 = BF * (unsigned)V.nb[0] * (unsigned)V.nb[1] - PL * B;
+    const unsigned a0 = (unsigned)i0 >> LB, a1 = (unsigned)i1 >> LB, a2 = (unsigned)i2 >> LB;
+    unsigned w = __umul24(a0, BF - B) + (unsigned)i0;
+    w = __umul24((unsigned)i1, (unsigned)E) + w;
+    w = __umul24(a1, k1) + w;
+    w = __umul24((unsigned)i2, (unsigned)PL) + w;
+    w = __umul24(a2, k2) + w;
+    const char *d = (const char *)gptr(V.bdata) + (size_t)(w * 4u);
+    F.p00 = *(const float2u *)(d);
+    F.p10 = *(const float2u *)(d + 4 * E);
+    F.p01 = *(const float2u *)(d + 4 * PL);
+    F.p11 = *(const float2u *)(d + 4 * (PL + E));
+    return;
+  }
   unsigned ty = __umul24((unsigned)(i1 >> LB), (unsigned)V.nb[0]);
   unsigned tz = __umul24((unsigned)(i2 >> LB), (unsigned)V.nb[0] * (unsigned)V.nb[1]);
   // (an empty statement that keeps the two products as they are: the
@@ -228,20 +253,11 @@ __device__ __forceinline__ void brick_fetch(const DVolume &V, int i0, int i1, in
   asm("" : "+v"(ty), "+v"(tz));
   const unsigned brick = (unsigned)(i0 >> LB) + ty + tz;
   const unsigned loc = (unsigned)(i0 & (B - 1)) + (unsigned)(i1 & (B - 1)) * E + (unsigned)(i2 & (B - 1)) * PL;
-  if (!BIG) {
-    const unsigned off = __umul24(brick, (unsigned)(4 * brick_floats(B))) + 4u * loc;
-    const char *d = (const char *)gptr(V.bdata) + (size_t)off;
-    F.p00 = *(const float2u *)(d);
-    F.p10 = *(const float2u *)(d + 4 * E);
-    F.p01 = *(const float2u *)(d + 4 * PL);
-    F.p11 = *(const float2u *)(d + 4 * (PL + E));
-  } else {
-    const float *d = gptr(V.bdata) + ((size_t)brick * brick_floats(B) + loc);
-    F.p00 = *(const float2u *)(d);
-    F.p10 = *(const float2u *)(d + E);
-    F.p01 = *(const float2u *)(d + PL);
-    F.p11 = *(const float2u *)(d + PL + E);
-  }
+  const float *d = gptr(V.bdata) + ((size_t)brick * brick_floats(B) + loc);
+  F.p00 = *(const float2u *)(d);
+  F.p10 = *(const float2u *)(d + E);
+  F.p01 = *(const float2u *)(d + PL);
+  F.p11 = *(const float2u *)(d + PL + E);
 }
 
 // Voxel addressing variants (SM): 4 / 8 the bricked copy with 32-bit offsets,
@@ -429,7 +445,9 @@ __device__ __forceinline__ f3 tf_color(const float *col, const DTF &T, float v) 
 }
 __device__ __forceinline__ float tf_opacity(const float *op, const DTF &T, float v) {
   const bool nan = v != v;
-  const float x = nan ? 0.f : __builtin_amdgcn_fmed3f((v - T.lo) * T.so, 0.f, T.xo);
+  // (a NaN v reads entry 0, v_cvt_i32_f32 mapping NaN to 0, and the result
+  // is replaced by zero)
+  const float x = __builtin_amdgcn_fmed3f((v - T.lo) * T.so, 0.f, T.xo);
   int i = (int)x;
   i = i < T.no - 2 ? i : T.no - 2;
   const float o = lerp_fma(op[i], op[i + 1], x - (float)i);

@@ -735,14 +735,15 @@ GXY_EXPORT int gxy_set_opacity_threshold(float alpha) {
 GXY_EXPORT float gxy_get_opacity_threshold(void) { return G.opaque_alpha; }
 GXY_EXPORT int gxy_set_trace_config(int kind, int chunk, int refill) {
   // kind 0: one ray per work-item; 10 + w: the same with a w-waves-per-SIMD
-  // occupancy hint (w = 5, 6); 1: persistent lanes
+  // occupancy hint (w = 5, 6; 7 and 8 for one bricked volume with at most one
+  // isovalue, 6 otherwise); 1: persistent lanes
   int wpe = 0;
   if (kind >= 10) {
     wpe = kind - 10;
-    if (wpe != 5 && wpe != 6) return fail("trace occupancy variant %d: 15 or 16", kind);
+    if (wpe < 5 || wpe > 8) return fail("trace occupancy variant %d: 15, 16, 17 or 18", kind);
     kind = 0;
   }
-  if (kind < 0 || kind > 1) return fail("trace kernel kind %d: 0, 1, 15 or 16", kind);
+  if (kind < 0 || kind > 1) return fail("trace kernel kind %d: 0, 1, 15, 16, 17 or 18", kind);
   if (chunk < 64 || chunk % 64 || refill < 1 || refill > 64)
     return fail("trace config chunk %d (multiple of 64) refill %d (1..64)", chunk, refill);
   G.trace_kind = kind;
@@ -990,7 +991,10 @@ GXY_EXPORT int gxy_volume_set_layout(void *vol, int brick) {
   v->d.bdata = v->bdata;
   v->d.brick = brick;
   // a cell's last load ends (PL + E + 2) floats past its offset, inside its own brick
-  v->d.bsmall = nbr * brick_floats(brick) * sizeof(float) < ((size_t)1 << 32) ? 1 : 0;
+  v->d.bsmall = nbr * brick_floats(brick) * sizeof(float) < ((size_t)1 << 32) &&
+                        (size_t)brick_floats(brick) * nb[0] * nb[1] < ((size_t)1 << 24)
+                    ? 1
+                    : 0;
   return volume_brickify(v);
 }
 GXY_EXPORT int gxy_volume_get_layout(void *vol) {

@@ -383,7 +383,9 @@ static void launch_trace_sm(const TraceParams &P, const RayList_ispc &R, int n, 
   if (wpe == 5 && one_iso) hipLaunchKernelGGL((k_trace<1, false, 5, SM, 0, 0, 1>), g, b, 0, s, P, R, n);
   else if (wpe == 5) hipLaunchKernelGGL((k_trace<1, false, 5, SM>), g, b, 0, s, P, R, n);
   else if (wpe == 6 && one_iso) hipLaunchKernelGGL((k_trace<1, false, 6, SM, 0, 0, 1>), g, b, 0, s, P, R, n);
-  else if (wpe == 6) hipLaunchKernelGGL((k_trace<1, false, 6, SM>), g, b, 0, s, P, R, n);
+  else if (wpe == 7 && one_iso) hipLaunchKernelGGL((k_trace<1, false, 7, SM, 0, 0, 1>), g, b, 0, s, P, R, n);
+  else if (wpe == 8 && one_iso) hipLaunchKernelGGL((k_trace<1, false, 8, SM, 0, 0, 1>), g, b, 0, s, P, R, n);
+  else if (wpe >= 6) hipLaunchKernelGGL((k_trace<1, false, 6, SM>), g, b, 0, s, P, R, n);
   else if (one_iso) hipLaunchKernelGGL((k_trace<1, false, 0, SM, 0, 0, 1>), g, b, 0, s, P, R, n);
   else hipLaunchKernelGGL((k_trace<1, false, 0, SM>), g, b, 0, s, P, R, n);
 }
@@ -425,10 +427,10 @@ static hipError_t launch_trace_nv(const TraceParams &P, const RayList_ispc &R, i
   } else if (P.has_model) hipLaunchKernelGGL((k_trace<NV, true, 0, -1>), g, b, 0, s, P, R, n);
   else if (NV == 1 && P.vv[0].vol.brick) launch_trace_bricked(P, R, n, wpe, sm_of(P.vv[0].vol), g, b, s);
   else if (small && wpe == 5) hipLaunchKernelGGL((k_trace<NV, false, 5, 1>), g, b, 0, s, P, R, n);
-  else if (small && wpe == 6) hipLaunchKernelGGL((k_trace<NV, false, 6, 1>), g, b, 0, s, P, R, n);
+  else if (small && wpe >= 6) hipLaunchKernelGGL((k_trace<NV, false, 6, 1>), g, b, 0, s, P, R, n);
   else if (small) hipLaunchKernelGGL((k_trace<NV, false, 0, 1>), g, b, 0, s, P, R, n);
   else if (wpe == 5) hipLaunchKernelGGL((k_trace<NV, false, 5, -1>), g, b, 0, s, P, R, n);
-  else if (wpe == 6) hipLaunchKernelGGL((k_trace<NV, false, 6, -1>), g, b, 0, s, P, R, n);
+  else if (wpe >= 6) hipLaunchKernelGGL((k_trace<NV, false, 6, -1>), g, b, 0, s, P, R, n);
   else hipLaunchKernelGGL((k_trace<NV, false, 0, -1>), g, b, 0, s, P, R, n);
   return hipGetLastError();
 }

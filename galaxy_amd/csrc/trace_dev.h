@@ -739,9 +739,16 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
               const DTF &T = P.vv[major].tf;
               float sVolume = (sLast[major] + sThis[major]) / 2;
               float sop = tf_opacity(s_op[major], T, sVolume);
+              // a primary ray reads the colour beside the opacity, so that the
+              // two LDS lookups wait once (the empty statement keeps the
+              // colour reads ahead of the test on the opacity)
+              f3 sc;
+              if (shadeFlag) {
+                sc = tf_color_num(s_col[major], T, sVolume);
+                asm volatile("" ::"v"(sc.x), "v"(sc.y), "v"(sc.z));
+              }
               if (sop > 0) {
                 if (shadeFlag) {
-                  f3 sc = tf_color_num(s_col[major], T, sVolume);
                   composite(cr, cg, cb, ca, sc, clamp01f(sop * P.vv[major].vol.irate));
                 } else {
                   float wop = ((tThis - tLast) * P.istep) * clamp01f(sop * P.vv[major].vol.irate);

@@ -191,8 +191,9 @@ size_t      gxy_device_bytes_in_use(void);
 int         gxy_set_opacity_threshold(float alpha);
 float       gxy_get_opacity_threshold(void);
 /* Launch configuration of the volume-only trace (no reference counterpart):
- * kind 0 = one ray per work-item; 15 / 16 = the same kernel compiled for at
- * least 5 / 6 waves per SIMD; kind 1 = persistent lanes, each wave owning
+ * kind 0 = one ray per work-item; 15 / 16 / 17 / 18 = the same kernel compiled
+ * for at least 5 / 6 / 7 / 8 waves per SIMD (17 and 18 for one bricked volume
+ * with at most one isovalue, 16 otherwise); kind 1 = persistent lanes, each wave owning
  * `chunk` consecutive rays (multiple of 64) and refilling idle lanes once
  * `refill` (1..64) of them are idle.  Results are identical for every setting;
  * galaxy_amd/autotune.py picks the fastest. */

@@ -497,6 +497,62 @@ def test_every_volume_layout_gives_identical_rays(radial_vols_small, hip):
         assert np.array_equal(parts, ref[1]) and st.samples == ref[2], lay
 
 
+@pytest.mark.parametrize("niso", [1, 2])
+def test_every_occupancy_variant_of_the_bricked_march_gives_identical_rays(niso, radial_vols_small, hip):
+    """The 5- to 8-waves-per-SIMD builds of the one-volume march (17 and 18
+    only for at most one isovalue, 16 otherwise) over both brick sizes give
+    rays bit-identical to the linear layout's default kernel."""
+    from galaxy_amd import lib as glib
+    L_ = glib.lib()
+    sc = _dvr_scene(radial_vols_small, 1)
+    vv0 = sc.volume_vis[0]
+    vv = scenes.VolumeVisDesc(vv0.volume, colormap=vv0.colormap, opacitymap=vv0.opacitymap,
+                              isovalues=np.array([0.6, 0.3][:niso], f32), volume_render=True)
+    sc = scenes.SceneDesc(volume_vis=[vv], local_box=sc.local_box, global_box=sc.global_box)
+    Bh = scenes.build_scene(hip, sc)
+    vol = Bh.handles[0]
+    hr = random_rays(50000, seed=85 + niso, tmax=1e30)
+    saved = (C.c_int(), C.c_int(), C.c_int())
+    L_.gxy_get_trace_config(*[C.byref(x) for x in saved])
+    ref = None
+    try:
+        for lay, kinds in ((0, (0,)), (4, (0, 15, 16, 17, 18)), (8, (0, 15, 16, 17, 18))):
+            glib.check(L_.gxy_volume_set_layout(vol, lay))
+            for kind in kinds:
+                glib.check(L_.gxy_set_trace_config(kind, 256, 32))
+                d = glib.DeviceRays.from_host(hr)
+                assert hip.trace(Bh.vis, d) == ""
+                g = d.download()
+                if ref is None:
+                    ref = g
+                    assert (g["term"] != 0).sum() > 10000
+                    continue
+                for f in abi.FIELDS:
+                    assert np.array_equal(g[f], ref[f]), (lay, kind, f)
+    finally:
+        L_.gxy_set_trace_config(*[x.value for x in saved])
+        L_.gxy_volume_set_layout(vol, 0)
+
+
+def test_a_nan_volume_step_is_rejected(radial_vols_small, hip):
+    """A NaN sampling rate makes the march step NaN: the trace entry point
+    reports it and launches nothing; a valid rate traces again."""
+    from galaxy_amd import lib as glib
+    L_ = glib.lib()
+    sc = _dvr_scene(radial_vols_small, 1)
+    Bh = scenes.build_scene(hip, sc)
+    vol = Bh.handles[0]
+    hr = random_rays(1000, seed=91, tmax=1e30)
+    try:
+        glib.check(L_.gxy_volume_set_sampling_rate(vol, float("nan")))
+        d = glib.DeviceRays.from_host(hr)
+        assert "NaN" in hip.trace(Bh.vis, d)
+    finally:
+        glib.check(L_.gxy_volume_set_sampling_rate(vol, sc.volume_vis[0].volume.sampling_rate))
+    d = glib.DeviceRays.from_host(hr)
+    assert hip.trace(Bh.vis, d) == ""
+
+
 def test_occlusion_rays_give_the_same_frames(hip):
     """Shadow and AO rays of a volume-free scene stop at their first hit in
     the rank loop (gxy_set_occlusion_rays): the frame and every count equal
