@@ -563,12 +563,14 @@ def main():
     lay = L.gxy_volume_get_layout(built.handles[0])
     sm = lay if lay else 1
     # the variant the launcher picks: one volume, bricked (32-bit offsets under 4 GiB), one isovalue
-    mi = ",0,0,1" if lay and len(sc.volume_vis[0].isovalues) <= 1 else ""
+    one = bool(lay) and len(sc.volume_vis[0].isovalues) <= 1
     k = kcfg[0].value
     w = k - 10 if k >= 10 else 0
-    if w > 6 and not mi:
+    if w > 6 and not one:
         w = 6                               # 17 / 18 act as 16 beyond one bricked volume and isovalue
-    roof["kernel"] = "k_trace_pl<1>" if k == 1 else "k_trace<1,false,%d,%d%s>" % (w, sm, mi)
+    bs = kcfg[1].value if (one and w == 6 and kcfg[1].value in (64, 128)) else 256
+    roof["kernel"] = ("k_trace_pl<1>" if k == 1 else
+                      "k_trace<1,false,%d,%d,0,0,%d,%d>" % (w, sm, 1 if one else 8, bs))
     roof["volume_layout"] = lay
     roof["kernel_ms_per_launch"] = round(tr_ms / max(1, tr_n), 4)
     roof["alg_bytes_per_launch"] = int(alg_bytes / max(1, tr_n))

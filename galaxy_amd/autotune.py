@@ -6,7 +6,9 @@ measures: it renders the caller's frame once per candidate with HIP-event
 timing of the trace kernel enabled and keeps the fastest.
 
 Candidates: one ray per work-item (kind 0, and its 5- and 6-waves-per-SIMD
-occupancy variants 15 and 16), or persistent lanes (kind 1) with a per-wave
+occupancy variants 15 and 16, the latter also in 64-item workgroups, which
+the one-bricked-volume, one-isovalue march takes: the chunk field is the
+workgroup size for these kinds), or persistent lanes (kind 1) with a per-wave
 chunk of 128 or 256 consecutive rays and a refill threshold of 16 or 32 idle
 lanes out of 64.  (Measured on the volume-render frame: the persistent form
 was 2-5x slower than one ray per work-item; it stays a candidate for other
@@ -18,7 +20,7 @@ only make the pick noisy.)
 """
 import ctypes as C
 
-DEFAULT_CANDIDATES = ([(0, 256, 32), (15, 256, 32), (16, 256, 32)] +
+DEFAULT_CANDIDATES = ([(0, 256, 32), (15, 256, 32), (16, 256, 32), (16, 64, 32)] +
                       [(1, chunk, refill) for chunk in (128, 256) for refill in (16, 32)])
 # BVH traversal stacks of geometry-only traces: private memory (64 entries) or
 # 16 / 24 / 32 entries in LDS (a model deeper than the stack keeps the private one)

@@ -47,8 +47,13 @@ struct CameraParams {
 // private memory; 16, 24, 32: entries in LDS, used when the tree fits)
 // gwaves: 8 asks for 8 waves per SIMD (64 VGPRs) in the binary private-stack geometry walks (with or without
 // path lines), 0 the compiler's choice
+// bs: work-items per workgroup of the one-bricked-volume, one-isovalue march
+// at 6 waves per SIMD (64, 128 or 256; GXY_BLOCK elsewhere):
+// trace_block_size(P, wpe, bs) is the size a launch uses (its workgroup count
+// sizes P.sample_partials)
 hipError_t launch_trace(const TraceParams &P, const RayList_ispc &R, int n, int wpe, int stack, hipStream_t s,
-                        int gwaves = 0);
+                        int gwaves = 0, int bs = GXY_BLOCK);
+int trace_block_size(const TraceParams &P, int wpe, int bs);
 hipError_t launch_sampler(const TraceParams &P, const RayList_ispc &R, int n, hipStream_t s);
 // Persistent-lane trace for volume-only visualizations (1..4 volumes): each
 // wave owns `chunk` consecutive rays; idle lanes are refilled when at least

@@ -736,7 +736,8 @@ GXY_EXPORT float gxy_get_opacity_threshold(void) { return G.opaque_alpha; }
 GXY_EXPORT int gxy_set_trace_config(int kind, int chunk, int refill) {
   // kind 0: one ray per work-item; 10 + w: the same with a w-waves-per-SIMD
   // occupancy hint (w = 5, 6; 7 and 8 for one bricked volume with at most one
-  // isovalue, 6 otherwise); 1: persistent lanes
+  // isovalue, 6 otherwise), chunk 64 or 128 being the workgroup size of w = 6
+  // on one bricked volume with at most one isovalue; 1: persistent lanes
   int wpe = 0;
   if (kind >= 10) {
     wpe = kind - 10;
@@ -1732,14 +1733,16 @@ GXY_EXPORT int gxy_model_set_hit_output(void *model, int *g, int *p) {
 static int trace_launch(TraceParams P, const RayList_ispc &R, int n, bool sampler) {
   if (n <= 0) return 0;
   const bool pl = !sampler && G.trace_kind == 1 && P.nvv >= 1 && !P.has_model;
-  int nb = pl ? trace_pl_blocks(n, G.trace_chunk) : (n + GXY_BLOCK - 1) / GXY_BLOCK;
+  const int bs = sampler ? GXY_BLOCK : trace_block_size(P, G.trace_wpe, G.trace_chunk);
+  int nb = pl ? trace_pl_blocks(n, G.trace_chunk) : (n + bs - 1) / bs;
   unsigned long long *part = (unsigned long long *)dalloc(sizeof(unsigned long long) * nb);
   if (!part) return -1;
   P.sample_partials = part;
   {
     KTimer kt(sampler ? "sampler" : "trace");
     if (pl) HIPCHK(launch_trace_pl(P, R, n, G.trace_chunk, G.trace_refill, S()));
-    else HIPCHK(sampler ? launch_sampler(P, R, n, S()) : launch_trace(P, R, n, G.trace_wpe, G.geom_stack, S(), G.geom_waves));
+    else HIPCHK(sampler ? launch_sampler(P, R, n, S())
+                        : launch_trace(P, R, n, G.trace_wpe, G.geom_stack, S(), G.geom_waves, G.trace_chunk));
   }
   HIPCHK(launch_reduce_add(part, nb, CX().counters, S()));
   dfree_stream_ordered(part);

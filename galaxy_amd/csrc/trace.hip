@@ -17,14 +17,19 @@ namespace gxy {
 
 // Workgroup total of a per-item count written to out[blockIdx.x] (no atomics:
 // same-address atomics from every wave serialise in one L2 channel).
+template <int BS = GXY_BLOCK>
 __device__ __forceinline__ void block_sum_store(unsigned long long v, unsigned long long *out) {
-  __shared__ unsigned long long ws[GXY_BLOCK / 64];
   v = wave_sum_u64(v);
+  if (BS == 64) {
+    if (threadIdx.x == 0) out[blockIdx.x] = v;
+    return;
+  }
+  __shared__ unsigned long long ws[BS / 64];
   if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned long long t = 0;
-    for (int w = 0; w < GXY_BLOCK / 64; w++) t += ws[w];
+    for (int w = 0; w < BS / 64; w++) t += ws[w];
     out[blockIdx.x] = t;
   }
 }
@@ -58,32 +63,36 @@ hipError_t launch_reduce_add(const unsigned long long *part, int nb, unsigned lo
 // W: BVH walked by geometry variants (2 binary, 4 four-wide, 0 per model).
 // MI: isovalues per volume the iso test is unrolled for (1 when no volume
 // has more than one, fewer scalar branches per sample).
-template <int NV, bool GEOM, int WPE, int SM, int STK = 0, int W = 0, int MI = GXY_MAX_ISO>
-__global__ __launch_bounds__(GXY_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, 8)))
+// BS: work-items per workgroup (64, 128 or 256).  A workgroup's slots are
+// refilled only as a whole, so smaller workgroups let a SIMD take new rays as
+// soon as one wave's rays are done, at the price of staging the transfer
+// function tables once per wave.
+template <int NV, bool GEOM, int WPE, int SM, int STK = 0, int W = 0, int MI = GXY_MAX_ISO, int BS = GXY_BLOCK>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, 8)))
 void k_trace(TraceParams P, RayList_ispc R, int n) {
   constexpr int NVS = NV > 0 ? NV : 1;
   __shared__ float s_col[NVS][3 * GXY_TF_MAX];
   __shared__ float s_op[NVS][GXY_TF_MAX];
 #pragma unroll
   for (int v = 0; v < NV; v++) {
-    for (int k = threadIdx.x; k < 3 * P.vv[v].tf.nc; k += GXY_BLOCK) s_col[v][k] = P.vv[v].tf.col[k];
-    for (int k = threadIdx.x; k < P.vv[v].tf.no; k += GXY_BLOCK) s_op[v][k] = P.vv[v].tf.op[k];
+    for (int k = threadIdx.x; k < 3 * P.vv[v].tf.nc; k += BS) s_col[v][k] = P.vv[v].tf.col[k];
+    for (int k = threadIdx.x; k < P.vv[v].tf.no; k += BS) s_op[v][k] = P.vv[v].tf.op[k];
   }
   if (NV > 0) __syncthreads();
 
-  const int i = blockIdx.x * GXY_BLOCK + threadIdx.x;
+  const int i = blockIdx.x * BS + threadIdx.x;
   unsigned long long nsamp = 0;
   if (STK > 0) {
-    __shared__ int s_stack[STK > 0 ? STK : 1][GXY_BLOCK];
+    __shared__ int s_stack[STK > 0 ? STK : 1][BS];
     LdsStack<(STK > 0 ? STK : 1)> stack;
     stack.base = &s_stack[0][threadIdx.x];
-    stack.stride = GXY_BLOCK;
+    stack.stride = BS;
     if (i < n) nsamp = trace_one<NV, GEOM, LdsStack<(STK > 0 ? STK : 1)>, SM, W, MI>(P, R, i, s_col, s_op, stack);
   } else {
     PrivStack<GXY_BVH_STACK> stack;
     if (i < n) nsamp = trace_one<NV, GEOM, PrivStack<GXY_BVH_STACK>, SM, W, MI>(P, R, i, s_col, s_op, stack);
   }
-  if (P.sample_partials) block_sum_store(nsamp, P.sample_partials);
+  if (P.sample_partials) block_sum_store<BS>(nsamp, P.sample_partials);
 }
 
 // ------------------------------------------------------------- persistent-lane march
@@ -377,9 +386,15 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_sampler(TraceParams P, RayList_is
 // one volume with a bricked copy: the variant for its addressing (32-bit
 // offsets when the copy spans under 4 GiB) and the occupancy request
 template <int SM>
-static void launch_trace_sm(const TraceParams &P, const RayList_ispc &R, int n, int wpe, dim3 g, dim3 b,
+static void launch_trace_sm(const TraceParams &P, const RayList_ispc &R, int n, int wpe, int bs, dim3 g, dim3 b,
                             hipStream_t s) {
   const bool one_iso = P.vv[0].nIso <= 1;
+  if (one_iso && wpe == 6 && bs != 256) {
+    dim3 gs((n + bs - 1) / bs), bb(bs);
+    if (bs == 64) hipLaunchKernelGGL((k_trace<1, false, 6, SM, 0, 0, 1, 64>), gs, bb, 0, s, P, R, n);
+    else hipLaunchKernelGGL((k_trace<1, false, 6, SM, 0, 0, 1, 128>), gs, bb, 0, s, P, R, n);
+    return;
+  }
   if (wpe == 5 && one_iso) hipLaunchKernelGGL((k_trace<1, false, 5, SM, 0, 0, 1>), g, b, 0, s, P, R, n);
   else if (wpe == 5) hipLaunchKernelGGL((k_trace<1, false, 5, SM>), g, b, 0, s, P, R, n);
   else if (wpe == 6 && one_iso) hipLaunchKernelGGL((k_trace<1, false, 6, SM, 0, 0, 1>), g, b, 0, s, P, R, n);
@@ -389,19 +404,26 @@ static void launch_trace_sm(const TraceParams &P, const RayList_ispc &R, int n, 
   else if (one_iso) hipLaunchKernelGGL((k_trace<1, false, 0, SM, 0, 0, 1>), g, b, 0, s, P, R, n);
   else hipLaunchKernelGGL((k_trace<1, false, 0, SM>), g, b, 0, s, P, R, n);
 }
-static void launch_trace_bricked(const TraceParams &P, const RayList_ispc &R, int n, int wpe, int sm, dim3 g, dim3 b,
-                                 hipStream_t s) {
+static void launch_trace_bricked(const TraceParams &P, const RayList_ispc &R, int n, int wpe, int bs, int sm, dim3 g,
+                                 dim3 b, hipStream_t s) {
   switch (sm) {
-    case 4: launch_trace_sm<4>(P, R, n, wpe, g, b, s); break;
-    case 8: launch_trace_sm<8>(P, R, n, wpe, g, b, s); break;
-    case 12: launch_trace_sm<12>(P, R, n, wpe, g, b, s); break;
-    default: launch_trace_sm<16>(P, R, n, wpe, g, b, s); break;
+    case 4: launch_trace_sm<4>(P, R, n, wpe, bs, g, b, s); break;
+    case 8: launch_trace_sm<8>(P, R, n, wpe, bs, g, b, s); break;
+    case 12: launch_trace_sm<12>(P, R, n, wpe, bs, g, b, s); break;
+    default: launch_trace_sm<16>(P, R, n, wpe, bs, g, b, s); break;
   }
+}
+
+// the workgroup size launch_trace uses for these parameters
+int trace_block_size(const TraceParams &P, int wpe, int bs) {
+  return (P.nvv == 1 && !P.has_model && P.vv[0].vol.brick && P.vv[0].nIso <= 1 && wpe == 6 && (bs == 64 || bs == 128))
+             ? bs
+             : GXY_BLOCK;
 }
 
 template <int NV>
 static hipError_t launch_trace_nv(const TraceParams &P, const RayList_ispc &R, int n, int wpe, int stack,
-                                  hipStream_t s, int gwaves) {
+                                  hipStream_t s, int gwaves, int bs) {
   dim3 g((n + GXY_BLOCK - 1) / GXY_BLOCK), b(GXY_BLOCK);
   bool small = true;
   for (int v = 0; v < NV; v++) small = small && P.vv[v].vol.small;
@@ -425,7 +447,7 @@ static hipError_t launch_trace_nv(const TraceParams &P, const RayList_ispc &R, i
     else if (gwaves == 8) hipLaunchKernelGGL((k_trace<0, true, 8, -1, 0, 2>), g, b, 0, s, P, R, n);
     else hipLaunchKernelGGL((k_trace<0, true, 0, -1, 0, 2>), g, b, 0, s, P, R, n);
   } else if (P.has_model) hipLaunchKernelGGL((k_trace<NV, true, 0, -1>), g, b, 0, s, P, R, n);
-  else if (NV == 1 && P.vv[0].vol.brick) launch_trace_bricked(P, R, n, wpe, sm_of(P.vv[0].vol), g, b, s);
+  else if (NV == 1 && P.vv[0].vol.brick) launch_trace_bricked(P, R, n, wpe, trace_block_size(P, wpe, bs), sm_of(P.vv[0].vol), g, b, s);
   else if (small && wpe == 5) hipLaunchKernelGGL((k_trace<NV, false, 5, 1>), g, b, 0, s, P, R, n);
   else if (small && wpe >= 6) hipLaunchKernelGGL((k_trace<NV, false, 6, 1>), g, b, 0, s, P, R, n);
   else if (small) hipLaunchKernelGGL((k_trace<NV, false, 0, 1>), g, b, 0, s, P, R, n);
@@ -436,14 +458,14 @@ static hipError_t launch_trace_nv(const TraceParams &P, const RayList_ispc &R, i
 }
 
 hipError_t launch_trace(const TraceParams &P, const RayList_ispc &R, int n, int wpe, int stack, hipStream_t s,
-                        int gwaves) {
+                        int gwaves, int bs) {
   if (n <= 0) return hipSuccess;
   switch (P.nvv) {
-    case 0: return launch_trace_nv<0>(P, R, n, wpe, stack, s, gwaves);
-    case 1: return launch_trace_nv<1>(P, R, n, wpe, stack, s, gwaves);
-    case 2: return launch_trace_nv<2>(P, R, n, wpe, stack, s, gwaves);
-    case 3: return launch_trace_nv<3>(P, R, n, wpe, stack, s, gwaves);
-    case 4: return launch_trace_nv<4>(P, R, n, wpe, stack, s, gwaves);
+    case 0: return launch_trace_nv<0>(P, R, n, wpe, stack, s, gwaves, bs);
+    case 1: return launch_trace_nv<1>(P, R, n, wpe, stack, s, gwaves, bs);
+    case 2: return launch_trace_nv<2>(P, R, n, wpe, stack, s, gwaves, bs);
+    case 3: return launch_trace_nv<3>(P, R, n, wpe, stack, s, gwaves, bs);
+    case 4: return launch_trace_nv<4>(P, R, n, wpe, stack, s, gwaves, bs);
   }
   return hipErrorInvalidValue;
 }

@@ -193,7 +193,9 @@ float       gxy_get_opacity_threshold(void);
 /* Launch configuration of the volume-only trace (no reference counterpart):
  * kind 0 = one ray per work-item; 15 / 16 / 17 / 18 = the same kernel compiled
  * for at least 5 / 6 / 7 / 8 waves per SIMD (17 and 18 for one bricked volume
- * with at most one isovalue, 16 otherwise); kind 1 = persistent lanes, each wave owning
+ * with at most one isovalue, 16 otherwise), where `chunk` 64 or 128 selects that
+ * many work-items per workgroup for kind 16 on one bricked volume with at most
+ * one isovalue (256 otherwise); kind 1 = persistent lanes, each wave owning
  * `chunk` consecutive rays (multiple of 64) and refilling idle lanes once
  * `refill` (1..64) of them are idle.  Results are identical for every setting;
  * galaxy_amd/autotune.py picks the fastest. */

@@ -500,8 +500,9 @@ def test_every_volume_layout_gives_identical_rays(radial_vols_small, hip):
 @pytest.mark.parametrize("niso", [1, 2])
 def test_every_occupancy_variant_of_the_bricked_march_gives_identical_rays(niso, radial_vols_small, hip):
     """The 5- to 8-waves-per-SIMD builds of the one-volume march (17 and 18
-    only for at most one isovalue, 16 otherwise) over both brick sizes give
-    rays bit-identical to the linear layout's default kernel."""
+    only for at most one isovalue, 16 otherwise) and its 64- and 128-item
+    workgroups (at 6 waves per SIMD and one isovalue; 256 otherwise) over both
+    brick sizes give rays bit-identical to the linear layout's default kernel."""
     from galaxy_amd import lib as glib
     L_ = glib.lib()
     sc = _dvr_scene(radial_vols_small, 1)
@@ -516,10 +517,11 @@ def test_every_occupancy_variant_of_the_bricked_march_gives_identical_rays(niso,
     L_.gxy_get_trace_config(*[C.byref(x) for x in saved])
     ref = None
     try:
-        for lay, kinds in ((0, (0,)), (4, (0, 15, 16, 17, 18)), (8, (0, 15, 16, 17, 18))):
+        cfgs = [(0, 256), (15, 256), (16, 256), (16, 128), (16, 64), (17, 256), (18, 256)]
+        for lay, kinds in ((0, [(0, 256), (16, 64)]), (4, cfgs), (8, cfgs)):
             glib.check(L_.gxy_volume_set_layout(vol, lay))
-            for kind in kinds:
-                glib.check(L_.gxy_set_trace_config(kind, 256, 32))
+            for kind, bs in kinds:
+                glib.check(L_.gxy_set_trace_config(kind, bs, 32))
                 d = glib.DeviceRays.from_host(hr)
                 assert hip.trace(Bh.vis, d) == ""
                 g = d.download()
@@ -528,7 +530,7 @@ def test_every_occupancy_variant_of_the_bricked_march_gives_identical_rays(niso,
                     assert (g["term"] != 0).sum() > 10000
                     continue
                 for f in abi.FIELDS:
-                    assert np.array_equal(g[f], ref[f]), (lay, kind, f)
+                    assert np.array_equal(g[f], ref[f]), (lay, kind, bs, f)
     finally:
         L_.gxy_set_trace_config(*[x.value for x in saved])
         L_.gxy_volume_set_layout(vol, 0)
