@@ -62,6 +62,7 @@ struct DTF {
   float lo, hi;
   float sc, so;   // (nc-1)/(hi-lo), (no-1)/(hi-lo)
   float xc, xo;   // nc-1, no-1 as floats (the clamp bounds of the table coordinate)
+  float lo2, hsc, hso;  // 2 lo, sc / 2, so / 2: coordinates of a doubled value
 };
 
 struct DVolVis {
@@ -452,6 +453,26 @@ __device__ __forceinline__ float tf_opacity(const float *op, const DTF &T, float
   i = i < T.no - 2 ? i : T.no - 2;
   const float o = lerp_fma(op[i], op[i + 1], x - (float)i);
   return nan ? 0.f : o;
+}
+// The opacity and colour lookups of v = s / 2 given s = 2 v and its offset
+// d = s - 2 lo: d * (so / 2) rounds exactly as (v - lo) * so (halving and
+// doubling are exact), so the march looks both up from one sum and one
+// difference, without the halving.
+__device__ __forceinline__ float tf_opacity_d2(const float *op, const DTF &T, float d) {
+  const bool nan = d != d;
+  const float x = __builtin_amdgcn_fmed3f(d * T.hso, 0.f, T.xo);
+  int i = (int)x;
+  i = i < T.no - 2 ? i : T.no - 2;
+  const float o = lerp_fma(op[i], op[i + 1], x - (float)i);
+  return nan ? 0.f : o;
+}
+__device__ __forceinline__ f3 tf_color_num_d2(const float *col, const DTF &T, float d) {
+  const float x = __builtin_amdgcn_fmed3f(d * T.hsc, 0.f, T.xc);
+  int i = (int)x;
+  i = i < T.nc - 2 ? i : T.nc - 2;
+  const float r = x - (float)i;
+  const float *c = col + __umul24((unsigned)i, 3u);
+  return mk3(lerp_fma(c[0], c[3], r), lerp_fma(c[1], c[4], r), lerp_fma(c[2], c[5], r));
 }
 // The colour lookup of a value known not to be NaN (the march composites a
 // sample only when its opacity lookup is positive, which a NaN's is not).

@@ -1087,6 +1087,9 @@ static void *tf_make(std::vector<float> col, std::vector<float> op, float lo, fl
   t->d.so = t->hop.size() > 1 ? (float)(t->d.no - 1) / (hi - lo) : 0.f;
   t->d.xc = (float)(t->d.nc - 1);
   t->d.xo = (float)(t->d.no - 1);
+  t->d.lo2 = 2.f * lo;
+  t->d.hsc = 0.5f * t->d.sc;
+  t->d.hso = 0.5f * t->d.so;
   std::lock_guard<std::recursive_mutex> lk(G.mu);
   G.tfs.insert(t);
   return t;

@@ -681,10 +681,12 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
 #pragma unroll
         for (int v = 0; v < NV; v++) F[v] = vox_fetch<SM>(P.vv[v].vol, VR[v], nx0);
         tLast = tThis;
-        go = nx0 > tThis;
+        go = nx0 > tThis && nx0 <= tTermination;
         tThis = nx0;
       }
-      while (go && tThis <= tTermination && !opaque && !hit_iso && iter++ < GXY_MAX_MARCH) {
+      // (every pass leaves tThis <= tTermination: the next position is cut
+      // there, and a pass that moves tTermination ends the march)
+      while (go && !opaque && !hit_iso && iter++ < GXY_MAX_MARCH) {
 #pragma unroll
         for (int v = 0; v < NV; v++) sThis[v] = vol_finish_fma(F[v]);
         nsamp += NV;
@@ -737,14 +739,16 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
           for (int major = 0; major < NV; major++) {
             if (P.vv[major].dvr) {
               const DTF &T = P.vv[major].tf;
-              float sVolume = (sLast[major] + sThis[major]) / 2;
-              float sop = tf_opacity(s_op[major], T, sVolume);
+              // the lookups of sVolume = (sLast + sThis) / 2 from the sum and
+              // its offset (bit-identical to looking up sVolume)
+              const float dVolume = (sLast[major] + sThis[major]) - T.lo2;
+              float sop = tf_opacity_d2(s_op[major], T, dVolume);
               // a primary ray reads the colour beside the opacity, so that the
               // two LDS lookups wait once (the empty statement keeps the
               // colour reads ahead of the test on the opacity)
               f3 sc;
               if (shadeFlag) {
-                sc = tf_color_num(s_col[major], T, sVolume);
+                sc = tf_color_num_d2(s_col[major], T, dVolume);
                 asm volatile("" ::"v"(sc.x), "v"(sc.y), "v"(sc.z));
               }
               if (sop > 0) {
