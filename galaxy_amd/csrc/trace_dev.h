@@ -685,8 +685,9 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
         tThis = nx0;
       }
       // (every pass leaves tThis <= tTermination: the next position is cut
-      // there, and a pass that moves tTermination ends the march)
-      while (go && !opaque && !hit_iso && iter++ < GXY_MAX_MARCH) {
+      // there, and a pass that moves tTermination ends the march; go is the
+      // one exit condition, set at the end of each pass)
+      while (go && iter++ < GXY_MAX_MARCH) {
 #pragma unroll
         for (int v = 0; v < NV; v++) sThis[v] = vol_finish_fma(F[v]);
         nsamp += NV;
@@ -701,13 +702,18 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
         const float nx0 = __builtin_amdgcn_fmed3f(tThis + step, tTermination, ninf);
 #pragma unroll
         for (int v = 0; v < NV; v++) F[v] = vox_fetch<SM>(P.vv[v].vol, VR[v], nx0);
-        if (tLast >= epsilon) {
+        {
+          // The interval (tLast, tThis] counts only once tLast >= epsilon (a
+          // ray entering before epsilon skips its first one).  Taken without
+          // a branch: a skipped interval sees no isovalue crossing (sl = st)
+          // and zero opacity, which leave the ray as it is.
+          const bool live = tLast >= epsilon;
           // LookForIsoHit: the last matching isovalue in list order wins
           bool ih = false;
           int ivid = -1;
 #pragma unroll
           for (int major = 0; major < NV; major++) {
-            float sl = sLast[major], st = sThis[major];
+            float sl = live ? sLast[major] : sThis[major], st = sThis[major];
             const int ni = P.vv[major].nIso;
             // unrolled to the fixed maximum: the isovalues stay in registers
 #pragma unroll
@@ -743,6 +749,7 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
               // its offset (bit-identical to looking up sVolume)
               const float dVolume = (sLast[major] + sThis[major]) - T.lo2;
               float sop = tf_opacity_d2(s_op[major], T, dVolume);
+              sop = live ? sop : 0.f;
               // a primary ray reads the colour beside the opacity, so that the
               // two LDS lookups wait once (the empty statement keeps the
               // colour reads ahead of the test on the opacity)
@@ -770,8 +777,9 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
         for (int v = 0; v < NV; v++) sLast[v] = sThis[v];
         if (opaque) tTermination = tThis;
         tLast = tThis;
-        // a step that no longer advances t ends the march (the reference would spin)
-        if (!(nx0 > tThis) && !opaque && !hit_iso) break;
+        // the march goes on unless opacity or an iso hit ended it here or the
+        // next position no longer advances t (the reference would spin)
+        go = !opaque && !hit_iso && nx0 > tThis;
         tThis = nx0;
       }
       ray_t = tTermination;
