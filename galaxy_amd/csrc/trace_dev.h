@@ -649,7 +649,7 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
       float tLast = tEntry + epsilon;
       bool hit_iso = false;
       int iso_vid = -1;
-      bool opaque = opaque_of(cr, cg, cb, ca, P.opaque_alpha);
+      const bool opaque0 = opaque_of(cr, cg, cb, ca, P.opaque_alpha);
       float tThis = tEntry;
       int iter = 0;
       // the gathers of the sample at tThis are issued one iteration ahead
@@ -666,7 +666,7 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
       // and the next position is tEntry + epsilon.  Taken before the loop, it
       // leaves every pass of the loop with tThis > tEntry (t strictly grows,
       // or the march ends).
-      bool go = tThis <= tTermination && !opaque;
+      bool go = tThis <= tTermination && !opaque0;
       iter = 1;
       // -inf held in a register the compiler cannot see into, so that the
       // three-way median below stays one v_med3_f32 (folded to a minimum it
@@ -688,6 +688,9 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
       // there, and a pass that moves tTermination ends the march; go is the
       // one exit condition, set at the end of each pass)
       while (go && iter++ < GXY_MAX_MARCH) {
+        // (the ray is not opaque when a pass begins: only this pass's samples
+        // can make it so)
+        bool opaque = false;
 #pragma unroll
         for (int v = 0; v < NV; v++) sThis[v] = vol_finish_fma(F[v]);
         nsamp += NV;
@@ -735,7 +738,6 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
             iso_vid = ivid;
             tTermination = hit.t;
             tThis = hit.t;
-            surface_hit = true;
             hit_iso = true;
 #pragma unroll
             for (int v = 0; v < NV; v++) sThis[v] = vox_sample<SM>(P.vv[v].vol, VR[v], tThis);
@@ -784,6 +786,7 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
       }
       ray_t = tTermination;
       if (hit_iso) {
+        surface_hit = true;
         // LookForIsoHit shading (TraceRays.ispc:284-307)
         hit.point = add3(org, scl3(hit.t, dir));
         if (shadeFlag) {
