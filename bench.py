@@ -279,9 +279,9 @@ def bench_batched(L, hip, steps, tune=True, fixed=None):
             glib.check(L.gxy_volume_set_layout(v, fixed[4]))
         B.refresh_datasets(built)
     elif tune:
-        best, table = gb.tune(B, max_depth=0, graph_passes=2)   # a stack too short for a model is skipped
-        res["tuned"] = dict(best=list(best), ms=[[list(c), round(m, 2)] for c, m in table])
-        # voxel layout of every volume dataset: linear, 4^3 or 8^3 bricks (datasets re-registered)
+        # voxel layout of every volume dataset first (linear, 4^3 or 8^3 bricks; datasets
+        # re-registered) under the default launch configuration, then the launch
+        # configuration under that layout
         lay_ms = {}
         for lay in (0, 4, 8):
             for v in vols:
@@ -297,6 +297,8 @@ def bench_batched(L, hip, steps, tune=True, fixed=None):
         for v in vols:
             glib.check(L.gxy_volume_set_layout(v, layout))
         B.refresh_datasets(built)
+        best, table = gb.tune(B, max_depth=0, graph_passes=2)   # a stack too short for a model is skipped
+        res["tuned"] = dict(best=list(best), ms=[[list(c), round(m, 2)] for c, m in table])
         res["tuned"]["volume_layout"] = layout
         res["tuned"]["layout_ms"] = {str(k): round(v, 2) for k, v in lay_ms.items()}
         log("batched lists tuned", res["tuned"])

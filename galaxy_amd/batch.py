@@ -140,7 +140,8 @@ def build_datasets(hip, descs):
 DEFAULT_CANDIDATES = [(256, 0, 0, 4, 0), (256, 0, 4, 4, 0), (128, 0, 4, 8, 0), (512, 0, 4, 4, 0),
                       (256, 0, 5, 5, 0), (256, 16, 4, 4, 0), (256, 24, 0, 3, 0), (256, 32, 0, 3, 0),
                       (256, 0, 0, 4, 1), (256, 0, 5, 4, 1), (256, 24, 5, 4, 1), (128, 0, 5, 8, 1),
-                      (64, 0, 0, 4, 0), (64, 0, 5, 4, 1), (256, 0, 6, 8, 1), (128, 0, 6, 8, 1)]
+                      (64, 0, 0, 4, 0), (64, 0, 5, 4, 1), (256, 0, 6, 8, 1), (128, 0, 6, 8, 1),
+                      (128, 0, 5, 5, 0), (256, 0, 6, 6, 0)]
 
 
 def tune(batch, candidates=None, max_depth=64, graph_passes=2, reps=1):

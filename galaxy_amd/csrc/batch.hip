@@ -91,33 +91,40 @@ __device__ __forceinline__ void block_partials(unsigned long long *v, int k0, in
   }
 }
 
-// voxel addressing of a dataset's volume: bricked (8 or 4, with 64-bit addresses: the batched
-// kernels are bound by HBM lines, not by address arithmetic), small linear (1), large linear (0)
-__device__ __forceinline__ int chunk_layout(const TraceParams &P) {
-  const int b = P.vv[0].vol.brick;
-  return b == 8 ? 16 : (b == 4 ? 12 : (P.vv[0].vol.small ? 1 : 0));
-}
+// voxel addressing of a dataset's volume (the SM codes of the trace kernels):
+// bricked (8 or 4) with 32-bit offsets when the bricked copy allows them
+// (DVolume::bsmall), else with 64-bit addresses (16 or 12), small linear (1),
+// large linear (0)
+__device__ __forceinline__ int chunk_layout(const TraceParams &P) { return sm_of(P.vv[0].vol); }
 
-// The render rays of one volume chunk by the chunk's layout, one ray per lane at a time.
-template <int SM>
+// The render rays of one volume chunk by the chunk's layout, one ray per lane
+// at a time (MI = 1: the volume has at most one isovalue).
+template <int SM, int MI>
 __device__ __forceinline__ unsigned long long render_chunk(const TraceParams &P, const RayList_ispc &R,
                                                            const BatchChunk &ch, const float (*col)[3 * GXY_TF_MAX],
                                                            const float (*op)[GXY_TF_MAX],
                                                            PrivStack<GXY_BVH_STACK> &pstack) {
   unsigned long long n = 0;
   for (int k = threadIdx.x & 63; k < ch.count; k += 64)
-    n += trace_one<1, false, PrivStack<GXY_BVH_STACK>, SM>(P, R, ch.start + k, col, op, pstack);
+    n += trace_one<1, false, PrivStack<GXY_BVH_STACK>, SM, 0, MI>(P, R, ch.start + k, col, op, pstack);
   return n;
 }
+// (bricked volumes with more than one isovalue take the 64-bit variants,
+// which serve any bricked copy)
 __device__ __forceinline__ unsigned long long render_chunk_any(const TraceParams &P, const RayList_ispc &R,
                                                                const BatchChunk &ch, int lay,
                                                                const float (*col)[3 * GXY_TF_MAX],
                                                                const float (*op)[GXY_TF_MAX],
                                                                PrivStack<GXY_BVH_STACK> &pstack) {
-  if (lay == 16) return render_chunk<16>(P, R, ch, col, op, pstack);
-  if (lay == 12) return render_chunk<12>(P, R, ch, col, op, pstack);
-  if (lay == 1) return render_chunk<1>(P, R, ch, col, op, pstack);
-  return render_chunk<0>(P, R, ch, col, op, pstack);
+  const bool one = P.vv[0].nIso <= 1;
+  if (one && lay == 4) return render_chunk<4, 1>(P, R, ch, col, op, pstack);
+  if (one && lay == 12) return render_chunk<12, 1>(P, R, ch, col, op, pstack);
+  if (one && lay == 8) return render_chunk<8, 1>(P, R, ch, col, op, pstack);
+  if (one && lay == 16) return render_chunk<16, 1>(P, R, ch, col, op, pstack);
+  if (lay == 8 || lay == 16) return render_chunk<16, GXY_MAX_ISO>(P, R, ch, col, op, pstack);
+  if (lay == 4 || lay == 12) return render_chunk<12, GXY_MAX_ISO>(P, R, ch, col, op, pstack);
+  if (lay == 1) return render_chunk<1, GXY_MAX_ISO>(P, R, ch, col, op, pstack);
+  return render_chunk<0, GXY_MAX_ISO>(P, R, ch, col, op, pstack);
 }
 
 // The sampler rays of one chunk (samples and ray passes into acc).
@@ -134,7 +141,9 @@ __device__ __forceinline__ void sample_chunk_as(const BatchArgs &A, const TraceP
 __device__ __forceinline__ void sample_chunk_any(const BatchArgs &A, const TraceParams &P, const RayList_ispc &R,
                                                  const BatchChunk &ch, int lay, float *xyz, int *pc, int cap,
                                                  unsigned long long *acc) {
-  if (lay == 16) sample_chunk_as<16>(A, P, R, ch, xyz, pc, cap, acc);
+  if (lay == 4) sample_chunk_as<4>(A, P, R, ch, xyz, pc, cap, acc);
+  else if (lay == 8) sample_chunk_as<8>(A, P, R, ch, xyz, pc, cap, acc);
+  else if (lay == 16) sample_chunk_as<16>(A, P, R, ch, xyz, pc, cap, acc);
   else if (lay == 12) sample_chunk_as<12>(A, P, R, ch, xyz, pc, cap, acc);
   else if (lay == 1) sample_chunk_as<1>(A, P, R, ch, xyz, pc, cap, acc);
   else sample_chunk_as<0>(A, P, R, ch, xyz, pc, cap, acc);
@@ -230,7 +239,6 @@ void k_btrace(BatchArgs A) {
     const int ds = Ls.ds, mode = Ls.mode;
     const TraceParams &P = kconst(A.ds + ds);
     const RayList_ispc R = chunk_rays(A, ch);
-    // voxel addressing of the dataset's volume: bricked (8 or 4), small linear (1), large linear (0)
     const int lay = chunk_layout(P);
     if (mode == BM_SAMPLE) {
       // every crossing of the ray in this launch (segments counted as ray passes)
@@ -740,6 +748,7 @@ hipError_t launch_batch_pass(const BatchArgs &A, const BatchLaunch &K, hipStream
   } else {
     if (K.wpe == 4) btrace_as<0, 4>(A, tb, s);
     else if (K.wpe == 5) btrace_as<0, 5>(A, tb, s);
+    else if (K.wpe == 6) btrace_as<0, 6>(A, tb, s);
     else btrace_as<0, 0>(A, tb, s);
   }
   hipLaunchKernelGGL(k_bcount, dim3(K.post_blocks), dim3(GXY_BLOCK), 0, s, A);
