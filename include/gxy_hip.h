@@ -523,8 +523,8 @@ int   gxy_batch_remove_list(void *batch, int list);
 /* Launch configuration: chunk (rays per queue entry, 64..1024), stack (BVH traversal
  * stack: 0 = 64 entries in private memory, 16/24/32 = entries in LDS; must
  * exceed the deepest tree by 2), wpe (waves per SIMD requested: 0, 4, 5 or 6; with an
- * LDS stack the fused kernel ignores 5, the fused kernel ignores 6, split mode applies it
- * to the volume kernel; in split mode the sampler kernel always asks for 8),
+ * LDS stack the fused kernel ignores 5 and 6, split mode applies it to the volume
+ * kernel; in split mode the sampler kernel always asks for 8),
  * blocks_per_cu (persistent workgroups per compute unit, 1..16). */
 int   gxy_batch_set_config(void *batch, int chunk, int stack, int wpe, int blocks_per_cu);
 int   gxy_batch_get_config(void *batch, int *chunk, int *stack, int *wpe, int *blocks_per_cu);
