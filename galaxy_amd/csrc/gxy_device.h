@@ -499,3 +499,45 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
+
+// Exclusive sum of in[0, m) into out by one workgroup of 1024 work-items
+// (m <= 16 * 1024): work-item t takes the run [t * per, t * per + per) of
+// per = ceil(m / 1024) ints, loads all of it before using any (one memory
+// latency instead of one per element), and the 1024 run totals are scanned
+// with wave shuffles and one exchange of the 16 wave totals through LDS.
+// Returns the sum of all m ints (to every work-item).
+__device__ __forceinline__ int scan1024_runs(const int *in, int *out, int m) {
+  __shared__ int wtot[16];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int per = (m + 1023) >> 10;
+  const int s0 = t * per;
+  int v[16];
+#pragma unroll
+  for (int k = 0; k < 16; k++) v[k] = (k < per && s0 + k < m) ? in[s0 + k] : 0;
+  int sum = 0;
+#pragma unroll
+  for (int k = 0; k < 16; k++) sum += v[k];
+  int x = sum;   // inclusive scan of the run totals within the wave
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wtot[w] = x;
+  __syncthreads();
+  int wb = 0, total = 0;
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    const int z = wtot[k];
+    wb += k < w ? z : 0;
+    total += z;
+  }
+  int base = wb + x - sum;
+#pragma unroll
+  for (int k = 0; k < 16; k++)
+    if (k < per && s0 + k < m) {
+      out[s0 + k] = base;
+      base += v[k];
+    }
+  return total;
+}

@@ -94,27 +94,9 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_part_count(RayList_ispc R, int n,
 // one workgroup per category: exclusive sum of its nb counts (relative to the
 // category's start) and the category's total, for nb <= GXY_SMALL_SCAN
 __global__ __launch_bounds__(1024) void k_part_scan_small(const int *counts, int nb, int *off, int *tot) {
-  __shared__ int part[1024];
   const int64_t base0 = (int64_t)blockIdx.x * nb;
-  const int t = threadIdx.x, per = (nb + 1023) / 1024;
-  const int s0 = min(nb, t * per), s1 = min(nb, s0 + per);
-  int sum = 0;
-  for (int k = s0; k < s1; k++) sum += counts[base0 + k];
-  part[t] = sum;
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    const int v = t >= o ? part[t - o] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  int base = t > 0 ? part[t - 1] : 0;
-  for (int k = s0; k < s1; k++) {
-    const int v = counts[base0 + k];
-    off[base0 + k] = base;
-    base += v;
-  }
-  if (t == 1023) tot[blockIdx.x] = part[1023];
+  const int total = scan1024_runs(counts + base0, off + base0, nb);
+  if (threadIdx.x == 0) tot[blockIdx.x] = total;
 }
 
 // category totals from the exclusive sum over the category-major counts

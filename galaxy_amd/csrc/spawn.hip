@@ -62,30 +62,10 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_flag_rank(RayList_ispc R, int n, 
 #define SCAN_ITEMS 8
 #define SCAN_TILE (GXY_BLOCK * SCAN_ITEMS)
 
-// Exclusive sum of m ints (in -> out) by one workgroup of 1024 work-items,
-// each summing a run of ceil(m / 1024): one launch where the device-wide scan
-// takes two, for the count arrays of lists up to a few million rays.
-__global__ __launch_bounds__(1024) void k_scan_small(const int *in, int *out, int m) {
-  __shared__ int part[1024];
-  const int t = threadIdx.x, per = (m + 1023) / 1024;
-  const int s0 = min(m, t * per), s1 = min(m, s0 + per);
-  int sum = 0;
-  for (int k = s0; k < s1; k++) sum += in[k];
-  part[t] = sum;
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    const int v = t >= o ? part[t - o] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  int base = t > 0 ? part[t - 1] : 0;
-  for (int k = s0; k < s1; k++) {
-    const int v = in[k];
-    out[k] = base;
-    base += v;
-  }
-}
+// Exclusive sum of m ints (in -> out, m <= GXY_SMALL_SCAN) by one workgroup
+// of 1024 work-items: one launch where the device-wide scan takes two, for the
+// count arrays of lists up to a few million rays.
+__global__ __launch_bounds__(1024) void k_scan_small(const int *in, int *out, int m) { scan1024_runs(in, out, m); }
 
 __device__ __forceinline__ int block_sum256(int v, int *red) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

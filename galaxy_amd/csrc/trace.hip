@@ -38,7 +38,14 @@ __device__ __forceinline__ void block_sum_store(unsigned long long v, unsigned l
 __global__ __launch_bounds__(1024) void k_reduce_add(const unsigned long long *part, int nb, unsigned long long *acc) {
   __shared__ unsigned long long ws[16];
   unsigned long long v = 0;
-  for (int k = threadIdx.x; k < nb; k += 1024) v += part[k];
+  // eight independent loads in flight per work-item and round
+  for (int k0 = threadIdx.x; k0 < nb; k0 += 8 * 1024) {
+    unsigned long long x[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) x[j] = k0 + j * 1024 < nb ? part[k0 + j * 1024] : 0ull;
+#pragma unroll
+    for (int j = 0; j < 8; j++) v += x[j];
+  }
   v = wave_sum_u64(v);
   if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
   __syncthreads();
