@@ -159,6 +159,10 @@ struct TraceParams {
   unsigned long long *sample_partials; // per-workgroup sample counts (may be null)
   int *hit_geom;                       // side-band geomID per ray, may be null
   int *hit_prim;
+  // a launch queued before its list's size is known: when dn is set, the
+  // launch covers min(n, *dn * dn_mul) rays (n is then the list's capacity)
+  const int *dn;
+  int dn_mul;
 };
 
 // ------------------------------------------------------------------ math

@@ -19,6 +19,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -235,12 +236,18 @@ struct Readback {
 // sequence number the readback kernel writes last (the kernel runs after all
 // work queued before it); every 256 polls it asks the stream, so an error or
 // a kernel that never wrote ends the wait through hipStreamSynchronize.
-static int readback(const int *ints, int nints, const int *one, const unsigned long long *u64, Readback &out) {
+// readback_start / readback_wait split it, so that work queued between them
+// runs while the host waits (one readback in flight per context).
+static int readback_start(const int *ints, int nints, const int *one, const unsigned long long *u64, int &seq) {
   Ctx &cx = CX();
   int *dev = readback_area(cx);
   if (!dev) return -1;
-  const int seq = ++cx.rb_seq;
+  seq = ++cx.rb_seq;
   HIPCHK(launch_readback(ints, nints, one, u64, dev, seq, S()));
+  return 0;
+}
+static int readback_wait(int seq, int nints, bool one, bool u64, Readback &out) {
+  Ctx &cx = CX();
   const volatile int *h = cx.hread;
   for (unsigned spins = 1; h[32] != seq; spins++) {
     if ((spins & 255) == 0) {
@@ -257,6 +264,11 @@ static int readback(const int *ints, int nints, const int *one, const unsigned l
   out.one = one ? h[16] : 0;
   out.u64 = u64 ? ((unsigned long long)(unsigned)h[19] << 32) | (unsigned)h[18] : 0;
   return 0;
+}
+static int readback(const int *ints, int nints, const int *one, const unsigned long long *u64, Readback &out) {
+  int seq = 0;
+  if (readback_start(ints, nints, one, u64, seq) != 0) return -1;
+  return readback_wait(seq, nints, one != nullptr, u64 != nullptr, out);
 }
 
 // ------------------------------------------------------------------ timing
@@ -445,6 +457,7 @@ struct RayListObj {
   int kinds = 0;
   int unique = 0;
   bool hdr_dirty = false;   // the device copy of hdr not written yet (lists only the loop sees)
+  bool pretraced = false;   // its rays' trace is already queued (the rank loop's next pass skips it)
   gxy_raylist_hdr hdr;
   RayList_ispc ispc;
 };
@@ -2220,9 +2233,12 @@ static void camera_params(const float eye[3], const float cdir[3], int is_dir, c
 
 // counter_out (may be null): the context's sample counter as of the camera
 // rays, read in the same wait
+// between(total): work queued after the readback of the ray count and before
+// the host waits for it, given the count's device address (may be null)
 static int camera_generate(const float eye[3], const float cdir[3], int is_dir, const float up[3], float aov,
                            int w, int h, const gxy_box3f *lb, const gxy_box3f *gb, const RayList_ispc &R,
-                           int capacity, int order, int *n_out, unsigned long long *counter_out = nullptr) {
+                           int capacity, int order, int *n_out, unsigned long long *counter_out = nullptr,
+                           const std::function<int(const int *)> &between = nullptr) {
   CameraParams C;
   camera_params(eye, cdir, is_dir, up, aov, w, h, lb, gb, order, C);
   int npix = order == 0 ? w * h : ((w + 7) / 8) * ((h + 7) / 8) * 64;
@@ -2236,7 +2252,10 @@ static int camera_generate(const float eye[3], const float cdir[3], int is_dir, 
     HIPCHK(launch_camera(C, R, capacity, scratch, nullptr, S()));
   }
   Readback rb;
-  if (readback(nullptr, 0, total, counter_out ? CX().counters : nullptr, rb) != 0) return -1;
+  int seq = 0;
+  if (readback_start(nullptr, 0, total, counter_out ? CX().counters : nullptr, seq) != 0) return -1;
+  if (between && between(total) != 0) return -1;
+  if (readback_wait(seq, 0, true, counter_out != nullptr, rb) != 0) return -1;
   const int T = rb.one;
   if (counter_out) *counter_out = rb.u64;
   if (T > capacity) return fail("camera produced %d rays for a list of %d", T, capacity);
@@ -2323,11 +2342,27 @@ static int enqueue_split(LoopCtx &C, RayListObj *out, int prio) {
 }
 
 // processRays_task::work (Renderer.cpp:511-651) for one list and its keepers
+// Queues the trace of list L before its size is known: the launch covers its
+// capacity ncap and the kernel reads the size, *dn * mul, from the device.
+// Only the one-ray-per-item kernels read it, so only they are queued ahead.
+static bool can_pretrace(const LoopCtx &C, long long ncap) {
+  return !C.sampler && G.trace_kind != 1 && ncap > 0 && ncap <= C.maxlist;
+}
+static int pretrace(LoopCtx &C, RayListObj *L, int ncap, const int *dn, int mul) {
+  TraceParams P = C.P;
+  P.dn = dn;
+  P.dn_mul = mul;
+  if (trace_launch(P, L->ispc, ncap, false) != 0) return -1;
+  L->pretraced = true;
+  return 0;
+}
+
 static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
   while (L && L->hdr.size > 0) {
     const int n = L->hdr.size;
     const RayList_ispc &R = L->ispc;
-    if (trace_launch(C.P, R, n, C.sampler) != 0) return -1;
+    if (L->pretraced) L->pretraced = false;
+    else if (trace_launch(C.P, R, n, C.sampler) != 0) return -1;
     C.st.ray_passes += n;
     C.st.passes++;
     C.st.lists++;
@@ -2411,9 +2446,15 @@ static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
       HIPCHK(launch_partition(R, n, table, ntable, K, nullptr, scr, dcounts, S()));
     }
     // one wait per list: the partition counts, the parent count and the
-    // sample counter (after the loop's last list no trace follows)
+    // sample counter (after the loop's last list no trace follows).  The
+    // children's trace is queued behind the readback when they will form one
+    // list (the loop traces them next, or after lists whose rays they do not
+    // touch), so the GPU traces while the host waits and decides.
     Readback rb;
-    if (readback(dcounts, K, dT, CX().counters, rb) != 0) return -1;
+    int seq = 0;
+    if (readback_start(dcounts, K, dT, CX().counters, seq) != 0) return -1;
+    if (out.p && dT && can_pretrace(C, (long long)n * nsp) && pretrace(C, out.p, n * nsp, dT, nsp) != 0) return -1;
+    if (readback_wait(seq, K, dT != nullptr, true, rb) != 0) return -1;
     for (int k = 0; k < K; k++) counts[k] = rb.ints[k];
     const int T = rb.one;
     C.samples_read = rb.u64;
@@ -2570,21 +2611,27 @@ GXY_EXPORT int gxy_render_frame(void *vis, void *lights, const float eye[3], con
   if (!prim) return -1;
   prim->kinds = 1;    // camera rays: all primary, one per pixel
   prim->unique = 1;
-  int np = 0;
-  unsigned long long base = 0;
-  if (camera_generate(eye, center, 0, up, aov, w, h, &V->local_bb, &V->global_bb, prim->ispc, w * h, 1, &np,
-                      &base) != 0) {
-    raylist_delete(prim);
-    return -1;
-  }
   LoopCtx C;
-  // the sample counter is not cleared: the frame's samples are counted from its value here
-  C.have_base = true;
-  C.samples_base = base;
   if (loop_setup(C, vis, lights, nb, eps, maxlist) != 0 || !C.have_lights) {
     raylist_delete(prim);
     return C.have_lights ? -1 : fail("gxy_render_frame needs a Lighting_ispc");
   }
+  int np = 0;
+  unsigned long long base = 0;
+  // the camera rays' trace is queued behind the readback of their count (the
+  // list is processed in place when it fits one list)
+  auto ahead = [&](const int *dcount) {
+    return can_pretrace(C, (long long)w * h) ? pretrace(C, prim, w * h, dcount, 1) : 0;
+  };
+  if (camera_generate(eye, center, 0, up, aov, w, h, &V->local_bb, &V->global_bb, prim->ispc, w * h, 1, &np,
+                      &base, ahead) != 0) {
+    raylist_delete(prim);
+    return -1;
+  }
+  // the sample counter is not cleared: the frame's samples are counted from its
+  // value read before the camera rays' trace
+  C.have_base = true;
+  C.samples_base = base;
   C.fb = fb;
   C.w = w;
   C.h = h;

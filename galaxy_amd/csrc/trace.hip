@@ -88,6 +88,7 @@ void k_trace(TraceParams P, RayList_ispc R, int n) {
   if (NV > 0) __syncthreads();
 
   const int i = blockIdx.x * BS + threadIdx.x;
+  if (P.dn) n = min(n, *P.dn * P.dn_mul);
   unsigned long long nsamp = 0;
   if (STK > 0) {
     __shared__ int s_stack[STK > 0 ? STK : 1][BS];
