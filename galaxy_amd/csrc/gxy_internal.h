@@ -114,9 +114,14 @@ hipError_t launch_route(const RouteParams &P, const RayList_ispc &R, int n, hipS
 // counts: K ints (device).  scratch: partition_scratch_bytes(n, K) bytes, kept
 // intact between the count phase (lists == nullptr) and launch_partition_scatter,
 // which takes the same n and K (lists: K entries, those with a null ox not copied).
+// counted: the per-workgroup counts are already in scratch (launch_route_count:
+// the route step and the count phase in one pass over the rays).
 size_t partition_scratch_bytes(int n, int K);
 hipError_t launch_partition(const RayList_ispc &R, int n, const int *cat_table, int ntable, int K,
-                            const RayList_ispc *lists, int *scratch, int *counts, hipStream_t s);
+                            const RayList_ispc *lists, int *scratch, int *counts, hipStream_t s,
+                            bool counted = false);
+hipError_t launch_route_count(const RouteParams &P, const RayList_ispc &R, int n, const int *cat_table, int ntable,
+                              int K, int *scratch, hipStream_t s);
 hipError_t launch_partition_scatter(const RayList_ispc &R, int n, const int *cat_table, int ntable, int K,
                                     const RayList_ispc *lists, const int *scratch, hipStream_t s);
 // scratch: block_scan_scratch_bytes(ceil(n/256)) bytes (camera: of the pixel

@@ -91,6 +91,54 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_part_count(RayList_ispc R, int n,
   if (blockIdx.x == 0 && threadIdx.x == 0) counts[K * nb] = 0;
 }
 
+// k_route followed by k_part_count's counting, from the classification in
+// registers (one pass over the rays instead of two)
+__global__ __launch_bounds__(GXY_BLOCK) void k_route_count(RouteParams P, RayList_ispc R, int n, CatTable table,
+                                                           int K, int nb, int *counts) {
+  __shared__ int wc[GXY_BLOCK / 64][GXY_MAX_CAT];
+  int i = blockIdx.x * GXY_BLOCK + threadIdx.x;
+  bool acc = false;
+  int ax = 0, ay = 0;
+  float ar = 0.f, ag = 0.f, ab = 0.f, ao = 0.f;
+  int cat = -1;
+  if (i < n) {
+    const int typ = R.type[i];
+    int c = R.classification[i];
+    if (P.mode & RM_CLASSIFY) c = classify_ray(typ, R.term[i], c, P.reverse);
+    if ((P.mode & RM_ASSIGN) && c == RAY_BOUNDARY) {
+      int f = exit_face(P.blo, P.bhi, R.ox[i], R.oy[i], R.oz[i], R.dx[i], R.dy[i], R.dz[i]);
+      if (P.neighbors[f] >= 0) c = P.neighbors[f];
+      else if (typ == RAY_SHADOW || typ == RAY_AO) c = P.reverse ? GXY_DROP_ON_FLOOR : GXY_TERMINATED;
+      else c = GXY_TERMINATED;
+    }
+    if (P.mode & (RM_CLASSIFY | RM_ASSIGN)) R.classification[i] = c;
+    if ((P.mode & RM_ACCUMULATE) && c == GXY_TERMINATED) {
+      acc = true;
+      ax = R.x[i];
+      ay = R.y[i];
+      ar = R.r[i];
+      ag = R.g[i];
+      ab = R.b[i];
+      ao = R.o[i];
+    }
+    const int ci = c + 4;
+    cat = (ci >= 0 && ci < table.n) ? table.t[ci] : -1;
+  }
+  if (P.mode & RM_ACCUMULATE) fb_accumulate_wave(P.fb, P.width, P.height, acc, ax, ay, ar, ag, ab, ao, P.unique != 0);
+  const int w = threadIdx.x >> 6;
+  for (int k = 0; k < K; k++) {
+    unsigned long long m = __ballot(cat == k);
+    if ((threadIdx.x & 63) == 0) wc[w][k] = __popcll(m);
+  }
+  __syncthreads();
+  if (threadIdx.x < K) {
+    int s = 0;
+    for (int q = 0; q < GXY_BLOCK / 64; q++) s += wc[q][threadIdx.x];
+    counts[threadIdx.x * nb + blockIdx.x] = s;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) counts[K * nb] = 0;
+}
+
 // one workgroup per category: exclusive sum of its nb counts (relative to the
 // category's start) and the category's total, for nb <= GXY_SMALL_SCAN
 __global__ __launch_bounds__(1024) void k_part_scan_small(const int *counts, int nb, int *off, int *tot) {
@@ -167,8 +215,18 @@ __global__ void k_zero_ints(int *p, int m) {
   if ((int)threadIdx.x < m) p[threadIdx.x] = 0;
 }
 
+hipError_t launch_route_count(const RouteParams &P, const RayList_ispc &R, int n, const int *cat_table, int ntable,
+                              int K, int *scratch, hipStream_t s) {
+  if (K > GXY_MAX_CAT || ntable > GXY_CAT_TABLE) return hipErrorInvalidValue;
+  const int nb = (n + GXY_BLOCK - 1) / GXY_BLOCK;
+  if (nb == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_route_count, dim3(nb), dim3(GXY_BLOCK), 0, s, P, R, n, make_table(cat_table, ntable), K, nb,
+                     scratch);
+  return hipGetLastError();
+}
+
 hipError_t launch_partition(const RayList_ispc &R, int n, const int *cat_table, int ntable, int K,
-                            const RayList_ispc *lists, int *scratch, int *counts, hipStream_t s) {
+                            const RayList_ispc *lists, int *scratch, int *counts, hipStream_t s, bool counted) {
   if (K > GXY_MAX_CAT || ntable > GXY_CAT_TABLE) return hipErrorInvalidValue;
   int nb = (n + GXY_BLOCK - 1) / GXY_BLOCK;
   if (nb == 0) {
@@ -178,7 +236,7 @@ hipError_t launch_partition(const RayList_ispc &R, int n, const int *cat_table, 
   CatTable T = make_table(cat_table, ntable);
   const size_t pad = part_pad(n, K);
   int *off = scratch + pad;
-  hipLaunchKernelGGL(k_part_count, dim3(nb), dim3(GXY_BLOCK), 0, s, R, n, T, K, nb, scratch);
+  if (!counted) hipLaunchKernelGGL(k_part_count, dim3(nb), dim3(GXY_BLOCK), 0, s, R, n, T, K, nb, scratch);
   if (nb <= GXY_SMALL_SCAN) {
     // offsets relative to each category's start (the scatter subtracts off[cat * nb] = 0)
     hipLaunchKernelGGL(k_part_scan_small, dim3(K), dim3(1024), 0, s, scratch, nb, off, counts);

@@ -2016,8 +2016,8 @@ GXY_EXPORT int gxy_spawn_offsets_noalloc(int n, RayList_ispc *R, int nAO, int nL
   return 0;
 }
 
-static int route(int n, const RayList_ispc &R, int mode, int reverse, const gxy_box3f *box, const int nb[6],
-                 float *fb, int w, int h, int unique = 0) {
+static RouteParams route_params(int mode, int reverse, const gxy_box3f *box, const int nb[6], float *fb, int w, int h,
+                                int unique) {
   RouteParams P;
   memset(&P, 0, sizeof P);
   P.unique = unique;
@@ -2029,6 +2029,11 @@ static int route(int n, const RayList_ispc &R, int mode, int reverse, const gxy_
   P.width = w;
   P.height = h;
   P.mode = mode;
+  return P;
+}
+static int route(int n, const RayList_ispc &R, int mode, int reverse, const gxy_box3f *box, const int nb[6],
+                 float *fb, int w, int h, int unique = 0) {
+  const RouteParams P = route_params(mode, reverse, box, nb, fb, w, h, unique);
   KTimer kt("route");
   HIPCHK(launch_route(P, R, n, S()));
   return 0;
@@ -2408,20 +2413,6 @@ static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
       }
       dT = total;
     }
-    // Classify, AssignDestinations, HandleTerminatedRays (Renderer.cpp:550-556)
-    if (route(n, R, C.sampler ? (RM_CLASSIFY | RM_ASSIGN) : (RM_CLASSIFY | RM_ASSIGN | RM_ACCUMULATE),
-              G.reverse_lighting,
-              &C.box, C.nb, C.fb, C.w, C.h, L->unique) != 0)
-      return -1;
-    if (C.sampler) {
-      // Sampler::HandleTerminatedRays (Sampler.cpp:51-89)
-      Scratch sc;
-      int nbk = (n + GXY_BLOCK - 1) / GXY_BLOCK;
-      int *scr = (int *)sc.get(block_scan_scratch_bytes(nbk));
-      if (!scr) return -1;
-      KTimer kt("sampler_collect");
-      HIPCHK(launch_accumulate_sampler(R, n, C.xyz, C.capacity, C.dcount, scr, S()));
-    }
     // split by destination: keepers + one list per neighbour face
     // (Renderer.cpp:561-634); terminated and dropped rays are counted only
     int table[64];
@@ -2440,10 +2431,28 @@ static int process_list(LoopCtx &C, RayListObj *L, int prio, bool owned) {
     int *scr = (int *)sc.get(partition_scratch_bytes(n, K));
     int *dcounts = (int *)sc.get(sizeof(int) * K);
     if (!scr || !dcounts) return -1;
+    // Classify, AssignDestinations, HandleTerminatedRays (Renderer.cpp:550-556),
+    // with the partition's count phase in the same pass over the rays
+    {
+      const RouteParams RP =
+          route_params(C.sampler ? (RM_CLASSIFY | RM_ASSIGN) : (RM_CLASSIFY | RM_ASSIGN | RM_ACCUMULATE),
+                       G.reverse_lighting, &C.box, C.nb, C.fb, C.w, C.h, L->unique);
+      KTimer kt("route");
+      HIPCHK(launch_route_count(RP, R, n, table, ntable, K, scr, S()));
+    }
+    if (C.sampler) {
+      // Sampler::HandleTerminatedRays (Sampler.cpp:51-89)
+      Scratch ssc;
+      int nbk = (n + GXY_BLOCK - 1) / GXY_BLOCK;
+      int *sscr = (int *)ssc.get(block_scan_scratch_bytes(nbk));
+      if (!sscr) return -1;
+      KTimer kt("sampler_collect");
+      HIPCHK(launch_accumulate_sampler(R, n, C.xyz, C.capacity, C.dcount, sscr, S()));
+    }
     int counts[K];
     {
       KTimer kt("partition");
-      HIPCHK(launch_partition(R, n, table, ntable, K, nullptr, scr, dcounts, S()));
+      HIPCHK(launch_partition(R, n, table, ntable, K, nullptr, scr, dcounts, S(), true));
     }
     // one wait per list: the partition counts, the parent count and the
     // sample counter (after the loop's last list no trace follows).  The

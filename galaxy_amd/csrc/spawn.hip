@@ -317,6 +317,31 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_parents_light(LightParams L, int 
   }
 }
 
+// k_children and k_parents_light in one pass when each parent has exactly one
+// child (one AO ray and no shadow ray, or one shadow ray and no AO ray): the
+// work-item of parent p writes child p from the parent's unmodified fields,
+// then lights the parent.
+__global__ __launch_bounds__(GXY_BLOCK) void k_child_parent(LightParams L, int mode, RayList_ispc R, const int *pidx,
+                                                            const int *Td, RayList_ispc O) {
+  const int T = *Td;
+  const bool ao = (mode & LM_AO) && L.nAO == 1;
+  for (int p = blockIdx.x * GXY_BLOCK + threadIdx.x; p < T; p += gridDim.x * GXY_BLOCK) {
+    const int i = pidx[p];
+    ParentRay P = parent_load(R, i);
+    if (ao) {
+      light_ao_child(L, P, 0, O, p);
+    } else {
+      const float o = (mode & LM_DIFFUSE) ? light_diffused_alpha(L, P.o) : P.o;
+      light_shadow_child(L, P, o, 0, O, p);
+    }
+    if (mode & (LM_AMBIENT | LM_DIFFUSE)) {
+      if (mode & LM_AMBIENT) light_ambient(L, P);
+      if (mode & LM_DIFFUSE) light_diffuse(L, P);
+      R.r[i] = P.r; R.g[i] = P.g; R.b[i] = P.b; R.o[i] = P.o;
+    }
+  }
+}
+
 hipError_t launch_lighting_children(const LightParams &L, int mode, const RayList_ispc &R, const int *pidx, int T,
                                     const RayList_ispc &O, hipStream_t s) {
   if (T <= 0) return hipSuccess;
@@ -338,7 +363,13 @@ hipError_t launch_lighting_children_dev(const LightParams &L, int mode, const Ra
   // work-items in flight hide more of that latency (0.727 -> 0.675 ms for the
   // 29 M children of a 4K nine-icosphere frame against 4096 workgroups)
   const long long cap = 32768;
-  const long long nc = (long long)n * (((mode & LM_AO) ? L.nAO : 0) + ((mode & LM_SHADOW) ? L.nLights : 0));
+  const int per = ((mode & LM_AO) ? L.nAO : 0) + ((mode & LM_SHADOW) ? L.nLights : 0);
+  const long long nc = (long long)n * per;
+  if (per == 1) {
+    hipLaunchKernelGGL(k_child_parent, dim3((unsigned)std::min(cap, (long long)(n + GXY_BLOCK - 1) / GXY_BLOCK)),
+                       dim3(GXY_BLOCK), 0, s, L, mode, R, pidx, T, O);
+    return hipGetLastError();
+  }
   if (nc > 0)
     hipLaunchKernelGGL(k_children, dim3((unsigned)std::min(cap, (nc + GXY_BLOCK - 1) / GXY_BLOCK)), dim3(GXY_BLOCK),
                        0, s, L, mode, R, pidx, 0, T, O);
