@@ -156,7 +156,7 @@ struct TraceParams {
   float opaque_alpha;  // early termination: alpha above this is opaque (0.999)
   DVolVis vv[GXY_MAX_VOLVIS];
   DModel model;
-  unsigned long long *sample_partials; // per-workgroup sample counts (may be null)
+  unsigned long long *sample_count;    // volume samples are added here, one atomic per wave (may be null)
   int *hit_geom;                       // side-band geomID per ray, may be null
   int *hit_prim;
   // a launch queued before its list's size is known: when dn is set, the

@@ -49,19 +49,17 @@ struct CameraParams {
 // path lines), 0 the compiler's choice
 // bs: work-items per workgroup of the one-bricked-volume, one-isovalue march
 // at 6 waves per SIMD (64, 128 or 256; GXY_BLOCK elsewhere):
-// trace_block_size(P, wpe, bs) is the size a launch uses (its workgroup count
-// sizes P.sample_partials)
+// trace_block_size(P, wpe, bs) is the size a launch uses
 hipError_t launch_trace(const TraceParams &P, const RayList_ispc &R, int n, int wpe, int stack, hipStream_t s,
                         int gwaves = 0, int bs = GXY_BLOCK);
 int trace_block_size(const TraceParams &P, int wpe, int bs);
 hipError_t launch_sampler(const TraceParams &P, const RayList_ispc &R, int n, hipStream_t s);
 // Persistent-lane trace for volume-only visualizations (1..4 volumes): each
 // wave owns `chunk` consecutive rays; idle lanes are refilled when at least
-// `refill` of the 64 are idle.  Sample partials: trace_pl_blocks(n, chunk).
+// `refill` of the 64 are idle.
 hipError_t launch_trace_pl(const TraceParams &P, const RayList_ispc &R, int n, int chunk, int refill,
                            hipStream_t s);
 int trace_pl_blocks(int n, int chunk);
-hipError_t launch_reduce_add(const unsigned long long *part, int nb, unsigned long long *acc, hipStream_t s);
 
 // Exclusive sum of m ints by the device-wide scan; temp holds scan_temp_bytes(m).
 size_t scan_temp_bytes(long long m);

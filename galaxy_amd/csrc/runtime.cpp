@@ -580,7 +580,7 @@ static int make_trace_params(const Visualization_ispc *vis, float global_eps, Tr
     P.hit_geom = M->hit_geom;
     P.hit_prim = M->hit_prim;
   }
-  P.sample_partials = nullptr;
+  P.sample_count = nullptr;
   P.opaque_alpha = G.opaque_alpha;
   return 0;
 }
@@ -1744,24 +1744,16 @@ GXY_EXPORT int gxy_model_set_hit_output(void *model, int *g, int *p) {
   return 0;
 }
 
-// Trace (or sampler) launch with the per-workgroup sample counts reduced into
-// the runtime's sample counter (counters[0]).
+// Trace (or sampler) launch; its volume samples are added to the context's
+// sample counter (counters[0]) by the kernel's waves.
 static int trace_launch(TraceParams P, const RayList_ispc &R, int n, bool sampler) {
   if (n <= 0) return 0;
   const bool pl = !sampler && G.trace_kind == 1 && P.nvv >= 1 && !P.has_model;
-  const int bs = sampler ? GXY_BLOCK : trace_block_size(P, G.trace_wpe, G.trace_chunk);
-  int nb = pl ? trace_pl_blocks(n, G.trace_chunk) : (n + bs - 1) / bs;
-  unsigned long long *part = (unsigned long long *)dalloc(sizeof(unsigned long long) * nb);
-  if (!part) return -1;
-  P.sample_partials = part;
-  {
-    KTimer kt(sampler ? "sampler" : "trace");
-    if (pl) HIPCHK(launch_trace_pl(P, R, n, G.trace_chunk, G.trace_refill, S()));
-    else HIPCHK(sampler ? launch_sampler(P, R, n, S())
-                        : launch_trace(P, R, n, G.trace_wpe, G.geom_stack, S(), G.geom_waves, G.trace_chunk));
-  }
-  HIPCHK(launch_reduce_add(part, nb, CX().counters, S()));
-  dfree_stream_ordered(part);
+  P.sample_count = CX().counters;
+  KTimer kt(sampler ? "sampler" : "trace");
+  if (pl) HIPCHK(launch_trace_pl(P, R, n, G.trace_chunk, G.trace_refill, S()));
+  else HIPCHK(sampler ? launch_sampler(P, R, n, S())
+                      : launch_trace(P, R, n, G.trace_wpe, G.geom_stack, S(), G.geom_waves, G.trace_chunk));
   return 0;
 }
 
@@ -3097,7 +3089,7 @@ static int batch_dataset_params(BatchObj *B, const Visualization_ispc *vis, floa
   P.occlusion = (G.occlusion && geometry) ? 1 : 0;
   P.hit_geom = nullptr;
   P.hit_prim = nullptr;
-  P.sample_partials = nullptr;
+  P.sample_count = nullptr;
   depth = 0;
   if (geometry) {
     ModelObj *M = lookup<ModelObj>(G.models, vis->model);

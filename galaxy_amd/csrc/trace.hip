@@ -15,51 +15,11 @@
 
 namespace gxy {
 
-// Workgroup total of a per-item count written to out[blockIdx.x] (no atomics:
-// same-address atomics from every wave serialise in one L2 channel).
-template <int BS = GXY_BLOCK>
-__device__ __forceinline__ void block_sum_store(unsigned long long v, unsigned long long *out) {
+// A per-item count added to *ctr: the wave's total, one 64-bit atomic per wave
+// (every lane of the wave calls it)
+__device__ __forceinline__ void wave_add_count(unsigned long long v, unsigned long long *ctr) {
   v = wave_sum_u64(v);
-  if (BS == 64) {
-    if (threadIdx.x == 0) out[blockIdx.x] = v;
-    return;
-  }
-  __shared__ unsigned long long ws[BS / 64];
-  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long t = 0;
-    for (int w = 0; w < BS / 64; w++) t += ws[w];
-    out[blockIdx.x] = t;
-  }
-}
-
-// adds the nb workgroup partials into *acc (one workgroup)
-__global__ __launch_bounds__(1024) void k_reduce_add(const unsigned long long *part, int nb, unsigned long long *acc) {
-  __shared__ unsigned long long ws[16];
-  unsigned long long v = 0;
-  // eight independent loads in flight per work-item and round
-  for (int k0 = threadIdx.x; k0 < nb; k0 += 8 * 1024) {
-    unsigned long long x[8];
-#pragma unroll
-    for (int j = 0; j < 8; j++) x[j] = k0 + j * 1024 < nb ? part[k0 + j * 1024] : 0ull;
-#pragma unroll
-    for (int j = 0; j < 8; j++) v += x[j];
-  }
-  v = wave_sum_u64(v);
-  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long t = 0;
-    for (int w = 0; w < 16; w++) t += ws[w];
-    *acc += t;
-  }
-}
-
-hipError_t launch_reduce_add(const unsigned long long *part, int nb, unsigned long long *acc, hipStream_t s) {
-  if (nb <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_reduce_add, dim3(1), dim3(1024), 0, s, part, nb, acc);
-  return hipGetLastError();
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(ctr, v);
 }
 
 // WPE > 0 asks the compiler for at least WPE waves per SIMD (fewer VGPRs);
@@ -100,7 +60,7 @@ void k_trace(TraceParams P, RayList_ispc R, int n) {
     PrivStack<GXY_BVH_STACK> stack;
     if (i < n) nsamp = trace_one<NV, GEOM, PrivStack<GXY_BVH_STACK>, SM, W, MI>(P, R, i, s_col, s_op, stack);
   }
-  if (P.sample_partials) block_sum_store<BS>(nsamp, P.sample_partials);
+  if (P.sample_count) wave_add_count(nsamp, P.sample_count);
 }
 
 // ------------------------------------------------------------- persistent-lane march
@@ -354,7 +314,7 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_trace_pl(TraceParams P, RayList_i
       if (!pl_step<NV>(P, m, sLast, iter, s_col, s_op, nsamp)) pl_finish<NV>(P, R, m, s_col);
     }
   }
-  if (P.sample_partials) block_sum_store(nsamp, P.sample_partials);
+  if (P.sample_count) wave_add_count(nsamp, P.sample_count);
 }
 
 template <int NV>
@@ -387,7 +347,7 @@ __global__ __launch_bounds__(GXY_BLOCK) void k_sampler(TraceParams P, RayList_is
   const int i = blockIdx.x * GXY_BLOCK + threadIdx.x;
   unsigned long long nsamp = 0;
   if (i < n) nsamp = sample_one<NV, SM>(P, R, i);
-  if (P.sample_partials) block_sum_store(nsamp, P.sample_partials);
+  if (P.sample_count) wave_add_count(nsamp, P.sample_count);
 }
 
 // ------------------------------------------------------------- launchers
