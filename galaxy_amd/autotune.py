@@ -6,9 +6,7 @@ measures: it renders the caller's frame once per candidate with HIP-event
 timing of the trace kernel enabled and keeps the fastest.
 
 Candidates: one ray per work-item (kind 0, and its 5- and 6-waves-per-SIMD
-occupancy variants 15 and 16, the latter also in 64-item workgroups, which
-the one-bricked-volume, one-isovalue march takes: the chunk field is the
-workgroup size for these kinds), or persistent lanes (kind 1) with a per-wave
+occupancy variants 15 and 16), or persistent lanes (kind 1) with a per-wave
 chunk of 128 or 256 consecutive rays and a refill threshold of 16 or 32 idle
 lanes out of 64.  (Measured on the volume-render frame: the persistent form
 was 2-5x slower than one ray per work-item; it stays a candidate for other
@@ -16,11 +14,14 @@ ray mixes.  The 7- and 8-waves-per-SIMD variants 17 and 18, which apply to
 one bricked volume with at most one isovalue, are selectable but not
 candidates: over two runs on that frame 17 timed 0.948-0.969 and 18
 0.974-1.001 ms of trace per frame against 0.949-0.971 for 16, so they would
-only make the pick noisy.)
+only make the pick noisy.  The same holds for kind 16 in 64-item workgroups
+(chunk 64, for one bricked volume with at most one isovalue): 3.5 % faster
+than 256-item workgroups before the march loop lost its scalar overhead,
+0.879 against 0.873 ms of trace per frame after.)
 """
 import ctypes as C
 
-DEFAULT_CANDIDATES = ([(0, 256, 32), (15, 256, 32), (16, 256, 32), (16, 64, 32)] +
+DEFAULT_CANDIDATES = ([(0, 256, 32), (15, 256, 32), (16, 256, 32)] +
                       [(1, chunk, refill) for chunk in (128, 256) for refill in (16, 32)])
 # BVH traversal stacks of geometry-only traces: private memory (64 entries) or
 # 16 / 24 / 32 entries in LDS (a model deeper than the stack keeps the private one)
