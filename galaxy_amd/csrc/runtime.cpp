@@ -2341,9 +2341,11 @@ static int enqueue_split(LoopCtx &C, RayListObj *out, int prio) {
 // processRays_task::work (Renderer.cpp:511-651) for one list and its keepers
 // Queues the trace of list L before its size is known: the launch covers its
 // capacity ncap and the kernel reads the size, *dn * mul, from the device.
-// Only the one-ray-per-item kernels read it, so only they are queued ahead.
+// Only the one-ray-per-item kernels read it, so only they are queued ahead;
+// and not when the model's per-ray hit side-band is on (a list traced ahead
+// would write it out of the loop's list order).
 static bool can_pretrace(const LoopCtx &C, long long ncap) {
-  return !C.sampler && G.trace_kind != 1 && ncap > 0 && ncap <= C.maxlist;
+  return !C.sampler && G.trace_kind != 1 && !C.P.hit_geom && ncap > 0 && ncap <= C.maxlist;
 }
 static int pretrace(LoopCtx &C, RayListObj *L, int ncap, const int *dn, int mul) {
   TraceParams P = C.P;

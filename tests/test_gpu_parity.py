@@ -555,6 +555,41 @@ def test_a_nan_volume_step_is_rejected(radial_vols_small, hip):
     assert hip.trace(Bh.vis, d) == ""
 
 
+def test_traces_queued_ahead_give_the_same_frames(hip):
+    """The rank loop queues a list's trace behind the readback of its size
+    (camera rays, children) for the one-ray-per-item kernels; the persistent-
+    lane kernel and a list cap below the children's capacity take the plain
+    order.  Every way gives the same framebuffer and frame statistics."""
+    from galaxy_amd import lib as glib
+    from galaxy_amd import workloads
+    L_ = glib.lib()
+    sc, alpha = workloads.volume_render(n=96, width=256, height=192)
+    saved_alpha = L_.gxy_get_opacity_threshold()
+    glib.check(L_.gxy_set_opacity_threshold(alpha))
+    b = scenes.build_scene(hip, sc)
+    glib.check(L_.gxy_volume_set_layout(b.handles[0], 4))
+    c = sc.cameras[0]
+    saved = (C.c_int(), C.c_int(), C.c_int())
+    L_.gxy_get_trace_config(*[C.byref(x) for x in saved])
+    out = []
+    try:
+        for kind, maxlist in ((16, 1 << 30), (1, 1 << 30), (16, 30000)):
+            glib.check(L_.gxy_set_trace_config(kind, 256, 32))
+            out.append(hip.render_frame(b, c, maxlist=maxlist))
+    finally:
+        L_.gxy_set_trace_config(*[x.value for x in saved])
+        L_.gxy_volume_set_layout(b.handles[0], 0)
+        L_.gxy_set_opacity_threshold(saved_alpha)
+    (fb0, st0) = out[0]
+    d0 = st0.as_dict()
+    assert st0.secondary_rays > 1000 and st0.samples > 100000
+    for fb, st in out[1:]:
+        d = st.as_dict()
+        for k in ("ray_passes", "samples", "primary_rays", "secondary_rays", "terminated", "dropped", "kept"):
+            assert d[k] == d0[k], k
+        assert np.array_equal(fb, fb0)
+
+
 def test_occlusion_rays_give_the_same_frames(hip):
     """Shadow and AO rays of a volume-free scene stop at their first hit in
     the rank loop (gxy_set_occlusion_rays): the frame and every count equal
