@@ -687,6 +687,8 @@ __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayLis
       // (every pass leaves tThis <= tTermination: the next position is cut
       // there, and a pass that moves tTermination ends the march; go is the
       // one exit condition, set at the end of each pass)
+      // (unrolled by two: the loop's scalar bookkeeping is shared by two passes)
+#pragma unroll 2
       while (go && iter++ < GXY_MAX_MARCH) {
         // (the ray is not opaque when a pass begins: only this pass's samples
         // can make it so)
