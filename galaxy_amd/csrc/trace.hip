@@ -40,6 +40,12 @@ void k_trace(TraceParams P, RayList_ispc R, int n) {
   constexpr int NVS = NV > 0 ? NV : 1;
   __shared__ float s_col[NVS][3 * GXY_TF_MAX];
   __shared__ float s_op[NVS][GXY_TF_MAX];
+  // a launch queued before its list's size was known covers the capacity:
+  // workgroups past the size leave at once (before staging the TF tables)
+  if (P.dn) {
+    n = min(n, *P.dn * P.dn_mul);
+    if ((int)blockIdx.x * BS >= n) return;
+  }
 #pragma unroll
   for (int v = 0; v < NV; v++) {
     for (int k = threadIdx.x; k < 3 * P.vv[v].tf.nc; k += BS) s_col[v][k] = P.vv[v].tf.col[k];
@@ -48,7 +54,6 @@ void k_trace(TraceParams P, RayList_ispc R, int n) {
   if (NV > 0) __syncthreads();
 
   const int i = blockIdx.x * BS + threadIdx.x;
-  if (P.dn) n = min(n, *P.dn * P.dn_mul);
   unsigned long long nsamp = 0;
   if (STK > 0) {
     __shared__ int s_stack[STK > 0 ? STK : 1][BS];
