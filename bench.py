@@ -497,7 +497,7 @@ def main():
                     help="also run one 256 MiB device-to-device copy (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs one partition on one MI355X; "
+        print("bench.py measures one GPU: one partition traced on one MI355X; "
               "--gpus %d is not supported (use --gpus 1)" % args.gpus, flush=True)
         sys.exit(2)
 
