@@ -857,7 +857,10 @@ __device__ __forceinline__ unsigned sample_one(const TraceParams &P, const RayLi
   VFetch F[NV];
 #pragma unroll
   for (int k = 0; k < NV; k++) F[k] = vox_fetch<SM>(P.vv[k].vol, VR[k], tThis);
-  while (tThis <= tExit && hit == -1 && iter++ < GXY_MAX_MARCH) {
+  // one exit condition, set at the end of each pass (every pass leaves
+  // tThis <= tExit: the next position is cut there)
+  bool go = tThis <= tExit;
+  while (go && iter++ < GXY_MAX_MARCH) {
     float sCur[NV];
 #pragma unroll
     for (int k = 0; k < NV; k++) sCur[k] = vol_finish_fma(F[k]);
@@ -883,9 +886,10 @@ __device__ __forceinline__ unsigned sample_one(const TraceParams &P, const RayLi
         if (h) { tThis = tHit; hit = k; }
       }
     }
-    if (hit != -1 || tThis == tExit) break;
-    if (!(nt0 > tThis)) break;
-    tThis = nt0;
+    // a crossing or the volume's exit ends the march, and so does a step that
+    // no longer advances t
+    go = hit == -1 && tThis != tExit && nt0 > tThis;
+    if (go) tThis = nt0;
   }
   gptr(R.t)[i] = hit != -1 ? tThis + 0.001f : tThis;
   gptr(R.term)[i] = hit != -1 ? RAY_SURFACE : RAY_BOUNDARY;
