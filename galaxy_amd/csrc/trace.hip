@@ -61,6 +61,17 @@ void k_trace(TraceParams P, RayList_ispc R, int n) {
     stack.base = &s_stack[0][threadIdx.x];
     stack.stride = BS;
     if (i < n) nsamp = trace_one<NV, GEOM, LdsStack<(STK > 0 ? STK : 1)>, SM, W, MI>(P, R, i, s_col, s_op, stack);
+  } else if (NV > 0 && !GEOM) {
+    // volumes only: a wave whose rays are all primary, or all not, takes the
+    // march compiled for that kind (lists of one kind are the usual case)
+    PrivStack<GXY_BVH_STACK> stack;
+    if (i < n) {
+      const unsigned long long pm = __builtin_amdgcn_ballot_w64(gptr(R.type)[i] == RAY_PRIMARY);
+      const unsigned long long am = __builtin_amdgcn_ballot_w64(true);
+      if (pm == am) nsamp = trace_one<NV, GEOM, PrivStack<GXY_BVH_STACK>, SM, W, MI, 1>(P, R, i, s_col, s_op, stack);
+      else if (pm == 0) nsamp = trace_one<NV, GEOM, PrivStack<GXY_BVH_STACK>, SM, W, MI, 2>(P, R, i, s_col, s_op, stack);
+      else nsamp = trace_one<NV, GEOM, PrivStack<GXY_BVH_STACK>, SM, W, MI>(P, R, i, s_col, s_op, stack);
+    }
   } else {
     PrivStack<GXY_BVH_STACK> stack;
     if (i < n) nsamp = trace_one<NV, GEOM, PrivStack<GXY_BVH_STACK>, SM, W, MI>(P, R, i, s_col, s_op, stack);

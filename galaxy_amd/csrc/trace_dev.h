@@ -559,14 +559,16 @@ __device__ __forceinline__ void composite(float &cr, float &cg, float &cb, float
 // NV transfer-function tables (LDS).  Returns the number of volume samples.
 // MI: the isovalues per volume the iso test is unrolled for (the volume's
 // nIso must not exceed it).
-template <int NV, bool GEOM, class Stack, int SM = -1, int W = 0, int MI = GXY_MAX_ISO>
+// SH: 1 when the caller has found every active lane's ray primary, 2 when it
+// has found none primary (the choice is then known to the compiler), 0 per lane.
+template <int NV, bool GEOM, class Stack, int SM = -1, int W = 0, int MI = GXY_MAX_ISO, int SH = 0>
 __device__ __forceinline__ unsigned trace_one(const TraceParams &P, const RayList_ispc &R, const int i,
                                               const float (*s_col)[3 * GXY_TF_MAX],
                                               const float (*s_op)[GXY_TF_MAX], Stack &stack) {
   constexpr int NVS = NV > 0 ? NV : 1;
   unsigned nsamp = 0;
   {
-    const bool shadeFlag = gptr(R.type)[i] == RAY_PRIMARY;
+    const bool shadeFlag = SH == 1 ? true : SH == 2 ? false : gptr(R.type)[i] == RAY_PRIMARY;
     f3 org = mk3(gptr(R.ox)[i], gptr(R.oy)[i], gptr(R.oz)[i]);
     f3 dir = mk3(gptr(R.dx)[i], gptr(R.dy)[i], gptr(R.dz)[i]);
     if (dir.x == 0.f) dir.x = 1e-6f;
