@@ -481,10 +481,16 @@ __device__ __forceinline__ void pixel_of(const CameraParams &C, int q, int &x, i
     x = q % C.width;
     y = q / C.width;
   } else {
-    int tw = (C.width + 7) / 8;
-    int tile = q / 64, r = q % 64;
-    x = (tile % tw) * 8 + (r % 8);
-    y = (tile / tw) * 8 + (r / 8);
+    // 8x8 tiles in bands of two tile rows, a band's tiles column by column,
+    // so four consecutive tiles (one 256-ray workgroup) make a 16x16 square;
+    // a last band of one tile row runs along its row
+    const int tw = (C.width + 7) / 8, th = (C.height + 7) / 8;
+    const int tile = q / 64, r = q % 64, band = tile / (2 * tw), k = tile % (2 * tw);
+    int tx, ty;
+    if (band * 2 + 1 < th) { tx = k / 2; ty = band * 2 + (k & 1); }
+    else { tx = k; ty = band * 2; }
+    x = tx * 8 + (r % 8);
+    y = ty * 8 + (r / 8);
   }
 }
 

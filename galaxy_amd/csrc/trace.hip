@@ -34,6 +34,21 @@ __device__ __forceinline__ void wave_add_count(unsigned long long v, unsigned lo
 // refilled only as a whole, so smaller workgroups let a SIMD take new rays as
 // soon as one wave's rays are done, at the price of staging the transfer
 // function tables once per wave.
+// Workgroups reach the eight XCDs (one L2 each) round robin; this renumbering
+// hands each XCD runs of C consecutive workgroups' rays instead (neighbouring
+// camera tiles read the same bricks), the runs still dealt round robin so the
+// XCDs' shares of the image stay alike.  C = 1 is the dispatch order itself;
+// the workgroups past the last whole round of 8*C keep their own numbers.
+template <int C>
+__device__ __forceinline__ int xcd_block() {
+  const int b = blockIdx.x;
+  if (C <= 1) return b;
+  const int full = (int)gridDim.x / (8 * C) * (8 * C);
+  if (b >= full) return b;
+  const int x = b & 7, j = b >> 3;
+  return ((j / C) * 8 + x) * C + j % C;
+}
+
 template <int NV, bool GEOM, int WPE, int SM, int STK = 0, int W = 0, int MI = GXY_MAX_ISO, int BS = GXY_BLOCK>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, 8)))
 void k_trace(TraceParams P, RayList_ispc R, int n) {
@@ -42,9 +57,10 @@ void k_trace(TraceParams P, RayList_ispc R, int n) {
   __shared__ float s_op[NVS][GXY_TF_MAX];
   // a launch queued before its list's size was known covers the capacity:
   // workgroups past the size leave at once (before staging the TF tables)
+  const int blk = xcd_block<(NV > 0 && !GEOM && STK == 0) ? 8 : 1>();
   if (P.dn) {
     n = min(n, *P.dn * P.dn_mul);
-    if ((int)blockIdx.x * BS >= n) return;
+    if (blk * BS >= n) return;
   }
 #pragma unroll
   for (int v = 0; v < NV; v++) {
@@ -53,7 +69,7 @@ void k_trace(TraceParams P, RayList_ispc R, int n) {
   }
   if (NV > 0) __syncthreads();
 
-  const int i = blockIdx.x * BS + threadIdx.x;
+  const int i = blk * BS + threadIdx.x;
   unsigned long long nsamp = 0;
   if (STK > 0) {
     __shared__ int s_stack[STK > 0 ? STK : 1][BS];

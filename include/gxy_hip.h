@@ -411,7 +411,9 @@ int gxy_partition_rays(int nRays, RayList_ispc *rays, int ndest,
 /* Camera::generate_initial_rays + SpawnRays (Camera.cpp:379-493, 528-829):
  * writes up to width*height primary rays (pixels whose ray meets the boxes)
  * into rays; returns the count through n_out (synchronises).  order: 0 = row
- * major as the reference, 1 = 8x8 pixel tiles (same ray set, better locality). */
+ * major as the reference, 1 = 8x8 pixel tiles (same ray set, better locality:
+ * bands of two tile rows, each band's tiles column by column, so every four
+ * consecutive tiles cover a 16x16 square; a last single tile row in order). */
 int gxy_camera_generate(const float eye[3], const float center_or_dir[3], int is_dir,
                         const float up[3], float aov, int width, int height,
                         const gxy_box3f *local_box, const gxy_box3f *global_box,

@@ -210,6 +210,12 @@ def test_camera_rays_bit_exact(hip, oracle):
     o1 = np.argsort(key1)
     assert np.array_equal(key1[o1], key0)
     assert np.array_equal(g1["dx"][o1], g["dx"])
+    # each 8x8 tile's rays together, in pixel order within the tile
+    tile = (g1["y"] // 8).astype(np.int64) * 1000 + g1["x"] // 8
+    starts = np.r_[True, tile[1:] != tile[:-1]]
+    assert len(np.unique(tile)) == starts.sum()
+    within = (g1["y"] % 8) * 8 + g1["x"] % 8
+    assert np.all(np.diff(within)[~starts[1:]] > 0)
 
 
 def test_geometry_meshes_and_spheres(hip, oracle):
